@@ -1,0 +1,230 @@
+"""Benchmark of the Stable Cosserat Rods step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: the synthetic groom generator at 100k strands x 100 vertices
+(1e7 vertices), FP32, dt = 1/600 s per step = 5 substeps x 10 VBD iterations,
+no SDF and no contacts in the timed sweep (the with-SDF step is reported
+beside it).  One step is one launch of the fused step kernel; inputs are
+resident in HBM before the timed region starts.
+
+Metric: strand-vertex VBD updates per second, one update being one vertex's
+position solve plus orientation solve in one iteration, i.e.
+updates/step = vertices x substeps x iterations.
+
+Prints ONE JSON line.  The CPU baseline is the FP32 C++ restatement of the
+reference step (oracle/, OpenMP over vertices) run on a bounded sample of the
+same workload on this machine's host cores.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+# Compulsory HBM bytes of one fused-step launch per vertex: read x, v, u (12 B
+# each), q, q_rest, mat (16 B each) = 84 B; write x, v, u (12 B each), q (16 B)
+# = 52 B.
+FUSED_BYTES_PER_VERTEX = 136
+# Streaming-sweep model of the VBD iteration (bytes per update when every
+# iteration goes through HBM): read x, dx, y (12 B each), q, q_rest (16 B
+# each), invMass/lRest/kStretch (12 B), flags (4 B); write dx (12 B), q (16 B).
+STREAMING_BYTES_PER_UPDATE = 112
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--strands", type=int, default=100_000)
+    ap.add_argument("--verts", type=int, default=100)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-sdf-leg", action="store_true", help="skip the with-SDF step timing")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def ensure_built():
+    from kamisimulator_amd import build
+    if not os.path.exists(build.LIB):
+        build.build()
+
+
+def cpu_baseline(scene, params, seconds):
+    """FP32 restatement on the host cores: the first 10k strands of the same
+    groom, stepped until `seconds` of work or 50 steps."""
+    from kamisimulator_amd import scenes
+    from oracle import pyoracle
+
+    pyoracle.build()
+    ns = min(10_000, scene.num_strands)
+    nv = int(scene.strand_offsets[ns])
+    sub = scenes.Scene("cpu-sample", scene.positions[:nv].copy(), scene.strand_offsets[:ns + 1].copy(),
+                       material=scene.material)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = pyoracle.OracleSim(params, sub.positions, sub.strand_offsets, precision=32)
+    o.set_threads(threads)
+    scenes.setup_sim(o, sub)
+    dt = np.float32(1 / 600)
+    S = int(np.ceil(dt / np.float32(params.max_h)))
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < 50:
+        o.advance(dt)
+        steps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    ups = nv * S * params.num_itr * steps / el
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return {
+        "value": ups,
+        "unit": "strand-vertex VBD updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP32 C++ restatement of the reference step (oracle/), {ns} strands x {scene.positions.shape[0] // scene.num_strands} "
+                  f"vertices of the same groom, {steps} steps of dt=1/600 ({el:.1f} s), OpenMP {threads} threads, {model}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the strand-sharded multi-GPU "
+              "layout is described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+    ensure_built()
+    import torch
+
+    from kamisimulator_amd import scenes
+    from kamisimulator_amd._lib import Sim, default_params
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    t0 = time.perf_counter()
+    scene = scenes.config_groom(args.strands, args.verts, seed=1, sdf_dim=128, with_sdf=True)
+    params = scenes.make_params(default_params, scene.material, block_size=args.block)
+    sim = Sim(params, scene.positions, scene.strand_offsets, device=0)
+    sim.set_stream(stream.cuda_stream)
+    sdf = scene.sdf
+    scene.sdf = None
+    scenes.setup_sim(sim, scene)
+    print(f"[bench] setup {time.perf_counter() - t0:.1f}s  verts={scene.num_verts} strands={scene.num_strands} "
+          f"block={sim.block_size}", file=sys.stderr, flush=True)
+
+    dt = np.float32(1 / 600)
+    S = int(np.ceil(dt / np.float32(params.max_h)))
+    updates_per_step = scene.num_verts * S * params.num_itr
+
+    for _ in range(args.warmup):
+        sim.advance(dt)
+    torch.cuda.synchronize()
+
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        sim.advance(dt)
+        ev[k][1].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms_per_step = total_ms / args.steps
+    value = updates_per_step * args.steps / (total_ms / 1e3)
+    print(f"[bench] {args.steps} steps: {total_ms:.2f} ms (wall {wall * 1e3:.1f} ms), kernel avg {kern_ms:.3f} ms, "
+          f"{value:.3e} updates/s", file=sys.stderr, flush=True)
+
+    achieved = FUSED_BYTES_PER_VERTEX * scene.num_verts / (kern_ms / 1e3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "kernel": "step_fused_kernel",
+        "algorithmic_bytes_per_launch": FUSED_BYTES_PER_VERTEX * scene.num_verts,
+        "avg_launch_ms": round(kern_ms, 4),
+        "note": "one launch = one step (5 substeps x 10 iterations) with vertex state resident in registers/LDS; "
+                "algorithmic bytes = compulsory 136 B/vertex per launch, so the kernel is VALU-bound, not HBM-bound",
+    }
+    streaming_equiv = value * STREAMING_BYTES_PER_UPDATE / (HBM_PEAK_GBS * 1e9)
+
+    sdf_leg = None
+    if not args.no_sdf_leg:
+        sim.set_sdf(sdf, threshold=0.0)
+        sim.solve_next_step(dt)
+        torch.cuda.synchronize()
+        n2 = max(5, args.steps // 5)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(n2):
+            sim.solve_next_step(dt)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms2 = a.elapsed_time(b) / n2
+        sdf_leg = {"ms_per_step": round(ms2, 4), "updates_per_s": updates_per_step / (ms2 / 1e3),
+                   "sdf_grid": "128^3 sphere SDF, threshold 0"}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, params, args.cpu_seconds)
+        print(f"[bench] cpu baseline {cpu['value']:.3e} updates/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "strand-vertex VBD updates/s @100k x 100 FP32",
+        "value": value,
+        "unit": "strand-vertex VBD updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"synthetic groom {scene.num_strands} strands x {args.verts} vertices on a 0.1 m sphere scalp, "
+                        f"FP32, dt=1/600 s = {S} substeps x {params.num_itr} VBD iterations, Jacobi schedule, "
+                        "no SDF or contacts in the timed step",
+            "strands": scene.num_strands,
+            "vertices": scene.num_verts,
+            "updates_per_step": updates_per_step,
+            "block": sim.block_size,
+        },
+        "roofline": roofline,
+        "streaming_sweep_equivalent_frac": round(streaming_equiv, 4),
+        "with_sdf_step": sdf_leg,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

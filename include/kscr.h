@@ -1,0 +1,170 @@
+/*
+ * kscr.h - C ABI of the MI355X-native Stable Cosserat Rods step.
+ *
+ * This is the inner seam below KamiSimulator's Solver plugin interface
+ * (src/solvers/Solver.h:13-85).  It mirrors the lifecycle of the reference's
+ * YarnBall::Sim (src/solvers/StableCosseratRods/YarnBall/YBSimulator.h:18-99)
+ * and of the SDF body-collision pass
+ * (src/solvers/SDFCollisionSolver/SDFBodyCollisionSolver.cpp:122-263), with
+ * plain pointers and sizes only.  Every entry point returns a kscr_status;
+ * no C++ exception crosses this boundary and nothing calls exit().
+ *
+ * Layout conventions (host arrays, caller owned, copied in and out):
+ *   positions / velocities : float[num_verts*3], xyz interleaved
+ *   orientations q, q_rest : float[num_verts*4], (x, y, z, w), w is the scalar
+ *   strand_offsets         : int[num_strands+1], vertex ranges of each strand
+ * Device buffers are owned by the handle.  Calls on one handle must be
+ * serialised by the caller; distinct handles are independent.
+ */
+#ifndef KSCR_H_
+#define KSCR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kscr_handle kscr_handle;
+
+/* Status codes.  KSCR_ERR_MAX_COLLISIONS mirrors the device error word of
+ * src/solvers/StableCosseratRods/YarnBall/YarnBall.h:16; the validation codes
+ * mirror the throws of YBSimulator.cu:82, :185-186, :197, :326-332. */
+typedef enum kscr_status {
+    KSCR_OK = 0,
+    KSCR_ERR_INVALID_ARGUMENT = 1,
+    KSCR_ERR_TOO_FEW_VERTICES = 2,     /* Sim::Sim, numVerts < 3            */
+    KSCR_ERR_DANGLING_SEGMENT = 3,     /* Sim::configure, 1-vertex strand   */
+    KSCR_ERR_ZERO_LENGTH_SEGMENT = 4,  /* Sim::configure, |x_{i+1}-x_i| = 0 */
+    KSCR_ERR_DETECTION_SCALER = 5,     /* uploadMeta, detectionScaler <= 1  */
+    KSCR_ERR_SEGMENT_TOO_SHORT = 6,    /* uploadMeta, 2 minSegLen < 2r + b  */
+    KSCR_ERR_NOT_CONFIGURED = 7,       /* setYoung/setK* before configure   */
+    KSCR_ERR_MAX_COLLISIONS = 8,       /* > 16 contacts on one segment      */
+    KSCR_ERR_STRAND_TOO_LONG = 9,      /* strand longer than one workgroup  */
+    KSCR_ERR_HIP = 10,                 /* HIP runtime failure               */
+    KSCR_ERR_OUT_OF_MEMORY = 11,
+    KSCR_ERR_UNSUPPORTED = 12
+} kscr_status;
+
+enum {
+    KSCR_SCHEDULE_JACOBI = 0,   /* reference semantics (deterministic snapshot) */
+    KSCR_SCHEDULE_COLORED = 1   /* even/odd vertex colouring (Gauss-Seidel)    */
+};
+
+/* Vertex flags, same bit values as YarnBall::VertexFlags (YarnBall.h:33-39). */
+enum {
+    KSCR_FLAG_HAS_PREV = 1,
+    KSCR_FLAG_HAS_NEXT = 2,
+    KSCR_FLAG_HAS_NEXT_ORIENTATION = 4,
+    KSCR_FLAG_FIX_ORIENTATION = 8
+};
+
+/* Simulation parameters: the MetaData fields the host sets
+ * (YarnBall.h:61-124) plus Sim::maxH (YBSimulator.h:26) and the two
+ * additions of this build (schedule, block_size).  kscr_params_default()
+ * fills the values of Sim::Sim (YBSimulator.cu:81-112). */
+typedef struct kscr_params {
+    float gravity[3];
+    int32_t num_itr;
+    float max_h;
+    float damping;
+    float velocity_decay;
+    float radius;
+    float barrier_thickness;
+    float acceleration_ratio;
+    float k_collision;
+    float k_friction;
+    float friction_coeff;
+    float detection_scaler;
+    float bvh_rebuild_period;
+    int32_t detection_period;
+    int32_t use_step_size_limit;
+    int32_t use_velocity_radius;
+    int32_t enable_hair_hair_coll;
+    float external_force_multiplier;
+    float wind_peak;
+    float wind_y_freq;
+    float wind_z_freq;
+    float wind_time_freq;
+    float wind_sharpness;
+    int32_t schedule;    /* KSCR_SCHEDULE_*                         */
+    int32_t block_size;  /* vertices per workgroup, 0 = autotuned   */
+} kscr_params;
+
+/* Sim::Sim defaults (YBSimulator.cu:81-112). */
+int kscr_params_default(kscr_params* p);
+
+/* Sim::Sim(numVerts) + YarnBall::createFromCurves (YB/io/reader.cpp:16-50):
+ * every vertex gets hasNext except the last one of each strand.  Strands
+ * must be stored contiguously; strand_offsets[0] = 0 and
+ * strand_offsets[num_strands] = num_verts.  device < 0 selects the last
+ * visible device, as SelectCuda does (StableCosseratRodsSolver.cpp:64-74). */
+int kscr_create(const kscr_params* p, int num_verts, const float* positions, int num_strands,
+                const int* strand_offsets, int device, kscr_handle** out);
+void kscr_destroy(kscr_handle* h);
+
+/* Sim::configure (YBSimulator.cu:161-274): flags, rest lengths, lumped
+ * masses, initial and rest orientations; allocates device memory. */
+int kscr_configure(kscr_handle* h, float density);
+/* Sim::setYoung (YBSimulator.cu:300-320), young in GPa. */
+int kscr_set_young(kscr_handle* h, float young_gpa, float deviation);
+/* Sim::setKStretch / Sim::setKBend (YBSimulator.cu:276-298). */
+int kscr_set_kstretch(kscr_handle* h, float k);
+int kscr_set_kbend(kscr_handle* h, float k);
+/* Pins vertices: invMass = 0 (StableCosseratRodsSolver.cpp:203-205). */
+int kscr_pin_vertices(kscr_handle* h, const int* ids, int n);
+/* Sets VertexFlags::fixOrientation on vertices (a true clamp of a root
+ * segment; the flag is honoured by cosserat.cu:303). */
+int kscr_fix_orientation(kscr_handle* h, const int* ids, int n);
+/* Replaces the simulation parameters; runs the uploadMeta validation
+ * (YBSimulator.cu:322-335). */
+int kscr_set_params(kscr_handle* h, const kscr_params* p);
+int kscr_get_params(const kscr_handle* h, kscr_params* p);
+/* Sim::upload (YBSimulator.cu:345-353): host mirror -> device. */
+int kscr_upload(kscr_handle* h);
+
+/* Sim::uploadPosAndVel / downloadPosAndVel (YBSimulator.cu:375-386). */
+int kscr_set_pos_vel(kscr_handle* h, const float* positions, const float* velocities);
+int kscr_get_pos_vel(kscr_handle* h, float* positions, float* velocities);
+/* Orientation state q (float[num_verts*4]) in and out. */
+int kscr_get_q(kscr_handle* h, float* q);
+int kscr_set_q(kscr_handle* h, const float* q);
+/* Rest state after configure/setYoung (host mirror; synchronous). */
+int kscr_get_rest(const kscr_handle* h, float* inv_mass, float* l_rest, float* k_stretch, uint32_t* flags,
+                  float* q_rest);
+
+/* Body SDF on a regular grid (SDFBodyCollisionSolver.cpp:122-167 and
+ * src/PNMesh.cpp:109-171): node (i,j,k) sits at min + (i,j,k)*ds with
+ * ds = (max - min)/dim; phi[dim.x*dim.y*dim.z] and grad[3*...] are laid out
+ * x fastest.  rotation (x,y,z,w) and translation place the body grid in the
+ * world.  threshold is SDFThreshold (SDFBodyCollisionSolver.h:25). */
+int kscr_set_sdf(kscr_handle* h, const int dim[3], const float grid_min[3], const float grid_max[3],
+                 const float* phi, const float* grad, float threshold, const float rotation[4],
+                 const float translation[3]);
+int kscr_enable_sdf(kscr_handle* h, int enable);
+
+/* Sim::advance (YB/sim/step.cpp:38-62): splits dt [s] into
+ * S = max(1, ceil(dt/max_h)) substeps of numItr VBD iterations. */
+int kscr_advance(kscr_handle* h, float dt);
+/* StableCosseratRodsSolver::SolveNextStep (StableCosseratRodsSolver.cpp:216-234)
+ * without the host round trips: advance, then the SDF pass on device. */
+int kscr_solve_next_step(kscr_handle* h, float dt);
+
+/* Execution control.  stream is a hipStream_t (NULL = the null stream). */
+int kscr_set_stream(kscr_handle* h, void* stream);
+int kscr_synchronize(kscr_handle* h);
+int kscr_num_verts(const kscr_handle* h);
+int kscr_num_strands(const kscr_handle* h);
+/* Vertices per workgroup actually used by the step kernel. */
+int kscr_block_size(const kscr_handle* h);
+/* Copies the last error message (NUL terminated) into buf. */
+int kscr_last_error(const kscr_handle* h, char* buf, size_t n);
+/* Library-wide message of the last kscr_create failure. */
+const char* kscr_create_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KSCR_H_ */

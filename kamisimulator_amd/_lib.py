@@ -1,0 +1,283 @@
+"""ctypes binding of libkscr.so (the C ABI of include/kscr.h).
+
+The library is the product path.  If it is missing this module raises: there
+is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libkscr.so")
+
+SCHEDULE_JACOBI = 0
+SCHEDULE_COLORED = 1
+
+FLAG_HAS_PREV = 1
+FLAG_HAS_NEXT = 2
+FLAG_HAS_NEXT_ORIENTATION = 4
+FLAG_FIX_ORIENTATION = 8
+
+STATUS = {
+    0: "KSCR_OK",
+    1: "KSCR_ERR_INVALID_ARGUMENT",
+    2: "KSCR_ERR_TOO_FEW_VERTICES",
+    3: "KSCR_ERR_DANGLING_SEGMENT",
+    4: "KSCR_ERR_ZERO_LENGTH_SEGMENT",
+    5: "KSCR_ERR_DETECTION_SCALER",
+    6: "KSCR_ERR_SEGMENT_TOO_SHORT",
+    7: "KSCR_ERR_NOT_CONFIGURED",
+    8: "KSCR_ERR_MAX_COLLISIONS",
+    9: "KSCR_ERR_STRAND_TOO_LONG",
+    10: "KSCR_ERR_HIP",
+    11: "KSCR_ERR_OUT_OF_MEMORY",
+    12: "KSCR_ERR_UNSUPPORTED",
+}
+
+
+class KscrParams(C.Structure):
+    """Mirror of ``kscr_params`` (include/kscr.h)."""
+
+    _fields_ = [
+        ("gravity", C.c_float * 3),
+        ("num_itr", C.c_int32),
+        ("max_h", C.c_float),
+        ("damping", C.c_float),
+        ("velocity_decay", C.c_float),
+        ("radius", C.c_float),
+        ("barrier_thickness", C.c_float),
+        ("acceleration_ratio", C.c_float),
+        ("k_collision", C.c_float),
+        ("k_friction", C.c_float),
+        ("friction_coeff", C.c_float),
+        ("detection_scaler", C.c_float),
+        ("bvh_rebuild_period", C.c_float),
+        ("detection_period", C.c_int32),
+        ("use_step_size_limit", C.c_int32),
+        ("use_velocity_radius", C.c_int32),
+        ("enable_hair_hair_coll", C.c_int32),
+        ("external_force_multiplier", C.c_float),
+        ("wind_peak", C.c_float),
+        ("wind_y_freq", C.c_float),
+        ("wind_z_freq", C.c_float),
+        ("wind_time_freq", C.c_float),
+        ("wind_sharpness", C.c_float),
+        ("schedule", C.c_int32),
+        ("block_size", C.c_int32),
+    ]
+
+
+# Every entry point declared in include/kscr.h, with its ctypes signature.
+_P = C.c_void_p
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
+_UP = C.POINTER(C.c_uint32)
+SIGNATURES = {
+    "kscr_params_default": (C.c_int, [C.POINTER(KscrParams)]),
+    "kscr_create": (C.c_int, [C.POINTER(KscrParams), C.c_int, _FP, C.c_int, _IP, C.c_int, C.POINTER(_P)]),
+    "kscr_destroy": (None, [_P]),
+    "kscr_configure": (C.c_int, [_P, C.c_float]),
+    "kscr_set_young": (C.c_int, [_P, C.c_float, C.c_float]),
+    "kscr_set_kstretch": (C.c_int, [_P, C.c_float]),
+    "kscr_set_kbend": (C.c_int, [_P, C.c_float]),
+    "kscr_pin_vertices": (C.c_int, [_P, _IP, C.c_int]),
+    "kscr_fix_orientation": (C.c_int, [_P, _IP, C.c_int]),
+    "kscr_set_params": (C.c_int, [_P, C.POINTER(KscrParams)]),
+    "kscr_get_params": (C.c_int, [_P, C.POINTER(KscrParams)]),
+    "kscr_upload": (C.c_int, [_P]),
+    "kscr_set_pos_vel": (C.c_int, [_P, _FP, _FP]),
+    "kscr_get_pos_vel": (C.c_int, [_P, _FP, _FP]),
+    "kscr_get_q": (C.c_int, [_P, _FP]),
+    "kscr_set_q": (C.c_int, [_P, _FP]),
+    "kscr_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
+    "kscr_set_sdf": (C.c_int, [_P, _IP, _FP, _FP, _FP, _FP, C.c_float, _FP, _FP]),
+    "kscr_enable_sdf": (C.c_int, [_P, C.c_int]),
+    "kscr_advance": (C.c_int, [_P, C.c_float]),
+    "kscr_solve_next_step": (C.c_int, [_P, C.c_float]),
+    "kscr_set_stream": (C.c_int, [_P, _P]),
+    "kscr_synchronize": (C.c_int, [_P]),
+    "kscr_num_verts": (C.c_int, [_P]),
+    "kscr_num_strands": (C.c_int, [_P]),
+    "kscr_block_size": (C.c_int, [_P]),
+    "kscr_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    "kscr_create_error": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libkscr.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m kamisimulator_amd.build` "
+            "(the Stable Cosserat Rods step has no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class KscrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def fptr(a):
+    return a.ctypes.data_as(_FP)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_IP)
+
+
+def default_params(**kw):
+    lib = load()
+    p = KscrParams()
+    lib.kscr_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity = (C.c_float * 3)(*v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def params_to_dict(p):
+    d = {}
+    for name, _ in KscrParams._fields_:
+        val = getattr(p, name)
+        d[name] = list(val) if name == "gravity" else val
+    return d
+
+
+class Sim:
+    """Python view of one ``kscr_handle`` (mirrors YarnBall::Sim's lifecycle)."""
+
+    def __init__(self, params, positions, strand_offsets, device=-1):
+        self.lib = load()
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        off = np.ascontiguousarray(strand_offsets, dtype=np.int32)
+        self.n = pos.shape[0]
+        self.num_strands = off.shape[0] - 1
+        self.strand_offsets = off
+        h = C.c_void_p()
+        rc = self.lib.kscr_create(C.byref(params), self.n, fptr(pos), self.num_strands, iptr(off), device, C.byref(h))
+        if rc:
+            raise KscrError(rc, self.lib.kscr_create_error().decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc:
+            buf = C.create_string_buffer(512)
+            self.lib.kscr_last_error(self.h, buf, 512)
+            raise KscrError(rc, buf.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kscr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, density):
+        self._check(self.lib.kscr_configure(self.h, density))
+
+    def set_young(self, young_gpa, deviation=1.0):
+        self._check(self.lib.kscr_set_young(self.h, young_gpa, deviation))
+
+    def set_kstretch(self, k):
+        self._check(self.lib.kscr_set_kstretch(self.h, k))
+
+    def set_kbend(self, k):
+        self._check(self.lib.kscr_set_kbend(self.h, k))
+
+    def pin(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        self._check(self.lib.kscr_pin_vertices(self.h, iptr(ids), ids.size))
+
+    def fix_orientation(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        self._check(self.lib.kscr_fix_orientation(self.h, iptr(ids), ids.size))
+
+    def set_params(self, params):
+        self._check(self.lib.kscr_set_params(self.h, C.byref(params)))
+
+    def get_params(self):
+        p = KscrParams()
+        self._check(self.lib.kscr_get_params(self.h, C.byref(p)))
+        return p
+
+    def upload(self):
+        self._check(self.lib.kscr_upload(self.h))
+
+    def set_pos_vel(self, pos=None, vel=None):
+        p = None if pos is None else np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
+        self._check(self.lib.kscr_set_pos_vel(self.h, None if p is None else fptr(p), None if v is None else fptr(v)))
+
+    def get_pos_vel(self):
+        p = np.empty((self.n, 3), np.float32)
+        v = np.empty((self.n, 3), np.float32)
+        self._check(self.lib.kscr_get_pos_vel(self.h, fptr(p), fptr(v)))
+        return p, v
+
+    def get_q(self):
+        q = np.empty((self.n, 4), np.float32)
+        self._check(self.lib.kscr_get_q(self.h, fptr(q)))
+        return q
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float32).reshape(-1)
+        self._check(self.lib.kscr_set_q(self.h, fptr(q)))
+
+    def get_rest(self):
+        im = np.empty(self.n, np.float32)
+        lr = np.empty(self.n, np.float32)
+        ks = np.empty(self.n, np.float32)
+        fl = np.empty(self.n, np.uint32)
+        qr = np.empty((self.n, 4), np.float32)
+        self._check(self.lib.kscr_get_rest(self.h, fptr(im), fptr(lr), fptr(ks), fl.ctypes.data_as(_UP), fptr(qr)))
+        return dict(inv_mass=im, l_rest=lr, k_stretch=ks, flags=fl, q_rest=qr)
+
+    def set_sdf(self, grid, threshold=0.0, rotation=(0, 0, 0, 1), translation=(0, 0, 0)):
+        dim = np.ascontiguousarray(grid.dim, dtype=np.int32)
+        gmin = np.ascontiguousarray(grid.gmin, dtype=np.float32)
+        gmax = np.ascontiguousarray(grid.gmax, dtype=np.float32)
+        phi = np.ascontiguousarray(grid.phi, dtype=np.float32).reshape(-1)
+        grad = np.ascontiguousarray(grid.grad, dtype=np.float32).reshape(-1)
+        rot = np.ascontiguousarray(rotation, dtype=np.float32)
+        tr = np.ascontiguousarray(translation, dtype=np.float32)
+        self._check(self.lib.kscr_set_sdf(self.h, iptr(dim), fptr(gmin), fptr(gmax), fptr(phi), fptr(grad),
+                                          threshold, fptr(rot), fptr(tr)))
+
+    def enable_sdf(self, on=True):
+        self._check(self.lib.kscr_enable_sdf(self.h, int(bool(on))))
+
+    def advance(self, dt):
+        self._check(self.lib.kscr_advance(self.h, dt))
+
+    def solve_next_step(self, dt):
+        self._check(self.lib.kscr_solve_next_step(self.h, dt))
+
+    def set_stream(self, stream_ptr):
+        self._check(self.lib.kscr_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def synchronize(self):
+        self._check(self.lib.kscr_synchronize(self.h))
+
+    @property
+    def block_size(self):
+        return self.lib.kscr_block_size(self.h)

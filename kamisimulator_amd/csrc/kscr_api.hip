@@ -1,0 +1,626 @@
+// kscr_api.hip -- host side of the C ABI declared in include/kscr.h.
+//
+// The handle mirrors YarnBall::Sim (YB/YBSimulator.h:18-99): a host copy of
+// the per-vertex state that setup code fills (configure, setYoung, setK*),
+// device buffers owned by the handle, and the step driver.  Setup follows the
+// reference's host code expression for expression, including its float and
+// double mixing:
+//   Sim::Sim                 YB/YBSimulator.cu:81-127
+//   createFromCurves         YB/io/reader.cpp:16-50
+//   Sim::configure           YB/YBSimulator.cu:161-274
+//   setKStretch/KBend/Young  YB/YBSimulator.cu:276-320
+//   uploadMeta validation    YB/YBSimulator.cu:322-335
+//   Sim::advance             YB/sim/step.cpp:38-62
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kscr.h"
+#include "kscr_device.h"
+#include "kscr_math.h"
+
+using namespace kscr;
+
+static_assert(sizeof(kscr_params) == 27 * 4, "kscr_params layout is part of the ABI");
+
+namespace {
+const double kPi = 3.141592653589793238462643;
+
+thread_local std::string g_create_error;
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+}  // namespace
+
+struct kscr_handle {
+    kscr_params p{};
+    int n = 0;
+    int device = 0;
+    std::vector<int> strand_off;
+    // host mirror (YarnBall::Sim::verts/vels/qs/qRests)
+    std::vector<float> pos, vel;
+    std::vector<float> inv_mass, l_rest, k_stretch;
+    std::vector<uint32_t> flags;
+    std::vector<f4> q, q_rest;
+    float min_seg = FLT_MAX, max_seg = 0, density = 0;
+    bool configured = false;
+    // step state
+    float h = 4e-4f, last_h = 4e-4f, time = 0.f;
+    long step_counter = 0;
+    // device
+    DevBuf<float> d_x, d_v, d_u;
+    DevBuf<float4> d_q, d_qr, d_mat;
+    DevBuf<int2> d_packs;
+    DevBuf<float4> d_grid;
+    int num_packs = 0;
+    int block = 0;
+    bool uploaded = false;
+    hipStream_t stream = nullptr;
+    // SDF
+    bool sdf_set = false, sdf_on = false;
+    SdfArgs sdf{};
+    std::string err;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return KSCR_ERR_HIP;
+    }
+};
+
+namespace {
+
+int validate_meta(kscr_handle* h) {
+    // uploadMeta (YBSimulator.cu:322-335)
+    if (h->p.detection_scaler <= 1) return h->fail(KSCR_ERR_DETECTION_SCALER, "Detection scaler must be greater than 1.");
+    if (2 * h->min_seg < 2 * h->p.radius + h->p.barrier_thickness)
+        return h->fail(KSCR_ERR_SEGMENT_TOO_SHORT,
+                       "Use thinner yarn or use longer segments. (2 * min_seg_length must be > 2 * radius + "
+                       "barrier_thickness)");
+    if (h->p.num_itr < 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr must be >= 0");
+    if (!(h->p.max_h > 0)) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "max_h must be > 0");
+    if (h->p.enable_hair_hair_coll)
+        return h->fail(KSCR_ERR_UNSUPPORTED, "hair-hair collision requires the contact step path (not in this library build)");
+    if (h->p.schedule != KSCR_SCHEDULE_JACOBI)
+        return h->fail(KSCR_ERR_UNSUPPORTED, "only the Jacobi schedule is available in this library build");
+    return KSCR_OK;
+}
+
+int choose_block(const kscr_handle* h) {
+    int longest = 0;
+    const int ns = (int)h->strand_off.size() - 1;
+    for (int s = 0; s < ns; s++) longest = std::max(longest, h->strand_off[s + 1] - h->strand_off[s]);
+    if (h->p.block_size > 0) return h->p.block_size;
+    // Smallest instantiated block that holds the longest strand, but at least
+    // 512 so that several ~100-vertex strands share a workgroup (fewer idle lanes).
+    for (int b : kBlockSizes)
+        if (b >= longest && b >= 512) return b;
+    for (int b : kBlockSizes)
+        if (b >= longest) return b;
+    return -1;
+}
+
+// Greedy packing of whole strands into workgroups of at most `block` vertices.
+int build_packs(kscr_handle* h, int block, std::vector<int2>& packs) {
+    packs.clear();
+    const int ns = (int)h->strand_off.size() - 1;
+    int s = 0;
+    while (s < ns) {
+        const int beg = h->strand_off[s];
+        int end = beg;
+        while (s < ns && h->strand_off[s + 1] - beg <= block) {
+            end = h->strand_off[s + 1];
+            s++;
+        }
+        if (end == beg)
+            return h->fail(KSCR_ERR_STRAND_TOO_LONG, "strand " + std::to_string(s) + " has " +
+                                                         std::to_string(h->strand_off[s + 1] - h->strand_off[s]) +
+                                                         " vertices, more than the workgroup size " + std::to_string(block));
+        packs.push_back(make_int2(beg, end));
+    }
+    // The kernel reads neighbours at t-1 / t+1 only through hasPrev / hasNext;
+    // a pack must therefore start without hasPrev and end without hasNext.
+    for (const int2& pk : packs) {
+        if (h->flags[pk.x] & FLAG_PREV) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "pack starts inside a strand");
+        if (h->flags[pk.y - 1] & FLAG_NEXT) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "pack ends inside a strand");
+    }
+    return KSCR_OK;
+}
+
+int upload_all(kscr_handle* h) {
+    const int n = h->n;
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    std::vector<float4> mat(n), qv(n), qr(n);
+    for (int i = 0; i < n; i++) {
+        float fb;
+        std::memcpy(&fb, &h->flags[i], 4);
+        mat[i] = make_float4(h->inv_mass[i], h->l_rest[i], h->k_stretch[i], fb);
+        qv[i] = make_float4(h->q[i].x, h->q[i].y, h->q[i].z, h->q[i].w);
+        qr[i] = make_float4(h->q_rest[i].x, h->q_rest[i].y, h->q_rest[i].z, h->q_rest[i].w);
+    }
+    int block = choose_block(h);
+    if (block < 0) return h->fail(KSCR_ERR_STRAND_TOO_LONG, "a strand is longer than the largest workgroup (1024)");
+    std::vector<int2> packs;
+    int rc = build_packs(h, block, packs);
+    if (rc) return rc;
+    h->block = block;
+    h->num_packs = (int)packs.size();
+    if ((e = h->d_packs.alloc(packs.size())) != hipSuccess) return h->hip_fail(e, "hipMalloc packs");
+    const size_t n3 = (size_t)n * 3 * sizeof(float);
+    if ((e = hipMemcpyAsync(h->d_x.p, h->pos.data(), n3, hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->d_v.p, h->vel.data(), n3, hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->d_q.p, qv.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->d_qr.p, qr.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->d_mat.p, mat.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->d_packs.p, packs.data(), packs.size() * sizeof(int2), hipMemcpyHostToDevice, h->stream)) !=
+            hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync upload");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    h->uploaded = true;
+    return KSCR_OK;
+}
+
+int launch_step(kscr_handle* h, float dt, bool with_sdf) {
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    if (!(dt > 0)) return KSCR_OK;  // Sim::advance returns for h <= 0
+    int rc = validate_meta(h);
+    if (rc) return rc;
+    const int steps = std::max(1, (int)std::ceil(dt / h->p.max_h));
+    h->last_h = h->h;
+    h->h = dt / steps;
+    StepArgs a{};
+    a.x = h->d_x.p;
+    a.v = h->d_v.p;
+    a.u = h->d_u.p;
+    a.q = h->d_q.p;
+    a.q_rest = h->d_qr.p;
+    a.mat = h->d_mat.p;
+    a.packs = h->d_packs.p;
+    a.num_packs = h->num_packs;
+    a.num_substeps = steps;
+    a.num_itr = h->p.num_itr;
+    a.h = h->h;
+    a.last_h = h->last_h;
+    a.time = h->time;
+    for (int k = 0; k < 3; k++) a.gravity[k] = h->p.gravity[k];
+    a.damping = h->p.damping;
+    a.velocity_decay = h->p.velocity_decay;
+    a.acceleration_ratio = h->p.acceleration_ratio;
+    a.external_force_multiplier = h->p.external_force_multiplier;
+    a.wind_peak = h->p.wind_peak;
+    a.wind_y_freq = h->p.wind_y_freq;
+    a.wind_z_freq = h->p.wind_z_freq;
+    a.wind_time_freq = h->p.wind_time_freq;
+    a.wind_sharpness = h->p.wind_sharpness;
+    a.sdf_on = (with_sdf && h->sdf_on && h->sdf_set) ? 1 : 0;
+    a.sdf = h->sdf;
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    e = launch_step_fused(a, h->block, h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
+    h->step_counter += steps;
+    h->time += dt;
+    return KSCR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kscr_params_default(kscr_params* p) {
+    if (!p) return KSCR_ERR_INVALID_ARGUMENT;
+    std::memset(p, 0, sizeof(*p));
+    p->gravity[0] = 0.f;
+    p->gravity[1] = -9.8f;
+    p->gravity[2] = 0.f;
+    p->num_itr = 8;
+    p->max_h = 4e-4f;
+    p->velocity_decay = 0.2f;
+    p->damping = 1e-6f;
+    p->radius = 1e-4f;
+    p->barrier_thickness = 8e-4f;
+    p->acceleration_ratio = 1.f;
+    p->k_collision = 1e-5f;
+    p->detection_scaler = 1.2f;
+    p->friction_coeff = 0.1f;
+    p->k_friction = 5.f;
+    p->external_force_multiplier = 1.f;
+    p->detection_period = 1;
+    p->use_step_size_limit = 0;
+    p->use_velocity_radius = 1;
+    p->bvh_rebuild_period = 1 / 10.f;
+    p->enable_hair_hair_coll = 0;
+    p->wind_peak = 0.f;
+    p->wind_y_freq = 0.f;
+    p->wind_z_freq = 0.f;
+    p->wind_time_freq = 0.f;
+    p->wind_sharpness = 1.f;
+    p->schedule = KSCR_SCHEDULE_JACOBI;
+    p->block_size = 0;
+    return KSCR_OK;
+}
+
+const char* kscr_create_error(void) { return g_create_error.c_str(); }
+
+int kscr_create(const kscr_params* p, int num_verts, const float* positions, int num_strands, const int* strand_offsets,
+                int device, kscr_handle** out) {
+    g_create_error.clear();
+    if (!p || !positions || !strand_offsets || !out || num_strands <= 0) {
+        g_create_error = "null argument or no strands";
+        return KSCR_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (num_verts < 3) {
+        g_create_error = "Too little vertices";
+        return KSCR_ERR_TOO_FEW_VERTICES;
+    }
+    if (strand_offsets[0] != 0 || strand_offsets[num_strands] != num_verts) {
+        g_create_error = "strand_offsets must start at 0 and end at num_verts";
+        return KSCR_ERR_INVALID_ARGUMENT;
+    }
+    for (int s = 0; s < num_strands; s++)
+        if (strand_offsets[s + 1] <= strand_offsets[s]) {
+            g_create_error = "strand_offsets must be strictly increasing";
+            return KSCR_ERR_INVALID_ARGUMENT;
+        }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        g_create_error = std::string("no HIP device: ") + hipGetErrorString(e);
+        return KSCR_ERR_HIP;
+    }
+    if (device < 0) device = count - 1;
+    if (device >= count) {
+        g_create_error = "device ordinal out of range";
+        return KSCR_ERR_INVALID_ARGUMENT;
+    }
+    kscr_handle* h = new kscr_handle;
+    h->p = *p;
+    h->n = num_verts;
+    h->device = device;
+    h->strand_off.assign(strand_offsets, strand_offsets + num_strands + 1);
+    h->pos.assign(positions, positions + (size_t)num_verts * 3);
+    h->vel.assign((size_t)num_verts * 3, 0.f);
+    // Sim::Sim initial values (YBSimulator.cu:118-126)
+    h->inv_mass.assign(num_verts, 1.f);
+    h->l_rest.assign(num_verts, 1.f);
+    h->k_stretch.assign(num_verts, 100.f);
+    h->flags.assign(num_verts, FLAG_NEXT);
+    h->q.assign(num_verts, f4{0, 0, 0, 1});
+    h->q_rest.assign(num_verts, f4{0, 0, 0, 1});
+    h->flags[num_verts - 1] = 0;
+    // createFromCurves: cut off the end of each curve
+    for (int s = 0; s < num_strands; s++) h->flags[strand_offsets[s + 1] - 1] = 0;
+    h->h = p->max_h;
+    h->last_h = p->max_h;
+    *out = h;
+    return KSCR_OK;
+}
+
+void kscr_destroy(kscr_handle* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+}
+
+int kscr_configure(kscr_handle* h, float density) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    const int n = h->n;
+    h->density = density;
+    h->max_seg = 0;
+    h->min_seg = FLT_MAX;
+    f4 lastQ{0, 0, 0, 1};
+    for (int i = 0; i < n; i++) {
+        if (i < n - 1) {
+            const bool hasNext = (h->flags[i] & FLAG_NEXT) != 0;
+            h->flags[i + 1] = hasNext ? (h->flags[i + 1] | FLAG_PREV) : (h->flags[i + 1] & ~FLAG_PREV);
+            const bool hasNextNext = hasNext && (h->flags[i + 1] & FLAG_NEXT);
+            h->flags[i] = hasNextNext ? (h->flags[i] | FLAG_NEXT_ORI) : (h->flags[i] & ~FLAG_NEXT_ORI);
+            if (!hasNext) h->flags[i] |= FLAG_FIX_ORI;
+        }
+        if (!(h->flags[i] & FLAG_PREV) && !(h->flags[i] & FLAG_NEXT))
+            return h->fail(KSCR_ERR_DANGLING_SEGMENT, "Dangling segment. Yarns must be atleast 2 segments long");
+        h->l_rest[i] = 1.f / n;
+        float mass = 0;
+        if (h->flags[i] & FLAG_PREV) mass += h->l_rest[i - 1];
+        if (h->flags[i] & FLAG_NEXT) {
+            const f3 a{h->pos[3 * i], h->pos[3 * i + 1], h->pos[3 * i + 2]};
+            const f3 b{h->pos[3 * i + 3], h->pos[3 * i + 4], h->pos[3 * i + 5]};
+            const f3 seg = b - a;
+            h->l_rest[i] = length(seg);
+            if (h->l_rest[i] == 0 || !std::isfinite(h->l_rest[i]))
+                return h->fail(KSCR_ERR_ZERO_LENGTH_SEGMENT, "0 length segment at vertex " + std::to_string(i));
+            lastQ = from_to(f3{1, 0, 0}, normalize(seg));
+            mass += h->l_rest[i];
+            h->max_seg = std::max(h->max_seg, h->l_rest[i]);
+            h->min_seg = std::min(h->min_seg, h->l_rest[i]);
+        }
+        h->q[i] = lastQ;
+        // mass = (l_{i-1} + l_i) * r^2 * pi * density / 2, with the reference's
+        // float * double promotion
+        mass = (float)((double)mass * ((double)(h->p.radius * h->p.radius) * kPi * (double)density * (double)0.5f));
+        if (mass != 0)
+            h->inv_mass[i] *= 1 / mass;
+        else
+            h->inv_mass[i] = 0;
+    }
+    for (int i = 0; i < n - 1; i++) h->q_rest[i] = normalize(rmul(conj(h->q[i]), h->q[i + 1]));
+
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if (!h->stream) {
+        // the null stream unless the caller installs one with kscr_set_stream
+    }
+    if ((e = h->d_x.alloc((size_t)n * 3)) != hipSuccess || (e = h->d_v.alloc((size_t)n * 3)) != hipSuccess ||
+        (e = h->d_u.alloc((size_t)n * 3)) != hipSuccess || (e = h->d_q.alloc(n)) != hipSuccess ||
+        (e = h->d_qr.alloc(n)) != hipSuccess || (e = h->d_mat.alloc(n)) != hipSuccess)
+        return h->hip_fail(e, "hipMalloc state");
+    // lastVels start at zero (YBSimulator.cu:253-254)
+    if ((e = hipMemsetAsync(h->d_u.p, 0, (size_t)n * 3 * sizeof(float), h->stream)) != hipSuccess)
+        return h->hip_fail(e, "hipMemsetAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    h->configured = true;
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_set_young(kscr_handle* h, float young, float deviation) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->configured) return h->fail(KSCR_ERR_NOT_CONFIGURED, "No rest length. Must call configure()");
+    const float mmrad = h->p.radius * 1000;
+    for (int i = 0; i < h->n; i++) {
+        const float len = h->l_rest[i];
+        const float ss = (float)((double)young * kPi * (double)mmrad * (double)mmrad);
+        const float bt = (float)((double)(ss * deviation * mmrad * mmrad) * 0.25 / (double)(len * len));
+        h->k_stretch[i] = ss;
+        h->q_rest[i] = bt * h->q_rest[i];
+    }
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_set_kstretch(kscr_handle* h, float k) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->configured) return h->fail(KSCR_ERR_NOT_CONFIGURED, "No rest length. Must call configure()");
+    for (int i = 0; i < h->n; i++) h->k_stretch[i] = k * h->l_rest[i];
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_set_kbend(kscr_handle* h, float k) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->configured) return h->fail(KSCR_ERR_NOT_CONFIGURED, "No rest length. Must call configure()");
+    k *= 4;
+    for (int i = 0; i < h->n; i++) h->q_rest[i] = (k / h->l_rest[i]) * normalize(h->q_rest[i]);
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_pin_vertices(kscr_handle* h, const int* ids, int n) {
+    if (!h || (n > 0 && !ids)) return KSCR_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < n; k++) {
+        if (ids[k] < 0 || ids[k] >= h->n) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "vertex id out of range");
+        h->inv_mass[ids[k]] = 0;
+    }
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_fix_orientation(kscr_handle* h, const int* ids, int n) {
+    if (!h || (n > 0 && !ids)) return KSCR_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < n; k++) {
+        if (ids[k] < 0 || ids[k] >= h->n) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "vertex id out of range");
+        h->flags[ids[k]] |= FLAG_FIX_ORI;
+    }
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_set_params(kscr_handle* h, const kscr_params* p) {
+    if (!h || !p) return KSCR_ERR_INVALID_ARGUMENT;
+    const int old_block = h->p.block_size;
+    h->p = *p;
+    if (h->configured) {
+        int rc = validate_meta(h);
+        if (rc) return rc;
+    }
+    if (p->block_size != old_block) h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_get_params(const kscr_handle* h, kscr_params* p) {
+    if (!h || !p) return KSCR_ERR_INVALID_ARGUMENT;
+    *p = h->p;
+    return KSCR_OK;
+}
+
+int kscr_upload(kscr_handle* h) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->configured) return h->fail(KSCR_ERR_NOT_CONFIGURED, "Must call configure() before upload()");
+    int rc = validate_meta(h);
+    if (rc) return rc;
+    return upload_all(h);
+}
+
+int kscr_set_pos_vel(kscr_handle* h, const float* positions, const float* velocities) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    const size_t n3 = (size_t)h->n * 3;
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if (positions) {
+        std::memcpy(h->pos.data(), positions, n3 * sizeof(float));
+        if ((e = hipMemcpyAsync(h->d_x.p, h->pos.data(), n3 * sizeof(float), hipMemcpyHostToDevice, h->stream)) !=
+            hipSuccess)
+            return h->hip_fail(e, "hipMemcpyAsync");
+    }
+    if (velocities) {
+        std::memcpy(h->vel.data(), velocities, n3 * sizeof(float));
+        if ((e = hipMemcpyAsync(h->d_v.p, h->vel.data(), n3 * sizeof(float), hipMemcpyHostToDevice, h->stream)) !=
+            hipSuccess)
+            return h->hip_fail(e, "hipMemcpyAsync");
+    }
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    return KSCR_OK;
+}
+
+int kscr_get_pos_vel(kscr_handle* h, float* positions, float* velocities) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    const size_t n3 = (size_t)h->n * 3;
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = hipMemcpyAsync(h->pos.data(), h->d_x.p, n3 * sizeof(float), hipMemcpyDeviceToHost, h->stream)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(h->vel.data(), h->d_v.p, n3 * sizeof(float), hipMemcpyDeviceToHost, h->stream)) !=
+            hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    if (positions) std::memcpy(positions, h->pos.data(), n3 * sizeof(float));
+    if (velocities) std::memcpy(velocities, h->vel.data(), n3 * sizeof(float));
+    return KSCR_OK;
+}
+
+int kscr_get_q(kscr_handle* h, float* q) {
+    if (!h || !q) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = hipMemcpyAsync(q, h->d_q.p, (size_t)h->n * sizeof(float4), hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    for (int i = 0; i < h->n; i++) h->q[i] = f4{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+    return KSCR_OK;
+}
+
+int kscr_set_q(kscr_handle* h, const float* q) {
+    if (!h || !q) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    for (int i = 0; i < h->n; i++) h->q[i] = f4{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = hipMemcpyAsync(h->d_q.p, h->q.data(), (size_t)h->n * sizeof(float4), hipMemcpyHostToDevice, h->stream)) !=
+        hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    return KSCR_OK;
+}
+
+int kscr_get_rest(const kscr_handle* h, float* inv_mass, float* l_rest, float* k_stretch, uint32_t* flags,
+                  float* q_rest) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    for (int i = 0; i < h->n; i++) {
+        if (inv_mass) inv_mass[i] = h->inv_mass[i];
+        if (l_rest) l_rest[i] = h->l_rest[i];
+        if (k_stretch) k_stretch[i] = h->k_stretch[i];
+        if (flags) flags[i] = h->flags[i];
+        if (q_rest) {
+            q_rest[4 * i] = h->q_rest[i].x;
+            q_rest[4 * i + 1] = h->q_rest[i].y;
+            q_rest[4 * i + 2] = h->q_rest[i].z;
+            q_rest[4 * i + 3] = h->q_rest[i].w;
+        }
+    }
+    return KSCR_OK;
+}
+
+int kscr_set_sdf(kscr_handle* h, const int dim[3], const float grid_min[3], const float grid_max[3], const float* phi,
+                 const float* grad, float threshold, const float rotation[4], const float translation[3]) {
+    if (!h || !dim || !grid_min || !grid_max || !phi || !grad) return KSCR_ERR_INVALID_ARGUMENT;
+    size_t cells = 1;
+    SdfArgs s{};
+    for (int a = 0; a < 3; a++) {
+        if (dim[a] <= 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "SDF dimensions must be positive");
+        s.dim[a] = dim[a];
+        s.gmin[a] = grid_min[a];
+        // SDFBodyCollisionSolver::InitParams (:148-159): float spacing and its inverse
+        const float dist = grid_max[a] - grid_min[a];
+        s.ds[a] = dist / (1.f * dim[a]);
+        s.inv_ds[a] = 1.f / s.ds[a];
+        cells *= (size_t)dim[a];
+    }
+    s.thresh = threshold;
+    for (int a = 0; a < 4; a++) s.rot[a] = rotation ? rotation[a] : (a == 3 ? 1.f : 0.f);
+    for (int a = 0; a < 3; a++) s.trans[a] = translation ? translation[a] : 0.f;
+    std::vector<float4> g(cells);
+    for (size_t c = 0; c < cells; c++) g[c] = make_float4(grad[3 * c], grad[3 * c + 1], grad[3 * c + 2], phi[c]);
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = h->d_grid.alloc(cells)) != hipSuccess) return h->hip_fail(e, "hipMalloc SDF grid");
+    if ((e = hipMemcpyAsync(h->d_grid.p, g.data(), cells * sizeof(float4), hipMemcpyHostToDevice, h->stream)) !=
+        hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync SDF grid");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    s.grid = h->d_grid.p;
+    h->sdf = s;
+    h->sdf_set = true;
+    h->sdf_on = true;
+    return KSCR_OK;
+}
+
+int kscr_enable_sdf(kscr_handle* h, int enable) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    h->sdf_on = enable != 0;
+    return KSCR_OK;
+}
+
+int kscr_advance(kscr_handle* h, float dt) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    return launch_step(h, dt, false);
+}
+
+int kscr_solve_next_step(kscr_handle* h, float dt) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    return launch_step(h, dt, true);
+}
+
+int kscr_set_stream(kscr_handle* h, void* stream) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    h->stream = (hipStream_t)stream;
+    return KSCR_OK;
+}
+
+int kscr_synchronize(kscr_handle* h) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    return KSCR_OK;
+}
+
+int kscr_num_verts(const kscr_handle* h) { return h ? h->n : -1; }
+int kscr_num_strands(const kscr_handle* h) { return h ? (int)h->strand_off.size() - 1 : -1; }
+int kscr_block_size(const kscr_handle* h) { return h ? h->block : -1; }
+
+int kscr_last_error(const kscr_handle* h, char* buf, size_t n) {
+    if (!h || !buf || n == 0) return KSCR_ERR_INVALID_ARGUMENT;
+    std::snprintf(buf, n, "%s", h->err.c_str());
+    return KSCR_OK;
+}
+
+}  // extern "C"

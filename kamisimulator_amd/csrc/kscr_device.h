@@ -1,0 +1,63 @@
+// kscr_device.h -- device-side data layout of one simulation handle and the
+// launchers of the step kernels.
+//
+// HBM layout (structure of arrays, one entry per vertex, strands contiguous):
+//   x, v, u     float[3N]  position, velocity, velocity at the previous
+//                          substep start ("lastVels")          read/write
+//   q           float4[N]  segment orientation (x, y, z, w)    read/write
+//   q_rest      float4[N]  rest Darboux rotor; its norm is the bend stiffness
+//   mat         float4[N]  (invMass, lRest, kStretch, flags as bits)
+//   packs       int2[P]    vertex range [begin, end) of each workgroup; a
+//                          range always holds whole strands
+//   sdf grid    float4[dim.x*dim.y*dim.z]  (dphi/dx, dphi/dy, dphi/dz, phi)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kscr {
+
+struct SdfArgs {
+    const float4* grid;  // (grad, phi) per node, x fastest
+    int dim[3];
+    float gmin[3];
+    float ds[3];
+    float inv_ds[3];
+    float thresh;
+    float rot[4];
+    float trans[3];
+};
+
+struct StepArgs {
+    float* x;
+    float* v;
+    float* u;
+    float4* q;
+    const float4* q_rest;
+    const float4* mat;
+    const int2* packs;
+    int num_packs;
+    int num_substeps;
+    int num_itr;
+    float h;
+    float last_h;
+    float time;
+    float gravity[3];
+    float damping;
+    float velocity_decay;
+    float acceleration_ratio;
+    float external_force_multiplier;
+    float wind_peak, wind_y_freq, wind_z_freq, wind_time_freq, wind_sharpness;
+    int sdf_on;
+    SdfArgs sdf;
+};
+
+// One launch advances every strand by num_substeps substeps (and applies the
+// SDF pass afterwards when sdf_on).  block is the workgroup size in vertices.
+hipError_t launch_step_fused(const StepArgs& a, int block, hipStream_t stream);
+
+// Block sizes the fused kernel is instantiated for.
+constexpr int kBlockSizes[] = {64, 128, 256, 512, 1024};
+
+}  // namespace kscr

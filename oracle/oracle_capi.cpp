@@ -1,0 +1,276 @@
+// oracle_capi.cpp -- extern "C" access to the CPU restatement (scr_oracle.hpp).
+//
+// TEST INFRASTRUCTURE ONLY (see the header of scr_oracle.hpp).  The entry
+// points mirror include/kscr.h so that parity tests drive the oracle and the
+// HIP library with the same call sequence.
+#include <cstdint>
+#include <cstring>
+#include <memory>
+
+#include "scr_oracle.hpp"
+
+namespace {
+struct Base {
+    virtual ~Base() = default;
+    virtual int create(const orc::Params& p, int nv, const float* P, int ns, const int* off) = 0;
+    virtual int configure(float d) = 0;
+    virtual int set_young(float y, float dev) = 0;
+    virtual int set_kstretch(float k) = 0;
+    virtual int set_kbend(float k) = 0;
+    virtual void pin(const int* ids, int m) = 0;
+    virtual void fix(const int* ids, int m) = 0;
+    virtual int set_params(const orc::Params& p) = 0;
+    virtual int upload() = 0;
+    virtual void set_pos_vel(const float* p, const float* v) = 0;
+    virtual void get_pos_vel(float* p, float* v) const = 0;
+    virtual void get_pos_vel64(double* p, double* v) const = 0;
+    virtual void get_q(float* q) const = 0;
+    virtual void get_q64(double* q) const = 0;
+    virtual void set_q(const float* q) = 0;
+    virtual void get_rest(float* im, float* lr, float* ks, uint32_t* fl, float* qr) const = 0;
+    virtual orc::Sdf& sdf() = 0;
+    virtual void advance(float dt) = 0;
+    virtual void solve_next_step(float dt) = 0;
+    virtual void set_threads(int t) = 0;
+    virtual int nverts() const = 0;
+};
+
+template <class T>
+struct Impl : Base {
+    orc::Sim<T> s;
+    int create(const orc::Params& p, int nv, const float* P, int ns, const int* off) override {
+        return s.create(p, nv, P, ns, off);
+    }
+    int configure(float d) override { return s.configure(d); }
+    int set_young(float y, float dev) override { return s.set_young(y, dev); }
+    int set_kstretch(float k) override { return s.set_kstretch(k); }
+    int set_kbend(float k) override { return s.set_kbend(k); }
+    void pin(const int* ids, int m) override { s.pin(ids, m); }
+    void fix(const int* ids, int m) override { s.fix_orientation(ids, m); }
+    int set_params(const orc::Params& p) override {
+        s.p = p;
+        return s.configured ? s.validate() : 0;
+    }
+    int upload() override {
+        if (!s.configured) return 7;
+        int e = s.validate();
+        if (e) return e;
+        s.upload();
+        return 0;
+    }
+    void set_pos_vel(const float* p, const float* v) override {
+        for (int i = 0; i < s.n; i++) {
+            if (p) s.x[i] = orc::V3<T>{(T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2]};
+            if (v) s.v[i] = orc::V3<T>{(T)v[3 * i], (T)v[3 * i + 1], (T)v[3 * i + 2]};
+        }
+    }
+    void get_pos_vel(float* p, float* v) const override {
+        for (int i = 0; i < s.n; i++) {
+            if (p) {
+                p[3 * i] = (float)s.x[i].x;
+                p[3 * i + 1] = (float)s.x[i].y;
+                p[3 * i + 2] = (float)s.x[i].z;
+            }
+            if (v) {
+                v[3 * i] = (float)s.v[i].x;
+                v[3 * i + 1] = (float)s.v[i].y;
+                v[3 * i + 2] = (float)s.v[i].z;
+            }
+        }
+    }
+    void get_pos_vel64(double* p, double* v) const override {
+        for (int i = 0; i < s.n; i++) {
+            if (p) {
+                p[3 * i] = (double)s.x[i].x;
+                p[3 * i + 1] = (double)s.x[i].y;
+                p[3 * i + 2] = (double)s.x[i].z;
+            }
+            if (v) {
+                v[3 * i] = (double)s.v[i].x;
+                v[3 * i + 1] = (double)s.v[i].y;
+                v[3 * i + 2] = (double)s.v[i].z;
+            }
+        }
+    }
+    void get_q(float* q) const override {
+        for (int i = 0; i < s.n; i++) {
+            q[4 * i] = (float)s.q[i].x;
+            q[4 * i + 1] = (float)s.q[i].y;
+            q[4 * i + 2] = (float)s.q[i].z;
+            q[4 * i + 3] = (float)s.q[i].w;
+        }
+    }
+    void get_q64(double* q) const override {
+        for (int i = 0; i < s.n; i++) {
+            q[4 * i] = (double)s.q[i].x;
+            q[4 * i + 1] = (double)s.q[i].y;
+            q[4 * i + 2] = (double)s.q[i].z;
+            q[4 * i + 3] = (double)s.q[i].w;
+        }
+    }
+    void set_q(const float* q) override {
+        for (int i = 0; i < s.n; i++) s.q[i] = orc::R4<T>{(T)q[4 * i], (T)q[4 * i + 1], (T)q[4 * i + 2], (T)q[4 * i + 3]};
+    }
+    void get_rest(float* im, float* lr, float* ks, uint32_t* fl, float* qr) const override {
+        for (int i = 0; i < s.n; i++) {
+            if (im) im[i] = s.invMassF[i];
+            if (lr) lr[i] = s.lRestF[i];
+            if (ks) ks[i] = s.kStretchF[i];
+            if (fl) fl[i] = s.flags[i];
+            if (qr) {
+                qr[4 * i] = s.qRestF[i].x;
+                qr[4 * i + 1] = s.qRestF[i].y;
+                qr[4 * i + 2] = s.qRestF[i].z;
+                qr[4 * i + 3] = s.qRestF[i].w;
+            }
+        }
+    }
+    orc::Sdf& sdf() override { return s.sdf; }
+    void advance(float dt) override { s.advance(dt); }
+    void solve_next_step(float dt) override { s.solve_next_step(dt); }
+    void set_threads(int t) override { s.threads = t < 1 ? 1 : t; }
+    int nverts() const override { return s.n; }
+};
+}  // namespace
+
+static_assert(sizeof(orc::Params) == 27 * 4, "orc::Params must mirror kscr_params");
+
+extern "C" {
+
+struct oracle_handle {
+    std::unique_ptr<Base> b;
+};
+
+// precision: 32 (float restatement) or 64 (FP64 oracle)
+int oracle_create(int precision, const void* params, int nv, const float* pos, int ns, const int* off,
+                  oracle_handle** out) {
+    if (!params || !pos || !off || !out || nv <= 0 || ns <= 0) return 1;
+    auto* h = new oracle_handle;
+    if (precision == 64)
+        h->b.reset(new Impl<double>);
+    else
+        h->b.reset(new Impl<float>);
+    orc::Params p;
+    std::memcpy(&p, params, sizeof(p));
+    int e = h->b->create(p, nv, pos, ns, off);
+    if (e) {
+        delete h;
+        return e;
+    }
+    *out = h;
+    return 0;
+}
+void oracle_destroy(oracle_handle* h) { delete h; }
+int oracle_configure(oracle_handle* h, float density) { return h->b->configure(density); }
+int oracle_set_young(oracle_handle* h, float y, float dev) { return h->b->set_young(y, dev); }
+int oracle_set_kstretch(oracle_handle* h, float k) { return h->b->set_kstretch(k); }
+int oracle_set_kbend(oracle_handle* h, float k) { return h->b->set_kbend(k); }
+int oracle_pin_vertices(oracle_handle* h, const int* ids, int n) {
+    h->b->pin(ids, n);
+    return 0;
+}
+int oracle_fix_orientation(oracle_handle* h, const int* ids, int n) {
+    h->b->fix(ids, n);
+    return 0;
+}
+int oracle_set_params(oracle_handle* h, const void* params) {
+    orc::Params p;
+    std::memcpy(&p, params, sizeof(p));
+    return h->b->set_params(p);
+}
+int oracle_upload(oracle_handle* h) { return h->b->upload(); }
+int oracle_set_pos_vel(oracle_handle* h, const float* p, const float* v) {
+    h->b->set_pos_vel(p, v);
+    return 0;
+}
+int oracle_get_pos_vel(oracle_handle* h, float* p, float* v) {
+    h->b->get_pos_vel(p, v);
+    return 0;
+}
+int oracle_get_pos_vel64(oracle_handle* h, double* p, double* v) {
+    h->b->get_pos_vel64(p, v);
+    return 0;
+}
+int oracle_get_q(oracle_handle* h, float* q) {
+    h->b->get_q(q);
+    return 0;
+}
+int oracle_get_q64(oracle_handle* h, double* q) {
+    h->b->get_q64(q);
+    return 0;
+}
+int oracle_set_q(oracle_handle* h, const float* q) {
+    h->b->set_q(q);
+    return 0;
+}
+int oracle_get_rest(oracle_handle* h, float* im, float* lr, float* ks, uint32_t* fl, float* qr) {
+    h->b->get_rest(im, lr, ks, fl, qr);
+    return 0;
+}
+// Grid spacing as in SDFBodyCollisionSolver::InitParams (:142-159):
+// ds = (max - min) / (1.f * dim), invDs = 1.f / ds, all in float.
+int oracle_set_sdf(oracle_handle* h, const int* dim, const float* gmin, const float* gmax, const float* phi,
+                   const float* grad, float thresh, const float* rot, const float* trans) {
+    orc::Sdf& s = h->b->sdf();
+    size_t cells = 1;
+    for (int a = 0; a < 3; a++) {
+        if (dim[a] <= 0) return 1;
+        s.dim[a] = dim[a];
+        s.gmin[a] = gmin[a];
+        s.gmax[a] = gmax[a];
+        float dist = gmax[a] - gmin[a];
+        s.ds[a] = dist / (1.f * dim[a]);
+        s.inv_ds[a] = 1.f / s.ds[a];
+        cells *= (size_t)dim[a];
+    }
+    s.thresh = thresh;
+    for (int a = 0; a < 4; a++) s.rot[a] = rot ? rot[a] : (a == 3 ? 1.f : 0.f);
+    for (int a = 0; a < 3; a++) s.trans[a] = trans ? trans[a] : 0.f;
+    s.phi.assign(phi, phi + cells);
+    s.grad.assign(grad, grad + 3 * cells);
+    s.enabled = true;
+    return 0;
+}
+int oracle_enable_sdf(oracle_handle* h, int on) {
+    h->b->sdf().enabled = on != 0;
+    return 0;
+}
+int oracle_advance(oracle_handle* h, float dt) {
+    h->b->advance(dt);
+    return 0;
+}
+int oracle_solve_next_step(oracle_handle* h, float dt) {
+    h->b->solve_next_step(dt);
+    return 0;
+}
+int oracle_set_threads(oracle_handle* h, int t) {
+    h->b->set_threads(t);
+    return 0;
+}
+int oracle_num_verts(oracle_handle* h) { return h->b->nverts(); }
+
+// Primitive access for unit tests.
+void oracle_rotor_mul(const float* a, const float* b, float* out) {
+    orc::R4<float> r = orc::rmul(orc::R4<float>{a[0], a[1], a[2], a[3]}, orc::R4<float>{b[0], b[1], b[2], b[3]});
+    out[0] = r.x, out[1] = r.y, out[2] = r.z, out[3] = r.w;
+}
+void oracle_rotor_rotate(const float* q, const float* v, float* out) {
+    orc::V3<float> r = orc::rotate(orc::R4<float>{q[0], q[1], q[2], q[3]}, orc::V3<float>{v[0], v[1], v[2]});
+    out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+void oracle_rotor_from_to(const float* a, const float* b, float* out) {
+    orc::R4<float> r = orc::from_to(orc::V3<float>{a[0], a[1], a[2]}, orc::V3<float>{b[0], b[1], b[2]});
+    out[0] = r.x, out[1] = r.y, out[2] = r.z, out[3] = r.w;
+}
+void oracle_segment_closest_points(const float* a0, const float* a1, const float* b0, const float* b1, float* uv) {
+    orc::segment_closest_points(orc::V3<float>{a0[0], a0[1], a0[2]}, orc::V3<float>{a1[0], a1[1], a1[2]},
+                                orc::V3<float>{b0[0], b0[1], b0[2]}, orc::V3<float>{b1[0], b1[1], b1[2]}, uv[0], uv[1]);
+}
+uint32_t oracle_morton3(int x, int y, int z) { return orc::morton3(x, y, z); }
+// Solves H x = f with H packed (d0,d1,d2,m01,m02,m12), glm inverse semantics.
+void oracle_sym_solve(const float* H, const float* f, float* out) {
+    orc::Sym3<float> S{{H[0], H[1], H[2], H[3], H[4], H[5]}};
+    orc::V3<float> r = orc::sym_solve(S, orc::V3<float>{f[0], f[1], f[2]});
+    out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+}  // extern "C"

@@ -1,0 +1,251 @@
+"""ctypes access to the CPU restatement (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or as the timed CPU baseline.
+The product path never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+_FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+_UP = C.POINTER(C.c_uint32)
+_P = C.c_void_p
+
+SIGNATURES = {
+    "oracle_create": (C.c_int, [C.c_int, _P, C.c_int, _FP, C.c_int, _IP, C.POINTER(_P)]),
+    "oracle_destroy": (None, [_P]),
+    "oracle_configure": (C.c_int, [_P, C.c_float]),
+    "oracle_set_young": (C.c_int, [_P, C.c_float, C.c_float]),
+    "oracle_set_kstretch": (C.c_int, [_P, C.c_float]),
+    "oracle_set_kbend": (C.c_int, [_P, C.c_float]),
+    "oracle_pin_vertices": (C.c_int, [_P, _IP, C.c_int]),
+    "oracle_fix_orientation": (C.c_int, [_P, _IP, C.c_int]),
+    "oracle_set_params": (C.c_int, [_P, _P]),
+    "oracle_upload": (C.c_int, [_P]),
+    "oracle_set_pos_vel": (C.c_int, [_P, _FP, _FP]),
+    "oracle_get_pos_vel": (C.c_int, [_P, _FP, _FP]),
+    "oracle_get_pos_vel64": (C.c_int, [_P, _DP, _DP]),
+    "oracle_get_q": (C.c_int, [_P, _FP]),
+    "oracle_get_q64": (C.c_int, [_P, _DP]),
+    "oracle_set_q": (C.c_int, [_P, _FP]),
+    "oracle_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
+    "oracle_set_sdf": (C.c_int, [_P, _IP, _FP, _FP, _FP, _FP, C.c_float, _FP, _FP]),
+    "oracle_enable_sdf": (C.c_int, [_P, C.c_int]),
+    "oracle_advance": (C.c_int, [_P, C.c_float]),
+    "oracle_solve_next_step": (C.c_int, [_P, C.c_float]),
+    "oracle_set_threads": (C.c_int, [_P, C.c_int]),
+    "oracle_num_verts": (C.c_int, [_P]),
+    "oracle_rotor_mul": (None, [_FP, _FP, _FP]),
+    "oracle_rotor_rotate": (None, [_FP, _FP, _FP]),
+    "oracle_rotor_from_to": (None, [_FP, _FP, _FP]),
+    "oracle_segment_closest_points": (None, [_FP, _FP, _FP, _FP, _FP]),
+    "oracle_morton3": (C.c_uint32, [C.c_int, C.c_int, C.c_int]),
+    "oracle_sym_solve": (None, [_FP, _FP, _FP]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return a.ctypes.data_as(_FP)
+
+
+def _i(a):
+    return a.ctypes.data_as(_IP)
+
+
+class OracleSim:
+    """Same call sequence as kamisimulator_amd._lib.Sim, on the CPU restatement."""
+
+    def __init__(self, params, positions, strand_offsets, precision=32):
+        self.lib = load()
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        off = np.ascontiguousarray(strand_offsets, dtype=np.int32)
+        self.n = pos.shape[0]
+        self.precision = precision
+        h = C.c_void_p()
+        self._params = params  # keep alive
+        rc = self.lib.oracle_create(precision, C.cast(C.byref(params), C.c_void_p), self.n, _f(pos), off.size - 1,
+                                    _i(off), C.byref(h))
+        if rc:
+            raise RuntimeError(f"oracle_create failed with status {rc}")
+        self.h = h
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(f"oracle call failed with status {rc}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, density):
+        self._check(self.lib.oracle_configure(self.h, density))
+
+    def set_young(self, y, dev=1.0):
+        self._check(self.lib.oracle_set_young(self.h, y, dev))
+
+    def set_kstretch(self, k):
+        self._check(self.lib.oracle_set_kstretch(self.h, k))
+
+    def set_kbend(self, k):
+        self._check(self.lib.oracle_set_kbend(self.h, k))
+
+    def pin(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        self._check(self.lib.oracle_pin_vertices(self.h, _i(ids), ids.size))
+
+    def fix_orientation(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        self._check(self.lib.oracle_fix_orientation(self.h, _i(ids), ids.size))
+
+    def set_params(self, params):
+        self._params = params
+        self._check(self.lib.oracle_set_params(self.h, C.cast(C.byref(params), C.c_void_p)))
+
+    def upload(self):
+        self._check(self.lib.oracle_upload(self.h))
+
+    def set_threads(self, t):
+        self.lib.oracle_set_threads(self.h, int(t))
+
+    def set_pos_vel(self, pos=None, vel=None):
+        p = None if pos is None else np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
+        self.lib.oracle_set_pos_vel(self.h, None if p is None else _f(p), None if v is None else _f(v))
+
+    def get_pos_vel(self, f64=False):
+        if f64:
+            p = np.empty((self.n, 3), np.float64)
+            v = np.empty((self.n, 3), np.float64)
+            self.lib.oracle_get_pos_vel64(self.h, p.ctypes.data_as(_DP), v.ctypes.data_as(_DP))
+        else:
+            p = np.empty((self.n, 3), np.float32)
+            v = np.empty((self.n, 3), np.float32)
+            self.lib.oracle_get_pos_vel(self.h, _f(p), _f(v))
+        return p, v
+
+    def get_q(self, f64=False):
+        if f64:
+            q = np.empty((self.n, 4), np.float64)
+            self.lib.oracle_get_q64(self.h, q.ctypes.data_as(_DP))
+        else:
+            q = np.empty((self.n, 4), np.float32)
+            self.lib.oracle_get_q(self.h, _f(q))
+        return q
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float32).reshape(-1)
+        self.lib.oracle_set_q(self.h, _f(q))
+
+    def get_rest(self):
+        im = np.empty(self.n, np.float32)
+        lr = np.empty(self.n, np.float32)
+        ks = np.empty(self.n, np.float32)
+        fl = np.empty(self.n, np.uint32)
+        qr = np.empty((self.n, 4), np.float32)
+        self.lib.oracle_get_rest(self.h, _f(im), _f(lr), _f(ks), fl.ctypes.data_as(_UP), _f(qr))
+        return dict(inv_mass=im, l_rest=lr, k_stretch=ks, flags=fl, q_rest=qr)
+
+    def set_sdf(self, grid, threshold=0.0, rotation=(0, 0, 0, 1), translation=(0, 0, 0)):
+        dim = np.ascontiguousarray(grid.dim, dtype=np.int32)
+        gmin = np.ascontiguousarray(grid.gmin, dtype=np.float32)
+        gmax = np.ascontiguousarray(grid.gmax, dtype=np.float32)
+        phi = np.ascontiguousarray(grid.phi, dtype=np.float32).reshape(-1)
+        grad = np.ascontiguousarray(grid.grad, dtype=np.float32).reshape(-1)
+        rot = np.ascontiguousarray(rotation, dtype=np.float32)
+        tr = np.ascontiguousarray(translation, dtype=np.float32)
+        self._check(self.lib.oracle_set_sdf(self.h, _i(dim), _f(gmin), _f(gmax), _f(phi), _f(grad), threshold,
+                                            _f(rot), _f(tr)))
+
+    def enable_sdf(self, on=True):
+        self.lib.oracle_enable_sdf(self.h, int(bool(on)))
+
+    def advance(self, dt):
+        self.lib.oracle_advance(self.h, dt)
+
+    def solve_next_step(self, dt):
+        self.lib.oracle_solve_next_step(self.h, dt)
+
+
+# ------------------------------------------------------------------ primitives
+def rotor_mul(a, b):
+    lib = load()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.empty(4, np.float32)
+    lib.oracle_rotor_mul(_f(a), _f(b), _f(out))
+    return out
+
+
+def rotor_rotate(q, v):
+    lib = load()
+    q = np.ascontiguousarray(q, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.empty(3, np.float32)
+    lib.oracle_rotor_rotate(_f(q), _f(v), _f(out))
+    return out
+
+
+def rotor_from_to(a, b):
+    lib = load()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.empty(4, np.float32)
+    lib.oracle_rotor_from_to(_f(a), _f(b), _f(out))
+    return out
+
+
+def segment_closest_points(a0, a1, b0, b1):
+    lib = load()
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (a0, a1, b0, b1)]
+    out = np.empty(2, np.float32)
+    lib.oracle_segment_closest_points(*[_f(x) for x in arrs], _f(out))
+    return out
+
+
+def morton3(x, y, z):
+    return load().oracle_morton3(int(x), int(y), int(z))
+
+
+def sym_solve(H6, f):
+    lib = load()
+    H6 = np.ascontiguousarray(H6, np.float32)
+    f = np.ascontiguousarray(f, np.float32)
+    out = np.empty(3, np.float32)
+    lib.oracle_sym_solve(_f(H6), _f(f), _f(out))
+    return out

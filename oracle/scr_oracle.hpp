@@ -1,0 +1,668 @@
+// scr_oracle.hpp -- CPU restatement of KamiSimulator's Stable Cosserat Rods step.
+//
+// TEST INFRASTRUCTURE ONLY.  This is the oracle that the HIP product path is
+// checked against.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker (or as the CPU
+// baseline that is timed beside the GPU); the product never calls it.
+//
+// Semantics: "Jacobi snapshot".  The reference kernels are racy (blocks
+// overlap by a halo vertex, q is updated in place), so this restatement fixes
+// a deterministic order: every read of iteration k sees iteration k-1 values,
+// except that the orientation half-step sees the new dx of the same iteration
+// (it is a separate launch after the position kernel, YBSimulator.cu:415-418).
+//
+// T = float is the FP32 restatement (also the CPU baseline), T = double is the
+// FP64 oracle.  The float/double mixing of the host setup code is kept
+// expression for expression (cited per function); the device formulas are
+// evaluated in T.  Build with -ffp-contract=off so every operation rounds once.
+//
+// Pinning of each part (tests/test_oracle_*.py):
+//   rotor algebra, segment closest points, Morton codes, 3x3 inverse:
+//       vectors from the reference's own header-only math (oracle/_ref probe)
+//       committed under tests/golden/, plus closed forms;
+//   initItr/endItr: closed form of a lone free vertex (symplectic Euler with
+//       decay) and the adaptive-initial-guess clamp cases;
+//   position solve: rest-state invariance, rigid-motion equivariance, the
+//       two-vertex scalar closed form;
+//   orientation solve: the rest fixed point, unit norm, q -> -q symmetry;
+//   setup: mass/stiffness closed forms and Rotor::fromTo vectors;
+//   SDF: affine-field exactness of the half-cell convention, sphere projection;
+//   float vs double: the FP64 error is reported per config.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------- vectors
+template <class T>
+struct V3 {
+    T x, y, z;
+};
+template <class T>
+inline V3<T> v3(T x, T y, T z) {
+    return V3<T>{x, y, z};
+}
+template <class T>
+inline V3<T> operator+(V3<T> a, V3<T> b) {
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <class T>
+inline V3<T> operator-(V3<T> a, V3<T> b) {
+    return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <class T>
+inline V3<T> operator-(V3<T> a) {
+    return {-a.x, -a.y, -a.z};
+}
+template <class T>
+inline V3<T> operator*(T s, V3<T> a) {
+    return {s * a.x, s * a.y, s * a.z};
+}
+template <class T>
+inline V3<T> operator*(V3<T> a, T s) {
+    return {a.x * s, a.y * s, a.z * s};
+}
+// glm divides componentwise by the scalar (no reciprocal)
+template <class T>
+inline V3<T> operator/(V3<T> a, T s) {
+    return {a.x / s, a.y / s, a.z / s};
+}
+// glm dot: (x*x' + y*y') + z*z'   (func_geometric.inl)
+template <class T>
+inline T dot(V3<T> a, V3<T> b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <class T>
+inline V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <class T>
+inline T length2(V3<T> a) {
+    return dot(a, a);
+}
+template <class T>
+inline T length(V3<T> a) {
+    return std::sqrt(dot(a, a));
+}
+// glm normalize = v * inversesqrt(dot(v,v)), inversesqrt = 1/sqrt
+template <class T>
+inline V3<T> normalize(V3<T> a) {
+    return a * (T(1) / std::sqrt(dot(a, a)));
+}
+// glm mix(x, y, a) = x + a*(y - x)   (func_common.inl:149)
+template <class T>
+inline V3<T> mix(V3<T> a, V3<T> b, T t) {
+    return a + t * (b - a);
+}
+
+// Rotor (quaternion) laid out (x, y, z, w), KE/includes/modules/Rotor.h:39-48
+template <class T>
+struct R4 {
+    T x, y, z, w;
+    V3<T> q() const { return {x, y, z}; }
+};
+template <class T>
+inline R4<T> r4(V3<T> v, T w) {
+    return R4<T>{v.x, v.y, v.z, w};
+}
+template <class T>
+inline T dot4(R4<T> a, R4<T> b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+}
+template <class T>
+inline T length4(R4<T> a) {
+    return std::sqrt(dot4(a, a));
+}
+template <class T>
+inline R4<T> normalize4(R4<T> a) {
+    T s = T(1) / std::sqrt(dot4(a, a));
+    return {a.x * s, a.y * s, a.z * s, a.w * s};
+}
+template <class T>
+inline R4<T> add4(R4<T> a, R4<T> b) {
+    return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
+}
+template <class T>
+inline R4<T> scale4(T s, R4<T> a) {
+    return {s * a.x, s * a.y, s * a.z, s * a.w};
+}
+// Rotor::inverse = conjugate (Rotor.h:109)
+template <class T>
+inline R4<T> conj(R4<T> a) {
+    return {-a.x, -a.y, -a.z, a.w};
+}
+// Hamilton product (Rotor.h:210-212)
+template <class T>
+inline R4<T> rmul(R4<T> a, R4<T> b) {
+    V3<T> aq = a.q(), bq = b.q();
+    V3<T> v = a.w * bq + b.w * aq + cross(aq, bq);
+    return r4(v, a.w * b.w - dot(aq, bq));
+}
+// Rotor::rotate (Rotor.h:114-123)
+template <class T>
+inline V3<T> rotate(R4<T> r, V3<T> v) {
+    V3<T> q = r.q();
+    V3<T> a = r.w * v + cross(q, v);
+    T c = dot(v, q);
+    return r.w * a + cross(q, a) + c * q;
+}
+// orthoBasisX(n)[1] (KE/includes/modules/Common.h:220-236), used by fromTo for
+// antiparallel vectors.
+template <class T>
+inline V3<T> ortho_basis_x1(V3<T> n) {
+    if (n.z >= n.y) {
+        const T a = T(1) / (T(1) + n.z);
+        const T b = -n.x * n.y * a;
+        return {b, T(1) - n.y * n.y * a, -n.y};
+    } else {
+        const T a = T(1) / (T(1) + n.y);
+        return {T(1) - n.x * n.x * a, -n.x, -n.x * n.z * a};
+    }
+}
+// Rotor::fromTo (Rotor.h:79-88)
+template <class T>
+inline R4<T> from_to(V3<T> from, V3<T> to) {
+    V3<T> h = (from + to) * T(0.5f);
+    T l = length2(h);
+    if (l > 0)
+        h = h * (T(1) / std::sqrt(l));
+    else
+        h = ortho_basis_x1(from);
+    return r4(cross(from, h), dot(from, h));
+}
+
+// Packed symmetric 3x3 (KE/includes/modules/SymMat.h:6-87):
+// d[0..2] = diagonal, d[3] = m01, d[4] = m02, d[5] = m12.
+template <class T>
+struct Sym3 {
+    T d[6];
+};
+// glm inverse(mat3) (glm/detail/func_matrix.inl:272-294) applied to the
+// symmetric matrix, followed by glm mat3*vec3 (type_mat3x3.inl:418-424).
+template <class T>
+inline V3<T> sym_solve(const Sym3<T>& H, V3<T> f) {
+    // m[c][r]; symmetric so m[c][r] = m[r][c]
+    const T m00 = H.d[0], m11 = H.d[1], m22 = H.d[2];
+    const T m01 = H.d[3], m02 = H.d[4], m12 = H.d[5];
+    const T m10 = m01, m20 = m02, m21 = m12;
+    T ood = T(1) / (+m00 * (m11 * m22 - m21 * m12) - m10 * (m01 * m22 - m21 * m02) + m20 * (m01 * m12 - m11 * m02));
+    T i00 = +(m11 * m22 - m21 * m12) * ood;
+    T i10 = -(m10 * m22 - m20 * m12) * ood;
+    T i20 = +(m10 * m21 - m20 * m11) * ood;
+    T i01 = -(m01 * m22 - m21 * m02) * ood;
+    T i11 = +(m00 * m22 - m20 * m02) * ood;
+    T i21 = -(m00 * m21 - m20 * m01) * ood;
+    T i02 = +(m01 * m12 - m11 * m02) * ood;
+    T i12 = -(m00 * m12 - m10 * m02) * ood;
+    T i22 = +(m00 * m11 - m10 * m01) * ood;
+    return {i00 * f.x + i10 * f.y + i20 * f.z, i01 * f.x + i11 * f.y + i21 * f.z, i02 * f.x + i12 * f.y + i22 * f.z};
+}
+
+// Kit::segmentClosestPoints (KE/includes/modules/Common.h:294-312)
+template <class T>
+inline void segment_closest_points(V3<T> a0, V3<T> a1, V3<T> b0, V3<T> b1, T& u, T& v) {
+    V3<T> A0 = a1 - a0, A1 = b1 - b0;
+    T a = length2(A0);
+    T b = dot(A0, A1);
+    T c = length2(A1);
+    V3<T> diff = b0 - a0;
+    T uy = b * dot(A0, diff) - a * dot(A1, diff);
+    uy /= a * c - b * b;
+    if (!std::isfinite(uy)) uy = T(0.5f);
+    uy = std::min(std::max(uy, T(0)), T(1));
+    T ux = std::min(std::max(dot(mix(b0, b1, uy) - a0, A0) / a, T(0)), T(1));
+    uy = std::min(std::max(dot(mix(a0, a1, ux) - b0, A1) / c, T(0)), T(1));
+    u = ux;
+    v = uy;
+}
+
+// Kit::mortonExpandBits / getMorton (KE/includes/modules/Common.h:566-579)
+inline uint32_t morton_expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+inline uint32_t morton3(int x, int y, int z) {
+    return morton_expand_bits((uint32_t)x) * 4 + morton_expand_bits((uint32_t)y) * 2 + morton_expand_bits((uint32_t)z);
+}
+
+// ---------------------------------------------------------------- params
+// Same field order as kscr_params (include/kscr.h); MetaData in the reference
+// (YB/YarnBall.h:61-124).
+struct Params {
+    float gravity[3];
+    int32_t num_itr;
+    float max_h;
+    float damping;
+    float velocity_decay;
+    float radius;
+    float barrier_thickness;
+    float acceleration_ratio;
+    float k_collision;
+    float k_friction;
+    float friction_coeff;
+    float detection_scaler;
+    float bvh_rebuild_period;
+    int32_t detection_period;
+    int32_t use_step_size_limit;
+    int32_t use_velocity_radius;
+    int32_t enable_hair_hair_coll;
+    float external_force_multiplier;
+    float wind_peak;
+    float wind_y_freq;
+    float wind_z_freq;
+    float wind_time_freq;
+    float wind_sharpness;
+    int32_t schedule;
+    int32_t block_size;
+};
+
+enum : uint32_t { F_PREV = 1, F_NEXT = 2, F_NEXT_ORI = 4, F_FIX_ORI = 8 };
+
+static const double kPi = 3.141592653589793238462643;  // YBSimulator.cu:55
+
+struct Sdf {
+    bool enabled = false;
+    int dim[3] = {0, 0, 0};
+    float gmin[3], gmax[3];
+    float ds[3], inv_ds[3];
+    float thresh = 0;
+    float rot[4] = {0, 0, 0, 1};
+    float trans[3] = {0, 0, 0};
+    std::vector<float> phi;   // dim^3
+    std::vector<float> grad;  // 3*dim^3
+};
+
+// ---------------------------------------------------------------- simulator
+template <class T>
+struct Sim {
+    int n = 0;
+    Params p;
+    // Host-side state as in YarnBall::Sim (YBSimulator.h:18-47).  Setup quantities
+    // are stored in float exactly as the reference computes them; the dynamic
+    // state is in T.
+    std::vector<float> pos0;  // positions used by configure (float)
+    std::vector<float> invMassF, lRestF, kStretchF;
+    std::vector<uint32_t> flags;
+    std::vector<R4<float>> qF, qRestF;
+    bool configured = false;
+    float density = 0, minSegLen = 0, maxSegLen = 0;
+
+    // dynamic state in T
+    std::vector<V3<T>> x, v, u;  // pos, vels, lastVels
+    std::vector<R4<T>> q;
+    std::vector<R4<T>> qRest;
+    std::vector<T> invMass, lRest, kStretch;
+    // per-substep scratch
+    std::vector<V3<T>> y, dx, dxNew;
+    std::vector<R4<T>> qNew;
+
+    T h = T(4e-4f), lastH = T(4e-4f), time = 0;
+    float hF = 4e-4f, lastHF = 4e-4f, timeF = 0;
+    long step_counter = 0;
+    Sdf sdf;
+    int threads = 1;
+
+    // Sim::Sim + createFromCurves (YBSimulator.cu:81-127; YB/io/reader.cpp:16-50)
+    int create(const Params& prm, int nv, const float* P, int ns, const int* off) {
+        if (nv < 3) return 2;
+        n = nv;
+        p = prm;
+        pos0.assign(P, P + 3 * (size_t)nv);
+        invMassF.assign(nv, 1.f);
+        lRestF.assign(nv, 1.f);
+        kStretchF.assign(nv, 100.f);
+        flags.assign(nv, F_NEXT);
+        qF.assign(nv, R4<float>{0, 0, 0, 1});
+        qRestF.assign(nv, R4<float>{0, 0, 0, 1});
+        flags[nv - 1] = 0;
+        for (int s = 0; s < ns; s++) flags[off[s + 1] - 1] = 0;  // cut off the end of each curve
+        x.resize(nv);
+        v.assign(nv, V3<T>{0, 0, 0});
+        u.assign(nv, V3<T>{0, 0, 0});
+        for (int i = 0; i < nv; i++) x[i] = V3<T>{(T)P[3 * i], (T)P[3 * i + 1], (T)P[3 * i + 2]};
+        h = T(prm.max_h);
+        hF = prm.max_h;
+        return 0;
+    }
+
+    // Sim::configure (YBSimulator.cu:161-241), float arithmetic with the double
+    // sub-expression of the lumped mass (:229).
+    int configure(float dens) {
+        density = dens;
+        maxSegLen = 0;
+        minSegLen = std::numeric_limits<float>::max();
+        R4<float> lastQ{0, 0, 0, 1};
+        for (int i = 0; i < n; i++) {
+            if (i < n - 1) {
+                bool hasNext = (flags[i] & F_NEXT) != 0;
+                flags[i + 1] = hasNext ? (flags[i + 1] | F_PREV) : (flags[i + 1] & ~F_PREV);
+                bool hasNextNext = hasNext && (flags[i + 1] & F_NEXT);
+                flags[i] = hasNextNext ? (flags[i] | F_NEXT_ORI) : (flags[i] & ~F_NEXT_ORI);
+                if (!hasNext) flags[i] |= F_FIX_ORI;
+            }
+            if (!(flags[i] & F_PREV) && !(flags[i] & F_NEXT)) return 3;
+            lRestF[i] = 1.f / n;
+            float mass = 0;
+            if (flags[i] & F_PREV) mass += lRestF[i - 1];
+            if (flags[i] & F_NEXT) {
+                V3<float> a{pos0[3 * i], pos0[3 * i + 1], pos0[3 * i + 2]};
+                V3<float> b{pos0[3 * i + 3], pos0[3 * i + 4], pos0[3 * i + 5]};
+                V3<float> seg = b - a;
+                lRestF[i] = length(seg);
+                if (lRestF[i] == 0 || !std::isfinite(lRestF[i])) return 4;
+                lastQ = from_to(V3<float>{1, 0, 0}, normalize(seg));
+                mass += lRestF[i];
+                maxSegLen = std::max(maxSegLen, lRestF[i]);
+                minSegLen = std::min(minSegLen, lRestF[i]);
+            }
+            qF[i] = lastQ;
+            mass = (float)((double)mass * ((double)(p.radius * p.radius) * kPi * (double)dens * (double)0.5f));
+            if (mass != 0)
+                invMassF[i] *= 1 / mass;
+            else
+                invMassF[i] = 0;
+        }
+        for (int i = 0; i < n - 1; i++) qRestF[i] = normalize4(rmul(conj(qF[i]), qF[i + 1]));
+        configured = true;
+        return 0;
+    }
+
+    // Sim::setYoung (YBSimulator.cu:300-320)
+    int set_young(float young, float deviation) {
+        if (!configured) return 7;
+        float mmrad = p.radius * 1000;
+        for (int i = 0; i < n; i++) {
+            float len = lRestF[i];
+            float ss = (float)((double)young * kPi * (double)mmrad * (double)mmrad);
+            float bt = (float)((double)(ss * deviation * mmrad * mmrad) * 0.25 / (double)(len * len));
+            kStretchF[i] = ss;
+            qRestF[i] = R4<float>{qRestF[i].x * bt, qRestF[i].y * bt, qRestF[i].z * bt, qRestF[i].w * bt};
+        }
+        return 0;
+    }
+    // Sim::setKStretch (YBSimulator.cu:276-284)
+    int set_kstretch(float k) {
+        if (!configured) return 7;
+        for (int i = 0; i < n; i++) kStretchF[i] = k * lRestF[i];
+        return 0;
+    }
+    // Sim::setKBend (YBSimulator.cu:286-298)
+    int set_kbend(float k) {
+        if (!configured) return 7;
+        k *= 4;
+        for (int i = 0; i < n; i++) {
+            float s = k / lRestF[i];
+            R4<float> nq = normalize4(qRestF[i]);
+            qRestF[i] = R4<float>{s * nq.x, s * nq.y, s * nq.z, s * nq.w};
+        }
+        return 0;
+    }
+    void pin(const int* ids, int m) {
+        for (int k = 0; k < m; k++) invMassF[ids[k]] = 0;
+    }
+    void fix_orientation(const int* ids, int m) {
+        for (int k = 0; k < m; k++) flags[ids[k]] |= F_FIX_ORI;
+    }
+
+    // uploadMeta validation (YBSimulator.cu:322-335)
+    int validate() const {
+        if (p.detection_scaler <= 1) return 5;
+        if (2 * minSegLen < 2 * p.radius + p.barrier_thickness) return 6;
+        return 0;
+    }
+
+    // Sim::upload: host mirror -> state in T
+    void upload() {
+        q.resize(n);
+        qRest.resize(n);
+        invMass.resize(n);
+        lRest.resize(n);
+        kStretch.resize(n);
+        for (int i = 0; i < n; i++) {
+            q[i] = R4<T>{(T)qF[i].x, (T)qF[i].y, (T)qF[i].z, (T)qF[i].w};
+            qRest[i] = R4<T>{(T)qRestF[i].x, (T)qRestF[i].y, (T)qRestF[i].z, (T)qRestF[i].w};
+            invMass[i] = (T)invMassF[i];
+            lRest[i] = (T)lRestF[i];
+            kStretch[i] = (T)kStretchF[i];
+        }
+        y.assign(n, V3<T>{0, 0, 0});
+        dx.assign(n, V3<T>{0, 0, 0});
+        dxNew.assign(n, V3<T>{0, 0, 0});
+        qNew.assign(n, R4<T>{0, 0, 0, 1});
+    }
+
+    // ------------------------------------------------------------ kernels
+    // initItr (YB/sim/iteration.cu:8-59)
+    void init_itr() {
+        const T hh = h;
+        const V3<T> g{(T)p.gravity[0], (T)p.gravity[1], (T)p.gravity[2]};
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; i++) {
+            const V3<T> vel = v[i];
+            T windx = 0;
+            if (p.wind_peak != 0) {
+                // cos in double of a float argument (M_PI is double), pow in T
+                T yp = x[i].y, zp = x[i].z;
+                T wy = yp > 0 ? (T)std::cos(2 * M_PI * (double)((T)p.wind_y_freq * yp + (T)p.wind_time_freq * time)) : T(0);
+                T wz = (T)std::cos(2 * M_PI * (double)((T)p.wind_z_freq * std::fabs(zp) + (T)p.wind_time_freq * time));
+                T yz = wy * wz;
+                T sg = (T)((T(0) < yz) - (yz < T(0)));
+                T pw = std::pow(yz, (T)p.wind_sharpness) * sg;
+                T mx = std::isnan(pw) ? T(0) : std::max(T(0), pw);  // fmax(0, NaN) = 0
+                windx = (T)p.wind_peak * mx;
+            }
+            V3<T> ext = (g + V3<T>{windx, 0, 0}) * (T)p.external_force_multiplier;
+            V3<T> d = hh * vel;
+            V3<T> lastVel = u[i];
+            u[i] = vel;
+            if (invMass[i] != 0) {
+                y[i] = d + (hh * hh) * ext;
+                T ext2 = length2(ext);
+                if (ext2 > 0) {
+                    V3<T> a = (vel - lastVel) / lastH;
+                    T s = std::min(std::max(dot(a, ext) / ext2, T(0)), T(1));
+                    d = d + (hh * hh * s) * ext;
+                }
+            } else {
+                y[i] = vel;  // the reference leaves vels untouched for pinned vertices
+            }
+            dx[i] = d;
+        }
+    }
+
+    // Position solve of vertex i (YB/sim/cosserat.cu:112-287), Jacobi: reads
+    // dx (iteration k), q (iteration k).  Contacts are not part of this oracle
+    // path (hair-hair collision is off in every shipped configuration).
+    // Returns the contribution of segment i to vertex i+1 ("f2"/"H2" of the
+    // reference) through the out parameters when asked.
+    void segment_terms(int i, T damp, V3<T>& fi, T& di, V3<T>& fj) const {
+        const T invl = T(1) / lRest[i];
+        V3<T> c = ((x[i + 1] - x[i]) + (dx[i + 1] - dx[i])) * invl - rotate(q[i], V3<T>{1, 0, 0});
+        T k = kStretch[i] * invl;
+        T d = k * invl;
+        fi = k * c - (damp * d) * dx[i];
+        fj = (-k) * c - (damp * d) * dx[i + 1];
+        di = d * (1 + damp);
+    }
+
+    V3<T> position_solve(int i) const {
+        if (invMass[i] == 0) return dx[i];
+        const T hh = h;
+        const T damp = T(p.damping) / hh;
+        const V3<T> dxi = dx[i];
+        const T a_in = T(1) / (invMass[i] * hh * hh);
+        Sym3<T> H{{a_in, a_in, a_in, a_in, a_in, a_in}};
+        V3<T> f = (T(1) / (hh * hh * invMass[i])) * (y[i] - dxi);
+        if (flags[i] & F_NEXT) {
+            V3<T> fi, fj;
+            T di;
+            segment_terms(i, damp, fi, di, fj);
+            f = f + fi;
+            H.d[0] += di;
+            H.d[1] += di;
+            H.d[2] += di;
+        }
+        if (flags[i] & F_PREV) {
+            V3<T> fi, fj;
+            T dj;
+            segment_terms(i - 1, damp, fi, dj, fj);
+            f = f + fj;
+            H.d[0] += dj;
+            H.d[1] += dj;
+            H.d[2] += dj;
+        }
+        V3<T> delta = T(p.acceleration_ratio) * sym_solve(H, f);
+        return dxi + delta;  // step limit is +inf without hair-hair contacts
+    }
+
+    // Orientation solve of vertex i (YB/sim/cosserat.cu:290-337), Jacobi: reads
+    // the new dx and the previous-iteration q of the neighbours.
+    R4<T> orientation_solve(int i) const {
+        const uint32_t fl = flags[i];
+        if ((fl & F_FIX_ORI) || !(fl & F_NEXT)) return q[i];
+        V3<T> vv = ((x[i + 1] - x[i]) + (dxNew[i + 1] - dxNew[i])) / lRest[i];
+        vv = vv * (T(-2) * kStretch[i]);
+        R4<T> b{0, 0, 0, 0};
+        const R4<T> qi = q[i];
+        if (fl & F_PREV) {
+            const R4<T> qr = qRest[i - 1];
+            const R4<T> qj = q[i - 1];
+            T phi = dot4(rmul(conj(qj), qi), qr) > 0 ? T(1) : T(-1);
+            b = add4(b, scale4(phi, rmul(qj, qr)));
+        }
+        if (fl & F_NEXT_ORI) {
+            const R4<T> qr = qRest[i];
+            const R4<T> qj = q[i + 1];
+            T phi = dot4(rmul(conj(qi), qj), qr) > 0 ? T(1) : T(-1);
+            b = add4(b, scale4(phi, rmul(qj, conj(qr))));
+        }
+        T lambda = length(vv) + length4(b);
+        R4<T> t = rmul(rmul(r4(vv, T(0)), b), R4<T>{1, 0, 0, 0});
+        return normalize4(add4(t, scale4(lambda, b)));
+    }
+
+    // endItr (YB/sim/iteration.cu:62-74)
+    void end_itr() {
+        const T invH = T(1) / h;
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; i++) {
+            if (invMass[i] != 0) v[i] = dx[i] * invH * (T(1) - T(p.velocity_decay) * h);
+            x[i] = x[i] + dx[i];
+        }
+    }
+
+    // One VBD iteration: cosseratItr then quaternionLambdaItr
+    // (YBSimulator.cu:415-418) with snapshot reads.
+    void iterate() {
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; i++) dxNew[i] = position_solve(i);
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; i++) qNew[i] = orientation_solve(i);
+        std::swap(dx, dxNew);
+        std::swap(q, qNew);
+    }
+
+    // Sim::advance (YB/sim/step.cpp:38-62)
+    void advance(float dtF) {
+        if (dtF <= 0) return;
+        int steps = std::max(1, (int)std::ceil(dtF / p.max_h));
+        lastHF = hF;
+        hF = dtF / steps;
+        // The FP64 oracle uses the same (float) substep lengths, so that the
+        // float-vs-double difference measures arithmetic rounding only.
+        lastH = (T)lastHF;
+        h = (T)hF;
+        for (int s = 0; s < steps; s++, step_counter++) {
+            init_itr();
+            for (int k = 0; k < p.num_itr; k++) iterate();
+            end_itr();
+        }
+        timeF += dtF;
+        time = (T)timeF;
+    }
+
+    // ------------------------------------------------------------ SDF pass
+    // SDFHeadCollision (SDF/cudaSDFCollision.cu:88-155, :205-268).  The index
+    // and the unit-cube weights are evaluated with the same float/double
+    // mixing as the reference kernel.
+    bool sdf_inside(int ix, int iy, int iz) const {
+        return ix >= 0 && ix < sdf.dim[0] && iy >= 0 && iy < sdf.dim[1] && iz >= 0 && iz < sdf.dim[2];
+    }
+    size_t sdf_idx(int ix, int iy, int iz) const {
+        return (size_t)ix + (size_t)iy * sdf.dim[0] + (size_t)iz * sdf.dim[0] * sdf.dim[1];
+    }
+    void sdf_corner(int ix, int iy, int iz, T c[4]) const {
+        if (sdf_inside(ix, iy, iz)) {
+            size_t k = sdf_idx(ix, iy, iz);
+            c[0] = sdf.grad[3 * k];
+            c[1] = sdf.grad[3 * k + 1];
+            c[2] = sdf.grad[3 * k + 2];
+        } else {
+            c[0] = c[1] = c[2] = 0;
+        }
+        int cx = std::max(0, std::min(ix, sdf.dim[0] - 1));
+        int cy = std::max(0, std::min(iy, sdf.dim[1] - 1));
+        int cz = std::max(0, std::min(iz, sdf.dim[2] - 1));
+        c[3] = sdf.phi[sdf_idx(cx, cy, cz)];
+    }
+    void apply_sdf() {
+        if (!sdf.enabled) return;
+        const R4<T> invq{-(T)sdf.rot[0], -(T)sdf.rot[1], -(T)sdf.rot[2], (T)sdf.rot[3]};
+        const V3<T> tr{(T)sdf.trans[0], (T)sdf.trans[1], (T)sdf.trans[2]};
+        const V3<T> mn{(T)sdf.gmin[0], (T)sdf.gmin[1], (T)sdf.gmin[2]};
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int i = 0; i < n; i++) {
+            if (!(flags[i] & F_PREV)) continue;  // LocalIdx == 0: strand root
+            V3<T> lp = rotate(invq, x[i] - tr) - mn;
+            T l[3] = {lp.x, lp.y, lp.z};
+            int idx[3];
+            T ud[3];
+            for (int a = 0; a < 3; a++) idx[a] = (int)((double)(l[a] * (T)sdf.inv_ds[a]) + 0.5);
+            if (!sdf_inside(idx[0], idx[1], idx[2])) continue;
+            for (int a = 0; a < 3; a++)
+                ud[a] = (T)((double)sdf.inv_ds[a] * ((double)l[a] - ((double)idx[a] - 0.5) * (double)sdf.ds[a]));
+            T c[8][4];
+            // corner order c000 c001 c010 c011 c100 c101 c110 c111 (x y z)
+            for (int o = 0; o < 8; o++) sdf_corner(idx[0] + ((o >> 2) & 1), idx[1] + ((o >> 1) & 1), idx[2] + (o & 1), c[o]);
+            T res[4];
+            for (int k = 0; k < 4; k++) {
+                T c00 = c[0][k] * (1.f - ud[0]) + c[4][k] * ud[0];
+                T c01 = c[1][k] * (1.f - ud[0]) + c[5][k] * ud[0];
+                T c10 = c[2][k] * (1.f - ud[0]) + c[6][k] * ud[0];
+                T c11 = c[3][k] * (1.f - ud[0]) + c[7][k] * ud[0];
+                T c0 = c00 * (1.f - ud[1]) + c10 * ud[1];
+                T c1 = c01 * (1.f - ud[1]) + c11 * ud[1];
+                res[k] = c0 * (1.f - ud[2]) + c1 * ud[2];
+            }
+            V3<T> nrm{res[0], res[1], res[2]};
+            // helper_math normalize: v * rsqrtf(dot(v, v))
+            nrm = nrm * (T(1) / std::sqrt(dot(nrm, nrm)));
+            T phi = res[3] - (T)sdf.thresh;
+            if (phi < 0) {
+                x[i] = x[i] - phi * nrm;
+                T vl = length(v[i]);
+                if ((double)vl >= 1e-6) {  // double literal in the reference
+                    V3<T> vn = v[i] / vl;
+                    v[i] = v[i] - dot(vn, nrm) * nrm;
+                }
+            }
+        }
+    }
+
+    // StableCosseratRodsSolver::SolveNextStep (StableCosseratRodsSolver.cpp:216-234)
+    void solve_next_step(float dt) {
+        advance(dt);
+        apply_sdf();
+    }
+};
+
+}  // namespace orc

@@ -1,0 +1,151 @@
+"""GPU parity: the HIP step (libkscr.so through the C ABI) against the FP32
+CPU restatement (oracle) on identical seeded inputs.
+
+Both sides evaluate the same FP32 operations in the same order with IEEE
+division/square root and no FMA contraction, so the expected agreement is
+bit-exact; the asserted bar is the 1e-5 relative tolerance
+max|x - x_ref| / max|x_ref| for positions and orientations.  The FP64 error of
+the FP32 result is printed per workload (it is rounding-noise dominated: the
+stretch stiffness is ~1e5 x the inertia)."""
+import numpy as np
+import pytest
+
+from kamisimulator_amd import scenes
+from kamisimulator_amd._lib import Sim, default_params
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+DT = F32(1 / 600)
+TOL = 1e-5
+
+
+def _pair(sc, block=0, fp64=False):
+    p = scenes.make_params(default_params, sc.material, block_size=block)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets, precision=32), sc)
+    o.set_threads(8)
+    o64 = None
+    if fp64:
+        o64 = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets, precision=64), sc)
+        o64.set_threads(8)
+    return g, o, o64
+
+
+def _rel(a, b):
+    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _run(sc, steps, block=0, fp64=False, label=""):
+    print(f"[gpu-parity] {label} verts={sc.num_verts} strands={sc.num_strands} steps={steps} block={block}", flush=True)
+    g, o, o64 = _pair(sc, block, fp64)
+    for s in range(steps):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+        if o64 is not None:
+            o64.solve_next_step(DT)
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    gq, oq = g.get_q(), o.get_q()
+    ep, eq, ev = _rel(gp, op), _rel(gq, oq), _rel(gv, ov)
+    nbit = int(np.count_nonzero(gp.view(np.uint32) != op.view(np.uint32)))
+    msg = f"[gpu-parity] {label}: pos rel {ep:.3e} q rel {eq:.3e} vel rel {ev:.3e} differing pos words {nbit}"
+    if o64 is not None:
+        p64, _ = o64.get_pos_vel(f64=True)
+        msg += f" | FP32 vs FP64 pos rel {_rel(gp, p64):.3e}"
+    print(msg, flush=True)
+    return ep, eq, ev, g, o
+
+
+def test_setup_matches_oracle():
+    sc = scenes.config_groom(300, 100, with_sdf=False)
+    g, o, _ = _pair(sc)
+    rg, ro = g.get_rest(), o.get_rest()
+    for k in rg:
+        assert np.array_equal(rg[k].view(np.uint32), ro[k].view(np.uint32)), k
+    assert np.array_equal(g.get_q().view(np.uint32), o.get_q().view(np.uint32))
+
+
+def test_cantilever_100_steps():
+    ep, eq, ev, _, _ = _run(scenes.config_cantilever(), 100, fp64=True, label="cantilever")
+    assert ep <= TOL and eq <= TOL
+
+
+def test_cantilever_root_pin_only():
+    sc = scenes.cantilever(64, 1e-3, clamp=False)
+    ep, eq, _, _, _ = _run(sc, 100, label="cantilever-pin-only")
+    assert ep <= TOL and eq <= TOL
+
+
+def test_groom_with_sdf_100_steps():
+    sc = scenes.config_groom(1000, 100, sdf_dim=128)
+    ep, eq, _, _, _ = _run(sc, 100, fp64=True, label="groom1k-sdf128")
+    assert ep <= TOL and eq <= TOL
+
+
+def test_lognormal_lengths_with_sdf():
+    counts = scenes.lognormal_counts(600, seed=5)
+    sc = scenes.groom(600, counts, seed=5)
+    sc.sdf = scenes.config_large(1, with_sdf=True, sdf_dim=96).sdf
+    ep, eq, _, _, _ = _run(sc, 30, label="lognormal600-sdf96")
+    assert ep <= TOL and eq <= TOL
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_block_sizes_agree(block):
+    sc = scenes.config_groom(400, 100, sdf_dim=64)
+    ep, eq, _, _, _ = _run(sc, 10, block=block, label=f"block{block}")
+    assert ep <= TOL and eq <= TOL
+
+
+def test_run_to_run_deterministic():
+    sc = scenes.config_groom(500, 100, sdf_dim=64)
+    p = scenes.make_params(default_params, sc.material)
+    outs = []
+    for _ in range(2):
+        g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+        for s in range(20):
+            g.solve_next_step(DT)
+        outs.append(np.concatenate([g.get_pos_vel()[0].ravel(), g.get_q().ravel()]))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_advance_without_sdf_matches():
+    sc = scenes.config_groom(200, 100, sdf_dim=64)
+    p = scenes.make_params(default_params, sc.material)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets), sc)
+    for s in range(10):
+        g.advance(DT)
+        o.advance(DT)
+    assert _rel(g.get_pos_vel()[0], o.get_pos_vel()[0]) <= TOL
+
+
+def test_headline_size_properties():
+    """100k strands x 100 vertices (the headline workload): size-independent
+    properties after 5 steps, since the oracle is too slow at this size."""
+    sc = scenes.config_groom(100_000, 100, with_sdf=False)
+    p = scenes.make_params(default_params, sc.material)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    for s in range(5):
+        g.solve_next_step(DT)
+    x, v = g.get_pos_vel()
+    q = g.get_q()
+    assert np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(q).all()
+    roots = sc.roots
+    assert np.array_equal(x[roots].view(np.uint32), sc.positions[roots].view(np.uint32))  # pinned roots never move
+    np.testing.assert_allclose(np.linalg.norm(q.astype(np.float64), axis=1), 1.0, atol=1e-5)
+    # segment lengths stay within 1% of rest (stretching is stiff)
+    rest = g.get_rest()["l_rest"]
+    tips = sc.strand_offsets[1:] - 1
+    mask = np.ones(len(x), bool)
+    mask[tips] = False
+    seg = np.linalg.norm(x[1:] - x[:-1], axis=1)[mask[:-1]]
+    assert np.abs(seg / rest[:-1][mask[:-1]] - 1).max() < 1e-2
+    # a strand sample matches the oracle run on that strand alone
+    s0, s1 = int(sc.strand_offsets[12345]), int(sc.strand_offsets[12346])
+    sub = scenes.Scene("one", sc.positions[s0:s1].copy(), np.array([0, s1 - s0], np.int32))
+    o = scenes.setup_sim(OracleSim(p, sub.positions, sub.strand_offsets), sub)
+    for s in range(5):
+        o.solve_next_step(DT)
+    assert _rel(x[s0:s1], o.get_pos_vel()[0]) <= TOL

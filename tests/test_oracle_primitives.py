@@ -1,0 +1,80 @@
+"""Pins the oracle's arithmetic primitives to the reference's own header-only
+math (Kitten Rotor/SymMat/Common + glm 0.9.9), evaluated by oracle/_ref's probe
+and committed as tests/golden/kitten_vectors.npz.  Both sides are IEEE FP32
+without contraction, so agreement must be bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "kitten_vectors.npz"))
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("name,fn", [
+    ("rotor_mul", lambda x: pyoracle.rotor_mul(x[:4], x[4:8])),
+    ("rotor_rotate", lambda x: pyoracle.rotor_rotate(x[:4], x[4:7])),
+    ("rotor_from_to", lambda x: pyoracle.rotor_from_to(x[:3], x[3:6])),
+    ("seg_closest", lambda x: pyoracle.segment_closest_points(x[:3], x[3:6], x[6:9], x[9:12])),
+    ("sym_solve", lambda x: pyoracle.sym_solve(x[:6], x[6:9])),
+])
+def test_primitive_bit_exact(oracle_lib, name, fn):
+    xin, xout = GOLD[name + "_in"], GOLD[name + "_out"]
+    got = np.stack([fn(x) for x in xin])
+    mism = np.nonzero(np.any(_bits(got) != _bits(xout), axis=1))[0]
+    assert mism.size == 0, f"{name}: {mism.size} mismatches, first {mism[:5]} got {got[mism[:2]]} want {xout[mism[:2]]}"
+
+
+def test_morton_bit_exact(oracle_lib):
+    xin, xout = GOLD["morton_in"], GOLD["morton_out"].view(np.uint32)[:, 0]
+    got = np.array([pyoracle.morton3(*map(int, c)) for c in xin], np.uint32)
+    assert np.array_equal(got, xout)
+
+
+def test_morton_closed_forms(oracle_lib):
+    # unit axes interleave as x<<2 | y<<1 | z
+    assert pyoracle.morton3(1, 0, 0) == 4
+    assert pyoracle.morton3(0, 1, 0) == 2
+    assert pyoracle.morton3(0, 0, 1) == 1
+    assert pyoracle.morton3(1023, 1023, 1023) == 0x3FFFFFFF
+    assert pyoracle.morton3(2, 0, 0) == 32
+
+
+def test_orientation_update_expression(oracle_lib):
+    """SCR Eq. 18 composed from the primitives matches the reference types."""
+    xin, xout = GOLD["ori_update_in"], GOLD["ori_update_out"]
+    for x, want in zip(xin, xout):
+        v, b = x[:3].astype(np.float32), x[3:7].astype(np.float32)
+        lam = np.float32(np.sqrt(np.float32(np.dot(v, v)))) + np.float32(np.sqrt(np.float32(b[0] * b[0] + b[1] * b[1] + b[2] * b[2] + b[3] * b[3])))
+        t = pyoracle.rotor_mul(pyoracle.rotor_mul(np.array([v[0], v[1], v[2], 0], np.float32), b),
+                               np.array([1, 0, 0, 0], np.float32))
+        s = (t + lam * b).astype(np.float32)
+        # glm normalize: v * (1/sqrt(dot))
+        d = np.float32(np.float32(np.float32(s[0] * s[0] + s[1] * s[1]) + s[2] * s[2]) + s[3] * s[3])
+        got = (s * np.float32(np.float32(1) / np.sqrt(d))).astype(np.float32)
+        np.testing.assert_allclose(got, want, rtol=2e-6, atol=1e-7)
+
+
+def test_rotor_algebra_identities(oracle_lib):
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        a = rng.normal(size=4).astype(np.float32)
+        a /= np.linalg.norm(a)
+        v = rng.normal(size=3).astype(np.float32)
+        # rotation preserves length; conj undoes it; product with conj is identity
+        r = pyoracle.rotor_rotate(a, v)
+        np.testing.assert_allclose(np.linalg.norm(r), np.linalg.norm(v), rtol=1e-5)
+        back = pyoracle.rotor_rotate(np.array([-a[0], -a[1], -a[2], a[3]], np.float32), r)
+        np.testing.assert_allclose(back, v, atol=1e-5)
+        e = pyoracle.rotor_mul(a, np.array([-a[0], -a[1], -a[2], a[3]], np.float32))
+        np.testing.assert_allclose(e, [0, 0, 0, 1], atol=1e-6)
+        # fromTo maps its first argument onto its second
+        t = rng.normal(size=3).astype(np.float32)
+        t /= np.linalg.norm(t)
+        q = pyoracle.rotor_from_to(np.array([1, 0, 0], np.float32), t)
+        np.testing.assert_allclose(pyoracle.rotor_rotate(q, np.array([1, 0, 0], np.float32)), t, atol=1e-5)
