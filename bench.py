@@ -50,7 +50,23 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-sdf-leg", action="store_true", help="skip the with-SDF step timing")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (rocprof byte-counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: reads and writes mib MiB once per
+    launch, so FETCH_SIZE / WRITE_SIZE per dispatch can be scaled."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
 
 
 def ensure_built():
@@ -105,6 +121,9 @@ def cpu_baseline(scene, params, seconds):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the strand-sharded multi-GPU "
               "layout is described in DESIGN.md but not built", file=sys.stderr)

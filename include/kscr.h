@@ -158,6 +158,30 @@ int kscr_num_verts(const kscr_handle* h);
 int kscr_num_strands(const kscr_handle* h);
 /* Vertices per workgroup actually used by the step kernel. */
 int kscr_block_size(const kscr_handle* h);
+/* Autotuner: times `steps` steps of dt for every instantiated workgroup size
+ * that holds the longest strand (state is saved and restored), keeps the
+ * fastest and returns it in best_block (may be NULL). */
+int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block);
+
+/* ---- Batched form (no counterpart in the reference: its closest analogue is
+ * CantileverTestSolver, which steps independent solvers one after another,
+ * src/solvers/CantileverTestSolver.cpp:106-111).  A batch advances many
+ * independent handles (grooms, each with its own gravity, SDF and parameters)
+ * in one kernel launch per step, replayed from a captured hipGraph.  The
+ * handles keep ownership of their state; kscr_get_pos_vel etc. stay valid. */
+typedef struct kscr_batch kscr_batch;
+/* block_size 0 = the same choice as kscr_create's (>= the longest strand). */
+int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_size, kscr_batch** out);
+void kscr_batch_destroy(kscr_batch* b);
+/* nsteps steps of dt for every groom; with_sdf applies each groom's SDF pass
+ * after each step (kscr_solve_next_step semantics) where one is set. */
+int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf);
+/* use_graph = 0 launches the kernels directly instead of replaying a graph. */
+int kscr_batch_set_graph(kscr_batch* b, int use_graph);
+int kscr_batch_synchronize(kscr_batch* b);
+int kscr_batch_block_size(const kscr_batch* b);
+int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n);
+
 /* Copies the last error message (NUL terminated) into buf. */
 int kscr_last_error(const kscr_handle* h, char* buf, size_t n);
 /* Library-wide message of the last kscr_create failure. */

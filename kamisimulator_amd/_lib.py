@@ -102,6 +102,14 @@ SIGNATURES = {
     "kscr_block_size": (C.c_int, [_P]),
     "kscr_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "kscr_create_error": (C.c_char_p, []),
+    "kscr_autotune": (C.c_int, [_P, C.c_float, C.c_int, _IP]),
+    "kscr_batch_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.POINTER(_P)]),
+    "kscr_batch_destroy": (None, [_P]),
+    "kscr_batch_step": (C.c_int, [_P, C.c_float, C.c_int, C.c_int]),
+    "kscr_batch_set_graph": (C.c_int, [_P, C.c_int]),
+    "kscr_batch_synchronize": (C.c_int, [_P]),
+    "kscr_batch_block_size": (C.c_int, [_P]),
+    "kscr_batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
 }
 
 _lib = None
@@ -281,3 +289,52 @@ class Sim:
     @property
     def block_size(self):
         return self.lib.kscr_block_size(self.h)
+
+    def autotune(self, dt, steps=3):
+        best = C.c_int(0)
+        self._check(self.lib.kscr_autotune(self.h, dt, steps, C.byref(best)))
+        return best.value
+
+
+class Batch:
+    """Many handles (grooms) advanced by one launch per step (kscr_batch_*)."""
+
+    def __init__(self, sims, block_size=0):
+        self.lib = load()
+        self.sims = list(sims)  # keep the handles alive
+        arr = (C.c_void_p * len(self.sims))(*[s.h for s in self.sims])
+        b = C.c_void_p()
+        rc = self.lib.kscr_batch_create(arr, len(self.sims), block_size, C.byref(b))
+        if rc:
+            raise KscrError(rc, "kscr_batch_create failed")
+        self.b = b
+
+    def _check(self, rc):
+        if rc:
+            buf = C.create_string_buffer(512)
+            self.lib.kscr_batch_last_error(self.b, buf, 512)
+            raise KscrError(rc, buf.value.decode())
+
+    def step(self, dt, nsteps=1, with_sdf=True):
+        self._check(self.lib.kscr_batch_step(self.b, dt, nsteps, int(bool(with_sdf))))
+
+    def set_graph(self, on):
+        self._check(self.lib.kscr_batch_set_graph(self.b, int(bool(on))))
+
+    def synchronize(self):
+        self._check(self.lib.kscr_batch_synchronize(self.b))
+
+    @property
+    def block_size(self):
+        return self.lib.kscr_batch_block_size(self.b)
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.kscr_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

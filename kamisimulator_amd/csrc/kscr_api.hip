@@ -111,10 +111,10 @@ int validate_meta(kscr_handle* h) {
     return KSCR_OK;
 }
 
+int longest_strand(const kscr_handle* h);
+
 int choose_block(const kscr_handle* h) {
-    int longest = 0;
-    const int ns = (int)h->strand_off.size() - 1;
-    for (int s = 0; s < ns; s++) longest = std::max(longest, h->strand_off[s + 1] - h->strand_off[s]);
+    const int longest = longest_strand(h);
     if (h->p.block_size > 0) return h->p.block_size;
     // Smallest instantiated block that holds the longest strand, but at least
     // 512 so that several ~100-vertex strands share a workgroup (fewer idle lanes).
@@ -186,11 +186,9 @@ int upload_all(kscr_handle* h) {
     return KSCR_OK;
 }
 
-int launch_step(kscr_handle* h, float dt, bool with_sdf) {
-    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
-    if (!(dt > 0)) return KSCR_OK;  // Sim::advance returns for h <= 0
-    int rc = validate_meta(h);
-    if (rc) return rc;
+// Per-groom kernel arguments of one step (substep split of Sim::advance,
+// YB/sim/step.cpp:41-43); advances the handle's clock.
+StepArgs step_args(kscr_handle* h, float dt, bool with_sdf) {
     const int steps = std::max(1, (int)std::ceil(dt / h->p.max_h));
     h->last_h = h->h;
     h->h = dt / steps;
@@ -201,8 +199,6 @@ int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     a.q = h->d_q.p;
     a.q_rest = h->d_qr.p;
     a.mat = h->d_mat.p;
-    a.packs = h->d_packs.p;
-    a.num_packs = h->num_packs;
     a.num_substeps = steps;
     a.num_itr = h->p.num_itr;
     a.h = h->h;
@@ -220,13 +216,28 @@ int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     a.wind_sharpness = h->p.wind_sharpness;
     a.sdf_on = (with_sdf && h->sdf_on && h->sdf_set) ? 1 : 0;
     a.sdf = h->sdf;
-    hipError_t e = hipSetDevice(h->device);
-    if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
-    e = launch_step_fused(a, h->block, h->stream);
-    if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
     h->step_counter += steps;
     h->time += dt;
+    return a;
+}
+
+int launch_step(kscr_handle* h, float dt, bool with_sdf) {
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    if (!(dt > 0)) return KSCR_OK;  // Sim::advance returns for h <= 0
+    int rc = validate_meta(h);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    const StepArgs a = step_args(h, dt, with_sdf);
+    e = launch_step_fused(a, h->d_packs.p, h->num_packs, h->block, h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
     return KSCR_OK;
+}
+
+int longest_strand(const kscr_handle* h) {
+    int longest = 0;
+    for (size_t s = 0; s + 1 < h->strand_off.size(); s++) longest = std::max(longest, h->strand_off[s + 1] - h->strand_off[s]);
+    return longest;
 }
 
 }  // namespace
@@ -620,6 +631,266 @@ int kscr_block_size(const kscr_handle* h) { return h ? h->block : -1; }
 int kscr_last_error(const kscr_handle* h, char* buf, size_t n) {
     if (!h || !buf || n == 0) return KSCR_ERR_INVALID_ARGUMENT;
     std::snprintf(buf, n, "%s", h->err.c_str());
+    return KSCR_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
+    if (!h || steps <= 0 || !(dt > 0)) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    const size_t n3 = (size_t)h->n * 3;
+    DevBuf<float> sx, sv, su;
+    DevBuf<float4> sq;
+    if ((e = sx.alloc(n3)) != hipSuccess || (e = sv.alloc(n3)) != hipSuccess || (e = su.alloc(n3)) != hipSuccess ||
+        (e = sq.alloc(h->n)) != hipSuccess)
+        return h->hip_fail(e, "hipMalloc autotune scratch");
+    auto save = [&](bool restore) -> hipError_t {
+        hipError_t r;
+        const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        if (!restore) {
+            if ((r = hipMemcpyAsync(sx.p, h->d_x.p, n3 * 4, k, h->stream)) != hipSuccess) return r;
+            if ((r = hipMemcpyAsync(sv.p, h->d_v.p, n3 * 4, k, h->stream)) != hipSuccess) return r;
+            if ((r = hipMemcpyAsync(su.p, h->d_u.p, n3 * 4, k, h->stream)) != hipSuccess) return r;
+            return hipMemcpyAsync(sq.p, h->d_q.p, (size_t)h->n * 16, k, h->stream);
+        }
+        if ((r = hipMemcpyAsync(h->d_x.p, sx.p, n3 * 4, k, h->stream)) != hipSuccess) return r;
+        if ((r = hipMemcpyAsync(h->d_v.p, sv.p, n3 * 4, k, h->stream)) != hipSuccess) return r;
+        if ((r = hipMemcpyAsync(h->d_u.p, su.p, n3 * 4, k, h->stream)) != hipSuccess) return r;
+        return hipMemcpyAsync(h->d_q.p, sq.p, (size_t)h->n * 16, k, h->stream);
+    };
+    if ((e = save(false)) != hipSuccess) return h->hip_fail(e, "autotune save");
+    const float h0 = h->h, lh0 = h->last_h, t0 = h->time;
+    const long c0 = h->step_counter;
+    const int longest = longest_strand(h);
+    const int saved_block = h->p.block_size;
+    hipEvent_t ev0, ev1;
+    if ((e = hipEventCreate(&ev0)) != hipSuccess || (e = hipEventCreate(&ev1)) != hipSuccess)
+        return h->hip_fail(e, "hipEventCreate");
+    int best = -1;
+    float best_ms = 0;
+    int rc = KSCR_OK;
+    for (int b : kBlockSizes) {
+        if (b < longest) continue;
+        h->p.block_size = b;
+        if ((rc = upload_all(h)) != KSCR_OK) break;
+        if ((e = save(true)) != hipSuccess) { rc = h->hip_fail(e, "autotune restore"); break; }
+        h->h = h0, h->last_h = lh0, h->time = t0;
+        if ((rc = launch_step(h, dt, false)) != KSCR_OK) break;  // warm-up
+        (void)hipEventRecord(ev0, h->stream);
+        for (int k = 0; k < steps && rc == KSCR_OK; k++) rc = launch_step(h, dt, false);
+        (void)hipEventRecord(ev1, h->stream);
+        if ((e = hipEventSynchronize(ev1)) != hipSuccess) { rc = h->hip_fail(e, "hipEventSynchronize"); break; }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        if (best < 0 || ms < best_ms) best = b, best_ms = ms;
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    h->p.block_size = (rc == KSCR_OK && best > 0) ? best : saved_block;
+    int rc2 = upload_all(h);
+    if ((e = save(true)) != hipSuccess) return h->hip_fail(e, "autotune restore");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    h->h = h0, h->last_h = lh0, h->time = t0, h->step_counter = c0;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (best_block) *best_block = h->block;
+    return KSCR_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+struct kscr_batch {
+    std::vector<kscr_handle*> grooms;
+    int device = 0;
+    int block = 0;
+    int num_packs = 0;
+    DevBuf<StepArgs> d_args;
+    DevBuf<int4> d_packs;
+    std::vector<StepArgs> args;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool use_graph = true;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    float graph_dt = 0.f;
+    std::string err;
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return KSCR_ERR_HIP;
+    }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_steps = 0;
+    }
+};
+
+extern "C" {
+
+int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_size, kscr_batch** out) {
+    if (!handles || num_handles <= 0 || !out) return KSCR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* b = new kscr_batch;
+    int longest = 0;
+    for (int g = 0; g < num_handles; g++) {
+        kscr_handle* h = handles[g];
+        if (!h || !h->uploaded) {
+            delete b;
+            return KSCR_ERR_NOT_CONFIGURED;
+        }
+        if (g > 0 && h->device != handles[0]->device) {
+            delete b;
+            return KSCR_ERR_INVALID_ARGUMENT;
+        }
+        longest = std::max(longest, longest_strand(h));
+        b->grooms.push_back(h);
+    }
+    b->device = handles[0]->device;
+    int block = block_size;
+    if (block <= 0) {
+        for (int c : kBlockSizes)
+            if (c >= longest && c >= 512) { block = c; break; }
+        if (block <= 0)
+            for (int c : kBlockSizes)
+                if (c >= longest) { block = c; break; }
+    }
+    bool ok = false;
+    for (int c : kBlockSizes) ok |= (c == block);
+    if (!ok || block < longest) {
+        delete b;
+        return KSCR_ERR_STRAND_TOO_LONG;
+    }
+    b->block = block;
+    std::vector<int4> packs;
+    for (int g = 0; g < num_handles; g++) {
+        kscr_handle* h = handles[g];
+        std::vector<int2> p2;
+        int rc = build_packs(h, block, p2);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        for (const int2& pk : p2) packs.push_back(make_int4(pk.x, pk.y, g, 0));
+    }
+    b->num_packs = (int)packs.size();
+    hipError_t e;
+    if ((e = hipSetDevice(b->device)) != hipSuccess || (e = b->d_args.alloc(num_handles)) != hipSuccess ||
+        (e = b->d_packs.alloc(packs.size())) != hipSuccess) {
+        delete b;
+        return KSCR_ERR_HIP;
+    }
+    if ((e = hipStreamCreate(&b->stream)) != hipSuccess) {
+        delete b;
+        return KSCR_ERR_HIP;
+    }
+    b->own_stream = true;
+    if ((e = hipMemcpyAsync(b->d_packs.p, packs.data(), packs.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(b->stream)) != hipSuccess) {
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return KSCR_ERR_HIP;
+    }
+    b->args.resize(num_handles);
+    *out = b;
+    return KSCR_OK;
+}
+
+void kscr_batch_destroy(kscr_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->drop_graph();
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int kscr_batch_set_graph(kscr_batch* b, int use_graph) {
+    if (!b) return KSCR_ERR_INVALID_ARGUMENT;
+    b->use_graph = use_graph != 0;
+    return KSCR_OK;
+}
+
+int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
+    if (!b || nsteps < 0) return KSCR_ERR_INVALID_ARGUMENT;
+    if (nsteps == 0 || !(dt > 0)) return KSCR_OK;
+    hipError_t e;
+    if ((e = hipSetDevice(b->device)) != hipSuccess) return b->hip_fail(e, "hipSetDevice");
+    for (size_t g = 0; g < b->grooms.size(); g++) {
+        kscr_handle* h = b->grooms[g];
+        int rc = validate_meta(h);
+        if (rc) return b->fail(rc, "groom " + std::to_string(g) + ": " + h->err);
+        // the handles' own streams may still hold work on their state
+        if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return b->hip_fail(e, "hipStreamSynchronize");
+        b->args[g] = step_args(h, dt, with_sdf != 0);
+        // later steps of this call: the clock kernel advances the device copy;
+        // the host mirror follows the same float arithmetic
+        for (int k = 1; k < nsteps; k++) {
+            h->last_h = h->h;
+            h->time += dt;
+            h->step_counter += b->args[g].num_substeps;
+        }
+    }
+    if ((e = hipMemcpyAsync(b->d_args.p, b->args.data(), b->args.size() * sizeof(StepArgs), hipMemcpyHostToDevice,
+                            b->stream)) != hipSuccess)
+        return b->hip_fail(e, "hipMemcpyAsync batch arguments");
+    const int ng = (int)b->grooms.size();
+    if (!b->use_graph) {
+        for (int k = 0; k < nsteps; k++) {
+            if ((e = launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream)) != hipSuccess)
+                return b->hip_fail(e, "batch step launch");
+            if ((e = launch_batch_clock(b->d_args.p, ng, dt, b->stream)) != hipSuccess)
+                return b->hip_fail(e, "batch clock launch");
+        }
+        return KSCR_OK;
+    }
+    if (!b->graph || b->graph_steps != nsteps || b->graph_dt != dt) {
+        b->drop_graph();
+        hipGraph_t graph;
+        if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return b->hip_fail(e, "hipStreamBeginCapture");
+        hipError_t le = hipSuccess;
+        for (int k = 0; k < nsteps && le == hipSuccess; k++) {
+            le = launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream);
+            if (le == hipSuccess) le = launch_batch_clock(b->d_args.p, ng, dt, b->stream);
+        }
+        if ((e = hipStreamEndCapture(b->stream, &graph)) != hipSuccess) return b->hip_fail(e, "hipStreamEndCapture");
+        if (le != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return b->hip_fail(le, "batch launch under capture");
+        }
+        e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            return b->hip_fail(e, "hipGraphInstantiate");
+        }
+        b->graph_steps = nsteps;
+        b->graph_dt = dt;
+    }
+    if ((e = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) return b->hip_fail(e, "hipGraphLaunch");
+    return KSCR_OK;
+}
+
+int kscr_batch_synchronize(kscr_batch* b) {
+    if (!b) return KSCR_ERR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return b->hip_fail(e, "hipStreamSynchronize");
+    return KSCR_OK;
+}
+
+int kscr_batch_block_size(const kscr_batch* b) { return b ? b->block : -1; }
+
+int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n) {
+    if (!b || !buf || n == 0) return KSCR_ERR_INVALID_ARGUMENT;
+    std::snprintf(buf, n, "%s", b->err.c_str());
     return KSCR_OK;
 }
 

@@ -36,8 +36,6 @@ struct StepArgs {
     float4* q;
     const float4* q_rest;
     const float4* mat;
-    const int2* packs;
-    int num_packs;
     int num_substeps;
     int num_itr;
     float h;
@@ -53,9 +51,15 @@ struct StepArgs {
     SdfArgs sdf;
 };
 
-// One launch advances every strand by num_substeps substeps (and applies the
-// SDF pass afterwards when sdf_on).  block is the workgroup size in vertices.
-hipError_t launch_step_fused(const StepArgs& a, int block, hipStream_t stream);
+// One launch advances every strand of the packs by num_substeps substeps (and
+// applies the SDF pass afterwards when sdf_on).  block is the workgroup size
+// in vertices; packs[p] = [begin, end) of workgroup p.
+hipError_t launch_step_fused(const StepArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream);
+// Batched form: packs[p] = (begin, end, groom, 0) index the device table of
+// per-groom arguments.
+hipError_t launch_step_batch(const StepArgs* grooms, const int4* packs, int num_packs, int block, hipStream_t stream);
+// last_h <- h, time <- time + dt for n grooms of a device table.
+hipError_t launch_batch_clock(StepArgs* grooms, int n, float dt, hipStream_t stream);
 
 // Block sizes the fused kernel is instantiated for.
 constexpr int kBlockSizes[] = {64, 128, 256, 512, 1024};

@@ -1,0 +1,60 @@
+// kscr_lbvh.h -- linear BVH over axis-aligned boxes on gfx950.
+//
+// Same tree as the reference's Kitten GPU LBVH
+// (KE/KittenGpuLBVH/lbvh.cu:53-426): 30-bit Morton codes of the box centres
+// in the root box, a stable sort of the codes, Karras's internal-node build
+// with keys (code << 32 | index), a bottom-up refit, and a self query that
+// reports each overlapping pair once.  The sort is this library's own stable
+// LSD radix sort.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kscr {
+
+// 64-byte node: parent/left/right carry the child side / leaf bit in the MSB,
+// fence is the other end of the node's leaf range; two child boxes.
+struct alignas(64) LbvhNode {
+    uint32_t parent, left, right, fence;
+    float b[2][6];  // (min xyz, max xyz) of the left and right child
+};
+
+struct LbvhDev {
+    int n = 0;
+    int cap = 0;
+    int tiles = 0;
+    float* aabb = nullptr;  // 6 floats per object
+    uint32_t* codes = nullptr;
+    uint32_t* ids = nullptr;
+    uint32_t* codes_tmp = nullptr;
+    uint32_t* ids_tmp = nullptr;
+    uint32_t* hist = nullptr;      // 256 * tiles
+    uint32_t* hist_sums = nullptr;  // scan partials
+    float* partial = nullptr;       // root reduction partials (6 per block)
+    float* root = nullptr;          // 6 floats
+    LbvhNode* nodes = nullptr;
+    uint32_t* leaf_parent = nullptr;
+    int* flags = nullptr;
+    int* max_depth = nullptr;
+    int* count = nullptr;  // query pair counter
+    bool built = false;
+};
+
+hipError_t lbvh_reserve(LbvhDev& d, int n);
+void lbvh_release(LbvhDev& d);
+// Builds the tree over d.aabb[0 .. n) (all device work on `stream`).
+hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream);
+// Recomputes the node boxes from d.aabb for the existing structure.
+hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream);
+// Self query: writes pairs (found object, query object), the found leaf lying
+// at a larger sorted position than the query, into pairs[0 .. cap) and the
+// number found (which may exceed cap) into *d.count.  Pairs whose object ids
+// differ by at most min_gap are skipped (min_gap < 0 keeps all).
+hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, hipStream_t stream);
+
+// Fills n ints with a value (a kernel, so it is safe under graph capture).
+hipError_t fill_i32(int* p, int value, size_t n, hipStream_t stream);
+
+}  // namespace kscr

@@ -1,0 +1,462 @@
+// kscr_lbvh.hip -- LBVH build / refit / self query for gfx950.
+//
+// Reference: KE/KittenGpuLBVH/lbvh.cu.  The Morton codes, the sorted order
+// and the tree are bit-exact with the reference's (the sort is stable, the
+// Karras build is deterministic given the sorted keys); refit results do not
+// depend on thread order (min/max and the stack-depth recursion commute).
+#include <cstdio>
+
+#include "kscr_lbvh.h"
+
+namespace kscr {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kTile = kSortThreads * kSortItems;  // keys per radix tile
+constexpr int kScanChunk = 1024;
+constexpr int kQueryBlock = 128;
+constexpr int kQueryStack = 64;
+
+__device__ inline uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// float -> int toward zero, saturating, NaN -> 0 (the GPU conversion the
+// reference relies on for empty boxes, whose centres are NaN)
+__device__ inline int cvt_rzi(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (int)0x80000000;
+    return (int)f;
+}
+
+__device__ inline bool intersects(const float* a, const float* b) {
+    return !(a[3] <= b[0] || a[4] <= b[1] || a[5] <= b[2] || a[0] >= b[3] || a[1] >= b[4] || a[2] >= b[5]);
+}
+
+// ---------------------------------------------------------------- root box
+__global__ void root_partial_kernel(const float* __restrict__ aabb, int n, float* __restrict__ partial) {
+    __shared__ float s[6][256];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* a = aabb + 6 * (size_t)i;
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fminf(mn[k], a[k]);
+            mx[k] = fmaxf(mx[k], a[3 + k]);
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = mn[k];
+        s[3 + k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; k++) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void root_final_kernel(const float* __restrict__ partial, int m, float* __restrict__ root) {
+    if (threadIdx.x >= 6) return;
+    const int k = threadIdx.x;
+    float v = k < 3 ? INFINITY : -INFINITY;
+    for (int b = 0; b < m; b++) v = k < 3 ? fminf(v, partial[6 * b + k]) : fmaxf(v, partial[6 * b + k]);
+    root[k] = v;
+}
+
+// mortonKernel (lbvh.cu:53-60)
+__global__ void morton_kernel(const float* __restrict__ aabb, const float* __restrict__ root, int n,
+                              uint32_t* __restrict__ codes, uint32_t* __restrict__ ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* a = aabb + 6 * (size_t)i;
+    int c[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float centre = (a[k] + a[3 + k]) * 0.5f;
+        const float coord = (centre - root[k]) / (root[3 + k] - root[k]) * 1024.f;
+        const int v = cvt_rzi(coord);
+        c[k] = v < 0 ? 0 : (v > 1023 ? 1023 : v);
+    }
+    codes[i] = expand_bits((uint32_t)c[0]) * 4 + expand_bits((uint32_t)c[1]) * 2 + expand_bits((uint32_t)c[2]);
+    ids[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, int n, int shift,
+                                                                  int tiles, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTile;
+    for (int r = 0; r < kSortItems; r++) {
+        const int i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan of chunks of kScanChunk values; chunk totals to sums
+__global__ __launch_bounds__(256) void scan_chunk_kernel(uint32_t* __restrict__ v, size_t total,
+                                                          uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s[kScanChunk];
+    const size_t base = (size_t)blockIdx.x * kScanChunk;
+    for (int k = threadIdx.x; k < kScanChunk; k += 256) s[k] = (base + k < total) ? v[base + k] : 0u;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 values with 256 threads x 4
+    for (int off = 1; off < kScanChunk; off <<= 1) {
+        uint32_t tmp[4];
+        for (int j = 0; j < 4; j++) {
+            const int k = threadIdx.x + 256 * j;
+            tmp[j] = (k >= off) ? s[k - off] : 0u;
+        }
+        __syncthreads();
+        for (int j = 0; j < 4; j++) s[threadIdx.x + 256 * j] += tmp[j];
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < kScanChunk; k += 256)
+        if (base + k < total) v[base + k] = (k == 0) ? 0u : s[k - 1];
+    if (threadIdx.x == 0) sums[blockIdx.x] = s[kScanChunk - 1];
+}
+
+// exclusive scan of the chunk totals in one workgroup (sequential over
+// blocks of 256 values with a running carry)
+__global__ __launch_bounds__(256) void scan_sums_kernel(uint32_t* __restrict__ sums, int m) {
+    __shared__ uint32_t s[256];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < m; base += 256) {
+        const int k = base + threadIdx.x;
+        s[threadIdx.x] = k < m ? sums[k] : 0u;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const uint32_t t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        const uint32_t incl = s[threadIdx.x];
+        const uint32_t excl = carry + incl - (k < m ? sums[k] : 0u);
+        __syncthreads();
+        if (k < m) sums[k] = excl;
+        if (threadIdx.x == 255) carry += incl;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void scan_add_kernel(uint32_t* __restrict__ v, size_t total,
+                                                        const uint32_t* __restrict__ sums) {
+    const size_t base = (size_t)blockIdx.x * kScanChunk;
+    const uint32_t add = sums[blockIdx.x];
+    for (int k = threadIdx.x; k < kScanChunk; k += 256)
+        if (base + k < total) v[base + k] += add;
+}
+
+// Stable scatter: within a tile, items keep their order (round, wave, lane);
+// tiles keep their order through the digit-major, tile-minor offsets.
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t* __restrict__ keys,
+                                                                     const uint32_t* __restrict__ vals, int n, int shift,
+                                                                     int tiles, const uint32_t* __restrict__ offsets,
+                                                                     uint32_t* __restrict__ keys_out,
+                                                                     uint32_t* __restrict__ vals_out) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wave_cnt[kWaves][256];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    running[t] = offsets[(size_t)t * tiles + blockIdx.x];
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int base = blockIdx.x * kTile;
+    for (int r = 0; r < kSortItems; r++) {
+        for (int w = 0; w < kWaves; w++) wave_cnt[w][t] = 0;
+        __syncthreads();
+        const int i = base + r * kSortThreads + t;
+        const bool valid = i < n;
+        uint32_t key = 0, val = 0, digit = 0;
+        if (valid) {
+            key = keys[i];
+            val = vals[i];
+            digit = (key >> shift) & 255u;
+        }
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (digit >> b) & 1u;
+            const uint64_t ones = __ballot(valid && bit);
+            match &= bit ? ones : ~ones;
+        }
+        const uint32_t rank = (uint32_t)__popcll(match & lt_mask);
+        if (valid && (match & lt_mask) == 0) wave_cnt[wave][digit] = (uint32_t)__popcll(match);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = running[digit] + rank;
+            for (int w = 0; w < wave; w++) off += wave_cnt[w][digit];
+            keys_out[off] = key;
+            vals_out[off] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < kWaves; w++) add += wave_cnt[w][t];
+        running[t] += add;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- build
+__device__ inline int clz64(uint64_t x) { return x ? __clzll(x) : 64; }
+__device__ inline uint64_t sort_key(const uint32_t* codes, int i) { return ((uint64_t)codes[i] << 32) | (uint32_t)i; }
+
+// Karras's internal node construction (lbvh.cu:68-150)
+__global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n, LbvhNode* __restrict__ nodes,
+                                      uint32_t* __restrict__ leaf_parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int lo, hi;
+    if (i == 0) {
+        lo = 0;
+        hi = n - 1;
+    } else {
+        const uint64_t self = sort_key(codes, i);
+        const int dl = clz64(self ^ sort_key(codes, i - 1));
+        const int dr = clz64(self ^ sort_key(codes, i + 1));
+        const int d = dr > dl ? 1 : -1;
+        const int dmin = min(dl, dr);
+        int lmax = 2;
+        for (;;) {
+            const int j = i + d * lmax;
+            if (j < 0 || j >= n) break;
+            if (clz64(self ^ sort_key(codes, j)) <= dmin) break;
+            lmax <<= 1;
+        }
+        int l = 0;
+        for (int t = lmax >> 1; t > 0; t >>= 1) {
+            const int j = i + (l + t) * d;
+            if (j >= 0 && j < n && clz64(self ^ sort_key(codes, j)) > dmin) l += t;
+        }
+        const int j = i + l * d;
+        lo = min(i, j);
+        hi = max(i, j);
+    }
+    nodes[i].fence = (i == lo) ? (uint32_t)hi : (uint32_t)lo;
+    const uint64_t fk = sort_key(codes, lo);
+    const int dnode = clz64(fk ^ sort_key(codes, hi));
+    int split = lo, stride = hi - lo;
+    do {
+        stride = (stride + 1) >> 1;
+        const int mid = split + stride;
+        if (mid < hi && clz64(fk ^ sort_key(codes, mid)) > dnode) split = mid;
+    } while (stride > 1);
+    uint32_t left, right;
+    if (lo == split) {
+        leaf_parent[split] = (uint32_t)i;
+        left = (uint32_t)split | 0x80000000u;
+    } else {
+        left = (uint32_t)split;
+        nodes[split].parent = (uint32_t)i;
+    }
+    if (hi == split + 1) {
+        leaf_parent[split + 1] = (uint32_t)i | 0x80000000u;
+        right = (uint32_t)(split + 1) | 0x80000000u;
+    } else {
+        right = (uint32_t)(split + 1);
+        nodes[split + 1].parent = (uint32_t)i | 0x80000000u;
+    }
+    nodes[i].left = left;
+    nodes[i].right = right;
+}
+
+// Bottom-up refit (lbvh.cu:153-188): the second thread to reach a node
+// merges both children and continues; depth = max(dL, dR + 1).
+__global__ void refit_kernel(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
+                             const uint32_t* __restrict__ leaf_parent, LbvhNode* nodes, int* flags, int n,
+                             int* max_depth) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int depth = 1;
+    float last[6];
+    const float* a = aabb + 6 * (size_t)ids[t];
+    for (int k = 0; k < 6; k++) last[k] = a[k];
+    uint32_t parent = leaf_parent[t];
+    for (;;) {
+        const int right = (parent >> 31) & 1;
+        const uint32_t p = parent & 0x7FFFFFFFu;
+        volatile float* dst = nodes[p].b[right];
+        for (int k = 0; k < 6; k++) dst[k] = last[k];
+        __threadfence();
+        const int other = atomicOr(flags + p, depth);
+        if (!other) return;
+        __threadfence();
+        depth = right ? max(depth + 1, other) : max(depth, other + 1);
+        if (p == 0) {
+            *max_depth = depth;
+            return;
+        }
+        const volatile float* sib = nodes[p].b[1 - right];
+        for (int k = 0; k < 3; k++) {
+            const float smn = sib[k], smx = sib[3 + k];
+            last[k] = fminf(last[k], smn);
+            last[3 + k] = fmaxf(last[3 + k], smx);
+        }
+        parent = *(volatile uint32_t*)&nodes[p].parent;
+    }
+}
+
+// ---------------------------------------------------------------- query
+// Self query with IGNORE_SELF (lbvh.cu:196-289).  Each lane keeps its DFS
+// stack in LDS (column per lane).
+__global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restrict__ aabb,
+                                                            const uint32_t* __restrict__ ids,
+                                                            const LbvhNode* __restrict__ nodes, int n,
+                                                            int2* __restrict__ pairs, int cap, int min_gap,
+                                                            int* __restrict__ count) {
+    __shared__ uint32_t stack[kQueryStack][kQueryBlock];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || n < 2) return;
+    const int qid = (int)ids[t];
+    float q[6];
+    for (int k = 0; k < 6; k++) q[k] = aabb[6 * (size_t)qid + k];
+    if (!(q[0] <= q[3])) return;  // empty box (strand tip): intersects nothing
+    int sp = 0;
+    stack[sp++][threadIdx.x] = 0;
+    while (sp > 0) {
+        uint32_t c = stack[--sp][threadIdx.x];
+        if (c & 0x80000000u) {
+            c &= 0x7FFFFFFFu;
+            if ((int)c <= t) continue;
+            const int fid = (int)ids[c];
+            if (min_gap >= 0 && abs(fid - qid) <= min_gap) continue;
+            const int slot = atomicAdd(count, 1);
+            if (slot < cap) pairs[slot] = make_int2(fid, qid);
+        } else {
+            const LbvhNode& nd = nodes[c];
+            if (max(c, nd.fence) <= (uint32_t)t) continue;
+            // the tree depth is checked on the host (max_depth <= kQueryStack)
+            if (intersects(nd.b[0], q)) stack[sp++][threadIdx.x] = nd.left;
+            if (intersects(nd.b[1], q)) stack[sp++][threadIdx.x] = nd.right;
+        }
+    }
+}
+
+__global__ void fill_i32_kernel(int* p, int value, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+template <class T>
+hipError_t dmalloc(T*& p, size_t count) {
+    return hipMalloc((void**)&p, count * sizeof(T) + 64);
+}
+
+}  // namespace
+
+hipError_t fill_i32(int* p, int value, size_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, value, n);
+    return hipGetLastError();
+}
+
+void lbvh_release(LbvhDev& d) {
+    void* ps[] = {d.aabb, d.codes, d.ids, d.codes_tmp, d.ids_tmp, d.hist, d.hist_sums, d.partial,
+                  d.root, d.nodes, d.leaf_parent, d.flags, d.max_depth, d.count};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    d = LbvhDev{};
+}
+
+hipError_t lbvh_reserve(LbvhDev& d, int n) {
+    if (n <= d.cap) return hipSuccess;
+    lbvh_release(d);
+    const int tiles = (n + kTile - 1) / kTile;
+    const size_t hist_total = (size_t)256 * tiles;
+    const size_t chunks = (hist_total + kScanChunk - 1) / kScanChunk;
+    hipError_t e;
+    if ((e = dmalloc(d.aabb, (size_t)6 * n)) != hipSuccess || (e = dmalloc(d.codes, n)) != hipSuccess ||
+        (e = dmalloc(d.ids, n)) != hipSuccess || (e = dmalloc(d.codes_tmp, n)) != hipSuccess ||
+        (e = dmalloc(d.ids_tmp, n)) != hipSuccess || (e = dmalloc(d.hist, hist_total)) != hipSuccess ||
+        (e = dmalloc(d.hist_sums, chunks)) != hipSuccess || (e = dmalloc(d.partial, 6 * 1024)) != hipSuccess ||
+        (e = dmalloc(d.root, 6)) != hipSuccess || (e = dmalloc(d.nodes, (size_t)(n > 1 ? n - 1 : 1))) != hipSuccess ||
+        (e = dmalloc(d.leaf_parent, n)) != hipSuccess || (e = dmalloc(d.flags, n)) != hipSuccess ||
+        (e = dmalloc(d.max_depth, 1)) != hipSuccess || (e = dmalloc(d.count, 1)) != hipSuccess) {
+        lbvh_release(d);
+        return e;
+    }
+    d.cap = n;
+    return hipSuccess;
+}
+
+hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
+    if (n > d.cap) return hipErrorInvalidValue;
+    d.n = n;
+    d.built = false;
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + 255) / 256;
+    const int pblocks = nb < 1024 ? nb : 1024;
+    hipLaunchKernelGGL(root_partial_kernel, dim3(pblocks), dim3(256), 0, stream, d.aabb, n, d.partial);
+    hipLaunchKernelGGL(root_final_kernel, dim3(1), dim3(64), 0, stream, d.partial, pblocks, d.root);
+    hipLaunchKernelGGL(morton_kernel, dim3(nb), dim3(256), 0, stream, d.aabb, d.root, n, d.codes, d.ids);
+    const int tiles = (n + kTile - 1) / kTile;
+    d.tiles = tiles;
+    const size_t total = (size_t)256 * tiles;
+    const int chunks = (int)((total + kScanChunk - 1) / kScanChunk);
+    uint32_t *kin = d.codes, *vin = d.ids, *kout = d.codes_tmp, *vout = d.ids_tmp;
+    for (int shift = 0; shift < 32; shift += 8) {
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, n, shift, tiles, d.hist);
+        hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+        hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, stream, d.hist_sums, chunks);
+        hipLaunchKernelGGL(scan_add_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, n, shift, tiles,
+                           d.hist, kout, vout);
+        uint32_t* t1 = kin;
+        kin = kout;
+        kout = t1;
+        t1 = vin;
+        vin = vout;
+        vout = t1;
+    }
+    // four passes: the sorted data is back in d.codes / d.ids
+    if (n > 1)
+        hipLaunchKernelGGL(build_internal_kernel, dim3((n - 1 + 255) / 256), dim3(256), 0, stream, d.codes, n, d.nodes,
+                           d.leaf_parent);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    d.built = true;
+    return lbvh_refit(d, stream);
+}
+
+hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
+    if (!d.built) return hipErrorInvalidValue;
+    const int n = d.n;
+    if (n <= 1) return fill_i32(d.max_depth, 1, 1, stream);
+    hipError_t e = fill_i32(d.flags, 0, (size_t)n - 1, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(refit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d.aabb, d.ids, d.leaf_parent, d.nodes,
+                       d.flags, n, d.max_depth);
+    return hipGetLastError();
+}
+
+hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, hipStream_t stream) {
+    if (!d.built) return hipErrorInvalidValue;
+    hipError_t e = fill_i32(d.count, 0, 1, stream);
+    if (e != hipSuccess) return e;
+    if (d.n < 2) return hipSuccess;
+    hipLaunchKernelGGL(query_kernel, dim3((d.n + kQueryBlock - 1) / kQueryBlock), dim3(kQueryBlock), 0, stream, d.aabb,
+                       d.ids, d.nodes, d.n, pairs, cap, min_gap, d.count);
+    return hipGetLastError();
+}
+
+int lbvh_query_stack_limit() { return kQueryStack; }
+
+}  // namespace kscr

@@ -85,16 +85,23 @@ __device__ inline void sdf_project(const SdfArgs& s, f3& x, f3& v) {
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void step_fused_kernel(StepArgs a) {
-    __shared__ float s_x[3][BLOCK];
-    __shared__ float s_dx[2][3][BLOCK];
-    __shared__ float4 s_q[2][BLOCK];
-    __shared__ float4 s_f[BLOCK];
+struct StepLds {
+    float x[3][BLOCK];
+    float dx[2][3][BLOCK];
+    float4 q[2][BLOCK];
+    float4 f[BLOCK];
+};
 
-    const int2 pk = a.packs[blockIdx.x];
+// Advances the strands of one pack [vbeg, vend) through a whole step.
+template <int BLOCK>
+__device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, const int vend, StepLds<BLOCK>& L) {
+    float(&s_x)[3][BLOCK] = L.x;
+    float(&s_dx)[2][3][BLOCK] = L.dx;
+    float4(&s_q)[2][BLOCK] = L.q;
+    float4(&s_f)[BLOCK] = L.f;
     const int t = threadIdx.x;
-    const int i = pk.x + t;
-    const bool act = i < pk.y;
+    const int i = vbeg + t;
+    const bool act = i < vend;
 
     f3 x{0, 0, 0}, v{0, 0, 0}, u{0, 0, 0};
     f4 q{0, 0, 0, 1}, qr{0, 0, 0, 1}, qrp{0, 0, 0, 1};
@@ -240,30 +247,78 @@ __global__ __launch_bounds__(BLOCK) void step_fused_kernel(StepArgs a) {
     }
 }
 
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void step_fused_kernel(StepArgs a, const int2* __restrict__ packs) {
+    __shared__ StepLds<BLOCK> lds;
+    const int2 pk = packs[blockIdx.x];
+    run_pack<BLOCK>(a, pk.x, pk.y, lds);
+}
+
+// Batched form: every workgroup reads its groom's arguments (pointers,
+// substep lengths, gravity, SDF grid) from a device table.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void step_batch_kernel(const StepArgs* __restrict__ grooms,
+                                                           const int4* __restrict__ packs) {
+    __shared__ StepLds<BLOCK> lds;
+    const int4 pk = packs[blockIdx.x];
+    const StepArgs a = grooms[pk.z];
+    run_pack<BLOCK>(a, pk.x, pk.y, lds);
+}
+
+// After a batched step: last_h <- h and time <- time + dt for every groom
+// (Sim::advance, YB/sim/step.cpp:41-43 and :59).
+__global__ void batch_clock_kernel(StepArgs* grooms, int n, float dt) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    grooms[g].last_h = grooms[g].h;
+    grooms[g].time = grooms[g].time + dt;
+}
+
 }  // namespace
 
-hipError_t launch_step_fused(const StepArgs& a, int block, hipStream_t stream) {
-    if (a.num_packs <= 0) return hipSuccess;
-    const dim3 grid(a.num_packs);
+hipError_t launch_step_fused(const StepArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream) {
+    if (num_packs <= 0) return hipSuccess;
+    const dim3 grid(num_packs);
     switch (block) {
-        case 64:
-            hipLaunchKernelGGL(step_fused_kernel<64>, grid, dim3(64), 0, stream, a);
-            break;
-        case 128:
-            hipLaunchKernelGGL(step_fused_kernel<128>, grid, dim3(128), 0, stream, a);
-            break;
-        case 256:
-            hipLaunchKernelGGL(step_fused_kernel<256>, grid, dim3(256), 0, stream, a);
-            break;
-        case 512:
-            hipLaunchKernelGGL(step_fused_kernel<512>, grid, dim3(512), 0, stream, a);
-            break;
-        case 1024:
-            hipLaunchKernelGGL(step_fused_kernel<1024>, grid, dim3(1024), 0, stream, a);
-            break;
+#define KSCR_CASE(B)                                                                      \
+    case B:                                                                               \
+        hipLaunchKernelGGL(step_fused_kernel<B>, grid, dim3(B), 0, stream, a, packs);     \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
         default:
             return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_step_batch(const StepArgs* grooms, const int4* packs, int num_packs, int block, hipStream_t stream) {
+    if (num_packs <= 0) return hipSuccess;
+    const dim3 grid(num_packs);
+    switch (block) {
+#define KSCR_CASE(B)                                                                       \
+    case B:                                                                                \
+        hipLaunchKernelGGL(step_batch_kernel<B>, grid, dim3(B), 0, stream, grooms, packs); \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_clock(StepArgs* grooms, int n, float dt, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_clock_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, grooms, n, dt);
     return hipGetLastError();
 }
 

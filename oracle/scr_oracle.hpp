@@ -38,6 +38,11 @@
 #include <vector>
 
 namespace orc {
+inline uint32_t morton3(int x, int y, int z);
+}
+#include "lbvh_oracle.hpp"
+
+namespace orc {
 
 // ---------------------------------------------------------------- vectors
 template <class T>
@@ -234,6 +239,53 @@ inline uint32_t morton3(int x, int y, int z) {
     return morton_expand_bits((uint32_t)x) * 4 + morton_expand_bits((uint32_t)y) * 2 + morton_expand_bits((uint32_t)z);
 }
 
+// Natural logarithm evaluated by a fixed sequence of IEEE float operations
+// (frexp-style split d = m 2^e with m in [sqrt(1/2), sqrt(2)), then
+// log m = 2 atanh(s), s = (m-1)/(m+1), by its odd series to s^13, plus e ln 2
+// with ln 2 split into a high and a low part).  The device code uses the same
+// sequence, so the barrier energy rounds identically on both sides.
+inline float det_logf(float d) {
+    int e = 0;
+    float m = std::frexp(d, &e);  // d = m 2^e, m in [0.5, 1)
+    if (m < 0.70710678f) {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 1.0f / 13.0f;
+    p = p * s2 + 1.0f / 11.0f;
+    p = p * s2 + 1.0f / 9.0f;
+    p = p * s2 + 1.0f / 7.0f;
+    p = p * s2 + 1.0f / 5.0f;
+    p = p * s2 + 1.0f / 3.0f;
+    p = p * s2 + 1.0f;
+    const float lm = 2.0f * s * p;
+    const float fe = (float)e;
+    return fe * 6.93145752e-1f + (lm + fe * 1.42860677e-6f);
+}
+inline double det_logf(double d) { return std::log(d); }
+
+// Symmetric 3x3 accumulation helpers (hess3 operators, SymMat.h:31-64)
+template <class T>
+inline void sym_add(Sym3<T>& a, const Sym3<T>& b) {
+    for (int k = 0; k < 6; k++) a.d[k] += b.d[k];
+}
+template <class T>
+inline Sym3<T> sym_outer(V3<T> v) {
+    return Sym3<T>{{v.x * v.x, v.y * v.y, v.z * v.z, v.x * v.y, v.x * v.z, v.y * v.z}};
+}
+template <class T>
+inline Sym3<T> sym_scale(const Sym3<T>& a, T s) {
+    Sym3<T> r;
+    for (int k = 0; k < 6; k++) r.d[k] = a.d[k] * s;
+    return r;
+}
+template <class T>
+inline void sym_sub(Sym3<T>& a, const Sym3<T>& b) {
+    for (int k = 0; k < 6; k++) a.d[k] -= b.d[k];
+}
+
 // ---------------------------------------------------------------- params
 // Same field order as kscr_params (include/kscr.h); MetaData in the reference
 // (YB/YarnBall.h:61-124).
@@ -304,6 +356,15 @@ struct Sim {
     // per-substep scratch
     std::vector<V3<T>> y, dx, dxNew;
     std::vector<R4<T>> qNew;
+
+    // hair-hair contacts (YB/sim/collision.cu, YBSimulator.cu:424-470)
+    static const int kMaxCols = 16;  // MAX_COLLISIONS_PER_SEGMENT (YarnBall.h:19)
+    std::vector<std::vector<int>> cols;  // per segment, sorted ascending
+    std::vector<T> maxStep;
+    Lbvh bvh;
+    float lastBVHRebuild = std::numeric_limits<float>::infinity();
+    int error = 0;     // 8 = more than kMaxCols contacts on a segment
+    long num_pairs = 0;  // broad-phase pairs of the last detection
 
     T h = T(4e-4f), lastH = T(4e-4f), time = 0;
     float hF = 4e-4f, lastHF = 4e-4f, timeF = 0;
@@ -435,6 +496,8 @@ struct Sim {
             kStretch[i] = (T)kStretchF[i];
         }
         y.assign(n, V3<T>{0, 0, 0});
+        cols.assign(n, std::vector<int>());
+        maxStep.assign(n, T(1));
         dx.assign(n, V3<T>{0, 0, 0});
         dxNew.assign(n, V3<T>{0, 0, 0});
         qNew.assign(n, R4<T>{0, 0, 0, 1});
@@ -479,19 +542,93 @@ struct Sim {
         }
     }
 
-    // Position solve of vertex i (YB/sim/cosserat.cu:112-287), Jacobi: reads
-    // dx (iteration k), q (iteration k).  Contacts are not part of this oracle
-    // path (hair-hair collision is off in every shipped configuration).
-    // Returns the contribution of segment i to vertex i+1 ("f2"/"H2" of the
-    // reference) through the out parameters when asked.
-    void segment_terms(int i, T damp, V3<T>& fi, T& di, V3<T>& fj) const {
+    // Position solve (YB/sim/cosserat.cu:112-287), Jacobi: every read sees the
+    // iteration-k dx and q.  The reference splits a vertex's work into two
+    // "sectors": sector 0 starts from the inertia terms and adds the stretch
+    // term of segment i and the contacts at even list positions; sector 1 sums
+    // the contacts at odd positions from zero; then sector 1 is added to
+    // sector 0, and finally the contribution of segment i-1 (computed the same
+    // way by vertex i-1) is added.  The summation order below is that order.
+    struct SegSums {
+        V3<T> f0, f1, f2a, f2b;  // own sector 0 / sector 1; upper vertex sector 0 / 1
+        Sym3<T> H0, H1, H2a, H2b;
+        T limit;
+    };
+
+    // One contact of segment i with segment c (cosserat.cu:171-227).
+    void contact_terms(int i, int c, T damp, V3<T>& f, Sym3<T>& H, V3<T>& f2, Sym3<T>& H2) const {
+        const T invb = T(1) / T(p.barrier_thickness);
+        const T radius = T(2) * T(p.radius);
+        const T kCol = T(p.k_collision) * invb;
+        const V3<T> xi = x[i], xj = x[i + 1], dxi = dx[i], dxj = dx[i + 1];
+        const V3<T> b0 = x[c], b1 = x[c + 1], db0 = dx[c], db1 = dx[c + 1];
+        T ux, uy;
+        segment_closest_points(V3<T>{0, 0, 0}, (xj - xi) + (dxj - dxi), (b0 - xi) + (db0 - dxi), (b1 - xi) + (db1 - dxi),
+                               ux, uy);
+        const V3<T> dpos = mix(xi, xj, ux) - mix(b0, b1, uy);
+        const V3<T> ddpos = mix(dxi, dxj, ux) - mix(db0, db1, uy);
+        V3<T> nrm = dpos + ddpos;
+        T d = length(nrm);
+        nrm = nrm * (T(1) / d);
+        const T wj = ux;
+        const T wi = T(1) - ux;
+        d = d - radius;
+        d = d * invb;
+        if (d > 1) return;
+        d = std::max(d, T(1e-3f));
+        const T invd = T(1) / d;
+        const T logd = det_logf(d);
+        T dH = (T(-3) + (T(2) + invd) * invd - T(2) * logd) * kCol * invb;
+        const T ff = -(T(1) - d) * (d - T(1) + T(2) * d * logd) * invd * kCol;
+        f = f + (ff * wi - damp * dH * wi * wi * dot(nrm, dxi)) * nrm;
+        f2 = f2 + (ff * wj - damp * dH * wj * wj * dot(nrm, dxj)) * nrm;
+        dH = dH * (T(1) + damp);
+        Sym3<T> op = sym_outer(nrm);
+        sym_add(H, sym_scale(op, dH * wi * wi));
+        sym_add(H2, sym_scale(op, dH * wj * wj));
+        const V3<T> u = ddpos - dot(nrm, ddpos) * nrm;
+        const T ul = length(u);
+        if (ul > 0) {
+            const T cand = T(p.friction_coeff) * ff / ul;
+            const T f1 = cand < T(p.k_friction) ? cand : T(p.k_friction);  // glm::min(fricK, cand)
+            op.d[0] -= 1;
+            op.d[1] -= 1;
+            op.d[2] -= 1;
+            f = f - (f1 * wi) * u;
+            sym_sub(H, sym_scale(op, (wi * wi) * f1));
+            f2 = f2 - (f1 * wj) * u;
+            sym_sub(H2, sym_scale(op, (wj * wj) * f1));
+        }
+    }
+
+    // Everything segment i (between vertices i and i+1) adds, sector by sector.
+    // f0/H0 enter holding vertex i's inertia terms.
+    void segment_sums(int i, T damp, SegSums& s) const {
+        s.f1 = V3<T>{0, 0, 0};
+        s.f2a = V3<T>{0, 0, 0};
+        s.f2b = V3<T>{0, 0, 0};
+        s.H1 = Sym3<T>{{0, 0, 0, 0, 0, 0}};
+        s.H2a = Sym3<T>{{0, 0, 0, 0, 0, 0}};
+        s.H2b = Sym3<T>{{0, 0, 0, 0, 0, 0}};
+        s.limit = std::numeric_limits<T>::infinity();
+        if (!(flags[i] & F_NEXT)) return;
+        s.limit = maxStep[i];
         const T invl = T(1) / lRest[i];
-        V3<T> c = ((x[i + 1] - x[i]) + (dx[i + 1] - dx[i])) * invl - rotate(q[i], V3<T>{1, 0, 0});
-        T k = kStretch[i] * invl;
+        const V3<T> c = ((x[i + 1] - x[i]) + (dx[i + 1] - dx[i])) * invl - rotate(q[i], V3<T>{1, 0, 0});
+        const T k = kStretch[i] * invl;
         T d = k * invl;
-        fi = k * c - (damp * d) * dx[i];
-        fj = (-k) * c - (damp * d) * dx[i + 1];
-        di = d * (1 + damp);
+        s.f0 = s.f0 + (k * c - (damp * d) * dx[i]);
+        s.f2a = s.f2a + ((-k) * c - (damp * d) * dx[i + 1]);
+        d = d * (1 + damp);
+        s.H0.d[0] += d;
+        s.H0.d[1] += d;
+        s.H0.d[2] += d;
+        s.H2a.d[0] += d;
+        s.H2a.d[1] += d;
+        s.H2a.d[2] += d;
+        const std::vector<int>& L = cols[i];
+        for (size_t k2 = 0; k2 < L.size(); k2 += 2) contact_terms(i, L[k2], damp, s.f0, s.H0, s.f2a, s.H2a);
+        for (size_t k2 = 1; k2 < L.size(); k2 += 2) contact_terms(i, L[k2], damp, s.f1, s.H1, s.f2b, s.H2b);
     }
 
     V3<T> position_solve(int i) const {
@@ -500,28 +637,125 @@ struct Sim {
         const T damp = T(p.damping) / hh;
         const V3<T> dxi = dx[i];
         const T a_in = T(1) / (invMass[i] * hh * hh);
-        Sym3<T> H{{a_in, a_in, a_in, a_in, a_in, a_in}};
-        V3<T> f = (T(1) / (hh * hh * invMass[i])) * (y[i] - dxi);
-        if (flags[i] & F_NEXT) {
-            V3<T> fi, fj;
-            T di;
-            segment_terms(i, damp, fi, di, fj);
-            f = f + fi;
-            H.d[0] += di;
-            H.d[1] += di;
-            H.d[2] += di;
-        }
+        SegSums s;
+        s.H0 = Sym3<T>{{a_in, a_in, a_in, a_in, a_in, a_in}};
+        s.f0 = (T(1) / (hh * hh * invMass[i])) * (y[i] - dxi);
+        segment_sums(i, damp, s);
+        V3<T> f = s.f0 + s.f1;
+        Sym3<T> H = s.H0;
+        sym_add(H, s.H1);
+        T limit = s.limit;
         if (flags[i] & F_PREV) {
-            V3<T> fi, fj;
-            T dj;
-            segment_terms(i - 1, damp, fi, dj, fj);
-            f = f + fj;
-            H.d[0] += dj;
-            H.d[1] += dj;
-            H.d[2] += dj;
+            SegSums t;
+            t.f0 = V3<T>{0, 0, 0};
+            t.H0 = Sym3<T>{{0, 0, 0, 0, 0, 0}};
+            segment_sums(i - 1, damp, t);
+            const V3<T> f2 = t.f2a + t.f2b;
+            Sym3<T> H2 = t.H2a;
+            sym_add(H2, t.H2b);
+            limit = std::min(limit, t.limit);
+            f = f + f2;
+            sym_add(H, H2);
         }
-        V3<T> delta = T(p.acceleration_ratio) * sym_solve(H, f);
-        return dxi + delta;  // step limit is +inf without hair-hair contacts
+        V3<T> out = dxi + T(p.acceleration_ratio) * sym_solve(H, f);
+        const T l = length(out);
+        if (l > limit && l > 0) out = out * (limit / l);
+        return out;
+    }
+
+    // ------------------------------------------------------------ contacts
+    // buildAABBs<USE_VEL_RADIUS> (YB/sim/collision.cu:7-34), in float.
+    std::vector<Aabb> segment_aabbs() const {
+        const float detR = p.radius + 0.5f * p.barrier_thickness;
+        const float scaled = detR * p.detection_scaler;
+        std::vector<Aabb> out(n, aabb_empty());
+        for (int i = 0; i < n; i++) {
+            if (!(flags[i] & F_NEXT)) continue;
+            const float p0[3] = {(float)x[i].x, (float)x[i].y, (float)x[i].z};
+            const float p1[3] = {(float)x[i + 1].x, (float)x[i + 1].y, (float)x[i + 1].z};
+            Aabb a = aabb_empty();
+            aabb_absorb(a, p0);
+            aabb_absorb(a, p1);
+            float pad = scaled;
+            if (p.use_velocity_radius) {
+                const V3<float> d0{(float)dx[i].x, (float)dx[i].y, (float)dx[i].z};
+                const V3<float> d1{(float)dx[i + 1].x, (float)dx[i + 1].y, (float)dx[i + 1].z};
+                pad += std::sqrt(std::max(length2(d0), length2(d1)));
+            }
+            for (int k = 0; k < 3; k++) {
+                a.mn[k] -= pad;
+                a.mx[k] += pad;
+            }
+            out[i] = a;
+        }
+        return out;
+    }
+
+    // Sim::detectCollisions (YBSimulator.cu:424-457) with the reference's two
+    // bugs fixed (bvh.reset() nulling the tree; the pair buffer sized in ints)
+    // and deterministic, ascending per-segment contact lists.
+    void detect_collisions() {
+        const std::vector<Aabb> boxes = segment_aabbs();
+        if (lastBVHRebuild >= p.bvh_rebuild_period) {
+            bvh.build(boxes);
+            lastBVHRebuild = 0;
+        } else {
+            bvh.refit(boxes);
+            lastBVHRebuild += hF * p.detection_period;
+        }
+        const auto pairs = bvh.query(boxes);
+        num_pairs = (long)pairs.size();
+        const float detR = p.radius + 0.5f * p.barrier_thickness;
+        const T scaled = T(detR * p.detection_scaler);
+        for (auto& L : cols) L.clear();
+        for (const auto& pr : pairs) {
+            const int a = pr.first, b = pr.second;
+            if (std::abs(b - a) <= 2) continue;  // neighbouring segments
+            const V3<T> a0 = x[a];
+            const V3<T> a1 = x[a + 1] - a0;
+            const V3<T> b0 = x[b] - a0;
+            const V3<T> b1 = x[b + 1] - a0;
+            T u, v;
+            segment_closest_points(V3<T>{0, 0, 0}, a1, b0, b1, u, v);
+            const V3<T> nrm = u * a1 - mix(b0, b1, v);
+            const T d2 = length2(nrm);
+            T r = T(2) * scaled;
+            if (p.use_velocity_radius) {
+                r += std::sqrt(std::max(length2(dx[a]), length2(dx[a + 1])));
+                r += std::sqrt(std::max(length2(dx[b]), length2(dx[b + 1])));
+            }
+            if (d2 < r * r) {
+                if ((int)cols[a].size() >= kMaxCols || (int)cols[b].size() >= kMaxCols) error = 8;
+                if ((int)cols[a].size() < kMaxCols) cols[a].push_back(b);
+                if ((int)cols[b].size() < kMaxCols) cols[b].push_back(a);
+            }
+        }
+        for (auto& L : cols) std::sort(L.begin(), L.end());
+    }
+
+    // recomputeStepLimitKernel<LIMIT, USE_VEL_RADIUS> (YB/sim/collision.cu:94-131)
+    void recompute_step_limit() {
+        const T inf = std::numeric_limits<T>::infinity();
+        const T detR = T(p.radius + 0.5f * p.barrier_thickness);
+        const T margin = T((1 - 0.1f) * 0.5f);
+        for (int i = 0; i < n; i++) {
+            T m = inf;
+            if (p.use_step_size_limit && (flags[i] & F_NEXT)) {
+                const V3<T> a0 = x[i];
+                const V3<T> a1 = x[i + 1] - a0;
+                m = detR * (T(p.detection_scaler) - 1);
+                if (p.use_velocity_radius) m += length(dx[i]);
+                for (int c : cols[i]) {
+                    const V3<T> b0 = x[c] - a0;
+                    const V3<T> b1 = x[c + 1] - a0;
+                    T u, v;
+                    segment_closest_points(V3<T>{0, 0, 0}, a1, b0, b1, u, v);
+                    const V3<T> nrm = u * a1 - mix(b0, b1, v);
+                    m = std::min(m, margin * length(nrm));
+                }
+            }
+            maxStep[i] = m;
+        }
     }
 
     // Orientation solve of vertex i (YB/sim/cosserat.cu:290-337), Jacobi: reads
@@ -583,6 +817,9 @@ struct Sim {
         h = (T)hF;
         for (int s = 0; s < steps; s++, step_counter++) {
             init_itr();
+            if (p.enable_hair_hair_coll && p.detection_period > 0 && step_counter % p.detection_period == 0)
+                detect_collisions();
+            recompute_step_limit();
             for (int k = 0; k < p.num_itr; k++) iterate();
             end_itr();
         }
