@@ -182,6 +182,31 @@ int kscr_batch_synchronize(kscr_batch* b);
 int kscr_batch_block_size(const kscr_batch* b);
 int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n);
 
+/* Contact lists of the last collision detection (hair-hair contacts only):
+ * num[num_verts] contacts per segment and cols[16*num_verts] partner segment
+ * ids, ascending, -1 in unused slots (YB/YarnBall.h:74-76, :19). */
+int kscr_get_contacts(kscr_handle* h, int* num, int* cols);
+
+/* ---- Linear BVH (Kitten::LBVH, KE/KittenGpuLBVH/lbvh.cuh:14-83).  Boxes are
+ * float[6*n] (min xyz, max xyz); an empty box (+inf, -inf) is ignored. */
+typedef struct kscr_lbvh kscr_lbvh;
+int kscr_lbvh_create(int device, kscr_lbvh** out);
+void kscr_lbvh_destroy(kscr_lbvh* b);
+/* LBVH::compute (lbvh.cu:357-394): root box, Morton codes, stable sort,
+ * Karras build, refit. */
+int kscr_lbvh_compute(kscr_lbvh* b, const float* aabbs, int n);
+/* LBVH::refit (lbvh.cu:348-356) with new boxes for the same objects. */
+int kscr_lbvh_refit(kscr_lbvh* b, const float* aabbs);
+/* LBVH::query (lbvh.cu:411-426): unique overlapping pairs as int[2*k]
+ * (found object, query object); *count receives the number found, which may
+ * exceed cap (only cap pairs are written). */
+int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count);
+/* Tree contents: sorted Morton codes and object ids (n), internal nodes as
+ * (parent, left, right, fence) (4 per node) and child boxes (12 per node),
+ * leaf parents (n), root box (6), DFS stack bound. */
+int kscr_lbvh_get(kscr_lbvh* b, uint32_t* codes, uint32_t* ids, uint32_t* links, float* bounds,
+                  uint32_t* leaf_parent, float* root, int* max_stack);
+
 /* Copies the last error message (NUL terminated) into buf. */
 int kscr_last_error(const kscr_handle* h, char* buf, size_t n);
 /* Library-wide message of the last kscr_create failure. */

@@ -110,6 +110,13 @@ SIGNATURES = {
     "kscr_batch_synchronize": (C.c_int, [_P]),
     "kscr_batch_block_size": (C.c_int, [_P]),
     "kscr_batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    "kscr_get_contacts": (C.c_int, [_P, _IP, _IP]),
+    "kscr_lbvh_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "kscr_lbvh_destroy": (None, [_P]),
+    "kscr_lbvh_compute": (C.c_int, [_P, _FP, C.c_int]),
+    "kscr_lbvh_refit": (C.c_int, [_P, _FP]),
+    "kscr_lbvh_query": (C.c_int, [_P, _IP, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "kscr_lbvh_get": (C.c_int, [_P, _UP, _UP, _UP, _FP, _UP, _FP, _IP]),
 }
 
 _lib = None
@@ -290,6 +297,12 @@ class Sim:
     def block_size(self):
         return self.lib.kscr_block_size(self.h)
 
+    def get_contacts(self):
+        num = np.empty(self.n, np.int32)
+        cols = np.empty((self.n, 16), np.int32)
+        self._check(self.lib.kscr_get_contacts(self.h, iptr(num), iptr(cols)))
+        return num, cols
+
     def autotune(self, dt, steps=3):
         best = C.c_int(0)
         self._check(self.lib.kscr_autotune(self.h, dt, steps, C.byref(best)))
@@ -331,6 +344,68 @@ class Batch:
     def close(self):
         if getattr(self, "b", None):
             self.lib.kscr_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Lbvh:
+    """Python view of a ``kscr_lbvh`` (the reference's Kitten::LBVH interface)."""
+
+    def __init__(self, device=-1):
+        self.lib = load()
+        b = C.c_void_p()
+        rc = self.lib.kscr_lbvh_create(device, C.byref(b))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_create failed")
+        self.b = b
+        self.n = 0
+
+    def compute(self, boxes):
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        self.n = b.shape[0]
+        rc = self.lib.kscr_lbvh_compute(self.b, fptr(b), self.n)
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_compute failed")
+
+    def refit(self, boxes):
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        rc = self.lib.kscr_lbvh_refit(self.b, fptr(b))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_refit failed")
+
+    def query(self, cap=None):
+        cap = cap if cap is not None else 64 * self.n
+        out = np.empty((cap, 2), np.int32)
+        cnt = C.c_size_t(0)
+        rc = self.lib.kscr_lbvh_query(self.b, iptr(out), cap, C.byref(cnt))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_query failed")
+        return out[:min(cnt.value, cap)].copy(), cnt.value
+
+    def get(self):
+        n = self.n
+        codes = np.empty(n, np.uint32)
+        ids = np.empty(n, np.uint32)
+        links = np.empty((max(n - 1, 1), 4), np.uint32)
+        bounds = np.empty((max(n - 1, 1), 12), np.float32)
+        lp = np.empty(n, np.uint32)
+        root = np.empty(6, np.float32)
+        ms = C.c_int(0)
+        u = lambda a: a.ctypes.data_as(_UP)
+        rc = self.lib.kscr_lbvh_get(self.b, u(codes), u(ids), u(links), fptr(bounds), u(lp), fptr(root), C.byref(ms))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_get failed")
+        return dict(codes=codes, ids=ids, links=links[:n - 1], bounds=bounds[:n - 1], leaf_parent=lp, root=root,
+                    max_stack=ms.value)
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.kscr_lbvh_destroy(self.b)
             self.b = None
 
     def __del__(self):

@@ -22,7 +22,9 @@
 #include <vector>
 
 #include "../../include/kscr.h"
+#include "kscr_contact.h"
 #include "kscr_device.h"
+#include "kscr_lbvh.h"
 #include "kscr_math.h"
 
 using namespace kscr;
@@ -81,7 +83,11 @@ struct kscr_handle {
     // SDF
     bool sdf_set = false, sdf_on = false;
     SdfArgs sdf{};
+    // hair-hair contacts
+    ContactDev contact;
+    float last_bvh_rebuild = INFINITY;
     std::string err;
+    ~kscr_handle() { contact_release(contact); }
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -104,8 +110,6 @@ int validate_meta(kscr_handle* h) {
                        "barrier_thickness)");
     if (h->p.num_itr < 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr must be >= 0");
     if (!(h->p.max_h > 0)) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "max_h must be > 0");
-    if (h->p.enable_hair_hair_coll)
-        return h->fail(KSCR_ERR_UNSUPPORTED, "hair-hair collision requires the contact step path (not in this library build)");
     if (h->p.schedule != KSCR_SCHEDULE_JACOBI)
         return h->fail(KSCR_ERR_UNSUPPORTED, "only the Jacobi schedule is available in this library build");
     return KSCR_OK;
@@ -221,6 +225,68 @@ StepArgs step_args(kscr_handle* h, float dt, bool with_sdf) {
     return a;
 }
 
+// Step with hair-hair contacts: the reference's per-substep sequence
+// (YB/sim/step.cpp:49-57) as separate kernels, see kscr_contact.hip.
+int launch_contact_step(kscr_handle* h, float dt, bool with_sdf) {
+    hipError_t e;
+    if ((e = contact_reserve(h->contact, h->n)) != hipSuccess) return h->hip_fail(e, "contact buffers");
+    const long counter0 = h->step_counter;
+    const StepArgs a = step_args(h, dt, with_sdf);
+    ContactParams cp{};
+    cp.radius = h->p.radius;
+    cp.barrier = h->p.barrier_thickness;
+    cp.k_collision = h->p.k_collision;
+    cp.k_friction = h->p.k_friction;
+    cp.friction_coeff = h->p.friction_coeff;
+    cp.detection_radius = h->p.radius + 0.5f * h->p.barrier_thickness;  // uploadMeta (YBSimulator.cu:323-324)
+    cp.scaled_detection_radius = cp.detection_radius * h->p.detection_scaler;
+    cp.detection_scaler = h->p.detection_scaler;
+    cp.use_step_size_limit = h->p.use_step_size_limit;
+    cp.use_velocity_radius = h->p.use_velocity_radius;
+    ContactDev& c = h->contact;
+    for (int s = 0; s < a.num_substeps; s++) {
+        if ((e = contact_init(a, c, h->stream)) != hipSuccess) return h->hip_fail(e, "contact init");
+        if (h->p.enable_hair_hair_coll && h->p.detection_period > 0 && (counter0 + s) % h->p.detection_period == 0) {
+            const bool rebuild = h->last_bvh_rebuild >= h->p.bvh_rebuild_period;
+            if ((e = contact_detect(a, cp, c, rebuild, h->stream)) != hipSuccess)
+                return h->hip_fail(e, "collision detection");
+            if (rebuild)
+                h->last_bvh_rebuild = 0;
+            else
+                h->last_bvh_rebuild += h->h * h->p.detection_period;
+        }
+        if ((e = contact_step_limit(a, cp, c, h->stream)) != hipSuccess) return h->hip_fail(e, "step limit");
+        int cur = 0;
+        for (int k = 0; k < a.num_itr; k++) {
+            if ((e = contact_iterate(a, cp, c, cur, h->d_q.p, c.q2, h->stream)) != hipSuccess)
+                return h->hip_fail(e, "contact iteration");
+            std::swap(h->d_q.p, c.q2);  // the new orientations become the current ones
+            cur ^= 1;
+        }
+        if ((e = contact_end(a, c, cur, h->stream)) != hipSuccess) return h->hip_fail(e, "contact end");
+    }
+    StepArgs a2 = a;
+    a2.q = h->d_q.p;
+    if (a.sdf_on && (e = sdf_pass(a2, h->n, h->stream)) != hipSuccess) return h->hip_fail(e, "SDF pass");
+    return KSCR_OK;
+}
+
+// Reads and clears the device error word of the contact path.
+int check_device_error(kscr_handle* h) {
+    if (!h->contact.error) return KSCR_OK;
+    int word = 0;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(&word, h->contact.error, sizeof(int), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return h->hip_fail(e, "read device error word");
+    if (!word) return KSCR_OK;
+    if ((e = fill_i32(h->contact.error, 0, 1, h->stream)) != hipSuccess) return h->hip_fail(e, "reset error word");
+    if (word & DEV_ERR_MAX_COLLISIONS)
+        return h->fail(KSCR_ERR_MAX_COLLISIONS, "MAX_COLLISIONS_PER_SEGMENT exceeded. Current simulation state may be corrupted!");
+    if (word & DEV_ERR_PAIR_OVERFLOW) return h->fail(KSCR_ERR_MAX_COLLISIONS, "broad-phase pair buffer overflow");
+    return h->fail(KSCR_ERR_MAX_COLLISIONS, "LBVH query deeper than its stack");
+}
+
 int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
     if (!(dt > 0)) return KSCR_OK;  // Sim::advance returns for h <= 0
@@ -228,6 +294,7 @@ int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     if (rc) return rc;
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit) return launch_contact_step(h, dt, with_sdf);
     const StepArgs a = step_args(h, dt, with_sdf);
     e = launch_step_fused(a, h->d_packs.p, h->num_packs, h->block, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
@@ -515,7 +582,7 @@ int kscr_get_pos_vel(kscr_handle* h, float* positions, float* velocities) {
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
     if (positions) std::memcpy(positions, h->pos.data(), n3 * sizeof(float));
     if (velocities) std::memcpy(velocities, h->vel.data(), n3 * sizeof(float));
-    return KSCR_OK;
+    return check_device_error(h);
 }
 
 int kscr_get_q(kscr_handle* h, float* q) {
@@ -558,6 +625,28 @@ int kscr_get_rest(const kscr_handle* h, float* inv_mass, float* l_rest, float* k
             q_rest[4 * i + 3] = h->q_rest[i].w;
         }
     }
+    return KSCR_OK;
+}
+
+int kscr_get_contacts(kscr_handle* h, int* num, int* cols) {
+    if (!h || !num || !cols) return KSCR_ERR_INVALID_ARGUMENT;
+    const int n = h->n;
+    if (!h->contact.num_cols) {
+        for (int i = 0; i < n; i++) {
+            num[i] = 0;
+            for (int k = 0; k < kMaxCols; k++) cols[kMaxCols * i + k] = -1;
+        }
+        return KSCR_OK;
+    }
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = hipMemcpyAsync(num, h->contact.num_cols, n * sizeof(int), hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(cols, h->contact.cols, (size_t)kMaxCols * n * sizeof(int), hipMemcpyDeviceToHost,
+                            h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync contacts");
+    for (int i = 0; i < n; i++)
+        for (int k = num[i]; k < kMaxCols; k++) cols[kMaxCols * i + k] = -1;
     return KSCR_OK;
 }
 
@@ -621,7 +710,7 @@ int kscr_synchronize(kscr_handle* h) {
     if (!h) return KSCR_ERR_INVALID_ARGUMENT;
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
-    return KSCR_OK;
+    return check_device_error(h);
 }
 
 int kscr_num_verts(const kscr_handle* h) { return h ? h->n : -1; }
@@ -744,6 +833,10 @@ int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_si
         if (!h || !h->uploaded) {
             delete b;
             return KSCR_ERR_NOT_CONFIGURED;
+        }
+        if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit) {
+            delete b;
+            return KSCR_ERR_UNSUPPORTED;  // the batched kernel is the contact-free fused step
         }
         if (g > 0 && h->device != handles[0]->device) {
             delete b;
@@ -891,6 +984,122 @@ int kscr_batch_block_size(const kscr_batch* b) { return b ? b->block : -1; }
 int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n) {
     if (!b || !buf || n == 0) return KSCR_ERR_INVALID_ARGUMENT;
     std::snprintf(buf, n, "%s", b->err.c_str());
+    return KSCR_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ LBVH
+struct kscr_lbvh {
+    int device = 0;
+    LbvhDev d;
+    int2* pairs = nullptr;
+    int* error = nullptr;
+    size_t pair_cap = 0;
+    ~kscr_lbvh() {
+        lbvh_release(d);
+        if (pairs) (void)hipFree(pairs);
+        if (error) (void)hipFree(error);
+    }
+};
+
+extern "C" {
+
+int kscr_lbvh_create(int device, kscr_lbvh** out) {
+    if (!out) return KSCR_ERR_INVALID_ARGUMENT;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return KSCR_ERR_HIP;
+    if (device < 0) device = count - 1;
+    if (device >= count) return KSCR_ERR_INVALID_ARGUMENT;
+    auto* b = new kscr_lbvh;
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipMalloc((void**)&b->error, sizeof(int)) != hipSuccess) {
+        delete b;
+        return KSCR_ERR_HIP;
+    }
+    *out = b;
+    return KSCR_OK;
+}
+
+void kscr_lbvh_destroy(kscr_lbvh* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    delete b;
+}
+
+static int lbvh_upload(kscr_lbvh* b, const float* aabbs, int n) {
+    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
+    if (lbvh_reserve(b->d, n) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
+    if (hipMemcpy(b->d.aabb, aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return KSCR_ERR_HIP;
+    return KSCR_OK;
+}
+
+int kscr_lbvh_compute(kscr_lbvh* b, const float* aabbs, int n) {
+    if (!b || !aabbs || n <= 0) return KSCR_ERR_INVALID_ARGUMENT;
+    int rc = lbvh_upload(b, aabbs, n);
+    if (rc) return rc;
+    if (lbvh_build(b->d, n, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KSCR_ERR_HIP;
+    return KSCR_OK;
+}
+
+int kscr_lbvh_refit(kscr_lbvh* b, const float* aabbs) {
+    if (!b || !aabbs || !b->d.built) return KSCR_ERR_INVALID_ARGUMENT;
+    int rc = lbvh_upload(b, aabbs, b->d.n);
+    if (rc) return rc;
+    if (lbvh_refit(b->d, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KSCR_ERR_HIP;
+    return KSCR_OK;
+}
+
+int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count) {
+    if (!b || !count || !b->d.built || (cap > 0 && !pairs)) return KSCR_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
+    if (cap > b->pair_cap) {
+        if (b->pairs) (void)hipFree(b->pairs);
+        b->pairs = nullptr;
+        if (hipMalloc((void**)&b->pairs, cap * sizeof(int2)) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
+        b->pair_cap = cap;
+    }
+    if (fill_i32(b->error, 0, 1, nullptr) != hipSuccess) return KSCR_ERR_HIP;
+    if (lbvh_query(b->d, b->pairs, (int)cap, -1, b->error, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return KSCR_ERR_HIP;
+    int found = 0, err = 0;
+    if (hipMemcpy(&found, b->d.count, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&err, b->error, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return KSCR_ERR_HIP;
+    const size_t m = std::min((size_t)found, cap);
+    if (m && hipMemcpy(pairs, b->pairs, m * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    *count = (size_t)found;
+    return err ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
+}
+
+int kscr_lbvh_get(kscr_lbvh* b, uint32_t* codes, uint32_t* ids, uint32_t* links, float* bounds, uint32_t* leaf_parent,
+                  float* root, int* max_stack) {
+    if (!b || !b->d.built) return KSCR_ERR_INVALID_ARGUMENT;
+    const int n = b->d.n;
+    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
+    if (codes && hipMemcpy(codes, b->d.codes, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    if (ids && hipMemcpy(ids, b->d.ids, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    if (leaf_parent && hipMemcpy(leaf_parent, b->d.leaf_parent, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return KSCR_ERR_HIP;
+    if (root && hipMemcpy(root, b->d.root, 6 * 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    if (max_stack && hipMemcpy(max_stack, b->d.max_depth, 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    if ((links || bounds) && n > 1) {
+        std::vector<LbvhNode> nodes(n - 1);
+        if (hipMemcpy(nodes.data(), b->d.nodes, (size_t)(n - 1) * sizeof(LbvhNode), hipMemcpyDeviceToHost) != hipSuccess)
+            return KSCR_ERR_HIP;
+        for (int i = 0; i < n - 1; i++) {
+            if (links) {
+                links[4 * i] = nodes[i].parent;
+                links[4 * i + 1] = nodes[i].left;
+                links[4 * i + 2] = nodes[i].right;
+                links[4 * i + 3] = nodes[i].fence;
+            }
+            if (bounds)
+                for (int k = 0; k < 12; k++) bounds[12 * i + k] = nodes[i].b[k / 6][k % 6];
+        }
+    }
     return KSCR_OK;
 }
 

@@ -1,0 +1,59 @@
+// kscr_contact.h -- the step with hair-hair contacts (per-substep kernels).
+//
+// With contacts a vertex's solve reads the dx of partner segments of other
+// strands every iteration, so the step cannot stay inside one workgroup: it
+// runs as a sequence of kernels per substep (the reference's own structure,
+// YB/sim/step.cpp:9-62), with the Jacobi snapshot kept by ping-pong buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kscr_device.h"
+#include "kscr_lbvh.h"
+
+namespace kscr {
+
+constexpr int kMaxCols = 16;  // MAX_COLLISIONS_PER_SEGMENT (YarnBall.h:19)
+
+enum : int { DEV_ERR_NONE = 0, DEV_ERR_MAX_COLLISIONS = 1, DEV_ERR_PAIR_OVERFLOW = 2, DEV_ERR_QUERY_STACK = 4 };
+
+struct ContactDev {
+    int n = 0;
+    float* dx[2] = {nullptr, nullptr};  // float[3n] ping-pong iterates
+    float* y = nullptr;                 // float[3n] inertial target (as a delta)
+    float4* q2 = nullptr;               // ping-pong partner of the handle's q
+    float* max_step = nullptr;          // float[n]
+    int* num_cols = nullptr;            // int[n]
+    int* cols = nullptr;                // int[16n], 16 slots per segment, sorted
+    int2* pairs = nullptr;
+    int pair_cap = 0;
+    int* error = nullptr;  // device error word (DEV_ERR_* bits)
+    LbvhDev bvh;
+};
+
+hipError_t contact_reserve(ContactDev& c, int n);
+void contact_release(ContactDev& c);
+
+struct ContactParams {
+    float radius, barrier, k_collision, k_friction, friction_coeff;
+    float detection_radius, scaled_detection_radius, detection_scaler;
+    int use_step_size_limit, use_velocity_radius;
+};
+
+// initItr into c.dx[0] and c.y (and lastVel update).
+hipError_t contact_init(const StepArgs& a, ContactDev& c, hipStream_t s);
+// detectCollisions: AABBs, LBVH build or refit, query, narrow phase, sorted lists.
+hipError_t contact_detect(const StepArgs& a, const ContactParams& cp, ContactDev& c, bool rebuild, hipStream_t s);
+// recomputeStepLimitKernel
+hipError_t contact_step_limit(const StepArgs& a, const ContactParams& cp, ContactDev& c, hipStream_t s);
+// one Jacobi iteration: dx[cur] -> dx[cur^1], q_in -> q_out
+hipError_t contact_iterate(const StepArgs& a, const ContactParams& cp, ContactDev& c, int cur, const float4* q_in,
+                           float4* q_out, hipStream_t s);
+// endItr from dx[cur]
+hipError_t contact_end(const StepArgs& a, ContactDev& c, int cur, hipStream_t s);
+// SDF pass over all non-root vertices
+hipError_t sdf_pass(const StepArgs& a, int n, hipStream_t s);
+
+}  // namespace kscr

@@ -1,0 +1,401 @@
+// kscr_contact.hip -- Stable Cosserat Rods step with hair-hair contacts.
+//
+// Per substep (YB/sim/step.cpp:49-57 and the captured graph of :17-30):
+//   contact_init_kernel        initItr                     (iteration.cu:8-59)
+//   detection (when due)       buildAABBs, LBVH, query,   (YBSimulator.cu:424-457,
+//                              buildCollisionList          collision.cu:7-92)
+//   step_limit_kernel          recomputeStepLimitKernel    (collision.cu:94-131)
+//   numItr x position_kernel   cosseratItr with IPC barrier and friction
+//            orientation_kernel quaternionLambdaItr        (cosserat.cu:25-337)
+//   end_kernel                 endItr                      (iteration.cu:62-74)
+// The reference's races are replaced by ping-pong buffers (Jacobi snapshot)
+// and its atomically ordered contact lists by ascending lists, so results are
+// deterministic and equal to the CPU restatement's.
+#include "kscr_contact.h"
+#include "kscr_math.h"
+#include "kscr_solve.h"
+
+namespace kscr {
+namespace {
+
+constexpr int kThreads = 256;
+
+inline dim3 grid_for(int n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+__global__ void contact_init_kernel(StepArgs a, int n, float* __restrict__ dx0, float* __restrict__ yout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 x = ld3(a.x, i), v = ld3(a.v, i);
+    f3 u = ld3(a.u, i);
+    const float im = a.mat[i].x;
+    f3 y, dx;
+    init_vertex(a, im != 0.f, x, v, u, y, dx);
+    st3(a.u, i, u);
+    st3(dx0, i, dx);
+    st3(yout, i, y);
+}
+
+// buildAABBs<USE_VEL_RADIUS> (collision.cu:7-34): padded box of segment i
+__global__ void aabb_kernel(StepArgs a, ContactParams cp, int n, const float* __restrict__ dx, float* __restrict__ box) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t fl = __float_as_uint(a.mat[i].w);
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (fl & FLAG_NEXT) {
+        const f3 p0 = ld3(a.x, i), p1 = ld3(a.x, i + 1);
+        const float P0[3] = {p0.x, p0.y, p0.z}, P1[3] = {p1.x, p1.y, p1.z};
+        for (int k = 0; k < 3; k++) {
+            b[k] = gmin(gmin(b[k], P0[k]), P1[k]);
+            b[3 + k] = gmax(gmax(b[3 + k], P0[k]), P1[k]);
+        }
+        float pad = cp.scaled_detection_radius;
+        if (cp.use_velocity_radius) {
+            const f3 d0 = ld3(dx, i), d1 = ld3(dx, i + 1);
+            pad += sqrtf(gmax(dot(d0, d0), dot(d1, d1)));
+        }
+        for (int k = 0; k < 3; k++) {
+            b[k] -= pad;
+            b[3 + k] += pad;
+        }
+    }
+    for (int k = 0; k < 6; k++) box[6 * (size_t)i + k] = b[k];
+}
+
+// buildCollisionList<USE_VEL_RADIUS> (collision.cu:36-92) over a device-sized
+// pair list; grid-stride so no host read of the pair count is needed.
+__global__ void narrow_kernel(StepArgs a, ContactParams cp, const int2* __restrict__ pairs, const int* __restrict__ count,
+                              int cap, const float* __restrict__ dx, int* __restrict__ num_cols, int* __restrict__ cols,
+                              int* __restrict__ error) {
+    const int total = *count;
+    if (total > cap && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(error, DEV_ERR_PAIR_OVERFLOW);
+    const int m = total < cap ? total : cap;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const int2 ids = pairs[k];
+        const int ia = ids.x, ib = ids.y;
+        if (abs(ib - ia) <= 2) continue;  // neighbouring segments
+        const f3 a0 = ld3(a.x, ia);
+        const f3 a1 = ld3(a.x, ia + 1) - a0;
+        const f3 b0 = ld3(a.x, ib) - a0;
+        const f3 b1 = ld3(a.x, ib + 1) - a0;
+        float u, v;
+        segment_closest(f3{0, 0, 0}, a1, b0, b1, u, v);
+        const f3 nrm = u * a1 - mix3(b0, b1, v);
+        const float d2 = dot(nrm, nrm);
+        float r = 2 * cp.scaled_detection_radius;
+        if (cp.use_velocity_radius) {
+            const f3 da0 = ld3(dx, ia), da1 = ld3(dx, ia + 1), db0 = ld3(dx, ib), db1 = ld3(dx, ib + 1);
+            r += sqrtf(gmax(dot(da0, da0), dot(da1, da1)));
+            r += sqrtf(gmax(dot(db0, db0), dot(db1, db1)));
+        }
+        if (d2 < r * r) {
+            int slot = atomicAdd(&num_cols[ia], 1);
+            if (slot >= kMaxCols)
+                atomicOr(error, DEV_ERR_MAX_COLLISIONS);
+            else
+                cols[kMaxCols * (size_t)ia + slot] = ib;
+            slot = atomicAdd(&num_cols[ib], 1);
+            if (slot >= kMaxCols)
+                atomicOr(error, DEV_ERR_MAX_COLLISIONS);
+            else
+                cols[kMaxCols * (size_t)ib + slot] = ia;
+        }
+    }
+}
+
+// ascending contact list per segment (deterministic summation order)
+__global__ void sort_lists_kernel(int n, int* __restrict__ num_cols, int* __restrict__ cols) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int m = num_cols[i];
+    if (m > kMaxCols) {
+        m = kMaxCols;
+        num_cols[i] = m;
+    }
+    int v[kMaxCols];
+    for (int k = 0; k < kMaxCols; k++) v[k] = k < m ? cols[kMaxCols * (size_t)i + k] : 0;
+    for (int k = 1; k < m; k++) {
+        const int key = v[k];
+        int j = k - 1;
+        while (j >= 0 && v[j] > key) {
+            v[j + 1] = v[j];
+            j--;
+        }
+        v[j + 1] = key;
+    }
+    for (int k = 0; k < m; k++) cols[kMaxCols * (size_t)i + k] = v[k];
+}
+
+// recomputeStepLimitKernel<LIMIT, USE_VEL_RADIUS> (collision.cu:94-131)
+__global__ void step_limit_kernel(StepArgs a, ContactParams cp, int n, const float* __restrict__ dx,
+                                  const int* __restrict__ num_cols, const int* __restrict__ cols,
+                                  float* __restrict__ max_step) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float m = INFINITY;
+    const uint32_t fl = __float_as_uint(a.mat[i].w);
+    if (cp.use_step_size_limit && (fl & FLAG_NEXT)) {
+        const f3 a0 = ld3(a.x, i);
+        const f3 a1 = ld3(a.x, i + 1) - a0;
+        m = cp.detection_radius * (cp.detection_scaler - 1);
+        if (cp.use_velocity_radius) m += length(ld3(dx, i));
+        const float margin = (1 - 0.1f) * 0.5f;
+        const int nc = num_cols[i];
+        for (int k = 0; k < nc; k++) {
+            const int c = cols[kMaxCols * (size_t)i + k];
+            const f3 b0 = ld3(a.x, c) - a0;
+            const f3 b1 = ld3(a.x, c + 1) - a0;
+            float u, v;
+            segment_closest(f3{0, 0, 0}, a1, b0, b1, u, v);
+            const f3 nrm = u * a1 - mix3(b0, b1, v);
+            m = gmin(m, margin * length(nrm));
+        }
+    }
+    max_step[i] = m;
+}
+
+// One contact of segment i with segment c (cosserat.cu:171-227).
+__device__ inline void contact_terms(const StepArgs& a, const ContactParams& cp, const float* __restrict__ dx, int i,
+                                     int c, float damp, f3& f, s3& H, f3& f2, s3& H2) {
+    const float invb = 1.0f / cp.barrier;
+    const float radius = 2.0f * cp.radius;
+    const float kCol = cp.k_collision * invb;
+    const f3 xi = ld3(a.x, i), xj = ld3(a.x, i + 1), dxi = ld3(dx, i), dxj = ld3(dx, i + 1);
+    const f3 b0 = ld3(a.x, c), b1 = ld3(a.x, c + 1), db0 = ld3(dx, c), db1 = ld3(dx, c + 1);
+    float ux, uy;
+    segment_closest(f3{0, 0, 0}, (xj - xi) + (dxj - dxi), (b0 - xi) + (db0 - dxi), (b1 - xi) + (db1 - dxi), ux, uy);
+    const f3 dpos = mix3(xi, xj, ux) - mix3(b0, b1, uy);
+    const f3 ddpos = mix3(dxi, dxj, ux) - mix3(db0, db1, uy);
+    f3 nrm = dpos + ddpos;
+    float d = length(nrm);
+    nrm = nrm * (1.0f / d);
+    const float wj = ux;
+    const float wi = 1.0f - ux;
+    d = d - radius;
+    d = d * invb;
+    if (d > 1) return;
+    d = gmax(d, 1e-3f);
+    const float invd = 1.0f / d;
+    const float logd = det_logf(d);
+    float dH = (-3.0f + (2.0f + invd) * invd - 2.0f * logd) * kCol * invb;
+    const float ff = -(1.0f - d) * (d - 1.0f + 2.0f * d * logd) * invd * kCol;
+    f = f + (ff * wi - damp * dH * wi * wi * dot(nrm, dxi)) * nrm;
+    f2 = f2 + (ff * wj - damp * dH * wj * wj * dot(nrm, dxj)) * nrm;
+    dH = dH * (1.0f + damp);
+    s3 op = souter(nrm);
+    sadd(H, sscale(op, dH * wi * wi));
+    sadd(H2, sscale(op, dH * wj * wj));
+    const f3 u = ddpos - dot(nrm, ddpos) * nrm;
+    const float ul = length(u);
+    if (ul > 0) {
+        const float cand = cp.friction_coeff * ff / ul;
+        const float f1 = cand < cp.k_friction ? cand : cp.k_friction;
+        op.d[0] -= 1;
+        op.d[1] -= 1;
+        op.d[2] -= 1;
+        f = f - (f1 * wi) * u;
+        ssub(H, sscale(op, (wi * wi) * f1));
+        f2 = f2 - (f1 * wj) * u;
+        ssub(H2, sscale(op, (wj * wj) * f1));
+    }
+}
+
+struct SegSums {
+    f3 f0, f1, f2a, f2b;
+    s3 H0, H1, H2a, H2b;
+    float limit;
+};
+
+// Everything segment i adds, by sector (sector 0: stretch + even-position
+// contacts on top of vertex i's inertia terms; sector 1: odd positions).
+__device__ inline void segment_sums(const StepArgs& a, const ContactParams& cp, const float* __restrict__ dx,
+                                    const float4* __restrict__ q, const int* __restrict__ num_cols,
+                                    const int* __restrict__ cols, const float* __restrict__ max_step, int i, float damp,
+                                    SegSums& s) {
+    s.f1 = f3{0, 0, 0};
+    s.f2a = f3{0, 0, 0};
+    s.f2b = f3{0, 0, 0};
+    s.H1 = s3{{0, 0, 0, 0, 0, 0}};
+    s.H2a = s3{{0, 0, 0, 0, 0, 0}};
+    s.H2b = s3{{0, 0, 0, 0, 0, 0}};
+    s.limit = INFINITY;
+    const float4 mi = a.mat[i];
+    if (!(__float_as_uint(mi.w) & FLAG_NEXT)) return;
+    s.limit = max_step[i];
+    const float invl = 1.0f / mi.y;
+    const f3 xi = ld3(a.x, i), xj = ld3(a.x, i + 1), dxi = ld3(dx, i), dxj = ld3(dx, i + 1);
+    const f3 c = ((xj - xi) + (dxj - dxi)) * invl - rotate_e1(to4(q[i]));
+    const float k = mi.z * invl;
+    float d = k * invl;
+    s.f0 = s.f0 + (k * c - (damp * d) * dxi);
+    s.f2a = s.f2a + ((-k) * c - (damp * d) * dxj);
+    d = d * (1 + damp);
+    s.H0.d[0] += d;
+    s.H0.d[1] += d;
+    s.H0.d[2] += d;
+    s.H2a.d[0] += d;
+    s.H2a.d[1] += d;
+    s.H2a.d[2] += d;
+    const int nc = num_cols[i];
+    const int* L = cols + kMaxCols * (size_t)i;
+    for (int k2 = 0; k2 < nc; k2 += 2) contact_terms(a, cp, dx, i, L[k2], damp, s.f0, s.H0, s.f2a, s.H2a);
+    for (int k2 = 1; k2 < nc; k2 += 2) contact_terms(a, cp, dx, i, L[k2], damp, s.f1, s.H1, s.f2b, s.H2b);
+}
+
+__global__ void position_kernel(StepArgs a, ContactParams cp, int n, const float* __restrict__ dx,
+                                const float4* __restrict__ q, const float* __restrict__ y,
+                                const int* __restrict__ num_cols,
+                                const int* __restrict__ cols, const float* __restrict__ max_step,
+                                float* __restrict__ dx_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 mi = a.mat[i];
+    const f3 dxi = ld3(dx, i);
+    if (mi.x == 0.f) {
+        st3(dx_out, i, dxi);
+        return;
+    }
+    const float h = a.h;
+    const float damp = a.damping / h;
+    const float a_in = 1.0f / (mi.x * h * h);
+    SegSums s;
+    s.H0 = s3{{a_in, a_in, a_in, a_in, a_in, a_in}};
+    s.f0 = (1.0f / (h * h * mi.x)) * (ld3(y, i) - dxi);
+    segment_sums(a, cp, dx, q, num_cols, cols, max_step, i, damp, s);
+    f3 f = s.f0 + s.f1;
+    s3 H = s.H0;
+    sadd(H, s.H1);
+    float limit = s.limit;
+    if (__float_as_uint(mi.w) & FLAG_PREV) {
+        SegSums t;
+        t.f0 = f3{0, 0, 0};
+        t.H0 = s3{{0, 0, 0, 0, 0, 0}};
+        segment_sums(a, cp, dx, q, num_cols, cols, max_step, i - 1, damp, t);
+        const f3 f2 = t.f2a + t.f2b;
+        s3 H2 = t.H2a;
+        sadd(H2, t.H2b);
+        limit = gmin(limit, t.limit);
+        f = f + f2;
+        sadd(H, H2);
+    }
+    f3 out = dxi + a.acceleration_ratio * sym_solve(H.d[0], H.d[1], H.d[2], H.d[3], H.d[4], H.d[5], f);
+    const float l = length(out);
+    if (l > limit && l > 0) out = out * (limit / l);
+    st3(dx_out, i, out);
+}
+
+__global__ void orientation_kernel(StepArgs a, int n, const float* __restrict__ dxn, const float4* __restrict__ q_in,
+                                   float4* __restrict__ q_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 mi = a.mat[i];
+    const uint32_t fl = __float_as_uint(mi.w);
+    const f4 q = to4(q_in[i]);
+    f4 qn = q;
+    if ((fl & FLAG_NEXT) && !(fl & FLAG_FIX_ORI)) {
+        const f4 qm = (fl & FLAG_PREV) ? to4(q_in[i - 1]) : f4{0, 0, 0, 1};
+        const f4 qp = (fl & FLAG_NEXT_ORI) ? to4(q_in[i + 1]) : f4{0, 0, 0, 1};
+        const f4 qrp = (fl & FLAG_PREV) ? to4(a.q_rest[i - 1]) : f4{0, 0, 0, 1};
+        qn = orientation_update(ld3(a.x, i), ld3(a.x, i + 1), ld3(dxn, i), ld3(dxn, i + 1), mi.y, mi.z, q, qm, qp, qrp,
+                                to4(a.q_rest[i]), fl);
+    }
+    q_out[i] = make_float4(qn.x, qn.y, qn.z, qn.w);
+}
+
+__global__ void end_kernel(StepArgs a, int n, const float* __restrict__ dx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 x = ld3(a.x, i), v = ld3(a.v, i);
+    end_vertex(a, a.mat[i].x != 0.f, ld3(dx, i), x, v);
+    st3(a.x, i, x);
+    st3(a.v, i, v);
+}
+
+__global__ void sdf_kernel(StepArgs a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!(__float_as_uint(a.mat[i].w) & FLAG_PREV)) return;  // strand root
+    f3 x = ld3(a.x, i), v = ld3(a.v, i);
+    sdf_project(a.sdf, x, v);
+    st3(a.x, i, x);
+    st3(a.v, i, v);
+}
+
+template <class T>
+hipError_t dalloc(T*& p, size_t count) {
+    return hipMalloc((void**)&p, count * sizeof(T) + 64);
+}
+
+}  // namespace
+
+void contact_release(ContactDev& c) {
+    void* ps[] = {c.dx[0], c.dx[1], c.y, c.q2, c.max_step, c.num_cols, c.cols, c.pairs, c.error};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    lbvh_release(c.bvh);
+    c = ContactDev{};
+}
+
+hipError_t contact_reserve(ContactDev& c, int n) {
+    if (c.n == n && c.dx[0]) return hipSuccess;
+    contact_release(c);
+    hipError_t e;
+    const size_t n3 = (size_t)n * 3;
+    // Each segment overlaps a handful of non-neighbour boxes at most in
+    // practice; 4 pairs per segment leaves a wide margin (overflow is flagged).
+    c.pair_cap = 4 * n;
+    if ((e = dalloc(c.dx[0], n3)) != hipSuccess || (e = dalloc(c.dx[1], n3)) != hipSuccess ||
+        (e = dalloc(c.y, n3)) != hipSuccess || (e = dalloc(c.q2, n)) != hipSuccess ||
+        (e = dalloc(c.max_step, n)) != hipSuccess || (e = dalloc(c.num_cols, n)) != hipSuccess ||
+        (e = dalloc(c.cols, (size_t)kMaxCols * n)) != hipSuccess || (e = dalloc(c.pairs, (size_t)c.pair_cap)) != hipSuccess ||
+        (e = dalloc(c.error, 1)) != hipSuccess || (e = lbvh_reserve(c.bvh, n)) != hipSuccess) {
+        contact_release(c);
+        return e;
+    }
+    c.n = n;
+    if ((e = fill_i32(c.num_cols, 0, n, nullptr)) != hipSuccess) return e;
+    if ((e = fill_i32(c.error, 0, 1, nullptr)) != hipSuccess) return e;
+    return hipDeviceSynchronize();
+}
+
+hipError_t contact_init(const StepArgs& a, ContactDev& c, hipStream_t s) {
+    hipLaunchKernelGGL(contact_init_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, c.n, c.dx[0], c.y);
+    return hipGetLastError();
+}
+
+hipError_t contact_detect(const StepArgs& a, const ContactParams& cp, ContactDev& c, bool rebuild, hipStream_t s) {
+    hipLaunchKernelGGL(aabb_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, cp, c.n, c.dx[0], c.bvh.aabb);
+    hipError_t e = rebuild ? lbvh_build(c.bvh, c.n, s) : lbvh_refit(c.bvh, s);
+    if (e != hipSuccess) return e;
+    if ((e = lbvh_query(c.bvh, c.pairs, c.pair_cap, 2, c.error, s)) != hipSuccess) return e;
+    if ((e = fill_i32(c.num_cols, 0, c.n, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(narrow_kernel, dim3(1024), dim3(kThreads), 0, s, a, cp, c.pairs, c.bvh.count, c.pair_cap, c.dx[0],
+                       c.num_cols, c.cols, c.error);
+    hipLaunchKernelGGL(sort_lists_kernel, grid_for(c.n), dim3(kThreads), 0, s, c.n, c.num_cols, c.cols);
+    return hipGetLastError();
+}
+
+hipError_t contact_step_limit(const StepArgs& a, const ContactParams& cp, ContactDev& c, hipStream_t s) {
+    hipLaunchKernelGGL(step_limit_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, cp, c.n, c.dx[0], c.num_cols, c.cols,
+                       c.max_step);
+    return hipGetLastError();
+}
+
+hipError_t contact_iterate(const StepArgs& a, const ContactParams& cp, ContactDev& c, int cur, const float4* q_in,
+                           float4* q_out, hipStream_t s) {
+    hipLaunchKernelGGL(position_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, cp, c.n, c.dx[cur], q_in, c.y,
+                       c.num_cols, c.cols, c.max_step, c.dx[cur ^ 1]);
+    hipLaunchKernelGGL(orientation_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, c.n, c.dx[cur ^ 1], q_in, q_out);
+    return hipGetLastError();
+}
+
+hipError_t contact_end(const StepArgs& a, ContactDev& c, int cur, hipStream_t s) {
+    hipLaunchKernelGGL(end_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, c.n, c.dx[cur]);
+    return hipGetLastError();
+}
+
+hipError_t sdf_pass(const StepArgs& a, int n, hipStream_t s) {
+    hipLaunchKernelGGL(sdf_kernel, grid_for(n), dim3(kThreads), 0, s, a, n);
+    return hipGetLastError();
+}
+
+}  // namespace kscr

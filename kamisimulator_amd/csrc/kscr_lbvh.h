@@ -51,8 +51,9 @@ hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream);
 // Self query: writes pairs (found object, query object), the found leaf lying
 // at a larger sorted position than the query, into pairs[0 .. cap) and the
 // number found (which may exceed cap) into *d.count.  Pairs whose object ids
-// differ by at most min_gap are skipped (min_gap < 0 keeps all).
-hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, hipStream_t stream);
+// differ by at most min_gap are skipped (min_gap < 0 keeps all).  A traversal
+// deeper than the stack limit ends early and ORs 4 into *error (if given).
+hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error, hipStream_t stream);
 
 // Fills n ints with a value (a kernel, so it is safe under graph capture).
 hipError_t fill_i32(int* p, int value, size_t n, hipStream_t stream);

@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
                                                             const uint32_t* __restrict__ ids,
                                                             const LbvhNode* __restrict__ nodes, int n,
                                                             int2* __restrict__ pairs, int cap, int min_gap,
-                                                            int* __restrict__ count) {
+                                                            int* __restrict__ count, int* __restrict__ error) {
     __shared__ uint32_t stack[kQueryStack][kQueryBlock];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n || n < 2) return;
@@ -343,7 +343,10 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
         } else {
             const LbvhNode& nd = nodes[c];
             if (max(c, nd.fence) <= (uint32_t)t) continue;
-            // the tree depth is checked on the host (max_depth <= kQueryStack)
+            if (sp + 2 > kQueryStack) {  // deeper than the LDS stack: report, stop this query
+                if (error) atomicOr(error, 4);
+                break;
+            }
             if (intersects(nd.b[0], q)) stack[sp++][threadIdx.x] = nd.left;
             if (intersects(nd.b[1], q)) stack[sp++][threadIdx.x] = nd.right;
         }
@@ -447,13 +450,13 @@ hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, hipStream_t stream) {
+hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
     hipError_t e = fill_i32(d.count, 0, 1, stream);
     if (e != hipSuccess) return e;
     if (d.n < 2) return hipSuccess;
     hipLaunchKernelGGL(query_kernel, dim3((d.n + kQueryBlock - 1) / kQueryBlock), dim3(kQueryBlock), 0, stream, d.aabb,
-                       d.ids, d.nodes, d.n, pairs, cap, min_gap, d.count);
+                       d.ids, d.nodes, d.n, pairs, cap, min_gap, d.count, error);
     return hipGetLastError();
 }
 

@@ -30,6 +30,11 @@ struct f4 {
     float x, y, z, w;
 };
 
+// glm's min/max/clamp (comparison based: a NaN first argument propagates)
+KS_FN float gmin(float x, float y) { return (y < x) ? y : x; }
+KS_FN float gmax(float x, float y) { return (x < y) ? y : x; }
+KS_FN float clamp01(float x) { return gmin(gmax(x, 0.0f), 1.0f); }
+
 KS_FN f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
 KS_FN f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 KS_FN f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -114,6 +119,71 @@ KS_FN f3 sym_solve(float m00, float m11, float m22, float m01, float m02, float 
     return f3{i00 * f.x + i10 * f.y + i20 * f.z, i01 * f.x + i11 * f.y + i21 * f.z,
               i02 * f.x + i12 * f.y + i22 * f.z};
 }
+
+// Natural logarithm by a fixed sequence of IEEE float operations: d = m 2^e
+// with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), by its
+// odd series to s^13, plus e ln2 with ln2 split into a high and a low part.
+// Host and device evaluate the identical sequence (used by the contact
+// barrier, whose argument lies in [1e-3, 1]).
+KS_FN float det_logf(float d) {
+    int e = 0;
+    float m = frexpf(d, &e);
+    if (m < 0.70710678f) {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 1.0f / 13.0f;
+    p = p * s2 + 1.0f / 11.0f;
+    p = p * s2 + 1.0f / 9.0f;
+    p = p * s2 + 1.0f / 7.0f;
+    p = p * s2 + 1.0f / 5.0f;
+    p = p * s2 + 1.0f / 3.0f;
+    p = p * s2 + 1.0f;
+    const float lm = 2.0f * s * p;
+    const float fe = (float)e;
+    return fe * 6.93145752e-1f + (lm + fe * 1.42860677e-6f);
+}
+
+// Clamped closest parameters (u on segment a, v on segment b) of two segments
+// a0-a1 and b0-b1, with the 0.5 fallback for parallel segments.
+KS_FN void segment_closest(f3 a0, f3 a1, f3 b0, f3 b1, float& u, float& v) {
+    const f3 A0 = a1 - a0, A1 = b1 - b0;
+    const float a = dot(A0, A0);
+    const float b = dot(A0, A1);
+    const float c = dot(A1, A1);
+    const f3 diff = b0 - a0;
+    float uy = b * dot(A0, diff) - a * dot(A1, diff);
+    uy = uy / (a * c - b * b);
+    if (!(uy - uy == 0.0f)) uy = 0.5f;  // not finite
+    uy = clamp01(uy);
+    const f3 pb = b0 + uy * (b1 - b0);
+    const float ux = clamp01(dot(pb - a0, A0) / a);
+    const f3 pa = a0 + ux * (a1 - a0);
+    uy = clamp01(dot(pa - b0, A1) / c);
+    u = ux;
+    v = uy;
+}
+
+KS_FN f3 mix3(f3 a, f3 b, float t) { return a + t * (b - a); }
+
+// Packed symmetric 3x3 (d0, d1, d2, m01, m02, m12)
+struct s3 {
+    float d[6];
+};
+KS_FN void sadd(s3& a, const s3& b) {
+    for (int k = 0; k < 6; k++) a.d[k] += b.d[k];
+}
+KS_FN void ssub(s3& a, const s3& b) {
+    for (int k = 0; k < 6; k++) a.d[k] -= b.d[k];
+}
+KS_FN s3 sscale(const s3& a, float s) {
+    s3 r;
+    for (int k = 0; k < 6; k++) r.d[k] = a.d[k] * s;
+    return r;
+}
+KS_FN s3 souter(f3 v) { return s3{{v.x * v.x, v.y * v.y, v.z * v.z, v.x * v.y, v.x * v.z, v.y * v.z}}; }
 
 // Vertex flag bits (same values as the reference's VertexFlags).
 enum : uint32_t { FLAG_PREV = 1u, FLAG_NEXT = 2u, FLAG_NEXT_ORI = 4u, FLAG_FIX_ORI = 8u };

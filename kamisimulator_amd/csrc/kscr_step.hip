@@ -20,69 +20,10 @@
 //   SDF pass               SDF/cudaSDFCollision.cu:88-155, :205-268
 #include "kscr_device.h"
 #include "kscr_math.h"
+#include "kscr_solve.h"
 
 namespace kscr {
 namespace {
-
-__device__ inline f3 ld3(const float* p, int i) { return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
-__device__ inline void st3(float* p, int i, f3 a) {
-    p[3 * i] = a.x;
-    p[3 * i + 1] = a.y;
-    p[3 * i + 2] = a.z;
-}
-__device__ inline f4 to4(float4 a) { return f4{a.x, a.y, a.z, a.w}; }
-
-// Trilinear SDF lookup and projection of one particle; mirrors
-// SDFHeadCollision with its float/double index arithmetic.
-__device__ inline void sdf_project(const SdfArgs& s, f3& x, f3& v) {
-    const f4 invq{-s.rot[0], -s.rot[1], -s.rot[2], s.rot[3]};
-    const f3 lp = rotate(invq, x - f3{s.trans[0], s.trans[1], s.trans[2]}) - f3{s.gmin[0], s.gmin[1], s.gmin[2]};
-    const float l[3] = {lp.x, lp.y, lp.z};
-    int idx[3];
-    float ud[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) idx[a] = (int)((double)(l[a] * s.inv_ds[a]) + 0.5);
-    if (idx[0] < 0 || idx[0] >= s.dim[0] || idx[1] < 0 || idx[1] >= s.dim[1] || idx[2] < 0 || idx[2] >= s.dim[2]) return;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-        ud[a] = (float)((double)s.inv_ds[a] * ((double)l[a] - ((double)idx[a] - 0.5) * (double)s.ds[a]));
-    // corners c000 c001 c010 c011 c100 c101 c110 c111 (bits x y z)
-    f4 c[8];
-#pragma unroll
-    for (int o = 0; o < 8; o++) {
-        const int cx = idx[0] + ((o >> 2) & 1), cy = idx[1] + ((o >> 1) & 1), cz = idx[2] + (o & 1);
-        const bool in = cx < s.dim[0] && cy < s.dim[1] && cz < s.dim[2];  // cx, cy, cz >= 0 here
-        const int kx = min(cx, s.dim[0] - 1), ky = min(cy, s.dim[1] - 1), kz = min(cz, s.dim[2] - 1);
-        const float4 g = s.grid[(size_t)kx + (size_t)ky * s.dim[0] + (size_t)kz * s.dim[0] * s.dim[1]];
-        // gradient is zero outside the grid, phi is clamped to the border
-        c[o] = in ? f4{g.x, g.y, g.z, g.w} : f4{0.0f, 0.0f, 0.0f, g.w};
-    }
-    f4 r;
-    float* rp = &r.x;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const float c000 = (&c[0].x)[k], c001 = (&c[1].x)[k], c010 = (&c[2].x)[k], c011 = (&c[3].x)[k];
-        const float c100 = (&c[4].x)[k], c101 = (&c[5].x)[k], c110 = (&c[6].x)[k], c111 = (&c[7].x)[k];
-        const float c00 = c000 * (1.f - ud[0]) + c100 * ud[0];
-        const float c01 = c001 * (1.f - ud[0]) + c101 * ud[0];
-        const float c10 = c010 * (1.f - ud[0]) + c110 * ud[0];
-        const float c11 = c011 * (1.f - ud[0]) + c111 * ud[0];
-        const float c0 = c00 * (1.f - ud[1]) + c10 * ud[1];
-        const float c1 = c01 * (1.f - ud[1]) + c11 * ud[1];
-        rp[k] = c0 * (1.f - ud[2]) + c1 * ud[2];
-    }
-    f3 n = vec(r);
-    n = n * (1.0f / sqrtf(dot(n, n)));
-    const float phi = r.w - s.thresh;
-    if (phi < 0) {
-        x = x - phi * n;
-        const float vl = length(v);
-        if ((double)vl >= 1e-6) {
-            const f3 vn = v / vl;
-            v = v - dot(vn, n) * n;
-        }
-    }
-}
 
 template <int BLOCK>
 struct StepLds {
@@ -132,36 +73,11 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
     const float dd = kk * invl;
     const float a_in = 1.0f / (im * h * h);
     const float f_in = 1.0f / (h * h * im);
-    const f3 g{a.gravity[0], a.gravity[1], a.gravity[2]};
 
     for (int s = 0; s < a.num_substeps; s++) {
         // ---- initItr: inertial target y and VBD adaptive initial guess
-        f3 y = v;
-        f3 dx;
-        {
-            float windx = 0.f;
-            if (a.wind_peak != 0.f) {
-                const float wy = x.y > 0 ? (float)cos(2 * M_PI * (double)(a.wind_y_freq * x.y + a.wind_time_freq * a.time))
-                                         : 0.f;
-                const float wz = (float)cos(2 * M_PI * (double)(a.wind_z_freq * fabsf(x.z) + a.wind_time_freq * a.time));
-                const float yz = wy * wz;
-                const float sg = (float)((0.f < yz) - (yz < 0.f));
-                windx = a.wind_peak * fmaxf(0.f, powf(yz, a.wind_sharpness) * sg);
-            }
-            const f3 ext = (g + f3{windx, 0.f, 0.f}) * a.external_force_multiplier;
-            dx = h * v;
-            const f3 lastVel = u;
-            u = v;
-            if (free) {
-                y = dx + (h * h) * ext;
-                const float ext2 = dot(ext, ext);
-                if (ext2 > 0) {
-                    const f3 acc = (v - lastVel) / a.last_h;
-                    const float sc = fminf(fmaxf(dot(acc, ext) / ext2, 0.f), 1.f);
-                    dx = dx + (h * h * sc) * ext;
-                }
-            }
-        }
+        f3 y, dx;
+        init_vertex(a, free, x, v, u, y, dx);
         __syncthreads();  // previous substep's LDS reads are complete
         s_x[0][t] = x.x;
         s_x[1][t] = x.y;
@@ -209,21 +125,9 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             f4 qn = q;
             if (rotates) {
                 const f3 dxj{s_dx[cur ^ 1][0][t + 1], s_dx[cur ^ 1][1][t + 1], s_dx[cur ^ 1][2][t + 1]};
-                f3 vv = ((xj - x) + (dxj - dxn)) / lr;
-                vv = vv * (-2.0f * ks);
-                f4 b{0, 0, 0, 0};
-                if (hasPrev) {
-                    const f4 qj = to4(s_q[cur][t - 1]);
-                    const float phi = dot(rmul(conj(qj), q), qrp) > 0 ? 1.f : -1.f;
-                    b = b + phi * rmul(qj, qrp);
-                }
-                if (fl & FLAG_NEXT_ORI) {
-                    const f4 qj = to4(s_q[cur][t + 1]);
-                    const float phi = dot(rmul(conj(q), qj), qr) > 0 ? 1.f : -1.f;
-                    b = b + phi * rmul(qj, conj(qr));
-                }
-                const float lambda = length(vv) + length(b);
-                qn = normalize(rmul(rmul(mk4(vv, 0.f), b), f4{1.f, 0.f, 0.f, 0.f}) + lambda * b);
+                const f4 qm = hasPrev ? to4(s_q[cur][t - 1]) : f4{0, 0, 0, 1};
+                const f4 qp = (fl & FLAG_NEXT_ORI) ? to4(s_q[cur][t + 1]) : f4{0, 0, 0, 1};
+                qn = orientation_update(x, xj, dxn, dxj, lr, ks, q, qm, qp, qrp, qr, fl);
             }
             s_q[cur ^ 1][t] = make_float4(qn.x, qn.y, qn.z, qn.w);
             dx = dxn;
@@ -233,8 +137,7 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             // target the buffers this iteration only read before that barrier.
         }
         // ---- endItr
-        if (free) v = dx * (1.0f / h) * (1 - a.velocity_decay * h);
-        x = x + dx;
+        end_vertex(a, free, dx, x, v);
     }
 
     if (a.sdf_on && act && hasPrev) sdf_project(a.sdf, x, v);

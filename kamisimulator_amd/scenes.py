@@ -243,3 +243,25 @@ def config_large(num_strands=100_000, seed=2, sdf_dim=256, with_sdf=True):
         fn = union_sdf(sphere_sdf((0, 0, 0), R), capsule_sdf((0, -0.05, 0), (0, -0.3, 0), 0.045))
         sc.sdf = sdf_grid(fn, (-0.4, -0.45, -0.4), (0.4, 0.35, 0.4), (sdf_dim,) * 3)
     return sc
+
+
+def bundle(nx=6, ny=6, nv=32, spacing=6e-5, seg_len=1e-3, jitter=1e-5, seed=7):
+    """A grid of nx x ny horizontal strands (along +x) with lateral spacing
+    `spacing`; neighbours sit inside the contact barrier (hair-hair contacts)."""
+    rng = np.random.default_rng(seed)
+    ys, zs = np.meshgrid(np.arange(ny) * spacing, np.arange(nx) * spacing, indexing="ij")
+    roots = np.stack([np.zeros(nx * ny), ys.ravel(), zs.ravel()], 1)
+    x = np.arange(nv) * seg_len
+    pos = roots[:, None, :] + np.stack([x, np.zeros(nv), np.zeros(nv)], 1)[None]
+    pos[:, 1:, 1:] += rng.uniform(-jitter, jitter, size=(nx * ny, nv - 1, 2))
+    off = np.arange(0, nx * ny * nv + 1, nv).astype(np.int32)
+    return Scene(f"bundle{nx}x{ny}x{nv}", pos.reshape(-1, 3).astype(F32), off)
+
+
+def contact_params(lib_default_params, material, **kw):
+    """kscr_params with hair-hair contacts on (EnableHairHairColl: contacts and
+    step limits, StableCosseratRodsSolver.cpp:184-186)."""
+    p = make_params(lib_default_params, material, **kw)
+    p.enable_hair_hair_coll = 1
+    p.use_step_size_limit = 1
+    return p

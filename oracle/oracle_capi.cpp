@@ -33,6 +33,9 @@ struct Base {
     virtual void solve_next_step(float dt) = 0;
     virtual void set_threads(int t) = 0;
     virtual int nverts() const = 0;
+    virtual long num_pairs() const = 0;
+    virtual int error() const = 0;
+    virtual void contacts(int* num, int* cols) const = 0;
 };
 
 template <class T>
@@ -130,6 +133,14 @@ struct Impl : Base {
     void solve_next_step(float dt) override { s.solve_next_step(dt); }
     void set_threads(int t) override { s.threads = t < 1 ? 1 : t; }
     int nverts() const override { return s.n; }
+    long num_pairs() const override { return s.num_pairs; }
+    int error() const override { return s.error; }
+    void contacts(int* num, int* cols) const override {
+        for (int i = 0; i < s.n; i++) {
+            num[i] = (int)s.cols[i].size();
+            for (int k = 0; k < 16; k++) cols[16 * i + k] = k < num[i] ? s.cols[i][k] : -1;
+        }
+    }
 };
 }  // namespace
 
@@ -248,6 +259,59 @@ int oracle_set_threads(oracle_handle* h, int t) {
     return 0;
 }
 int oracle_num_verts(oracle_handle* h) { return h->b->nverts(); }
+long oracle_num_pairs(oracle_handle* h) { return h->b->num_pairs(); }
+int oracle_error(oracle_handle* h) { return h->b->error(); }
+void oracle_contacts(oracle_handle* h, int* num, int* cols) { h->b->contacts(num, cols); }
+
+// ---- LBVH restatement (lbvh_oracle.hpp) for the bit-exact tree tests
+struct oracle_lbvh {
+    orc::Lbvh t;
+    std::vector<orc::Aabb> boxes;
+};
+static std::vector<orc::Aabb> to_boxes(const float* a, int n) {
+    std::vector<orc::Aabb> b(n);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) b[i].mn[k] = a[6 * i + k], b[i].mx[k] = a[6 * i + 3 + k];
+    return b;
+}
+oracle_lbvh* oracle_lbvh_build(const float* aabbs, int n) {
+    auto* o = new oracle_lbvh;
+    o->boxes = to_boxes(aabbs, n);
+    o->t.build(o->boxes);
+    return o;
+}
+void oracle_lbvh_refit(oracle_lbvh* o, const float* aabbs) {
+    o->boxes = to_boxes(aabbs, o->t.n);
+    o->t.refit(o->boxes);
+}
+void oracle_lbvh_destroy(oracle_lbvh* o) { delete o; }
+long oracle_lbvh_query(oracle_lbvh* o, int* pairs, long cap) {
+    auto pr = o->t.query(o->boxes);
+    for (long k = 0; k < (long)pr.size() && k < cap; k++) pairs[2 * k] = pr[k].first, pairs[2 * k + 1] = pr[k].second;
+    return (long)pr.size();
+}
+void oracle_lbvh_get(oracle_lbvh* o, uint32_t* codes, uint32_t* ids, uint32_t* links, float* bounds, uint32_t* lp,
+                     float* root, int* max_stack) {
+    const int n = o->t.n;
+    for (int i = 0; i < n; i++) {
+        if (codes) codes[i] = o->t.codes[i];
+        if (ids) ids[i] = o->t.ids[i];
+        if (lp) lp[i] = o->t.leaf_parent[i];
+    }
+    for (int i = 0; i < n - 1; i++) {
+        const orc::LbvhNode& nd = o->t.nodes[i];
+        if (links) links[4 * i] = nd.parent, links[4 * i + 1] = nd.left, links[4 * i + 2] = nd.right, links[4 * i + 3] = nd.fence;
+        if (bounds)
+            for (int c = 0; c < 2; c++)
+                for (int k = 0; k < 3; k++) bounds[12 * i + 6 * c + k] = nd.b[c].mn[k], bounds[12 * i + 6 * c + 3 + k] = nd.b[c].mx[k];
+    }
+    if (root)
+        for (int k = 0; k < 3; k++) root[k] = o->t.root.mn[k], root[3 + k] = o->t.root.mx[k];
+    if (max_stack) *max_stack = o->t.max_stack;
+}
+// Contact state of the step oracle after the last detection.
+long oracle_num_pairs(oracle_handle* h);
+int oracle_error(oracle_handle* h);
 
 // Primitive access for unit tests.
 void oracle_rotor_mul(const float* a, const float* b, float* out) {

@@ -49,6 +49,14 @@ SIGNATURES = {
     "oracle_segment_closest_points": (None, [_FP, _FP, _FP, _FP, _FP]),
     "oracle_morton3": (C.c_uint32, [C.c_int, C.c_int, C.c_int]),
     "oracle_sym_solve": (None, [_FP, _FP, _FP]),
+    "oracle_lbvh_build": (_P, [_FP, C.c_int]),
+    "oracle_lbvh_refit": (None, [_P, _FP]),
+    "oracle_lbvh_destroy": (None, [_P]),
+    "oracle_lbvh_query": (C.c_long, [_P, _IP, C.c_long]),
+    "oracle_lbvh_get": (None, [_P, _UP, _UP, _UP, _FP, _UP, _FP, _IP]),
+    "oracle_num_pairs": (C.c_long, [_P]),
+    "oracle_error": (C.c_int, [_P]),
+    "oracle_contacts": (None, [_P, _IP, _IP]),
 }
 
 _lib = None
@@ -200,6 +208,61 @@ class OracleSim:
 
     def solve_next_step(self, dt):
         self.lib.oracle_solve_next_step(self.h, dt)
+
+
+    def num_pairs(self):
+        return self.lib.oracle_num_pairs(self.h)
+
+    def error(self):
+        return self.lib.oracle_error(self.h)
+
+    def contacts(self):
+        num = np.empty(self.n, np.int32)
+        cols = np.empty((self.n, 16), np.int32)
+        self.lib.oracle_contacts(self.h, _i(num), _i(cols))
+        return num, cols
+
+
+class OracleLbvh:
+    """CPU restatement of Kitten::LBVH (lbvh_oracle.hpp)."""
+
+    def __init__(self, boxes):
+        self.lib = load()
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        self.n = b.shape[0]
+        self.boxes = b
+        self.h = self.lib.oracle_lbvh_build(_f(b), self.n)
+
+    def refit(self, boxes):
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        self.boxes = b
+        self.lib.oracle_lbvh_refit(self.h, _f(b))
+
+    def query(self, cap=None):
+        cap = cap if cap is not None else 64 * self.n
+        out = np.empty((cap, 2), np.int32)
+        m = self.lib.oracle_lbvh_query(self.h, _i(out), cap)
+        return out[:min(m, cap)].copy()
+
+    def get(self):
+        n = self.n
+        codes = np.empty(n, np.uint32)
+        ids = np.empty(n, np.uint32)
+        links = np.empty((max(n - 1, 1), 4), np.uint32)
+        bounds = np.empty((max(n - 1, 1), 12), np.float32)
+        lp = np.empty(n, np.uint32)
+        root = np.empty(6, np.float32)
+        ms = C.c_int(0)
+        u = lambda a: a.ctypes.data_as(_UP)
+        self.lib.oracle_lbvh_get(self.h, u(codes), u(ids), u(links), _f(bounds), u(lp), _f(root), C.byref(ms))
+        return dict(codes=codes, ids=ids, links=links[:n - 1], bounds=bounds[:n - 1], leaf_parent=lp, root=root,
+                    max_stack=ms.value)
+
+    def __del__(self):
+        try:
+            self.lib.oracle_lbvh_destroy(self.h)
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ primitives

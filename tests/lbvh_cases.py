@@ -1,0 +1,49 @@
+"""Box sets for the LBVH tests.
+
+testlbvh_points follows the reference's own recipe (Kitten::testLBVH,
+KE/KittenGpuLBVH/lbvh.cu:554-565): srand(1), points (rand()/RAND_MAX)^3 in the
+unit cube, each box padded by R = 0.001.  glibc's rand() is called through
+ctypes, so the points are the reference's points on this platform."""
+import ctypes
+import ctypes.util
+
+import numpy as np
+
+
+def testlbvh_points(n=100_000, pad=0.001):
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.rand.restype = ctypes.c_int
+    libc.srand(1)
+    RAND_MAX = 2147483647
+    raw = np.array([libc.rand() for _ in range(3 * n)], np.float64).reshape(n, 3)
+    # rand() / (float)RAND_MAX in float
+    p = (raw.astype(np.float32) / np.float32(RAND_MAX)).astype(np.float32)
+    boxes = np.concatenate([p - np.float32(pad), p + np.float32(pad)], axis=1).astype(np.float32)
+    return boxes
+
+
+def segment_boxes(positions, offsets, pad):
+    """Padded segment boxes of strands; tips get empty boxes (+inf, -inf)."""
+    n = positions.shape[0]
+    boxes = np.empty((n, 6), np.float32)
+    boxes[:, :3] = np.inf
+    boxes[:, 3:] = -np.inf
+    has_next = np.ones(n, bool)
+    has_next[np.asarray(offsets[1:]) - 1] = False
+    i = np.nonzero(has_next)[0]
+    p0, p1 = positions[i], positions[i + 1]
+    boxes[i, :3] = np.minimum(p0, p1) - np.float32(pad)
+    boxes[i, 3:] = np.maximum(p0, p1) + np.float32(pad)
+    return boxes
+
+
+def brute_pairs(boxes):
+    """All strictly overlapping pairs (i < j), Bound::intersects semantics."""
+    b = boxes.astype(np.float32)
+    n = b.shape[0]
+    out = []
+    for i in range(n):
+        ov = ~(np.any(b[i, 3:] <= b[i + 1:, :3], axis=1) | np.any(b[i, :3] >= b[i + 1:, 3:], axis=1))
+        for j in np.nonzero(ov)[0]:
+            out.append((i, i + 1 + j))
+    return np.array(out, np.int64).reshape(-1, 2)

@@ -1,0 +1,76 @@
+"""The step with hair-hair contacts (LBVH broad phase, narrow phase, IPC
+barrier and friction, step limits) against the FP32 restatement."""
+import numpy as np
+import pytest
+
+from kamisimulator_amd import scenes
+from kamisimulator_amd._lib import KscrError, Sim, default_params
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+DT = np.float32(1 / 600)
+
+
+def _pair(sc, **kw):
+    p = scenes.contact_params(default_params, sc.material, **kw)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets), sc)
+    o.set_threads(8)
+    return g, o
+
+
+def _rel(a, b):
+    return float(np.abs(a.astype(np.float64) - b).max() / np.abs(b).max())
+
+
+def test_bundle_contacts_match_oracle():
+    sc = scenes.bundle(6, 6, 32)
+    rng = np.random.default_rng(3)
+    vel = rng.uniform(-0.02, 0.02, size=sc.positions.shape).astype(np.float32)
+    g, o = _pair(sc)
+    g.set_pos_vel(sc.positions, vel)
+    o.set_pos_vel(sc.positions, vel)
+    print(f"[gpu-contacts] bundle verts={sc.num_verts}", flush=True)
+    for s in range(10):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+    gn, gc = g.get_contacts()
+    on, oc = o.contacts()
+    print(f"[gpu-contacts] contacts gpu {gn.sum()} oracle {on.sum()} pairs(oracle) {o.num_pairs()}", flush=True)
+    assert on.sum() > 50
+    assert np.array_equal(gn, on) and np.array_equal(gc, oc)
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    ep, eq = _rel(gp, op), _rel(g.get_q(), o.get_q())
+    print(f"[gpu-contacts] pos rel {ep:.3e} q rel {eq:.3e} differing pos words "
+          f"{int(np.count_nonzero(gp.view(np.uint32) != op.view(np.uint32)))}", flush=True)
+    assert ep <= 1e-5 and eq <= 1e-5
+
+
+def test_groom_with_contacts_and_sdf():
+    counts = scenes.lognormal_counts(300, seed=8)
+    sc = scenes.groom(300, counts, seed=8, sphere_radius=0.02)  # dense scalp -> contacts
+    sc.sdf = scenes.sdf_grid(scenes.sphere_sdf((0, 0, 0), 0.02), (-0.1,) * 3, (0.1,) * 3, (64,) * 3)
+    g, o = _pair(sc)
+    for s in range(5):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+    gn, _ = g.get_contacts()
+    on, _ = o.contacts()
+    print(f"[gpu-contacts] groom300 contacts gpu {gn.sum()} oracle {on.sum()}", flush=True)
+    assert np.array_equal(gn, on)
+    assert _rel(g.get_pos_vel()[0], o.get_pos_vel()[0]) <= 1e-5
+
+
+def test_too_many_contacts_is_reported():
+    # 30 strands packed within one barrier width: > 16 contacts per segment
+    sc = scenes.bundle(6, 5, 16, spacing=1.5e-5, jitter=2e-6)
+    p = scenes.contact_params(default_params, sc.material)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    g.solve_next_step(DT)
+    with pytest.raises(KscrError) as ei:
+        g.get_pos_vel()
+    assert "MAX_COLLISIONS" in str(ei.value)
+    o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets), sc)
+    o.solve_next_step(DT)
+    assert o.error() == 8

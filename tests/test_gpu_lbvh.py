@@ -1,0 +1,74 @@
+"""Device LBVH (kscr_lbvh_*) against the restatement: Morton codes, sorted
+order, tree links, node boxes, leaf parents, root box and stack bound are
+compared bit for bit; the query against the restatement's pair set, on the
+reference's own testLBVH recipe (100k points, srand(1), R = 0.001)."""
+import time
+
+import numpy as np
+import pytest
+
+from kamisimulator_amd._lib import Lbvh
+from oracle.pyoracle import OracleLbvh
+from tests.lbvh_cases import segment_boxes, testlbvh_points
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(boxes, label):
+    print(f"[gpu-lbvh] {label}: n={len(boxes)}", flush=True)
+    g = Lbvh(device=0)
+    t0 = time.perf_counter()
+    g.compute(boxes)
+    t1 = time.perf_counter()
+    o = OracleLbvh(boxes)
+    tg, to = g.get(), o.get()
+    for k in ("codes", "ids", "links", "leaf_parent"):
+        assert np.array_equal(tg[k], to[k]), k
+    for k in ("bounds", "root"):
+        assert np.array_equal(tg[k].view(np.uint32), to[k].view(np.uint32)), k
+    assert tg["max_stack"] == to["max_stack"]
+    cap = 200 * len(boxes)
+    gp, cnt = g.query(cap)
+    op = o.query(cap)
+    assert cnt == len(op) and cnt <= cap
+    key = lambda p: np.sort(p[:, 0].astype(np.int64) * len(boxes) + p[:, 1])
+    assert np.array_equal(key(gp), key(op))  # same oriented pairs
+    print(f"[gpu-lbvh] {label}: {cnt} pairs, build+download {1e3 * (t1 - t0):.1f} ms, stack {tg['max_stack']}",
+          flush=True)
+    return g, o
+
+
+def test_reference_recipe_100k():
+    _compare(testlbvh_points(100_000), "testLBVH recipe")
+
+
+def test_segments_and_refit():
+    rng = np.random.default_rng(9)
+    ns, nv = 2000, 50
+    roots = rng.uniform(0, 0.05, size=(ns, 3))
+    pos = (roots[:, None, :] + np.linspace(0, 0.05, nv)[None, :, None] * np.array([0, -1.0, 0.2])
+           + rng.normal(scale=3e-4, size=(ns, nv, 3))).reshape(-1, 3).astype(np.float32)
+    off = np.arange(0, ns * nv + 1, nv)
+    boxes = segment_boxes(pos, off, 1e-4)
+    g, o = _compare(boxes, "segments")
+    boxes2 = segment_boxes(pos + rng.normal(scale=4e-4, size=pos.shape).astype(np.float32), off, 1e-4)
+    g.refit(boxes2)
+    o.refit(boxes2)
+    tg, to = g.get(), o.get()
+    assert np.array_equal(tg["bounds"].view(np.uint32), to["bounds"].view(np.uint32))
+    assert tg["max_stack"] == to["max_stack"]
+
+
+def test_ties_and_one_object():
+    boxes = np.zeros((5000, 6), np.float32)
+    boxes[:, 3:] = 1.0
+    boxes[::7] = [0, 0, 0, 2, 2, 2]
+    g = Lbvh(device=0)
+    g.compute(boxes)
+    o = OracleLbvh(boxes)
+    for k in ("codes", "ids", "links", "leaf_parent"):
+        assert np.array_equal(g.get()[k], o.get()[k]), k
+    one = np.array([[0, 0, 0, 1, 1, 1]], np.float32)
+    g.compute(one)
+    pairs, cnt = g.query(16)
+    assert cnt == 0
