@@ -37,6 +37,11 @@ FUSED_BYTES_PER_VERTEX = 136
 # iteration goes through HBM): read x, dx, y (12 B each), q, q_rest (16 B
 # each), invMass/lRest/kStretch (12 B), flags (4 B); write dx (12 B), q (16 B).
 STREAMING_BYTES_PER_UPDATE = 112
+# HBM bytes per vertex the fused step kernel actually moves, measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE on the 100k x 100 groom, each scaled by
+# what the counter reads for a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0):
+# 833,258 KiB read + 508,750 KiB written per launch of 1e7 vertices.
+MEASURED_TRAFFIC_BYTES_PER_VERTEX = (833_258 + 508_750) * 1024 / 1e7
 
 
 def parse():
@@ -185,7 +190,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": round(MEASURED_TRAFFIC_BYTES_PER_VERTEX * scene.num_verts),
         "kernel": "step_fused_kernel",
         "algorithmic_bytes_per_launch": FUSED_BYTES_PER_VERTEX * scene.num_verts,
         "avg_launch_ms": round(kern_ms, 4),

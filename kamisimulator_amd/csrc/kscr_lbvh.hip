@@ -228,6 +228,7 @@ __global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n,
     if (i == 0) {
         lo = 0;
         hi = n - 1;
+        nodes[0].parent = 0;  // the root has no parent (value-initialised in the reference)
     } else {
         const uint64_t self = sort_key(codes, i);
         const int dl = clz64(self ^ sort_key(codes, i - 1));

@@ -258,6 +258,13 @@ def bundle(nx=6, ny=6, nv=32, spacing=6e-5, seg_len=1e-3, jitter=1e-5, seed=7):
     return Scene(f"bundle{nx}x{ny}x{nv}", pos.reshape(-1, 3).astype(F32), off)
 
 
+def ribbon(num_strands=12, nv=32, spacing=6e-5, seg_len=1e-3, jitter=1e-5, seed=7):
+    """One row of horizontal strands `spacing` apart: each segment touches only
+    the two neighbouring strands (inside the contact barrier, well below the
+    16-contact list capacity)."""
+    return bundle(1, num_strands, nv, spacing=spacing, seg_len=seg_len, jitter=jitter, seed=seed)
+
+
 def contact_params(lib_default_params, material, **kw):
     """kscr_params with hair-hair contacts on (EnableHairHairColl: contacts and
     step limits, StableCosseratRodsSolver.cpp:184-186)."""

@@ -1,0 +1,357 @@
+// adjoint_oracle.hpp -- discrete adjoint (reverse mode) of the contact-free
+// Stable Cosserat Rods step of scr_oracle.hpp.
+//
+// TEST INFRASTRUCTURE ONLY (see scr_oracle.hpp).  The reference has no
+// adjoint; this differentiates the oracle's own Jacobi-snapshot step, in exact
+// arithmetic, treating the quaternion sign switch phi (cosserat.cu:320, :328)
+// and the clamp of the adaptive initial guess (iteration.cu:50) as piecewise
+// constant.  T = double is checked against central finite differences of the
+// FP64 forward (tests/test_oracle_adjoint.py); T = float is the restatement
+// the HIP adjoint is compared with.
+//
+// Tape of one substep: x, v, u at its start, y, and every iterate dx^k, q^k.
+#pragma once
+
+#include <vector>
+
+#include "scr_oracle.hpp"
+
+namespace orc {
+
+template <class T>
+struct SubstepTape {
+    std::vector<V3<T>> x, v, u, y;
+    std::vector<std::vector<V3<T>>> dx;  // numItr + 1 iterates
+    std::vector<std::vector<R4<T>>> q;   // numItr + 1 iterates
+};
+
+template <class T>
+struct StepTape {
+    std::vector<SubstepTape<T>> sub;
+    T h, lastH;
+};
+
+template <class T>
+struct Adjoint {
+    // adjoints of the state (x, v, u = lastVel, q)
+    std::vector<V3<T>> xb, vb, ub;
+    std::vector<R4<T>> qb;
+    // parameter adjoints: per-vertex stretch stiffness and rest rotor
+    std::vector<T> ksb;
+    std::vector<R4<T>> qrb;
+    void resize(int n) {
+        xb.assign(n, V3<T>{0, 0, 0});
+        vb.assign(n, V3<T>{0, 0, 0});
+        ub.assign(n, V3<T>{0, 0, 0});
+        qb.assign(n, R4<T>{0, 0, 0, 0});
+        ksb.assign(n, T(0));
+        qrb.assign(n, R4<T>{0, 0, 0, 0});
+    }
+};
+
+// d/dq of rotate(q, e1) = (w^2 - |qv|^2) e1 + 2w (qv x e1) + 2 (qv.e1) qv, applied
+// transposed to rb.
+template <class T>
+inline R4<T> rot_e1_vjp(R4<T> q, V3<T> rb) {
+    const V3<T> e{1, 0, 0};
+    const V3<T> qv = q.q();
+    const T w = q.w;
+    const T wb = dot(rb, T(2) * w * e + T(2) * cross(qv, e));
+    V3<T> qvb = T(-2) * dot(rb, e) * qv + T(2) * w * cross(e, rb) + T(2) * dot(rb, qv) * e + T(2) * dot(qv, e) * rb;
+    return R4<T>{qvb.x, qvb.y, qvb.z, wb};
+}
+
+// c = a (x) b:  abar = cbar (x) conj(b),  bbar = conj(a) (x) cbar
+template <class T>
+inline void rmul_vjp(R4<T> a, R4<T> b, R4<T> cb, R4<T>& ab, R4<T>& bb) {
+    ab = add4(ab, rmul(cb, conj(b)));
+    bb = add4(bb, rmul(conj(a), cb));
+}
+
+template <class T>
+struct AdjointSim : Sim<T> {
+    using Sim<T>::n;
+    using Sim<T>::p;
+    using Sim<T>::x;
+    using Sim<T>::v;
+    using Sim<T>::u;
+    using Sim<T>::q;
+    using Sim<T>::qRest;
+    using Sim<T>::invMass;
+    using Sim<T>::lRest;
+    using Sim<T>::kStretch;
+    using Sim<T>::flags;
+    using Sim<T>::y;
+    using Sim<T>::dx;
+    using Sim<T>::h;
+    using Sim<T>::lastH;
+    using Sim<T>::hF;
+    using Sim<T>::lastHF;
+    using Sim<T>::timeF;
+    using Sim<T>::time;
+    using Sim<T>::step_counter;
+
+    // Forward step (advance) recording the tape.
+    void record_step(float dtF, StepTape<T>& tape) {
+        const int steps = std::max(1, (int)std::ceil(dtF / p.max_h));
+        lastHF = hF;
+        hF = dtF / steps;
+        lastH = (T)lastHF;
+        h = (T)hF;
+        tape.h = h;
+        tape.lastH = lastH;
+        tape.sub.assign(steps, SubstepTape<T>{});
+        for (int s = 0; s < steps; s++, step_counter++) {
+            SubstepTape<T>& t = tape.sub[s];
+            t.x = x;
+            t.v = v;
+            t.u = u;
+            this->init_itr();
+            this->recompute_step_limit();
+            t.y = y;
+            t.dx.push_back(dx);
+            t.q.push_back(q);
+            for (int k = 0; k < p.num_itr; k++) {
+                this->iterate();
+                t.dx.push_back(dx);
+                t.q.push_back(q);
+            }
+            this->end_itr();
+        }
+        timeF += dtF;
+        time = (T)timeF;
+    }
+
+    // Reverse of one substep: adj holds the adjoints of the substep's outputs
+    // (x', v', u', q') on entry and of its inputs (x, v, u, q) on exit.
+    void backward_substep(const SubstepTape<T>& t, Adjoint<T>& adj) {
+        const int K = p.num_itr;
+        const T hh = h;
+        const T damp = T(p.damping) / hh;
+        const T invH = T(1) / hh;
+        const T decay = T(1) - T(p.velocity_decay) * hh;
+        std::vector<V3<T>> dxb(n), dxb2(n), fb(n), yb(n, V3<T>{0, 0, 0});
+        std::vector<T> Db(n);
+        std::vector<R4<T>> qb2(n);
+        // endItr: x' = x + dx^K; v' = dx^K invH decay (free) or v (pinned)
+        for (int i = 0; i < n; i++) {
+            const bool free = invMass[i] != 0;
+            dxb[i] = adj.xb[i] + (free ? adj.vb[i] * (invH * decay) : V3<T>{0, 0, 0});
+            if (free) adj.vb[i] = V3<T>{0, 0, 0};  // pinned: v' = v keeps its adjoint
+        }
+        for (int k = K - 1; k >= 0; k--) {
+            const std::vector<V3<T>>& DX = t.dx[k];
+            const std::vector<V3<T>>& DXn = t.dx[k + 1];
+            const std::vector<R4<T>>& Q = t.q[k];
+            // ---- orientation solve (uses DXn and Q)
+            for (int i = 0; i < n; i++) qb2[i] = R4<T>{0, 0, 0, 0};
+            for (int i = 0; i < n; i++) {
+                const uint32_t fl = flags[i];
+                if ((fl & F_FIX_ORI) || !(fl & F_NEXT)) {
+                    qb2[i] = add4(qb2[i], adj.qb[i]);
+                    continue;
+                }
+                const V3<T> e = (t.x[i + 1] - t.x[i]) + (DXn[i + 1] - DXn[i]);
+                const T sc = T(-2) * kStretch[i] / lRest[i];
+                const V3<T> vv = e * sc;
+                R4<T> b{0, 0, 0, 0}, bm{0, 0, 0, 0}, bp{0, 0, 0, 0};
+                T phim = 0, phip = 0;
+                if (fl & F_PREV) {
+                    phim = dot4(rmul(conj(Q[i - 1]), Q[i]), qRest[i - 1]) > 0 ? T(1) : T(-1);
+                    bm = rmul(Q[i - 1], qRest[i - 1]);
+                    b = add4(b, scale4(phim, bm));
+                }
+                if (fl & F_NEXT_ORI) {
+                    phip = dot4(rmul(conj(Q[i]), Q[i + 1]), qRest[i]) > 0 ? T(1) : T(-1);
+                    bp = rmul(Q[i + 1], conj(qRest[i]));
+                    b = add4(b, scale4(phip, bp));
+                }
+                const T lv = length(vv), lb = length4(b);
+                const T lambda = lv + lb;
+                const R4<T> V = r4(vv, T(0));
+                const R4<T> m = rmul(V, b);
+                const R4<T> e1{1, 0, 0, 0};
+                const R4<T> tq = rmul(m, e1);
+                const R4<T> w = add4(tq, scale4(lambda, b));
+                const T lw = length4(w);
+                const R4<T> qn = scale4(T(1) / lw, w);
+                const R4<T> g = adj.qb[i];
+                // normalize
+                const R4<T> wb = scale4(T(1) / lw, add4(g, scale4(-dot4(qn, g), qn)));
+                // w = t + lambda b
+                R4<T> bb = scale4(lambda, wb);
+                const T lamb = dot4(wb, b);
+                // t = m (x) e1
+                R4<T> mb{0, 0, 0, 0}, dummy{0, 0, 0, 0};
+                rmul_vjp(m, e1, wb, mb, dummy);
+                // m = V (x) b
+                R4<T> Vb{0, 0, 0, 0};
+                rmul_vjp(V, b, mb, Vb, bb);
+                V3<T> vvb = Vb.q();
+                // lambda = |v| + |b|
+                if (lv > 0) vvb = vvb + (lamb / lv) * vv;
+                if (lb > 0) bb = add4(bb, scale4(lamb / lb, b));
+                // b terms
+                if (fl & F_PREV) {
+                    R4<T> qmb{0, 0, 0, 0}, qrb{0, 0, 0, 0};
+                    rmul_vjp(Q[i - 1], qRest[i - 1], scale4(phim, bb), qmb, qrb);
+                    qb2[i - 1] = add4(qb2[i - 1], qmb);
+                    adj.qrb[i - 1] = add4(adj.qrb[i - 1], qrb);
+                }
+                if (fl & F_NEXT_ORI) {
+                    R4<T> qpb{0, 0, 0, 0}, cqb{0, 0, 0, 0};
+                    rmul_vjp(Q[i + 1], conj(qRest[i]), scale4(phip, bb), qpb, cqb);
+                    qb2[i + 1] = add4(qb2[i + 1], qpb);
+                    adj.qrb[i] = add4(adj.qrb[i], conj(cqb));
+                }
+                // v = e * sc
+                const V3<T> eb = vvb * sc;
+                adj.ksb[i] += dot(vvb, e) * (T(-2) / lRest[i]);
+                dxb[i + 1] = dxb[i + 1] + eb;
+                dxb[i] = dxb[i] - eb;
+                adj.xb[i + 1] = adj.xb[i + 1] + eb;
+                adj.xb[i] = adj.xb[i] - eb;
+            }
+            // ---- position solve (uses DX and Q): dx^{k+1} = dx^k + accel H^-1 f
+            for (int i = 0; i < n; i++) {
+                dxb2[i] = dxb[i];
+                fb[i] = V3<T>{0, 0, 0};
+                Db[i] = 0;
+            }
+            for (int i = 0; i < n; i++) {
+                if (invMass[i] == 0) continue;
+                const T a_in = T(1) / (invMass[i] * hh * hh);
+                T D = 0;
+                V3<T> f = (T(1) / (hh * hh * invMass[i])) * (t.y[i] - DX[i]);
+                if (flags[i] & F_NEXT) {
+                    const T il = T(1) / lRest[i];
+                    const V3<T> c = ((t.x[i + 1] - t.x[i]) + (DX[i + 1] - DX[i])) * il - rotate(Q[i], V3<T>{1, 0, 0});
+                    const T kk = kStretch[i] * il, dd = kk * il;
+                    f = f + (kk * c - (damp * dd) * DX[i]);
+                    D += dd * (1 + damp);
+                }
+                if (flags[i] & F_PREV) {
+                    const int j = i - 1;
+                    const T il = T(1) / lRest[j];
+                    const V3<T> c = ((t.x[i] - t.x[j]) + (DX[i] - DX[j])) * il - rotate(Q[j], V3<T>{1, 0, 0});
+                    const T kk = kStretch[j] * il, dd = kk * il;
+                    f = f + ((-kk) * c - (damp * dd) * DX[i]);
+                    D += dd * (1 + damp);
+                }
+                const Sym3<T> H{{a_in + D, a_in + D, a_in + D, a_in, a_in, a_in}};
+                const V3<T> delta = sym_solve(H, f);
+                fb[i] = sym_solve(H, T(p.acceleration_ratio) * dxb[i]);
+                Db[i] = -dot(fb[i], delta);  // D enters the diagonal of H: Hbar = -fbar delta^T
+                yb[i] = yb[i] + (T(1) / (hh * hh * invMass[i])) * fb[i];
+                dxb2[i] = dxb2[i] - (T(1) / (hh * hh * invMass[i])) * fb[i];
+            }
+            for (int j = 0; j < n; j++) {
+                if (!(flags[j] & F_NEXT)) continue;
+                const T il = T(1) / lRest[j];
+                const V3<T> c = ((t.x[j + 1] - t.x[j]) + (DX[j + 1] - DX[j])) * il - rotate(Q[j], V3<T>{1, 0, 0});
+                const T kk = kStretch[j] * il, dd = kk * il;
+                const V3<T> cb = kk * (fb[j] - fb[j + 1]);
+                const V3<T> s = cb * il;
+                adj.xb[j + 1] = adj.xb[j + 1] + s;
+                adj.xb[j] = adj.xb[j] - s;
+                dxb2[j + 1] = dxb2[j + 1] + s;
+                dxb2[j] = dxb2[j] - s;
+                const R4<T> qrot = rot_e1_vjp(Q[j], -cb);
+                qb2[j] = add4(qb2[j], qrot);
+                dxb2[j] = dxb2[j] - (damp * dd) * fb[j];
+                dxb2[j + 1] = dxb2[j + 1] - (damp * dd) * fb[j + 1];
+                adj.ksb[j] += dot(c, fb[j] - fb[j + 1]) * il - damp * (dot(DX[j], fb[j]) + dot(DX[j + 1], fb[j + 1])) * il * il +
+                              (Db[j] + Db[j + 1]) * (1 + damp) * il * il;
+            }
+            dxb.swap(dxb2);
+            adj.qb.swap(qb2);
+        }
+        // ---- initItr: dx^0 = h v (+ h^2 sigma ext), y = h v + h^2 ext, u' = v
+        const V3<T> g{(T)p.gravity[0], (T)p.gravity[1], (T)p.gravity[2]};
+        const V3<T> ext = g * (T)p.external_force_multiplier;  // no wind on the adjoint path
+        const T ext2 = length2(ext);
+        std::vector<V3<T>> ubn(n, V3<T>{0, 0, 0});
+        for (int i = 0; i < n; i++) {
+            V3<T> vb = adj.vb[i] + adj.ub[i];  // u' = v
+            vb = vb + hh * dxb[i];
+            if (invMass[i] != 0) {
+                vb = vb + hh * yb[i];
+                if (ext2 > 0) {
+                    const V3<T> a = (t.v[i] - t.u[i]) / lastH;
+                    const T sraw = dot(a, ext) / ext2;
+                    if (sraw > 0 && sraw < 1) {
+                        const T sb = hh * hh * dot(dxb[i], ext);
+                        const V3<T> ab = (sb / ext2) * ext;
+                        vb = vb + ab / lastH;
+                        ubn[i] = ubn[i] - ab / lastH;
+                    }
+                }
+            }
+            adj.vb[i] = vb;
+        }
+        adj.ub.swap(ubn);
+        // x: the substep's x enters through x' = x + dx (xb passes through) and
+        // through the stretch/orientation terms (accumulated above).
+    }
+
+    struct Checkpoint {
+        std::vector<V3<T>> x, v, u;
+        std::vector<R4<T>> q;
+        T h, lastH, time;
+        float hF, lastHF, timeF;
+        long counter;
+    };
+    Checkpoint save() const {
+        return Checkpoint{x, v, u, q, h, lastH, time, hF, lastHF, timeF, step_counter};
+    }
+    void restore(const Checkpoint& c) {
+        x = c.x, v = c.v, u = c.u, q = c.q;
+        h = c.h, lastH = c.lastH, time = c.time;
+        hF = c.hF, lastHF = c.lastHF, timeF = c.timeF;
+        step_counter = c.counter;
+    }
+
+    // Loss L = sum_s |x_tip(s) - target_s|^2 after nsteps steps of dtF, and its
+    // gradient with respect to every vertex's kStretch and qRest.  Forward with
+    // checkpoints every K steps; backward segment by segment, recomputing the
+    // segment's steps from its checkpoint to record their tapes.
+    T gradient(float dtF, int nsteps, int K, const std::vector<int>& tips, const std::vector<T>& targets,
+               std::vector<T>& ksb, std::vector<R4<T>>& qrb) {
+        std::vector<Checkpoint> ck;
+        for (int s = 0; s < nsteps; s++) {
+            if (s % K == 0) ck.push_back(save());
+            this->advance(dtF);
+        }
+        T loss = 0;
+        Adjoint<T> adj;
+        adj.resize(n);
+        for (size_t k = 0; k < tips.size(); k++) {
+            const V3<T> d = x[tips[k]] - V3<T>{targets[3 * k], targets[3 * k + 1], targets[3 * k + 2]};
+            loss += dot(d, d);
+            adj.xb[tips[k]] = T(2) * d;
+        }
+        const Checkpoint end = save();
+        for (int seg = (int)ck.size() - 1; seg >= 0; seg--) {
+            restore(ck[seg]);
+            const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
+            std::vector<StepTape<T>> tapes(s1 - s0);
+            for (int s = s0; s < s1; s++) record_step(dtF, tapes[s - s0]);
+            for (int s = s1 - 1; s >= s0; s--) backward_step(tapes[s - s0], adj);
+        }
+        restore(end);
+        ksb = adj.ksb;
+        qrb = adj.qrb;
+        return loss;
+    }
+
+    void backward_step(const StepTape<T>& tape, Adjoint<T>& adj) {
+        const T h_save = h, lh_save = lastH;
+        h = tape.h;
+        lastH = tape.lastH;
+        for (int s = (int)tape.sub.size() - 1; s >= 0; s--) backward_substep(tape.sub[s], adj);
+        h = h_save;
+        lastH = lh_save;
+    }
+};
+
+}  // namespace orc

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 
+#include "adjoint_oracle.hpp"
 #include "scr_oracle.hpp"
 
 namespace {
@@ -36,11 +37,15 @@ struct Base {
     virtual long num_pairs() const = 0;
     virtual int error() const = 0;
     virtual void contacts(int* num, int* cols) const = 0;
+    virtual void set_material(const float* ks, const float* qr) = 0;
+    virtual void set_material_t(const double* ks, const double* qr) = 0;
+    virtual double gradient(float dt, int nsteps, int K, const int* tips, int ntips, const double* targets, double* ksb,
+                            double* qrb) = 0;
 };
 
 template <class T>
 struct Impl : Base {
-    orc::Sim<T> s;
+    orc::AdjointSim<T> s;
     int create(const orc::Params& p, int nv, const float* P, int ns, const int* off) override {
         return s.create(p, nv, P, ns, off);
     }
@@ -135,6 +140,23 @@ struct Impl : Base {
     int nverts() const override { return s.n; }
     long num_pairs() const override { return s.num_pairs; }
     int error() const override { return s.error; }
+    void set_material(const float* ks, const float* qr) override { s.set_material(ks, qr); }
+    void set_material_t(const double* ks, const double* qr) override { s.set_material_t(ks, qr); }
+    double gradient(float dt, int nsteps, int K, const int* tips, int ntips, const double* targets, double* ksb,
+                    double* qrb) override {
+        std::vector<int> tp(tips, tips + ntips);
+        std::vector<T> tg(3 * ntips);
+        for (int k = 0; k < 3 * ntips; k++) tg[k] = (T)targets[k];
+        std::vector<T> kb;
+        std::vector<orc::R4<T>> qb;
+        const T loss = s.gradient(dt, nsteps, K, tp, tg, kb, qb);
+        for (int i = 0; i < s.n; i++) {
+            ksb[i] = (double)kb[i];
+            qrb[4 * i] = (double)qb[i].x, qrb[4 * i + 1] = (double)qb[i].y, qrb[4 * i + 2] = (double)qb[i].z,
+                     qrb[4 * i + 3] = (double)qb[i].w;
+        }
+        return (double)loss;
+    }
     void contacts(int* num, int* cols) const override {
         for (int i = 0; i < s.n; i++) {
             num[i] = (int)s.cols[i].size();
@@ -262,6 +284,19 @@ int oracle_num_verts(oracle_handle* h) { return h->b->nverts(); }
 long oracle_num_pairs(oracle_handle* h) { return h->b->num_pairs(); }
 int oracle_error(oracle_handle* h) { return h->b->error(); }
 void oracle_contacts(oracle_handle* h, int* num, int* cols) { h->b->contacts(num, cols); }
+int oracle_set_material(oracle_handle* h, const float* ks, const float* qr) {
+    h->b->set_material(ks, qr);
+    return 0;
+}
+int oracle_set_material64(oracle_handle* h, const double* ks, const double* qr) {
+    h->b->set_material_t(ks, qr);
+    return 0;
+}
+// Loss and dLoss/d(kStretch, qRest) of nsteps steps (checkpoints every K).
+double oracle_gradient(oracle_handle* h, float dt, int nsteps, int K, const int* tips, int ntips, const double* targets,
+                       double* ksb, double* qrb) {
+    return h->b->gradient(dt, nsteps, K, tips, ntips, targets, ksb, qrb);
+}
 
 // ---- LBVH restatement (lbvh_oracle.hpp) for the bit-exact tree tests
 struct oracle_lbvh {

@@ -57,6 +57,9 @@ SIGNATURES = {
     "oracle_num_pairs": (C.c_long, [_P]),
     "oracle_error": (C.c_int, [_P]),
     "oracle_contacts": (None, [_P, _IP, _IP]),
+    "oracle_set_material": (C.c_int, [_P, _FP, _FP]),
+    "oracle_set_material64": (C.c_int, [_P, _DP, _DP]),
+    "oracle_gradient": (C.c_double, [_P, C.c_float, C.c_int, C.c_int, _IP, C.c_int, _DP, _DP, _DP]),
 }
 
 _lib = None
@@ -209,6 +212,27 @@ class OracleSim:
     def solve_next_step(self, dt):
         self.lib.oracle_solve_next_step(self.h, dt)
 
+
+    def set_material(self, k_stretch=None, q_rest=None):
+        ks = None if k_stretch is None else np.ascontiguousarray(k_stretch, np.float32)
+        qr = None if q_rest is None else np.ascontiguousarray(q_rest, np.float32).reshape(-1)
+        self.lib.oracle_set_material(self.h, None if ks is None else _f(ks), None if qr is None else _f(qr))
+
+    def set_material64(self, k_stretch, q_rest):
+        """Per-vertex materials in the state's precision (call after upload)."""
+        ks = np.ascontiguousarray(k_stretch, np.float64)
+        qr = np.ascontiguousarray(q_rest, np.float64).reshape(-1)
+        self.lib.oracle_set_material64(self.h, ks.ctypes.data_as(_DP), qr.ctypes.data_as(_DP))
+
+    def gradient(self, dt, nsteps, ckpt_every, tips, targets):
+        """Loss and (dL/dkStretch[n], dL/dqRest[n,4]) in float64."""
+        tips = np.ascontiguousarray(tips, np.int32)
+        tg = np.ascontiguousarray(targets, np.float64).reshape(-1)
+        ksb = np.empty(self.n, np.float64)
+        qrb = np.empty((self.n, 4), np.float64)
+        loss = self.lib.oracle_gradient(self.h, dt, nsteps, ckpt_every, _i(tips), tips.size, tg.ctypes.data_as(_DP),
+                                        ksb.ctypes.data_as(_DP), qrb.ctypes.data_as(_DP))
+        return loss, ksb, qrb
 
     def num_pairs(self):
         return self.lib.oracle_num_pairs(self.h)

@@ -467,6 +467,20 @@ struct Sim {
         }
         return 0;
     }
+    // Per-vertex stretch stiffness and rest rotors (per-strand materials).
+    void set_material(const float* ks, const float* qr) {
+        for (int i = 0; i < n; i++) {
+            if (ks) kStretchF[i] = ks[i];
+            if (qr) qRestF[i] = R4<float>{qr[4 * i], qr[4 * i + 1], qr[4 * i + 2], qr[4 * i + 3]};
+        }
+    }
+    // Same, in the state's precision (after upload; for the FP64 gradient checks).
+    void set_material_t(const double* ks, const double* qr) {
+        for (int i = 0; i < n; i++) {
+            if (ks) kStretch[i] = (T)ks[i];
+            if (qr) qRest[i] = R4<T>{(T)qr[4 * i], (T)qr[4 * i + 1], (T)qr[4 * i + 2], (T)qr[4 * i + 3]};
+        }
+    }
     void pin(const int* ids, int m) {
         for (int k = 0; k < m; k++) invMassF[ids[k]] = 0;
     }

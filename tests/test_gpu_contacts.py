@@ -23,8 +23,8 @@ def _rel(a, b):
     return float(np.abs(a.astype(np.float64) - b).max() / np.abs(b).max())
 
 
-def test_bundle_contacts_match_oracle():
-    sc = scenes.bundle(6, 6, 32)
+def test_ribbon_contacts_match_oracle():
+    sc = scenes.ribbon(12, 32)
     rng = np.random.default_rng(3)
     vel = rng.uniform(-0.02, 0.02, size=sc.positions.shape).astype(np.float32)
     g, o = _pair(sc)
@@ -34,12 +34,13 @@ def test_bundle_contacts_match_oracle():
     for s in range(10):
         g.solve_next_step(DT)
         o.solve_next_step(DT)
+    gp, gv = g.get_pos_vel()  # raises if the device flagged an overflow
+    assert o.error() == 0
     gn, gc = g.get_contacts()
     on, oc = o.contacts()
     print(f"[gpu-contacts] contacts gpu {gn.sum()} oracle {on.sum()} pairs(oracle) {o.num_pairs()}", flush=True)
     assert on.sum() > 50
     assert np.array_equal(gn, on) and np.array_equal(gc, oc)
-    gp, gv = g.get_pos_vel()
     op, ov = o.get_pos_vel()
     ep, eq = _rel(gp, op), _rel(g.get_q(), o.get_q())
     print(f"[gpu-contacts] pos rel {ep:.3e} q rel {eq:.3e} differing pos words "
