@@ -163,6 +163,36 @@ int kscr_block_size(const kscr_handle* h);
  * fastest and returns it in best_block (may be NULL). */
 int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block);
 
+/* Per-vertex material: stretch stiffness (float[n]) and rest Darboux rotor
+ * (float[4n], x y z w; its norm is the bend stiffness).  Writes the values
+ * Sim::setYoung / setKStretch / setKBend compute into Vertex::kStretch and
+ * Sim::qRests (YarnBall.h:55, YBSimulator.h:22), for materials that vary per
+ * strand or per vertex.  Either pointer may be NULL (kept).  Takes effect on
+ * the device at once when the handle is uploaded. */
+int kscr_set_material(kscr_handle* h, const float* k_stretch, const float* q_rest);
+
+/* ---- Differentiation (no counterpart in the reference, which has no
+ * adjoint).  The reverse mode of the contact-free step, with
+ * checkpoint-and-recompute.
+ *
+ * kscr_adjoint_forward advances nsteps steps of dt (kscr_advance semantics)
+ * and keeps a device copy of the state every checkpoint_every steps.
+ * kscr_adjoint_backward then takes the adjoints of the final positions and
+ * velocities (float[3n] each, NULL = zero) back through the rollout: for each
+ * segment of checkpoint_every steps, last first, it restores the segment's
+ * checkpoint, re-runs the segment recording its tape, and sweeps its steps
+ * backwards.  It returns d(loss)/d(kStretch) (float[n]) and d(loss)/d(qRest)
+ * (float[4n]) and leaves the state at the end of the rollout.  Returns
+ * KSCR_ERR_UNSUPPORTED for contacts, step size limits or wind.  Steps follow
+ * kscr_advance (no SDF pass). */
+int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every);
+int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar, float* k_stretch_bar,
+                          float* q_rest_bar);
+/* Device time (ms) of the last kscr_adjoint_backward: re-recording the
+ * segments and the reverse sweeps, measured with HIP events on the handle's
+ * stream. */
+int kscr_adjoint_last_time(const kscr_handle* h, float* record_ms, float* sweep_ms);
+
 /* ---- Batched form (no counterpart in the reference: its closest analogue is
  * CantileverTestSolver, which steps independent solvers one after another,
  * src/solvers/CantileverTestSolver.cpp:106-111).  A batch advances many

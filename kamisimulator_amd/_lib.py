@@ -111,6 +111,10 @@ SIGNATURES = {
     "kscr_batch_block_size": (C.c_int, [_P]),
     "kscr_batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "kscr_get_contacts": (C.c_int, [_P, _IP, _IP]),
+    "kscr_set_material": (C.c_int, [_P, _FP, _FP]),
+    "kscr_adjoint_forward": (C.c_int, [_P, C.c_float, C.c_int, C.c_int]),
+    "kscr_adjoint_backward": (C.c_int, [_P, _FP, _FP, _FP, _FP]),
+    "kscr_adjoint_last_time": (C.c_int, [_P, _FP, _FP]),
     "kscr_lbvh_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "kscr_lbvh_destroy": (None, [_P]),
     "kscr_lbvh_compute": (C.c_int, [_P, _FP, C.c_int]),
@@ -302,6 +306,41 @@ class Sim:
         cols = np.empty((self.n, 16), np.int32)
         self._check(self.lib.kscr_get_contacts(self.h, iptr(num), iptr(cols)))
         return num, cols
+
+    def set_material(self, k_stretch=None, q_rest=None):
+        """Per-vertex kStretch (n) and qRest (n, 4)."""
+        ks = None if k_stretch is None else np.ascontiguousarray(k_stretch, dtype=np.float32).reshape(-1)
+        qr = None if q_rest is None else np.ascontiguousarray(q_rest, dtype=np.float32).reshape(-1)
+        if ks is not None and ks.size != self.n:
+            raise ValueError("k_stretch must hold one value per vertex")
+        if qr is not None and qr.size != 4 * self.n:
+            raise ValueError("q_rest must hold one rotor per vertex")
+        self._check(self.lib.kscr_set_material(self.h, None if ks is None else fptr(ks),
+                                               None if qr is None else fptr(qr)))
+
+    def adjoint_forward(self, dt, nsteps, checkpoint_every):
+        """nsteps steps of dt keeping a checkpoint every checkpoint_every steps."""
+        self._check(self.lib.kscr_adjoint_forward(self.h, dt, nsteps, checkpoint_every))
+
+    def adjoint_backward(self, x_bar=None, v_bar=None):
+        """Adjoints of the final (x, v) -> (d/dkStretch (n,), d/dqRest (n, 4))."""
+        xb = None if x_bar is None else np.ascontiguousarray(x_bar, dtype=np.float32).reshape(-1)
+        vb = None if v_bar is None else np.ascontiguousarray(v_bar, dtype=np.float32).reshape(-1)
+        for a in (xb, vb):
+            if a is not None and a.size != 3 * self.n:
+                raise ValueError("adjoint seeds must hold 3 floats per vertex")
+        ksb = np.empty(self.n, np.float32)
+        qrb = np.empty((self.n, 4), np.float32)
+        self._check(self.lib.kscr_adjoint_backward(self.h, None if xb is None else fptr(xb),
+                                                   None if vb is None else fptr(vb), fptr(ksb), fptr(qrb)))
+        return ksb, qrb
+
+    def adjoint_time(self):
+        """(record ms, reverse sweep ms) of the last adjoint_backward."""
+        r = C.c_float(0)
+        w = C.c_float(0)
+        self._check(self.lib.kscr_adjoint_last_time(self.h, C.byref(r), C.byref(w)))
+        return r.value, w.value
 
     def autotune(self, dt, steps=3):
         best = C.c_int(0)

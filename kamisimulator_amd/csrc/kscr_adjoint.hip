@@ -1,0 +1,345 @@
+// kscr_adjoint.hip -- reverse mode (adjoint) of the contact-free Stable
+// Cosserat Rods step, one fused kernel per step.
+//
+// The forward step is the Jacobi-snapshot VBD step of kscr_step.hip
+// (YB/sim/iteration.cu:8-74, YB/sim/cosserat.cu:25-337).  Its tape -- x, v, u
+// and y of every substep and every iterate dx^k, q^k -- is written by
+// step_record_kernel.  This kernel walks the substeps and iterations of one
+// step backwards, strands packed in workgroups exactly as in the forward, the
+// adjoint state of each vertex in registers and the neighbour contributions
+// exchanged through LDS.  The per-vertex accumulation order is the order in
+// which the sequential restatement adds the same terms, so FP32 results are
+// reproducible bit for bit on the CPU.
+//
+// Differentiated quantities: the state (x, v, u, q) and the per-vertex
+// parameters kStretch (stretch stiffness) and qRest (rest Darboux rotor, whose
+// norm is the bend stiffness).  The quaternion sign switch of
+// quaternionLambdaItr (cosserat.cu:320, :328) and the clamp of the adaptive
+// initial guess (iteration.cu:50) are piecewise constant.
+#include "kscr_device.h"
+#include "kscr_math.h"
+#include "kscr_solve.h"
+
+namespace kscr {
+namespace {
+
+KS_FN f3 neg(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+KS_FN f4 zero4() { return f4{0.f, 0.f, 0.f, 0.f}; }
+KS_FN f3 zero3() { return f3{0.f, 0.f, 0.f}; }
+
+// Transposed Jacobian of rotate(q, e1) with respect to q, applied to rb.
+__device__ inline f4 rot_e1_vjp(f4 q, f3 rb) {
+    const f3 e{1.0f, 0.0f, 0.0f};
+    const f3 qv = vec(q);
+    const float w = q.w;
+    const float wb = dot(rb, (2.0f * w) * e + 2.0f * cross(qv, e));
+    const f3 qvb = ((-2.0f) * dot(rb, e)) * qv + (2.0f * w) * cross(e, rb) + (2.0f * dot(rb, qv)) * e +
+                   (2.0f * dot(qv, e)) * rb;
+    return f4{qvb.x, qvb.y, qvb.z, wb};
+}
+
+__device__ inline f4 ld4(const float4* p, size_t i) { return to4(p[i]); }
+__device__ inline void st4(float4* p, size_t i, f4 a) { p[i] = make_float4(a.x, a.y, a.z, a.w); }
+
+template <int BLOCK>
+struct AdjLds {
+    float eb[3][BLOCK];   // d(segment vector e) of the orientation solve
+    float4 qpb[BLOCK];    // contribution to the next vertex's rotor adjoint
+    float4 qmb[BLOCK];    // contribution to the previous vertex's rotor adjoint
+    float4 qrp[BLOCK];    // contribution to the previous vertex's qRest adjoint
+    int mask[BLOCK];      // 1: orientation solved, 2: previous term, 4: next term
+    float4 fbD[BLOCK];    // adjoint of the position solve's force, and of D
+    float s[3][BLOCK];    // stretch-constraint adjoint of the segment
+};
+
+template <int BLOCK>
+__device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbeg, const int vend, AdjLds<BLOCK>& L) {
+    const int t = threadIdx.x;
+    const int i = vbeg + t;
+    const bool act = i < vend;
+    const int n = a.tape.n;
+    const int K = a.num_itr;
+    const float4 mi = act ? a.mat[i] : make_float4(0.f, 1.f, 0.f, 0.f);
+    const uint32_t fl = act ? __float_as_uint(mi.w) : 0u;
+    const bool free = mi.x != 0.0f;
+    const bool has_prev = (fl & FLAG_PREV) != 0, has_next = (fl & FLAG_NEXT) != 0;
+    const bool has_next_ori = (fl & FLAG_NEXT_ORI) != 0;
+    const bool pass = (fl & FLAG_FIX_ORI) || !has_next;
+    const float4 mp = has_prev ? a.mat[i - 1] : make_float4(0.f, 1.f, 0.f, 0.f);
+    const float im = mi.x, lr = mi.y, ks = mi.z;
+    const float lrp = mp.y, ksp = mp.z;
+    const f4 qr = act ? ld4(a.q_rest, i) : zero4();
+    const f4 qrm = has_prev ? ld4(a.q_rest, i - 1) : zero4();
+
+    // adjoint state of the step's outputs
+    f3 xb = act ? ld3(a.xb, i) : zero3();
+    f3 vb = act ? ld3(a.vb, i) : zero3();
+    f3 ub = act ? ld3(a.ub, i) : zero3();
+    f4 qb = act ? ld4(a.qb, i) : zero4();
+    float ksb = act ? a.ksb[i] : 0.0f;
+    f4 qrb = act ? ld4(a.qrb, i) : zero4();
+
+    const float hh = a.h;
+    const float damp = a.damping / hh;
+    const float invH = 1.0f / hh;
+    const float decay = 1.0f - a.velocity_decay * hh;
+    const f3 e1v{1.0f, 0.0f, 0.0f};
+    const f4 e1{1.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int sub = a.num_substeps - 1; sub >= 0; sub--) {
+        const size_t so = (size_t)sub * n;
+        const f3 x = act ? ld3(a.tape.x + so * 3, i) : zero3();
+        const f3 xn = has_next ? ld3(a.tape.x + so * 3, i + 1) : zero3();
+        const f3 xp = has_prev ? ld3(a.tape.x + so * 3, i - 1) : zero3();
+        const f3 y = act ? ld3(a.tape.y + so * 3, i) : zero3();
+
+        // ---- endItr: x' = x + dx^K, v' = dx^K / h * decay (free vertices)
+        f3 dxb = xb + (free ? vb * (invH * decay) : zero3());
+        if (free) vb = zero3();
+        f3 yb = zero3();
+
+        for (int k = K - 1; k >= 0; k--) {
+            const size_t ok = ((size_t)sub * (K + 1) + k) * n;  // iterate k
+            const size_t on = ok + n;                            // iterate k + 1
+            const float* DXk = a.tape.dx + ok * 3;
+            const float* DXn = a.tape.dx + on * 3;
+            const float4* Qk = a.tape.q + ok;
+            const f3 dx = act ? ld3(DXk, i) : zero3();
+            const f3 dxn_n = has_next ? ld3(DXn, i + 1) : zero3();
+            const f3 dxn = act ? ld3(DXn, i) : zero3();
+            const f3 dx_n = has_next ? ld3(DXk, i + 1) : zero3();
+            const f3 dx_p = has_prev ? ld3(DXk, i - 1) : zero3();
+            const f4 Q = act ? ld4(Qk, i) : zero4();
+            const f4 Qm = has_prev ? ld4(Qk, i - 1) : zero4();
+            const f4 Qp = has_next_ori ? ld4(Qk, i + 1) : zero4();
+
+            // ---- phase A: orientation solve of segment i (quaternionLambdaItr)
+            f3 eb = zero3();
+            f4 qpb = zero4(), qmb = zero4(), qrp = zero4();
+            int mask = 0;
+            if (act && !pass) {
+                mask = 1;
+                const f3 e = (xn - x) + (dxn_n - dxn);
+                const float sc = (-2.0f * ks) / lr;
+                const f3 vv = e * sc;
+                f4 b = zero4(), bm = zero4(), bp = zero4();
+                float phim = 0.0f, phip = 0.0f;
+                if (has_prev) {
+                    phim = dot(rmul(conj(Qm), Q), qrm) > 0 ? 1.0f : -1.0f;
+                    bm = rmul(Qm, qrm);
+                    b = b + phim * bm;
+                }
+                if (has_next_ori) {
+                    phip = dot(rmul(conj(Q), Qp), qr) > 0 ? 1.0f : -1.0f;
+                    bp = rmul(Qp, conj(qr));
+                    b = b + phip * bp;
+                }
+                const float lv = length(vv), lb = length(b);
+                const float lambda = lv + lb;
+                const f4 V = mk4(vv, 0.0f);
+                const f4 m = rmul(V, b);
+                const f4 tq = rmul(m, e1);
+                const f4 w = tq + lambda * b;
+                const float lw = length(w);
+                const f4 qn = (1.0f / lw) * w;
+                const f4 g = qb;
+                const f4 wb = (1.0f / lw) * (g + (-dot(qn, g)) * qn);
+                f4 bb = lambda * wb;
+                const float lamb = dot(wb, b);
+                const f4 mb = zero4() + rmul(wb, conj(e1));
+                const f4 Vb = zero4() + rmul(mb, conj(b));
+                bb = bb + rmul(conj(V), mb);
+                f3 vvb = vec(Vb);
+                if (lv > 0) vvb = vvb + (lamb / lv) * vv;
+                if (lb > 0) bb = bb + (lamb / lb) * b;
+                if (has_prev) {
+                    const f4 c = phim * bb;
+                    qmb = zero4() + rmul(c, conj(qrm));
+                    qrp = zero4() + rmul(conj(Qm), c);
+                    mask |= 2;
+                }
+                if (has_next_ori) {
+                    const f4 c = phip * bb;
+                    qpb = zero4() + rmul(c, conj(conj(qr)));
+                    const f4 cqb = zero4() + rmul(conj(Qp), c);
+                    qrb = qrb + conj(cqb);
+                    mask |= 4;
+                }
+                eb = vvb * sc;
+                ksb += dot(vvb, e) * (-2.0f / lr);
+            }
+            L.eb[0][t] = eb.x;
+            L.eb[1][t] = eb.y;
+            L.eb[2][t] = eb.z;
+            L.qpb[t] = make_float4(qpb.x, qpb.y, qpb.z, qpb.w);
+            L.qmb[t] = make_float4(qmb.x, qmb.y, qmb.z, qmb.w);
+            L.qrp[t] = make_float4(qrp.x, qrp.y, qrp.z, qrp.w);
+            L.mask[t] = mask;
+            __syncthreads();
+            // gather, in the order the sequential sweep adds the terms
+            f4 qb2 = zero4();
+            if (has_prev) {
+                const int mm = L.mask[t - 1];
+                if ((mm & 4)) qb2 = qb2 + to4(L.qpb[t - 1]);
+                if (mm & 1) {
+                    const f3 ebm{L.eb[0][t - 1], L.eb[1][t - 1], L.eb[2][t - 1]};
+                    dxb = dxb + ebm;
+                    xb = xb + ebm;
+                }
+            }
+            if (act && pass) qb2 = qb2 + qb;
+            if (mask & 1) {
+                dxb = dxb - eb;
+                xb = xb - eb;
+            }
+            if (has_next) {
+                const int mn = L.mask[t + 1];
+                if (mn & 2) {
+                    qb2 = qb2 + to4(L.qmb[t + 1]);
+                    qrb = qrb + to4(L.qrp[t + 1]);
+                }
+            }
+
+            // ---- phase B1: position solve of vertex i (cosseratItr)
+            f3 dxb2 = dxb;
+            f3 fb = zero3();
+            float Db = 0.0f;
+            f3 c_next = zero3();
+            float dd_prev = 0.0f;
+            if (has_next) {
+                const float il = 1.0f / lr;
+                c_next = ((xn - x) + (dx_n - dx)) * il - rotate(Q, e1v);
+            }
+            if (has_prev) {
+                const float il = 1.0f / lrp;
+                const float kk = ksp * il;
+                dd_prev = kk * il;
+            }
+            if (act && free) {
+                const float a_in = 1.0f / (im * hh * hh);
+                float D = 0.0f;
+                const float c1 = 1.0f / (hh * hh * im);
+                f3 f = c1 * (y - dx);
+                if (has_next) {
+                    const float il = 1.0f / lr;
+                    const float kk = ks * il, dd = kk * il;
+                    f = f + (kk * c_next - (damp * dd) * dx);
+                    D += dd * (1.0f + damp);
+                }
+                if (has_prev) {
+                    const float il = 1.0f / lrp;
+                    const f3 c = ((x - xp) + (dx - dx_p)) * il - rotate(Qm, e1v);
+                    const float kk = ksp * il, dd = kk * il;
+                    f = f + ((-kk) * c - (damp * dd) * dx);
+                    D += dd * (1.0f + damp);
+                }
+                const float hd = a_in + D;
+                const f3 delta = sym_solve(hd, hd, hd, a_in, a_in, a_in, f);
+                fb = sym_solve(hd, hd, hd, a_in, a_in, a_in, a.acceleration_ratio * dxb);
+                Db = -dot(fb, delta);
+                yb = yb + c1 * fb;
+                dxb2 = dxb2 - c1 * fb;
+            }
+            L.fbD[t] = make_float4(fb.x, fb.y, fb.z, Db);
+            __syncthreads();
+
+            // ---- phase B2: stretch constraint of segment i
+            f3 s_own = zero3();
+            f4 qrot = zero4();
+            float dd_own = 0.0f;
+            if (has_next) {
+                const float4 nb = L.fbD[t + 1];
+                const f3 fbn{nb.x, nb.y, nb.z};
+                const float Dbn = nb.w;
+                const float il = 1.0f / lr;
+                const float kk = ks * il, dd = kk * il;
+                dd_own = dd;
+                const f3 cb = kk * (fb - fbn);
+                s_own = cb * il;
+                qrot = rot_e1_vjp(Q, neg(cb));
+                ksb += dot(c_next, fb - fbn) * il - damp * (dot(dx, fb) + dot(dx_n, fbn)) * il * il +
+                       (Db + Dbn) * (1.0f + damp) * il * il;
+            }
+            L.s[0][t] = s_own.x;
+            L.s[1][t] = s_own.y;
+            L.s[2][t] = s_own.z;
+            __syncthreads();
+            if (has_prev) {
+                const f3 sp{L.s[0][t - 1], L.s[1][t - 1], L.s[2][t - 1]};
+                xb = xb + sp;
+                dxb2 = dxb2 + sp;
+                dxb2 = dxb2 - (damp * dd_prev) * fb;
+            }
+            if (has_next) {
+                xb = xb - s_own;
+                dxb2 = dxb2 - s_own;
+                qb2 = qb2 + qrot;
+                dxb2 = dxb2 - (damp * dd_own) * fb;
+            }
+            dxb = dxb2;
+            qb = qb2;
+        }
+
+        // ---- initItr: y = x + h v + h^2 ext (free), dx^0 = h v (+ h^2 s ext), u' = v
+        f3 vbn = vb + ub;
+        vbn = vbn + hh * dxb;
+        f3 ubn = zero3();
+        if (act && free) {
+            vbn = vbn + hh * yb;
+            if (a.ext2 > 0) {
+                const f3 ext{a.ext[0], a.ext[1], a.ext[2]};
+                const f3 v = ld3(a.tape.v + so * 3, i);
+                const f3 u = ld3(a.tape.u + so * 3, i);
+                const f3 acc = (v - u) / a.last_h;
+                const float sraw = dot(acc, ext) / a.ext2;
+                if (sraw > 0 && sraw < 1) {
+                    const float sb = hh * hh * dot(dxb, ext);
+                    const f3 ab = (sb / a.ext2) * ext;
+                    vbn = vbn + ab / a.last_h;
+                    ubn = ubn - ab / a.last_h;
+                }
+            }
+        }
+        vb = vbn;
+        ub = ubn;
+    }
+    if (act) {
+        st3(a.xb, i, xb);
+        st3(a.vb, i, vb);
+        st3(a.ub, i, ub);
+        st4(a.qb, i, qb);
+        a.ksb[i] = ksb;
+        st4(a.qrb, i, qrb);
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void step_adjoint_kernel(AdjointArgs a, const int2* __restrict__ packs) {
+    __shared__ AdjLds<BLOCK> lds;
+    const int2 pk = packs[blockIdx.x];
+    adjoint_pack<BLOCK>(a, pk.x, pk.y, lds);
+}
+
+}  // namespace
+
+hipError_t launch_step_adjoint(const AdjointArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream) {
+    if (num_packs <= 0) return hipSuccess;
+    const dim3 grid(num_packs);
+    switch (block) {
+#define KSCR_CASE(B)                                                                          \
+    case B:                                                                                   \
+        hipLaunchKernelGGL(step_adjoint_kernel<B>, grid, dim3(B), 0, stream, a, packs);       \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace kscr

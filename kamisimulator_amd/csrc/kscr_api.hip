@@ -54,6 +54,28 @@ struct DevBuf {
     }
     ~DevBuf() { release(); }
 };
+// Device state of a differentiable rollout (kscr_adjoint_forward/backward).
+struct AdjointRun {
+    float dt = 0;
+    int nsteps = 0, every = 0, substeps = 0, num_itr = 0;
+    bool ready = false;
+    // checkpoints: state at steps 0, every, 2 every, ... and at the end
+    DevBuf<float> ck_x, ck_v, ck_u;
+    DevBuf<float4> ck_q;
+    struct Clock {
+        float h, last_h, time;
+        long counter;
+    };
+    std::vector<Clock> clocks;
+    Clock end_clock{};
+    // tape of one segment of `every` steps
+    DevBuf<float> t_x, t_v, t_u, t_y, t_dx;
+    DevBuf<float4> t_q;
+    // adjoint state
+    DevBuf<float> xb, vb, ub, ksb;
+    DevBuf<float4> qb, qrb;
+    float record_ms = 0, sweep_ms = 0;
+};
 }  // namespace
 
 struct kscr_handle {
@@ -86,6 +108,7 @@ struct kscr_handle {
     // hair-hair contacts
     ContactDev contact;
     float last_bvh_rebuild = INFINITY;
+    AdjointRun adj;
     std::string err;
     ~kscr_handle() { contact_release(contact); }
 
@@ -305,6 +328,59 @@ int longest_strand(const kscr_handle* h) {
     int longest = 0;
     for (size_t s = 0; s + 1 < h->strand_off.size(); s++) longest = std::max(longest, h->strand_off[s + 1] - h->strand_off[s]);
     return longest;
+}
+
+// Re-uploads (invMass, lRest, kStretch, flags) and qRest from the host mirror,
+// leaving the dynamic state on the device untouched.
+int upload_material(kscr_handle* h) {
+    const int n = h->n;
+    std::vector<float4> mat(n), qr(n);
+    for (int i = 0; i < n; i++) {
+        float fb;
+        std::memcpy(&fb, &h->flags[i], 4);
+        mat[i] = make_float4(h->inv_mass[i], h->l_rest[i], h->k_stretch[i], fb);
+        qr[i] = make_float4(h->q_rest[i].x, h->q_rest[i].y, h->q_rest[i].z, h->q_rest[i].w);
+    }
+    hipError_t e;
+    if ((e = hipMemcpyAsync(h->d_mat.p, mat.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->d_qr.p, qr.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return h->hip_fail(e, "upload material");
+    return KSCR_OK;
+}
+
+// Copies the dynamic state (x, v, u, q) into checkpoint slot k (save) or back.
+hipError_t checkpoint_copy(kscr_handle* h, size_t k, bool save) {
+    AdjointRun& A = h->adj;
+    const size_t n = h->n, n3 = 3 * n;
+    const hipMemcpyKind kind = hipMemcpyDeviceToDevice;
+    float* x = A.ck_x.p + k * n3;
+    float* v = A.ck_v.p + k * n3;
+    float* u = A.ck_u.p + k * n3;
+    float4* q = A.ck_q.p + k * n;
+    hipError_t e;
+    if (save) {
+        if ((e = hipMemcpyAsync(x, h->d_x.p, n3 * 4, kind, h->stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(v, h->d_v.p, n3 * 4, kind, h->stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(u, h->d_u.p, n3 * 4, kind, h->stream)) != hipSuccess) return e;
+        return hipMemcpyAsync(q, h->d_q.p, n * 16, kind, h->stream);
+    }
+    if ((e = hipMemcpyAsync(h->d_x.p, x, n3 * 4, kind, h->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(h->d_v.p, v, n3 * 4, kind, h->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(h->d_u.p, u, n3 * 4, kind, h->stream)) != hipSuccess) return e;
+    return hipMemcpyAsync(h->d_q.p, q, n * 16, kind, h->stream);
+}
+
+AdjointRun::Clock clock_of(const kscr_handle* h) { return AdjointRun::Clock{h->h, h->last_h, h->time, h->step_counter}; }
+void set_clock(kscr_handle* h, const AdjointRun::Clock& c) {
+    h->h = c.h, h->last_h = c.last_h, h->time = c.time, h->step_counter = c.counter;
+}
+
+int adjoint_supported(kscr_handle* h) {
+    if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit)
+        return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the contact-free step (no contacts, no step size limit)");
+    if (h->p.wind_peak != 0) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the step without wind");
+    return KSCR_OK;
 }
 
 }  // namespace
@@ -720,6 +796,187 @@ int kscr_block_size(const kscr_handle* h) { return h ? h->block : -1; }
 int kscr_last_error(const kscr_handle* h, char* buf, size_t n) {
     if (!h || !buf || n == 0) return KSCR_ERR_INVALID_ARGUMENT;
     std::snprintf(buf, n, "%s", h->err.c_str());
+    return KSCR_OK;
+}
+
+int kscr_set_material(kscr_handle* h, const float* k_stretch, const float* q_rest) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->configured) return h->fail(KSCR_ERR_NOT_CONFIGURED, "No rest length. Must call configure()");
+    for (int i = 0; i < h->n; i++) {
+        if (k_stretch) h->k_stretch[i] = k_stretch[i];
+        if (q_rest) h->q_rest[i] = f4{q_rest[4 * i], q_rest[4 * i + 1], q_rest[4 * i + 2], q_rest[4 * i + 3]};
+    }
+    if (!h->uploaded) return KSCR_OK;
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    return upload_material(h);
+}
+
+// ------------------------------------------------------------------ adjoint
+int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every) {
+    if (!h || !(dt > 0) || nsteps <= 0 || checkpoint_every <= 0) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    int rc = validate_meta(h);
+    if (rc) return rc;
+    if ((rc = adjoint_supported(h))) return rc;
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    AdjointRun& A = h->adj;
+    A.ready = false;
+    const size_t n = h->n;
+    const size_t slots = (size_t)(nsteps + checkpoint_every - 1) / checkpoint_every + 1;  // + end state
+    if (A.ck_q.n < slots * n) {
+        if ((e = A.ck_x.alloc(slots * n * 3)) != hipSuccess || (e = A.ck_v.alloc(slots * n * 3)) != hipSuccess ||
+            (e = A.ck_u.alloc(slots * n * 3)) != hipSuccess || (e = A.ck_q.alloc(slots * n)) != hipSuccess)
+            return h->hip_fail(e, "hipMalloc checkpoints");
+    }
+    A.clocks.clear();
+    for (int s = 0; s < nsteps; s++) {
+        if (s % checkpoint_every == 0) {
+            if ((e = checkpoint_copy(h, (size_t)(s / checkpoint_every), true)) != hipSuccess)
+                return h->hip_fail(e, "save checkpoint");
+            A.clocks.push_back(clock_of(h));
+        }
+        if ((rc = launch_step(h, dt, false)) != KSCR_OK) return rc;
+    }
+    A.dt = dt;
+    A.nsteps = nsteps;
+    A.every = checkpoint_every;
+    A.substeps = std::max(1, (int)std::ceil(dt / h->p.max_h));
+    A.num_itr = h->p.num_itr;
+    A.ready = true;
+    return KSCR_OK;
+}
+
+int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar, float* k_stretch_bar,
+                          float* q_rest_bar) {
+    if (!h || !k_stretch_bar || !q_rest_bar) return KSCR_ERR_INVALID_ARGUMENT;
+    AdjointRun& A = h->adj;
+    if (!A.ready) return h->fail(KSCR_ERR_NOT_CONFIGURED, "kscr_adjoint_forward has not been run");
+    if (h->p.num_itr != A.num_itr || std::max(1, (int)std::ceil(A.dt / h->p.max_h)) != A.substeps)
+        return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr or max_h changed since the forward pass");
+    int rc = adjoint_supported(h);
+    if (rc) return rc;
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    const size_t n = h->n, n3 = 3 * n;
+    const size_t S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = (size_t)A.every;
+    // tape of one segment: `every` steps of S substeps
+    const size_t per_step_sub = S * n, per_step_itr = S * I1 * n;
+    if (A.t_q.n < every * per_step_itr || A.t_x.n < every * per_step_sub * 3) {
+        if ((e = A.t_x.alloc(every * per_step_sub * 3)) != hipSuccess ||
+            (e = A.t_v.alloc(every * per_step_sub * 3)) != hipSuccess ||
+            (e = A.t_u.alloc(every * per_step_sub * 3)) != hipSuccess ||
+            (e = A.t_y.alloc(every * per_step_sub * 3)) != hipSuccess ||
+            (e = A.t_dx.alloc(every * per_step_itr * 3)) != hipSuccess || (e = A.t_q.alloc(every * per_step_itr)) != hipSuccess)
+            return h->hip_fail(e, "hipMalloc tape");
+    }
+    if (A.qb.n != n) {
+        if ((e = A.xb.alloc(n3)) != hipSuccess || (e = A.vb.alloc(n3)) != hipSuccess || (e = A.ub.alloc(n3)) != hipSuccess ||
+            (e = A.ksb.alloc(n)) != hipSuccess || (e = A.qb.alloc(n)) != hipSuccess || (e = A.qrb.alloc(n)) != hipSuccess)
+            return h->hip_fail(e, "hipMalloc adjoint state");
+    }
+    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+    if ((e = x_bar ? hipMemcpyAsync(A.xb.p, x_bar, n3 * 4, h2d, h->stream) : hipMemsetAsync(A.xb.p, 0, n3 * 4, h->stream)) !=
+            hipSuccess ||
+        (e = v_bar ? hipMemcpyAsync(A.vb.p, v_bar, n3 * 4, h2d, h->stream) : hipMemsetAsync(A.vb.p, 0, n3 * 4, h->stream)) !=
+            hipSuccess ||
+        (e = hipMemsetAsync(A.ub.p, 0, n3 * 4, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(A.ksb.p, 0, n * 4, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(A.qb.p, 0, n * 16, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(A.qrb.p, 0, n * 16, h->stream)) != hipSuccess)
+        return h->hip_fail(e, "seed adjoint state");
+    const int nseg = (int)A.clocks.size();
+    const size_t end_slot = (size_t)nseg;
+    if ((e = checkpoint_copy(h, end_slot, true)) != hipSuccess) return h->hip_fail(e, "save end state");
+    const AdjointRun::Clock end_clock = clock_of(h);
+
+    AdjointArgs aa{};
+    aa.mat = h->d_mat.p;
+    aa.q_rest = h->d_qr.p;
+    aa.num_substeps = A.substeps;
+    aa.num_itr = A.num_itr;
+    aa.damping = h->p.damping;
+    aa.velocity_decay = h->p.velocity_decay;
+    aa.acceleration_ratio = h->p.acceleration_ratio;
+    for (int k = 0; k < 3; k++) aa.ext[k] = h->p.gravity[k] * h->p.external_force_multiplier;
+    aa.ext2 = aa.ext[0] * aa.ext[0] + aa.ext[1] * aa.ext[1] + aa.ext[2] * aa.ext[2];
+    aa.xb = A.xb.p;
+    aa.vb = A.vb.p;
+    aa.ub = A.ub.p;
+    aa.qb = A.qb.p;
+    aa.ksb = A.ksb.p;
+    aa.qrb = A.qrb.p;
+    auto tape_of = [&](size_t j) {
+        TapeArgs t{};
+        t.x = A.t_x.p + j * per_step_sub * 3;
+        t.v = A.t_v.p + j * per_step_sub * 3;
+        t.u = A.t_u.p + j * per_step_sub * 3;
+        t.y = A.t_y.p + j * per_step_sub * 3;
+        t.dx = A.t_dx.p + j * per_step_itr * 3;
+        t.q = A.t_q.p + j * per_step_itr;
+        t.n = (int)n;
+        return t;
+    };
+    std::vector<hipEvent_t> ev(3 * (size_t)nseg, nullptr);
+    for (auto& x : ev)
+        if ((e = hipEventCreate(&x)) != hipSuccess) {
+            rc = h->hip_fail(e, "hipEventCreate");
+            break;
+        }
+    std::vector<float> hs(every), lhs(every);
+    for (int seg = nseg - 1; seg >= 0 && rc == KSCR_OK; seg--) {
+        if ((e = checkpoint_copy(h, (size_t)seg, false)) != hipSuccess) {
+            rc = h->hip_fail(e, "restore checkpoint");
+            break;
+        }
+        set_clock(h, A.clocks[seg]);
+        const int s0 = seg * A.every, s1 = std::min(A.nsteps, s0 + A.every);
+        (void)hipEventRecord(ev[3 * seg], h->stream);
+        for (int s = s0; s < s1; s++) {
+            const StepArgs sa = step_args(h, A.dt, false);
+            hs[s - s0] = sa.h;
+            lhs[s - s0] = sa.last_h;
+            if ((e = launch_step_record(sa, tape_of(s - s0), h->d_packs.p, h->num_packs, h->block, h->stream)) !=
+                hipSuccess) {
+                rc = h->hip_fail(e, "record kernel launch");
+                break;
+            }
+        }
+        (void)hipEventRecord(ev[3 * seg + 1], h->stream);
+        for (int s = s1 - 1; s >= s0 && rc == KSCR_OK; s--) {
+            aa.tape = tape_of(s - s0);
+            aa.h = hs[s - s0];
+            aa.last_h = lhs[s - s0];
+            if ((e = launch_step_adjoint(aa, h->d_packs.p, h->num_packs, h->block, h->stream)) != hipSuccess)
+                rc = h->hip_fail(e, "adjoint kernel launch");
+        }
+        (void)hipEventRecord(ev[3 * seg + 2], h->stream);
+    }
+    if ((e = checkpoint_copy(h, end_slot, false)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "restore end state");
+    set_clock(h, end_clock);
+    if (rc == KSCR_OK &&
+        ((e = hipMemcpyAsync(k_stretch_bar, A.ksb.p, n * 4, hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+         (e = hipMemcpyAsync(q_rest_bar, A.qrb.p, n * 16, hipMemcpyDeviceToHost, h->stream)) != hipSuccess))
+        rc = h->hip_fail(e, "download gradient");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "hipStreamSynchronize");
+    A.record_ms = A.sweep_ms = 0;
+    for (int seg = 0; seg < nseg && rc == KSCR_OK; seg++) {
+        float r = 0, w = 0;
+        (void)hipEventElapsedTime(&r, ev[3 * seg], ev[3 * seg + 1]);
+        (void)hipEventElapsedTime(&w, ev[3 * seg + 1], ev[3 * seg + 2]);
+        A.record_ms += r;
+        A.sweep_ms += w;
+    }
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    return rc;
+}
+
+int kscr_adjoint_last_time(const kscr_handle* h, float* record_ms, float* sweep_ms) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (record_ms) *record_ms = h->adj.record_ms;
+    if (sweep_ms) *sweep_ms = h->adj.sweep_ms;
     return KSCR_OK;
 }
 

@@ -340,9 +340,11 @@ hipError_t contact_reserve(ContactDev& c, int n) {
     contact_release(c);
     hipError_t e;
     const size_t n3 = (size_t)n * 3;
-    // Each segment overlaps a handful of non-neighbour boxes at most in
-    // practice; 4 pairs per segment leaves a wide margin (overflow is flagged).
-    c.pair_cap = 4 * n;
+    // Broad-phase pair buffer of numVerts * MAX_COLLISIONS_PER_SEGMENT pairs,
+    // the size YarnBall gives it (YBSimulator.cu:449).  The reference truncates
+    // silently beyond it; here an overflow is flagged (KSCR_ERR_MAX_COLLISIONS).
+    if (n > INT32_MAX / kMaxCols) return hipErrorInvalidValue;
+    c.pair_cap = kMaxCols * n;
     if ((e = dalloc(c.dx[0], n3)) != hipSuccess || (e = dalloc(c.dx[1], n3)) != hipSuccess ||
         (e = dalloc(c.y, n3)) != hipSuccess || (e = dalloc(c.q2, n)) != hipSuccess ||
         (e = dalloc(c.max_step, n)) != hipSuccess || (e = dalloc(c.num_cols, n)) != hipSuccess ||

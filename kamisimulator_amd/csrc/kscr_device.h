@@ -51,10 +51,53 @@ struct StepArgs {
     SdfArgs sdf;
 };
 
+// Tape of the differentiable step: for every substep of one recorded step,
+// x, v, u at the substep start, y, and every iterate dx^k, q^k (k = 0..numItr).
+// Arrays are indexed [(sub * (num_itr + 1) + k) * n + i] (iterates) and
+// [sub * n + i] (per-substep values), float3 packed as 3 floats.
+struct TapeArgs {
+    float* x;
+    float* v;
+    float* u;
+    float* y;
+    float* dx;
+    float4* q;
+    int n;
+};
+
+// Arguments of the reverse sweep over one recorded step.  xb..qrb hold the
+// adjoints of the step's outputs on entry and of its inputs on exit (ksb and
+// qrb accumulate over all steps).
+struct AdjointArgs {
+    const float4* mat;
+    const float4* q_rest;
+    TapeArgs tape;
+    int num_substeps;
+    int num_itr;
+    float h;
+    float last_h;
+    float damping;
+    float velocity_decay;
+    float acceleration_ratio;
+    float ext[3];  // gravity * external_force_multiplier
+    float ext2;    // |ext|^2
+    float* xb;
+    float* vb;
+    float* ub;
+    float4* qb;
+    float* ksb;
+    float4* qrb;
+};
+
 // One launch advances every strand of the packs by num_substeps substeps (and
 // applies the SDF pass afterwards when sdf_on).  block is the workgroup size
 // in vertices; packs[p] = [begin, end) of workgroup p.
 hipError_t launch_step_fused(const StepArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream);
+// Same step, writing the tape of the step (contact-free step, no SDF pass).
+hipError_t launch_step_record(const StepArgs& a, const TapeArgs& tape, const int2* packs, int num_packs, int block,
+                              hipStream_t stream);
+// Reverse sweep over one recorded step.
+hipError_t launch_step_adjoint(const AdjointArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream);
 // Batched form: packs[p] = (begin, end, groom, 0) index the device table of
 // per-groom arguments.
 hipError_t launch_step_batch(const StepArgs* grooms, const int4* packs, int num_packs, int block, hipStream_t stream);

@@ -33,9 +33,13 @@ struct StepLds {
     float4 f[BLOCK];
 };
 
-// Advances the strands of one pack [vbeg, vend) through a whole step.
-template <int BLOCK>
-__device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, const int vend, StepLds<BLOCK>& L) {
+__device__ inline void tape3(float* base, int slot, int n, int i, f3 v) { st3(base + (size_t)slot * n * 3, i, v); }
+
+// Advances the strands of one pack [vbeg, vend) through a whole step; with
+// RECORD it also writes the tape of the step.
+template <int BLOCK, bool RECORD = false>
+__device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, const int vend, StepLds<BLOCK>& L,
+                                         const TapeArgs* tp = nullptr) {
     float(&s_x)[3][BLOCK] = L.x;
     float(&s_dx)[2][3][BLOCK] = L.dx;
     float4(&s_q)[2][BLOCK] = L.q;
@@ -77,7 +81,18 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
     for (int s = 0; s < a.num_substeps; s++) {
         // ---- initItr: inertial target y and VBD adaptive initial guess
         f3 y, dx;
+        if (RECORD && act) {
+            tape3(tp->x, s, tp->n, i, x);
+            tape3(tp->v, s, tp->n, i, v);
+            tape3(tp->u, s, tp->n, i, u);
+        }
         init_vertex(a, free, x, v, u, y, dx);
+        if (RECORD && act) {
+            const int base = s * (a.num_itr + 1);
+            tape3(tp->y, s, tp->n, i, y);
+            tape3(tp->dx, base, tp->n, i, dx);
+            tp->q[(size_t)base * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
+        }
         __syncthreads();  // previous substep's LDS reads are complete
         s_x[0][t] = x.x;
         s_x[1][t] = x.y;
@@ -132,6 +147,11 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             s_q[cur ^ 1][t] = make_float4(qn.x, qn.y, qn.z, qn.w);
             dx = dxn;
             q = qn;
+            if (RECORD && act) {
+                const int idx = s * (a.num_itr + 1) + k + 1;
+                tape3(tp->dx, idx, tp->n, i, dx);
+                tp->q[(size_t)idx * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
+            }
             // The next iteration's first LDS write (s_f) is ordered after this
             // iteration's s_f reads by the barrier above; its s_dx/s_q writes
             // target the buffers this iteration only read before that barrier.
@@ -155,6 +175,13 @@ __global__ __launch_bounds__(BLOCK) void step_fused_kernel(StepArgs a, const int
     __shared__ StepLds<BLOCK> lds;
     const int2 pk = packs[blockIdx.x];
     run_pack<BLOCK>(a, pk.x, pk.y, lds);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void step_record_kernel(StepArgs a, TapeArgs tape, const int2* __restrict__ packs) {
+    __shared__ StepLds<BLOCK> lds;
+    const int2 pk = packs[blockIdx.x];
+    run_pack<BLOCK, true>(a, pk.x, pk.y, lds, &tape);
 }
 
 // Batched form: every workgroup reads its groom's arguments (pointers,
@@ -186,6 +213,27 @@ hipError_t launch_step_fused(const StepArgs& a, const int2* packs, int num_packs
 #define KSCR_CASE(B)                                                                      \
     case B:                                                                               \
         hipLaunchKernelGGL(step_fused_kernel<B>, grid, dim3(B), 0, stream, a, packs);     \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_step_record(const StepArgs& a, const TapeArgs& tape, const int2* packs, int num_packs, int block,
+                              hipStream_t stream) {
+    if (num_packs <= 0) return hipSuccess;
+    const dim3 grid(num_packs);
+    switch (block) {
+#define KSCR_CASE(B)                                                                              \
+    case B:                                                                                       \
+        hipLaunchKernelGGL(step_record_kernel<B>, grid, dim3(B), 0, stream, a, tape, packs);      \
         break;
         KSCR_CASE(64)
         KSCR_CASE(128)

@@ -468,10 +468,15 @@ struct Sim {
         return 0;
     }
     // Per-vertex stretch stiffness and rest rotors (per-strand materials).
+    // Per-vertex materials (kscr_set_material): the host mirror, and the live
+    // state too once uploaded.
     void set_material(const float* ks, const float* qr) {
+        const bool live = (int)kStretch.size() == n;
         for (int i = 0; i < n; i++) {
             if (ks) kStretchF[i] = ks[i];
             if (qr) qRestF[i] = R4<float>{qr[4 * i], qr[4 * i + 1], qr[4 * i + 2], qr[4 * i + 3]};
+            if (live && ks) kStretch[i] = (T)kStretchF[i];
+            if (live && qr) qRest[i] = R4<T>{(T)qRestF[i].x, (T)qRestF[i].y, (T)qRestF[i].z, (T)qRestF[i].w};
         }
     }
     // Same, in the state's precision (after upload; for the FP64 gradient checks).

@@ -1,6 +1,6 @@
 """Box sets for the LBVH tests.
 
-testlbvh_points follows the reference's own recipe (Kitten::testLBVH,
+recipe_points follows the reference's own recipe (Kitten::testLBVH,
 KE/KittenGpuLBVH/lbvh.cu:554-565): srand(1), points (rand()/RAND_MAX)^3 in the
 unit cube, each box padded by R = 0.001.  glibc's rand() is called through
 ctypes, so the points are the reference's points on this platform."""
@@ -10,7 +10,7 @@ import ctypes.util
 import numpy as np
 
 
-def testlbvh_points(n=100_000, pad=0.001):
+def recipe_points(n=100_000, pad=0.001):
     libc = ctypes.CDLL(ctypes.util.find_library("c"))
     libc.rand.restype = ctypes.c_int
     libc.srand(1)

@@ -1,0 +1,161 @@
+"""The HIP adjoint (kscr_adjoint_forward / kscr_adjoint_backward) against the
+FP32 restatement of the same reverse sweep (oracle/adjoint_oracle.hpp), bit
+for bit, and the torch.autograd binding against the FP64 oracle gradient.
+
+The FP64 oracle gradient itself is pinned by central finite differences in
+tests/test_oracle_adjoint.py."""
+import numpy as np
+import pytest
+import torch
+
+from kamisimulator_amd import materials, scenes
+from kamisimulator_amd._lib import Sim, default_params
+from kamisimulator_amd.autograd import Rollout
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+DT = np.float32(1 / 600)
+
+
+def _workload(ns, nv, seed=21):
+    sc = scenes.groom(ns, nv, seed=seed)
+    sc.material = scenes.Material(num_itr=10)
+    return sc, scenes.make_params(default_params, sc.material)
+
+
+def _perturbed_material(sim, sc, p, rng):
+    rest = sim.get_rest()
+    ns = sc.strand_offsets.size - 1
+    E = 3.0 * (1 + 0.2 * rng.uniform(-1, 1, ns))
+    kap = 1 + 0.2 * rng.uniform(-1, 1, ns)
+    mat = materials.strand_materials(materials.rest_directions(rest["q_rest"]), rest["l_rest"], sc.strand_offsets, E,
+                                     kap, p.radius)
+    return mat["k_stretch"].astype(np.float32), mat["q_rest"].astype(np.float32)
+
+
+def _gpu_gradient(g, nsteps, K, tips, targets):
+    g.adjoint_forward(DT, nsteps, K)
+    x, _ = g.get_pos_vel()
+    d = x[tips] - targets.astype(np.float32)
+    xb = np.zeros_like(x)
+    xb[tips] = 2 * d
+    ksb, qrb = g.adjoint_backward(xb)
+    return x, ksb, qrb
+
+
+def _compare(name, ksb_g, qrb_g, ksb_o, qrb_o):
+    ko = ksb_o.astype(np.float32)
+    qo = qrb_o.astype(np.float32)
+    dk = int(np.sum(ksb_g != ko))
+    dq = int(np.sum(qrb_g != qo))
+    scale = max(np.abs(ko).max(), 1e-30)
+    rel = np.abs(ksb_g.astype(np.float64) - ko).max() / scale
+    print(f"[gpu-adjoint] {name}: differing kStretch adjoints {dk}, qRest adjoints {dq}, max rel {rel:.3e}", flush=True)
+    assert dk == 0 and dq == 0
+
+
+@pytest.mark.parametrize("K", [3, 10])
+def test_adjoint_matches_fp32_oracle(K):
+    ns, nv, nsteps = 24, 64, 10
+    sc, p = _workload(ns, nv)
+    print(f"[gpu-adjoint] {ns}x{nv} steps={nsteps} K={K}", flush=True)
+    g = Sim(p, sc.positions, sc.strand_offsets, device=0)
+    scenes.setup_sim(g, sc)
+    o = OracleSim(p, sc.positions, sc.strand_offsets, precision=32)
+    scenes.setup_sim(o, sc)
+    ks, qr = _perturbed_material(g, sc, p, np.random.default_rng(5))
+    g.set_material(ks, qr)
+    o.set_material(ks, qr)
+    tips = (sc.strand_offsets[1:] - 1).astype(np.int32)
+    targets = sc.positions[tips].astype(np.float64) + np.array([0.0, -2e-3, 1e-3])
+    x, ksb_g, qrb_g = _gpu_gradient(g, nsteps, K, tips, targets)
+    _, ksb_o, qrb_o = o.gradient(DT, nsteps, K, tips, targets)
+    _compare(f"K={K}", ksb_g, qrb_g, ksb_o, qrb_o)
+    # both leave the state at the end of the rollout
+    xo, _ = o.get_pos_vel()
+    xg, _ = g.get_pos_vel()
+    assert np.array_equal(xg, x) and np.array_equal(xg, xo)
+
+
+def test_adjoint_unsupported_configurations():
+    sc, p = _workload(2, 16)
+    p.enable_hair_hair_coll = 1
+    g = Sim(p, sc.positions, sc.strand_offsets, device=0)
+    scenes.setup_sim(g, sc)
+    with pytest.raises(Exception, match="UNSUPPORTED"):
+        g.adjoint_forward(DT, 2, 1)
+    p.enable_hair_hair_coll = 0
+    g.set_params(p)
+    with pytest.raises(Exception, match="NOT_CONFIGURED"):
+        g.adjoint_backward()
+
+
+def test_differentiation_workload_strand_subset():
+    """2k strands x 64 vertices, 50 steps, checkpoints every 10.  Strands are
+    independent without contacts, so the oracle reruns the first strands
+    alone and their adjoints must agree bit for bit."""
+    ns, nv, nsteps, K, sub = 2000, 64, 50, 10, 12
+    sc, p = _workload(ns, nv, seed=4)
+    print(f"[gpu-adjoint] differentiation workload {ns}x{nv} steps={nsteps} K={K}", flush=True)
+    g = Sim(p, sc.positions, sc.strand_offsets, device=0)
+    scenes.setup_sim(g, sc)
+    ks, qr = _perturbed_material(g, sc, p, np.random.default_rng(8))
+    g.set_material(ks, qr)
+    tips = (sc.strand_offsets[1:] - 1).astype(np.int32)
+    targets = sc.positions[tips].astype(np.float64) + np.array([1e-3, -3e-3, 0.0])
+    _, ksb_g, qrb_g = _gpu_gradient(g, nsteps, K, tips, targets)
+    rec, sweep = g.adjoint_time()
+    print(f"[gpu-adjoint] record {rec:.2f} ms, reverse sweep {sweep:.2f} ms", flush=True)
+    nv_sub = int(sc.strand_offsets[sub])
+    sc_sub = scenes.Scene("subset", sc.positions[:nv_sub], sc.strand_offsets[: sub + 1].copy(), material=sc.material)
+    o = OracleSim(p, sc_sub.positions, sc_sub.strand_offsets, precision=32)
+    scenes.setup_sim(o, sc_sub)
+    o.set_material(ks[:nv_sub], qr[:nv_sub])
+    _, ksb_o, qrb_o = o.gradient(DT, nsteps, K, tips[:sub], targets[:sub])
+    _compare("workload subset", ksb_g[:nv_sub], qrb_g[:nv_sub], ksb_o, qrb_o)
+
+
+def _oracle_strand_gradient(sc, p, prec, tips, targets, nsteps, K):
+    o = OracleSim(p, sc.positions, sc.strand_offsets, precision=prec)
+    scenes.setup_sim(o, sc)
+    rest = o.get_rest()
+    ns = sc.strand_offsets.size - 1
+    mat = materials.strand_materials(materials.rest_directions(rest["q_rest"]), rest["l_rest"], sc.strand_offsets,
+                                     np.full(ns, 3.0), np.ones(ns), p.radius)
+    if prec == 64:
+        o.set_material64(mat["k_stretch"], mat["q_rest"])
+    else:
+        o.set_material(mat["k_stretch"].astype(np.float32), mat["q_rest"].astype(np.float32))
+    _, ksb, qrb = o.gradient(DT, nsteps, K, tips, targets)
+    return materials.strand_gradient(mat, sc.strand_offsets, ksb, qrb)
+
+
+def test_autograd_function_against_oracle_gradients():
+    """loss.backward() through the torch.autograd.Function equals the chain
+    rule applied to the FP32 oracle adjoint; against the FP64 oracle gradient
+    the FP32 rollout differs by its rounding noise (reported; the rest-curvature
+    gradient within a few per cent, the stretch part of dL/dE noise dominated)."""
+    ns, nv, nsteps, K = 6, 64, 10, 5
+    sc, p = _workload(ns, nv, seed=21)
+    roll = Rollout(sc, DT, nsteps, K, device=0)
+    tips = torch.from_numpy((sc.strand_offsets[1:] - 1).astype(np.int64))
+    rng = np.random.default_rng(4)
+    E_t = torch.tensor(3.0 * (1 + 0.2 * rng.uniform(-1, 1, ns)))
+    k_t = torch.tensor(1 + 0.2 * rng.uniform(-1, 1, ns))
+    with torch.no_grad():
+        targets = roll(E_t, k_t)[0][tips].double()
+    E = torch.full((ns,), 3.0, dtype=torch.float64, requires_grad=True)
+    kap = torch.ones(ns, dtype=torch.float64, requires_grad=True)
+    x, _ = roll(E, kap)
+    loss = ((x[tips].double() - targets) ** 2).sum()
+    loss.backward()
+    tips32 = tips.numpy().astype(np.int32)
+    gE32, gk32 = _oracle_strand_gradient(sc, p, 32, tips32, targets.numpy(), nsteps, K)
+    np.testing.assert_allclose(E.grad.numpy(), gE32, rtol=1e-9, atol=1e-12 * np.abs(gE32).max())
+    np.testing.assert_allclose(kap.grad.numpy(), gk32, rtol=1e-9, atol=1e-12 * np.abs(gk32).max())
+    gE64, gk64 = _oracle_strand_gradient(sc, p, 64, tips32, targets.numpy(), nsteps, K)
+    relE = np.abs(E.grad.numpy() - gE64).max() / np.abs(gE64).max()
+    relk = np.abs(kap.grad.numpy() - gk64).max() / np.abs(gk64).max()
+    print(f"[gpu-adjoint] autograd FP32 vs FP64 oracle gradient: dE rel {relE:.3e}, dkappa rel {relk:.3e}", flush=True)
+    assert relk < 0.2

@@ -9,7 +9,7 @@ import pytest
 
 from kamisimulator_amd._lib import Lbvh
 from oracle.pyoracle import OracleLbvh
-from tests.lbvh_cases import segment_boxes, testlbvh_points
+from tests.lbvh_cases import segment_boxes, recipe_points
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +39,7 @@ def _compare(boxes, label):
 
 
 def test_reference_recipe_100k():
-    _compare(testlbvh_points(100_000), "testLBVH recipe")
+    _compare(recipe_points(100_000), "testLBVH recipe")
 
 
 def test_segments_and_refit():

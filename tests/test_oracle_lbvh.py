@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle.pyoracle import OracleLbvh
-from tests.lbvh_cases import brute_pairs, segment_boxes, testlbvh_points
+from tests.lbvh_cases import brute_pairs, segment_boxes, recipe_points
 
 
 def self_check(t, boxes):
@@ -63,7 +63,7 @@ def check_query(t, pairs, boxes):
 
 
 def test_reference_recipe_small(oracle_lib):
-    boxes = testlbvh_points(3000, pad=0.01)
+    boxes = recipe_points(3000, pad=0.01)
     o = OracleLbvh(boxes)
     t = o.get()
     assert self_check(t, boxes)
