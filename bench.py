@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-sdf-leg", action="store_true", help="skip the with-SDF step timing")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-adjoint-leg", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--no-batched-leg", action="store_true", help="skip the batched-vs-per-groom timing")
+    ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--only-legs", action="store_true", help="run the adjoint and batched legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (rocprof byte-counter calibration)")
     return ap.parse_args()
@@ -124,6 +128,138 @@ def cpu_baseline(scene, params, seconds):
     }
 
 
+def adjoint_leg(stream, nsteps=50, every=10):
+    """Differentiation workload: 2000 strands x 64 vertices, FP32, nsteps
+    steps of 1/600 s with a checkpoint every `every` steps; loss = squared tip
+    distances to targets.  Times the plain forward step, the checkpointed
+    forward, and the backward (re-recording each segment + reverse sweep)."""
+    import torch
+
+    from kamisimulator_amd import scenes
+    from kamisimulator_amd._lib import Sim, default_params
+
+    ns, nv = 2000, 64
+    sc = scenes.groom(ns, nv, seed=4)
+    sc.material = scenes.Material(num_itr=10)
+    p = scenes.make_params(default_params, sc.material)
+    sim = Sim(p, sc.positions, sc.strand_offsets, device=0)
+    sim.set_stream(stream.cuda_stream)
+    scenes.setup_sim(sim, sc)
+    dt = np.float32(1 / 600)
+    tips = (sc.strand_offsets[1:] - 1).astype(np.int64)
+
+    def seed():
+        x, _ = sim.get_pos_vel()
+        xb = np.zeros_like(x)
+        xb[tips] = 2 * (x[tips] - (sc.positions[tips] + np.float32(1e-3)))
+        return xb
+
+    # warm-up: code objects, tape and checkpoint buffers
+    for _ in range(3):
+        sim.advance(dt)
+    sim.adjoint_forward(dt, every, every)
+    sim.adjoint_backward(seed())
+    torch.cuda.synchronize()
+    a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    a.record(stream)
+    for _ in range(nsteps):
+        sim.advance(dt)
+    b.record(stream)
+    sim.adjoint_forward(dt, nsteps, every)
+    c.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = a.elapsed_time(b)
+    ckpt_fwd_ms = b.elapsed_time(c)
+    xb = seed()
+    w0 = time.perf_counter()
+    sim.adjoint_backward(xb)
+    wall_bwd = (time.perf_counter() - w0) * 1e3
+    rec_ms, sweep_ms = sim.adjoint_time()
+    fwd_step = fwd_ms / nsteps
+    adj_step = (rec_ms + sweep_ms) / nsteps
+    S = int(np.ceil(dt / np.float32(p.max_h)))
+    return {
+        "workload": f"{ns} strands x {nv} vertices FP32, {nsteps} steps of dt=1/600 s ({S} substeps x {p.num_itr} "
+                    f"iterations), checkpoints every {every} steps, loss on strand tips",
+        "forward_ms_per_step": round(fwd_step, 4),
+        "checkpointed_forward_ms_per_step": round(ckpt_fwd_ms / nsteps, 4),
+        "record_ms_per_step": round(rec_ms / nsteps, 4),
+        "reverse_sweep_ms_per_step": round(sweep_ms / nsteps, 4),
+        "adjoint_step_over_forward_step": round(adj_step / fwd_step, 3),
+        "gradient_over_forward": round((ckpt_fwd_ms + rec_ms + sweep_ms) / fwd_ms, 3),
+        "backward_wall_ms": round(wall_bwd, 2),
+        "note": "adjoint step = re-recording the step from its checkpoint + the reverse sweep over it; "
+                "gradient = checkpointed forward + every adjoint step",
+    }
+
+
+def batched_leg(stream, nsteps, strands=(1_000, 50_000), num_grooms=64):
+    """Batched form: num_grooms independent grooms (strand counts log-uniform
+    in `strands`, 32-256 vertices per strand, own 128^3 sphere SDF and gravity
+    direction) stepped nsteps times (a) one launch per groom per step and (b)
+    one batched launch per step replayed from a hipGraph, block autotuned."""
+    import torch
+
+    from kamisimulator_amd import scenes
+    from kamisimulator_amd._lib import Batch, Sim, default_params
+
+    t0 = time.perf_counter()
+    grooms = scenes.config_batched(num_grooms=num_grooms, strands=strands)
+    sims = []
+    for sc in grooms:
+        sim = Sim(scenes.make_params(default_params, sc.material), sc.positions, sc.strand_offsets, device=0)
+        sim.set_stream(stream.cuda_stream)
+        scenes.setup_sim(sim, sc)
+        sims.append(sim)
+    verts = sum(sc.num_verts for sc in grooms)
+    print(f"[bench] batched leg: {num_grooms} grooms, {verts} vertices, setup {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    dt = np.float32(1 / 600)
+    for sim in sims:
+        sim.solve_next_step(dt)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(nsteps):
+        for sim in sims:
+            sim.solve_next_step(dt)
+    torch.cuda.synchronize()
+    per_groom_ms = (time.perf_counter() - w0) * 1e3
+    batch = Batch(sims)
+    default_block = batch.block_size
+    best = batch.autotune(dt, 3)
+    batch.set_graph(True)
+    batch.step(dt, nsteps, with_sdf=True)  # capture + first replay
+    batch.synchronize()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    batch.step(dt, nsteps, with_sdf=True)
+    batch.synchronize()
+    graph_ms = (time.perf_counter() - w0) * 1e3
+    batch.set_graph(False)
+    w0 = time.perf_counter()
+    batch.step(dt, nsteps, with_sdf=True)
+    batch.synchronize()
+    nograph_ms = (time.perf_counter() - w0) * 1e3
+    batch.close()
+    for sim in sims:
+        sim.close()
+    p0 = scenes.make_params(default_params, grooms[0].material)
+    S = int(np.ceil(dt / np.float32(p0.max_h)))
+    ups = verts * S * p0.num_itr
+    return {
+        "workload": f"{num_grooms} grooms, strands log-uniform {strands[0]}-{strands[1]} ({verts} vertices), "
+                    f"32-256 vertices per strand, own 128^3 sphere SDF and gravity, {nsteps} steps of 1/600 s",
+        "per_groom_launches_ms_per_step": round(per_groom_ms / nsteps, 4),
+        "batched_graph_ms_per_step": round(graph_ms / nsteps, 4),
+        "batched_no_graph_ms_per_step": round(nograph_ms / nsteps, 4),
+        "speedup_graph_vs_per_groom": round(per_groom_ms / graph_ms, 3),
+        "batched_graph_updates_per_s": ups * nsteps / (graph_ms / 1e3),
+        "autotuned_block": best,
+        "default_block": default_block,
+        "timing": "host wall clock between device synchronisations",
+    }
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -143,6 +279,15 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
+    if args.only_legs:
+        legs = {}
+        if not args.no_adjoint_leg:
+            legs["adjoint"] = adjoint_leg(stream)
+            print(f"[bench] adjoint leg {legs['adjoint']}", file=sys.stderr, flush=True)
+        if not args.no_batched_leg:
+            legs["batched"] = batched_leg(stream, args.batched_steps)
+        print(json.dumps(legs), flush=True)
+        return
 
     t0 = time.perf_counter()
     scene = scenes.config_groom(args.strands, args.verts, seed=1, sdf_dim=128, with_sdf=True)
@@ -152,6 +297,7 @@ def main():
     sdf = scene.sdf
     scene.sdf = None
     scenes.setup_sim(sim, scene)
+    block = sim.block_size
     print(f"[bench] setup {time.perf_counter() - t0:.1f}s  verts={scene.num_verts} strands={scene.num_strands} "
           f"block={sim.block_size}", file=sys.stderr, flush=True)
 
@@ -215,6 +361,18 @@ def main():
         sdf_leg = {"ms_per_step": round(ms2, 4), "updates_per_s": updates_per_step / (ms2 / 1e3),
                    "sdf_grid": "128^3 sphere SDF, threshold 0"}
 
+    sim.close()
+    adjoint = None
+    if not args.no_adjoint_leg:
+        adjoint = adjoint_leg(stream)
+        print(f"[bench] adjoint leg: forward {adjoint['forward_ms_per_step']} ms/step, adjoint step "
+              f"{adjoint['adjoint_step_over_forward_step']}x", file=sys.stderr, flush=True)
+    batched = None
+    if not args.no_batched_leg:
+        batched = batched_leg(stream, args.batched_steps)
+        print(f"[bench] batched leg: per-groom {batched['per_groom_launches_ms_per_step']} ms/step, graph "
+              f"{batched['batched_graph_ms_per_step']} ms/step", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, params, args.cpu_seconds)
@@ -240,11 +398,13 @@ def main():
             "strands": scene.num_strands,
             "vertices": scene.num_verts,
             "updates_per_step": updates_per_step,
-            "block": sim.block_size,
+            "block": block,
         },
         "roofline": roofline,
         "streaming_sweep_equivalent_frac": round(streaming_equiv, 4),
         "with_sdf_step": sdf_leg,
+        "adjoint": adjoint,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

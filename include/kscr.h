@@ -209,6 +209,11 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf);
 /* use_graph = 0 launches the kernels directly instead of replaying a graph. */
 int kscr_batch_set_graph(kscr_batch* b, int use_graph);
 int kscr_batch_synchronize(kscr_batch* b);
+/* Autotuner of the batched launch: times `steps` steps of dt for every
+ * instantiated workgroup size that holds the longest strand of any groom,
+ * keeps the fastest (every groom's state and clock are saved and restored)
+ * and returns it in best_block (may be NULL). */
+int kscr_batch_autotune(kscr_batch* b, float dt, int steps, int* best_block);
 int kscr_batch_block_size(const kscr_batch* b);
 int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n);
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "kscr_batch_step": (C.c_int, [_P, C.c_float, C.c_int, C.c_int]),
     "kscr_batch_set_graph": (C.c_int, [_P, C.c_int]),
     "kscr_batch_synchronize": (C.c_int, [_P]),
+    "kscr_batch_autotune": (C.c_int, [_P, C.c_float, C.c_int, _IP]),
     "kscr_batch_block_size": (C.c_int, [_P]),
     "kscr_batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "kscr_get_contacts": (C.c_int, [_P, _IP, _IP]),
@@ -369,6 +370,11 @@ class Batch:
 
     def step(self, dt, nsteps=1, with_sdf=True):
         self._check(self.lib.kscr_batch_step(self.b, dt, nsteps, int(bool(with_sdf))))
+
+    def autotune(self, dt, steps=3):
+        best = C.c_int(0)
+        self._check(self.lib.kscr_batch_autotune(self.b, dt, steps, C.byref(best)))
+        return best.value
 
     def set_graph(self, on):
         self._check(self.lib.kscr_batch_set_graph(self.b, int(bool(on))))

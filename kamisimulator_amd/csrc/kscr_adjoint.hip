@@ -98,20 +98,34 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
         if (free) vb = zero3();
         f3 yb = zero3();
 
+        // Iterates of the tape: k + 1 (dxn, dxn_n) is carried over from the
+        // previous pass of the loop, k is loaded one pass ahead (the loads stay
+        // in flight across the barriers, which wait on LDS traffic only).
+        const size_t o_sub = (size_t)sub * (K + 1) * n;
+        f3 dxn = zero3(), dxn_n = zero3();
+        f3 dx = zero3(), dx_n = zero3(), dx_p = zero3();
+        f4 Q = zero4(), Qm = zero4(), Qp = zero4();
+        if (K > 0) {
+            const float* DXK = a.tape.dx + (o_sub + (size_t)K * n) * 3;
+            const float* DXk = a.tape.dx + (o_sub + (size_t)(K - 1) * n) * 3;
+            const float4* Qk = a.tape.q + o_sub + (size_t)(K - 1) * n;
+            if (act) dxn = ld3(DXK, i), dx = ld3(DXk, i), Q = ld4(Qk, i);
+            if (has_next) dxn_n = ld3(DXK, i + 1), dx_n = ld3(DXk, i + 1);
+            if (has_prev) dx_p = ld3(DXk, i - 1), Qm = ld4(Qk, i - 1);
+            if (has_next_ori) Qp = ld4(Qk, i + 1);
+        }
         for (int k = K - 1; k >= 0; k--) {
-            const size_t ok = ((size_t)sub * (K + 1) + k) * n;  // iterate k
-            const size_t on = ok + n;                            // iterate k + 1
-            const float* DXk = a.tape.dx + ok * 3;
-            const float* DXn = a.tape.dx + on * 3;
-            const float4* Qk = a.tape.q + ok;
-            const f3 dx = act ? ld3(DXk, i) : zero3();
-            const f3 dxn_n = has_next ? ld3(DXn, i + 1) : zero3();
-            const f3 dxn = act ? ld3(DXn, i) : zero3();
-            const f3 dx_n = has_next ? ld3(DXk, i + 1) : zero3();
-            const f3 dx_p = has_prev ? ld3(DXk, i - 1) : zero3();
-            const f4 Q = act ? ld4(Qk, i) : zero4();
-            const f4 Qm = has_prev ? ld4(Qk, i - 1) : zero4();
-            const f4 Qp = has_next_ori ? ld4(Qk, i + 1) : zero4();
+            // prefetch iterate k - 1
+            f3 pdx = zero3(), pdx_n = zero3(), pdx_p = zero3();
+            f4 pQ = zero4(), pQm = zero4(), pQp = zero4();
+            if (k > 0) {
+                const float* DXk = a.tape.dx + (o_sub + (size_t)(k - 1) * n) * 3;
+                const float4* Qk = a.tape.q + o_sub + (size_t)(k - 1) * n;
+                if (act) pdx = ld3(DXk, i), pQ = ld4(Qk, i);
+                if (has_next) pdx_n = ld3(DXk, i + 1);
+                if (has_prev) pdx_p = ld3(DXk, i - 1), pQm = ld4(Qk, i - 1);
+                if (has_next_ori) pQp = ld4(Qk, i + 1);
+            }
 
             // ---- phase A: orientation solve of segment i (quaternionLambdaItr)
             f3 eb = zero3();
@@ -278,6 +292,9 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             }
             dxb = dxb2;
             qb = qb2;
+            dxn = dx, dxn_n = dx_n;
+            dx = pdx, dx_n = pdx_n, dx_p = pdx_p;
+            Q = pQ, Qm = pQm, Qp = pQp;
         }
 
         // ---- initItr: y = x + h v + h^2 ext (free), dx^0 = h v (+ h^2 s ext), u' = v

@@ -74,6 +74,9 @@ struct AdjointRun {
     // adjoint state
     DevBuf<float> xb, vb, ub, ksb;
     DevBuf<float4> qb, qrb;
+    // the forward pass records the last segment's tape itself
+    bool last_recorded = false;
+    std::vector<float> last_h_of, last_lh_of;  // h, last_h of the last segment's steps
     float record_ms = 0, sweep_ms = 0;
 };
 }  // namespace
@@ -374,6 +377,35 @@ hipError_t checkpoint_copy(kscr_handle* h, size_t k, bool save) {
 AdjointRun::Clock clock_of(const kscr_handle* h) { return AdjointRun::Clock{h->h, h->last_h, h->time, h->step_counter}; }
 void set_clock(kscr_handle* h, const AdjointRun::Clock& c) {
     h->h = c.h, h->last_h = c.last_h, h->time = c.time, h->step_counter = c.counter;
+}
+
+// Tape of one checkpoint segment: `every` steps of `substeps` substeps.
+int reserve_tape(kscr_handle* h) {
+    AdjointRun& A = h->adj;
+    const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = (size_t)A.every;
+    const size_t per_sub = S * n * 3 * every, per_itr = S * I1 * n * every;
+    if (A.t_q.n >= per_itr && A.t_x.n >= per_sub) return KSCR_OK;
+    hipError_t e;
+    if ((e = A.t_x.alloc(per_sub)) != hipSuccess || (e = A.t_v.alloc(per_sub)) != hipSuccess ||
+        (e = A.t_u.alloc(per_sub)) != hipSuccess || (e = A.t_y.alloc(per_sub)) != hipSuccess ||
+        (e = A.t_dx.alloc(per_itr * 3)) != hipSuccess || (e = A.t_q.alloc(per_itr)) != hipSuccess)
+        return h->hip_fail(e, "hipMalloc tape");
+    return KSCR_OK;
+}
+
+// Tape slot of step j of a segment.
+TapeArgs tape_of(kscr_handle* h, size_t j) {
+    AdjointRun& A = h->adj;
+    const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1;
+    TapeArgs t{};
+    t.x = A.t_x.p + j * S * n * 3;
+    t.v = A.t_v.p + j * S * n * 3;
+    t.u = A.t_u.p + j * S * n * 3;
+    t.y = A.t_y.p + j * S * n * 3;
+    t.dx = A.t_dx.p + j * S * I1 * n * 3;
+    t.q = A.t_q.p + j * S * I1 * n;
+    t.n = (int)n;
+    return t;
 }
 
 int adjoint_supported(kscr_handle* h) {
@@ -830,6 +862,17 @@ int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_ev
             (e = A.ck_u.alloc(slots * n * 3)) != hipSuccess || (e = A.ck_q.alloc(slots * n)) != hipSuccess)
             return h->hip_fail(e, "hipMalloc checkpoints");
     }
+    A.dt = dt;
+    A.nsteps = nsteps;
+    A.every = checkpoint_every;
+    A.substeps = std::max(1, (int)std::ceil(dt / h->p.max_h));
+    A.num_itr = h->p.num_itr;
+    A.last_recorded = false;
+    if ((rc = reserve_tape(h)) != KSCR_OK) return rc;
+    // steps of the last segment write their tape as they run
+    const int last_s0 = ((nsteps - 1) / checkpoint_every) * checkpoint_every;
+    A.last_h_of.assign(checkpoint_every, 0.f);
+    A.last_lh_of.assign(checkpoint_every, 0.f);
     A.clocks.clear();
     for (int s = 0; s < nsteps; s++) {
         if (s % checkpoint_every == 0) {
@@ -837,13 +880,18 @@ int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_ev
                 return h->hip_fail(e, "save checkpoint");
             A.clocks.push_back(clock_of(h));
         }
-        if ((rc = launch_step(h, dt, false)) != KSCR_OK) return rc;
+        if (s < last_s0) {
+            if ((rc = launch_step(h, dt, false)) != KSCR_OK) return rc;
+            continue;
+        }
+        const StepArgs sa = step_args(h, dt, false);
+        A.last_h_of[s - last_s0] = sa.h;
+        A.last_lh_of[s - last_s0] = sa.last_h;
+        if ((e = launch_step_record(sa, tape_of(h, (size_t)(s - last_s0)), h->d_packs.p, h->num_packs, h->block,
+                                    h->stream)) != hipSuccess)
+            return h->hip_fail(e, "record kernel launch");
     }
-    A.dt = dt;
-    A.nsteps = nsteps;
-    A.every = checkpoint_every;
-    A.substeps = std::max(1, (int)std::ceil(dt / h->p.max_h));
-    A.num_itr = h->p.num_itr;
+    A.last_recorded = true;
     A.ready = true;
     return KSCR_OK;
 }
@@ -860,17 +908,8 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     hipError_t e;
     if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
     const size_t n = h->n, n3 = 3 * n;
-    const size_t S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = (size_t)A.every;
-    // tape of one segment: `every` steps of S substeps
-    const size_t per_step_sub = S * n, per_step_itr = S * I1 * n;
-    if (A.t_q.n < every * per_step_itr || A.t_x.n < every * per_step_sub * 3) {
-        if ((e = A.t_x.alloc(every * per_step_sub * 3)) != hipSuccess ||
-            (e = A.t_v.alloc(every * per_step_sub * 3)) != hipSuccess ||
-            (e = A.t_u.alloc(every * per_step_sub * 3)) != hipSuccess ||
-            (e = A.t_y.alloc(every * per_step_sub * 3)) != hipSuccess ||
-            (e = A.t_dx.alloc(every * per_step_itr * 3)) != hipSuccess || (e = A.t_q.alloc(every * per_step_itr)) != hipSuccess)
-            return h->hip_fail(e, "hipMalloc tape");
-    }
+    const size_t every = (size_t)A.every;
+    if ((rc = reserve_tape(h)) != KSCR_OK) return rc;
     if (A.qb.n != n) {
         if ((e = A.xb.alloc(n3)) != hipSuccess || (e = A.vb.alloc(n3)) != hipSuccess || (e = A.ub.alloc(n3)) != hipSuccess ||
             (e = A.ksb.alloc(n)) != hipSuccess || (e = A.qb.alloc(n)) != hipSuccess || (e = A.qrb.alloc(n)) != hipSuccess)
@@ -907,17 +946,6 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     aa.qb = A.qb.p;
     aa.ksb = A.ksb.p;
     aa.qrb = A.qrb.p;
-    auto tape_of = [&](size_t j) {
-        TapeArgs t{};
-        t.x = A.t_x.p + j * per_step_sub * 3;
-        t.v = A.t_v.p + j * per_step_sub * 3;
-        t.u = A.t_u.p + j * per_step_sub * 3;
-        t.y = A.t_y.p + j * per_step_sub * 3;
-        t.dx = A.t_dx.p + j * per_step_itr * 3;
-        t.q = A.t_q.p + j * per_step_itr;
-        t.n = (int)n;
-        return t;
-    };
     std::vector<hipEvent_t> ev(3 * (size_t)nseg, nullptr);
     for (auto& x : ev)
         if ((e = hipEventCreate(&x)) != hipSuccess) {
@@ -926,26 +954,32 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
         }
     std::vector<float> hs(every), lhs(every);
     for (int seg = nseg - 1; seg >= 0 && rc == KSCR_OK; seg--) {
-        if ((e = checkpoint_copy(h, (size_t)seg, false)) != hipSuccess) {
-            rc = h->hip_fail(e, "restore checkpoint");
-            break;
-        }
-        set_clock(h, A.clocks[seg]);
         const int s0 = seg * A.every, s1 = std::min(A.nsteps, s0 + A.every);
         (void)hipEventRecord(ev[3 * seg], h->stream);
-        for (int s = s0; s < s1; s++) {
-            const StepArgs sa = step_args(h, A.dt, false);
-            hs[s - s0] = sa.h;
-            lhs[s - s0] = sa.last_h;
-            if ((e = launch_step_record(sa, tape_of(s - s0), h->d_packs.p, h->num_packs, h->block, h->stream)) !=
-                hipSuccess) {
-                rc = h->hip_fail(e, "record kernel launch");
+        if (seg == nseg - 1 && A.last_recorded) {
+            // recorded by the forward pass
+            hs = A.last_h_of;
+            lhs = A.last_lh_of;
+        } else {
+            if ((e = checkpoint_copy(h, (size_t)seg, false)) != hipSuccess) {
+                rc = h->hip_fail(e, "restore checkpoint");
                 break;
+            }
+            set_clock(h, A.clocks[seg]);
+            for (int s = s0; s < s1; s++) {
+                const StepArgs sa = step_args(h, A.dt, false);
+                hs[s - s0] = sa.h;
+                lhs[s - s0] = sa.last_h;
+                if ((e = launch_step_record(sa, tape_of(h, (size_t)(s - s0)), h->d_packs.p, h->num_packs, h->block,
+                                            h->stream)) != hipSuccess) {
+                    rc = h->hip_fail(e, "record kernel launch");
+                    break;
+                }
             }
         }
         (void)hipEventRecord(ev[3 * seg + 1], h->stream);
         for (int s = s1 - 1; s >= s0 && rc == KSCR_OK; s--) {
-            aa.tape = tape_of(s - s0);
+            aa.tape = tape_of(h, (size_t)(s - s0));
             aa.h = hs[s - s0];
             aa.last_h = lhs[s - s0];
             if ((e = launch_step_adjoint(aa, h->d_packs.p, h->num_packs, h->block, h->stream)) != hipSuccess)
@@ -953,6 +987,7 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
         }
         (void)hipEventRecord(ev[3 * seg + 2], h->stream);
     }
+    if (nseg > 1) A.last_recorded = false;  // earlier segments reused the tape
     if ((e = checkpoint_copy(h, end_slot, false)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "restore end state");
     set_clock(h, end_clock);
     if (rc == KSCR_OK &&
@@ -1063,6 +1098,7 @@ struct kscr_batch {
     int graph_steps = 0;
     float graph_dt = 0.f;
     std::string err;
+    int longest = 0;
     int fail(int code, const std::string& m) {
         err = m;
         return code;
@@ -1077,6 +1113,31 @@ struct kscr_batch {
         graph_steps = 0;
     }
 };
+
+namespace {
+// Packs every groom's strands into workgroups of `block` vertices; a pack is
+// (begin, end, groom, 0).
+int batch_set_block(kscr_batch* b, int block) {
+    std::vector<int4> packs;
+    for (size_t g = 0; g < b->grooms.size(); g++) {
+        std::vector<int2> p2;
+        const int rc = build_packs(b->grooms[g], block, p2);
+        if (rc) return rc;
+        for (const int2& pk : p2) packs.push_back(make_int4(pk.x, pk.y, (int)g, 0));
+    }
+    hipError_t e;
+    if (b->d_packs.n < packs.size() && (e = b->d_packs.alloc(packs.size())) != hipSuccess)
+        return b->hip_fail(e, "hipMalloc batch packs");
+    if ((e = hipMemcpyAsync(b->d_packs.p, packs.data(), packs.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(b->stream)) != hipSuccess)
+        return b->hip_fail(e, "upload batch packs");
+    b->block = block;
+    b->num_packs = (int)packs.size();
+    b->drop_graph();
+    return KSCR_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -1117,22 +1178,9 @@ int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_si
         delete b;
         return KSCR_ERR_STRAND_TOO_LONG;
     }
-    b->block = block;
-    std::vector<int4> packs;
-    for (int g = 0; g < num_handles; g++) {
-        kscr_handle* h = handles[g];
-        std::vector<int2> p2;
-        int rc = build_packs(h, block, p2);
-        if (rc) {
-            delete b;
-            return rc;
-        }
-        for (const int2& pk : p2) packs.push_back(make_int4(pk.x, pk.y, g, 0));
-    }
-    b->num_packs = (int)packs.size();
+    b->longest = longest;
     hipError_t e;
-    if ((e = hipSetDevice(b->device)) != hipSuccess || (e = b->d_args.alloc(num_handles)) != hipSuccess ||
-        (e = b->d_packs.alloc(packs.size())) != hipSuccess) {
+    if ((e = hipSetDevice(b->device)) != hipSuccess || (e = b->d_args.alloc(num_handles)) != hipSuccess) {
         delete b;
         return KSCR_ERR_HIP;
     }
@@ -1141,12 +1189,11 @@ int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_si
         return KSCR_ERR_HIP;
     }
     b->own_stream = true;
-    if ((e = hipMemcpyAsync(b->d_packs.p, packs.data(), packs.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream)) !=
-            hipSuccess ||
-        (e = hipStreamSynchronize(b->stream)) != hipSuccess) {
+    const int rc = batch_set_block(b, block);
+    if (rc) {
         (void)hipStreamDestroy(b->stream);
         delete b;
-        return KSCR_ERR_HIP;
+        return rc;
     }
     b->args.resize(num_handles);
     *out = b;
@@ -1226,6 +1273,79 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
         b->graph_dt = dt;
     }
     if ((e = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) return b->hip_fail(e, "hipGraphLaunch");
+    return KSCR_OK;
+}
+
+int kscr_batch_autotune(kscr_batch* b, float dt, int steps, int* best_block) {
+    if (!b || steps <= 0 || !(dt > 0)) return KSCR_ERR_INVALID_ARGUMENT;
+    hipError_t e;
+    if ((e = hipSetDevice(b->device)) != hipSuccess) return b->hip_fail(e, "hipSetDevice");
+    // save every groom's state and clock
+    struct Saved {
+        DevBuf<float> x, v, u;
+        DevBuf<float4> q;
+        float h, last_h, time;
+        long counter;
+    };
+    std::vector<Saved> saved(b->grooms.size());
+    const hipMemcpyKind d2d = hipMemcpyDeviceToDevice;
+    auto copy_state = [&](kscr_handle* h, Saved& sv, bool save) -> hipError_t {
+        const size_t n3 = (size_t)h->n * 3 * 4, nq = (size_t)h->n * 16;
+        hipError_t r;
+        if (save) {
+            if ((r = sv.x.alloc((size_t)h->n * 3)) != hipSuccess || (r = sv.v.alloc((size_t)h->n * 3)) != hipSuccess ||
+                (r = sv.u.alloc((size_t)h->n * 3)) != hipSuccess || (r = sv.q.alloc(h->n)) != hipSuccess)
+                return r;
+            sv.h = h->h, sv.last_h = h->last_h, sv.time = h->time, sv.counter = h->step_counter;
+            if ((r = hipMemcpyAsync(sv.x.p, h->d_x.p, n3, d2d, b->stream)) != hipSuccess) return r;
+            if ((r = hipMemcpyAsync(sv.v.p, h->d_v.p, n3, d2d, b->stream)) != hipSuccess) return r;
+            if ((r = hipMemcpyAsync(sv.u.p, h->d_u.p, n3, d2d, b->stream)) != hipSuccess) return r;
+            return hipMemcpyAsync(sv.q.p, h->d_q.p, nq, d2d, b->stream);
+        }
+        h->h = sv.h, h->last_h = sv.last_h, h->time = sv.time, h->step_counter = sv.counter;
+        if ((r = hipMemcpyAsync(h->d_x.p, sv.x.p, n3, d2d, b->stream)) != hipSuccess) return r;
+        if ((r = hipMemcpyAsync(h->d_v.p, sv.v.p, n3, d2d, b->stream)) != hipSuccess) return r;
+        if ((r = hipMemcpyAsync(h->d_u.p, sv.u.p, n3, d2d, b->stream)) != hipSuccess) return r;
+        return hipMemcpyAsync(h->d_q.p, sv.q.p, nq, d2d, b->stream);
+    };
+    for (size_t g = 0; g < b->grooms.size(); g++) {
+        if ((e = hipStreamSynchronize(b->grooms[g]->stream)) != hipSuccess) return b->hip_fail(e, "hipStreamSynchronize");
+        if ((e = copy_state(b->grooms[g], saved[g], true)) != hipSuccess) return b->hip_fail(e, "autotune save");
+    }
+    const bool graph_was = b->use_graph;
+    b->use_graph = false;
+    hipEvent_t ev0, ev1;
+    if ((e = hipEventCreate(&ev0)) != hipSuccess || (e = hipEventCreate(&ev1)) != hipSuccess)
+        return b->hip_fail(e, "hipEventCreate");
+    const int start_block = b->block;
+    int best = -1, rc = KSCR_OK;
+    float best_ms = 0;
+    for (int c : kBlockSizes) {
+        if (c < b->longest) continue;
+        if ((rc = batch_set_block(b, c)) != KSCR_OK) break;
+        if ((rc = kscr_batch_step(b, dt, 1, 0)) != KSCR_OK) break;  // warm-up
+        (void)hipEventRecord(ev0, b->stream);
+        if ((rc = kscr_batch_step(b, dt, steps, 0)) != KSCR_OK) break;
+        (void)hipEventRecord(ev1, b->stream);
+        if ((e = hipEventSynchronize(ev1)) != hipSuccess) {
+            rc = b->hip_fail(e, "hipEventSynchronize");
+            break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        if (best < 0 || ms < best_ms) best = c, best_ms = ms;
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    b->use_graph = graph_was;
+    for (size_t g = 0; g < b->grooms.size(); g++)
+        if ((e = copy_state(b->grooms[g], saved[g], false)) != hipSuccess && rc == KSCR_OK)
+            rc = b->hip_fail(e, "autotune restore");
+    const int rc2 = batch_set_block(b, (rc == KSCR_OK && best > 0) ? best : start_block);
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess && rc == KSCR_OK) rc = b->hip_fail(e, "hipStreamSynchronize");
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (best_block) *best_block = b->block;
     return KSCR_OK;
 }
 

@@ -245,6 +245,30 @@ def config_large(num_strands=100_000, seed=2, sdf_dim=256, with_sdf=True):
     return sc
 
 
+def config_batched(num_grooms=64, strands=(1_000, 50_000), verts=(32, 256), seed=3, sdf_dim=128):
+    """Independent grooms for the batched form: strand counts log-uniform in
+    `strands`, vertex counts uniform in `verts` (inclusive), each groom on its
+    own 0.1 m sphere with that sphere's SDF on a sdf_dim^3 grid and its own
+    gravity direction (9.81 m/s^2, tilted up to ~35 degrees from -y)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(strands[0]), np.log(strands[1])
+    out = []
+    R = 0.1
+    sdf = sdf_grid(sphere_sdf((0, 0, 0), R), (-1.5 * R,) * 3, (1.5 * R,) * 3, (sdf_dim,) * 3) if sdf_dim else None
+    for g in range(num_grooms):
+        ns = int(round(np.exp(lo + (hi - lo) * rng.random())))
+        counts = rng.integers(verts[0], verts[1] + 1, size=ns)
+        sc = groom(ns, counts, seed=seed * 1000 + g)
+        sc.name = f"batched{g}"
+        # every groom owns its grid (same sphere, separate device copy per handle)
+        sc.sdf = sdf
+        d = np.array([rng.normal() * 0.35, -1.0, rng.normal() * 0.35])
+        d /= np.linalg.norm(d)
+        sc.material = Material(gravity=tuple(float(c) for c in 9.81 * d))
+        out.append(sc)
+    return out
+
+
 def bundle(nx=6, ny=6, nv=32, spacing=6e-5, seg_len=1e-3, jitter=1e-5, seed=7):
     """A grid of nx x ny horizontal strands (along +x) with lateral spacing
     `spacing`; neighbours sit inside the contact barrier (hair-hair contacts)."""

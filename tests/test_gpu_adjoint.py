@@ -76,6 +76,12 @@ def test_adjoint_matches_fp32_oracle(K):
     xo, _ = o.get_pos_vel()
     xg, _ = g.get_pos_vel()
     assert np.array_equal(xg, x) and np.array_equal(xg, xo)
+    # a second reverse pass over the same rollout (the forward pass's own tape
+    # of the last segment has been reused by then) gives the same adjoints
+    xb = np.zeros_like(x)
+    xb[tips] = 2 * (x[tips] - targets.astype(np.float32))
+    ksb2, qrb2 = g.adjoint_backward(xb)
+    assert np.array_equal(ksb2, ksb_g) and np.array_equal(qrb2, qrb_g)
 
 
 def test_adjoint_unsupported_configurations():
