@@ -44,6 +44,13 @@ def hipcc():
     return path
 
 
+HOST = os.path.join(PKG, "host")
+HOST_SOURCES = ["json_lite.cpp", "hair.cpp", "scr_solver.cpp", "solver_cli.cpp"]
+HOST_HEADERS = ["json_lite.h", "hair.h", "scr_solver.h"]
+CLI = os.path.join(PKG, "kscr_solver_cli")
+ROCM_LIB = "/opt/rocm/lib"
+
+
 def build(verbose=True):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -52,7 +59,23 @@ def build(verbose=True):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose=True):
+    """The host-side solver (C++ over the C ABI) and its command-line driver,
+    compiled with the host compiler and linked against libkscr.so."""
+    cxx = shutil.which("g++") or "g++"
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"), "-I", HOST, "-o", tmp] + \
+          [os.path.join(HOST, s) for s in HOST_SOURCES] + \
+          ["-L", PKG, "-lkscr", "-L", ROCM_LIB, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + ROCM_LIB]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 def needs_build():

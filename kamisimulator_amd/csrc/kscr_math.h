@@ -146,6 +146,117 @@ KS_FN float det_logf(float d) {
     return fe * 6.93145752e-1f + (lm + fe * 1.42860677e-6f);
 }
 
+// cos of a double by a fixed sequence of IEEE double operations (host and
+// device give identical bits): x = k pi/2 + r with pi/2 split into parts of
+// 30, 30 and 53 bits (Cody-Waite reduction, exact for |k| < 2^22), |r| <= pi/4,
+// then the Taylor series of cos r (to r^18) or sin r (to r^19) by Horner.
+KS_FN double det_cos(double x) {
+    const double k = rint(x * 0.6366197723675814);
+    const double r = ((x - k * 1.570796325802803) - k * 9.920935791635221e-10) - k * 5.170182981794105e-19;
+    const double r2 = r * r;
+    double c = 1.5619206968586225e-16;  // 1/18!
+    c = c * r2 - 4.779477332387385e-14;
+    c = c * r2 + 1.1470745597729725e-11;
+    c = c * r2 - 2.08767569878681e-09;
+    c = c * r2 + 2.755731922398589e-07;
+    c = c * r2 - 2.48015873015873e-05;
+    c = c * r2 + 0.001388888888888889;
+    c = c * r2 - 0.041666666666666664;
+    c = c * r2 + 0.5;
+    c = 1.0 - c * r2;
+    double sn = 8.22063524662433e-18;  // 1/19!
+    sn = sn * r2 - 2.8114572543455206e-15;
+    sn = sn * r2 + 7.647163731819816e-13;
+    sn = sn * r2 - 1.6059043836821613e-10;
+    sn = sn * r2 + 2.505210838544172e-08;
+    sn = sn * r2 - 2.7557319223985893e-06;
+    sn = sn * r2 + 0.0001984126984126984;
+    sn = sn * r2 - 0.008333333333333333;
+    sn = sn * r2 + 0.16666666666666666;
+    sn = r - (sn * r2) * r;
+    switch ((long long)k & 3) {
+        case 0: return c;
+        case 1: return -sn;
+        case 2: return -c;
+        default: return sn;
+    }
+}
+
+// Natural log of a finite double > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s), s = (m - 1)/(m + 1), odd series to s^25, plus e ln 2
+// with ln 2 split into a 30-bit and a 53-bit part.
+KS_FN double det_log(double x) {
+    int e = 0;
+    double m = frexp(x, &e);
+    if (m < 0.7071067811865476) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 0.04;  // 1/25
+    p = p * s2 + 0.043478260869565216;
+    p = p * s2 + 0.047619047619047616;
+    p = p * s2 + 0.05263157894736842;
+    p = p * s2 + 0.058823529411764705;
+    p = p * s2 + 0.06666666666666667;
+    p = p * s2 + 0.07692307692307693;
+    p = p * s2 + 0.09090909090909091;
+    p = p * s2 + 0.1111111111111111;
+    p = p * s2 + 0.14285714285714285;
+    p = p * s2 + 0.2;
+    p = p * s2 + 0.3333333333333333;
+    p = p * s2 + 1.0;
+    const double de = (double)e;
+    return de * 0.6931471796706319 + (de * 8.893134238886677e-10 + 2.0 * s * p);
+}
+
+// exp of a double: y = k ln 2 + r, |r| <= ln2/2, Taylor series of e^r to
+// r^20, scaled by 2^k.
+KS_FN double det_exp(double y) {
+    const double k = rint(y * 1.4426950408889634);
+    const double r = (y - k * 0.6931471796706319) - k * 8.893134238886677e-10;
+    double p = 4.110317623312165e-19;  // 1/20!
+    p = p * r + 8.22063524662433e-18;
+    p = p * r + 1.5619206968586225e-16;
+    p = p * r + 2.8114572543455206e-15;
+    p = p * r + 4.779477332387385e-14;
+    p = p * r + 7.647163731819816e-13;
+    p = p * r + 1.1470745597729725e-11;
+    p = p * r + 1.6059043836821613e-10;
+    p = p * r + 2.08767569878681e-09;
+    p = p * r + 2.505210838544172e-08;
+    p = p * r + 2.755731922398589e-07;
+    p = p * r + 2.7557319223985893e-06;
+    p = p * r + 2.48015873015873e-05;
+    p = p * r + 0.0001984126984126984;
+    p = p * r + 0.001388888888888889;
+    p = p * r + 0.008333333333333333;
+    p = p * r + 0.041666666666666664;
+    p = p * r + 0.16666666666666666;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, (int)k);
+}
+
+// pow(b, e) for floats through det_log / det_exp in double, rounded once to
+// float.  A negative base needs an integral exponent (else NaN), as powf.
+KS_FN float det_powf(float b, float e) {
+    if (e == 0.0f) return 1.0f;
+    if (b != b || e != e) return NAN;
+    bool neg = false;
+    if (b < 0.0f) {
+        const float ei = rintf(e);
+        if (ei != e) return NAN;
+        neg = fmodf(ei, 2.0f) != 0.0f;
+        b = -b;
+    }
+    if (b == 0.0f) return e > 0.0f ? (neg ? -0.0f : 0.0f) : (neg ? -INFINITY : INFINITY);
+    const float f = (float)det_exp((double)e * det_log((double)b));
+    return neg ? -f : f;
+}
+
 // Clamped closest parameters (u on segment a, v on segment b) of two segments
 // a0-a1 and b0-b1, with the 0.5 fallback for parallel segments.
 KS_FN void segment_closest(f3 a0, f3 a1, f3 b0, f3 b1, float& u, float& v) {

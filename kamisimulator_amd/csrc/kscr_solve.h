@@ -75,12 +75,15 @@ __device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, co
     const f3 g{a.gravity[0], a.gravity[1], a.gravity[2]};
     float windx = 0.f;
     if (a.wind_peak != 0.f) {
+        // cos of the double argument and pow in float (the reference's
+        // cos(2 * M_PI * float) and pow(float, float)), by the deterministic
+        // sequences of kscr_math.h
         const float wy =
-            x.y > 0 ? (float)cos(2 * M_PI * (double)(a.wind_y_freq * x.y + a.wind_time_freq * a.time)) : 0.f;
-        const float wz = (float)cos(2 * M_PI * (double)(a.wind_z_freq * fabsf(x.z) + a.wind_time_freq * a.time));
+            x.y > 0 ? (float)det_cos(2 * M_PI * (double)(a.wind_y_freq * x.y + a.wind_time_freq * a.time)) : 0.f;
+        const float wz = (float)det_cos(2 * M_PI * (double)(a.wind_z_freq * fabsf(x.z) + a.wind_time_freq * a.time));
         const float yz = wy * wz;
         const float sg = (float)((0.f < yz) - (yz < 0.f));
-        windx = a.wind_peak * fmaxf(0.f, powf(yz, a.wind_sharpness) * sg);
+        windx = a.wind_peak * fmaxf(0.f, det_powf(yz, a.wind_sharpness) * sg);
     }
     const f3 ext = (g + f3{windx, 0.f, 0.f}) * a.external_force_multiplier;
     y = v;

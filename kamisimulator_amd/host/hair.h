@@ -1,0 +1,58 @@
+// hair.h -- the solver's exchange view of a groom and the curve preparation
+// the reference applies before building the simulation.
+//
+// Hair mirrors Kami's Hair (src/Hair.h:10-46): hairSize = (strands, maximum
+// vertices per strand); initial and current vertices and current velocities
+// are padded [hairSize.x][hairSize.y] arrays of xyz floats, element (s, v) at
+// index hairSize.y * s + v (src/utilities/LinearUtil.cpp:243-300);
+// strandVertCount holds each strand's used length.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace kami_amd {
+
+struct Vec3 {
+    float x, y, z;
+};
+
+struct Hair {
+    uint32_t num_strands = 0;  // hairSize.x
+    uint32_t max_verts = 0;    // hairSize.y
+    std::vector<uint32_t> strand_vert_count;
+    std::vector<float> initial, current, velocity;  // 3 * num_strands * max_verts each
+    uint64_t vertex_count = 0;                      // Hair::verticesCount
+
+    Hair() = default;
+    Hair(uint32_t strands, uint32_t max_verts);
+    size_t index(uint32_t s, uint32_t v) const { return 3 * ((size_t)max_verts * s + v); }
+    Vec3 get(const std::vector<float>& a, uint32_t s, uint32_t v) const {
+        const size_t k = index(s, v);
+        return Vec3{a[k], a[k + 1], a[k + 2]};
+    }
+    void set(std::vector<float>& a, uint32_t s, uint32_t v, Vec3 p) const {
+        const size_t k = index(s, v);
+        a[k] = p.x, a[k + 1] = p.y, a[k + 2] = p.z;
+    }
+    // Binary exchange file: u32 strands, u32 max_verts, u32 counts[strands],
+    // f32 current[3 * strands * max_verts], f32 velocity[same].
+    static Hair read(const std::string& path);
+    void write(const std::string& path) const;
+};
+
+// Kami::HairMods::DivideHair (src/utilities/HairModification.cpp:12-86):
+// every segment split into `mul` linear pieces; hairSize.y becomes
+// (hairSize.y - 1) * mul + 1.  Empty strands stay empty.
+void divide_hair(Hair& hair, uint32_t mul);
+
+// Resample::resampleCMR (YB/io/resample.h:136-182): the Catmull-Rom spline
+// through `cmr` between parameters start and end, resampled at (close to)
+// equal arc length tar_seg_len.
+std::vector<Vec3> resample_cmr(const std::vector<Vec3>& cmr, double start, double end, double tar_seg_len);
+
+// The spline point at parameter t (Resample::sampleCurve, resample.h:10-18).
+void sample_cmr(const std::vector<Vec3>& cmr, double t, double out[3]);
+
+}  // namespace kami_amd

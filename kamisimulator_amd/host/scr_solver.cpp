@@ -1,0 +1,263 @@
+// scr_solver.cpp -- see scr_solver.h.
+#include "scr_solver.h"
+
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+
+#include "json_lite.h"
+
+namespace kami_amd {
+
+StableCosseratRodsSolver::StableCosseratRodsSolver(std::shared_ptr<Hair> hair, int device)
+    : hair_(std::move(hair)), device_(device) {}
+
+StableCosseratRodsSolver::~StableCosseratRodsSolver() {
+    if (sim_) kscr_destroy(sim_);
+}
+
+void StableCosseratRodsSolver::check(int rc, const char* what) const {
+    if (rc == KSCR_OK) return;
+    char buf[512] = {0};
+    if (sim_) kscr_last_error(sim_, buf, sizeof buf);
+    throw std::runtime_error(std::string(what) + " failed (status " + std::to_string(rc) + "): " + buf);
+}
+
+StableCosseratRodsSolver::JsonParams StableCosseratRodsSolver::LoadSettingJsonContent(const std::string& path,
+                                                                                      std::string* dump) {
+    const JsonFile j = JsonFile::load(path);
+    if (dump) *dump = j.dump();
+    JsonParams p{};
+    if (j.CheckKey("YBJsonPath")) p.YBJsonPath = j.GetString("YBJsonPath");
+    if (j.CheckKey("EnableResample")) p.EnableResample = j.GetBool("EnableResample");
+    if (j.CheckKey("EnableSDF")) p.EnableSDF = j.GetBool("EnableSDF");
+    if (j.CheckKey("EnableHairHairColl")) p.EnableHairHairColl = j.GetBool("EnableHairHairColl");
+    if (j.CheckKey("UseYoungsModule")) p.UseYoungsModule = j.GetBool("UseYoungsModule");
+    if (j.CheckKey("NumItr")) p.NumItr = j.GetInteger("NumItr");
+    if (j.CheckKey("SagFreeItr")) p.SagFreeItr = j.GetInteger("SagFreeItr");
+    if (j.CheckKey("SubdivCoeff")) p.SubdivCoeff = j.GetInteger("SubdivCoeff");
+    if (j.CheckKey("BendK")) p.BendK = j.GetFloat("BendK");
+    if (j.CheckKey("StretchK")) p.StretchK = j.GetFloat("StretchK");
+    if (j.CheckKey("CurveRadius")) p.CurveRadius = j.GetFloat("CurveRadius");
+    if (j.CheckKey("Density")) p.Density = j.GetFloat("Density");
+    if (j.CheckKey("YoungsModule")) p.YoungsModule = j.GetFloat("YoungsModule");
+    if (j.CheckKey("StiffnessDeviationCoeff")) p.StiffnessDeviationCoeff = j.GetFloat("StiffnessDeviationCoeff");
+    if (j.CheckKey("ResampleLen")) p.ResampleLen = j.GetFloat("ResampleLen");
+    if (j.CheckKey("VelocityDecay")) p.VelocityDecay = j.GetFloat("VelocityDecay");
+    if (j.CheckKey("ExternalForceMultiplier")) p.ExternalForceMultiplier = j.GetFloat("ExternalForceMultiplier");
+    if (j.CheckKey("WindPeak")) p.WindPeak = j.GetFloat("WindPeak");
+    if (j.CheckKey("WindYFreq")) p.WindYFreq = j.GetFloat("WindYFreq");
+    if (j.CheckKey("WindZFreq")) p.WindZFreq = j.GetFloat("WindZFreq");
+    if (j.CheckKey("WindTimeFreq")) p.WindTimeFreq = j.GetFloat("WindTimeFreq");
+    if (j.CheckKey("WindSharpness")) p.WindSharpness = j.GetFloat("WindSharpness");
+    if (j.CheckKey("Gravity")) {
+        const std::vector<double> g = j.GetNumbers("Gravity");
+        if (g.size() != 3) throw std::runtime_error("Gravity must be an array of 3 numbers");
+        p.HasGravity = true;
+        for (int k = 0; k < 3; k++) p.Gravity[k] = (float)g[k];
+    }
+    if (j.CheckKey("SDFThreshold")) p.SDFThreshold = j.GetFloat("SDFThreshold");
+    if (j.CheckKey("MaxSubstep")) p.MaxSubstep = j.GetFloat("MaxSubstep");
+    if (j.CheckKey("Schedule")) {
+        const std::string s = j.GetString("Schedule");
+        if (s == "jacobi")
+            p.Schedule = KSCR_SCHEDULE_JACOBI;
+        else if (s == "colored")
+            p.Schedule = KSCR_SCHEDULE_COLORED;
+        else
+            throw std::runtime_error("Schedule must be \"jacobi\" or \"colored\"");
+    }
+    if (j.CheckKey("BlockSize")) p.BlockSize = j.GetInteger("BlockSize");
+    return p;
+}
+
+void StableCosseratRodsSolver::SubstituteBodySdf(const BodySdf& sdf) {
+    const size_t nodes = (size_t)sdf.dim[0] * sdf.dim[1] * sdf.dim[2];
+    if (sdf.phi.size() != nodes || sdf.grad.size() != 3 * nodes) throw std::runtime_error("SDF grid size mismatch");
+    sdf_.reset(new BodySdf(sdf));
+    if (sim_)
+        check(kscr_set_sdf(sim_, sdf_->dim, sdf_->grid_min, sdf_->grid_max, sdf_->phi.data(), sdf_->grad.data(),
+                           jp_.SDFThreshold, sdf_->rotation, sdf_->translation),
+              "kscr_set_sdf");
+}
+
+// StableCosseratRodsSolver::BuildYarnBallHair (StableCosseratRodsSolver.cpp:112-173),
+// curve part.
+void StableCosseratRodsSolver::PrepareCurves() {
+    if (!hair_) throw std::runtime_error("no hair substituted");
+    Hair& h = *hair_;
+    if (jp_.SubdivCoeff > 1) divide_hair(h, (uint32_t)jp_.SubdivCoeff);
+    root_vertices_.clear();
+    strand_refs_.clear();
+    flat_pos_.clear();
+    flat_vel_.clear();
+    strand_offsets_.assign(1, 0);
+    int num_verts = 0;
+    for (uint32_t s = 0; s < h.num_strands; s++) {
+        std::vector<Vec3> curve;
+        for (uint32_t v = 0; v < h.strand_vert_count[s]; v++) curve.push_back(h.get(h.current, s, v));
+        if (curve.size() < 4) continue;
+        if (jp_.EnableResample) {
+            curve = resample_cmr(curve, 1, (double)curve.size() - 2, jp_.ResampleLen);
+            const size_t new_len = curve.size();
+            if (h.max_verts < new_len) {
+                // cap at hairSize.y keeping both ends
+                std::vector<Vec3> capped(h.max_verts);
+                const float stride = (float)(new_len - 2) / (float)(h.max_verts - 2);
+                capped[0] = curve[0];
+                for (uint32_t v = 1; v + 1 < h.max_verts; v++) capped[v] = curve[(uint32_t)(stride * v)];
+                capped[h.max_verts - 1] = curve.back();
+                curve = capped;
+            }
+            h.vertex_count += (uint64_t)curve.size() - h.strand_vert_count[s];
+            h.strand_vert_count[s] = (uint32_t)curve.size();
+            for (uint32_t v = 0; v < curve.size(); v++) h.set(h.current, s, v, curve[v]);
+        }
+        root_vertices_.push_back(num_verts);
+        strand_refs_.push_back((int)s);
+        num_verts += (int)h.strand_vert_count[s];
+        for (const Vec3& p : curve) {
+            flat_pos_.push_back(p.x);
+            flat_pos_.push_back(p.y);
+            flat_pos_.push_back(p.z);
+        }
+        strand_offsets_.push_back(num_verts);
+    }
+    flat_vel_.assign(flat_pos_.size(), 0.f);
+}
+
+void StableCosseratRodsSolver::build_sim() {
+    if (flat_pos_.empty()) throw std::runtime_error("no strand has 4 or more vertices");
+    kscr_params p;
+    kscr_params_default(&p);
+    const float r = jp_.CurveRadius;
+    const float ratio = 0.05f;
+    p.radius = ratio * r;
+    p.barrier_thickness = 2 * (1 - ratio) * r;
+    p.num_itr = jp_.NumItr;
+    p.velocity_decay = jp_.VelocityDecay;
+    p.use_step_size_limit = jp_.EnableHairHairColl;
+    p.use_velocity_radius = 1;
+    p.enable_hair_hair_coll = jp_.EnableHairHairColl;
+    p.external_force_multiplier = jp_.ExternalForceMultiplier;
+    p.wind_peak = jp_.WindPeak;
+    p.wind_y_freq = jp_.WindYFreq;
+    p.wind_z_freq = jp_.WindZFreq;
+    p.wind_time_freq = jp_.WindTimeFreq;
+    p.wind_sharpness = jp_.WindSharpness;
+    if (jp_.HasGravity)
+        for (int k = 0; k < 3; k++) p.gravity[k] = jp_.Gravity[k];
+    p.max_h = jp_.MaxSubstep;
+    p.schedule = jp_.Schedule;
+    p.block_size = jp_.BlockSize;
+    const int nv = NumSimVertices();
+    const int ns = (int)strand_offsets_.size() - 1;
+    int rc = kscr_create(&p, nv, flat_pos_.data(), ns, strand_offsets_.data(), device_, &sim_);
+    if (rc) throw std::runtime_error(std::string("kscr_create failed: ") + kscr_create_error());
+    check(kscr_configure(sim_, jp_.Density), "configure");
+    if (jp_.UseYoungsModule) {
+        check(kscr_set_young(sim_, jp_.YoungsModule, jp_.StiffnessDeviationCoeff), "setYoung");
+    } else {
+        check(kscr_set_kbend(sim_, jp_.BendK), "setKBend");
+        check(kscr_set_kstretch(sim_, jp_.StretchK), "setKStretch");
+    }
+    check(kscr_pin_vertices(sim_, root_vertices_.data(), (int)root_vertices_.size()), "pin roots");
+    check(kscr_upload(sim_), "upload");
+    if (sdf_)
+        check(kscr_set_sdf(sim_, sdf_->dim, sdf_->grid_min, sdf_->grid_max, sdf_->phi.data(), sdf_->grad.data(),
+                           jp_.SDFThreshold, sdf_->rotation, sdf_->translation),
+              "kscr_set_sdf");
+}
+
+// StableCosseratRodsSolver::Construct (StableCosseratRodsSolver.cpp:33-61)
+void StableCosseratRodsSolver::LoadSettings(const std::string& settingJsonPath) {
+    if (!settingJsonPath.empty() && std::ifstream(settingJsonPath).good())
+        jp_ = LoadSettingJsonContent(settingJsonPath, &json_dump_);
+    if (!jp_.YBJsonPath.empty() && std::ifstream(jp_.YBJsonPath).good())
+        throw std::runtime_error("YBJsonPath scenes are not supported by this solver build");
+    if (jp_.SagFreeItr > 0) throw std::runtime_error("SagFreeItr > 0 is not supported by this solver build");
+}
+
+void StableCosseratRodsSolver::Construct(const std::string& settingJsonPath) {
+    LoadSettings(settingJsonPath);
+    if (sim_) {
+        kscr_destroy(sim_);
+        sim_ = nullptr;
+    }
+    PrepareCurves();
+    build_sim();
+}
+
+// StableCosseratRodsSolver::SolveNextStep (StableCosseratRodsSolver.cpp:178-198):
+// advance, then the SDF pass (here on the device), then the Hair is updated.
+void StableCosseratRodsSolver::SolveNextStep(float dt) {
+    if (!sim_) throw std::runtime_error("Construct() has not been called");
+    if (jp_.EnableSDF) {
+        if (!sdf_) throw std::runtime_error("EnableSDF is set but no body SDF was substituted (SubstituteBodySdf)");
+        check(kscr_solve_next_step(sim_, dt), "solve_next_step");
+    } else {
+        check(kscr_advance(sim_, dt), "advance");
+    }
+    DownloadHair();
+}
+
+// StableCosseratRodsSolver::UploadHair (StableCosseratRodsSolver.cpp:202-221)
+void StableCosseratRodsSolver::UploadHair() {
+    if (!sim_) throw std::runtime_error("Construct() has not been called");
+    const Hair& h = *hair_;
+    const int nv = NumSimVertices();
+    for (size_t i = 0; i < strand_refs_.size(); i++) {
+        const uint32_t u = (uint32_t)strand_refs_[i];
+        const int root = root_vertices_[i];
+        const int next = (i + 1 == strand_refs_.size()) ? nv : root_vertices_[i + 1];
+        for (int v = 0; root + v < next; v++) {
+            const Vec3 p = h.get(h.current, u, (uint32_t)v), w = h.get(h.velocity, u, (uint32_t)v);
+            float* dp = &flat_pos_[3 * (size_t)(root + v)];
+            float* dv = &flat_vel_[3 * (size_t)(root + v)];
+            dp[0] = p.x, dp[1] = p.y, dp[2] = p.z;
+            dv[0] = w.x, dv[1] = w.y, dv[2] = w.z;
+        }
+    }
+    check(kscr_set_pos_vel(sim_, flat_pos_.data(), flat_vel_.data()), "upload positions and velocities");
+}
+
+// StableCosseratRodsSolver::DownloadHair (StableCosseratRodsSolver.cpp:223-243)
+void StableCosseratRodsSolver::DownloadHair() {
+    if (!sim_) throw std::runtime_error("Construct() has not been called");
+    check(kscr_get_pos_vel(sim_, flat_pos_.data(), flat_vel_.data()), "download positions and velocities");
+    Hair& h = *hair_;
+    const int nv = NumSimVertices();
+    for (size_t i = 0; i < strand_refs_.size(); i++) {
+        const uint32_t u = (uint32_t)strand_refs_[i];
+        const int root = root_vertices_[i];
+        const int next = (i + 1 == strand_refs_.size()) ? nv : root_vertices_[i + 1];
+        for (int v = 0; root + v < next; v++) {
+            const float* sp = &flat_pos_[3 * (size_t)(root + v)];
+            const float* sv = &flat_vel_[3 * (size_t)(root + v)];
+            h.set(h.current, u, (uint32_t)v, Vec3{sp[0], sp[1], sp[2]});
+            h.set(h.velocity, u, (uint32_t)v, Vec3{sv[0], sv[1], sv[2]});
+        }
+    }
+}
+
+// StableCosseratRodsSolver::GetInfoString (StableCosseratRodsSolver.cpp:245-269)
+std::string StableCosseratRodsSolver::GetInfoString() const {
+    std::stringstream s;
+    s << "Solver: Stable Cosserat Rods (MI355X HIP)" << std::endl;
+    s << "SubdivCoeff: " << jp_.SubdivCoeff << std::endl;
+    if (jp_.UseYoungsModule) {
+        s << "Use Youngs: Yes" << std::endl;
+        s << "Youngs: " << jp_.YoungsModule << std::endl;
+        s << "Deviation: " << jp_.StiffnessDeviationCoeff << std::endl;
+    } else {
+        s << "Use Youngs: No" << std::endl;
+        s << "StretchK: " << jp_.StretchK << std::endl;
+        s << "BendK: " << jp_.BendK << std::endl;
+    }
+    s << "Sag-free itr: " << jp_.SagFreeItr << std::endl;
+    if (sim_) s << "Simulated vertices: " << NumSimVertices() << ", workgroup " << kscr_block_size(sim_) << std::endl;
+    s << "Json:" << std::endl << json_dump_;
+    return s.str();
+}
+
+}  // namespace kami_amd

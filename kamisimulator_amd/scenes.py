@@ -32,6 +32,11 @@ class Material:
     velocity_decay: float = 0.05     # VelocityDecay
     gravity: tuple = (0.0, -9.81, 0.0)
     wind_peak: float = 0.0
+    # the wind shape of the SCR solver's JSON defaults (used when wind_peak != 0)
+    wind_y_freq: float = 1.0
+    wind_z_freq: float = 0.7
+    wind_time_freq: float = 0.4
+    wind_sharpness: float = 3.0
     external_force_multiplier: float = 1.0
 
 
@@ -82,6 +87,10 @@ def make_params(lib_default_params, material: Material, block_size=0, schedule=0
         enable_hair_hair_coll=0,
         external_force_multiplier=float(material.external_force_multiplier),
         wind_peak=float(material.wind_peak),
+        wind_y_freq=float(material.wind_y_freq),
+        wind_z_freq=float(material.wind_z_freq),
+        wind_time_freq=float(material.wind_time_freq),
+        wind_sharpness=float(material.wind_sharpness),
         gravity=tuple(float(g) for g in material.gravity),
         block_size=int(block_size),
         schedule=int(schedule),

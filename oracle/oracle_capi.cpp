@@ -372,4 +372,11 @@ void oracle_sym_solve(const float* H, const float* f, float* out) {
     orc::V3<float> r = orc::sym_solve(S, orc::V3<float>{f[0], f[1], f[2]});
     out[0] = r.x, out[1] = r.y, out[2] = r.z;
 }
+// Deterministic wind functions of the FP32 restatement (vectorised over n).
+void oracle_wind_cos(const double* x, double* out, long n) {
+    for (long i = 0; i < n; i++) out[i] = orc::wind_cos(x[i]);
+}
+void oracle_wind_powf(const float* b, const float* e, float* out, long n) {
+    for (long i = 0; i < n; i++) out[i] = orc::wind_powf(b[i], e[i]);
+}
 }  // extern "C"

@@ -44,6 +44,8 @@ SIGNATURES = {
     "oracle_set_threads": (C.c_int, [_P, C.c_int]),
     "oracle_num_verts": (C.c_int, [_P]),
     "oracle_rotor_mul": (None, [_FP, _FP, _FP]),
+    "oracle_wind_cos": (None, [_DP, _DP, C.c_long]),
+    "oracle_wind_powf": (None, [_FP, _FP, _FP, C.c_long]),
     "oracle_rotor_rotate": (None, [_FP, _FP, _FP]),
     "oracle_rotor_from_to": (None, [_FP, _FP, _FP]),
     "oracle_segment_closest_points": (None, [_FP, _FP, _FP, _FP, _FP]),
@@ -327,6 +329,26 @@ def segment_closest_points(a0, a1, b0, b1):
 
 def morton3(x, y, z):
     return load().oracle_morton3(int(x), int(y), int(z))
+
+
+def wind_cos(x):
+    """cos of doubles by the restatement's fixed operation sequence."""
+    lib = load()
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    lib.oracle_wind_cos(x.ctypes.data_as(_DP), out.ctypes.data_as(_DP), x.size)
+    return out
+
+
+def wind_powf(b, e):
+    """powf of float pairs by the restatement's fixed operation sequence."""
+    lib = load()
+    b, e = np.broadcast_arrays(np.asarray(b, np.float32), np.asarray(e, np.float32))
+    b = np.ascontiguousarray(b)
+    e = np.ascontiguousarray(e)
+    out = np.empty_like(b)
+    lib.oracle_wind_powf(_f(b), _f(e), _f(out), b.size)
+    return out
 
 
 def sym_solve(H6, f):

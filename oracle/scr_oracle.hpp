@@ -266,6 +266,89 @@ inline float det_logf(float d) {
 }
 inline double det_logf(double d) { return std::log(d); }
 
+// Wind terms of initItr (YB/sim/iteration.cu:22-25): cos of a double argument
+// and pow of floats.  The FP32 restatement evaluates them by fixed sequences
+// of IEEE double operations, the same the device uses:
+//   cos: x = k pi/2 + r, pi/2 in parts of 30, 30 and 53 bits, |r| <= pi/4,
+//        Taylor series of cos r to r^18 / sin r to r^19;
+//   log: x = m 2^e, m in [sqrt(1/2), sqrt(2)), 2 atanh((m-1)/(m+1)) to s^25;
+//   exp: y = k ln2 + r, Taylor series to r^20, times 2^k;
+//   powf(b, e) = (float) exp(e log b), negative bases need integral e.
+// Pinned against the C library (tests/test_oracle_primitives.py).  The FP64
+// oracle uses std::cos / std::pow.
+inline double wind_cos(double x) {
+    const double k = std::rint(x * 0.6366197723675814);
+    const double r = ((x - k * 1.570796325802803) - k * 9.920935791635221e-10) - k * 5.170182981794105e-19;
+    const double r2 = r * r;
+    static const double cc[9] = {1.5619206968586225e-16, -4.779477332387385e-14, 1.1470745597729725e-11,
+                                 -2.08767569878681e-09,  2.755731922398589e-07,  -2.48015873015873e-05,
+                                 0.001388888888888889,   -0.041666666666666664,  0.5};
+    static const double sc[9] = {8.22063524662433e-18,  -2.8114572543455206e-15, 7.647163731819816e-13,
+                                 -1.6059043836821613e-10, 2.505210838544172e-08,  -2.7557319223985893e-06,
+                                 0.0001984126984126984,  -0.008333333333333333,   0.16666666666666666};
+    double c = cc[0];
+    for (int j = 1; j < 9; j++) c = c * r2 + cc[j];
+    c = 1.0 - c * r2;
+    double sn = sc[0];
+    for (int j = 1; j < 9; j++) sn = sn * r2 + sc[j];
+    sn = r - (sn * r2) * r;
+    const long long q = (long long)k & 3;
+    return q == 0 ? c : (q == 1 ? -sn : (q == 2 ? -c : sn));
+}
+inline double wind_log(double x) {
+    int e = 0;
+    double m = std::frexp(x, &e);
+    if (m < 0.7071067811865476) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 0.04;
+    for (int d = 23; d >= 3; d -= 2) p = p * s2 + 1.0 / (double)d;
+    p = p * s2 + 1.0;
+    const double de = (double)e;
+    return de * 0.6931471796706319 + (de * 8.893134238886677e-10 + 2.0 * s * p);
+}
+inline double wind_exp(double y) {
+    const double k = std::rint(y * 1.4426950408889634);
+    const double r = (y - k * 0.6931471796706319) - k * 8.893134238886677e-10;
+    static const double inv_fact[21] = {4.110317623312165e-19, 8.22063524662433e-18,   1.5619206968586225e-16,
+                                        2.8114572543455206e-15, 4.779477332387385e-14, 7.647163731819816e-13,
+                                        1.1470745597729725e-11, 1.6059043836821613e-10, 2.08767569878681e-09,
+                                        2.505210838544172e-08,  2.755731922398589e-07,  2.7557319223985893e-06,
+                                        2.48015873015873e-05,   0.0001984126984126984,  0.001388888888888889,
+                                        0.008333333333333333,   0.041666666666666664,   0.16666666666666666,
+                                        0.5,                    1.0,                    1.0};
+    double p = inv_fact[0];
+    for (int j = 1; j < 21; j++) p = p * r + inv_fact[j];
+    return std::ldexp(p, (int)k);
+}
+inline float wind_powf(float b, float e) {
+    if (e == 0.0f) return 1.0f;
+    if (std::isnan(b) || std::isnan(e)) return NAN;
+    bool neg = false;
+    if (b < 0.0f) {
+        const float ei = std::rint(e);
+        if (ei != e) return NAN;
+        neg = std::fmod(ei, 2.0f) != 0.0f;
+        b = -b;
+    }
+    if (b == 0.0f) return e > 0.0f ? (neg ? -0.0f : 0.0f) : (neg ? -INFINITY : INFINITY);
+    const float f = (float)wind_exp((double)e * wind_log((double)b));
+    return neg ? -f : f;
+}
+template <class T>
+inline T wind_cos_t(double x) {
+    return (T)wind_cos(x);
+}
+template <>
+inline double wind_cos_t<double>(double x) {
+    return std::cos(x);
+}
+inline float wind_pow_t(float b, float e) { return wind_powf(b, e); }
+inline double wind_pow_t(double b, double e) { return std::pow(b, e); }
+
 // Symmetric 3x3 accumulation helpers (hess3 operators, SymMat.h:31-64)
 template <class T>
 inline void sym_add(Sym3<T>& a, const Sym3<T>& b) {
@@ -534,11 +617,11 @@ struct Sim {
             if (p.wind_peak != 0) {
                 // cos in double of a float argument (M_PI is double), pow in T
                 T yp = x[i].y, zp = x[i].z;
-                T wy = yp > 0 ? (T)std::cos(2 * M_PI * (double)((T)p.wind_y_freq * yp + (T)p.wind_time_freq * time)) : T(0);
-                T wz = (T)std::cos(2 * M_PI * (double)((T)p.wind_z_freq * std::fabs(zp) + (T)p.wind_time_freq * time));
+                T wy = yp > 0 ? wind_cos_t<T>(2 * M_PI * (double)((T)p.wind_y_freq * yp + (T)p.wind_time_freq * time)) : T(0);
+                T wz = wind_cos_t<T>(2 * M_PI * (double)((T)p.wind_z_freq * std::fabs(zp) + (T)p.wind_time_freq * time));
                 T yz = wy * wz;
                 T sg = (T)((T(0) < yz) - (yz < T(0)));
-                T pw = std::pow(yz, (T)p.wind_sharpness) * sg;
+                T pw = wind_pow_t(yz, (T)p.wind_sharpness) * sg;
                 T mx = std::isnan(pw) ? T(0) : std::max(T(0), pw);  // fmax(0, NaN) = 0
                 windx = (T)p.wind_peak * mx;
             }

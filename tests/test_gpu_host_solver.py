@@ -1,0 +1,64 @@
+"""The host-side solver (kamisimulator_amd/host: the reference plugin's
+Construct / SolveNextStep / UploadHair / DownloadHair over the C ABI) run end
+to end through its command-line driver, against the FP32 oracle stepping the
+same strands: Hair exchange in the padded [strands][hairSize.y] layout,
+strands shorter than 4 vertices left untouched, wind and the body SDF on."""
+import json
+
+import numpy as np
+import pytest
+
+from kamisimulator_amd import hairio, scenes
+from kamisimulator_amd._lib import default_params
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+DT = np.float32(1 / 600)
+
+
+def test_solver_cli_matches_oracle(tmp_path):
+    rng = np.random.default_rng(9)
+    ns, mv, steps = 80, 64, 30
+    counts = rng.integers(2, mv + 1, size=ns)
+    counts[:5] = [2, 3, 1, 64, 4]
+    g = scenes.groom(ns, counts, seed=9)
+    cur = np.zeros((ns, mv, 3), np.float32)
+    for s in range(ns):
+        a, b = g.strand_offsets[s], g.strand_offsets[s + 1]
+        cur[s, :b - a] = g.positions[a:b]
+    R = 0.1
+    grid = scenes.sdf_grid(scenes.sphere_sdf((0, 0, 0), R), (-1.5 * R,) * 3, (1.5 * R,) * 3, (48,) * 3)
+    settings = {"SubdivCoeff": 1, "EnableResample": False, "EnableSDF": True, "NumItr": 10, "YoungsModule": 3,
+                "CurveRadius": 4e-5, "Density": 1300, "Gravity": [0.0, -9.81, 0.0], "SDFThreshold": 0.0}
+    (tmp_path / "s.json").write_text(json.dumps(settings))
+    hairio.write_hair(tmp_path / "in.bin", counts, cur)
+    hairio.write_sdf(tmp_path / "sdf.bin", grid)
+    print(f"[gpu-host-solver] {ns} strands, hairSize.y {mv}, {steps} steps", flush=True)
+    r = hairio.cli("run", tmp_path / "s.json", tmp_path / "in.bin", tmp_path / "sdf.bin", steps, repr(float(DT)),
+                   tmp_path / "out.bin")
+    refs = json.loads(r.stdout)
+    c2, got, vel = hairio.read_hair(tmp_path / "out.bin")
+    keep = [s for s in range(ns) if counts[s] >= 4]
+    assert refs["strandRefs"] == keep
+    # the oracle on the kept strands, set up the way Construct does
+    pos = np.concatenate([cur[s, :counts[s]] for s in keep])
+    off = np.concatenate([[0], np.cumsum([counts[s] for s in keep])]).astype(np.int32)
+    assert refs["rootVertices"] == off[:-1].tolist()
+    m = scenes.Material(num_itr=10, young_gpa=3.0, wind_peak=40.0, gravity=(0.0, -9.81, 0.0))
+    p = scenes.make_params(default_params, m)
+    sc = scenes.Scene("kept", pos, off, material=m, sdf=grid, sdf_threshold=0.0)
+    o = scenes.setup_sim(OracleSim(p, pos, off, precision=32), sc)
+    o.set_threads(8)
+    for _ in range(steps):
+        o.solve_next_step(DT)
+    op, ov = o.get_pos_vel()
+    nbad = 0
+    for k, s in enumerate(keep):
+        a, b = off[k], off[k + 1]
+        nbad += int(np.count_nonzero(got[s, :counts[s]].view(np.uint32) != op[a:b].view(np.uint32)))
+        nbad += int(np.count_nonzero(vel[s, :counts[s]].view(np.uint32) != ov[a:b].view(np.uint32)))
+    print(f"[gpu-host-solver] differing position/velocity words: {nbad}", flush=True)
+    assert nbad == 0
+    for s in range(ns):
+        if counts[s] < 4:
+            np.testing.assert_array_equal(got[s], cur[s])

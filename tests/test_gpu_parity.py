@@ -71,6 +71,15 @@ def test_cantilever_100_steps():
     assert ep <= TOL and eq <= TOL
 
 
+def test_groom_with_wind_100_steps():
+    """Wind on (the SCR solver's default WindPeak 40, sharpness 3): cos and
+    pow of initItr come from the deterministic sequences both sides share."""
+    sc = scenes.config_groom(1000, 100, with_sdf=True, sdf_dim=64)
+    sc.material = scenes.Material(wind_peak=40.0)
+    ep, eq, ev, _, _ = _run(sc, 100, fp64=True, label="groom1k-wind")
+    assert ep <= TOL and eq <= TOL and ev <= TOL
+
+
 def test_cantilever_root_pin_only():
     sc = scenes.cantilever(64, 1e-3, clamp=False)
     ep, eq, _, _, _ = _run(sc, 100, label="cantilever-pin-only")

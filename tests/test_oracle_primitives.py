@@ -78,3 +78,35 @@ def test_rotor_algebra_identities(oracle_lib):
         t /= np.linalg.norm(t)
         q = pyoracle.rotor_from_to(np.array([1, 0, 0], np.float32), t)
         np.testing.assert_allclose(pyoracle.rotor_rotate(q, np.array([1, 0, 0], np.float32)), t, atol=1e-5)
+
+
+def test_wind_cos_against_libm():
+    """The restatement's cos (Cody-Waite + Taylor, in double) against the C
+    library on the arguments initItr produces (2 pi (f y + f_t t)) and on a
+    wide sweep, including the quadrant boundaries."""
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-50, 50, 200_000), rng.uniform(-3e4, 3e4, 50_000),
+                        np.arange(-64, 65) * (np.pi / 2), [0.0, 1e-300, -1e-300]])
+    got = pyoracle.wind_cos(x)
+    want = np.cos(x)
+    ulp = np.spacing(np.maximum(np.abs(want), 1e-16))
+    err = np.abs(got - want)
+    # a few ulp of the result, or of the argument's own reduction error near zeros
+    assert (err <= 4 * ulp + 2e-16 * np.maximum(1.0, np.abs(x) / 1e3)).all(), err.max()
+    # cast to float as initItr does: equal to the libm float in all but rare halfway cases
+    assert np.mean(got.astype(np.float32) == want.astype(np.float32)) > 0.9999
+
+
+def test_wind_powf_against_libm():
+    rng = np.random.default_rng(2)
+    b = rng.uniform(0, 1, 200_000).astype(np.float32)
+    e = rng.choice(np.array([3.0, 1.0, 2.0, 0.5, 2.5, 7.0], np.float32), b.size)
+    got = pyoracle.wind_powf(b, e)
+    want = np.power(b.astype(np.float64), e.astype(np.float64)).astype(np.float32)
+    assert np.mean(got == want) > 0.9999
+    ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1
+    # powf's special cases: negative base with integral / non-integral exponent, zeros
+    np.testing.assert_array_equal(pyoracle.wind_powf([-0.5, -0.5, 0.0, 0.3], [3.0, 2.0, 3.0, 0.0]),
+                                  np.array([-0.125, 0.25, 0.0, 1.0], np.float32))
+    assert np.isnan(pyoracle.wind_powf([-0.5], [2.5])[0])
