@@ -38,6 +38,9 @@ struct Base {
     virtual int error() const = 0;
     virtual void contacts(int* num, int* cols) const = 0;
     virtual void set_material(const float* ks, const float* qr) = 0;
+    virtual void detect_only(float dt) = 0;
+    virtual void max_step(float* out) const = 0;
+    virtual void get_dx(float* out) const = 0;
     virtual void set_material_t(const double* ks, const double* qr) = 0;
     virtual double gradient(float dt, int nsteps, int K, const int* tips, int ntips, const double* targets, double* ksb,
                             double* qrb) = 0;
@@ -141,6 +144,13 @@ struct Impl : Base {
     long num_pairs() const override { return s.num_pairs; }
     int error() const override { return s.error; }
     void set_material(const float* ks, const float* qr) override { s.set_material(ks, qr); }
+    void detect_only(float dt) override { s.detect_only(dt); }
+    void max_step(float* out) const override {
+        for (int i = 0; i < s.n; i++) out[i] = (float)s.maxStep[i];
+    }
+    void get_dx(float* out) const override {
+        for (int i = 0; i < s.n; i++) out[3 * i] = (float)s.dx[i].x, out[3 * i + 1] = (float)s.dx[i].y, out[3 * i + 2] = (float)s.dx[i].z;
+    }
     void set_material_t(const double* ks, const double* qr) override { s.set_material_t(ks, qr); }
     double gradient(float dt, int nsteps, int K, const int* tips, int ntips, const double* targets, double* ksb,
                     double* qrb) override {
@@ -284,6 +294,10 @@ int oracle_num_verts(oracle_handle* h) { return h->b->nverts(); }
 long oracle_num_pairs(oracle_handle* h) { return h->b->num_pairs(); }
 int oracle_error(oracle_handle* h) { return h->b->error(); }
 void oracle_contacts(oracle_handle* h, int* num, int* cols) { h->b->contacts(num, cols); }
+// initItr + forced detection + step limits of one substep of dt (no iterations)
+void oracle_detect_only(oracle_handle* h, float dt) { h->b->detect_only(dt); }
+void oracle_max_step(oracle_handle* h, float* out) { h->b->max_step(out); }
+void oracle_dx(oracle_handle* h, float* out) { h->b->get_dx(out); }
 int oracle_set_material(oracle_handle* h, const float* ks, const float* qr) {
     h->b->set_material(ks, qr);
     return 0;
@@ -371,6 +385,24 @@ void oracle_sym_solve(const float* H, const float* f, float* out) {
     orc::Sym3<float> S{{H[0], H[1], H[2], H[3], H[4], H[5]}};
     orc::V3<float> r = orc::sym_solve(S, orc::V3<float>{f[0], f[1], f[2]});
     out[0] = r.x, out[1] = r.y, out[2] = r.z;
+}
+// One contact pair (orc::contact_pair) in double: in = xi, xj, dxi, dxj, b0,
+// b1, db0, db1 (24 values); out = f (3), H (6), f2 (3), H2 (6), starting from 0.
+void oracle_contact_pair(const void* params, const double* in, double damp, double* out) {
+    const orc::Params& p = *(const orc::Params*)params;
+    orc::V3<double> v[8];
+    for (int k = 0; k < 8; k++) v[k] = orc::V3<double>{in[3 * k], in[3 * k + 1], in[3 * k + 2]};
+    orc::V3<double> f{0, 0, 0}, f2{0, 0, 0};
+    orc::Sym3<double> H{{0, 0, 0, 0, 0, 0}}, H2{{0, 0, 0, 0, 0, 0}};
+    orc::contact_pair<double>(p, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], damp, f, H, f2, H2);
+    out[0] = f.x, out[1] = f.y, out[2] = f.z;
+    for (int k = 0; k < 6; k++) out[3 + k] = H.d[k];
+    out[9] = f2.x, out[10] = f2.y, out[11] = f2.z;
+    for (int k = 0; k < 6; k++) out[12 + k] = H2.d[k];
+}
+// det_logf of the FP32 restatement (vectorised).
+void oracle_det_logf(const float* d, float* out, long n) {
+    for (long i = 0; i < n; i++) out[i] = orc::det_logf(d[i]);
 }
 // Deterministic wind functions of the FP32 restatement (vectorised over n).
 void oracle_wind_cos(const double* x, double* out, long n) {

@@ -45,6 +45,11 @@ SIGNATURES = {
     "oracle_num_verts": (C.c_int, [_P]),
     "oracle_rotor_mul": (None, [_FP, _FP, _FP]),
     "oracle_wind_cos": (None, [_DP, _DP, C.c_long]),
+    "oracle_contact_pair": (None, [C.c_void_p, _DP, C.c_double, _DP]),
+    "oracle_det_logf": (None, [_FP, _FP, C.c_long]),
+    "oracle_detect_only": (None, [C.c_void_p, C.c_float]),
+    "oracle_max_step": (None, [C.c_void_p, _FP]),
+    "oracle_dx": (None, [C.c_void_p, _FP]),
     "oracle_wind_powf": (None, [_FP, _FP, _FP, C.c_long]),
     "oracle_rotor_rotate": (None, [_FP, _FP, _FP]),
     "oracle_rotor_from_to": (None, [_FP, _FP, _FP]),
@@ -248,6 +253,20 @@ class OracleSim:
         self.lib.oracle_contacts(self.h, _i(num), _i(cols))
         return num, cols
 
+    def detect_only(self, dt):
+        """initItr + a forced detection + step limits of one substep (no iterations)."""
+        self.lib.oracle_detect_only(self.h, dt)
+
+    def max_step(self):
+        out = np.empty(self.n, np.float32)
+        self.lib.oracle_max_step(self.h, _f(out))
+        return out
+
+    def dx(self):
+        out = np.empty((self.n, 3), np.float32)
+        self.lib.oracle_dx(self.h, _f(out))
+        return out
+
 
 class OracleLbvh:
     """CPU restatement of Kitten::LBVH (lbvh_oracle.hpp)."""
@@ -329,6 +348,24 @@ def segment_closest_points(a0, a1, b0, b1):
 
 def morton3(x, y, z):
     return load().oracle_morton3(int(x), int(y), int(z))
+
+
+def contact_pair(params, xi, xj, dxi, dxj, b0, b1, db0, db1, damp=0.0):
+    """orc::contact_pair in double: (f, H6, f2, H2_6) from zero accumulators."""
+    lib = load()
+    inp = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).reshape(3)
+                                               for a in (xi, xj, dxi, dxj, b0, b1, db0, db1)]))
+    out = np.zeros(18, np.float64)
+    lib.oracle_contact_pair(C.byref(params), inp.ctypes.data_as(_DP), float(damp), out.ctypes.data_as(_DP))
+    return out[0:3], out[3:9], out[9:12], out[12:18]
+
+
+def det_logf(d):
+    lib = load()
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.empty_like(d)
+    lib.oracle_det_logf(_f(d), _f(out), d.size)
+    return out
 
 
 def wind_cos(x):

@@ -416,6 +416,61 @@ struct Sdf {
     std::vector<float> grad;  // 3*dim^3
 };
 
+// One contact of segment (xi, xj) with segment (b0, b1), both displaced by
+// their current dx (cosseratItr, YB/sim/cosserat.cu:171-227): IPC log barrier
+// on the normalised gap d = (|closest-point distance| - 2 radius) / barrier,
+// force -b'(d) kCol and Gauss-Newton Hessian b''(d) kCol / barrier along the
+// contact normal, b(d) = -(d - 1)^2 log d, split between the segment's two
+// vertices by the closest-point parameter; Rayleigh damping; and friction on
+// the tangential relative displacement, capped by the Coulomb cone.
+// Adds vertex i's terms to (f, H) and vertex i+1's to (f2, H2).
+template <class T>
+inline void contact_pair(const Params& p, V3<T> xi, V3<T> xj, V3<T> dxi, V3<T> dxj, V3<T> b0, V3<T> b1, V3<T> db0,
+                         V3<T> db1, T damp, V3<T>& f, Sym3<T>& H, V3<T>& f2, Sym3<T>& H2) {
+    const T invb = T(1) / T(p.barrier_thickness);
+    const T radius = T(2) * T(p.radius);
+    const T kCol = T(p.k_collision) * invb;
+    T ux, uy;
+
+        segment_closest_points(V3<T>{0, 0, 0}, (xj - xi) + (dxj - dxi), (b0 - xi) + (db0 - dxi), (b1 - xi) + (db1 - dxi),
+                               ux, uy);
+        const V3<T> dpos = mix(xi, xj, ux) - mix(b0, b1, uy);
+        const V3<T> ddpos = mix(dxi, dxj, ux) - mix(db0, db1, uy);
+        V3<T> nrm = dpos + ddpos;
+        T d = length(nrm);
+        nrm = nrm * (T(1) / d);
+        const T wj = ux;
+        const T wi = T(1) - ux;
+        d = d - radius;
+        d = d * invb;
+        if (d > 1) return;
+        d = std::max(d, T(1e-3f));
+        const T invd = T(1) / d;
+        const T logd = det_logf(d);
+        T dH = (T(-3) + (T(2) + invd) * invd - T(2) * logd) * kCol * invb;
+        const T ff = -(T(1) - d) * (d - T(1) + T(2) * d * logd) * invd * kCol;
+        f = f + (ff * wi - damp * dH * wi * wi * dot(nrm, dxi)) * nrm;
+        f2 = f2 + (ff * wj - damp * dH * wj * wj * dot(nrm, dxj)) * nrm;
+        dH = dH * (T(1) + damp);
+        Sym3<T> op = sym_outer(nrm);
+        sym_add(H, sym_scale(op, dH * wi * wi));
+        sym_add(H2, sym_scale(op, dH * wj * wj));
+        const V3<T> u = ddpos - dot(nrm, ddpos) * nrm;
+        const T ul = length(u);
+        if (ul > 0) {
+            const T cand = T(p.friction_coeff) * ff / ul;
+            const T f1 = cand < T(p.k_friction) ? cand : T(p.k_friction);  // glm::min(fricK, cand)
+            op.d[0] -= 1;
+            op.d[1] -= 1;
+            op.d[2] -= 1;
+            f = f - (f1 * wi) * u;
+            sym_sub(H, sym_scale(op, (wi * wi) * f1));
+            f2 = f2 - (f1 * wj) * u;
+            sym_sub(H2, sym_scale(op, (wj * wj) * f1));
+        }
+    }
+
+
 // ---------------------------------------------------------------- simulator
 template <class T>
 struct Sim {
@@ -659,48 +714,20 @@ struct Sim {
 
     // One contact of segment i with segment c (cosserat.cu:171-227).
     void contact_terms(int i, int c, T damp, V3<T>& f, Sym3<T>& H, V3<T>& f2, Sym3<T>& H2) const {
-        const T invb = T(1) / T(p.barrier_thickness);
-        const T radius = T(2) * T(p.radius);
-        const T kCol = T(p.k_collision) * invb;
-        const V3<T> xi = x[i], xj = x[i + 1], dxi = dx[i], dxj = dx[i + 1];
-        const V3<T> b0 = x[c], b1 = x[c + 1], db0 = dx[c], db1 = dx[c + 1];
-        T ux, uy;
-        segment_closest_points(V3<T>{0, 0, 0}, (xj - xi) + (dxj - dxi), (b0 - xi) + (db0 - dxi), (b1 - xi) + (db1 - dxi),
-                               ux, uy);
-        const V3<T> dpos = mix(xi, xj, ux) - mix(b0, b1, uy);
-        const V3<T> ddpos = mix(dxi, dxj, ux) - mix(db0, db1, uy);
-        V3<T> nrm = dpos + ddpos;
-        T d = length(nrm);
-        nrm = nrm * (T(1) / d);
-        const T wj = ux;
-        const T wi = T(1) - ux;
-        d = d - radius;
-        d = d * invb;
-        if (d > 1) return;
-        d = std::max(d, T(1e-3f));
-        const T invd = T(1) / d;
-        const T logd = det_logf(d);
-        T dH = (T(-3) + (T(2) + invd) * invd - T(2) * logd) * kCol * invb;
-        const T ff = -(T(1) - d) * (d - T(1) + T(2) * d * logd) * invd * kCol;
-        f = f + (ff * wi - damp * dH * wi * wi * dot(nrm, dxi)) * nrm;
-        f2 = f2 + (ff * wj - damp * dH * wj * wj * dot(nrm, dxj)) * nrm;
-        dH = dH * (T(1) + damp);
-        Sym3<T> op = sym_outer(nrm);
-        sym_add(H, sym_scale(op, dH * wi * wi));
-        sym_add(H2, sym_scale(op, dH * wj * wj));
-        const V3<T> u = ddpos - dot(nrm, ddpos) * nrm;
-        const T ul = length(u);
-        if (ul > 0) {
-            const T cand = T(p.friction_coeff) * ff / ul;
-            const T f1 = cand < T(p.k_friction) ? cand : T(p.k_friction);  // glm::min(fricK, cand)
-            op.d[0] -= 1;
-            op.d[1] -= 1;
-            op.d[2] -= 1;
-            f = f - (f1 * wi) * u;
-            sym_sub(H, sym_scale(op, (wi * wi) * f1));
-            f2 = f2 - (f1 * wj) * u;
-            sym_sub(H2, sym_scale(op, (wj * wj) * f1));
-        }
+        contact_pair<T>(p, x[i], x[i + 1], dx[i], dx[i + 1], x[c], x[c + 1], dx[c], dx[c + 1], damp, f, H, f2, H2);
+    }
+    // Substep preamble only (initItr, a forced detection, step limits), for
+    // the contact tests: x is unchanged, dx holds the initial guess.
+    void detect_only(float dtF) {
+        const int steps = std::max(1, (int)std::ceil(dtF / p.max_h));
+        lastHF = hF;
+        hF = dtF / steps;
+        lastH = (T)lastHF;
+        h = (T)hF;
+        init_itr();
+        lastBVHRebuild = std::numeric_limits<T>::infinity();
+        detect_collisions();
+        recompute_step_limit();
     }
 
     // Everything segment i (between vertices i and i+1) adds, sector by sector.

@@ -1,0 +1,207 @@
+"""Pins of the oracle's hair-hair contact path (oracle/scr_oracle.hpp:
+contact_pair, detect_collisions, recompute_step_limit, det_logf).
+
+* The contact force is minus the gradient, and the Hessian the Gauss-Newton
+  second derivative, of the IPC barrier energy E = kC b(d),
+  b(d) = -(d - 1)^2 log d, d = (distance - 2 radius) / barrier (central finite
+  differences of E over an exact segment-segment distance).
+* Friction acts against the tangential relative displacement, grows linearly
+  with it and saturates at the Coulomb cone mu |normal force|.
+* The contact lists equal a brute-force search over all segment pairs, and the
+  step limits the closed form min(detR (scaler - 1) + |dx|, 0.45 distance).
+* det_logf agrees with the C library's log to 2 ulp (relative error < 2^-22)."""
+import numpy as np
+import pytest
+
+from kamisimulator_amd import scenes
+from kamisimulator_amd._lib import default_params
+from oracle import pyoracle
+from oracle.pyoracle import OracleSim
+
+F32 = np.float32
+
+
+def _params(**kw):
+    m = scenes.Material()
+    p = scenes.contact_params(default_params, m)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _seg_dist(a0, a1, b0, b1):
+    """Exact distance between segments a0-a1 and b0-b1, and the parameter on a."""
+    a0, a1, b0, b1 = (np.asarray(v, np.float64) for v in (a0, a1, b0, b1))
+    d1, d2, r = a1 - a0, b1 - b0, a0 - b0
+    best = (np.inf, 0.0)
+
+    def point_seg(p, s0, s1):
+        d = s1 - s0
+        t = np.clip(np.dot(p - s0, d) / np.dot(d, d), 0, 1)
+        return np.linalg.norm(p - (s0 + t * d)), t
+
+    for s in (0.0, 1.0):
+        dist, _ = point_seg(a0 + s * d1, b0, b1)
+        best = min(best, (dist, s))
+    for t in (0.0, 1.0):
+        dist, s = point_seg(b0 + t * d2, a0, a1)
+        best = min(best, (dist, s))
+    A = np.array([[d1 @ d1, -(d1 @ d2)], [-(d1 @ d2), d2 @ d2]])
+    if abs(np.linalg.det(A)) > 1e-30:
+        s, t = np.linalg.solve(A, [-(d1 @ r), d2 @ r])
+        if 0 <= s <= 1 and 0 <= t <= 1:
+            best = min(best, (np.linalg.norm(a0 + s * d1 - b0 - t * d2), s))
+    return best
+
+
+def _barrier(d):
+    return -(d - 1) ** 2 * np.log(d)
+
+
+def _crossing(gap):
+    xi, xj = np.array([0.0, 0.0, 0.0]), np.array([1e-3, 0.0, 0.0])
+    b0, b1 = np.array([0.35e-3, gap, -0.5e-3]), np.array([0.55e-3, gap, 0.5e-3])
+    return xi, xj, b0, b1
+
+
+def test_barrier_force_and_hessian_are_energy_derivatives():
+    p = _params()
+    B, r2 = float(F32(p.barrier_thickness)), 2 * float(F32(p.radius))
+    kC = float(F32(p.k_collision))
+    z = np.zeros(3)
+    for d_target in (0.05, 0.37, 0.8):
+        xi, xj, b0, b1 = _crossing(r2 + d_target * B)
+        f, H, f2, H2 = pyoracle.contact_pair(p, xi, xj, z, z, b0, b1, z, z, damp=0.0)
+
+        def energy(xa, xb):
+            dist, _ = _seg_dist(xa, xb, b0, b1)
+            return kC * _barrier((dist - r2) / B)
+
+        h = 1e-10
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            gi = (energy(xi + e, xj) - energy(xi - e, xj)) / (2 * h)
+            gj = (energy(xi, xj + e) - energy(xi, xj - e)) / (2 * h)
+            assert f[k] == pytest.approx(-gi, rel=1e-5, abs=1e-9 * np.abs(f).max())
+            assert f2[k] == pytest.approx(-gj, rel=1e-5, abs=1e-9 * np.abs(f2).max())
+        # Hessian: rank one along the contact normal n; translating the whole
+        # segment along n changes the distance one-for-one, so
+        # E'' = kC b''(d) / B^2 = n^T H n / w_i^2
+        dist, s = _seg_dist(xi, xj, b0, b1)
+        n = np.array([0.0, -1.0, 0.0])
+        Hm = np.array([[H[0], H[3], H[4]], [H[3], H[1], H[5]], [H[4], H[5], H[2]]])
+        ht = 1e-9
+        e2 = (energy(xi + ht * n, xj + ht * n) - 2 * energy(xi, xj) + energy(xi - ht * n, xj - ht * n)) / ht ** 2
+        assert n @ Hm @ n / (1 - s) ** 2 == pytest.approx(e2, rel=2e-4)
+        np.testing.assert_allclose(Hm, (n @ Hm @ n) * np.outer(n, n), atol=1e-9 * abs(n @ Hm @ n))
+
+
+def test_barrier_range_and_clamp():
+    p = _params()
+    B, r2 = float(F32(p.barrier_thickness)), 2 * float(F32(p.radius))
+    z = np.zeros(3)
+    # beyond the barrier (d > 1): no contribution
+    f, H, f2, H2 = pyoracle.contact_pair(p, *_crossing(r2 + 1.01 * B)[:2], z, z, *_crossing(r2 + 1.01 * B)[2:], z, z)
+    assert not f.any() and not H.any() and not f2.any() and not H2.any()
+    # inside 1e-3 of the barrier the gap is clamped at d = 1e-3
+    fa = pyoracle.contact_pair(p, *_crossing(r2 + 1e-4 * B)[:2], z, z, *_crossing(r2 + 1e-4 * B)[2:], z, z)[0]
+    fb = pyoracle.contact_pair(p, *_crossing(r2 + 1e-3 * B)[:2], z, z, *_crossing(r2 + 1e-3 * B)[2:], z, z)[0]
+    np.testing.assert_allclose(fa, fb, rtol=1e-6, atol=1e-9 * np.abs(fb).max())
+    assert fa[1] < 0  # pushes segment a away from b (b lies at +y)
+
+
+def test_friction_opposes_sliding_and_saturates():
+    # a weak contact (d = 0.95) so the Coulomb cone is reached within a slide
+    # of ~20 um; segment a slides along its own axis, which keeps the gap
+    p = _params()
+    B, r2 = float(F32(p.barrier_thickness)), 2 * float(F32(p.radius))
+    xi, xj, b0, b1 = _crossing(r2 + 0.95 * B)
+    z = np.zeros(3)
+    mu, kF = float(F32(p.friction_coeff)), float(F32(p.k_friction))
+    _, s = _seg_dist(xi, xj, b0, b1)
+    wi = 1 - s
+    for t, regime in ((1e-10, "linear"), (1e-9, "linear"), (1e-4, "cone"), (3e-4, "cone")):
+        slide = np.array([t, 0.0, 0.0])
+        f, _, _, _ = pyoracle.contact_pair(p, xi, xj, slide, slide, b0, b1, z, z)
+        normal = abs(f[1])  # |ff| w_i along the contact normal (-y)
+        fr = f[0]
+        assert fr < 0  # against the motion
+        assert abs(fr) <= mu * normal * (1 + 1e-9)  # inside the Coulomb cone
+        if regime == "linear":
+            assert abs(fr) / (t * wi) == pytest.approx(kF, rel=1e-4)
+        else:
+            assert abs(fr) == pytest.approx(mu * normal, rel=1e-9)
+
+
+def test_det_logf_against_libm():
+    d = np.concatenate([np.geomspace(1e-3, 1.0, 100_000), np.linspace(1e-3, 1.0, 100_000)]).astype(F32)
+    got = pyoracle.det_logf(d)
+    want = np.log(d.astype(np.float64)).astype(F32)
+    ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    # a float series: within 2 ulp (relative error < 2^-22), mostly correctly rounded
+    rel = np.abs(got - np.log(d.astype(np.float64))) / np.abs(np.log(d.astype(np.float64)) + 1e-300)
+    assert ulps.max() <= 2 and rel.max() < 2.0 ** -22 and np.mean(ulps == 0) > 0.8
+
+
+def test_contact_lists_match_brute_force():
+    sc = scenes.ribbon(12, 32)
+    p = _params()
+    o = OracleSim(p, sc.positions, sc.strand_offsets, precision=32)
+    scenes.setup_sim(o, sc)
+    o.detect_only(F32(1 / 600))
+    num, cols = o.contacts()
+    x = sc.positions.astype(np.float64)
+    dx = o.dx().astype(np.float64)
+    rest = o.get_rest()
+    has_next = (rest["flags"] & 2) != 0
+    scaled = float(F32(F32(p.radius) + F32(0.5) * F32(p.barrier_thickness)) * F32(p.detection_scaler))
+    want = {i: [] for i in range(sc.num_verts)}
+    segs = np.nonzero(has_next)[0]
+    for ia in segs:
+        for ib in segs:
+            if ib <= ia + 2:
+                continue
+            dist, _ = _seg_dist(x[ia], x[ia + 1], x[ib], x[ib + 1])
+            r = 2 * scaled
+            r += np.sqrt(max(dx[ia] @ dx[ia], dx[ia + 1] @ dx[ia + 1]))
+            r += np.sqrt(max(dx[ib] @ dx[ib], dx[ib + 1] @ dx[ib + 1]))
+            assert abs(dist - r) > 1e-3 * r  # no borderline pair in this scene
+            if dist < r:
+                want[ia].append(ib)
+                want[ib].append(ia)
+    for i in range(sc.num_verts):
+        assert sorted(want[i]) == list(cols[i, :num[i]]), i
+    assert sum(len(v) for v in want.values()) > 0
+
+
+def test_step_limit_closed_form():
+    # two strands crossing at a 12 um gap: the crossing segments' limit is
+    # 0.45 x gap, every other segment's detR (scaler - 1) + |dx|
+    n = 16
+    a = np.stack([np.arange(n) * 1e-3, np.zeros(n), np.zeros(n)], 1)
+    b = np.stack([np.full(n, 7.5e-3), np.full(n, 1.2e-5), (np.arange(n) - 7.5) * 1e-3], 1)
+    pos = np.concatenate([a, b]).astype(F32)
+    off = np.array([0, n, 2 * n], np.int32)
+    sc = scenes.Scene("cross", pos, off)
+    p = _params()
+    o = OracleSim(p, pos, off, precision=64)
+    scenes.setup_sim(o, sc)
+    o.detect_only(F32(1 / 600))
+    num, cols = o.contacts()
+    ms = o.max_step().astype(np.float64)
+    dx = o.dx().astype(np.float64)
+    detR = float(F32(p.radius) + F32(0.5) * F32(p.barrier_thickness))
+    base = detR * (float(F32(p.detection_scaler)) - 1) + np.linalg.norm(dx, axis=1)
+    flags = o.get_rest()["flags"]
+    for i in range(2 * n):
+        if not flags[i] & 2:
+            assert np.isinf(ms[i])
+            continue
+        want = base[i]
+        for c in cols[i, :num[i]]:
+            dist, _ = _seg_dist(pos[i], pos[i + 1], pos[c], pos[c + 1])
+            want = min(want, 0.45 * dist)
+        assert ms[i] == pytest.approx(want, rel=1e-6), i
+    # the crossing pair is in contact and limited by the gap
+    assert num[7] >= 1 and ms[7] == pytest.approx(0.45 * 1.2e-5, rel=1e-5)
