@@ -361,6 +361,19 @@ def main():
         sdf_leg = {"ms_per_step": round(ms2, 4), "updates_per_s": updates_per_step / (ms2 / 1e3),
                    "sdf_grid": "128^3 sphere SDF, threshold 0"}
 
+    # Host exchange of the plugin interface: SolveNextStep downloads positions
+    # and velocities every step (StableCosseratRodsSolver.cpp:190, :223-243).
+    sim.get_pos_vel()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        sim.get_pos_vel()
+    dl_ms = (time.perf_counter() - w0) * 1e3 / reps
+    exchange = {"download_ms": round(dl_ms, 3), "bytes": int(24 * scene.num_verts),
+                "pcie_inclusive_updates_per_s": updates_per_step / ((ms_per_step + dl_ms) / 1e3),
+                "note": "kscr_get_pos_vel (synchronous device-to-host copy of x and v into pageable host "
+                        "arrays) once per step, as the plugin's DownloadHair; not part of value"}
     sim.close()
     adjoint = None
     if not args.no_adjoint_leg:
@@ -403,6 +416,7 @@ def main():
         "roofline": roofline,
         "streaming_sweep_equivalent_frac": round(streaming_equiv, 4),
         "with_sdf_step": sdf_leg,
+        "host_exchange": exchange,
         "adjoint": adjoint,
         "batched": batched,
         "cpu_baseline": cpu,
