@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-adjoint-leg", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-batched-leg", action="store_true", help="skip the batched-vs-per-groom timing")
+    ap.add_argument("--no-contacts-leg", action="store_true", help="skip the large groom with contacts")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--only-legs", action="store_true", help="run the adjoint and batched legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -260,6 +261,50 @@ def batched_leg(stream, nsteps, strands=(1_000, 50_000), num_grooms=64):
     }
 
 
+def contacts_leg(stream, nsteps=10, strands=100_000):
+    """Large groom with hair-hair contacts: 100k strands with lognormal vertex
+    counts (median 96, clipped 32-256) on evenly spaced (spiral) roots with
+    0.2-0.6 mm curls, sphere + capsule SDF on 256^3, LBVH broad phase, contacts
+    at 2 x CurveRadius, dt = 1/600.  (Uniformly random roots put some roots
+    microns apart; their strands then run side by side and exceed the 16
+    contacts per segment the step allows, which the reference treats as fatal.)"""
+    import torch
+
+    from kamisimulator_amd import scenes
+    from kamisimulator_amd._lib import KscrError, Sim, default_params
+
+    t0 = time.perf_counter()
+    sc = scenes.config_large(strands, seed=2, sdf_dim=256, curl=(2e-4, 6e-4), roots="spiral")
+    p = scenes.contact_params(default_params, sc.material)
+    sim = Sim(p, sc.positions, sc.strand_offsets, device=0)
+    sim.set_stream(stream.cuda_stream)
+    scenes.setup_sim(sim, sc)
+    setup_s = time.perf_counter() - t0
+    dt = np.float32(1 / 600)
+    S = int(np.ceil(dt / np.float32(p.max_h)))
+    out = {"workload": f"{strands} strands, lognormal vertex counts ({sc.num_verts} vertices), spiral roots, 0.2-0.6 mm "
+                       f"curls, 256^3 sphere + capsule SDF, hair-hair contacts, {nsteps} steps of 1/600 s ({S} substeps x "
+                       f"{p.num_itr} iterations)"}
+    try:
+        sim.solve_next_step(dt)
+        sim.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(nsteps):
+            sim.solve_next_step(dt)
+        b.record(stream)
+        sim.synchronize()
+        ms = a.elapsed_time(b) / nsteps
+        num, _ = sim.get_contacts()
+        out.update({"ms_per_step": round(ms, 3), "updates_per_s": sc.num_verts * S * p.num_itr / (ms / 1e3),
+                    "contacts": int(num.sum()), "max_contacts_per_segment": int(num.max()),
+                    "setup_s": round(setup_s, 1)})
+    except KscrError as e:
+        out["error"] = str(e)
+    sim.close()
+    return out
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -286,6 +331,8 @@ def main():
             print(f"[bench] adjoint leg {legs['adjoint']}", file=sys.stderr, flush=True)
         if not args.no_batched_leg:
             legs["batched"] = batched_leg(stream, args.batched_steps)
+        if not args.no_contacts_leg:
+            legs["contacts"] = contacts_leg(stream)
         print(json.dumps(legs), flush=True)
         return
 
@@ -386,6 +433,11 @@ def main():
         print(f"[bench] batched leg: per-groom {batched['per_groom_launches_ms_per_step']} ms/step, graph "
               f"{batched['batched_graph_ms_per_step']} ms/step", file=sys.stderr, flush=True)
 
+    contacts = None
+    if not args.no_contacts_leg:
+        contacts = contacts_leg(stream)
+        print(f"[bench] contacts leg: {contacts}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, params, args.cpu_seconds)
@@ -419,6 +471,7 @@ def main():
         "host_exchange": exchange,
         "adjoint": adjoint,
         "batched": batched,
+        "contacts": contacts,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

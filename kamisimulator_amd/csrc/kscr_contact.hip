@@ -167,6 +167,7 @@ __device__ inline void contact_terms(const StepArgs& a, const ContactParams& cp,
     const f3 ddpos = mix3(dxi, dxj, ux) - mix3(db0, db1, uy);
     f3 nrm = dpos + ddpos;
     float d = length(nrm);
+    if (!(d > 0.0f)) return;  // closest points coincide: no contact normal (the reference divides by 0)
     nrm = nrm * (1.0f / d);
     const float wj = ux;
     const float wi = 1.0f - ux;
@@ -278,6 +279,13 @@ __global__ void position_kernel(StepArgs a, ContactParams cp, int n, const float
         sadd(H, H2);
     }
     f3 out = dxi + a.acceleration_ratio * sym_solve(H.d[0], H.d[1], H.d[2], H.d[3], H.d[4], H.d[5], f);
+    // A contact Hessian can make H numerically singular in FP32 (the adjugate
+    // determinant cancels to 0); the reference's step is then inf/NaN, which
+    // no step limit removes.  Such a step is skipped.
+    if (!isfinite(out.x) || !isfinite(out.y) || !isfinite(out.z)) {
+        st3(dx_out, i, dxi);  // the previous iterate, as it is
+        return;
+    }
     const float l = length(out);
     if (l > limit && l > 0) out = out * (limit / l);
     st3(dx_out, i, out);

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace kscr {
 
@@ -39,8 +40,19 @@ struct LbvhDev {
     int* flags = nullptr;
     int* max_depth = nullptr;
     int* count = nullptr;  // query pair counter
+    // internal nodes grouped by height (leaves 0): refit runs one level per launch
+    uint32_t* node_height = nullptr;
+    uint32_t* level_nodes = nullptr;
+    uint32_t* level_count = nullptr;   // kMaxLevels counts, then kMaxLevels scatter cursors
+    uint32_t* level_offset = nullptr;  // kMaxLevels
+    std::vector<int2> levels;          // (begin, count) per height >= 1, host side
+    int max_depth_host = 64;           // DFS stack bound of the built tree
     bool built = false;
 };
+
+// Karras trees over 62-bit keys (30-bit Morton code, 32-bit index) are at
+// most 63 levels tall.
+constexpr int kMaxLevels = 64;
 
 hipError_t lbvh_reserve(LbvhDev& d, int n);
 void lbvh_release(LbvhDev& d);

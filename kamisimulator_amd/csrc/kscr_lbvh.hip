@@ -281,75 +281,129 @@ __global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n,
 
 // Bottom-up refit (lbvh.cu:153-188): the second thread to reach a node
 // merges both children and continues; depth = max(dL, dR + 1).
-__global__ void refit_kernel(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
-                             const uint32_t* __restrict__ leaf_parent, LbvhNode* nodes, int* flags, int n,
-                             int* max_depth) {
+// Tree levels, once per build.  Every leaf walks up; at each internal node the
+// first arriving child leaves its height, DFS stack depth and side in the
+// node's word with one atomicAdd and stops; the second reads them back from
+// the atomic's return value and continues.  No other memory passes between the
+// two, so no fence is needed (fences would write back and invalidate the L2 of
+// every XCD).  height(leaf) = 0, height(node) = 1 + max(children); stack depth
+// (the DFS bound of mergeUpKernel, lbvh.cu:153-188): leaf 1, node max(dL, dR + 1).
+__global__ void height_kernel(const uint32_t* __restrict__ leaf_parent, const LbvhNode* __restrict__ nodes,
+                              uint32_t* __restrict__ word, int n, uint32_t* __restrict__ node_height,
+                              uint32_t* __restrict__ level_count, int* __restrict__ max_depth) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    int depth = 1;
-    float last[6];
-    const float* a = aabb + 6 * (size_t)ids[t];
-    for (int k = 0; k < 6; k++) last[k] = a[k];
+    uint32_t h = 0, d = 1;
     uint32_t parent = leaf_parent[t];
     for (;;) {
-        const int right = (parent >> 31) & 1;
+        const uint32_t right = parent >> 31;
         const uint32_t p = parent & 0x7FFFFFFFu;
-        volatile float* dst = nodes[p].b[right];
-        for (int k = 0; k < 6; k++) dst[k] = last[k];
-        __threadfence();
-        const int other = atomicOr(flags + p, depth);
-        if (!other) return;
-        __threadfence();
-        depth = right ? max(depth + 1, other) : max(depth, other + 1);
+        const uint32_t old = atomicAdd(word + p, 1u | (h << 2) | (d << 10) | (right << 18));
+        if ((old & 3u) == 0) return;  // first child here
+        const uint32_t oh = (old >> 2) & 255u, od = (old >> 10) & 255u;
+        const uint32_t dl = right ? od : d, dr = right ? d : od;
+        h = 1 + (h > oh ? h : oh);
+        d = dl > dr + 1 ? dl : dr + 1;
+        node_height[p] = h;
+        // a height past the bound can only come from a corrupt tree; the host
+        // then finds the level counts short and reports an error
+        if (h < (uint32_t)kMaxLevels) atomicAdd(level_count + h, 1u);
         if (p == 0) {
-            *max_depth = depth;
+            *max_depth = (int)d;
             return;
         }
-        const volatile float* sib = nodes[p].b[1 - right];
-        for (int k = 0; k < 3; k++) {
-            const float smn = sib[k], smx = sib[3 + k];
-            last[k] = fminf(last[k], smn);
-            last[3 + k] = fmaxf(last[3 + k], smx);
+        parent = nodes[p].parent;
+    }
+}
+
+// Internal nodes grouped by height (offsets from the host).
+__global__ void level_scatter_kernel(const uint32_t* __restrict__ node_height, int m,
+                                     const uint32_t* __restrict__ level_offset, uint32_t* __restrict__ cursor,
+                                     uint32_t* __restrict__ level_nodes) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const uint32_t h = node_height[p];
+    level_nodes[level_offset[h] + atomicAdd(cursor + h, 1u)] = (uint32_t)p;
+}
+
+// Child boxes of the internal nodes of one height; their children are leaves
+// or nodes of smaller height, finished by earlier launches.  Unions take glm's
+// min/max (Bound::absorb), so a NaN coordinate propagates as in the reference.
+__global__ void refit_level_kernel(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
+                                   LbvhNode* __restrict__ nodes, const uint32_t* __restrict__ level_nodes, int begin,
+                                   int count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t p = level_nodes[begin + k];
+    const uint32_t child[2] = {nodes[p].left, nodes[p].right};
+    for (int side = 0; side < 2; side++) {
+        const uint32_t c = child[side];
+        float b[6];
+        if (c & 0x80000000u) {
+            const float* a = aabb + 6 * (size_t)ids[c & 0x7FFFFFFFu];
+            for (int j = 0; j < 6; j++) b[j] = a[j];
+        } else {
+            const LbvhNode& cn = nodes[c];
+            for (int j = 0; j < 3; j++) {
+                b[j] = cn.b[1][j] < cn.b[0][j] ? cn.b[1][j] : cn.b[0][j];
+                b[3 + j] = cn.b[0][3 + j] < cn.b[1][3 + j] ? cn.b[1][3 + j] : cn.b[0][3 + j];
+            }
         }
-        parent = *(volatile uint32_t*)&nodes[p].parent;
+        for (int j = 0; j < 6; j++) nodes[p].b[side][j] = b[j];
     }
 }
 
 // ---------------------------------------------------------------- query
 // Self query with IGNORE_SELF (lbvh.cu:196-289).  Each lane keeps its DFS
 // stack in LDS (column per lane).
+template <int STACK>
 __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restrict__ aabb,
                                                             const uint32_t* __restrict__ ids,
                                                             const LbvhNode* __restrict__ nodes, int n,
                                                             int2* __restrict__ pairs, int cap, int min_gap,
                                                             int* __restrict__ count, int* __restrict__ error) {
-    __shared__ uint32_t stack[kQueryStack][kQueryBlock];
+    __shared__ uint32_t stack[STACK][kQueryBlock];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n || n < 2) return;
     const int qid = (int)ids[t];
     float q[6];
     for (int k = 0; k < 6; k++) q[k] = aabb[6 * (size_t)qid + k];
     if (!(q[0] <= q[3])) return;  // empty box (strand tip): intersects nothing
+    const uint64_t lane_below = (1ull << (threadIdx.x & 63)) - 1;
     int sp = 0;
     stack[sp++][threadIdx.x] = 0;
     while (sp > 0) {
         uint32_t c = stack[--sp][threadIdx.x];
+        int fid = -1;
         if (c & 0x80000000u) {
             c &= 0x7FFFFFFFu;
-            if ((int)c <= t) continue;
-            const int fid = (int)ids[c];
-            if (min_gap >= 0 && abs(fid - qid) <= min_gap) continue;
-            const int slot = atomicAdd(count, 1);
-            if (slot < cap) pairs[slot] = make_int2(fid, qid);
+            if ((int)c > t) {
+                fid = (int)ids[c];
+                if (min_gap >= 0 && abs(fid - qid) <= min_gap) fid = -1;
+            }
         } else {
             const LbvhNode& nd = nodes[c];
-            if (max(c, nd.fence) <= (uint32_t)t) continue;
-            if (sp + 2 > kQueryStack) {  // deeper than the LDS stack: report, stop this query
-                if (error) atomicOr(error, 4);
-                break;
+            if (max(c, nd.fence) > (uint32_t)t) {
+                if (sp + 2 > STACK) {  // deeper than the LDS stack: report, stop this query
+                    if (error) atomicOr(error, 4);
+                    break;
+                }
+                if (intersects(nd.b[0], q)) stack[sp++][threadIdx.x] = nd.left;
+                if (intersects(nd.b[1], q)) stack[sp++][threadIdx.x] = nd.right;
             }
-            if (intersects(nd.b[0], q)) stack[sp++][threadIdx.x] = nd.left;
-            if (intersects(nd.b[1], q)) stack[sp++][threadIdx.x] = nd.right;
+        }
+        // the lanes of the wave that found a pair in this pass take consecutive
+        // slots with one atomic (a single global counter would serialise)
+        const uint64_t found = __ballot(fid >= 0);
+        if (found) {
+            const int leader = __ffsll((unsigned long long)found) - 1;
+            int base = 0;
+            if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, __popcll(found));
+            base = __shfl(base, leader);
+            if (fid >= 0) {
+                const int slot = base + __popcll(found & lane_below);
+                if (slot < cap) pairs[slot] = make_int2(fid, qid);
+            }
         }
     }
 }
@@ -373,8 +427,9 @@ hipError_t fill_i32(int* p, int value, size_t n, hipStream_t stream) {
 }
 
 void lbvh_release(LbvhDev& d) {
-    void* ps[] = {d.aabb, d.codes, d.ids, d.codes_tmp, d.ids_tmp, d.hist, d.hist_sums, d.partial,
-                  d.root, d.nodes, d.leaf_parent, d.flags, d.max_depth, d.count};
+    void* ps[] = {d.aabb,  d.codes,       d.ids,         d.codes_tmp, d.ids_tmp,   d.hist,
+                  d.hist_sums, d.partial,   d.root,        d.nodes,     d.leaf_parent, d.flags,
+                  d.max_depth, d.count,     d.node_height, d.level_nodes, d.level_count, d.level_offset};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     d = LbvhDev{};
@@ -393,7 +448,10 @@ hipError_t lbvh_reserve(LbvhDev& d, int n) {
         (e = dmalloc(d.hist_sums, chunks)) != hipSuccess || (e = dmalloc(d.partial, 6 * 1024)) != hipSuccess ||
         (e = dmalloc(d.root, 6)) != hipSuccess || (e = dmalloc(d.nodes, (size_t)(n > 1 ? n - 1 : 1))) != hipSuccess ||
         (e = dmalloc(d.leaf_parent, n)) != hipSuccess || (e = dmalloc(d.flags, n)) != hipSuccess ||
-        (e = dmalloc(d.max_depth, 1)) != hipSuccess || (e = dmalloc(d.count, 1)) != hipSuccess) {
+        (e = dmalloc(d.max_depth, 1)) != hipSuccess || (e = dmalloc(d.count, 1)) != hipSuccess ||
+        (e = dmalloc(d.node_height, (size_t)n)) != hipSuccess || (e = dmalloc(d.level_nodes, (size_t)n)) != hipSuccess ||
+        (e = dmalloc(d.level_count, (size_t)2 * kMaxLevels)) != hipSuccess ||
+        (e = dmalloc(d.level_offset, (size_t)kMaxLevels)) != hipSuccess) {
         lbvh_release(d);
         return e;
     }
@@ -436,18 +494,52 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
                            d.leaf_parent);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    d.levels.clear();
+    if (n <= 1) {
+        d.built = true;
+        d.max_depth_host = 1;
+        return fill_i32(d.max_depth, 1, 1, stream);
+    }
+    // heights and stack depths (topology only), then the nodes grouped by height
+    const int m = n - 1;
+    if ((e = fill_i32(d.flags, 0, (size_t)m, stream)) != hipSuccess ||
+        (e = fill_i32((int*)d.level_count, 0, (size_t)2 * kMaxLevels, stream)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(height_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d.leaf_parent, d.nodes,
+                       (uint32_t*)d.flags, n, d.node_height, d.level_count, d.max_depth);
+    uint32_t counts[kMaxLevels];
+    int depth = 0;
+    if ((e = hipMemcpyAsync(counts, d.level_count, sizeof counts, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(&depth, d.max_depth, sizeof depth, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(stream)) != hipSuccess)
+        return e;
+    d.max_depth_host = depth;
+    uint32_t offsets[kMaxLevels];
+    uint32_t acc = 0;
+    for (int h = 0; h < kMaxLevels; h++) {
+        offsets[h] = acc;
+        if (h > 0 && counts[h] > 0) d.levels.push_back(make_int2((int)acc, (int)counts[h]));
+        acc += counts[h];
+    }
+    if (acc != (uint32_t)m) return hipErrorUnknown;  // every internal node has one height
+    if ((e = hipMemcpyAsync(d.level_offset, offsets, sizeof offsets, hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(level_scatter_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, d.node_height, m,
+                       d.level_offset, d.level_count + kMaxLevels, d.level_nodes);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the host arrays above must outlive the copy
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
     d.built = true;
     return lbvh_refit(d, stream);
 }
 
+// One launch per tree level, lowest first; the launch boundaries order the
+// levels (no atomics, no fences).
 hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
-    const int n = d.n;
-    if (n <= 1) return fill_i32(d.max_depth, 1, 1, stream);
-    hipError_t e = fill_i32(d.flags, 0, (size_t)n - 1, stream);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(refit_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d.aabb, d.ids, d.leaf_parent, d.nodes,
-                       d.flags, n, d.max_depth);
+    for (const int2& lv : d.levels)
+        hipLaunchKernelGGL(refit_level_kernel, dim3((lv.y + 255) / 256), dim3(256), 0, stream, d.aabb, d.ids, d.nodes,
+                           d.level_nodes, lv.x, lv.y);
     return hipGetLastError();
 }
 
@@ -456,8 +548,14 @@ hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error,
     hipError_t e = fill_i32(d.count, 0, 1, stream);
     if (e != hipSuccess) return e;
     if (d.n < 2) return hipSuccess;
-    hipLaunchKernelGGL(query_kernel, dim3((d.n + kQueryBlock - 1) / kQueryBlock), dim3(kQueryBlock), 0, stream, d.aabb,
-                       d.ids, d.nodes, d.n, pairs, cap, min_gap, d.count, error);
+    // the LDS stack is sized from the tree's DFS bound (known since the build)
+    const dim3 grid((d.n + kQueryBlock - 1) / kQueryBlock);
+    if (d.max_depth_host + 2 <= 32)
+        hipLaunchKernelGGL(query_kernel<32>, grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes, d.n, pairs, cap,
+                           min_gap, d.count, error);
+    else
+        hipLaunchKernelGGL(query_kernel<kQueryStack>, grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes, d.n,
+                           pairs, cap, min_gap, d.count, error);
     return hipGetLastError();
 }
 

@@ -129,12 +129,23 @@ def _frames(n):
 
 
 def groom(num_strands, verts_per_strand, seed=1, sphere_radius=0.1, seg_len=1e-3, curl=(2e-3, 8e-3), pitch=32,
-          center=(0.0, 0.0, 0.0)):
-    """Strands on the upper hemisphere of a sphere; verts_per_strand is an int or an int array."""
+          center=(0.0, 0.0, 0.0), roots="random"):
+    """Strands on the upper hemisphere of a sphere; verts_per_strand is an int or an int array.
+
+    roots="random": heights and azimuths uniform (area-uniform, Poisson-like:
+    some roots nearly coincide); roots="spiral": a golden-angle spiral with
+    uniform height steps, so neighbouring roots are evenly spaced, as on a
+    scalp."""
     rng = np.random.default_rng(seed)
     counts = np.broadcast_to(np.asarray(verts_per_strand, np.int64), (num_strands,)).copy()
     u = rng.random(num_strands)
     phi = rng.random(num_strands) * 2 * np.pi
+    if roots == "spiral":
+        k = np.arange(num_strands)
+        u = (k + 0.5) / num_strands
+        phi = np.mod(k * np.pi * (3.0 - np.sqrt(5.0)) + phi[0], 2 * np.pi)
+    elif roots != "random":
+        raise ValueError(f"unknown root layout {roots!r}")
     rho = curl[0] + (curl[1] - curl[0]) * rng.random(num_strands)
     th0 = rng.random(num_strands) * 2 * np.pi
     R = sphere_radius
@@ -241,11 +252,12 @@ def config_groom(num_strands=10_000, verts=100, seed=1, sdf_dim=128, with_sdf=Tr
     return sc
 
 
-def config_large(num_strands=100_000, seed=2, sdf_dim=256, with_sdf=True):
+def config_large(num_strands=100_000, seed=2, sdf_dim=256, with_sdf=True, curl=(2e-3, 8e-3), roots="random"):
     """100k strands with lognormal vertex counts (median 96, clipped 32-256) and
-    a sphere + capsule (neck) SDF on a sdf_dim^3 grid."""
+    a sphere + capsule (neck) SDF on a sdf_dim^3 grid; `curl` is the range of
+    helix radii [m], `roots` the root layout of groom()."""
     counts = lognormal_counts(num_strands, seed=seed)
-    sc = groom(num_strands, counts, seed=seed)
+    sc = groom(num_strands, counts, seed=seed, curl=curl, roots=roots)
     sc.name = f"large{num_strands}"
     if with_sdf:
         R = 0.1

@@ -426,48 +426,49 @@ struct Sdf {
 // Adds vertex i's terms to (f, H) and vertex i+1's to (f2, H2).
 template <class T>
 inline void contact_pair(const Params& p, V3<T> xi, V3<T> xj, V3<T> dxi, V3<T> dxj, V3<T> b0, V3<T> b1, V3<T> db0,
-                         V3<T> db1, T damp, V3<T>& f, Sym3<T>& H, V3<T>& f2, Sym3<T>& H2) {
+                     V3<T> db1, T damp, V3<T>& f, Sym3<T>& H, V3<T>& f2, Sym3<T>& H2) {
     const T invb = T(1) / T(p.barrier_thickness);
     const T radius = T(2) * T(p.radius);
     const T kCol = T(p.k_collision) * invb;
     T ux, uy;
 
-        segment_closest_points(V3<T>{0, 0, 0}, (xj - xi) + (dxj - dxi), (b0 - xi) + (db0 - dxi), (b1 - xi) + (db1 - dxi),
-                               ux, uy);
-        const V3<T> dpos = mix(xi, xj, ux) - mix(b0, b1, uy);
-        const V3<T> ddpos = mix(dxi, dxj, ux) - mix(db0, db1, uy);
-        V3<T> nrm = dpos + ddpos;
-        T d = length(nrm);
-        nrm = nrm * (T(1) / d);
-        const T wj = ux;
-        const T wi = T(1) - ux;
-        d = d - radius;
-        d = d * invb;
-        if (d > 1) return;
-        d = std::max(d, T(1e-3f));
-        const T invd = T(1) / d;
-        const T logd = det_logf(d);
-        T dH = (T(-3) + (T(2) + invd) * invd - T(2) * logd) * kCol * invb;
-        const T ff = -(T(1) - d) * (d - T(1) + T(2) * d * logd) * invd * kCol;
-        f = f + (ff * wi - damp * dH * wi * wi * dot(nrm, dxi)) * nrm;
-        f2 = f2 + (ff * wj - damp * dH * wj * wj * dot(nrm, dxj)) * nrm;
-        dH = dH * (T(1) + damp);
-        Sym3<T> op = sym_outer(nrm);
-        sym_add(H, sym_scale(op, dH * wi * wi));
-        sym_add(H2, sym_scale(op, dH * wj * wj));
-        const V3<T> u = ddpos - dot(nrm, ddpos) * nrm;
-        const T ul = length(u);
-        if (ul > 0) {
-            const T cand = T(p.friction_coeff) * ff / ul;
-            const T f1 = cand < T(p.k_friction) ? cand : T(p.k_friction);  // glm::min(fricK, cand)
-            op.d[0] -= 1;
-            op.d[1] -= 1;
-            op.d[2] -= 1;
-            f = f - (f1 * wi) * u;
-            sym_sub(H, sym_scale(op, (wi * wi) * f1));
-            f2 = f2 - (f1 * wj) * u;
-            sym_sub(H2, sym_scale(op, (wj * wj) * f1));
-        }
+    segment_closest_points(V3<T>{0, 0, 0}, (xj - xi) + (dxj - dxi), (b0 - xi) + (db0 - dxi), (b1 - xi) + (db1 - dxi),
+                           ux, uy);
+    const V3<T> dpos = mix(xi, xj, ux) - mix(b0, b1, uy);
+    const V3<T> ddpos = mix(dxi, dxj, ux) - mix(db0, db1, uy);
+    V3<T> nrm = dpos + ddpos;
+    T d = length(nrm);
+    if (!(d > 0)) return;  // closest points coincide: no contact normal (the reference divides by 0)
+    nrm = nrm * (T(1) / d);
+    const T wj = ux;
+    const T wi = T(1) - ux;
+    d = d - radius;
+    d = d * invb;
+    if (d > 1) return;
+    d = std::max(d, T(1e-3f));
+    const T invd = T(1) / d;
+    const T logd = det_logf(d);
+    T dH = (T(-3) + (T(2) + invd) * invd - T(2) * logd) * kCol * invb;
+    const T ff = -(T(1) - d) * (d - T(1) + T(2) * d * logd) * invd * kCol;
+    f = f + (ff * wi - damp * dH * wi * wi * dot(nrm, dxi)) * nrm;
+    f2 = f2 + (ff * wj - damp * dH * wj * wj * dot(nrm, dxj)) * nrm;
+    dH = dH * (T(1) + damp);
+    Sym3<T> op = sym_outer(nrm);
+    sym_add(H, sym_scale(op, dH * wi * wi));
+    sym_add(H2, sym_scale(op, dH * wj * wj));
+    const V3<T> u = ddpos - dot(nrm, ddpos) * nrm;
+    const T ul = length(u);
+    if (ul > 0) {
+        const T cand = T(p.friction_coeff) * ff / ul;
+        const T f1 = cand < T(p.k_friction) ? cand : T(p.k_friction);  // glm::min(fricK, cand)
+        op.d[0] -= 1;
+        op.d[1] -= 1;
+        op.d[2] -= 1;
+        f = f - (f1 * wi) * u;
+        sym_sub(H, sym_scale(op, (wi * wi) * f1));
+        f2 = f2 - (f1 * wj) * u;
+        sym_sub(H2, sym_scale(op, (wj * wj) * f1));
+    }
     }
 
 
@@ -787,6 +788,10 @@ struct Sim {
             sym_add(H, H2);
         }
         V3<T> out = dxi + T(p.acceleration_ratio) * sym_solve(H, f);
+        // A contact Hessian can make H numerically singular in FP32 (its
+        // adjugate determinant cancels to 0): the reference's step is then
+        // inf/NaN, which no step limit removes.  Such a step is skipped.
+        if (!std::isfinite(out.x) || !std::isfinite(out.y) || !std::isfinite(out.z)) return dxi;
         const T l = length(out);
         if (l > limit && l > 0) out = out * (limit / l);
         return out;

@@ -205,3 +205,22 @@ def test_step_limit_closed_form():
         assert ms[i] == pytest.approx(want, rel=1e-6), i
     # the crossing pair is in contact and limited by the gap
     assert num[7] >= 1 and ms[7] == pytest.approx(0.45 * 1.2e-5, rel=1e-5)
+
+
+def test_dense_groom_with_contacts_stays_finite():
+    """A lognormal groom with contacts and a coarse SDF: interpenetrating
+    strands give contact Hessians that are numerically singular in FP32 and
+    closest points that coincide; the reference's step turns such vertices
+    into NaN (and NaN boxes then overlap everything in the broad phase).
+    The restatement skips those steps, so the state stays finite and the
+    pair count bounded."""
+    sc = scenes.config_large(500, seed=2, sdf_dim=64)
+    p = scenes.contact_params(default_params, sc.material)
+    o = OracleSim(p, sc.positions, sc.strand_offsets, precision=32)
+    o.set_threads(8)
+    scenes.setup_sim(o, sc)
+    for _ in range(3):
+        o.solve_next_step(F32(1 / 600))
+        x, v = o.get_pos_vel()
+        assert np.isfinite(x).all() and np.isfinite(v).all()
+        assert o.num_pairs() < 4 * sc.num_verts and o.error() == 0
