@@ -41,12 +41,18 @@ KS_FN f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 KS_FN f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
 KS_FN f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
 KS_FN f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
-KS_FN float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-KS_FN f3 cross(f3 a, f3 b) { return f3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+// Step-time algebra uses fused multiply-adds (one rounding for a*b + c), the
+// way the reference's CUDA compiler contracts its FP32 arithmetic.  The
+// grouping is fixed here and restated in the CPU oracle, so both sides round
+// identically.
+KS_FN float dot(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+KS_FN f3 cross(f3 a, f3 b) {
+    return f3{fmaf(a.y, b.z, -(b.y * a.z)), fmaf(a.z, b.x, -(b.z * a.x)), fmaf(a.x, b.y, -(b.x * a.y))};
+}
 KS_FN float length(f3 a) { return sqrtf(dot(a, a)); }
 KS_FN f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
 
-KS_FN float dot(f4 a, f4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+KS_FN float dot(f4 a, f4 b) { return fmaf(a.w, b.w, fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x))); }
 KS_FN float length(f4 a) { return sqrtf(dot(a, a)); }
 KS_FN f4 normalize(f4 a) {
     float s = 1.0f / sqrtf(dot(a, a));
@@ -62,17 +68,20 @@ KS_FN f4 conj(f4 a) { return f4{-a.x, -a.y, -a.z, a.w}; }
 
 // Hamilton product a*b
 KS_FN f4 rmul(f4 a, f4 b) {
-    f3 aq = vec(a), bq = vec(b);
-    f3 v = a.w * bq + b.w * aq + cross(aq, bq);
-    return mk4(v, a.w * b.w - dot(aq, bq));
+    const f3 aq = vec(a), bq = vec(b);
+    const f3 cr = cross(aq, bq);
+    return f4{fmaf(a.w, bq.x, fmaf(b.w, aq.x, cr.x)), fmaf(a.w, bq.y, fmaf(b.w, aq.y, cr.y)),
+              fmaf(a.w, bq.z, fmaf(b.w, aq.z, cr.z)), fmaf(a.w, b.w, -dot(aq, bq))};
 }
 
 // r * v * r^-1
 KS_FN f3 rotate(f4 r, f3 v) {
-    f3 q = vec(r);
-    f3 a = r.w * v + cross(q, v);
-    float c = dot(v, q);
-    return r.w * a + cross(q, a) + c * q;
+    const f3 q = vec(r);
+    const f3 c1 = cross(q, v);
+    const f3 a{fmaf(r.w, v.x, c1.x), fmaf(r.w, v.y, c1.y), fmaf(r.w, v.z, c1.z)};
+    const float c = dot(v, q);
+    const f3 c2 = cross(q, a);
+    return f3{fmaf(r.w, a.x, fmaf(c, q.x, c2.x)), fmaf(r.w, a.y, fmaf(c, q.y, c2.y)), fmaf(r.w, a.z, fmaf(c, q.z, c2.z))};
 }
 
 // rotate(r, (1,0,0)) with the same operation order as the general case
@@ -105,19 +114,22 @@ KS_FN f4 from_to(f3 a, f3 b) {
 // (m01, m02, m12).  Solves H x = f through the adjugate/determinant formula.
 KS_FN f3 sym_solve(float m00, float m11, float m22, float m01, float m02, float m12, f3 f) {
     const float m10 = m01, m20 = m02, m21 = m12;
-    const float ood =
-        1.0f / (+m00 * (m11 * m22 - m21 * m12) - m10 * (m01 * m22 - m21 * m02) + m20 * (m01 * m12 - m11 * m02));
-    const float i00 = +(m11 * m22 - m21 * m12) * ood;
-    const float i10 = -(m10 * m22 - m20 * m12) * ood;
-    const float i20 = +(m10 * m21 - m20 * m11) * ood;
-    const float i01 = -(m01 * m22 - m21 * m02) * ood;
-    const float i11 = +(m00 * m22 - m20 * m02) * ood;
-    const float i21 = -(m00 * m21 - m20 * m01) * ood;
-    const float i02 = +(m01 * m12 - m11 * m02) * ood;
-    const float i12 = -(m00 * m12 - m10 * m02) * ood;
-    const float i22 = +(m00 * m11 - m10 * m01) * ood;
-    return f3{i00 * f.x + i10 * f.y + i20 * f.z, i01 * f.x + i11 * f.y + i21 * f.z,
-              i02 * f.x + i12 * f.y + i22 * f.z};
+    // 2x2 minors (glm inverse's cofactors), each a*b - c*d with one fma
+    const float c00 = fmaf(m11, m22, -(m21 * m12));
+    const float c01 = fmaf(m01, m22, -(m21 * m02));
+    const float c02 = fmaf(m01, m12, -(m11 * m02));
+    const float c10 = fmaf(m10, m22, -(m20 * m12));
+    const float c20 = fmaf(m10, m21, -(m20 * m11));
+    const float c11 = fmaf(m00, m22, -(m20 * m02));
+    const float c21 = fmaf(m00, m21, -(m20 * m01));
+    const float c12 = fmaf(m00, m12, -(m10 * m02));
+    const float c22 = fmaf(m00, m11, -(m10 * m01));
+    const float ood = 1.0f / fmaf(m20, c02, fmaf(-m10, c01, m00 * c00));
+    const float i00 = c00 * ood, i10 = -c10 * ood, i20 = c20 * ood;
+    const float i01 = -c01 * ood, i11 = c11 * ood, i21 = -c21 * ood;
+    const float i02 = c02 * ood, i12 = -c12 * ood, i22 = c22 * ood;
+    return f3{fmaf(i20, f.z, fmaf(i10, f.y, i00 * f.x)), fmaf(i21, f.z, fmaf(i11, f.y, i01 * f.x)),
+              fmaf(i22, f.z, fmaf(i12, f.y, i02 * f.x))};
 }
 
 // Natural logarithm by a fixed sequence of IEEE float operations: d = m 2^e

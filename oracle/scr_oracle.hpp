@@ -78,14 +78,18 @@ template <class T>
 inline V3<T> operator/(V3<T> a, T s) {
     return {a.x / s, a.y / s, a.z / s};
 }
-// glm dot: (x*x' + y*y') + z*z'   (func_geometric.inl)
+// The step-time algebra (dot, cross, rotor product and rotation, the 3x3
+// solve) is evaluated with fused multiply-adds, one rounding per a*b + c, as
+// the reference's CUDA compiler contracts FP32 arithmetic; the grouping is the
+// device code's (kamisimulator_amd/csrc/kscr_math.h), so FP32 results agree
+// bit for bit.  glm dot (func_geometric.inl) as fma(z, z', fma(y, y', x x')).
 template <class T>
 inline T dot(V3<T> a, V3<T> b) {
-    return a.x * b.x + a.y * b.y + a.z * b.z;
+    return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x));
 }
 template <class T>
 inline V3<T> cross(V3<T> a, V3<T> b) {
-    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+    return {std::fma(a.y, b.z, -(b.y * a.z)), std::fma(a.z, b.x, -(b.z * a.x)), std::fma(a.x, b.y, -(b.x * a.y))};
 }
 template <class T>
 inline T length2(V3<T> a) {
@@ -118,7 +122,7 @@ inline R4<T> r4(V3<T> v, T w) {
 }
 template <class T>
 inline T dot4(R4<T> a, R4<T> b) {
-    return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+    return std::fma(a.w, b.w, std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x)));
 }
 template <class T>
 inline T length4(R4<T> a) {
@@ -145,17 +149,21 @@ inline R4<T> conj(R4<T> a) {
 // Hamilton product (Rotor.h:210-212)
 template <class T>
 inline R4<T> rmul(R4<T> a, R4<T> b) {
-    V3<T> aq = a.q(), bq = b.q();
-    V3<T> v = a.w * bq + b.w * aq + cross(aq, bq);
-    return r4(v, a.w * b.w - dot(aq, bq));
+    const V3<T> aq = a.q(), bq = b.q();
+    const V3<T> cr = cross(aq, bq);
+    return R4<T>{std::fma(a.w, bq.x, std::fma(b.w, aq.x, cr.x)), std::fma(a.w, bq.y, std::fma(b.w, aq.y, cr.y)),
+                 std::fma(a.w, bq.z, std::fma(b.w, aq.z, cr.z)), std::fma(a.w, b.w, -dot(aq, bq))};
 }
 // Rotor::rotate (Rotor.h:114-123)
 template <class T>
 inline V3<T> rotate(R4<T> r, V3<T> v) {
-    V3<T> q = r.q();
-    V3<T> a = r.w * v + cross(q, v);
-    T c = dot(v, q);
-    return r.w * a + cross(q, a) + c * q;
+    const V3<T> q = r.q();
+    const V3<T> c1 = cross(q, v);
+    const V3<T> a{std::fma(r.w, v.x, c1.x), std::fma(r.w, v.y, c1.y), std::fma(r.w, v.z, c1.z)};
+    const T c = dot(v, q);
+    const V3<T> c2 = cross(q, a);
+    return {std::fma(r.w, a.x, std::fma(c, q.x, c2.x)), std::fma(r.w, a.y, std::fma(c, q.y, c2.y)),
+            std::fma(r.w, a.z, std::fma(c, q.z, c2.z))};
 }
 // orthoBasisX(n)[1] (KE/includes/modules/Common.h:220-236), used by fromTo for
 // antiparallel vectors.
@@ -196,17 +204,21 @@ inline V3<T> sym_solve(const Sym3<T>& H, V3<T> f) {
     const T m00 = H.d[0], m11 = H.d[1], m22 = H.d[2];
     const T m01 = H.d[3], m02 = H.d[4], m12 = H.d[5];
     const T m10 = m01, m20 = m02, m21 = m12;
-    T ood = T(1) / (+m00 * (m11 * m22 - m21 * m12) - m10 * (m01 * m22 - m21 * m02) + m20 * (m01 * m12 - m11 * m02));
-    T i00 = +(m11 * m22 - m21 * m12) * ood;
-    T i10 = -(m10 * m22 - m20 * m12) * ood;
-    T i20 = +(m10 * m21 - m20 * m11) * ood;
-    T i01 = -(m01 * m22 - m21 * m02) * ood;
-    T i11 = +(m00 * m22 - m20 * m02) * ood;
-    T i21 = -(m00 * m21 - m20 * m01) * ood;
-    T i02 = +(m01 * m12 - m11 * m02) * ood;
-    T i12 = -(m00 * m12 - m10 * m02) * ood;
-    T i22 = +(m00 * m11 - m10 * m01) * ood;
-    return {i00 * f.x + i10 * f.y + i20 * f.z, i01 * f.x + i11 * f.y + i21 * f.z, i02 * f.x + i12 * f.y + i22 * f.z};
+    const T c00 = std::fma(m11, m22, -(m21 * m12));
+    const T c01 = std::fma(m01, m22, -(m21 * m02));
+    const T c02 = std::fma(m01, m12, -(m11 * m02));
+    const T c10 = std::fma(m10, m22, -(m20 * m12));
+    const T c20 = std::fma(m10, m21, -(m20 * m11));
+    const T c11 = std::fma(m00, m22, -(m20 * m02));
+    const T c21 = std::fma(m00, m21, -(m20 * m01));
+    const T c12 = std::fma(m00, m12, -(m10 * m02));
+    const T c22 = std::fma(m00, m11, -(m10 * m01));
+    const T ood = T(1) / std::fma(m20, c02, std::fma(-m10, c01, m00 * c00));
+    const T i00 = c00 * ood, i10 = -c10 * ood, i20 = c20 * ood;
+    const T i01 = -c01 * ood, i11 = c11 * ood, i21 = -c21 * ood;
+    const T i02 = c02 * ood, i12 = -c12 * ood, i22 = c22 * ood;
+    return {std::fma(i20, f.z, std::fma(i10, f.y, i00 * f.x)), std::fma(i21, f.z, std::fma(i11, f.y, i01 * f.x)),
+            std::fma(i22, f.z, std::fma(i12, f.y, i02 * f.x))};
 }
 
 // Kit::segmentClosestPoints (KE/includes/modules/Common.h:294-312)
