@@ -84,8 +84,25 @@ KS_FN f3 rotate(f4 r, f3 v) {
     return f3{fmaf(r.w, a.x, fmaf(c, q.x, c2.x)), fmaf(r.w, a.y, fmaf(c, q.y, c2.y)), fmaf(r.w, a.z, fmaf(c, q.z, c2.z))};
 }
 
-// rotate(r, (1,0,0)) with the same operation order as the general case
-KS_FN f3 rotate_e1(f4 r) { return rotate(r, f3{1.0f, 0.0f, 0.0f}); }
+// rotate(r, (1,0,0)): the general formula with the products by 0 and 1 of
+// e1 done exactly (r.w e1 + q x e1 = (w, q.z, -q.y), e1.q = q.x); the values
+// are the general formula's, up to the sign of exact zeros.
+KS_FN f3 rotate_e1(f4 r) {
+    const f3 q = vec(r);
+    const f3 a{r.w, q.z, -q.y};
+    const float c = q.x;
+    const f3 c2 = cross(q, a);
+    return f3{fmaf(r.w, a.x, fmaf(c, q.x, c2.x)), fmaf(r.w, a.y, fmaf(c, q.y, c2.y)), fmaf(r.w, a.z, fmaf(c, q.z, c2.z))};
+}
+
+// m (x) (e1, 0) = (m.w, m.z, -m.y, -m.x), and (v, 0) (x) b: rmul with the
+// zero and unit factors done exactly (same values up to the sign of zeros).
+KS_FN f4 rmul_e1(f4 m) { return f4{m.w, m.z, -m.y, -m.x}; }
+KS_FN f4 rmul_pure(f3 v, f4 b) {
+    const f3 bq = vec(b);
+    const f3 cr = cross(v, bq);
+    return f4{fmaf(b.w, v.x, cr.x), fmaf(b.w, v.y, cr.y), fmaf(b.w, v.z, cr.z), -dot(v, bq)};
+}
 
 // Second column of the orthonormal basis built around n (used for the
 // antiparallel case of from_to).

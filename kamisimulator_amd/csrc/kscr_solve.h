@@ -119,7 +119,7 @@ __device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, f
         b = b + phi * rmul(qp, conj(qr));
     }
     const float lambda = length(vv) + length(b);
-    return normalize(rmul(rmul(mk4(vv, 0.f), b), f4{1.f, 0.f, 0.f, 0.f}) + lambda * b);
+    return normalize(rmul_e1(rmul_pure(vv, b)) + lambda * b);
 }
 
 // endItr (YB/sim/iteration.cu:62-74)

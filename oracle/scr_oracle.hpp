@@ -165,6 +165,30 @@ inline V3<T> rotate(R4<T> r, V3<T> v) {
     return {std::fma(r.w, a.x, std::fma(c, q.x, c2.x)), std::fma(r.w, a.y, std::fma(c, q.y, c2.y)),
             std::fma(r.w, a.z, std::fma(c, q.z, c2.z))};
 }
+// rotate(r, e1) with e1's zero and unit factors done exactly: r.w e1 + q x e1
+// = (w, q.z, -q.y) and e1.q = q.x, then the general formula (same values up to
+// the sign of exact zeros).
+template <class T>
+inline V3<T> rotate_e1(R4<T> r) {
+    const V3<T> q = r.q();
+    const V3<T> a{r.w, q.z, -q.y};
+    const T c = q.x;
+    const V3<T> c2 = cross(q, a);
+    return {std::fma(r.w, a.x, std::fma(c, q.x, c2.x)), std::fma(r.w, a.y, std::fma(c, q.y, c2.y)),
+            std::fma(r.w, a.z, std::fma(c, q.z, c2.z))};
+}
+// m (x) (e1, 0) = (m.w, m.z, -m.y, -m.x); (v, 0) (x) b without the products
+// by the zero scalar part.
+template <class T>
+inline R4<T> rmul_e1(R4<T> m) {
+    return R4<T>{m.w, m.z, -m.y, -m.x};
+}
+template <class T>
+inline R4<T> rmul_pure(V3<T> v, R4<T> b) {
+    const V3<T> bq = b.q();
+    const V3<T> cr = cross(v, bq);
+    return R4<T>{std::fma(b.w, v.x, cr.x), std::fma(b.w, v.y, cr.y), std::fma(b.w, v.z, cr.z), -dot(v, bq)};
+}
 // orthoBasisX(n)[1] (KE/includes/modules/Common.h:220-236), used by fromTo for
 // antiparallel vectors.
 template <class T>
@@ -756,7 +780,7 @@ struct Sim {
         if (!(flags[i] & F_NEXT)) return;
         s.limit = maxStep[i];
         const T invl = T(1) / lRest[i];
-        const V3<T> c = ((x[i + 1] - x[i]) + (dx[i + 1] - dx[i])) * invl - rotate(q[i], V3<T>{1, 0, 0});
+        const V3<T> c = ((x[i + 1] - x[i]) + (dx[i + 1] - dx[i])) * invl - rotate_e1(q[i]);
         const T k = kStretch[i] * invl;
         T d = k * invl;
         s.f0 = s.f0 + (k * c - (damp * d) * dx[i]);
@@ -926,7 +950,7 @@ struct Sim {
             b = add4(b, scale4(phi, rmul(qj, conj(qr))));
         }
         T lambda = length(vv) + length4(b);
-        R4<T> t = rmul(rmul(r4(vv, T(0)), b), R4<T>{1, 0, 0, 0});
+        R4<T> t = rmul_e1(rmul_pure(vv, b));
         return normalize4(add4(t, scale4(lambda, b)));
     }
 
