@@ -27,16 +27,18 @@ KS_FN f3 neg(f3 a) { return f3{-a.x, -a.y, -a.z}; }
 KS_FN f4 zero4() { return f4{0.f, 0.f, 0.f, 0.f}; }
 KS_FN f3 zero3() { return f3{0.f, 0.f, 0.f}; }
 
-// Transposed Jacobian of rotate(q, e1) with respect to q, applied to rb.
+// Transposed Jacobian of rotate(q, e1) = (w^2 + x^2 - y^2 - z^2, 2(xy + wz),
+// 2(xz - wy)) with respect to q = (x, y, z, w), applied to rb.
 __device__ inline f4 rot_e1_vjp(f4 q, f3 rb) {
-    const f3 e{1.0f, 0.0f, 0.0f};
-    const f3 qv = vec(q);
-    const float w = q.w;
-    const float wb = dot(rb, (2.0f * w) * e + 2.0f * cross(qv, e));
-    const f3 qvb = ((-2.0f) * dot(rb, e)) * qv + (2.0f * w) * cross(e, rb) + (2.0f * dot(rb, qv)) * e +
-                   (2.0f * dot(qv, e)) * rb;
-    return f4{qvb.x, qvb.y, qvb.z, wb};
+    const float wb = 2.0f * fmaf(q.w, rb.x, fmaf(q.z, rb.y, -(q.y * rb.z)));
+    const float xb = 2.0f * dot(rb, vec(q));
+    const float yb = 2.0f * fmaf(q.x, rb.y, -fmaf(q.y, rb.x, q.w * rb.z));
+    const float zb = 2.0f * fmaf(q.x, rb.z, fmaf(q.w, rb.y, -(q.z * rb.x)));
+    return f4{xb, yb, zb, wb};
 }
+
+// a (x) conj(e1) = -(a (x) e1)
+KS_FN f4 rmul_conj_e1(f4 a) { return f4{-a.w, -a.z, a.y, a.x}; }
 
 __device__ inline f4 ld4(const float4* p, size_t i) { return to4(p[i]); }
 __device__ inline void st4(float4* p, size_t i, f4 a) { p[i] = make_float4(a.x, a.y, a.z, a.w); }
@@ -49,7 +51,6 @@ struct AdjLds {
     float4 qrp[BLOCK];    // contribution to the previous vertex's qRest adjoint
     int mask[BLOCK];      // 1: orientation solved, 2: previous term, 4: next term
     float4 fbD[BLOCK];    // adjoint of the position solve's force, and of D
-    float s[3][BLOCK];    // stretch-constraint adjoint of the segment
 };
 
 template <int BLOCK>
@@ -83,8 +84,17 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
     const float damp = a.damping / hh;
     const float invH = 1.0f / hh;
     const float decay = 1.0f - a.velocity_decay * hh;
-    const f3 e1v{1.0f, 0.0f, 0.0f};
-    const f4 e1{1.0f, 0.0f, 0.0f, 0.0f};
+    // per-vertex constants of the step, with the forward's operations
+    const float il = 1.0f / lr, kk = ks * il, dd = kk * il;          // own segment
+    const float ilp = 1.0f / lrp, kkp = ksp * ilp, ddp = kkp * ilp;  // previous segment
+    const float sc = (-2.0f * ks) / lr;                              // d(v)/d(e) of the orientation solve
+    const float m2lr = -2.0f / lr;
+    const float a_in = 1.0f / (im * hh * hh);
+    const float c1 = 1.0f / (hh * hh * im);
+    float hd = a_in;
+    if (has_next) hd = hd + dd * (1.0f + damp);
+    if (has_prev) hd = hd + ddp * (1.0f + damp);
+    const InertiaInv hinv = inertia_inverse(hd, a_in);
 
     for (int sub = a.num_substeps - 1; sub >= 0; sub--) {
         const size_t so = (size_t)sub * n;
@@ -134,25 +144,23 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             if (act && !pass) {
                 mask = 1;
                 const f3 e = (xn - x) + (dxn_n - dxn);
-                const float sc = (-2.0f * ks) / lr;
                 const f3 vv = e * sc;
                 f4 b = zero4(), bm = zero4(), bp = zero4();
                 float phim = 0.0f, phip = 0.0f;
                 if (has_prev) {
-                    phim = dot(rmul(conj(Qm), Q), qrm) > 0 ? 1.0f : -1.0f;
                     bm = rmul(Qm, qrm);
+                    phim = dot(Q, bm) > 0 ? 1.0f : -1.0f;
                     b = b + phim * bm;
                 }
                 if (has_next_ori) {
-                    phip = dot(rmul(conj(Q), Qp), qr) > 0 ? 1.0f : -1.0f;
                     bp = rmul(Qp, conj(qr));
+                    phip = dot(Q, bp) > 0 ? 1.0f : -1.0f;
                     b = b + phip * bp;
                 }
                 const float lv = length(vv), lb = length(b);
                 const float lambda = lv + lb;
-                const f4 V = mk4(vv, 0.0f);
-                const f4 m = rmul(V, b);
-                const f4 tq = rmul(m, e1);
+                const f4 m = rmul_pure(vv, b);
+                const f4 tq = rmul_e1(m);
                 const f4 w = tq + lambda * b;
                 const float lw = length(w);
                 const f4 qn = (1.0f / lw) * w;
@@ -160,27 +168,26 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
                 const f4 wb = (1.0f / lw) * (g + (-dot(qn, g)) * qn);
                 f4 bb = lambda * wb;
                 const float lamb = dot(wb, b);
-                const f4 mb = zero4() + rmul(wb, conj(e1));
-                const f4 Vb = zero4() + rmul(mb, conj(b));
-                bb = bb + rmul(conj(V), mb);
-                f3 vvb = vec(Vb);
+                const f4 mb = rmul_conj_e1(wb);
+                f3 vvb = vec(rmul(mb, conj(b)));
+                bb = bb + rmul_pure(neg(vv), mb);
                 if (lv > 0) vvb = vvb + (lamb / lv) * vv;
                 if (lb > 0) bb = bb + (lamb / lb) * b;
                 if (has_prev) {
                     const f4 c = phim * bb;
-                    qmb = zero4() + rmul(c, conj(qrm));
-                    qrp = zero4() + rmul(conj(Qm), c);
+                    qmb = rmul(c, conj(qrm));
+                    qrp = rmul(conj(Qm), c);
                     mask |= 2;
                 }
                 if (has_next_ori) {
                     const f4 c = phip * bb;
-                    qpb = zero4() + rmul(c, conj(conj(qr)));
-                    const f4 cqb = zero4() + rmul(conj(Qp), c);
+                    qpb = rmul(c, qr);
+                    const f4 cqb = rmul(conj(Qp), c);
                     qrb = qrb + conj(cqb);
                     mask |= 4;
                 }
                 eb = vvb * sc;
-                ksb += dot(vvb, e) * (-2.0f / lr);
+                ksb += dot(vvb, e) * m2lr;
             }
             L.eb[0][t] = eb.x;
             L.eb[1][t] = eb.y;
@@ -219,37 +226,16 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             f3 fb = zero3();
             float Db = 0.0f;
             f3 c_next = zero3();
-            float dd_prev = 0.0f;
-            if (has_next) {
-                const float il = 1.0f / lr;
-                c_next = ((xn - x) + (dx_n - dx)) * il - rotate(Q, e1v);
-            }
-            if (has_prev) {
-                const float il = 1.0f / lrp;
-                const float kk = ksp * il;
-                dd_prev = kk * il;
-            }
+            if (has_next) c_next = ((xn - x) + (dx_n - dx)) * il - rotate_e1(Q);
             if (act && free) {
-                const float a_in = 1.0f / (im * hh * hh);
-                float D = 0.0f;
-                const float c1 = 1.0f / (hh * hh * im);
                 f3 f = c1 * (y - dx);
-                if (has_next) {
-                    const float il = 1.0f / lr;
-                    const float kk = ks * il, dd = kk * il;
-                    f = f + (kk * c_next - (damp * dd) * dx);
-                    D += dd * (1.0f + damp);
-                }
+                if (has_next) f = f + (kk * c_next - (damp * dd) * dx);
                 if (has_prev) {
-                    const float il = 1.0f / lrp;
-                    const f3 c = ((x - xp) + (dx - dx_p)) * il - rotate(Qm, e1v);
-                    const float kk = ksp * il, dd = kk * il;
-                    f = f + ((-kk) * c - (damp * dd) * dx);
-                    D += dd * (1.0f + damp);
+                    const f3 c = ((x - xp) + (dx - dx_p)) * ilp - rotate_e1(Qm);
+                    f = f + ((-kkp) * c - (damp * ddp) * dx);
                 }
-                const float hd = a_in + D;
-                const f3 delta = sym_solve(hd, hd, hd, a_in, a_in, a_in, f);
-                fb = sym_solve(hd, hd, hd, a_in, a_in, a_in, a.acceleration_ratio * dxb);
+                const f3 delta = inertia_solve(hinv, f);
+                fb = inertia_solve(hinv, a.acceleration_ratio * dxb);
                 Db = -dot(fb, delta);
                 yb = yb + c1 * fb;
                 dxb2 = dxb2 - c1 * fb;
@@ -257,38 +243,36 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             L.fbD[t] = make_float4(fb.x, fb.y, fb.z, Db);
             __syncthreads();
 
-            // ---- phase B2: stretch constraint of segment i
-            f3 s_own = zero3();
+            // ---- phase B2: stretch constraint of segment i, and the same
+            // value for segment i - 1 recomputed from the published adjoints
+            // (identical operations, so no third exchange is needed)
+            f3 s_own = zero3(), s_prev = zero3();
             f4 qrot = zero4();
-            float dd_own = 0.0f;
             if (has_next) {
                 const float4 nb = L.fbD[t + 1];
                 const f3 fbn{nb.x, nb.y, nb.z};
                 const float Dbn = nb.w;
-                const float il = 1.0f / lr;
-                const float kk = ks * il, dd = kk * il;
-                dd_own = dd;
                 const f3 cb = kk * (fb - fbn);
                 s_own = cb * il;
                 qrot = rot_e1_vjp(Q, neg(cb));
                 ksb += dot(c_next, fb - fbn) * il - damp * (dot(dx, fb) + dot(dx_n, fbn)) * il * il +
                        (Db + Dbn) * (1.0f + damp) * il * il;
             }
-            L.s[0][t] = s_own.x;
-            L.s[1][t] = s_own.y;
-            L.s[2][t] = s_own.z;
-            __syncthreads();
             if (has_prev) {
-                const f3 sp{L.s[0][t - 1], L.s[1][t - 1], L.s[2][t - 1]};
-                xb = xb + sp;
-                dxb2 = dxb2 + sp;
-                dxb2 = dxb2 - (damp * dd_prev) * fb;
+                const float4 pb = L.fbD[t - 1];
+                const f3 fbp{pb.x, pb.y, pb.z};
+                s_prev = (kkp * (fbp - fb)) * ilp;
+            }
+            if (has_prev) {
+                xb = xb + s_prev;
+                dxb2 = dxb2 + s_prev;
+                dxb2 = dxb2 - (damp * ddp) * fb;
             }
             if (has_next) {
                 xb = xb - s_own;
                 dxb2 = dxb2 - s_own;
                 qb2 = qb2 + qrot;
-                dxb2 = dxb2 - (damp * dd_own) * fb;
+                dxb2 = dxb2 - (damp * dd) * fb;
             }
             dxb = dxb2;
             qb = qb2;

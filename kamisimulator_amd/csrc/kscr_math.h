@@ -149,6 +149,27 @@ KS_FN f3 sym_solve(float m00, float m11, float m22, float m01, float m02, float 
               fmaf(i22, f.z, fmaf(i12, f.y, i02 * f.x))};
 }
 
+// The contact-free position-solve matrix has every entry a (the reference's
+// hess3(1/(m h^2)) fills all six entries) and diagonal hd (a plus the stretch
+// terms).  sym_solve's cofactors of it take three distinct values (c00 = c11
+// = c22, c01 = c10 = c12 = c21, c02 = c20, fma being symmetric in its
+// factors), so its inverse is fixed by three numbers; computed once, they give
+// exactly sym_solve(hd, hd, hd, a, a, a, f) for every f.
+struct InertiaInv {
+    float p, q, r;  // diagonal, entries (0,1) (1,2), entries (0,2)
+};
+KS_FN InertiaInv inertia_inverse(float hd, float a) {
+    const float c00 = fmaf(hd, hd, -(a * a));
+    const float c01 = fmaf(a, hd, -(a * a));
+    const float c02 = fmaf(a, a, -(hd * a));
+    const float ood = 1.0f / fmaf(a, c02, fmaf(-a, c01, hd * c00));
+    return InertiaInv{c00 * ood, -c01 * ood, c02 * ood};
+}
+KS_FN f3 inertia_solve(InertiaInv m, f3 f) {
+    return f3{fmaf(m.r, f.z, fmaf(m.q, f.y, m.p * f.x)), fmaf(m.q, f.z, fmaf(m.p, f.y, m.q * f.x)),
+              fmaf(m.p, f.z, fmaf(m.q, f.y, m.r * f.x))};
+}
+
 // Natural logarithm by a fixed sequence of IEEE float operations: d = m 2^e
 // with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), by its
 // odd series to s^13, plus e ln2 with ln2 split into a high and a low part.

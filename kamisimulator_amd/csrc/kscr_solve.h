@@ -105,18 +105,24 @@ __device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, co
 // x, xj positions of the segment ends, dxn, dxj their new dx, qm / qp the
 // previous-iteration orientations of the neighbouring segments, qrp / qr the
 // rest rotors of segments i-1 and i.
+// The sign switches phi (cosserat.cu:320, :328) test <conj(qm) q, qrp> and
+// <conj(q) qp, qr>.  With <a, b> = Re(a conj(b)) and Re(ABC) = Re(BCA) these
+// equal <q, qm qrp> and <q, qp conj(qr)>, the dot products of q with the two
+// rotors b is built from, which are formed here anyway.
 __device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ks, f4 q, f4 qm, f4 qp, f4 qrp,
                                         f4 qr, uint32_t fl) {
     f3 vv = ((xj - x) + (dxj - dxn)) / lr;
     vv = vv * (-2.0f * ks);
     f4 b{0, 0, 0, 0};
     if (fl & FLAG_PREV) {
-        const float phi = dot(rmul(conj(qm), q), qrp) > 0 ? 1.f : -1.f;
-        b = b + phi * rmul(qm, qrp);
+        const f4 bm = rmul(qm, qrp);
+        const float phi = dot(q, bm) > 0 ? 1.f : -1.f;
+        b = b + phi * bm;
     }
     if (fl & FLAG_NEXT_ORI) {
-        const float phi = dot(rmul(conj(q), qp), qr) > 0 ? 1.f : -1.f;
-        b = b + phi * rmul(qp, conj(qr));
+        const f4 bp = rmul(qp, conj(qr));
+        const float phi = dot(q, bp) > 0 ? 1.f : -1.f;
+        b = b + phi * bp;
     }
     const float lambda = length(vv) + length(b);
     return normalize(rmul_e1(rmul_pure(vv, b)) + lambda * b);

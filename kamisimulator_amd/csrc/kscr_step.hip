@@ -50,7 +50,7 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
 
     f3 x{0, 0, 0}, v{0, 0, 0}, u{0, 0, 0};
     f4 q{0, 0, 0, 1}, qr{0, 0, 0, 1}, qrp{0, 0, 0, 1};
-    float im = 0.f, lr = 1.f, ks = 0.f;
+    float im = 0.f, lr = 1.f, ks = 0.f, lrp = 1.f, ksp = 0.f;
     uint32_t fl = 0;
     if (act) {
         x = ld3(a.x, i);
@@ -63,7 +63,12 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
         lr = m.y;
         ks = m.z;
         fl = __float_as_uint(m.w);
-        if (fl & FLAG_PREV) qrp = to4(a.q_rest[i - 1]);
+        if (fl & FLAG_PREV) {
+            qrp = to4(a.q_rest[i - 1]);
+            const float4 mp = a.mat[i - 1];
+            lrp = mp.y;
+            ksp = mp.z;
+        }
     }
     const bool hasNext = (fl & FLAG_NEXT) != 0;
     const bool hasPrev = (fl & FLAG_PREV) != 0;
@@ -77,6 +82,19 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
     const float dd = kk * invl;
     const float a_in = 1.0f / (im * h * h);
     const float f_in = 1.0f / (h * h * im);
+    // Diagonal of the position-solve matrix: a_in, plus the own segment's
+    // stretch term, plus the previous segment's (the value the previous
+    // vertex computes as dd (1 + damp)); constant over the step, so its
+    // inverse is formed once.
+    float hd = a_in;
+    if (hasNext) hd = hd + dd * (1 + damp);
+    if (hasPrev) {
+        const float invlp = 1.0f / lrp;
+        const float kkp = ksp * invlp;
+        const float ddp = kkp * invlp;
+        hd = hd + ddp * (1 + damp);
+    }
+    const InertiaInv hinv = inertia_inverse(hd, a_in);
 
     for (int s = 0; s < a.num_substeps; s++) {
         // ---- initItr: inertial target y and VBD adaptive initial guess
@@ -113,24 +131,19 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
                 const f3 c = ((xj - x) + (dxj - dx)) * invl - rotate_e1(q);
                 fo = kk * c - (damp * dd) * dx;
                 const f3 f2 = (-kk) * c - (damp * dd) * dxj;
-                s_f[t] = make_float4(f2.x, f2.y, f2.z, dd * (1 + damp));
+                s_f[t] = make_float4(f2.x, f2.y, f2.z, 0.0f);
             }
             __syncthreads();
             // ---- position solve, part 2: local 3x3 Newton step
             f3 dxn = dx;
             if (free) {
-                float hd = a_in;
                 f3 f = f_in * (y - dx);
-                if (hasNext) {
-                    f = f + fo;
-                    hd = hd + dd * (1 + damp);
-                }
+                if (hasNext) f = f + fo;
                 if (hasPrev) {
                     const float4 p = s_f[t - 1];
                     f = f + f3{p.x, p.y, p.z};
-                    hd = hd + p.w;
                 }
-                dxn = dx + a.acceleration_ratio * sym_solve(hd, hd, hd, a_in, a_in, a_in, f);
+                dxn = dx + a.acceleration_ratio * inertia_solve(hinv, f);
             }
             s_dx[cur ^ 1][0][t] = dxn.x;
             s_dx[cur ^ 1][1][t] = dxn.y;

@@ -49,23 +49,21 @@ struct Adjoint {
     }
 };
 
-// d/dq of rotate(q, e1) = (w^2 - |qv|^2) e1 + 2w (qv x e1) + 2 (qv.e1) qv, applied
-// transposed to rb.
+// rotate(q, e1) = (w^2 + x^2 - y^2 - z^2, 2(xy + wz), 2(xz - wy)) for
+// q = (x, y, z, w); its transposed Jacobian with respect to q applied to rb.
 template <class T>
 inline R4<T> rot_e1_vjp(R4<T> q, V3<T> rb) {
-    const V3<T> e{1, 0, 0};
-    const V3<T> qv = q.q();
-    const T w = q.w;
-    const T wb = dot(rb, T(2) * w * e + T(2) * cross(qv, e));
-    V3<T> qvb = T(-2) * dot(rb, e) * qv + T(2) * w * cross(e, rb) + T(2) * dot(rb, qv) * e + T(2) * dot(qv, e) * rb;
-    return R4<T>{qvb.x, qvb.y, qvb.z, wb};
+    const T wb = T(2) * std::fma(q.w, rb.x, std::fma(q.z, rb.y, -(q.y * rb.z)));
+    const T xb = T(2) * dot(rb, q.q());
+    const T yb = T(2) * std::fma(q.x, rb.y, -std::fma(q.y, rb.x, q.w * rb.z));
+    const T zb = T(2) * std::fma(q.x, rb.z, std::fma(q.w, rb.y, -(q.z * rb.x)));
+    return R4<T>{xb, yb, zb, wb};
 }
 
-// c = a (x) b:  abar = cbar (x) conj(b),  bbar = conj(a) (x) cbar
+// a (x) conj(e1) = -(a (x) e1)
 template <class T>
-inline void rmul_vjp(R4<T> a, R4<T> b, R4<T> cb, R4<T>& ab, R4<T>& bb) {
-    ab = add4(ab, rmul(cb, conj(b)));
-    bb = add4(bb, rmul(conj(a), cb));
+inline R4<T> rmul_conj_e1(R4<T> a) {
+    return R4<T>{-a.w, -a.z, a.y, a.x};
 }
 
 template <class T>
@@ -157,21 +155,19 @@ struct AdjointSim : Sim<T> {
                 R4<T> b{0, 0, 0, 0}, bm{0, 0, 0, 0}, bp{0, 0, 0, 0};
                 T phim = 0, phip = 0;
                 if (fl & F_PREV) {
-                    phim = dot4(rmul(conj(Q[i - 1]), Q[i]), qRest[i - 1]) > 0 ? T(1) : T(-1);
                     bm = rmul(Q[i - 1], qRest[i - 1]);
+                    phim = dot4(Q[i], bm) > 0 ? T(1) : T(-1);
                     b = add4(b, scale4(phim, bm));
                 }
                 if (fl & F_NEXT_ORI) {
-                    phip = dot4(rmul(conj(Q[i]), Q[i + 1]), qRest[i]) > 0 ? T(1) : T(-1);
                     bp = rmul(Q[i + 1], conj(qRest[i]));
+                    phip = dot4(Q[i], bp) > 0 ? T(1) : T(-1);
                     b = add4(b, scale4(phip, bp));
                 }
                 const T lv = length(vv), lb = length4(b);
                 const T lambda = lv + lb;
-                const R4<T> V = r4(vv, T(0));
-                const R4<T> m = rmul(V, b);
-                const R4<T> e1{1, 0, 0, 0};
-                const R4<T> tq = rmul(m, e1);
+                const R4<T> m = rmul_pure(vv, b);
+                const R4<T> tq = rmul_e1(m);
                 const R4<T> w = add4(tq, scale4(lambda, b));
                 const T lw = length4(w);
                 const R4<T> qn = scale4(T(1) / lw, w);
@@ -181,28 +177,25 @@ struct AdjointSim : Sim<T> {
                 // w = t + lambda b
                 R4<T> bb = scale4(lambda, wb);
                 const T lamb = dot4(wb, b);
-                // t = m (x) e1
-                R4<T> mb{0, 0, 0, 0}, dummy{0, 0, 0, 0};
-                rmul_vjp(m, e1, wb, mb, dummy);
-                // m = V (x) b
-                R4<T> Vb{0, 0, 0, 0};
-                rmul_vjp(V, b, mb, Vb, bb);
-                V3<T> vvb = Vb.q();
+                // t = m (x) e1: mbar = wbar (x) conj(e1)
+                const R4<T> mb = rmul_conj_e1(wb);
+                // m = (vv, 0) (x) b: vvbar = vec(mbar (x) conj(b)), bbar += (-vv, 0) (x) mbar
+                V3<T> vvb = rmul(mb, conj(b)).q();
+                bb = add4(bb, rmul_pure(-vv, mb));
                 // lambda = |v| + |b|
                 if (lv > 0) vvb = vvb + (lamb / lv) * vv;
                 if (lb > 0) bb = add4(bb, scale4(lamb / lb, b));
                 // b terms
+                // bm = Q[i-1] (x) qRest[i-1], bp = Q[i+1] (x) conj(qRest[i])
                 if (fl & F_PREV) {
-                    R4<T> qmb{0, 0, 0, 0}, qrb{0, 0, 0, 0};
-                    rmul_vjp(Q[i - 1], qRest[i - 1], scale4(phim, bb), qmb, qrb);
-                    qb2[i - 1] = add4(qb2[i - 1], qmb);
-                    adj.qrb[i - 1] = add4(adj.qrb[i - 1], qrb);
+                    const R4<T> c = scale4(phim, bb);
+                    qb2[i - 1] = add4(qb2[i - 1], rmul(c, conj(qRest[i - 1])));
+                    adj.qrb[i - 1] = add4(adj.qrb[i - 1], rmul(conj(Q[i - 1]), c));
                 }
                 if (fl & F_NEXT_ORI) {
-                    R4<T> qpb{0, 0, 0, 0}, cqb{0, 0, 0, 0};
-                    rmul_vjp(Q[i + 1], conj(qRest[i]), scale4(phip, bb), qpb, cqb);
-                    qb2[i + 1] = add4(qb2[i + 1], qpb);
-                    adj.qrb[i] = add4(adj.qrb[i], conj(cqb));
+                    const R4<T> c = scale4(phip, bb);
+                    qb2[i + 1] = add4(qb2[i + 1], rmul(c, qRest[i]));
+                    adj.qrb[i] = add4(adj.qrb[i], conj(rmul(conj(Q[i + 1]), c)));
                 }
                 // v = e * sc
                 const V3<T> eb = vvb * sc;
@@ -221,24 +214,25 @@ struct AdjointSim : Sim<T> {
             for (int i = 0; i < n; i++) {
                 if (invMass[i] == 0) continue;
                 const T a_in = T(1) / (invMass[i] * hh * hh);
-                T D = 0;
+                // diagonal accumulated as the forward's H = H0 + H1 + H2
+                T hd = a_in;
                 V3<T> f = (T(1) / (hh * hh * invMass[i])) * (t.y[i] - DX[i]);
                 if (flags[i] & F_NEXT) {
                     const T il = T(1) / lRest[i];
-                    const V3<T> c = ((t.x[i + 1] - t.x[i]) + (DX[i + 1] - DX[i])) * il - rotate(Q[i], V3<T>{1, 0, 0});
+                    const V3<T> c = ((t.x[i + 1] - t.x[i]) + (DX[i + 1] - DX[i])) * il - rotate_e1(Q[i]);
                     const T kk = kStretch[i] * il, dd = kk * il;
                     f = f + (kk * c - (damp * dd) * DX[i]);
-                    D += dd * (1 + damp);
+                    hd = hd + dd * (1 + damp);
                 }
                 if (flags[i] & F_PREV) {
                     const int j = i - 1;
                     const T il = T(1) / lRest[j];
-                    const V3<T> c = ((t.x[i] - t.x[j]) + (DX[i] - DX[j])) * il - rotate(Q[j], V3<T>{1, 0, 0});
+                    const V3<T> c = ((t.x[i] - t.x[j]) + (DX[i] - DX[j])) * il - rotate_e1(Q[j]);
                     const T kk = kStretch[j] * il, dd = kk * il;
                     f = f + ((-kk) * c - (damp * dd) * DX[i]);
-                    D += dd * (1 + damp);
+                    hd = hd + dd * (1 + damp);
                 }
-                const Sym3<T> H{{a_in + D, a_in + D, a_in + D, a_in, a_in, a_in}};
+                const Sym3<T> H{{hd, hd, hd, a_in, a_in, a_in}};
                 const V3<T> delta = sym_solve(H, f);
                 fb[i] = sym_solve(H, T(p.acceleration_ratio) * dxb[i]);
                 Db[i] = -dot(fb[i], delta);  // D enters the diagonal of H: Hbar = -fbar delta^T
@@ -248,7 +242,7 @@ struct AdjointSim : Sim<T> {
             for (int j = 0; j < n; j++) {
                 if (!(flags[j] & F_NEXT)) continue;
                 const T il = T(1) / lRest[j];
-                const V3<T> c = ((t.x[j + 1] - t.x[j]) + (DX[j + 1] - DX[j])) * il - rotate(Q[j], V3<T>{1, 0, 0});
+                const V3<T> c = ((t.x[j + 1] - t.x[j]) + (DX[j + 1] - DX[j])) * il - rotate_e1(Q[j]);
                 const T kk = kStretch[j] * il, dd = kk * il;
                 const V3<T> cb = kk * (fb[j] - fb[j + 1]);
                 const V3<T> s = cb * il;

@@ -937,17 +937,19 @@ struct Sim {
         vv = vv * (T(-2) * kStretch[i]);
         R4<T> b{0, 0, 0, 0};
         const R4<T> qi = q[i];
+        // phi = sign <conj(q_{i-1}) q_i, r_{i-1}> (cosserat.cu:320) is
+        // sign <q_i, q_{i-1} r_{i-1}>, and sign <conj(q_i) q_{i+1}, r_i>
+        // (:328) is sign <q_i, q_{i+1} conj(r_i)>: <a, b> = Re(a conj(b)) and
+        // Re(ABC) = Re(BCA).  Both rotors are the terms of b.
         if (fl & F_PREV) {
-            const R4<T> qr = qRest[i - 1];
-            const R4<T> qj = q[i - 1];
-            T phi = dot4(rmul(conj(qj), qi), qr) > 0 ? T(1) : T(-1);
-            b = add4(b, scale4(phi, rmul(qj, qr)));
+            const R4<T> bm = rmul(q[i - 1], qRest[i - 1]);
+            T phi = dot4(qi, bm) > 0 ? T(1) : T(-1);
+            b = add4(b, scale4(phi, bm));
         }
         if (fl & F_NEXT_ORI) {
-            const R4<T> qr = qRest[i];
-            const R4<T> qj = q[i + 1];
-            T phi = dot4(rmul(conj(qi), qj), qr) > 0 ? T(1) : T(-1);
-            b = add4(b, scale4(phi, rmul(qj, conj(qr))));
+            const R4<T> bp = rmul(q[i + 1], conj(qRest[i]));
+            T phi = dot4(qi, bp) > 0 ? T(1) : T(-1);
+            b = add4(b, scale4(phi, bp));
         }
         T lambda = length(vv) + length4(b);
         R4<T> t = rmul_e1(rmul_pure(vv, b));

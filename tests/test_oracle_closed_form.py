@@ -9,6 +9,12 @@ the mathematics fixes, independently of the oracle's own formulas.
 * rest invariance (g = 0): a relaxed curly strand stays put and the
   orientation solve returns q (the SCR Eq. 18 fixed point);
 * exact translation equivariance with a translation that is exact in FP32;
+* rotor sign invariance: q and -q are one orientation, and the sign switches
+  of the orientation solve (YB/sim/cosserat.cu:320, :328) make the step
+  independent of the signs the rotors are stored with; the switches are
+  evaluated as <q_i, q_{i-1} r_{i-1}> and <q_i, q_{i+1} conj(r_i)>, equal to
+  the reference's <conj(q_{i-1}) q_i, r_{i-1}> and <conj(q_i) q_{i+1}, r_i>
+  by Re(ABC) = Re(BCA) (checked on random quaternions);
 * the SDF pass on an affine field: trilinear interpolation is exact, so the
   reference's half-cell convention gives query(p) = phi(p + ds/2);
 * setup quantities against their closed forms (mass lumping, setYoung).
@@ -115,6 +121,55 @@ def test_rest_invariance(oracle_lib):
     assert np.abs(P - pos.astype(np.float64)).max() < 1e-8
     assert np.abs(q1 - q0).max() < 1e-5
     np.testing.assert_allclose(np.linalg.norm(q1, axis=1), 1.0, atol=1e-12)
+
+
+def _qmul(a, b):
+    """Hamilton product of (x, y, z, w) quaternions (rows)."""
+    av, aw, bv, bw = a[..., :3], a[..., 3:], b[..., :3], b[..., 3:]
+    v = aw * bv + bw * av + np.cross(av, bv)
+    w = aw * bw - np.sum(av * bv, axis=-1, keepdims=True)
+    return np.concatenate([v, w], axis=-1)
+
+
+def _conj(a):
+    return np.concatenate([-a[..., :3], a[..., 3:]], axis=-1)
+
+
+def test_sign_switch_identities():
+    rng = np.random.default_rng(7)
+    qm, q, qp, rm, r = (rng.normal(size=(10000, 4)) for _ in range(5))
+    ref_m = np.sum(_qmul(_conj(qm), q) * rm, axis=1)  # cosserat.cu:320
+    ref_p = np.sum(_qmul(_conj(q), qp) * r, axis=1)  # cosserat.cu:328
+    got_m = np.sum(q * _qmul(qm, rm), axis=1)
+    got_p = np.sum(q * _qmul(qp, _conj(r)), axis=1)
+    scale = np.prod([np.linalg.norm(a, axis=1) for a in (qm, q, rm)], axis=0)
+    np.testing.assert_allclose(got_m, ref_m, rtol=0, atol=1e-12 * scale.max())
+    np.testing.assert_allclose(got_p, ref_p, rtol=0, atol=1e-12 * scale.max())
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_rotor_sign_invariance(oracle_lib, precision):
+    sc = scenes.groom(8, 24, seed=11)
+    p = _params(gravity=(0.0, -9.81, 0.0))
+    vel = np.random.default_rng(3).normal(scale=0.05, size=sc.positions.shape).astype(F32)
+    res = []
+    for flip in (False, True):
+        o = OracleSim(p, sc.positions, sc.strand_offsets, precision=precision)
+        scenes.setup_sim(o, sc)
+        o.set_pos_vel(vel=vel)
+        q = o.get_q(f64=True)
+        sign = np.where(np.arange(len(q)) % 3 == 1, -1.0, 1.0) if flip else np.ones(len(q))
+        o.set_q(q * sign[:, None])  # both runs start from the same float rotors
+        for _ in range(3):
+            o.advance(F32(1 / 600))
+        x, v = o.get_pos_vel(f64=True)
+        res.append((x, v, o.get_q(f64=True) * sign[:, None], q))
+    # negating a rotor is exact and every use of it is even in its sign or
+    # goes through a sign switch: the runs agree bit for bit
+    for a, b in zip(res[0][:3], res[1][:3]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(res[0][0] - sc.positions).max() > 1e-4  # the strands moved
+    assert np.abs(res[0][2] - res[0][3]).max() > 1e-3  # and turned
 
 
 def test_translation_equivariance(oracle_lib):
