@@ -43,6 +43,13 @@ KS_FN f4 rmul_conj_e1(f4 a) { return f4{-a.w, -a.z, a.y, a.x}; }
 __device__ inline f4 ld4(const float4* p, size_t i) { return to4(p[i]); }
 __device__ inline void st4(float4* p, size_t i, f4 a) { p[i] = make_float4(a.x, a.y, a.z, a.w); }
 
+// Values of one tape iterate a thread reads: its own and the next vertex's
+// dx, and the rotors of its segment and of the two neighbouring ones.
+struct Iterate {
+    f3 dx, dx_n;
+    f4 q, qm, qp;
+};
+
 template <int BLOCK>
 struct AdjLds {
     float eb[3][BLOCK];   // d(segment vector e) of the orientation solve
@@ -51,6 +58,7 @@ struct AdjLds {
     float4 qrp[BLOCK];    // contribution to the previous vertex's qRest adjoint
     int mask[BLOCK];      // 1: orientation solved, 2: previous term, 4: next term
     float4 fbD[BLOCK];    // adjoint of the position solve's force, and of D
+    float c[3][BLOCK];    // stretch constraint c of the segment at iterate k
 };
 
 template <int BLOCK>
@@ -95,13 +103,18 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
     if (has_next) hd = hd + dd * (1.0f + damp);
     if (has_prev) hd = hd + ddp * (1.0f + damp);
     const InertiaInv hinv = inertia_inverse(hd, a_in);
+    // Tape reads use indices clamped into [0, n): a thread past the pack or a
+    // vertex without a neighbour loads some valid entry and never uses it
+    // (every use is under the same act / has_* condition), so the loads need
+    // no branches.
+    const int ic = min(i, n - 1);
+    const int inx = min(ic + 1, n - 1), ipv = max(ic - 1, 0);
 
     for (int sub = a.num_substeps - 1; sub >= 0; sub--) {
         const size_t so = (size_t)sub * n;
-        const f3 x = act ? ld3(a.tape.x + so * 3, i) : zero3();
-        const f3 xn = has_next ? ld3(a.tape.x + so * 3, i + 1) : zero3();
-        const f3 xp = has_prev ? ld3(a.tape.x + so * 3, i - 1) : zero3();
-        const f3 y = act ? ld3(a.tape.y + so * 3, i) : zero3();
+        const f3 x = ld3(a.tape.x + so * 3, ic);
+        const f3 xn = ld3(a.tape.x + so * 3, inx);
+        const f3 y = ld3(a.tape.y + so * 3, ic);
 
         // ---- endItr: x' = x + dx^K, v' = dx^K / h * decay (free vertices)
         f3 dxb = xb + (free ? vb * (invH * decay) : zero3());
@@ -111,35 +124,38 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
         // Iterates of the tape: k + 1 (dxn, dxn_n) is carried over from the
         // previous pass of the loop, k is loaded one pass ahead (the loads stay
         // in flight across the barriers, which wait on LDS traffic only).
-        const size_t o_sub = (size_t)sub * (K + 1) * n;
-        f3 dxn = zero3(), dxn_n = zero3();
-        f3 dx = zero3(), dx_n = zero3(), dx_p = zero3();
-        f4 Q = zero4(), Qm = zero4(), Qp = zero4();
-        if (K > 0) {
-            const float* DXK = a.tape.dx + (o_sub + (size_t)K * n) * 3;
-            const float* DXk = a.tape.dx + (o_sub + (size_t)(K - 1) * n) * 3;
-            const float4* Qk = a.tape.q + o_sub + (size_t)(K - 1) * n;
-            if (act) dxn = ld3(DXK, i), dx = ld3(DXk, i), Q = ld4(Qk, i);
-            if (has_next) dxn_n = ld3(DXK, i + 1), dx_n = ld3(DXk, i + 1);
-            if (has_prev) dx_p = ld3(DXk, i - 1), Qm = ld4(Qk, i - 1);
-            if (has_next_ori) Qp = ld4(Qk, i + 1);
-        }
-        for (int k = K - 1; k >= 0; k--) {
-            // prefetch iterate k - 1
-            f3 pdx = zero3(), pdx_n = zero3(), pdx_p = zero3();
-            f4 pQ = zero4(), pQm = zero4(), pQp = zero4();
-            if (k > 0) {
-                const float* DXk = a.tape.dx + (o_sub + (size_t)(k - 1) * n) * 3;
-                const float4* Qk = a.tape.q + o_sub + (size_t)(k - 1) * n;
-                if (act) pdx = ld3(DXk, i), pQ = ld4(Qk, i);
-                if (has_next) pdx_n = ld3(DXk, i + 1);
-                if (has_prev) pdx_p = ld3(DXk, i - 1), pQm = ld4(Qk, i - 1);
-                if (has_next_ori) pQp = ld4(Qk, i + 1);
+        const size_t o_sub = (size_t)sub * (K + 1) * n;  // dx^k: K + 1 per substep
+        const size_t q_sub = (size_t)sub * K * n;        // q^k: K per substep
+        // Reverse iteration k reads iterate k + 1 (dx, dx_n) and iterate k
+        // (dx, dx_n and the rotors) and prefetches iterate k - 1 (iterate 0
+        // again on the last pass).  Three register sets take the three roles
+        // in turn (loop unrolled by three), so nothing is copied between
+        // passes.
+        const auto load_iterate = [&](int k, Iterate& it) __attribute__((always_inline)) {
+            const float* DXk = a.tape.dx + (o_sub + (size_t)k * n) * 3;
+            const float4* Qk = a.tape.q + q_sub + (size_t)k * n;
+            it.dx = ld3(DXk, ic), it.dx_n = ld3(DXk, inx);
+            it.q = ld4(Qk, ic), it.qm = ld4(Qk, ipv), it.qp = ld4(Qk, inx);
+        };
+        const auto iteration = [&](int k, const Iterate& nx, const Iterate& cu, Iterate& pf)
+                                   __attribute__((always_inline)) {
+            load_iterate(max(k - 1, 0), pf);
+            const f3 dxn = nx.dx, dxn_n = nx.dx_n;
+            const f3 dx = cu.dx, dx_n = cu.dx_n;
+            const f4 Q = cu.q, Qm = cu.qm, Qp = cu.qp;
+
+            // stretch constraint of segment i at iterate k (the forward's c);
+            // the next vertex reads it for its previous segment
+            f3 c_next = zero3();
+            if (has_next) {
+                c_next = ((xn - x) + (dx_n - dx)) * il - rotate_e1(Q);
+                L.c[0][t] = c_next.x;
+                L.c[1][t] = c_next.y;
+                L.c[2][t] = c_next.z;
             }
 
             // ---- phase A: orientation solve of segment i (quaternionLambdaItr)
             f3 eb = zero3();
-            f4 qpb = zero4(), qmb = zero4(), qrp = zero4();
             int mask = 0;
             if (act && !pass) {
                 mask = 1;
@@ -173,28 +189,29 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
                 bb = bb + rmul_pure(neg(vv), mb);
                 if (lv > 0) vvb = vvb + (lamb / lv) * vv;
                 if (lb > 0) bb = bb + (lamb / lb) * b;
+                // neighbours read these entries only under the mask bits
                 if (has_prev) {
                     const f4 c = phim * bb;
-                    qmb = rmul(c, conj(qrm));
-                    qrp = rmul(conj(Qm), c);
+                    const f4 qmb = rmul(c, conj(qrm));
+                    const f4 qrp = rmul(conj(Qm), c);
+                    L.qmb[t] = make_float4(qmb.x, qmb.y, qmb.z, qmb.w);
+                    L.qrp[t] = make_float4(qrp.x, qrp.y, qrp.z, qrp.w);
                     mask |= 2;
                 }
                 if (has_next_ori) {
                     const f4 c = phip * bb;
-                    qpb = rmul(c, qr);
+                    const f4 qpb = rmul(c, qr);
                     const f4 cqb = rmul(conj(Qp), c);
                     qrb = qrb + conj(cqb);
+                    L.qpb[t] = make_float4(qpb.x, qpb.y, qpb.z, qpb.w);
                     mask |= 4;
                 }
                 eb = vvb * sc;
                 ksb += dot(vvb, e) * m2lr;
+                L.eb[0][t] = eb.x;
+                L.eb[1][t] = eb.y;
+                L.eb[2][t] = eb.z;
             }
-            L.eb[0][t] = eb.x;
-            L.eb[1][t] = eb.y;
-            L.eb[2][t] = eb.z;
-            L.qpb[t] = make_float4(qpb.x, qpb.y, qpb.z, qpb.w);
-            L.qmb[t] = make_float4(qmb.x, qmb.y, qmb.z, qmb.w);
-            L.qrp[t] = make_float4(qrp.x, qrp.y, qrp.z, qrp.w);
             L.mask[t] = mask;
             __syncthreads();
             // gather, in the order the sequential sweep adds the terms
@@ -225,13 +242,11 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             f3 dxb2 = dxb;
             f3 fb = zero3();
             float Db = 0.0f;
-            f3 c_next = zero3();
-            if (has_next) c_next = ((xn - x) + (dx_n - dx)) * il - rotate_e1(Q);
             if (act && free) {
                 f3 f = c1 * (y - dx);
                 if (has_next) f = f + (kk * c_next - (damp * dd) * dx);
                 if (has_prev) {
-                    const f3 c = ((x - xp) + (dx - dx_p)) * ilp - rotate_e1(Qm);
+                    const f3 c{L.c[0][t - 1], L.c[1][t - 1], L.c[2][t - 1]};
                     f = f + ((-kkp) * c - (damp * ddp) * dx);
                 }
                 const f3 delta = inertia_solve(hinv, f);
@@ -276,10 +291,21 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             }
             dxb = dxb2;
             qb = qb2;
-            dxn = dx, dxn_n = dx_n;
-            dx = pdx, dx_n = pdx_n, dx_p = pdx_p;
-            Q = pQ, Qm = pQm, Qp = pQp;
+        };
+        Iterate A, B, C;
+        {
+            const float* DXK = a.tape.dx + (o_sub + (size_t)K * n) * 3;
+            A.dx = ld3(DXK, ic), A.dx_n = ld3(DXK, inx);
         }
+        if (K > 0) load_iterate(K - 1, B);
+        int k = K - 1;
+        for (; k >= 2; k -= 3) {
+            iteration(k, A, B, C);
+            iteration(k - 1, B, C, A);
+            iteration(k - 2, C, A, B);
+        }
+        if (k >= 0) iteration(k, A, B, C);
+        if (k >= 1) iteration(k - 1, B, C, A);
 
         // ---- initItr: y = x + h v + h^2 ext (free), dx^0 = h v (+ h^2 s ext), u' = v
         f3 vbn = vb + ub;
@@ -289,11 +315,9 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             vbn = vbn + hh * yb;
             if (a.ext2 > 0) {
                 const f3 ext{a.ext[0], a.ext[1], a.ext[2]};
-                const f3 v = ld3(a.tape.v + so * 3, i);
-                const f3 u = ld3(a.tape.u + so * 3, i);
-                const f3 acc = (v - u) / a.last_h;
-                const float sraw = dot(acc, ext) / a.ext2;
-                if (sraw > 0 && sraw < 1) {
+                // the forward's sigma = clamp(a . ext / |ext|^2, 0, 1), inside
+                // its clamp range
+                if (a.tape.sigma_free[so + i]) {
                     const float sb = hh * hh * dot(dxb, ext);
                     const f3 ab = (sb / a.ext2) * ext;
                     vbn = vbn + ab / a.last_h;

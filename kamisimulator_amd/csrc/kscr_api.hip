@@ -69,7 +69,8 @@ struct AdjointRun {
     std::vector<Clock> clocks;
     Clock end_clock{};
     // tape of one segment of `every` steps
-    DevBuf<float> t_x, t_v, t_u, t_y, t_dx;
+    DevBuf<float> t_x, t_y, t_dx;
+    DevBuf<unsigned char> t_sigma;
     DevBuf<float4> t_q;
     // adjoint state
     DevBuf<float> xb, vb, ub, ksb;
@@ -383,12 +384,13 @@ void set_clock(kscr_handle* h, const AdjointRun::Clock& c) {
 int reserve_tape(kscr_handle* h) {
     AdjointRun& A = h->adj;
     const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = (size_t)A.every;
-    const size_t per_sub = S * n * 3 * every, per_itr = S * I1 * n * every;
-    if (A.t_q.n >= per_itr && A.t_x.n >= per_sub) return KSCR_OK;
+    const size_t K = (size_t)A.num_itr;
+    const size_t per_sub = S * n * every, per_dx = S * I1 * n * every, per_q = S * K * n * every;
+    if (A.t_dx.n >= per_dx * 3 && A.t_q.n >= per_q && A.t_x.n >= per_sub * 3) return KSCR_OK;
     hipError_t e;
-    if ((e = A.t_x.alloc(per_sub)) != hipSuccess || (e = A.t_v.alloc(per_sub)) != hipSuccess ||
-        (e = A.t_u.alloc(per_sub)) != hipSuccess || (e = A.t_y.alloc(per_sub)) != hipSuccess ||
-        (e = A.t_dx.alloc(per_itr * 3)) != hipSuccess || (e = A.t_q.alloc(per_itr)) != hipSuccess)
+    if ((e = A.t_x.alloc(per_sub * 3)) != hipSuccess || (e = A.t_y.alloc(per_sub * 3)) != hipSuccess ||
+        (e = A.t_sigma.alloc(per_sub)) != hipSuccess || (e = A.t_dx.alloc(per_dx * 3)) != hipSuccess ||
+        (e = A.t_q.alloc(per_q)) != hipSuccess)
         return h->hip_fail(e, "hipMalloc tape");
     return KSCR_OK;
 }
@@ -396,14 +398,13 @@ int reserve_tape(kscr_handle* h) {
 // Tape slot of step j of a segment.
 TapeArgs tape_of(kscr_handle* h, size_t j) {
     AdjointRun& A = h->adj;
-    const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1;
+    const size_t n = h->n, S = (size_t)A.substeps, K = (size_t)A.num_itr;
     TapeArgs t{};
     t.x = A.t_x.p + j * S * n * 3;
-    t.v = A.t_v.p + j * S * n * 3;
-    t.u = A.t_u.p + j * S * n * 3;
     t.y = A.t_y.p + j * S * n * 3;
-    t.dx = A.t_dx.p + j * S * I1 * n * 3;
-    t.q = A.t_q.p + j * S * I1 * n;
+    t.sigma_free = A.t_sigma.p + j * S * n;
+    t.dx = A.t_dx.p + j * S * (K + 1) * n * 3;
+    t.q = A.t_q.p ? A.t_q.p + j * S * K * n : nullptr;
     t.n = (int)n;
     return t;
 }
@@ -939,7 +940,8 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     aa.velocity_decay = h->p.velocity_decay;
     aa.acceleration_ratio = h->p.acceleration_ratio;
     for (int k = 0; k < 3; k++) aa.ext[k] = h->p.gravity[k] * h->p.external_force_multiplier;
-    aa.ext2 = aa.ext[0] * aa.ext[0] + aa.ext[1] * aa.ext[1] + aa.ext[2] * aa.ext[2];
+    // |ext|^2 as the device dot product forms it
+    aa.ext2 = std::fma(aa.ext[2], aa.ext[2], std::fma(aa.ext[1], aa.ext[1], aa.ext[0] * aa.ext[0]));
     aa.xb = A.xb.p;
     aa.vb = A.vb.p;
     aa.ub = A.ub.p;

@@ -52,14 +52,16 @@ struct StepArgs {
 };
 
 // Tape of the differentiable step: for every substep of one recorded step,
-// x, v, u at the substep start, y, and every iterate dx^k, q^k (k = 0..numItr).
-// Arrays are indexed [(sub * (num_itr + 1) + k) * n + i] (iterates) and
-// [sub * n + i] (per-substep values), float3 packed as 3 floats.
+// x at the substep start, y, whether the adaptive initial guess's factor
+// lies strictly inside (0, 1) (its clamp is then inactive), every iterate
+// dx^k (k = 0..numItr) and the rotors q^k the iterations start from
+// (k = 0..numItr-1).  Arrays are indexed [(sub * (num_itr + 1) + k) * n + i]
+// (dx), [(sub * num_itr + k) * n + i] (q) and [sub * n + i] (per-substep
+// values), float3 packed as 3 floats.
 struct TapeArgs {
     float* x;
-    float* v;
-    float* u;
     float* y;
+    unsigned char* sigma_free;
     float* dx;
     float4* q;
     int n;

@@ -69,8 +69,11 @@ __device__ inline void sdf_project(const SdfArgs& s, f3& x, f3& v) {
 
 
 // initItr for one vertex (YB/sim/iteration.cu:8-59): inertial target y (as a
-// displacement), VBD adaptive initial guess dx, lastVel update.
-__device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, const f3& v, f3& u, f3& y, f3& dx) {
+// displacement), VBD adaptive initial guess dx, lastVel update.  sigma_free
+// reports whether the guess's factor was strictly inside (0, 1).
+__device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, const f3& v, f3& u, f3& y, f3& dx,
+                                   bool& sigma_free) {
+    sigma_free = false;
     const float h = a.h;
     const f3 g{a.gravity[0], a.gravity[1], a.gravity[2]};
     float windx = 0.f;
@@ -95,10 +98,17 @@ __device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, co
         const float ext2 = dot(ext, ext);
         if (ext2 > 0) {
             const f3 acc = (v - lastVel) / a.last_h;
-            const float sc = clamp01(dot(acc, ext) / ext2);
+            const float raw = dot(acc, ext) / ext2;
+            sigma_free = raw > 0 && raw < 1;
+            const float sc = clamp01(raw);
             dx = dx + (h * h * sc) * ext;
         }
     }
+}
+
+__device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, const f3& v, f3& u, f3& y, f3& dx) {
+    bool sigma_free;
+    init_vertex(a, free, x, v, u, y, dx, sigma_free);
 }
 
 // Orientation solve of one segment (YB/sim/cosserat.cu:303-335, SCR Eq. 18):

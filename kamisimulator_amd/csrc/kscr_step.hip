@@ -99,17 +99,14 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
     for (int s = 0; s < a.num_substeps; s++) {
         // ---- initItr: inertial target y and VBD adaptive initial guess
         f3 y, dx;
+        if (RECORD && act) tape3(tp->x, s, tp->n, i, x);
+        bool sigma_free;
+        init_vertex(a, free, x, v, u, y, dx, sigma_free);
         if (RECORD && act) {
-            tape3(tp->x, s, tp->n, i, x);
-            tape3(tp->v, s, tp->n, i, v);
-            tape3(tp->u, s, tp->n, i, u);
-        }
-        init_vertex(a, free, x, v, u, y, dx);
-        if (RECORD && act) {
-            const int base = s * (a.num_itr + 1);
             tape3(tp->y, s, tp->n, i, y);
-            tape3(tp->dx, base, tp->n, i, dx);
-            tp->q[(size_t)base * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
+            tp->sigma_free[(size_t)s * tp->n + i] = sigma_free ? 1 : 0;
+            tape3(tp->dx, s * (a.num_itr + 1), tp->n, i, dx);
+            if (a.num_itr > 0) tp->q[(size_t)s * a.num_itr * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
         }
         __syncthreads();  // previous substep's LDS reads are complete
         s_x[0][t] = x.x;
@@ -161,9 +158,8 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             dx = dxn;
             q = qn;
             if (RECORD && act) {
-                const int idx = s * (a.num_itr + 1) + k + 1;
-                tape3(tp->dx, idx, tp->n, i, dx);
-                tp->q[(size_t)idx * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
+                tape3(tp->dx, s * (a.num_itr + 1) + k + 1, tp->n, i, dx);
+                if (k + 1 < a.num_itr) tp->q[(size_t)(s * a.num_itr + k + 1) * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
             }
             // The next iteration's first LDS write (s_f) is ordered after this
             // iteration's s_f reads by the barrier above; its s_dx/s_q writes
