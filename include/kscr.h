@@ -180,8 +180,8 @@ int kscr_set_material(kscr_handle* h, const float* k_stretch, const float* q_res
  * kscr_adjoint_backward then takes the adjoints of the final positions and
  * velocities (float[3n] each, NULL = zero) back through the rollout: for each
  * segment of checkpoint_every steps, last first, it restores the segment's
- * checkpoint, re-runs the segment recording its tape, and sweeps its steps
- * backwards.  It returns d(loss)/d(kStretch) (float[n]) and d(loss)/d(qRest)
+ * checkpoint, re-runs the segment recording its tape (the last segment's tape
+ * is kept from kscr_adjoint_forward), and sweeps its steps backwards.  It returns d(loss)/d(kStretch) (float[n]) and d(loss)/d(qRest)
  * (float[4n]) and leaves the state at the end of the rollout.  Returns
  * KSCR_ERR_UNSUPPORTED for contacts, step size limits or wind.  Steps follow
  * kscr_advance (no SDF pass). */
