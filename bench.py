@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--no-adjoint-leg", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-batched-leg", action="store_true", help="skip the batched-vs-per-groom timing")
     ap.add_argument("--no-contacts-leg", action="store_true", help="skip the large groom with contacts")
+    ap.add_argument("--no-colored-leg", action="store_true", help="skip the even/odd coloured schedule timing")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--only-legs", action="store_true", help="run the adjoint and batched legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -311,6 +312,36 @@ def contacts_leg(stream, nsteps=10, strands=100_000):
     return out
 
 
+def colored_leg(stream, scene, dt, updates_per_step, nsteps):
+    """The headline groom with the even/odd coloured (Gauss-Seidel) schedule:
+    same numItr, same updates per step; no reference counterpart."""
+    import torch
+
+    from kamisimulator_amd import scenes
+    from kamisimulator_amd._lib import SCHEDULE_COLORED, Sim, default_params
+
+    p = scenes.make_params(default_params, scene.material, schedule=SCHEDULE_COLORED)
+    sim = Sim(p, scene.positions, scene.strand_offsets, device=0)
+    sim.set_stream(stream.cuda_stream)
+    scenes.setup_sim(sim, scene)
+    for _ in range(2):
+        sim.advance(dt)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(nsteps):
+        sim.advance(dt)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / nsteps
+    out = {"ms_per_step": round(ms, 4), "updates_per_s": updates_per_step / (ms / 1e3), "block": sim.block_size,
+           "note": "even/odd vertex colouring: each colour solved with the other colour's newest dx and q "
+                   "(two vertices per lane); k coloured iterations reach about the accuracy of 2k Jacobi "
+                   "iterations (tests/test_oracle_colored.py)"}
+    sim.close()
+    return out
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -414,6 +445,11 @@ def main():
         sdf_leg = {"ms_per_step": round(ms2, 4), "updates_per_s": updates_per_step / (ms2 / 1e3),
                    "sdf_grid": "128^3 sphere SDF, threshold 0"}
 
+    colored = None
+    if not args.no_colored_leg:
+        colored = colored_leg(stream, scene, dt, updates_per_step, max(5, args.steps // 5))
+        print(f"[bench] colored leg: {colored['ms_per_step']} ms/step", file=sys.stderr, flush=True)
+
     # Host exchange of the plugin interface: SolveNextStep downloads positions
     # and velocities every step (StableCosseratRodsSolver.cpp:190, :223-243).
     sim.get_pos_vel()
@@ -474,6 +510,7 @@ def main():
         "roofline": roofline,
         "streaming_sweep_equivalent_frac": round(streaming_equiv, 4),
         "with_sdf_step": sdf_leg,
+        "colored_step": colored,
         "host_exchange": exchange,
         "adjoint": adjoint,
         "batched": batched,

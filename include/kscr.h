@@ -47,9 +47,18 @@ typedef enum kscr_status {
     KSCR_ERR_UNSUPPORTED = 12
 } kscr_status;
 
+/* Iteration schedule (no reference counterpart for COLORED, an addition of
+ * this build).  JACOBI: every solve of iteration k reads iteration-k values,
+ * the reference's semantics.  COLORED: vertices and segments coloured by the
+ * parity of their global index; the even colour is solved first and the odd
+ * colour reads its new values (position solves, then orientation solves).
+ * Per iteration it costs what JACOBI costs and converges about as far as two
+ * JACOBI iterations.  It covers the contact-free step: with hair-hair contacts
+ * or step size limits, with the adjoint, and in a batch it is
+ * KSCR_ERR_UNSUPPORTED; strands of up to 1023 vertices. */
 enum {
-    KSCR_SCHEDULE_JACOBI = 0,   /* reference semantics (deterministic snapshot) */
-    KSCR_SCHEDULE_COLORED = 1   /* even/odd vertex colouring (Gauss-Seidel)    */
+    KSCR_SCHEDULE_JACOBI = 0,
+    KSCR_SCHEDULE_COLORED = 1
 };
 
 /* Vertex flags, same bit values as YarnBall::VertexFlags (YarnBall.h:33-39). */

@@ -137,22 +137,34 @@ int validate_meta(kscr_handle* h) {
                        "barrier_thickness)");
     if (h->p.num_itr < 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr must be >= 0");
     if (!(h->p.max_h > 0)) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "max_h must be > 0");
-    if (h->p.schedule != KSCR_SCHEDULE_JACOBI)
-        return h->fail(KSCR_ERR_UNSUPPORTED, "only the Jacobi schedule is available in this library build");
+    if (h->p.schedule != KSCR_SCHEDULE_JACOBI && h->p.schedule != KSCR_SCHEDULE_COLORED)
+        return h->fail(KSCR_ERR_INVALID_ARGUMENT, "unknown schedule");
+    if (h->p.schedule == KSCR_SCHEDULE_COLORED && (h->p.enable_hair_hair_coll || h->p.use_step_size_limit))
+        return h->fail(KSCR_ERR_UNSUPPORTED, "the coloured schedule covers the contact-free step (no contacts, no step size limit)");
     return KSCR_OK;
 }
+
+bool colored(const kscr_handle* h) { return h->p.schedule == KSCR_SCHEDULE_COLORED; }
+
+// Vertices a workgroup of `block` threads holds: one per lane, or two for the
+// coloured step (whose pair alignment can leave one lane slot unused).
+int pack_capacity(const kscr_handle* h, int block) { return colored(h) ? 2 * block - 1 : block; }
+
+// Largest instantiated block of the step kernel in use.
+int max_block(const kscr_handle* h) { return colored(h) ? 512 : 1024; }
 
 int longest_strand(const kscr_handle* h);
 
 int choose_block(const kscr_handle* h) {
     const int longest = longest_strand(h);
-    if (h->p.block_size > 0) return h->p.block_size;
-    // Smallest instantiated block that holds the longest strand, but at least
-    // 512 so that several ~100-vertex strands share a workgroup (fewer idle lanes).
+    if (h->p.block_size > 0) return h->p.block_size <= max_block(h) ? h->p.block_size : -1;
+    // Smallest instantiated block that holds the longest strand, but holding
+    // at least 512 vertices so that several ~100-vertex strands share a
+    // workgroup (fewer idle lanes).
     for (int b : kBlockSizes)
-        if (b >= longest && b >= 512) return b;
+        if (b <= max_block(h) && pack_capacity(h, b) >= longest && pack_capacity(h, b) >= 511) return b;
     for (int b : kBlockSizes)
-        if (b >= longest) return b;
+        if (b <= max_block(h) && pack_capacity(h, b) >= longest) return b;
     return -1;
 }
 
@@ -196,9 +208,12 @@ int upload_all(kscr_handle* h) {
         qr[i] = make_float4(h->q_rest[i].x, h->q_rest[i].y, h->q_rest[i].z, h->q_rest[i].w);
     }
     int block = choose_block(h);
-    if (block < 0) return h->fail(KSCR_ERR_STRAND_TOO_LONG, "a strand is longer than the largest workgroup (1024)");
+    if (block < 0)
+        return h->fail(KSCR_ERR_STRAND_TOO_LONG, "no workgroup size holds the longest strand (at most " +
+                                                     std::to_string(pack_capacity(h, max_block(h))) +
+                                                     " vertices for this schedule)");
     std::vector<int2> packs;
-    int rc = build_packs(h, block, packs);
+    int rc = build_packs(h, pack_capacity(h, block), packs);
     if (rc) return rc;
     h->block = block;
     h->num_packs = (int)packs.size();
@@ -323,7 +338,8 @@ int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
     if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit) return launch_contact_step(h, dt, with_sdf);
     const StepArgs a = step_args(h, dt, with_sdf);
-    e = launch_step_fused(a, h->d_packs.p, h->num_packs, h->block, h->stream);
+    e = colored(h) ? launch_step_colored(a, h->d_packs.p, h->num_packs, h->block, h->stream)
+                   : launch_step_fused(a, h->d_packs.p, h->num_packs, h->block, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
     return KSCR_OK;
 }
@@ -413,6 +429,7 @@ int adjoint_supported(kscr_handle* h) {
     if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit)
         return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the contact-free step (no contacts, no step size limit)");
     if (h->p.wind_peak != 0) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the step without wind");
+    if (colored(h)) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the Jacobi schedule");
     return KSCR_OK;
 }
 
@@ -631,13 +648,14 @@ int kscr_fix_orientation(kscr_handle* h, const int* ids, int n) {
 
 int kscr_set_params(kscr_handle* h, const kscr_params* p) {
     if (!h || !p) return KSCR_ERR_INVALID_ARGUMENT;
-    const int old_block = h->p.block_size;
+    const int old_block = h->p.block_size, old_schedule = h->p.schedule;
     h->p = *p;
     if (h->configured) {
         int rc = validate_meta(h);
         if (rc) return rc;
     }
-    if (p->block_size != old_block) h->uploaded = false;
+    // the packing depends on both: upload() rebuilds it
+    if (p->block_size != old_block || p->schedule != old_schedule) h->uploaded = false;
     return KSCR_OK;
 }
 
@@ -1055,7 +1073,7 @@ int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
     float best_ms = 0;
     int rc = KSCR_OK;
     for (int b : kBlockSizes) {
-        if (b < longest) continue;
+        if (b > max_block(h) || pack_capacity(h, b) < longest) continue;
         h->p.block_size = b;
         if ((rc = upload_all(h)) != KSCR_OK) break;
         if ((e = save(true)) != hipSuccess) { rc = h->hip_fail(e, "autotune restore"); break; }
@@ -1154,9 +1172,9 @@ int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_si
             delete b;
             return KSCR_ERR_NOT_CONFIGURED;
         }
-        if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit) {
+        if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit || h->p.schedule != KSCR_SCHEDULE_JACOBI) {
             delete b;
-            return KSCR_ERR_UNSUPPORTED;  // the batched kernel is the contact-free fused step
+            return KSCR_ERR_UNSUPPORTED;  // the batched kernel is the contact-free Jacobi step
         }
         if (g > 0 && h->device != handles[0]->device) {
             delete b;

@@ -95,6 +95,9 @@ struct AdjointArgs {
 // applies the SDF pass afterwards when sdf_on).  block is the workgroup size
 // in vertices; packs[p] = [begin, end) of workgroup p.
 hipError_t launch_step_fused(const StepArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream);
+// Even/odd coloured step (kscr_colored.hip): block threads, two vertices per
+// lane, packs of at most 2 block - 1 vertices; block <= 512.
+hipError_t launch_step_colored(const StepArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream);
 // Same step, writing the tape of the step (contact-free step, no SDF pass).
 hipError_t launch_step_record(const StepArgs& a, const TapeArgs& tape, const int2* packs, int num_packs, int block,
                               hipStream_t stream);

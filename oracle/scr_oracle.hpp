@@ -969,12 +969,36 @@ struct Sim {
     // One VBD iteration: cosseratItr then quaternionLambdaItr
     // (YBSimulator.cu:415-418) with snapshot reads.
     void iterate() {
+        if (p.schedule == 1) {
+            iterate_colored();
+            return;
+        }
 #pragma omp parallel for schedule(static) num_threads(threads)
         for (int i = 0; i < n; i++) dxNew[i] = position_solve(i);
 #pragma omp parallel for schedule(static) num_threads(threads)
         for (int i = 0; i < n; i++) qNew[i] = orientation_solve(i);
         std::swap(dx, dxNew);
         std::swap(q, qNew);
+    }
+
+    // Even/odd coloured iteration (no reference counterpart): vertices and
+    // segments coloured by the parity of their global index; each colour is
+    // solved with the values the previous colour left (Gauss-Seidel over the
+    // two colours), using the same per-vertex updates as the Jacobi step.
+    void iterate_colored() {
+        for (int c = 0; c < 2; c++) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+            for (int i = c; i < n; i += 2) dxNew[i] = position_solve(i);
+#pragma omp parallel for schedule(static) num_threads(threads)
+            for (int i = c; i < n; i += 2) dx[i] = dxNew[i];
+        }
+        // dxNew holds the new dx of every vertex: the orientation solve reads it
+        for (int c = 0; c < 2; c++) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+            for (int i = c; i < n; i += 2) qNew[i] = orientation_solve(i);
+#pragma omp parallel for schedule(static) num_threads(threads)
+            for (int i = c; i < n; i += 2) q[i] = qNew[i];
+        }
     }
 
     // Sim::advance (YB/sim/step.cpp:38-62)
