@@ -126,6 +126,17 @@ int kscr_pin_vertices(kscr_handle* h, const int* ids, int n);
 /* Sets VertexFlags::fixOrientation on vertices (a true clamp of a root
  * segment; the flag is honoured by cosserat.cu:303). */
 int kscr_fix_orientation(kscr_handle* h, const int* ids, int n);
+/* SagFree initialisation, as Sim::SagFree(rootVertices, iteration)
+ * (YBSimulator.cu:480-676), called by the SCR solver when SagFreeItr > 0
+ * (StableCosseratRodsSolver.cpp:207): per strand, rest lengths shortened so
+ * that stretch carries the weight below each segment, then frames and rest
+ * rotors turned so that bending carries the weight's torque.  Host code, like
+ * the reference's; call it after kscr_configure, kscr_set_young and
+ * kscr_pin_vertices and before kscr_upload (it marks the handle not
+ * uploaded).  roots: the first vertex of every strand, ascending, each strand
+ * ending where the next begins (NULL = the strands of kscr_create);
+ * gravity is the one in the handle's parameters. */
+int kscr_sag_free(kscr_handle* h, const int* roots, int num_roots, int iterations);
 /* Replaces the simulation parameters; runs the uploadMeta validation
  * (YBSimulator.cu:322-335). */
 int kscr_set_params(kscr_handle* h, const kscr_params* p);

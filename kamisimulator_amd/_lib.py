@@ -83,6 +83,7 @@ SIGNATURES = {
     "kscr_set_kbend": (C.c_int, [_P, C.c_float]),
     "kscr_pin_vertices": (C.c_int, [_P, _IP, C.c_int]),
     "kscr_fix_orientation": (C.c_int, [_P, _IP, C.c_int]),
+    "kscr_sag_free": (C.c_int, [_P, _IP, C.c_int, C.c_int]),
     "kscr_set_params": (C.c_int, [_P, C.POINTER(KscrParams)]),
     "kscr_get_params": (C.c_int, [_P, C.POINTER(KscrParams)]),
     "kscr_upload": (C.c_int, [_P]),
@@ -231,6 +232,14 @@ class Sim:
     def fix_orientation(self, ids):
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         self._check(self.lib.kscr_fix_orientation(self.h, iptr(ids), ids.size))
+
+    def sag_free(self, roots=None, iterations=1):
+        """Sim::SagFree: after configure/set_young/pin, before upload."""
+        if roots is None:
+            self._check(self.lib.kscr_sag_free(self.h, None, 0, int(iterations)))
+            return
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        self._check(self.lib.kscr_sag_free(self.h, iptr(roots), roots.size, int(iterations)))
 
     def set_params(self, params):
         self._check(self.lib.kscr_set_params(self.h, C.byref(params)))

@@ -626,6 +626,27 @@ int kscr_set_kbend(kscr_handle* h, float k) {
     return KSCR_OK;
 }
 
+int kscr_sag_free(kscr_handle* h, const int* roots, int num_roots, int iterations) {
+    if (!h || iterations < 0 || (roots && num_roots <= 0)) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->configured) return h->fail(KSCR_ERR_NOT_CONFIGURED, "No rest length. Must call configure()");
+    std::vector<int> starts;
+    if (roots) {
+        starts.assign(roots, roots + num_roots);
+        for (int s = 0; s < num_roots; s++)
+            if (starts[s] < 0 || starts[s] >= h->n || (s > 0 && starts[s] <= starts[s - 1]))
+                return h->fail(KSCR_ERR_INVALID_ARGUMENT, "roots must be ascending vertex ids");
+    } else {
+        starts.assign(h->strand_off.begin(), h->strand_off.end() - 1);
+    }
+    if (iterations == 0) return KSCR_OK;
+    static_assert(sizeof(f4) == 4 * sizeof(float), "rotors are 4 packed floats");
+    sag_free_groom(h->pos.data(), h->inv_mass.data(), h->l_rest.data(), h->k_stretch.data(),
+                   reinterpret_cast<float*>(h->q.data()), reinterpret_cast<float*>(h->q_rest.data()), h->p.gravity,
+                   starts.data(), (int)starts.size(), h->n, iterations);
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
 int kscr_pin_vertices(kscr_handle* h, const int* ids, int n) {
     if (!h || (n > 0 && !ids)) return KSCR_ERR_INVALID_ARGUMENT;
     for (int k = 0; k < n; k++) {

@@ -98,6 +98,10 @@ hipError_t launch_step_fused(const StepArgs& a, const int2* packs, int num_packs
 // Even/odd coloured step (kscr_colored.hip): block threads, two vertices per
 // lane, packs of at most 2 block - 1 vertices; block <= 512.
 hipError_t launch_step_colored(const StepArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream);
+// SagFree initialisation on the host mirror of a groom (kscr_sagfree.hip);
+// q and q_rest are (x, y, z, w) rotors, 4 floats each.
+void sag_free_groom(const float* pos, const float* inv_mass, float* l_rest, const float* k_stretch, float* q,
+                    float* q_rest, const float gravity[3], const int* roots, int num_roots, int num_verts, int iterations);
 // Same step, writing the tape of the step (contact-free step, no SDF pass).
 hipError_t launch_step_record(const StepArgs& a, const TapeArgs& tape, const int2* packs, int num_packs, int block,
                               hipStream_t stream);

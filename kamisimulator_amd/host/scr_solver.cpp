@@ -162,6 +162,8 @@ void StableCosseratRodsSolver::build_sim() {
         check(kscr_set_kstretch(sim_, jp_.StretchK), "setKStretch");
     }
     check(kscr_pin_vertices(sim_, root_vertices_.data(), (int)root_vertices_.size()), "pin roots");
+    if (jp_.SagFreeItr > 0)
+        check(kscr_sag_free(sim_, root_vertices_.data(), (int)root_vertices_.size(), jp_.SagFreeItr), "SagFree");
     check(kscr_upload(sim_), "upload");
     if (sdf_)
         check(kscr_set_sdf(sim_, sdf_->dim, sdf_->grid_min, sdf_->grid_max, sdf_->phi.data(), sdf_->grad.data(),
@@ -175,7 +177,7 @@ void StableCosseratRodsSolver::LoadSettings(const std::string& settingJsonPath) 
         jp_ = LoadSettingJsonContent(settingJsonPath, &json_dump_);
     if (!jp_.YBJsonPath.empty() && std::ifstream(jp_.YBJsonPath).good())
         throw std::runtime_error("YBJsonPath scenes are not supported by this solver build");
-    if (jp_.SagFreeItr > 0) throw std::runtime_error("SagFreeItr > 0 is not supported by this solver build");
+    if (jp_.SagFreeItr < 0) throw std::runtime_error("SagFreeItr must be >= 0");
 }
 
 void StableCosseratRodsSolver::Construct(const std::string& settingJsonPath) {

@@ -38,6 +38,7 @@ class Material:
     wind_time_freq: float = 0.4
     wind_sharpness: float = 3.0
     external_force_multiplier: float = 1.0
+    sag_free_itr: int = 0            # SagFreeItr
 
 
 @dataclass
@@ -99,13 +100,16 @@ def make_params(lib_default_params, material: Material, block_size=0, schedule=0
 
 def setup_sim(sim, scene: Scene):
     """The reference's setup sequence on any object with the Sim interface:
-    configure(density) -> setYoung -> pin roots (-> clamp) -> upload -> SDF."""
+    configure(density) -> setYoung -> pin roots (-> clamp) -> SagFree ->
+    upload -> SDF (StableCosseratRodsSolver.cpp:194-209)."""
     m = scene.material
     sim.configure(m.density)
     sim.set_young(m.young_gpa, m.deviation)
     sim.pin(scene.roots)
     if scene.clamp_root_orientation:
         sim.fix_orientation(scene.roots)
+    if m.sag_free_itr:
+        sim.sag_free(scene.roots, m.sag_free_itr)
     sim.upload()
     if scene.sdf is not None:
         sim.set_sdf(scene.sdf, threshold=scene.sdf_threshold)

@@ -20,6 +20,7 @@ struct Base {
     virtual int set_kbend(float k) = 0;
     virtual void pin(const int* ids, int m) = 0;
     virtual void fix(const int* ids, int m) = 0;
+    virtual int sag_free(const int* roots, int m, int iterations) = 0;
     virtual int set_params(const orc::Params& p) = 0;
     virtual int upload() = 0;
     virtual void set_pos_vel(const float* p, const float* v) = 0;
@@ -58,6 +59,11 @@ struct Impl : Base {
     int set_kbend(float k) override { return s.set_kbend(k); }
     void pin(const int* ids, int m) override { s.pin(ids, m); }
     void fix(const int* ids, int m) override { s.fix_orientation(ids, m); }
+    int sag_free(const int* roots, int m, int iterations) override {
+        if (!s.configured) return 7;
+        s.sag_free(roots, m, iterations);
+        return 0;
+    }
     int set_params(const orc::Params& p) override {
         s.p = p;
         return s.configured ? s.validate() : 0;
@@ -215,6 +221,9 @@ int oracle_pin_vertices(oracle_handle* h, const int* ids, int n) {
 int oracle_fix_orientation(oracle_handle* h, const int* ids, int n) {
     h->b->fix(ids, n);
     return 0;
+}
+int oracle_sag_free(oracle_handle* h, const int* roots, int n, int iterations) {
+    return h->b->sag_free(roots, n, iterations);
 }
 int oracle_set_params(oracle_handle* h, const void* params) {
     orc::Params p;

@@ -28,6 +28,7 @@ SIGNATURES = {
     "oracle_set_kbend": (C.c_int, [_P, C.c_float]),
     "oracle_pin_vertices": (C.c_int, [_P, _IP, C.c_int]),
     "oracle_fix_orientation": (C.c_int, [_P, _IP, C.c_int]),
+    "oracle_sag_free": (C.c_int, [_P, _IP, C.c_int, C.c_int]),
     "oracle_set_params": (C.c_int, [_P, _P]),
     "oracle_upload": (C.c_int, [_P]),
     "oracle_set_pos_vel": (C.c_int, [_P, _FP, _FP]),
@@ -150,6 +151,10 @@ class OracleSim:
     def fix_orientation(self, ids):
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         self._check(self.lib.oracle_fix_orientation(self.h, _i(ids), ids.size))
+
+    def sag_free(self, roots, iterations=1):
+        roots = np.ascontiguousarray(roots, dtype=np.int32)
+        self._check(self.lib.oracle_sag_free(self.h, _i(roots), roots.size, int(iterations)))
 
     def set_params(self, params):
         self._params = params

@@ -668,6 +668,169 @@ struct Sim {
         for (int k = 0; k < m; k++) flags[ids[k]] |= F_FIX_ORI;
     }
 
+    // ------------------------------------------------------------ SagFree
+    // Sim::SagFree / SingleSagFree (YBSimulator.cu:480-676): host code of the
+    // reference, run between setYoung/root pinning and upload
+    // (StableCosseratRodsSolver.cpp:203-209).  Float and double mixed as
+    // written there (glm host arithmetic: no FMA; glm's dot/length/normalize
+    // forms), C library sqrt/acos/sinf/cosf, fitAngle's long-double fmod.
+    // Pass 1 (tip to root) shortens each rest length so that the stretch force
+    // carries the weight below the segment; pass 2 (root to tip) rotates the
+    // orientations and rest rotors so that bending carries the torque of the
+    // weight.
+    struct SfV {
+        float x, y, z;
+    };
+    static SfV sf_add(SfV a, SfV b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+    static SfV sf_sub(SfV a, SfV b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+    static SfV sf_mul(float s, SfV a) { return {s * a.x, s * a.y, s * a.z}; }
+    static float sf_dot(SfV a, SfV b) { return a.x * b.x + a.y * b.y + a.z * b.z; }  // glm: tmp.x + tmp.y + tmp.z
+    static SfV sf_cross(SfV a, SfV b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+    static float sf_length(SfV a) { return std::sqrt(sf_dot(a, a)); }
+    static SfV sf_normalize(SfV a) { return sf_mul(1.0f / std::sqrt(sf_dot(a, a)), a); }
+    static float sf_dot4(R4<float> a, R4<float> b) { return (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w); }
+    static SfV sf_q(R4<float> r) { return {r.x, r.y, r.z}; }
+    // Rotor product (Rotor.h:210-212) and rotate (Rotor.h:114-123)
+    static R4<float> sf_rmul(R4<float> a, R4<float> b) {
+        const SfV v = sf_add(sf_add(sf_mul(a.w, sf_q(b)), sf_mul(b.w, sf_q(a))), sf_cross(sf_q(a), sf_q(b)));
+        return {v.x, v.y, v.z, a.w * b.w - sf_dot(sf_q(a), sf_q(b))};
+    }
+    static SfV sf_rotate(R4<float> r, SfV v) {
+        const SfV a = sf_add(sf_mul(r.w, v), sf_cross(sf_q(r), v));
+        const float c = sf_dot(v, sf_q(r));
+        return sf_add(sf_add(sf_mul(r.w, a), sf_cross(sf_q(r), a)), sf_mul(c, sf_q(r)));
+    }
+    static SfV sf_d3(R4<float> r) { return sf_normalize(sf_rotate(r, SfV{1, 0, 0})); }  // getD3 (Rotor.h:206)
+    // fitAngle (YBSimulator.cu:59-78): wrap into [-pi, pi) through a long-double fmod
+    static double sf_fmod(double x, double y) {
+        const long double lx = x, ly = y;
+        if (ly == 0) return 0;
+        return (double)(lx - (long long)(lx / ly) * ly);
+    }
+    static double sf_fit_angle(double a) {
+        const double pi = 3.141592653589793238462643, two_pi = 6.2831853071795864769252867665590057683943;
+        return sf_fmod(sf_fmod(a + pi, two_pi) + two_pi, two_pi) - pi;
+    }
+    static double sf_sign(double x) { return (double)((0 < x) - (x < 0)); }
+
+    void sag_free(const int* roots, int num_roots, int iterations) {
+        for (int it = 0; it < iterations; it++) single_sag_free(roots, num_roots);
+    }
+    void single_sag_free(const int* roots, int num_roots) {
+        const SfV g{p.gravity[0], p.gravity[1], p.gravity[2]};
+        auto pos = [&](int i) { return SfV{pos0[3 * i], pos0[3 * i + 1], pos0[3 * i + 2]}; };
+        for (int s = 0; s < num_roots; s++) {
+            const float eps = 1e-6f;
+            const int rootv = roots[s];
+            const int nextv = (s == num_roots - 1) ? n : roots[s + 1];
+            std::vector<SfV> f_srs((size_t)(nextv - rootv), SfV{0, 0, 0});
+
+            // 1. rest lengths against the stretch force
+            float massFromTip = 1 / invMassF[nextv - 1];
+            for (int v = 0; rootv + v < nextv - 1; ++v) {
+                const int vId = nextv - 2 - v;
+                const SfV axis = sf_d3(qF[vId]);
+                const float kSt = kStretchF[vId];
+                const float oldRest = lRestF[vId];
+                if (invMassF[vId] != 0)
+                    massFromTip = (float)((double)massFromTip + 0.5 / (double)invMassF[vId]);
+                else
+                    massFromTip = (float)((double)massFromTip +
+                                          0.5 * (double)(lRestF[vId] / lRestF[vId + 1]) / (double)invMassF[vId + 1]);
+                const double f_s_g = (double)(sf_dot(axis, g) * massFromTip);
+                const double f_s = -f_s_g;
+                float rest;
+                if (f_s == 0) {
+                    rest = oldRest;
+                } else if (1 - 4 * f_s * (double)oldRest / (double)kSt < 0) {
+                    rest = 1.5f * oldRest;
+                } else {
+                    const double fsk = f_s / (double)kSt;
+                    const double fskinv = (double)kSt / f_s;
+                    const double disc = std::sqrt(1 - (double)(4 * oldRest) * fsk);
+                    const double ans1 = (1 + disc) * 0.5 * fskinv;
+                    const double ans2 = (1 - disc) * 0.5 * fskinv;
+                    rest = (float)(std::fabs((double)oldRest - ans1) >= std::fabs((double)oldRest - ans2) ? ans2 : ans1);
+                }
+                rest = std::min(std::max(rest, eps), 1.5f * oldRest);
+                lRestF[vId] = rest;
+                if (rootv + v < nextv - 2) f_srs[vId - rootv - 1] = sf_mul((float)-f_s, axis);
+                massFromTip = (float)((double)massFromTip + 0.5 / (double)invMassF[vId]);
+            }
+
+            // 2. orientations and rest rotors against the bending torque
+            const float nearest = 1e-4f;
+            float massToTip = 0;
+            for (int v = 1; rootv + v < nextv; ++v) {
+                if (invMassF[rootv + v] != 0)
+                    massToTip += 1 / invMassF[rootv + v];
+                else
+                    massToTip += (lRestF[rootv + v] / lRestF[rootv + v - 1]) / invMassF[rootv + v - 1];
+            }
+            R4<float> accum{0, 0, 0, 1};
+            for (int v = 1; rootv + v < nextv - 1; ++v) {
+                const int vId = rootv + v;
+                const R4<float> cQk = sf_rmul(accum, qF[vId]);
+                if (invMassF[vId] != 0)
+                    massToTip = (float)((double)massToTip - 0.5 / (double)invMassF[vId]);
+                else
+                    massToTip = (float)((double)massToTip -
+                                        0.5 * (double)(lRestF[vId] / lRestF[vId + 1]) / (double)invMassF[vId + 1]);
+                const SfV axisk = sf_d3(cQk);
+                const SfV f_b = sf_add(f_srs[v], sf_mul(massToTip, g));
+                if (sf_dot(f_b, axisk) < 1 - nearest) {
+                    const float kBend = std::sqrt(sf_dot4(qRestF[vId], qRestF[vId]));
+                    const SfV planeNormal = sf_normalize(sf_cross(axisk, f_b));
+                    const SfV torqueDir = sf_cross(planeNormal, axisk);
+                    const float torque = sf_dot(torqueDir, f_b) * sf_length(sf_sub(pos(vId + 1), pos(vId)));
+                    const float Tk = torque / kBend;
+                    const float lfb = sf_length(f_b);
+                    const SfV fbn = {f_b.x / lfb, f_b.y / lfb, f_b.z / lfb};
+                    const double d = ((double)axisk.x * -(double)fbn.x + (double)axisk.y * -(double)fbn.y) +
+                                     (double)axisk.z * -(double)fbn.z;
+                    const double maxrad = std::acos(d);
+                    const double maxTheta1 = sf_fit_angle(maxrad + (double)nearest);
+                    const double maxTheta2 = sf_fit_angle(maxrad - (double)nearest);
+                    const double maxTheta = std::fabs(maxTheta1) >= std::fabs(maxTheta2) ? maxTheta2 : maxTheta1;
+                    const float left = maxTheta >= 0 ? 0.f : (float)maxTheta;
+                    const float right = maxTheta >= 0 ? (float)maxTheta : 0.f;
+                    double resTheta = 0;
+                    if (std::fabs(Tk) > 2) {
+                        resTheta = maxTheta;
+                    } else {
+                        const double half_pi = 1.5707963267948966192313216916397;
+                        double theta1 = 2 * std::acos(1 - (0.5 * (double)Tk * (double)Tk));
+                        double theta2 = 2 * std::acos(-1 + (0.5 * (double)Tk * (double)Tk));
+                        theta1 = sf_fit_angle(theta1);
+                        theta2 = sf_fit_angle(theta2);
+                        double cT1 = -sf_sign(maxTheta) * 100, cT2 = -sf_sign(maxTheta) * 100;
+                        if (std::fabs(theta1) <= half_pi) cT1 = theta1;
+                        if (std::fabs(theta2) > half_pi) cT2 = theta2;
+                        if (sf_sign(cT1) == sf_sign(maxTheta) && sf_sign(cT2) == sf_sign(maxTheta)) {
+                            resTheta = std::min(cT1, cT2);
+                        } else {
+                            if (sf_sign(cT1) == sf_sign(maxTheta)) resTheta = cT1;
+                            if (sf_sign(cT2) == sf_sign(maxTheta)) resTheta = cT2;
+                        }
+                    }
+                    resTheta = std::min(std::max(resTheta, (double)left), (double)right);
+                    // Rotor::angleAxis (Rotor.h:51-54) in float
+                    float rad = (float)resTheta;
+                    rad = (float)(rad * 0.5);
+                    const float sn = std::sin(rad), cs = std::cos(rad);
+                    const R4<float> qTheta{sn * planeNormal.x, sn * planeNormal.y, sn * planeNormal.z, cs};
+                    accum = sf_rmul(accum, qTheta);
+                    qF[vId] = sf_rmul(accum, qF[vId]);
+                    const R4<float> rel = sf_rmul(R4<float>{-qF[vId - 1].x, -qF[vId - 1].y, -qF[vId - 1].z, qF[vId - 1].w}, qF[vId]);
+                    const float inv = 1.0f / std::sqrt(sf_dot4(rel, rel));
+                    const float len = std::sqrt(sf_dot4(qRestF[vId - 1], qRestF[vId - 1]));
+                    qRestF[vId - 1] = R4<float>{rel.x * inv * len, rel.y * inv * len, rel.z * inv * len, rel.w * inv * len};
+                }
+                massToTip = (float)((double)massToTip - 0.5 / (double)invMassF[vId]);
+            }
+        }
+    }
+
     // uploadMeta validation (YBSimulator.cu:322-335)
     int validate() const {
         if (p.detection_scaler <= 1) return 5;

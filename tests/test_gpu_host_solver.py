@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 DT = np.float32(1 / 600)
 
 
-def test_solver_cli_matches_oracle(tmp_path):
+@pytest.mark.parametrize("sag_free,young", [(0, 3.0), (1, 3.0), (2, 0.001)])
+def test_solver_cli_matches_oracle(tmp_path, sag_free, young):
     rng = np.random.default_rng(9)
     ns, mv, steps = 80, 64, 30
     counts = rng.integers(2, mv + 1, size=ns)
@@ -28,12 +29,14 @@ def test_solver_cli_matches_oracle(tmp_path):
         cur[s, :b - a] = g.positions[a:b]
     R = 0.1
     grid = scenes.sdf_grid(scenes.sphere_sdf((0, 0, 0), R), (-1.5 * R,) * 3, (1.5 * R,) * 3, (48,) * 3)
-    settings = {"SubdivCoeff": 1, "EnableResample": False, "EnableSDF": True, "NumItr": 10, "YoungsModule": 3,
-                "CurveRadius": 4e-5, "Density": 1300, "Gravity": [0.0, -9.81, 0.0], "SDFThreshold": 0.0}
+    settings = {"SubdivCoeff": 1, "EnableResample": False, "EnableSDF": True, "NumItr": 10, "YoungsModule": young,
+                "CurveRadius": 4e-5, "Density": 1300, "Gravity": [0.0, -9.81, 0.0], "SDFThreshold": 0.0,
+                "SagFreeItr": sag_free}
     (tmp_path / "s.json").write_text(json.dumps(settings))
     hairio.write_hair(tmp_path / "in.bin", counts, cur)
     hairio.write_sdf(tmp_path / "sdf.bin", grid)
-    print(f"[gpu-host-solver] {ns} strands, hairSize.y {mv}, {steps} steps", flush=True)
+    print(f"[gpu-host-solver] {ns} strands, hairSize.y {mv}, {steps} steps, SagFreeItr {sag_free}, E {young}",
+          flush=True)
     r = hairio.cli("run", tmp_path / "s.json", tmp_path / "in.bin", tmp_path / "sdf.bin", steps, repr(float(DT)),
                    tmp_path / "out.bin")
     refs = json.loads(r.stdout)
@@ -44,7 +47,8 @@ def test_solver_cli_matches_oracle(tmp_path):
     pos = np.concatenate([cur[s, :counts[s]] for s in keep])
     off = np.concatenate([[0], np.cumsum([counts[s] for s in keep])]).astype(np.int32)
     assert refs["rootVertices"] == off[:-1].tolist()
-    m = scenes.Material(num_itr=10, young_gpa=3.0, wind_peak=40.0, gravity=(0.0, -9.81, 0.0))
+    m = scenes.Material(num_itr=10, young_gpa=young, wind_peak=40.0, gravity=(0.0, -9.81, 0.0),
+                        sag_free_itr=sag_free)
     p = scenes.make_params(default_params, m)
     sc = scenes.Scene("kept", pos, off, material=m, sdf=grid, sdf_threshold=0.0)
     o = scenes.setup_sim(OracleSim(p, pos, off, precision=32), sc)

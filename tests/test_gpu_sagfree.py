@@ -1,0 +1,56 @@
+"""SagFree of the library (kscr_sag_free, host code in kscr_sagfree.hip)
+against the oracle's restatement (scr_oracle.hpp: sag_free) on identical
+grooms: rest lengths, rest rotors and frames bit for bit, then the steps that
+follow.  Runs with the GPU tests because a library handle owns device memory."""
+import numpy as np
+import pytest
+
+from kamisimulator_amd import scenes
+from kamisimulator_amd._lib import KscrError, Sim, default_params
+from oracle.pyoracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+DT = np.float32(1 / 600)
+
+
+@pytest.mark.parametrize("young,iterations", [(3.0, 1), (1e-3, 1), (1e-3, 3)])
+def test_sag_free_matches_oracle(young, iterations):
+    sc = scenes.config_large(400, seed=7, sdf_dim=32)
+    sc.sdf = None
+    sc.material = scenes.Material(young_gpa=young, sag_free_itr=iterations)
+    print(f"[gpu-sagfree] verts={sc.num_verts} strands={sc.num_strands} E={young} iterations={iterations}", flush=True)
+    p = scenes.make_params(default_params, sc.material)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets, precision=32), sc)
+    o.set_threads(8)
+    rg, ro = g.get_rest(), o.get_rest()
+    for k in rg:
+        assert np.array_equal(rg[k].view(np.uint32), ro[k].view(np.uint32)), k
+    assert np.array_equal(g.get_q().view(np.uint32), o.get_q().view(np.uint32))
+    # SagFree changed the setup
+    base = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets, precision=32),
+                            scenes.Scene("plain", sc.positions, sc.strand_offsets,
+                                         material=scenes.Material(young_gpa=young)))
+    assert not np.array_equal(base.get_q(), o.get_q())
+    for _ in range(10):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+    gp, op = g.get_pos_vel()[0], o.get_pos_vel()[0]
+    nbad = int(np.count_nonzero(gp.view(np.uint32) != op.view(np.uint32)))
+    print(f"[gpu-sagfree] differing position words after 10 steps: {nbad}", flush=True)
+    assert nbad == 0
+
+
+def test_sag_free_arguments():
+    sc = scenes.config_groom(20, 30, with_sdf=False)
+    p = scenes.make_params(default_params, sc.material)
+    g = Sim(p, sc.positions, sc.strand_offsets)
+    with pytest.raises(KscrError):  # before configure
+        g.sag_free(None, 1)
+    g.configure(sc.material.density)
+    g.set_young(sc.material.young_gpa, sc.material.deviation)
+    with pytest.raises(KscrError):  # roots not ascending
+        g.sag_free(sc.roots[::-1], 1)
+    g.pin(sc.roots)
+    g.sag_free(None, 1)  # the strands of kscr_create
+    g.upload()
