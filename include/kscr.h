@@ -147,6 +147,13 @@ int kscr_upload(kscr_handle* h);
 /* Sim::uploadPosAndVel / downloadPosAndVel (YBSimulator.cu:375-386). */
 int kscr_set_pos_vel(kscr_handle* h, const float* positions, const float* velocities);
 int kscr_get_pos_vel(kscr_handle* h, float* positions, float* velocities);
+/* The same download without the copy out: positions and velocities (either
+ * may be NULL) receive pointers to the handle's page-locked host mirror
+ * (float[3n] each), valid until the next call on the handle.  No reference
+ * counterpart: the exchange of Sim::downloadPosAndVel at link speed, for a
+ * caller that scatters the state into its own layout (the host solver's
+ * DownloadHair). */
+int kscr_download_pos_vel(kscr_handle* h, const float** positions, const float** velocities);
 /* Orientation state q (float[num_verts*4]) in and out. */
 int kscr_get_q(kscr_handle* h, float* q);
 int kscr_set_q(kscr_handle* h, const float* q);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "kscr_upload": (C.c_int, [_P]),
     "kscr_set_pos_vel": (C.c_int, [_P, _FP, _FP]),
     "kscr_get_pos_vel": (C.c_int, [_P, _FP, _FP]),
+    "kscr_download_pos_vel": (C.c_int, [_P, C.POINTER(_FP), C.POINTER(_FP)]),
     "kscr_get_q": (C.c_int, [_P, _FP]),
     "kscr_set_q": (C.c_int, [_P, _FP]),
     "kscr_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
@@ -262,6 +263,17 @@ class Sim:
         v = np.empty((self.n, 3), np.float32)
         self._check(self.lib.kscr_get_pos_vel(self.h, fptr(p), fptr(v)))
         return p, v
+
+    def download_pos_vel(self):
+        """Positions and velocities as read-only views of the handle's
+        page-locked host mirror (valid until the next call on the handle)."""
+        p, v = _FP(), _FP()
+        self._check(self.lib.kscr_download_pos_vel(self.h, C.byref(p), C.byref(v)))
+        pa = np.ctypeslib.as_array(p, shape=(self.n, 3))
+        va = np.ctypeslib.as_array(v, shape=(self.n, 3))
+        pa.flags.writeable = False
+        va.flags.writeable = False
+        return pa, va
 
     def get_q(self):
         q = np.empty((self.n, 4), np.float32)

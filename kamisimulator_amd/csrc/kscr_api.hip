@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/kscr.h"
@@ -54,6 +55,51 @@ struct DevBuf {
     }
     ~DevBuf() { release(); }
 };
+// Page-locked host array (hipHostMalloc): the host mirror of the exchanged
+// state, so that device-host copies run at full link speed.
+template <class T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t resize(size_t count) {
+        if (count == n) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* data() { return p; }
+    const T* data() const { return p; }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+    ~PinnedBuf() { release(); }
+};
+
+// memcpy of large host arrays split over host threads
+void host_copy(void* dst, const void* src, size_t bytes) {
+    const size_t chunk = (size_t)8 << 20;
+    const unsigned workers = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (bytes < 2 * chunk || workers == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = (bytes + workers - 1) / workers;
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < workers; w++) {
+        const size_t a = w * per, b = std::min(bytes, a + per);
+        if (a >= b) break;
+        pool.emplace_back([=] { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
+    }
+    for (auto& t : pool) t.join();
+}
+
 // Device state of a differentiable rollout (kscr_adjoint_forward/backward).
 struct AdjointRun {
     float dt = 0;
@@ -87,8 +133,9 @@ struct kscr_handle {
     int n = 0;
     int device = 0;
     std::vector<int> strand_off;
-    // host mirror (YarnBall::Sim::verts/vels/qs/qRests)
-    std::vector<float> pos, vel;
+    // host mirror (YarnBall::Sim::verts/vels/qs/qRests); positions and
+    // velocities page-locked for the per-step exchange
+    PinnedBuf<float> pos, vel;
     std::vector<float> inv_mass, l_rest, k_stretch;
     std::vector<uint32_t> flags;
     std::vector<f4> q, q_rest;
@@ -509,8 +556,14 @@ int kscr_create(const kscr_params* p, int num_verts, const float* positions, int
     h->n = num_verts;
     h->device = device;
     h->strand_off.assign(strand_offsets, strand_offsets + num_strands + 1);
-    h->pos.assign(positions, positions + (size_t)num_verts * 3);
-    h->vel.assign((size_t)num_verts * 3, 0.f);
+    const size_t n3 = (size_t)num_verts * 3;
+    if (h->pos.resize(n3) != hipSuccess || h->vel.resize(n3) != hipSuccess) {
+        g_create_error = "hipHostMalloc of the host mirror failed";
+        delete h;
+        return KSCR_ERR_OUT_OF_MEMORY;
+    }
+    std::memcpy(h->pos.data(), positions, n3 * sizeof(float));
+    std::memset(h->vel.data(), 0, n3 * sizeof(float));
     // Sim::Sim initial values (YBSimulator.cu:118-126)
     h->inv_mass.assign(num_verts, 1.f);
     h->l_rest.assign(num_verts, 1.f);
@@ -701,13 +754,13 @@ int kscr_set_pos_vel(kscr_handle* h, const float* positions, const float* veloci
     hipError_t e;
     if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
     if (positions) {
-        std::memcpy(h->pos.data(), positions, n3 * sizeof(float));
+        host_copy(h->pos.data(), positions, n3 * sizeof(float));
         if ((e = hipMemcpyAsync(h->d_x.p, h->pos.data(), n3 * sizeof(float), hipMemcpyHostToDevice, h->stream)) !=
             hipSuccess)
             return h->hip_fail(e, "hipMemcpyAsync");
     }
     if (velocities) {
-        std::memcpy(h->vel.data(), velocities, n3 * sizeof(float));
+        host_copy(h->vel.data(), velocities, n3 * sizeof(float));
         if ((e = hipMemcpyAsync(h->d_v.p, h->vel.data(), n3 * sizeof(float), hipMemcpyHostToDevice, h->stream)) !=
             hipSuccess)
             return h->hip_fail(e, "hipMemcpyAsync");
@@ -728,8 +781,25 @@ int kscr_get_pos_vel(kscr_handle* h, float* positions, float* velocities) {
             hipSuccess)
         return h->hip_fail(e, "hipMemcpyAsync");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
-    if (positions) std::memcpy(positions, h->pos.data(), n3 * sizeof(float));
-    if (velocities) std::memcpy(velocities, h->vel.data(), n3 * sizeof(float));
+    if (positions) host_copy(positions, h->pos.data(), n3 * sizeof(float));
+    if (velocities) host_copy(velocities, h->vel.data(), n3 * sizeof(float));
+    return check_device_error(h);
+}
+
+int kscr_download_pos_vel(kscr_handle* h, const float** positions, const float** velocities) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    const size_t n3 = (size_t)h->n * 3;
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = hipMemcpyAsync(h->pos.data(), h->d_x.p, n3 * sizeof(float), hipMemcpyDeviceToHost, h->stream)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(h->vel.data(), h->d_v.p, n3 * sizeof(float), hipMemcpyDeviceToHost, h->stream)) !=
+            hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    if (positions) *positions = h->pos.data();
+    if (velocities) *velocities = h->vel.data();
     return check_device_error(h);
 }
 

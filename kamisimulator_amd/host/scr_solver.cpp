@@ -1,13 +1,32 @@
 // scr_solver.cpp -- see scr_solver.h.
 #include "scr_solver.h"
 
+#include <algorithm>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 #include "json_lite.h"
 
 namespace kami_amd {
+
+// Runs body(i) for i in [0, n) over the host's threads (strands are
+// independent in the Hair exchange).
+template <class F>
+static void ParallelStrands(size_t n, const F& body) {
+    const unsigned workers = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 256 || workers == 1) {
+        for (size_t i = 0; i < n; i++) body(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < workers; w++)
+        pool.emplace_back([&, w] {
+            for (size_t i = w; i < n; i += workers) body(i);
+        });
+    for (auto& t : pool) t.join();
+}
 
 StableCosseratRodsSolver::StableCosseratRodsSolver(std::shared_ptr<Hair> hair, int device)
     : hair_(std::move(hair)), device_(device) {}
@@ -68,6 +87,7 @@ StableCosseratRodsSolver::JsonParams StableCosseratRodsSolver::LoadSettingJsonCo
             throw std::runtime_error("Schedule must be \"jacobi\" or \"colored\"");
     }
     if (j.CheckKey("BlockSize")) p.BlockSize = j.GetInteger("BlockSize");
+    if (j.CheckKey("DeviceResidentHair")) p.DeviceResidentHair = j.GetBool("DeviceResidentHair");
     return p;
 }
 
@@ -200,7 +220,10 @@ void StableCosseratRodsSolver::SolveNextStep(float dt) {
     } else {
         check(kscr_advance(sim_, dt), "advance");
     }
-    DownloadHair();
+    // The reference downloads every step because its SDF pass runs on the
+    // host; here the state stays on the device and a device-resident Hair is
+    // brought up to date by the driver's DownloadHair at draw frames.
+    if (!jp_.DeviceResidentHair) DownloadHair();
 }
 
 // StableCosseratRodsSolver::UploadHair (StableCosseratRodsSolver.cpp:202-221)
@@ -208,7 +231,7 @@ void StableCosseratRodsSolver::UploadHair() {
     if (!sim_) throw std::runtime_error("Construct() has not been called");
     const Hair& h = *hair_;
     const int nv = NumSimVertices();
-    for (size_t i = 0; i < strand_refs_.size(); i++) {
+    ParallelStrands(strand_refs_.size(), [&](size_t i) {
         const uint32_t u = (uint32_t)strand_refs_[i];
         const int root = root_vertices_[i];
         const int next = (i + 1 == strand_refs_.size()) ? nv : root_vertices_[i + 1];
@@ -219,27 +242,30 @@ void StableCosseratRodsSolver::UploadHair() {
             dp[0] = p.x, dp[1] = p.y, dp[2] = p.z;
             dv[0] = w.x, dv[1] = w.y, dv[2] = w.z;
         }
-    }
+    });
     check(kscr_set_pos_vel(sim_, flat_pos_.data(), flat_vel_.data()), "upload positions and velocities");
 }
 
 // StableCosseratRodsSolver::DownloadHair (StableCosseratRodsSolver.cpp:223-243)
 void StableCosseratRodsSolver::DownloadHair() {
     if (!sim_) throw std::runtime_error("Construct() has not been called");
-    check(kscr_get_pos_vel(sim_, flat_pos_.data(), flat_vel_.data()), "download positions and velocities");
+    // straight from the library's page-locked mirror into the padded Hair
+    // layout, strands split over host threads (the reference's OpenMP loop)
+    const float *P = nullptr, *V = nullptr;
+    check(kscr_download_pos_vel(sim_, &P, &V), "download positions and velocities");
     Hair& h = *hair_;
     const int nv = NumSimVertices();
-    for (size_t i = 0; i < strand_refs_.size(); i++) {
+    ParallelStrands(strand_refs_.size(), [&](size_t i) {
         const uint32_t u = (uint32_t)strand_refs_[i];
         const int root = root_vertices_[i];
         const int next = (i + 1 == strand_refs_.size()) ? nv : root_vertices_[i + 1];
         for (int v = 0; root + v < next; v++) {
-            const float* sp = &flat_pos_[3 * (size_t)(root + v)];
-            const float* sv = &flat_vel_[3 * (size_t)(root + v)];
+            const float* sp = P + 3 * (size_t)(root + v);
+            const float* sv = V + 3 * (size_t)(root + v);
             h.set(h.current, u, (uint32_t)v, Vec3{sp[0], sp[1], sp[2]});
             h.set(h.velocity, u, (uint32_t)v, Vec3{sv[0], sv[1], sv[2]});
         }
-    }
+    });
 }
 
 // StableCosseratRodsSolver::GetInfoString (StableCosseratRodsSolver.cpp:245-269)

@@ -11,8 +11,10 @@
 // Differences, all additive: the body SDF is handed over as a sampled grid
 // (SubstituteBodySdf) instead of being built from the body mesh; SolveNextStep
 // runs the SDF pass on the device instead of the host round trip; the
-// settings accept Gravity, SDFThreshold, MaxSubstep, Schedule and BlockSize.
-// Not available (Construct throws): YBJsonPath scenes and SagFreeItr > 0.
+// settings accept Gravity, SDFThreshold, MaxSubstep, Schedule, BlockSize and
+// DeviceResidentHair (SolveNextStep then leaves the Hair to DownloadHair,
+// which the driver calls once per draw frame).  Not available (Construct
+// throws): YBJsonPath scenes.
 #pragma once
 
 #include <memory>
@@ -68,6 +70,7 @@ class StableCosseratRodsSolver {
         float MaxSubstep = 4e-4f;  // Sim::maxH
         int Schedule = KSCR_SCHEDULE_JACOBI;
         int BlockSize = 0;
+        bool DeviceResidentHair = false;
     };
 
     explicit StableCosseratRodsSolver(std::shared_ptr<Hair> hair, int device = -1);

@@ -48,12 +48,12 @@ void print_params(const StableCosseratRodsSolver::JsonParams& p) {
         "\"Density\": %.9g, \"ResampleLen\": %.9g, \"VelocityDecay\": %.9g, \"ExternalForceMultiplier\": %.9g, "
         "\"WindPeak\": %.9g, \"WindYFreq\": %.9g, \"WindZFreq\": %.9g, \"WindTimeFreq\": %.9g, \"WindSharpness\": %.9g, "
         "\"HasGravity\": %d, \"Gravity\": [%.9g, %.9g, %.9g], \"SDFThreshold\": %.9g, \"MaxSubstep\": %.9g, "
-        "\"Schedule\": %d, \"BlockSize\": %d}\n",
+        "\"Schedule\": %d, \"BlockSize\": %d, \"DeviceResidentHair\": %d}\n",
         p.YBJsonPath.c_str(), p.EnableResample, p.EnableSDF, p.EnableHairHairColl, p.UseYoungsModule, p.NumItr,
         p.SagFreeItr, p.SubdivCoeff, p.StretchK, p.BendK, p.YoungsModule, p.StiffnessDeviationCoeff, p.CurveRadius,
         p.Density, p.ResampleLen, p.VelocityDecay, p.ExternalForceMultiplier, p.WindPeak, p.WindYFreq, p.WindZFreq,
         p.WindTimeFreq, p.WindSharpness, p.HasGravity, p.Gravity[0], p.Gravity[1], p.Gravity[2], p.SDFThreshold,
-        p.MaxSubstep, p.Schedule, p.BlockSize);
+        p.MaxSubstep, p.Schedule, p.BlockSize, p.DeviceResidentHair);
 }
 
 void print_refs(const StableCosseratRodsSolver& s) {
@@ -98,6 +98,7 @@ int main(int argc, char** argv) {
             const int steps = std::atoi(argv[5]);
             const float dt = (float)std::atof(argv[6]);
             for (int k = 0; k < steps; k++) s.SolveNextStep(dt);
+            s.DownloadHair();  // the driver's draw-frame download (KamiSimulator.cpp:389)
             hair->write(argv[7]);
             print_refs(s);
             std::fprintf(stderr, "%s", s.GetInfoString().c_str());

@@ -16,8 +16,9 @@ pytestmark = pytest.mark.gpu
 DT = np.float32(1 / 600)
 
 
-@pytest.mark.parametrize("sag_free,young", [(0, 3.0), (1, 3.0), (2, 0.001)])
-def test_solver_cli_matches_oracle(tmp_path, sag_free, young):
+@pytest.mark.parametrize("sag_free,young,resident", [(0, 3.0, False), (1, 3.0, False), (2, 0.001, False),
+                                                     (0, 3.0, True)])
+def test_solver_cli_matches_oracle(tmp_path, sag_free, young, resident):
     rng = np.random.default_rng(9)
     ns, mv, steps = 80, 64, 30
     counts = rng.integers(2, mv + 1, size=ns)
@@ -31,12 +32,12 @@ def test_solver_cli_matches_oracle(tmp_path, sag_free, young):
     grid = scenes.sdf_grid(scenes.sphere_sdf((0, 0, 0), R), (-1.5 * R,) * 3, (1.5 * R,) * 3, (48,) * 3)
     settings = {"SubdivCoeff": 1, "EnableResample": False, "EnableSDF": True, "NumItr": 10, "YoungsModule": young,
                 "CurveRadius": 4e-5, "Density": 1300, "Gravity": [0.0, -9.81, 0.0], "SDFThreshold": 0.0,
-                "SagFreeItr": sag_free}
+                "SagFreeItr": sag_free, "DeviceResidentHair": resident}
     (tmp_path / "s.json").write_text(json.dumps(settings))
     hairio.write_hair(tmp_path / "in.bin", counts, cur)
     hairio.write_sdf(tmp_path / "sdf.bin", grid)
-    print(f"[gpu-host-solver] {ns} strands, hairSize.y {mv}, {steps} steps, SagFreeItr {sag_free}, E {young}",
-          flush=True)
+    print(f"[gpu-host-solver] {ns} strands, hairSize.y {mv}, {steps} steps, SagFreeItr {sag_free}, E {young}, "
+          f"device-resident Hair {resident}", flush=True)
     r = hairio.cli("run", tmp_path / "s.json", tmp_path / "in.bin", tmp_path / "sdf.bin", steps, repr(float(DT)),
                    tmp_path / "out.bin")
     refs = json.loads(r.stdout)

@@ -57,6 +57,18 @@ def _run(sc, steps, block=0, fp64=False, label=""):
     return ep, eq, ev, g, o
 
 
+def test_download_views_equal_copies():
+    sc = scenes.config_groom(200, 64, with_sdf=False)
+    g, _, _ = _pair(sc)
+    for _ in range(3):
+        g.solve_next_step(DT)
+    p, v = g.get_pos_vel()
+    pv, vv = g.download_pos_vel()
+    assert np.array_equal(p.view(np.uint32), pv.view(np.uint32))
+    assert np.array_equal(v.view(np.uint32), vv.view(np.uint32))
+    assert not pv.flags.writeable
+
+
 def test_setup_matches_oracle():
     sc = scenes.config_groom(300, 100, with_sdf=False)
     g, o, _ = _pair(sc)

@@ -38,7 +38,7 @@ def test_every_reference_key_is_read(tmp_path):
          "VelocityDecay": 0.1, "ExternalForceMultiplier": 2, "WindPeak": 10, "WindYFreq": 2, "WindZFreq": 0.5,
          "WindTimeFreq": 0.3, "WindSharpness": 4.0, "YBJsonPath": "none.json",
          "Gravity": [0.0, -9.81, 1.0], "SDFThreshold": 0.002, "MaxSubstep": 2e-4, "Schedule": "colored",
-         "BlockSize": 256, "Unrelated": {"x": [1, "a"]}}
+         "BlockSize": 256, "DeviceResidentHair": True, "Unrelated": {"x": [1, "a"]}}
     d = _parse(tmp_path, s)
     for k, v in s.items():
         if k in ("Unrelated", "Schedule"):
