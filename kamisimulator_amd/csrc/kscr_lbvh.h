@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -21,6 +22,9 @@ struct alignas(64) LbvhNode {
     uint32_t parent, left, right, fence;
     float b[2][6];  // (min xyz, max xyz) of the left and right child
 };
+// the query reads a node as four float4 words
+static_assert(sizeof(LbvhNode) == 64 && offsetof(LbvhNode, fence) == 12 && offsetof(LbvhNode, b) == 16,
+              "LbvhNode layout");
 
 struct LbvhDev {
     int n = 0;

@@ -290,7 +290,7 @@ __global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n,
 // (the DFS bound of mergeUpKernel, lbvh.cu:153-188): leaf 1, node max(dL, dR + 1).
 __global__ void height_kernel(const uint32_t* __restrict__ leaf_parent, const LbvhNode* __restrict__ nodes,
                               uint32_t* __restrict__ word, int n, uint32_t* __restrict__ node_height,
-                              uint32_t* __restrict__ level_count, int* __restrict__ max_depth) {
+                              int* __restrict__ max_depth) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     uint32_t h = 0, d = 1;
@@ -305,9 +305,6 @@ __global__ void height_kernel(const uint32_t* __restrict__ leaf_parent, const Lb
         h = 1 + (h > oh ? h : oh);
         d = dl > dr + 1 ? dl : dr + 1;
         node_height[p] = h;
-        // a height past the bound can only come from a corrupt tree; the host
-        // then finds the level counts short and reports an error
-        if (h < (uint32_t)kMaxLevels) atomicAdd(level_count + h, 1u);
         if (p == 0) {
             *max_depth = (int)d;
             return;
@@ -316,25 +313,58 @@ __global__ void height_kernel(const uint32_t* __restrict__ leaf_parent, const Lb
     }
 }
 
-// Internal nodes grouped by height (offsets from the host).
-__global__ void level_scatter_kernel(const uint32_t* __restrict__ node_height, int m,
-                                     const uint32_t* __restrict__ level_offset, uint32_t* __restrict__ cursor,
-                                     uint32_t* __restrict__ level_nodes) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= m) return;
-    const uint32_t h = node_height[p];
-    level_nodes[level_offset[h] + atomicAdd(cursor + h, 1u)] = (uint32_t)p;
+// Internal nodes per height, and the nodes grouped by height (offsets from
+// the host).  A workgroup counts its kLevelChunk nodes in LDS and touches each
+// global counter once: per-node global atomics on a few dozen counters
+// serialise (most nodes sit at the lowest heights).  A height past the bound
+// can only come from a corrupt tree; the host then finds the counts short and
+// reports an error.
+constexpr int kLevelBlock = 256, kLevelPerThread = 16, kLevelChunk = kLevelBlock * kLevelPerThread;
+
+__global__ __launch_bounds__(kLevelBlock) void level_count_kernel(const uint32_t* __restrict__ node_height, int m,
+                                                                  uint32_t* __restrict__ level_count) {
+    __shared__ uint32_t bins[kMaxLevels];
+    for (int i = threadIdx.x; i < kMaxLevels; i += kLevelBlock) bins[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kLevelChunk;
+    for (int j = 0; j < kLevelPerThread; j++) {
+        const int p = base + j * kLevelBlock + threadIdx.x;
+        if (p < m && node_height[p] < (uint32_t)kMaxLevels) atomicAdd(&bins[node_height[p]], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxLevels; i += kLevelBlock)
+        if (bins[i]) atomicAdd(level_count + i, bins[i]);
+}
+
+__global__ __launch_bounds__(kLevelBlock) void level_scatter_kernel(const uint32_t* __restrict__ node_height, int m,
+                                                                    const uint32_t* __restrict__ level_offset,
+                                                                    uint32_t* __restrict__ cursor,
+                                                                    uint32_t* __restrict__ level_nodes) {
+    __shared__ uint32_t bins[kMaxLevels], start[kMaxLevels];
+    for (int i = threadIdx.x; i < kMaxLevels; i += kLevelBlock) bins[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kLevelChunk;
+    uint32_t h[kLevelPerThread], rank[kLevelPerThread];
+    for (int j = 0; j < kLevelPerThread; j++) {
+        const int p = base + j * kLevelBlock + threadIdx.x;
+        h[j] = p < m ? node_height[p] : (uint32_t)kMaxLevels;
+        if (h[j] < (uint32_t)kMaxLevels) rank[j] = atomicAdd(&bins[h[j]], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxLevels; i += kLevelBlock)
+        if (bins[i]) start[i] = atomicAdd(cursor + i, bins[i]);
+    __syncthreads();
+    for (int j = 0; j < kLevelPerThread; j++) {
+        const int p = base + j * kLevelBlock + threadIdx.x;
+        if (h[j] < (uint32_t)kMaxLevels) level_nodes[level_offset[h[j]] + start[h[j]] + rank[j]] = (uint32_t)p;
+    }
 }
 
 // Child boxes of the internal nodes of one height; their children are leaves
 // or nodes of smaller height, finished by earlier launches.  Unions take glm's
 // min/max (Bound::absorb), so a NaN coordinate propagates as in the reference.
-__global__ void refit_level_kernel(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
-                                   LbvhNode* __restrict__ nodes, const uint32_t* __restrict__ level_nodes, int begin,
-                                   int count) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    const uint32_t p = level_nodes[begin + k];
+__device__ inline void refit_node(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
+                                  LbvhNode* __restrict__ nodes, uint32_t p) {
     const uint32_t child[2] = {nodes[p].left, nodes[p].right};
     for (int side = 0; side < 2; side++) {
         const uint32_t c = child[side];
@@ -353,9 +383,40 @@ __global__ void refit_level_kernel(const float* __restrict__ aabb, const uint32_
     }
 }
 
+__global__ void refit_level_kernel(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
+                                   LbvhNode* __restrict__ nodes, const uint32_t* __restrict__ level_nodes, int begin,
+                                   int count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) refit_node(aabb, ids, nodes, level_nodes[begin + k]);
+}
+
+// The upper levels, each too small to fill the device, in one workgroup: a
+// barrier between levels orders them (the workgroup's writes are visible to
+// it after the barrier), which saves one launch per level.
+struct LevelList {
+    int num;
+    int2 lv[kMaxLevels];  // (offset, count) per level, lowest first
+};
+constexpr int kTopBlock = 1024;
+__global__ __launch_bounds__(kTopBlock) void refit_top_kernel(const float* __restrict__ aabb,
+                                                              const uint32_t* __restrict__ ids,
+                                                              LbvhNode* __restrict__ nodes,
+                                                              const uint32_t* __restrict__ level_nodes, LevelList levels) {
+    for (int l = 0; l < levels.num; l++) {
+        const int2 lv = levels.lv[l];
+        for (int k = threadIdx.x; k < lv.y; k += kTopBlock) refit_node(aabb, ids, nodes, level_nodes[lv.x + k]);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- query
-// Self query with IGNORE_SELF (lbvh.cu:196-289).  Each lane keeps its DFS
-// stack in LDS (column per lane).
+// Self query with IGNORE_SELF (lbvh.cu:196-289).  Each lane walks the tree
+// depth first: of the two children of a node it continues with the left one
+// and keeps the right one on its LDS stack (a column per lane) only when both
+// boxes overlap; leaf children are tested where they are found, so a pass
+// visits one internal node and reports up to two pairs.  Subtrees whose leaf
+// ranges end at or before the query's own sorted index are skipped (each
+// pair is reported once, by its lower index).
 template <int STACK>
 __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restrict__ aabb,
                                                             const uint32_t* __restrict__ ids,
@@ -370,41 +431,68 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
     for (int k = 0; k < 6; k++) q[k] = aabb[6 * (size_t)qid + k];
     if (!(q[0] <= q[3])) return;  // empty box (strand tip): intersects nothing
     const uint64_t lane_below = (1ull << (threadIdx.x & 63)) - 1;
+    constexpr uint32_t kLeaf = 0x80000000u, kNone = 0xFFFFFFFFu;
+    // object id of a leaf paired with the query, or -1
+    auto leaf_pair = [&](uint32_t leaf) -> int {
+        if ((int)leaf <= t) return -1;
+        const int fid = (int)ids[leaf];
+        if (min_gap >= 0 && abs(fid - qid) <= min_gap) return -1;
+        return fid;
+    };
     int sp = 0;
-    stack[sp++][threadIdx.x] = 0;
-    while (sp > 0) {
-        uint32_t c = stack[--sp][threadIdx.x];
-        int fid = -1;
-        if (c & 0x80000000u) {
-            c &= 0x7FFFFFFFu;
-            if ((int)c > t) {
-                fid = (int)ids[c];
-                if (min_gap >= 0 && abs(fid - qid) <= min_gap) fid = -1;
+    uint32_t node = 0;  // the root (internal: n >= 2)
+    while (node != kNone) {
+        int f0 = -1, f1 = -1;
+        uint32_t next = kNone;
+        // the whole 64-byte node in one round trip (four 16-byte loads)
+        const float4* np = reinterpret_cast<const float4*>(nodes + node);
+        const float4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+        const uint32_t fence = __float_as_uint(w0.w);
+        if (max(node, fence) > (uint32_t)t) {
+            const uint32_t lc = __float_as_uint(w0.y), rc = __float_as_uint(w0.z);
+            const float b0[6] = {w1.x, w1.y, w1.z, w1.w, w2.x, w2.y};
+            const float b1[6] = {w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+            bool h0 = intersects(b0, q), h1 = intersects(b1, q);
+            if (h0 && (lc & kLeaf)) {
+                f0 = leaf_pair(lc & ~kLeaf);
+                h0 = false;
             }
-        } else {
-            const LbvhNode& nd = nodes[c];
-            if (max(c, nd.fence) > (uint32_t)t) {
-                if (sp + 2 > STACK) {  // deeper than the LDS stack: report, stop this query
+            if (h1 && (rc & kLeaf)) {
+                f1 = leaf_pair(rc & ~kLeaf);
+                h1 = false;
+            }
+            if (h0 && h1) {
+                if (sp + 1 > STACK) {  // deeper than the LDS stack: report, stop this query
                     if (error) atomicOr(error, 4);
                     break;
                 }
-                if (intersects(nd.b[0], q)) stack[sp++][threadIdx.x] = nd.left;
-                if (intersects(nd.b[1], q)) stack[sp++][threadIdx.x] = nd.right;
+                stack[sp++][threadIdx.x] = rc;
+                next = lc;
+            } else if (h0) {
+                next = lc;
+            } else if (h1) {
+                next = rc;
             }
         }
-        // the lanes of the wave that found a pair in this pass take consecutive
+        if (next == kNone && sp > 0) next = stack[--sp][threadIdx.x];
+        // the lanes of the wave that found pairs in this pass take consecutive
         // slots with one atomic (a single global counter would serialise)
-        const uint64_t found = __ballot(fid >= 0);
-        if (found) {
-            const int leader = __ffsll((unsigned long long)found) - 1;
+        const uint64_t m0 = __ballot(f0 >= 0), m1 = __ballot(f1 >= 0);
+        if (m0 | m1) {
+            const int leader = __ffsll((unsigned long long)(m0 | m1)) - 1;
             int base = 0;
-            if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, __popcll(found));
+            if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, __popcll(m0) + __popcll(m1));
             base = __shfl(base, leader);
-            if (fid >= 0) {
-                const int slot = base + __popcll(found & lane_below);
-                if (slot < cap) pairs[slot] = make_int2(fid, qid);
+            if (f0 >= 0) {
+                const int slot = base + __popcll(m0 & lane_below);
+                if (slot < cap) pairs[slot] = make_int2(f0, qid);
+            }
+            if (f1 >= 0) {
+                const int slot = base + __popcll(m0) + __popcll(m1 & lane_below);
+                if (slot < cap) pairs[slot] = make_int2(f1, qid);
             }
         }
+        node = next;
     }
 }
 
@@ -506,7 +594,9 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
         (e = fill_i32((int*)d.level_count, 0, (size_t)2 * kMaxLevels, stream)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(height_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d.leaf_parent, d.nodes,
-                       (uint32_t*)d.flags, n, d.node_height, d.level_count, d.max_depth);
+                       (uint32_t*)d.flags, n, d.node_height, d.max_depth);
+    const dim3 level_grid((m + kLevelChunk - 1) / kLevelChunk);
+    hipLaunchKernelGGL(level_count_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, m, d.level_count);
     uint32_t counts[kMaxLevels];
     int depth = 0;
     if ((e = hipMemcpyAsync(counts, d.level_count, sizeof counts, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
@@ -524,7 +614,7 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
     if (acc != (uint32_t)m) return hipErrorUnknown;  // every internal node has one height
     if ((e = hipMemcpyAsync(d.level_offset, offsets, sizeof offsets, hipMemcpyHostToDevice, stream)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(level_scatter_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, d.node_height, m,
+    hipLaunchKernelGGL(level_scatter_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, m,
                        d.level_offset, d.level_count + kMaxLevels, d.level_nodes);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the host arrays above must outlive the copy
@@ -537,9 +627,21 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
 // levels (no atomics, no fences).
 hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
-    for (const int2& lv : d.levels)
+    // levels from `top` on all have fewer than 2 workgroups of nodes
+    size_t top = d.levels.size();
+    while (top > 0 && d.levels[top - 1].y < 2 * kTopBlock) top--;
+    for (size_t l = 0; l < top; l++) {
+        const int2 lv = d.levels[l];
         hipLaunchKernelGGL(refit_level_kernel, dim3((lv.y + 255) / 256), dim3(256), 0, stream, d.aabb, d.ids, d.nodes,
                            d.level_nodes, lv.x, lv.y);
+    }
+    if (top < d.levels.size()) {
+        LevelList list{};
+        list.num = (int)(d.levels.size() - top);
+        for (int l = 0; l < list.num; l++) list.lv[l] = d.levels[top + l];
+        hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, d.aabb, d.ids, d.nodes,
+                           d.level_nodes, list);
+    }
     return hipGetLastError();
 }
 
