@@ -414,9 +414,17 @@ __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(const float* __res
 // depth first: of the two children of a node it continues with the left one
 // and keeps the right one on its LDS stack (a column per lane) only when both
 // boxes overlap; leaf children are tested where they are found, so a pass
-// visits one internal node and reports up to two pairs.  Subtrees whose leaf
+// visits one internal node and finds up to two pairs.  Subtrees whose leaf
 // ranges end at or before the query's own sorted index are skipped (each
 // pair is reported once, by its lower index).
+//
+// Pairs collect in an LDS buffer per wave and go to the global list kWaveBuf
+// at a time, one atomic per flush (as the reference's shared pair buffer,
+// lbvh.cu:211-280): an atomic on the one global counter per found pair
+// serialises the whole query.
+constexpr int kWaveBuf = 128;
+constexpr int kQueryWaves = kQueryBlock / 64;
+
 template <int STACK>
 __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restrict__ aabb,
                                                             const uint32_t* __restrict__ ids,
@@ -424,13 +432,31 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
                                                             int2* __restrict__ pairs, int cap, int min_gap,
                                                             int* __restrict__ count, int* __restrict__ error) {
     __shared__ uint32_t stack[STACK][kQueryBlock];
+    __shared__ int2 wbuf[kQueryWaves][kWaveBuf];
+    __shared__ int wfill[kQueryWaves];
+    const int w = threadIdx.x >> 6;
+    const uint64_t lane_below = (1ull << (threadIdx.x & 63)) - 1;
+    if ((threadIdx.x & 63) == 0) wfill[w] = 0;
+    __builtin_amdgcn_wave_barrier();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n || n < 2) return;
     const int qid = (int)ids[t];
     float q[6];
     for (int k = 0; k < 6; k++) q[k] = aabb[6 * (size_t)qid + k];
     if (!(q[0] <= q[3])) return;  // empty box (strand tip): intersects nothing
-    const uint64_t lane_below = (1ull << (threadIdx.x & 63)) - 1;
+
+    // copies the wave's buffered pairs to the global list with the wave's
+    // active lanes; every active lane sees the same fill
+    auto flush = [&](int fill) {
+        const uint64_t act = __ballot(1);
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        int base = 0;
+        if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, fill);
+        base = __shfl(base, leader);
+        const int nact = __popcll(act);
+        for (int k = __popcll(act & lane_below); k < fill; k += nact)
+            if (base + k < cap) pairs[base + k] = wbuf[w][k];
+    };
     constexpr uint32_t kLeaf = 0x80000000u, kNone = 0xFFFFFFFFu;
     // object id of a leaf paired with the query, or -1
     auto leaf_pair = [&](uint32_t leaf) -> int {
@@ -448,6 +474,7 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
         const float4* np = reinterpret_cast<const float4*>(nodes + node);
         const float4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
         const uint32_t fence = __float_as_uint(w0.w);
+        bool overflow = false;
         if (max(node, fence) > (uint32_t)t) {
             const uint32_t lc = __float_as_uint(w0.y), rc = __float_as_uint(w0.z);
             const float b0[6] = {w1.x, w1.y, w1.z, w1.w, w2.x, w2.y};
@@ -464,36 +491,40 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
             if (h0 && h1) {
                 if (sp + 1 > STACK) {  // deeper than the LDS stack: report, stop this query
                     if (error) atomicOr(error, 4);
-                    break;
+                    overflow = true;
+                } else {
+                    stack[sp++][threadIdx.x] = rc;
+                    next = lc;
                 }
-                stack[sp++][threadIdx.x] = rc;
-                next = lc;
             } else if (h0) {
                 next = lc;
             } else if (h1) {
                 next = rc;
             }
         }
-        if (next == kNone && sp > 0) next = stack[--sp][threadIdx.x];
-        // the lanes of the wave that found pairs in this pass take consecutive
-        // slots with one atomic (a single global counter would serialise)
+        if (!overflow && next == kNone && sp > 0) next = stack[--sp][threadIdx.x];
+        // the lanes of the wave that found pairs in this pass append them to
+        // the wave's buffer in lane order
         const uint64_t m0 = __ballot(f0 >= 0), m1 = __ballot(f1 >= 0);
-        if (m0 | m1) {
-            const int leader = __ffsll((unsigned long long)(m0 | m1)) - 1;
-            int base = 0;
-            if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, __popcll(m0) + __popcll(m1));
-            base = __shfl(base, leader);
-            if (f0 >= 0) {
-                const int slot = base + __popcll(m0 & lane_below);
-                if (slot < cap) pairs[slot] = make_int2(f0, qid);
+        const int found = __popcll(m0) + __popcll(m1);
+        if (found) {
+            int fill = wfill[w];
+            if (fill + found > kWaveBuf) {
+                flush(fill);
+                fill = 0;
             }
-            if (f1 >= 0) {
-                const int slot = base + __popcll(m0) + __popcll(m1 & lane_below);
-                if (slot < cap) pairs[slot] = make_int2(f1, qid);
-            }
+            if (f0 >= 0) wbuf[w][fill + __popcll(m0 & lane_below)] = make_int2(f0, qid);
+            if (f1 >= 0) wbuf[w][fill + __popcll(m0) + __popcll(m1 & lane_below)] = make_int2(f1, qid);
+            __builtin_amdgcn_wave_barrier();
+            if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)(m0 | m1)) - 1) wfill[w] = fill + found;
+            __builtin_amdgcn_wave_barrier();
         }
-        node = next;
+        node = overflow ? kNone : next;
     }
+    // the lanes still here are the wave's lanes that ran a query
+    __builtin_amdgcn_wave_barrier();
+    const int fill = wfill[w];
+    if (fill > 0) flush(fill);
 }
 
 __global__ void fill_i32_kernel(int* p, int value, size_t n) {
