@@ -8,6 +8,14 @@
 //       strandRefs as JSON
 //   kscr_solver_cli run <settings.json> <hair.in> <sdf.bin|-> <steps> <dt> <hair.out>
 //       Construct, then `steps` SolveNextStep(dt); writes the Hair
+//   kscr_solver_cli usc2hair <in.data> <hair.out> [max_strands max_length]
+//   kscr_solver_cli hair2usc <hair.in> <name>            (writes name.data)
+//   kscr_solver_cli curves2hair <in.bcc|.obj|.poly> <hair.out> <target_seg_len> <resample 0|1>
+//                               <break_closed 0|1> [m0,...,m15]
+//       prints the closed flags as JSON; m: column-major transform
+//   kscr_solver_cli hair2bcc <hair.in> <out.bcc> <polyline 0|1>
+//   kscr_solver_cli hair2obj <hair.in> <out.obj>
+//   (no GPU for the file conversions)
 //
 // Hair files: see Hair::read.  SDF files: i32 dim[3], f32 min[3], f32 max[3],
 // f32 phi[n], f32 grad[3n].
@@ -16,8 +24,11 @@
 #include <fstream>
 #include <iostream>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
+#include <vector>
 
+#include "curve_io.h"
 #include "scr_solver.h"
 
 using namespace kami_amd;
@@ -64,6 +75,33 @@ void print_refs(const StableCosseratRodsSolver& s) {
     std::printf("], \"numVerts\": %d}\n", s.NumSimVertices());
 }
 
+// the used vertices of a Hair as a flat state: positions and strand offsets
+void flatten(const Hair& h, std::vector<float>& pos, std::vector<int>& off) {
+    off.assign(1, 0);
+    pos.clear();
+    for (uint32_t s = 0; s < h.num_strands; s++) {
+        for (uint32_t v = 0; v < h.strand_vert_count[s]; v++) {
+            const Vec3 p = h.get(h.current, s, v);
+            pos.insert(pos.end(), {p.x, p.y, p.z});
+        }
+        off.push_back(off.back() + (int)h.strand_vert_count[s]);
+    }
+}
+
+Transform parse_transform(const char* text) {
+    Transform t;
+    std::string str(text);
+    std::stringstream ss(str);
+    std::string item;
+    int k = 0;
+    while (std::getline(ss, item, ',')) {
+        if (k >= 16) throw std::runtime_error("transform: 16 numbers expected");
+        t.m[k++] = std::stof(item);
+    }
+    if (k != 16) throw std::runtime_error("transform: 16 numbers expected");
+    return t;
+}
+
 int usage() {
     std::fprintf(stderr,
                  "usage: kscr_solver_cli parse <json> | prep <json> <hair.in> <hair.out> | "
@@ -88,6 +126,35 @@ int main(int argc, char** argv) {
             s.PrepareCurves();
             hair->write(argv[4]);
             print_refs(s);
+            return 0;
+        }
+        if (cmd == "usc2hair" && (argc == 4 || argc == 6)) {
+            const uint32_t ms = argc == 6 ? (uint32_t)std::strtoul(argv[4], nullptr, 10) : 10000u;
+            const uint32_t ml = argc == 6 ? (uint32_t)std::strtoul(argv[5], nullptr, 10) : 100u;
+            import_usc_hairsalon(argv[2], ms, ml).write(argv[3]);
+            return 0;
+        }
+        if (cmd == "hair2usc" && argc == 4) {
+            export_usc_hairsalon(argv[3], Hair::read(argv[2]));
+            return 0;
+        }
+        if (cmd == "curves2hair" && (argc == 7 || argc == 8)) {
+            const Transform tf = argc == 8 ? parse_transform(argv[7]) : Transform{};
+            const Curves c = read_curves(argv[2], std::stof(argv[4]), tf, std::atoi(argv[6]) != 0, std::atoi(argv[5]) != 0);
+            hair_from_curves(c).write(argv[3]);
+            std::printf("{\"closed\": [");
+            for (size_t i = 0; i < c.closed.size(); i++) std::printf("%s%d", i ? ", " : "", (int)c.closed[i]);
+            std::printf("]}\n");
+            return 0;
+        }
+        if ((cmd == "hair2bcc" && argc == 5) || (cmd == "hair2obj" && argc == 4)) {
+            std::vector<float> pos;
+            std::vector<int> off;
+            flatten(Hair::read(argv[2]), pos, off);
+            if (cmd == "hair2bcc")
+                write_bcc(argv[3], pos, off, std::atoi(argv[4]) != 0);
+            else
+                write_obj(argv[3], pos, off);
             return 0;
         }
         if (cmd == "run" && argc == 8) {
