@@ -170,6 +170,31 @@ int kscr_set_sdf(kscr_handle* h, const int dim[3], const float grid_min[3], cons
                  const float* phi, const float* grad, float threshold, const float rotation[4],
                  const float translation[3]);
 int kscr_enable_sdf(kscr_handle* h, int enable);
+/* Body SDF grid from a closed triangle mesh, built on the device:
+ * PNMesh::GenerateSDF + PNMesh::StoreSDFQueries (src/PNMesh.cpp:100-171)
+ * through Kami::SDF::Init / CollisionTest / Distance
+ * (src/utilities/Geometry.cpp:5-59).  The mesh's signed distance (Discregrid
+ * TriangleMeshDistance: nearest point by Eberly's point-triangle distance,
+ * sign from angle-weighted pseudo-normals) fills a cubic Lagrange grid of dim
+ * cells over the vertex box padded by 0.1 (Discregrid
+ * CubicLagrangeDiscreteGrid); its interpolant is sampled at the nodes
+ * min + (i,j,k)*ds of the vertex box, ds = (max - min)/dim: phi, and grad =
+ * the normalized interpolant gradient where phi < 0, else the direction from
+ * the grid centre.  vertices float[3*num_vertices]; triangles
+ * int[3*num_triangles], counter-clockwise seen from outside; a triangle of
+ * zero area is KSCR_ERR_INVALID_ARGUMENT.  phi float[dim.x*dim.y*dim.z] and
+ * grad float[3*...] (x fastest; either may be NULL) and grid_min/grid_max
+ * (the vertex box; may be NULL) are written; they are kscr_set_sdf's inputs.
+ * device < 0 uses the current device.  err (may be NULL) receives a message
+ * on failure. */
+int kscr_mesh_sdf(int device, const float* vertices, int num_vertices, const int* triangles, int num_triangles,
+                  const int dim[3], float* phi, float* grad, float grid_min[3], float grid_max[3], char* err,
+                  size_t err_len);
+/* The same grid built straight into the handle's body SDF, followed by what
+ * kscr_set_sdf does with it (no host round trip). */
+int kscr_set_sdf_from_mesh(kscr_handle* h, const float* vertices, int num_vertices, const int* triangles,
+                           int num_triangles, const int dim[3], float threshold, const float rotation[4],
+                           const float translation[3]);
 
 /* Sim::advance (YB/sim/step.cpp:38-62): splits dt [s] into
  * S = max(1, ceil(dt/max_h)) substeps of numItr VBD iterations. */

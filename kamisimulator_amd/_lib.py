@@ -95,6 +95,9 @@ SIGNATURES = {
     "kscr_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
     "kscr_set_sdf": (C.c_int, [_P, _IP, _FP, _FP, _FP, _FP, C.c_float, _FP, _FP]),
     "kscr_enable_sdf": (C.c_int, [_P, C.c_int]),
+    "kscr_mesh_sdf": (C.c_int, [C.c_int, _FP, C.c_int, _IP, C.c_int, _IP, _FP, _FP, _FP, _FP, C.c_char_p,
+                                C.c_size_t]),
+    "kscr_set_sdf_from_mesh": (C.c_int, [_P, _FP, C.c_int, _IP, C.c_int, _IP, C.c_float, _FP, _FP]),
     "kscr_advance": (C.c_int, [_P, C.c_float]),
     "kscr_solve_next_step": (C.c_int, [_P, C.c_float]),
     "kscr_set_stream": (C.c_int, [_P, _P]),
@@ -179,6 +182,27 @@ def params_to_dict(p):
         val = getattr(p, name)
         d[name] = list(val) if name == "gravity" else val
     return d
+
+
+def mesh_sdf(vertices, triangles, dim, device=-1):
+    """Body SDF grid of a closed triangle mesh, built on the GPU
+    (kscr_mesh_sdf): a scenes.SdfGrid over the vertex box."""
+    from .scenes import SdfGrid
+    lib = load()
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    d = np.ascontiguousarray(dim, dtype=np.int32)
+    n = int(np.prod(d.astype(np.int64)))
+    phi = np.empty(n, np.float32)
+    grad = np.empty(3 * n, np.float32)
+    gmin = np.empty(3, np.float32)
+    gmax = np.empty(3, np.float32)
+    err = C.create_string_buffer(512)
+    rc = lib.kscr_mesh_sdf(int(device), fptr(v), len(v), iptr(t), len(t), iptr(d), fptr(phi), fptr(grad), fptr(gmin),
+                           fptr(gmax), err, 512)
+    if rc:
+        raise KscrError(rc, err.value.decode())
+    return SdfGrid(d, gmin, gmax, phi.reshape(d[2], d[1], d[0]), grad.reshape(d[2], d[1], d[0], 3))
 
 
 class Sim:
@@ -303,6 +327,18 @@ class Sim:
         tr = np.ascontiguousarray(translation, dtype=np.float32)
         self._check(self.lib.kscr_set_sdf(self.h, iptr(dim), fptr(gmin), fptr(gmax), fptr(phi), fptr(grad),
                                           threshold, fptr(rot), fptr(tr)))
+
+    def set_sdf_from_mesh(self, vertices, triangles, dim, threshold=0.0, rotation=(0, 0, 0, 1),
+                          translation=(0, 0, 0)):
+        """The body SDF grid built on the device from a closed triangle mesh
+        (kscr_set_sdf_from_mesh)."""
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+        d = np.ascontiguousarray(dim, dtype=np.int32)
+        rot = np.ascontiguousarray(rotation, dtype=np.float32)
+        tr = np.ascontiguousarray(translation, dtype=np.float32)
+        self._check(self.lib.kscr_set_sdf_from_mesh(self.h, fptr(v), len(v), iptr(t), len(t), iptr(d), threshold,
+                                                    fptr(rot), fptr(tr)))
 
     def enable_sdf(self, on=True):
         self._check(self.lib.kscr_enable_sdf(self.h, int(bool(on))))

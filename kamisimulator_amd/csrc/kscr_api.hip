@@ -868,24 +868,32 @@ int kscr_get_contacts(kscr_handle* h, int* num, int* cols) {
     return KSCR_OK;
 }
 
-int kscr_set_sdf(kscr_handle* h, const int dim[3], const float grid_min[3], const float grid_max[3], const float* phi,
-                 const float* grad, float threshold, const float rotation[4], const float translation[3]) {
-    if (!h || !dim || !grid_min || !grid_max || !phi || !grad) return KSCR_ERR_INVALID_ARGUMENT;
-    size_t cells = 1;
+// SdfArgs of a grid over [grid_min, grid_max] (SDFBodyCollisionSolver::InitParams, :148-159)
+static SdfArgs sdf_args(const int dim[3], const float grid_min[3], const float grid_max[3], float threshold,
+                        const float rotation[4], const float translation[3]) {
     SdfArgs s{};
     for (int a = 0; a < 3; a++) {
-        if (dim[a] <= 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "SDF dimensions must be positive");
         s.dim[a] = dim[a];
         s.gmin[a] = grid_min[a];
-        // SDFBodyCollisionSolver::InitParams (:148-159): float spacing and its inverse
         const float dist = grid_max[a] - grid_min[a];
         s.ds[a] = dist / (1.f * dim[a]);
         s.inv_ds[a] = 1.f / s.ds[a];
-        cells *= (size_t)dim[a];
     }
     s.thresh = threshold;
     for (int a = 0; a < 4; a++) s.rot[a] = rotation ? rotation[a] : (a == 3 ? 1.f : 0.f);
     for (int a = 0; a < 3; a++) s.trans[a] = translation ? translation[a] : 0.f;
+    return s;
+}
+
+int kscr_set_sdf(kscr_handle* h, const int dim[3], const float grid_min[3], const float grid_max[3], const float* phi,
+                 const float* grad, float threshold, const float rotation[4], const float translation[3]) {
+    if (!h || !dim || !grid_min || !grid_max || !phi || !grad) return KSCR_ERR_INVALID_ARGUMENT;
+    size_t cells = 1;
+    for (int a = 0; a < 3; a++) {
+        if (dim[a] <= 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "SDF dimensions must be positive");
+        cells *= (size_t)dim[a];
+    }
+    SdfArgs s = sdf_args(dim, grid_min, grid_max, threshold, rotation, translation);
     std::vector<float4> g(cells);
     for (size_t c = 0; c < cells; c++) g[c] = make_float4(grad[3 * c], grad[3 * c + 1], grad[3 * c + 2], phi[c]);
     hipError_t e;
@@ -895,6 +903,81 @@ int kscr_set_sdf(kscr_handle* h, const int dim[3], const float grid_min[3], cons
         hipSuccess)
         return h->hip_fail(e, "hipMemcpyAsync SDF grid");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    s.grid = h->d_grid.p;
+    h->sdf = s;
+    h->sdf_set = true;
+    h->sdf_on = true;
+    return KSCR_OK;
+}
+
+int kscr_mesh_sdf(int device, const float* vertices, int num_vertices, const int* triangles, int num_triangles,
+                  const int dim[3], float* phi, float* grad, float grid_min[3], float grid_max[3], char* err,
+                  size_t err_len) {
+    std::string msg;
+    auto finish = [&](int rc) {
+        if (rc != KSCR_OK && err && err_len) {
+            std::strncpy(err, msg.c_str(), err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        return rc;
+    };
+    if (!dim || dim[0] < 1 || dim[1] < 1 || dim[2] < 1) {
+        msg = "mesh SDF: grid dimensions must be positive";
+        return finish(KSCR_ERR_INVALID_ARGUMENT);
+    }
+    hipError_t e;
+    if (device >= 0 && (e = hipSetDevice(device)) != hipSuccess) {
+        msg = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return finish(KSCR_ERR_HIP);
+    }
+    const size_t cells = (size_t)dim[0] * dim[1] * dim[2];
+    DevBuf<float4> out;
+    if ((size_t)dim[0] * dim[1] * dim[2] > (size_t)INT32_MAX) {
+        msg = "mesh SDF: more than 2^31 - 1 grid nodes";
+        return finish(KSCR_ERR_INVALID_ARGUMENT);
+    }
+    if ((e = out.alloc(cells)) != hipSuccess) {
+        msg = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return finish(KSCR_ERR_OUT_OF_MEMORY);
+    }
+    float lo[3], hi[3];
+    int rc = kscr::mesh_sdf_build(vertices, num_vertices, triangles, num_triangles, dim, out.p, lo, hi, nullptr, msg);
+    if (rc != KSCR_OK) return finish(rc);
+    std::vector<float4> g(cells);
+    if ((e = hipMemcpy(g.data(), out.p, cells * sizeof(float4), hipMemcpyDeviceToHost)) != hipSuccess) {
+        msg = std::string("hipMemcpy: ") + hipGetErrorString(e);
+        return finish(KSCR_ERR_HIP);
+    }
+    for (size_t c = 0; c < cells; c++) {
+        if (phi) phi[c] = g[c].w;
+        if (grad) grad[3 * c] = g[c].x, grad[3 * c + 1] = g[c].y, grad[3 * c + 2] = g[c].z;
+    }
+    for (int a = 0; a < 3; a++) {
+        if (grid_min) grid_min[a] = lo[a];
+        if (grid_max) grid_max[a] = hi[a];
+    }
+    return finish(KSCR_OK);
+}
+
+int kscr_set_sdf_from_mesh(kscr_handle* h, const float* vertices, int num_vertices, const int* triangles,
+                           int num_triangles, const int dim[3], float threshold, const float rotation[4],
+                           const float translation[3]) {
+    if (!h || !dim || dim[0] < 1 || dim[1] < 1 || dim[2] < 1) return KSCR_ERR_INVALID_ARGUMENT;
+    const size_t cells = (size_t)dim[0] * dim[1] * dim[2];
+    if (cells > (size_t)INT32_MAX) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "more than 2^31 - 1 SDF grid nodes");
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = h->d_grid.alloc(cells)) != hipSuccess) return h->hip_fail(e, "hipMalloc SDF grid");
+    float lo[3], hi[3];
+    std::string msg;
+    const int rc = kscr::mesh_sdf_build(vertices, num_vertices, triangles, num_triangles, dim, h->d_grid.p, lo, hi,
+                                        h->stream, msg);
+    if (rc != KSCR_OK) {
+        h->sdf_set = false;
+        h->sdf_on = false;
+        return h->fail(rc, msg);
+    }
+    SdfArgs s = sdf_args(dim, lo, hi, threshold, rotation, translation);
     s.grid = h->d_grid.p;
     h->sdf = s;
     h->sdf_set = true;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 namespace kscr {
 
@@ -102,6 +103,12 @@ hipError_t launch_step_colored(const StepArgs& a, const int2* packs, int num_pac
 // q and q_rest are (x, y, z, w) rotors, 4 floats each.
 void sag_free_groom(const float* pos, const float* inv_mass, float* l_rest, const float* k_stretch, float* q,
                     float* q_rest, const float gravity[3], const int* roots, int num_roots, int num_verts, int iterations);
+// Body SDF grid of a triangle mesh (kscr_meshsdf.hip): the (grad, phi) float4
+// grid of dim[0]*dim[1]*dim[2] nodes into the device array out, on stream;
+// gmin / gmax receive the vertex box the grid spans.  Returns a kscr_status,
+// with a message in err on failure; synchronises the stream.
+int mesh_sdf_build(const float* verts, int nv, const int* tris, int nt, const int dim[3], float4* out, float gmin[3],
+                   float gmax[3], hipStream_t stream, std::string& err);
 // Same step, writing the tape of the step (contact-free step, no SDF pass).
 hipError_t launch_step_record(const StepArgs& a, const TapeArgs& tape, const int2* packs, int num_packs, int block,
                               hipStream_t stream);

@@ -357,4 +357,36 @@ Hair hair_from_curves(const Curves& c) {
     return h;
 }
 
+void read_obj_mesh(const std::string& path, std::vector<float>& vertices, std::vector<int>& triangles) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot open mesh file " + path);
+    vertices.clear();
+    triangles.clear();
+    std::string line;
+    std::vector<int> face;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string tag;
+        if (!(ls >> tag)) continue;
+        if (tag == "v") {
+            float x, y, z;
+            if (!(ls >> x >> y >> z)) throw std::runtime_error("mesh file " + path + ": bad vertex line");
+            vertices.insert(vertices.end(), {x, y, z});
+        } else if (tag == "f") {
+            face.clear();
+            std::string item;
+            const long nv = (long)vertices.size() / 3;
+            while (ls >> item) {
+                const long k = std::strtol(item.c_str(), nullptr, 10);  // the position index before any '/'
+                const long idx = k > 0 ? k - 1 : nv + k;
+                if (k == 0 || idx < 0 || idx >= nv) throw std::runtime_error("mesh file " + path + ": bad face index");
+                face.push_back((int)idx);
+            }
+            if (face.size() < 3) throw std::runtime_error("mesh file " + path + ": face with fewer than 3 corners");
+            for (size_t c = 1; c + 1 < face.size(); c++) triangles.insert(triangles.end(), {face[0], face[c], face[c + 1]});
+        }
+    }
+    if (triangles.empty()) throw std::runtime_error("mesh file " + path + ": no faces");
+}
+
 }  // namespace kami_amd

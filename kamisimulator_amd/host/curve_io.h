@@ -60,6 +60,12 @@ void write_bcc(const std::string& path, const std::vector<float>& positions, con
                bool as_polyline);
 void write_obj(const std::string& path, const std::vector<float>& positions, const std::vector<int>& strand_offsets);
 
+// Triangle mesh of an OBJ file ("v x y z" and "f a b c ..." with 1-based or
+// negative indices, "a/b/c" forms allowed): polygons split into fans from
+// their first corner, as assimp's aiProcess_Triangulate does for the convex
+// faces of body meshes (PNMesh loads with it, src/PNMesh.cpp:14).
+void read_obj_mesh(const std::string& path, std::vector<float>& vertices, std::vector<int>& triangles);
+
 // Curves into the padded Hair layout (hairSize.y = the longest curve).
 Hair hair_from_curves(const Curves& c);
 

@@ -95,10 +95,29 @@ void StableCosseratRodsSolver::SubstituteBodySdf(const BodySdf& sdf) {
     const size_t nodes = (size_t)sdf.dim[0] * sdf.dim[1] * sdf.dim[2];
     if (sdf.phi.size() != nodes || sdf.grad.size() != 3 * nodes) throw std::runtime_error("SDF grid size mismatch");
     sdf_.reset(new BodySdf(sdf));
-    if (sim_)
+    mesh_.reset();
+    if (sim_) apply_body();
+}
+
+void StableCosseratRodsSolver::SubstituteBodyMesh(const BodyMesh& mesh) {
+    if (mesh.vertices.size() % 3 || mesh.triangles.size() % 3 || mesh.triangles.empty())
+        throw std::runtime_error("body mesh: xyz per vertex and three indices per triangle expected");
+    mesh_.reset(new BodyMesh(mesh));
+    sdf_.reset();
+    if (sim_) apply_body();
+}
+
+// hands the body to the simulation: a grid as it is, a mesh built on the device
+void StableCosseratRodsSolver::apply_body() {
+    if (sdf_)
         check(kscr_set_sdf(sim_, sdf_->dim, sdf_->grid_min, sdf_->grid_max, sdf_->phi.data(), sdf_->grad.data(),
                            jp_.SDFThreshold, sdf_->rotation, sdf_->translation),
               "kscr_set_sdf");
+    else if (mesh_)
+        check(kscr_set_sdf_from_mesh(sim_, mesh_->vertices.data(), (int)mesh_->vertices.size() / 3,
+                                     mesh_->triangles.data(), (int)mesh_->triangles.size() / 3, mesh_->dim,
+                                     jp_.SDFThreshold, mesh_->rotation, mesh_->translation),
+              "kscr_set_sdf_from_mesh");
 }
 
 // StableCosseratRodsSolver::BuildYarnBallHair (StableCosseratRodsSolver.cpp:112-173),
@@ -185,10 +204,7 @@ void StableCosseratRodsSolver::build_sim() {
     if (jp_.SagFreeItr > 0)
         check(kscr_sag_free(sim_, root_vertices_.data(), (int)root_vertices_.size(), jp_.SagFreeItr), "SagFree");
     check(kscr_upload(sim_), "upload");
-    if (sdf_)
-        check(kscr_set_sdf(sim_, sdf_->dim, sdf_->grid_min, sdf_->grid_max, sdf_->phi.data(), sdf_->grad.data(),
-                           jp_.SDFThreshold, sdf_->rotation, sdf_->translation),
-              "kscr_set_sdf");
+    apply_body();
 }
 
 // StableCosseratRodsSolver::Construct (StableCosseratRodsSolver.cpp:33-61)
@@ -215,7 +231,9 @@ void StableCosseratRodsSolver::Construct(const std::string& settingJsonPath) {
 void StableCosseratRodsSolver::SolveNextStep(float dt) {
     if (!sim_) throw std::runtime_error("Construct() has not been called");
     if (jp_.EnableSDF) {
-        if (!sdf_) throw std::runtime_error("EnableSDF is set but no body SDF was substituted (SubstituteBodySdf)");
+        if (!sdf_ && !mesh_)
+            throw std::runtime_error(
+                "EnableSDF is set but no body was substituted (SubstituteBodySdf / SubstituteBodyMesh)");
         check(kscr_solve_next_step(sim_, dt), "solve_next_step");
     } else {
         check(kscr_advance(sim_, dt), "advance");

@@ -9,7 +9,8 @@
 // and strandRefs).  Simulation calls go through the C ABI of include/kscr.h.
 //
 // Differences, all additive: the body SDF is handed over as a sampled grid
-// (SubstituteBodySdf) instead of being built from the body mesh; SolveNextStep
+// (SubstituteBodySdf) or as a triangle mesh whose grid the library builds on
+// the device (SubstituteBodyMesh) instead of a Body object; SolveNextStep
 // runs the SDF pass on the device instead of the host round trip; the
 // settings accept Gravity, SDFThreshold, MaxSubstep, Schedule, BlockSize and
 // DeviceResidentHair (SolveNextStep then leaves the Hair to DownloadHair,
@@ -32,6 +33,17 @@ struct BodySdf {
     float grid_min[3] = {0, 0, 0}, grid_max[3] = {0, 0, 0};
     std::vector<float> phi;   // dim.x * dim.y * dim.z, x fastest
     std::vector<float> grad;  // 3 per node
+    float rotation[4] = {0, 0, 0, 1};
+    float translation[3] = {0, 0, 0};
+};
+
+// A closed triangle body mesh whose SDF grid the library builds on the device
+// (PNMesh::GenerateSDF + StoreSDFQueries, src/PNMesh.cpp:100-171); dim is
+// the grid resolution (PNMESH_DEFAULT_SDF_SIZE, src/PNMesh.cpp:18).
+struct BodyMesh {
+    std::vector<float> vertices;  // xyz per vertex
+    std::vector<int> triangles;   // three vertex indices per triangle
+    int dim[3] = {64, 128, 64};
     float rotation[4] = {0, 0, 0, 1};
     float translation[3] = {0, 0, 0};
 };
@@ -81,6 +93,9 @@ class StableCosseratRodsSolver {
     // Solver::SubstituteHair / SubstituteBody
     void SubstituteHair(std::shared_ptr<Hair> hair) { hair_ = std::move(hair); }
     void SubstituteBodySdf(const BodySdf& sdf);
+    // The body as a mesh: its SDF grid is built on the device (replaces a
+    // substituted grid, and the other way round).
+    void SubstituteBodyMesh(const BodyMesh& mesh);
 
     // Solver interface
     void Construct(const std::string& settingJsonPath = "");
@@ -116,6 +131,8 @@ class StableCosseratRodsSolver {
     std::string json_dump_;
     kscr_handle* sim_ = nullptr;
     std::unique_ptr<BodySdf> sdf_;
+    std::unique_ptr<BodyMesh> mesh_;
+    void apply_body();
     std::vector<int> root_vertices_, strand_refs_;
     std::vector<float> flat_pos_, flat_vel_;
     std::vector<int> strand_offsets_;

@@ -6,8 +6,12 @@
 //   kscr_solver_cli prep <settings.json> <hair.in> <hair.out>
 //       curve preparation of Construct only (no GPU); prints rootVertices and
 //       strandRefs as JSON
-//   kscr_solver_cli run <settings.json> <hair.in> <sdf.bin|-> <steps> <dt> <hair.out>
-//       Construct, then `steps` SolveNextStep(dt); writes the Hair
+//   kscr_solver_cli run <settings.json> <hair.in> <sdf.bin|mesh.obj|-> <steps> <dt> <hair.out>
+//       Construct, then `steps` SolveNextStep(dt); writes the Hair.  A .obj
+//       body is a triangle mesh whose SDF grid (64 x 128 x 64) is built on
+//       the device
+//   kscr_solver_cli mesh2sdf <mesh.obj> <dim_x> <dim_y> <dim_z> <sdf.out>
+//       the body SDF grid of a triangle mesh, built on the device
 //   kscr_solver_cli usc2hair <in.data> <hair.out> [max_strands max_length]
 //   kscr_solver_cli hair2usc <hair.in> <name>            (writes name.data)
 //   kscr_solver_cli curves2hair <in.bcc|.obj|.poly> <hair.out> <target_seg_len> <resample 0|1>
@@ -19,6 +23,7 @@
 //
 // Hair files: see Hair::read.  SDF files: i32 dim[3], f32 min[3], f32 max[3],
 // f32 phi[n], f32 grad[3n].
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -49,6 +54,21 @@ BodySdf read_sdf(const std::string& path) {
     in.read((char*)s.grad.data(), 12 * (std::streamsize)n);
     if (!in) throw std::runtime_error("short SDF file " + path);
     return s;
+}
+
+void write_sdf(const std::string& path, const BodySdf& s) {
+    std::ofstream out(path, std::ios::binary);
+    if (!out) throw std::runtime_error("cannot write SDF file " + path);
+    out.write((const char*)s.dim, sizeof s.dim);
+    out.write((const char*)s.grid_min, sizeof s.grid_min);
+    out.write((const char*)s.grid_max, sizeof s.grid_max);
+    out.write((const char*)s.phi.data(), 4 * (std::streamsize)s.phi.size());
+    out.write((const char*)s.grad.data(), 4 * (std::streamsize)s.grad.size());
+    if (!out) throw std::runtime_error("cannot write SDF file " + path);
+}
+
+bool ends_with(const std::string& s, const std::string& tail) {
+    return s.size() >= tail.size() && s.compare(s.size() - tail.size(), tail.size(), tail) == 0;
 }
 
 void print_params(const StableCosseratRodsSolver::JsonParams& p) {
@@ -105,7 +125,8 @@ Transform parse_transform(const char* text) {
 int usage() {
     std::fprintf(stderr,
                  "usage: kscr_solver_cli parse <json> | prep <json> <hair.in> <hair.out> | "
-                 "run <json> <hair.in> <sdf.bin|-> <steps> <dt> <hair.out>\n");
+                 "run <json> <hair.in> <sdf.bin|mesh.obj|-> <steps> <dt> <hair.out> | "
+                 "mesh2sdf <mesh.obj> <dx> <dy> <dz> <sdf.out> | (file conversions: see the source)\n");
     return 2;
 }
 
@@ -157,10 +178,33 @@ int main(int argc, char** argv) {
                 write_obj(argv[3], pos, off);
             return 0;
         }
+        if (cmd == "mesh2sdf" && argc == 7) {
+            std::vector<float> v;
+            std::vector<int> t;
+            read_obj_mesh(argv[2], v, t);
+            BodySdf g;
+            for (int a = 0; a < 3; a++) g.dim[a] = std::atoi(argv[3 + a]);
+            const size_t n = (size_t)std::max(g.dim[0], 0) * std::max(g.dim[1], 0) * std::max(g.dim[2], 0);
+            g.phi.resize(n);
+            g.grad.resize(3 * n);
+            char err[512] = {0};
+            const int rc = kscr_mesh_sdf(-1, v.data(), (int)v.size() / 3, t.data(), (int)t.size() / 3, g.dim,
+                                         g.phi.data(), g.grad.data(), g.grid_min, g.grid_max, err, sizeof err);
+            if (rc != KSCR_OK) throw std::runtime_error(std::string("kscr_mesh_sdf: ") + err);
+            write_sdf(argv[6], g);
+            return 0;
+        }
         if (cmd == "run" && argc == 8) {
             auto hair = std::make_shared<Hair>(Hair::read(argv[3]));
             StableCosseratRodsSolver s(hair);
-            if (std::string(argv[4]) != "-") s.SubstituteBodySdf(read_sdf(argv[4]));
+            const std::string body = argv[4];
+            if (ends_with(body, ".obj") || ends_with(body, ".OBJ")) {
+                BodyMesh m;
+                read_obj_mesh(body, m.vertices, m.triangles);
+                s.SubstituteBodyMesh(m);
+            } else if (body != "-") {
+                s.SubstituteBodySdf(read_sdf(body));
+            }
             s.Construct(argv[2]);
             const int steps = std::atoi(argv[5]);
             const float dt = (float)std::atof(argv[6]);

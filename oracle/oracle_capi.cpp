@@ -8,6 +8,7 @@
 #include <memory>
 
 #include "adjoint_oracle.hpp"
+#include "meshsdf_oracle.hpp"
 #include "scr_oracle.hpp"
 
 namespace {
@@ -224,6 +225,64 @@ int oracle_fix_orientation(oracle_handle* h, const int* ids, int n) {
 }
 int oracle_sag_free(oracle_handle* h, const int* roots, int n, int iterations) {
     return h->b->sag_free(roots, n, iterations);
+}
+// body SDF grid of a triangle mesh (meshsdf_oracle.hpp): phi[dim], grad[3 dim]
+// on the vertex box (written to gmin/gmax); coef (may be NULL) receives the
+// cubic grid's node values if it holds max_coef of them.  Returns the node count.
+long oracle_mesh_sdf(const float* verts, int nv, const int* tris, int nt, const int* dim, float* phi, float* grad,
+                     float* gmin, float* gmax, double* coef, long max_coef) {
+    orc::BodySdfOracle b;
+    b.build(verts, nv, tris, nt, dim);
+    b.sample(dim, phi, grad);
+    for (int a = 0; a < 3; a++) gmin[a] = b.gmin[a], gmax[a] = b.gmax[a];
+    const long n = (long)b.grid.coef.size();
+    if (coef && max_coef >= n) std::memcpy(coef, b.grid.coef.data(), sizeof(double) * n);
+    return n;
+}
+// signed distance, nearest point and nearest triangle of single points (double)
+int oracle_mesh_distance(const float* verts, int nv, const int* tris, int nt, const double* p, int np, double* dist,
+                         double* nearest, int* tri) {
+    orc::MeshSdfOracle m;
+    m.setup(verts, nv, tris, nt);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < np; i++) {
+        orc::D3 c{0, 0, 0};
+        dist[i] = m.signed_distance(orc::D3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}, c, tri + i);
+        nearest[3 * i] = c.x, nearest[3 * i + 1] = c.y, nearest[3 * i + 2] = c.z;
+    }
+    return 0;
+}
+// the cubic cell's shape functions at xi in [-1, 1]^3: N[32], dN[32][3]
+void oracle_cubic_shape(const double* xi, double* N, double* dN) {
+    orc::CubicGridOracle::shape(xi, N, reinterpret_cast<double(*)[3]>(dN));
+}
+// node positions of the cubic grid over the padded vertex box: pos[3 n]
+long oracle_cubic_nodes(const float* verts, int nv, const int* dim, double* pos, long max_nodes) {
+    std::vector<orc::D3> v(nv);
+    for (int i = 0; i < nv; i++) v[i] = orc::D3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+    orc::CubicGridOracle g;
+    const unsigned res[3] = {(unsigned)dim[0], (unsigned)dim[1], (unsigned)dim[2]};
+    g.setup(v, res);
+    const long n = (long)g.num_nodes();
+    if (pos && max_nodes >= n)
+        for (long l = 0; l < n; l++) {
+            const orc::D3 x = g.node_position((size_t)l);
+            pos[3 * l] = x.x, pos[3 * l + 1] = x.y, pos[3 * l + 2] = x.z;
+        }
+    return n;
+}
+// the cubic grid over the padded vertex box with the given node values,
+// interpolated at points: value[np], grad[3 np]
+int oracle_cubic_interpolate(const float* verts, int nv, const int* dim, const double* coef, const double* p, int np,
+                             double* value, double* grad) {
+    std::vector<orc::D3> v(nv);
+    for (int i = 0; i < nv; i++) v[i] = orc::D3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+    orc::CubicGridOracle g;
+    const unsigned res[3] = {(unsigned)dim[0], (unsigned)dim[1], (unsigned)dim[2]};
+    g.setup(v, res);
+    g.coef.assign(coef, coef + g.num_nodes());
+    for (int i = 0; i < np; i++) value[i] = g.interpolate(p + 3 * i, grad + 3 * i);
+    return 0;
 }
 int oracle_set_params(oracle_handle* h, const void* params) {
     orc::Params p;

@@ -47,6 +47,11 @@ SIGNATURES = {
     "oracle_rotor_mul": (None, [_FP, _FP, _FP]),
     "oracle_wind_cos": (None, [_DP, _DP, C.c_long]),
     "oracle_contact_pair": (None, [C.c_void_p, _DP, C.c_double, _DP]),
+    "oracle_mesh_sdf": (C.c_long, [_FP, C.c_int, _IP, C.c_int, _IP, _FP, _FP, _FP, _FP, _DP, C.c_long]),
+    "oracle_mesh_distance": (C.c_int, [_FP, C.c_int, _IP, C.c_int, _DP, C.c_int, _DP, _DP, _IP]),
+    "oracle_cubic_shape": (None, [_DP, _DP, _DP]),
+    "oracle_cubic_nodes": (C.c_long, [_FP, C.c_int, _IP, _DP, C.c_long]),
+    "oracle_cubic_interpolate": (C.c_int, [_FP, C.c_int, _IP, _DP, _DP, C.c_int, _DP, _DP]),
     "oracle_det_logf": (None, [_FP, _FP, C.c_long]),
     "oracle_detect_only": (None, [C.c_void_p, C.c_float]),
     "oracle_max_step": (None, [C.c_void_p, _FP]),
@@ -353,6 +358,77 @@ def segment_closest_points(a0, a1, b0, b1):
 
 def morton3(x, y, z):
     return load().oracle_morton3(int(x), int(y), int(z))
+
+
+def mesh_sdf(verts, tris, dim, with_coef=False):
+    """The body SDF grid of a triangle mesh (meshsdf_oracle.hpp): phi[dz, dy, dx],
+    grad[dz, dy, dx, 3], the grid box (gmin, gmax) and, with with_coef, the
+    cubic grid's node values."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.int32)
+    d = np.ascontiguousarray(dim, np.int32)
+    n = int(np.prod(d))
+    phi = np.empty(n, np.float32)
+    grad = np.empty(3 * n, np.float32)
+    gmin = np.empty(3, np.float32)
+    gmax = np.empty(3, np.float32)
+    nn = lib.oracle_cubic_nodes(_f(v), len(v), _i(d), None, 0)
+    coef = np.empty(nn if with_coef else 0, np.float64)
+    lib.oracle_mesh_sdf(_f(v), len(v), _i(t), len(t), _i(d), _f(phi), _f(grad), _f(gmin), _f(gmax),
+                        coef.ctypes.data_as(_DP) if with_coef else None, nn if with_coef else 0)
+    out = (phi.reshape(d[2], d[1], d[0]), grad.reshape(d[2], d[1], d[0], 3), gmin, gmax)
+    return out + (coef,) if with_coef else out
+
+
+def mesh_distance(verts, tris, points):
+    """Signed distance (double), nearest point and nearest triangle of each point."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.int32)
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    dist = np.empty(len(p), np.float64)
+    near = np.empty((len(p), 3), np.float64)
+    tri = np.empty(len(p), np.int32)
+    lib.oracle_mesh_distance(_f(v), len(v), _i(t), len(t), p.ctypes.data_as(_DP), len(p), dist.ctypes.data_as(_DP),
+                             near.ctypes.data_as(_DP), _i(tri))
+    return dist, near, tri
+
+
+def cubic_shape(xi):
+    """Shape functions N[32] and derivatives dN[32, 3] of the cubic grid cell."""
+    lib = load()
+    x = np.ascontiguousarray(xi, np.float64)
+    N = np.empty(32, np.float64)
+    dN = np.empty((32, 3), np.float64)
+    lib.oracle_cubic_shape(x.ctypes.data_as(_DP), N.ctypes.data_as(_DP), dN.ctypes.data_as(_DP))
+    return N, dN
+
+
+def cubic_interpolate(verts, dim, coef, points):
+    """The cubic grid with node values coef, interpolated: value[n], grad[n, 3]."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float32)
+    d = np.ascontiguousarray(dim, np.int32)
+    c = np.ascontiguousarray(coef, np.float64)
+    assert len(c) == lib.oracle_cubic_nodes(_f(v), len(v), _i(d), None, 0)
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    val = np.empty(len(p), np.float64)
+    grad = np.empty((len(p), 3), np.float64)
+    lib.oracle_cubic_interpolate(_f(v), len(v), _i(d), c.ctypes.data_as(_DP), p.ctypes.data_as(_DP), len(p),
+                                 val.ctypes.data_as(_DP), grad.ctypes.data_as(_DP))
+    return val, grad
+
+
+def cubic_nodes(verts, dim):
+    """Node positions [n, 3] of the cubic grid over the padded vertex box."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float32)
+    d = np.ascontiguousarray(dim, np.int32)
+    n = lib.oracle_cubic_nodes(_f(v), len(v), _i(d), None, 0)
+    pos = np.empty((n, 3), np.float64)
+    lib.oracle_cubic_nodes(_f(v), len(v), _i(d), pos.ctypes.data_as(_DP), n)
+    return pos
 
 
 def contact_pair(params, xi, xj, dxi, dxj, b0, b1, db0, db1, damp=0.0):
