@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-batched-leg", action="store_true", help="skip the batched-vs-per-groom timing")
     ap.add_argument("--no-contacts-leg", action="store_true", help="skip the large groom with contacts")
     ap.add_argument("--no-colored-leg", action="store_true", help="skip the even/odd coloured schedule timing")
+    ap.add_argument("--no-mesh-sdf-leg", action="store_true", help="skip the mesh -> body SDF grid build timing")
+    ap.add_argument("--mesh-sdf-only", action="store_true", help="run the mesh -> body SDF grid leg only")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--only-legs", action="store_true", help="run the adjoint and batched legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -342,6 +344,51 @@ def colored_leg(stream, scene, dt, updates_per_step, nsteps):
     return out
 
 
+def mesh_sdf_leg(reps=5, cpu=True):
+    """Body SDF grid of an 81,920-triangle sphere mesh on the reference's
+    default grid (64 x 128 x 64, PNMesh.cpp:18), built on the GPU
+    (kscr_mesh_sdf: host pseudo-normals, triangle BVH, cubic grid node
+    distances, body grid sampling, copy back); the CPU figure is the
+    reference's own construction (Kami::SDF + Discregrid, built from the
+    reference sources into oracle/_ref/discregrid_probe, OpenMP) on the same
+    mesh and grid, when that probe is present."""
+    from kamisimulator_amd import mesh_sdf, scenes
+
+    v, t = scenes.sphere_mesh(6, radius=0.1)
+    dim = (64, 128, 64)
+    mesh_sdf(v, t, dim, device=0)
+    ws = []
+    for _ in range(reps):
+        w0 = time.perf_counter()
+        mesh_sdf(v, t, dim, device=0)
+        ws.append((time.perf_counter() - w0) * 1e3)
+    n = [int(x) for x in dim]
+    cubic_nodes = (n[0] + 1) * (n[1] + 1) * (n[2] + 1) + 2 * (n[0] * (n[1] + 1) * (n[2] + 1) +
+                                                              (n[0] + 1) * n[1] * (n[2] + 1) +
+                                                              (n[0] + 1) * (n[1] + 1) * n[2])
+    out = {"triangles": int(len(t)), "grid": list(dim), "cubic_grid_nodes": cubic_nodes,
+           "gpu_ms_per_build": round(float(np.median(ws)), 2)}
+    probe = os.path.join(REPO, "oracle", "_ref", "discregrid_probe")
+    if cpu and os.path.exists(probe):
+        import tempfile
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        with tempfile.TemporaryDirectory() as td:
+            src, dst = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+            with open(src, "wb") as f:
+                f.write(np.array([len(v), len(t), *dim, 0], np.int32).tobytes())
+                f.write(v.tobytes())
+                f.write(t.tobytes())
+            w0 = time.perf_counter()
+            subprocess.run([probe, src, dst], check=True, stdout=subprocess.DEVNULL)
+            cpu_ms = (time.perf_counter() - w0) * 1e3
+        out["cpu_reference"] = {"ms_per_build": round(cpu_ms, 1), "cores": threads, "kind": "reference",
+                                "sample": "the whole build: Kami::SDF::Init (TriangleMeshDistance + "
+                                          "CubicLagrangeDiscreteGrid::addFunction) and the StoreSDFQueries node "
+                                          "loop, OpenMP"}
+        out["speedup_vs_cpu_reference"] = round(cpu_ms / out["gpu_ms_per_build"], 1)
+    return out
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -361,6 +408,9 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
+    if args.mesh_sdf_only:
+        print(json.dumps({"mesh_sdf": mesh_sdf_leg(cpu=not args.no_cpu)}), flush=True)
+        return
     if args.only_legs:
         legs = {}
         if not args.no_adjoint_leg:
@@ -489,6 +539,11 @@ def main():
         contacts = contacts_leg(stream)
         print(f"[bench] contacts leg: {contacts}", file=sys.stderr, flush=True)
 
+    mesh = None
+    if not args.no_mesh_sdf_leg:
+        mesh = mesh_sdf_leg(cpu=not args.no_cpu)
+        print(f"[bench] mesh sdf leg: {mesh}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, params, args.cpu_seconds)
@@ -524,6 +579,7 @@ def main():
         "adjoint": adjoint,
         "batched": batched,
         "contacts": contacts,
+        "mesh_sdf": mesh,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

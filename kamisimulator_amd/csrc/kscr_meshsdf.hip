@@ -17,13 +17,23 @@
 // (apart from the choice among triangles at exactly equal distance, which
 // leaves the distance unchanged).
 //
-//   node_distance_kernel  one lane per cubic grid node: nearest triangle by a
+//   node_distance_kernel  one lane per cubic grid node of the cells the body
+//                         grid samples (the nodes of the other cells, in the
+//                         0.1 padding, are never read): nearest triangle by a
 //                         depth-first search of a linear BVH over the
 //                         triangle boxes (nearer child first, children and
 //                         stacked subtrees pruned by their box distance), then
 //                         the pseudo-normal sign.  Ties of the squared
 //                         distance go to the smaller triangle index, so the
-//                         result does not depend on the traversal order.
+//                         result does not depend on the traversal order.  A
+//                         lane whose search passes 1024 leaf tests (near the
+//                         medial axis nearly every triangle is a candidate;
+//                         at a sphere's centre all are) lists its node for
+//                         the next kernel instead of holding its wave.
+//   node_brute_kernel     one workgroup per listed node: every triangle,
+//                         split over the lanes, then a reduction to the
+//                         smallest (squared distance, index) -- the same
+//                         minimum.
 //   sample_kernel         one lane per body grid node: the cell's 32 node
 //                         values against the shape functions, written as the
 //                         (grad, phi) float4 grid the SDF pass reads.
@@ -40,7 +50,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
-#include <unordered_map>
+#include <thread>
 #include <vector>
 
 #include "kscr.h"
@@ -186,6 +196,41 @@ __device__ inline Dbl3 node_position(const CubicGrid& g, unsigned long long l) {
     return Dbl3{x[0], x[1], x[2]};
 }
 
+// The cells the body grid samples: a box of cells [c0, c0 + m) (every body
+// node's cell lies in it; the cubic grid's padding cells are never read).
+// Its nodes are enumerated compactly -- corners, x-edge, y-edge, z-edge
+// nodes, each x fastest -- and mapped to their ids in the full layout.
+struct SubGrid {
+    unsigned c0[3], m[3];
+    unsigned long long nc, nx, ny, nz;  // node counts of each kind in the box
+    __host__ __device__ unsigned long long count() const { return nc + nx + ny + nz; }
+};
+
+__device__ inline unsigned long long sub_node_id(const CubicGrid& g, const SubGrid& s, unsigned long long q) {
+    const unsigned long long n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    unsigned long long e[3];  // extent of the index box of this kind, per axis
+    int kind;
+    if (q < s.nc) {
+        kind = 0;
+        for (int a = 0; a < 3; a++) e[a] = s.m[a] + 1ull;
+    } else {
+        q -= s.nc;
+        kind = 1;
+        if (q >= s.nx) q -= s.nx, kind = 2;
+        if (kind == 2 && q >= s.ny) q -= s.ny, kind = 3;
+        for (int a = 0; a < 3; a++) e[a] = s.m[a] + (a == kind - 1 ? 0ull : 1ull);
+    }
+    const unsigned long long h = kind ? q % 2 : 0;
+    if (kind) q /= 2;
+    const unsigned long long i = s.c0[0] + q % e[0], j = s.c0[1] + (q / e[0]) % e[1], k = s.c0[2] + q / (e[0] * e[1]);
+    switch (kind) {
+        case 0: return (n0 + 1) * (n1 + 1) * k + (n0 + 1) * j + i;
+        case 1: return g.nv + 2 * (n0 * (n1 + 1) * k + n0 * j + i) + h;
+        case 2: return g.nv + 2 * g.ne[0] + 2 * (n1 * (n2 + 1) * i + n1 * k + j) + h;
+        default: return g.nv + 2 * (g.ne[0] + g.ne[1]) + 2 * (n2 * (n0 + 1) * j + n2 * i + k) + h;
+    }
+}
+
 struct MeshDev {
     const float4* tri;  // three vertices per triangle (w unused)
     const double* pn;   // kNormalSlots pseudo-normals (xyz) per triangle
@@ -210,23 +255,29 @@ constexpr uint32_t kLeafBit = 0x80000000u, kNoNode = 0xFFFFFFFFu;
 template <int STACK>
 __global__ __launch_bounds__(kDistBlock) void node_distance_kernel(MeshDev m, const LbvhNode* __restrict__ nodes,
                                                                    const uint32_t* __restrict__ ids, CubicGrid g,
-                                                                   double slack, double* __restrict__ coef,
+                                                                   SubGrid sub, double slack, int budget,
+                                                                   double* __restrict__ coef,
+                                                                   unsigned long long* __restrict__ overflow,
+                                                                   unsigned* __restrict__ overflow_count,
                                                                    int* __restrict__ error) {
     __shared__ uint32_t st_node[STACK][kDistBlock];
     __shared__ float st_d2[STACK][kDistBlock];
-    const unsigned long long l = (unsigned long long)blockIdx.x * kDistBlock + threadIdx.x;
-    if (l >= g.num_nodes()) return;
+    const unsigned long long q = (unsigned long long)blockIdx.x * kDistBlock + threadIdx.x;
+    if (q >= sub.count()) return;
+    const unsigned long long l = sub_node_id(g, sub, q);
     const int lane = threadIdx.x;
     const Dbl3 p = node_position(g, l);
     double best = INFINITY;
     int best_k = 0x7fffffff, best_f = kFace;
     Dbl3 best_q = p;
+    int tests = 0;
     auto test = [&](int k) {
         const float4 a = m.tri[3 * k], b = m.tri[3 * k + 1], c = m.tri[3 * k + 2];
         Dbl3 q;
         int f;
         const double d2 = point_triangle_d2(p, Dbl3{a.x, a.y, a.z}, Dbl3{b.x, b.y, b.z}, Dbl3{c.x, c.y, c.z}, q, f);
         if (d2 < best || (d2 == best && k < best_k)) best = d2, best_k = k, best_f = f, best_q = q;
+        tests++;
     };
     if (m.nt == 1) {
         test(0);
@@ -273,6 +324,12 @@ __global__ __launch_bounds__(kDistBlock) void node_distance_kernel(MeshDev m, co
                 if ((double)st_d2[sp][lane] <= best + slack) next = st_node[sp][lane];
             }
             if (next == kNoNode) break;
+            if (tests > budget) {
+                // near the medial axis almost every triangle is a candidate:
+                // left to node_brute_kernel, which splits it over a workgroup
+                overflow[atomicAdd(overflow_count, 1u)] = l;
+                return;
+            }
             node = next;
         }
     }
@@ -285,6 +342,58 @@ __global__ __launch_bounds__(kDistBlock) void node_distance_kernel(MeshDev m, co
     const Dbl3 u = p - best_q;
     const double sign = (u.x * n[0] + u.y * n[1] + u.z * n[2]) >= 0.0 ? 1.0 : -1.0;
     coef[l] = sign * sqrt(best);
+}
+
+// The nodes whose search exceeded its budget: one workgroup per node tests
+// every triangle (a strided share per lane), then reduces to the smallest
+// (squared distance, triangle index), the same minimum the search returns.
+constexpr int kBruteBlock = 256;
+__global__ __launch_bounds__(kBruteBlock) void node_brute_kernel(MeshDev m, CubicGrid g,
+                                                                 const unsigned long long* __restrict__ list,
+                                                                 const unsigned* __restrict__ count,
+                                                                 double* __restrict__ coef) {
+    __shared__ double s_d2[kBruteBlock];
+    __shared__ int s_k[kBruteBlock];
+    const unsigned n = *count;
+    const int tid = threadIdx.x;
+    auto tri_d2 = [&](int k, Dbl3 p, Dbl3& q, int& f) {
+        const float4 a = m.tri[3 * k], b = m.tri[3 * k + 1], c = m.tri[3 * k + 2];
+        return point_triangle_d2(p, Dbl3{a.x, a.y, a.z}, Dbl3{b.x, b.y, b.z}, Dbl3{c.x, c.y, c.z}, q, f);
+    };
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {  // n is the same for every lane
+        const unsigned long long l = list[i];
+        const Dbl3 p = node_position(g, l);
+        double best = INFINITY;
+        int best_k = 0x7fffffff;
+        for (int k = tid; k < m.nt; k += kBruteBlock) {
+            Dbl3 q;
+            int f;
+            const double d2 = tri_d2(k, p, q, f);
+            if (d2 < best) best = d2, best_k = k;  // k increases: the first of equals is the smallest
+        }
+        s_d2[tid] = best;
+        s_k[tid] = best_k;
+        __syncthreads();
+        for (int w = kBruteBlock / 2; w > 0; w >>= 1) {
+            if (tid < w) {
+                const double d = s_d2[tid + w];
+                const int k = s_k[tid + w];
+                if (d < s_d2[tid] || (d == s_d2[tid] && k < s_k[tid])) s_d2[tid] = d, s_k[tid] = k;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const int k = s_k[0];
+            Dbl3 q;
+            int f;
+            tri_d2(k, p, q, f);  // the winner's nearest point and feature, recomputed
+            const double* nrm = m.pn + 3 * ((size_t)kNormalSlots * k + normal_slot(f));
+            const Dbl3 u = p - q;
+            const double sign = (u.x * nrm[0] + u.y * nrm[1] + u.z * nrm[2]) >= 0.0 ? 1.0 : -1.0;
+            coef[l] = sign * sqrt(s_d2[0]);
+        }
+        __syncthreads();
+    }
 }
 
 // The body grid: float nodes gmin + i ds, the vertex box's centre.
@@ -402,6 +511,26 @@ __global__ __launch_bounds__(256) void sample_kernel(const double* __restrict__ 
     out[i] = o;
 }
 
+__global__ void fill_nan_kernel(double* p, unsigned long long n) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = NAN;
+}
+
+// f(begin, end) over [0, n) split across up to 16 host threads
+template <class F>
+void parallel_ranges(int n, F f) {
+    const int workers = (int)std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    const int chunks = std::min(workers, std::max(1, n / 4096));
+    if (chunks <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int c = 0; c < chunks; c++)
+        pool.emplace_back(f, (int)((long long)n * c / chunks), (int)((long long)n * (c + 1) / chunks));
+    for (auto& t : pool) t.join();
+}
+
 template <class T>
 struct Scoped {
     T* p = nullptr;
@@ -471,65 +600,114 @@ int mesh_sdf_build(const float* verts, int nv, const int* tris, int nt, const in
     const unsigned long long n_nodes = g.num_nodes();
 
     // triangles, boxes and angle-weighted pseudo-normals
-    // (TriangleMeshDistance.h:342-441), in double on the host
+    // (TriangleMeshDistance.h:342-441), in double on the host.  The per-
+    // triangle work runs on a thread pool; the sums over triangles (vertex
+    // and edge normals) run in triangle order, as the reference adds them.
     std::vector<float4> tri(3 * (size_t)nt);
     std::vector<float> box(6 * (size_t)nt);
     std::vector<double> pn(3 * (size_t)kNormalSlots * nt);
-    std::vector<Dbl3> v(nv), n_vert(nv, Dbl3{0, 0, 0}), n_face(nt);
-    for (int i = 0; i < nv; i++) v[i] = Dbl3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
-    std::unordered_map<uint64_t, Dbl3> edge_sum;
-    edge_sum.reserve(3 * (size_t)nt);
-    auto key = [&](int i, int j) { return (uint64_t)std::min(i, j) * (uint64_t)nv + (uint64_t)std::max(i, j); };
-    auto add_edge = [&](int i, int j, Dbl3 n) {
-        auto it = edge_sum.find(key(i, j));
-        if (it == edge_sum.end()) edge_sum.emplace(key(i, j), n);
-        else it->second = it->second + n;
-    };
-    for (int k = 0; k < nt; k++) {
-        const int* t = tris + 3 * k;
-        const Dbl3 a = v[t[0]], b = v[t[1]], c = v[t[2]];
-        const Dbl3 cr = cross3(b - a, c - a);
-        if (!(dot3(cr, cr) > 0)) {
+    std::vector<Dbl3> n_vert(nv, Dbl3{0, 0, 0}), n_face(nt);
+    std::vector<double> w(3 * (size_t)nt);
+    std::vector<char> degenerate(nt, 0);
+    auto vtx = [&](int i) { return Dbl3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]}; };
+    parallel_ranges(nt, [&](int k0, int k1) {
+        for (int k = k0; k < k1; k++) {
+            const int* t = tris + 3 * k;
+            const Dbl3 a = vtx(t[0]), b = vtx(t[1]), c = vtx(t[2]);
+            const Dbl3 cr = cross3(b - a, c - a);
+            if (!(dot3(cr, cr) > 0)) {
+                degenerate[k] = 1;
+                continue;
+            }
+            n_face[k] = unit3(cr);
+            w[3 * k] = std::acos(std::fabs(dot3(unit3(b - a), unit3(c - a))));
+            w[3 * k + 1] = std::acos(std::fabs(dot3(unit3(a - b), unit3(c - b))));
+            w[3 * k + 2] = std::acos(std::fabs(dot3(unit3(b - c), unit3(a - c))));
+            for (int j = 0; j < 3; j++) {
+                const float* q = verts + 3 * t[j];
+                tri[3 * k + j] = make_float4(q[0], q[1], q[2], 0.f);
+                for (int a2 = 0; a2 < 3; a2++) {
+                    float& mn = box[6 * k + a2];
+                    float& mx = box[6 * k + 3 + a2];
+                    if (j == 0 || q[a2] < mn) mn = q[a2];
+                    if (j == 0 || q[a2] > mx) mx = q[a2];
+                }
+            }
+        }
+    });
+    for (int k = 0; k < nt; k++)
+        if (degenerate[k]) {
             err = "mesh SDF: triangle " + std::to_string(k) + " has zero area";
             return KSCR_ERR_INVALID_ARGUMENT;
         }
-        const Dbl3 n = unit3(cr);
-        n_face[k] = n;
-        const double w0 = std::acos(std::fabs(dot3(unit3(b - a), unit3(c - a))));
-        const double w1 = std::acos(std::fabs(dot3(unit3(a - b), unit3(c - b))));
-        const double w2 = std::acos(std::fabs(dot3(unit3(b - c), unit3(a - c))));
-        n_vert[t[0]] = n_vert[t[0]] + w0 * n;
-        n_vert[t[1]] = n_vert[t[1]] + w1 * n;
-        n_vert[t[2]] = n_vert[t[2]] + w2 * n;
-        add_edge(t[0], t[1], n);
-        add_edge(t[1], t[2], n);
-        add_edge(t[0], t[2], n);
-        for (int j = 0; j < 3; j++) {
-            const float* q = verts + 3 * t[j];
-            tri[3 * k + j] = make_float4(q[0], q[1], q[2], 0.f);
-            for (int a2 = 0; a2 < 3; a2++) {
-                float& mn = box[6 * k + a2];
-                float& mx = box[6 * k + 3 + a2];
-                if (j == 0 || q[a2] < mn) mn = q[a2];
-                if (j == 0 || q[a2] > mx) mx = q[a2];
-            }
-        }
-    }
-    for (auto& n : n_vert) n = unit3(n);
     for (int k = 0; k < nt; k++) {
         const int* t = tris + 3 * k;
-        const Dbl3 slot[kNormalSlots] = {n_face[k],
-                                         unit3(edge_sum[key(t[0], t[1])]),
-                                         unit3(edge_sum[key(t[1], t[2])]),
-                                         unit3(edge_sum[key(t[0], t[2])]),
-                                         n_vert[t[0]],
-                                         n_vert[t[1]],
-                                         n_vert[t[2]]};
-        for (int s = 0; s < kNormalSlots; s++) {
-            double* d = pn.data() + 3 * ((size_t)kNormalSlots * k + s);
-            d[0] = slot[s].x, d[1] = slot[s].y, d[2] = slot[s].z;
+        for (int j = 0; j < 3; j++) n_vert[t[j]] = n_vert[t[j]] + w[3 * k + j] * n_face[k];
+    }
+    // edge normals: the face normals of the triangles sharing an edge summed
+    // in triangle order (edges 01, 12, 02 within a triangle).  Edges are
+    // grouped by their smaller vertex with a stable counting sort, then by
+    // the larger one within the group.
+    const size_t ne = 3 * (size_t)nt;
+    std::vector<uint32_t> start(nv + 1, 0), order(ne);
+    auto edge_ends = [&](size_t e, int& lo, int& hi) {
+        const int* t = tris + 3 * (e / 3);
+        const int s = (int)(e % 3);
+        const int i = t[s == 2 ? 0 : s], j = t[s == 0 ? 1 : 2];
+        lo = std::min(i, j), hi = std::max(i, j);
+    };
+    for (size_t e = 0; e < ne; e++) {
+        int lo, hi;
+        edge_ends(e, lo, hi);
+        start[lo + 1]++;
+    }
+    for (int i = 0; i < nv; i++) start[i + 1] += start[i];
+    {
+        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+        for (size_t e = 0; e < ne; e++) {
+            int lo, hi;
+            edge_ends(e, lo, hi);
+            order[fill[lo]++] = (uint32_t)e;
         }
     }
+    std::vector<Dbl3> edge_n(ne);  // per triangle edge slot: the normalized sum
+    parallel_ranges(nv, [&](int v0, int v1) {
+        std::vector<char> done;
+        for (int i = v0; i < v1; i++) {
+            const uint32_t b0 = start[i], b1 = start[i + 1];
+            done.assign(b1 - b0, 0);
+            for (uint32_t x = b0; x < b1; x++) {
+                if (done[x - b0]) continue;
+                int lo, hi, lo2, hi2;
+                edge_ends(order[x], lo, hi);
+                Dbl3 sum = n_face[order[x] / 3];
+                for (uint32_t y = x + 1; y < b1; y++) {
+                    edge_ends(order[y], lo2, hi2);
+                    if (hi2 == hi) sum = sum + n_face[order[y] / 3];
+                }
+                const Dbl3 n = unit3(sum);
+                for (uint32_t y = x; y < b1; y++) {
+                    edge_ends(order[y], lo2, hi2);
+                    if (hi2 == hi) edge_n[order[y]] = n, done[y - b0] = 1;
+                }
+            }
+        }
+    });
+    parallel_ranges(nv, [&](int v0, int v1) {
+        for (int i = v0; i < v1; i++) n_vert[i] = unit3(n_vert[i]);
+    });
+    parallel_ranges(nt, [&](int k0, int k1) {
+        for (int k = k0; k < k1; k++) {
+            const int* t = tris + 3 * k;
+            const Dbl3 slot[kNormalSlots] = {n_face[k],         edge_n[3 * (size_t)k], edge_n[3 * (size_t)k + 1],
+                                             edge_n[3 * (size_t)k + 2], n_vert[t[0]], n_vert[t[1]],
+                                             n_vert[t[2]]};
+            for (int s2 = 0; s2 < kNormalSlots; s2++) {
+                double* d = pn.data() + 3 * ((size_t)kNormalSlots * k + s2);
+                d[0] = slot[s2].x, d[1] = slot[s2].y, d[2] = slot[s2].z;
+            }
+        }
+    });
 
     hipError_t e;
     auto fail = [&](hipError_t code, const char* what) {
@@ -539,9 +717,10 @@ int mesh_sdf_build(const float* verts, int nv, const int* tris, int nt, const in
     Scoped<float4> d_tri;
     Scoped<double> d_pn, d_coef;
     Scoped<int> d_err;
+    Scoped<unsigned long long> d_over;
     ScopedLbvh bvh;
     if ((e = d_tri.alloc(tri.size())) != hipSuccess || (e = d_pn.alloc(pn.size())) != hipSuccess ||
-        (e = d_coef.alloc(n_nodes)) != hipSuccess || (e = d_err.alloc(1)) != hipSuccess ||
+        (e = d_coef.alloc(n_nodes)) != hipSuccess || (e = d_err.alloc(2)) != hipSuccess ||
         (e = lbvh_reserve(bvh.d, nt)) != hipSuccess)
         return fail(e, "hipMalloc");
     if ((e = hipMemcpyAsync(d_tri.p, tri.data(), tri.size() * sizeof(float4), hipMemcpyHostToDevice, stream)) !=
@@ -550,8 +729,11 @@ int mesh_sdf_build(const float* verts, int nv, const int* tris, int nt, const in
             hipSuccess ||
         (e = hipMemcpyAsync(bvh.d.aabb, box.data(), box.size() * sizeof(float), hipMemcpyHostToDevice, stream)) !=
             hipSuccess ||
-        (e = fill_i32(d_err.p, 0, 1, stream)) != hipSuccess)
+        (e = fill_i32(d_err.p, 0, 2, stream)) != hipSuccess)
         return fail(e, "upload");
+    // nodes outside the sampled cells stay NaN: a read of one would show in the grid
+    hipLaunchKernelGGL(fill_nan_kernel, dim3((unsigned)((n_nodes + 255) / 256)), dim3(256), 0, stream, d_coef.p,
+                       n_nodes);
     if ((e = lbvh_build(bvh.d, nt, stream)) != hipSuccess) return fail(e, "triangle BVH build");
     if (bvh.d.max_depth_host > 64) {
         err = "mesh SDF: triangle BVH deeper than 64 levels";
@@ -559,14 +741,6 @@ int mesh_sdf_build(const float* verts, int nv, const int* tris, int nt, const in
     }
     MeshDev m{d_tri.p, d_pn.p, nt};
     const double slack = 1e-12 * diag2;
-    const dim3 grid((unsigned)((n_nodes + kDistBlock - 1) / kDistBlock));
-    if (bvh.d.max_depth_host <= 32)
-        hipLaunchKernelGGL(node_distance_kernel<32>, grid, dim3(kDistBlock), 0, stream, m, bvh.d.nodes, bvh.d.ids, g,
-                           slack, d_coef.p, d_err.p);
-    else
-        hipLaunchKernelGGL(node_distance_kernel<64>, grid, dim3(kDistBlock), 0, stream, m, bvh.d.nodes, bvh.d.ids, g,
-                           slack, d_coef.p, d_err.p);
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "node_distance_kernel");
     BodyGrid b;
     for (int a = 0; a < 3; a++) {
         b.dim[a] = dim[a];
@@ -574,6 +748,43 @@ int mesh_sdf_build(const float* verts, int nv, const int* tris, int nt, const in
         b.ds[a] = (gmax[a] - gmin[a]) / (float)dim[a];
         b.centre[a] = (gmax[a] + gmin[a]) / 2;
     }
+    // the cells the body nodes fall in: along each axis the first and the last
+    // node's cell (the cell index is monotone in the node index), with the
+    // sample kernel's arithmetic
+    SubGrid sub;
+    for (int a = 0; a < 3; a++) {
+        auto cell_of = [&](int i) {
+            const float x = b.gmin[a] + b.ds[a] * (float)i;
+            const double t = ((double)x - g.lo[a]) * g.inv_cell[a];
+            unsigned c = t > 0.0 ? (unsigned)t : 0u;
+            return c >= g.n[a] ? g.n[a] - 1 : c;
+        };
+        const unsigned lo_c = cell_of(0), hi_c = cell_of(dim[a] - 1);
+        sub.c0[a] = lo_c;
+        sub.m[a] = hi_c - lo_c + 1;
+    }
+    {
+        const unsigned long long m0 = sub.m[0], m1 = sub.m[1], m2 = sub.m[2];
+        sub.nc = (m0 + 1) * (m1 + 1) * (m2 + 1);
+        sub.nx = 2 * m0 * (m1 + 1) * (m2 + 1);
+        sub.ny = 2 * (m0 + 1) * m1 * (m2 + 1);
+        sub.nz = 2 * (m0 + 1) * (m1 + 1) * m2;
+    }
+    const dim3 grid((unsigned)((sub.count() + kDistBlock - 1) / kDistBlock));
+    if ((e = d_over.alloc(sub.count())) != hipSuccess) return fail(e, "hipMalloc");
+    unsigned* over_count = reinterpret_cast<unsigned*>(d_err.p + 1);
+    // leaf tests a lane may spend before its node goes to node_brute_kernel
+    const int budget = 1024;
+    if (bvh.d.max_depth_host <= 32)
+        hipLaunchKernelGGL(node_distance_kernel<32>, grid, dim3(kDistBlock), 0, stream, m, bvh.d.nodes, bvh.d.ids, g,
+                           sub, slack, budget, d_coef.p, d_over.p, over_count, d_err.p);
+    else
+        hipLaunchKernelGGL(node_distance_kernel<64>, grid, dim3(kDistBlock), 0, stream, m, bvh.d.nodes, bvh.d.ids, g,
+                           sub, slack, budget, d_coef.p, d_over.p, over_count, d_err.p);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "node_distance_kernel");
+    hipLaunchKernelGGL(node_brute_kernel, dim3(2048), dim3(kBruteBlock), 0, stream, m, g, d_over.p, over_count,
+                       d_coef.p);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "node_brute_kernel");
     hipLaunchKernelGGL(sample_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, d_coef.p, g, b, out);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e, "sample_kernel");
     int flag = 0;

@@ -240,6 +240,36 @@ def union_sdf(*fns):
 
 
 # ------------------------------------------------------------------ workloads
+def sphere_mesh(level=6, radius=0.1, center=(0.0, 0.0, 0.0)):
+    """Closed icosphere body mesh (20 * 4^level triangles, outward winding):
+    vertices float32[n, 3], triangles int32[m, 3]."""
+    p = (1 + 5 ** 0.5) / 2
+    v = [(-1, p, 0), (1, p, 0), (-1, -p, 0), (1, -p, 0), (0, -1, p), (0, 1, p), (0, -1, -p), (0, 1, -p),
+         (p, 0, -1), (p, 0, 1), (-p, 0, -1), (-p, 0, 1)]
+    v = [np.array(x, np.float64) / np.linalg.norm(x) for x in v]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        mid = {}
+
+        def midpoint(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                x = v[a] + v[b]
+                v.append(x / np.linalg.norm(x))
+                mid[k] = len(v) - 1
+            return mid[k]
+
+        g = []
+        for a, b, c in f:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            g += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = g
+    verts = np.array(v) * radius + np.asarray(center, np.float64)
+    return verts.astype(F32), np.array(f, np.int32)
+
+
 def config_cantilever():
     """1 strand x 64 vertices, 1 mm segments, clamped root, g = 9.81, E = 3 GPa,
     r = 40 um, rho = 1300, 10 iterations (dt = 1/600 s, 100 steps)."""
