@@ -44,10 +44,12 @@ __device__ inline f4 ld4(const float4* p, size_t i) { return to4(p[i]); }
 __device__ inline void st4(float4* p, size_t i, f4 a) { p[i] = make_float4(a.x, a.y, a.z, a.w); }
 
 // Values of one tape iterate a thread reads: its own and the next vertex's
-// dx, and the rotors of its segment and of the two neighbouring ones.
+// dx, the rotors of its segment and of the two neighbouring ones, and the
+// normalization factor of the orientation solve that starts from it.
 struct Iterate {
     f3 dx, dx_n;
     f4 q, qm, qp;
+    float ilw;
 };
 
 template <int BLOCK>
@@ -124,8 +126,8 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
         // Iterates of the tape: k + 1 (dxn, dxn_n) is carried over from the
         // previous pass of the loop, k is loaded one pass ahead (the loads stay
         // in flight across the barriers, which wait on LDS traffic only).
-        const size_t o_sub = (size_t)sub * (K + 1) * n;  // dx^k: K + 1 per substep
-        const size_t q_sub = (size_t)sub * K * n;        // q^k: K per substep
+        const size_t o_sub = (size_t)sub * (K + 1) * n;  // dx^k and q^k: K + 1 per substep
+        const size_t w_sub = (size_t)sub * K * n;        // 1 / |w| of iteration k: K per substep
         // Reverse iteration k reads iterate k + 1 (dx, dx_n) and iterate k
         // (dx, dx_n and the rotors) and prefetches iterate k - 1 (iterate 0
         // again on the last pass).  Three register sets take the three roles
@@ -133,9 +135,10 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
         // passes.
         const auto load_iterate = [&](int k, Iterate& it) __attribute__((always_inline)) {
             const float* DXk = a.tape.dx + (o_sub + (size_t)k * n) * 3;
-            const float4* Qk = a.tape.q + q_sub + (size_t)k * n;
+            const float4* Qk = a.tape.q + o_sub + (size_t)k * n;
             it.dx = ld3(DXk, ic), it.dx_n = ld3(DXk, inx);
             it.q = ld4(Qk, ic), it.qm = ld4(Qk, ipv), it.qp = ld4(Qk, inx);
+            it.ilw = a.tape.ilw[w_sub + (size_t)k * n + ic];
         };
         const auto iteration = [&](int k, const Iterate& nx, const Iterate& cu, Iterate& pf)
                                    __attribute__((always_inline)) {
@@ -175,13 +178,11 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
                 }
                 const float lv = length(vv), lb = length(b);
                 const float lambda = lv + lb;
-                const f4 m = rmul_pure(vv, b);
-                const f4 tq = rmul_e1(m);
-                const f4 w = tq + lambda * b;
-                const float lw = length(w);
-                const f4 qn = (1.0f / lw) * w;
+                // the solve's result q^{k+1} = w / |w| and 1 / |w| come from the
+                // tape (the forward's own values), not from a recomputation
+                const f4 qn = nx.q;
                 const f4 g = qb;
-                const f4 wb = (1.0f / lw) * (g + (-dot(qn, g)) * qn);
+                const f4 wb = cu.ilw * (g + (-dot(qn, g)) * qn);
                 f4 bb = lambda * wb;
                 const float lamb = dot(wb, b);
                 const f4 mb = rmul_conj_e1(wb);
@@ -296,6 +297,7 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
         {
             const float* DXK = a.tape.dx + (o_sub + (size_t)K * n) * 3;
             A.dx = ld3(DXK, ic), A.dx_n = ld3(DXK, inx);
+            A.q = ld4(a.tape.q + o_sub + (size_t)K * n, ic);
         }
         if (K > 0) load_iterate(K - 1, B);
         int k = K - 1;

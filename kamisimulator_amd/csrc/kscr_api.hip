@@ -118,6 +118,7 @@ struct AdjointRun {
     DevBuf<float> t_x, t_y, t_dx;
     DevBuf<unsigned char> t_sigma;
     DevBuf<float4> t_q;
+    DevBuf<float> t_ilw;
     // adjoint state
     DevBuf<float> xb, vb, ub, ksb;
     DevBuf<float4> qb, qrb;
@@ -448,12 +449,14 @@ int reserve_tape(kscr_handle* h) {
     AdjointRun& A = h->adj;
     const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = (size_t)A.every;
     const size_t K = (size_t)A.num_itr;
-    const size_t per_sub = S * n * every, per_dx = S * I1 * n * every, per_q = S * K * n * every;
-    if (A.t_dx.n >= per_dx * 3 && A.t_q.n >= per_q && A.t_x.n >= per_sub * 3) return KSCR_OK;
+    const size_t per_sub = S * n * every, per_dx = S * I1 * n * every, per_q = S * I1 * n * every;
+    const size_t per_ilw = S * K * n * every;
+    if (A.t_dx.n >= per_dx * 3 && A.t_q.n >= per_q && A.t_x.n >= per_sub * 3 && A.t_ilw.n >= per_ilw)
+        return KSCR_OK;
     hipError_t e;
     if ((e = A.t_x.alloc(per_sub * 3)) != hipSuccess || (e = A.t_y.alloc(per_sub * 3)) != hipSuccess ||
         (e = A.t_sigma.alloc(per_sub)) != hipSuccess || (e = A.t_dx.alloc(per_dx * 3)) != hipSuccess ||
-        (e = A.t_q.alloc(per_q)) != hipSuccess)
+        (e = A.t_q.alloc(per_q)) != hipSuccess || (e = A.t_ilw.alloc(std::max<size_t>(per_ilw, 1))) != hipSuccess)
         return h->hip_fail(e, "hipMalloc tape");
     return KSCR_OK;
 }
@@ -467,7 +470,8 @@ TapeArgs tape_of(kscr_handle* h, size_t j) {
     t.y = A.t_y.p + j * S * n * 3;
     t.sigma_free = A.t_sigma.p + j * S * n;
     t.dx = A.t_dx.p + j * S * (K + 1) * n * 3;
-    t.q = A.t_q.p ? A.t_q.p + j * S * K * n : nullptr;
+    t.q = A.t_q.p + j * S * (K + 1) * n;
+    t.ilw = A.t_ilw.p + j * S * K * n;
     t.n = (int)n;
     return t;
 }

@@ -64,7 +64,8 @@ struct TapeArgs {
     float* y;
     unsigned char* sigma_free;
     float* dx;
-    float4* q;
+    float4* q;   // q^0 .. q^K per substep
+    float* ilw;  // per iteration: the orientation solve's 1 / |w| (w normalized to q^{k+1})
     int n;
 };
 

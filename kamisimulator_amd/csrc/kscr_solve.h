@@ -119,8 +119,9 @@ __device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, co
 // <conj(q) qp, qr>.  With <a, b> = Re(a conj(b)) and Re(ABC) = Re(BCA) these
 // equal <q, qm qrp> and <q, qp conj(qr)>, the dot products of q with the two
 // rotors b is built from, which are formed here anyway.
+// inv_norm receives the factor 1 / |w| that normalizes w to the result.
 __device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ks, f4 q, f4 qm, f4 qp, f4 qrp,
-                                        f4 qr, uint32_t fl) {
+                                        f4 qr, uint32_t fl, float& inv_norm) {
     f3 vv = ((xj - x) + (dxj - dxn)) / lr;
     vv = vv * (-2.0f * ks);
     f4 b{0, 0, 0, 0};
@@ -135,7 +136,14 @@ __device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, f
         b = b + phi * bp;
     }
     const float lambda = length(vv) + length(b);
-    return normalize(rmul_e1(rmul_pure(vv, b)) + lambda * b);
+    const f4 w = rmul_e1(rmul_pure(vv, b)) + lambda * b;
+    inv_norm = 1.0f / sqrtf(dot(w, w));
+    return f4{w.x * inv_norm, w.y * inv_norm, w.z * inv_norm, w.w * inv_norm};
+}
+__device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ks, f4 q, f4 qm, f4 qp, f4 qrp,
+                                        f4 qr, uint32_t fl) {
+    float inv_norm;
+    return orientation_update(x, xj, dxn, dxj, lr, ks, q, qm, qp, qrp, qr, fl, inv_norm);
 }
 
 // endItr (YB/sim/iteration.cu:62-74)

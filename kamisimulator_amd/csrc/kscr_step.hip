@@ -33,7 +33,18 @@ struct StepLds {
     float4 f[BLOCK];
 };
 
-__device__ inline void tape3(float* base, int slot, int n, int i, f3 v) { st3(base + (size_t)slot * n * 3, i, v); }
+// Tape stores are non-temporal: the tape of a checkpoint segment is read back
+// only after the whole segment has been recorded, far beyond the caches.
+__device__ inline void tape3(float* base, int slot, int n, int i, f3 v) {
+    float* p = base + (size_t)slot * n * 3 + 3 * (size_t)i;
+    __builtin_nontemporal_store(v.x, p);
+    __builtin_nontemporal_store(v.y, p + 1);
+    __builtin_nontemporal_store(v.z, p + 2);
+}
+typedef float tape_f4 __attribute__((ext_vector_type(4)));
+__device__ inline void tape4(float4* p, f4 v) {
+    __builtin_nontemporal_store(tape_f4{v.x, v.y, v.z, v.w}, reinterpret_cast<tape_f4*>(p));
+}
 
 // Advances the strands of one pack [vbeg, vend) through a whole step; with
 // RECORD it also writes the tape of the step.
@@ -106,7 +117,7 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             tape3(tp->y, s, tp->n, i, y);
             tp->sigma_free[(size_t)s * tp->n + i] = sigma_free ? 1 : 0;
             tape3(tp->dx, s * (a.num_itr + 1), tp->n, i, dx);
-            if (a.num_itr > 0) tp->q[(size_t)s * a.num_itr * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
+            tape4(tp->q + (size_t)s * (a.num_itr + 1) * tp->n + i, q);
         }
         __syncthreads();  // previous substep's LDS reads are complete
         s_x[0][t] = x.x;
@@ -148,18 +159,20 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             __syncthreads();
             // ---- orientation solve (closed form, SCR Eq. 18)
             f4 qn = q;
+            float inv_norm = 0.0f;
             if (rotates) {
                 const f3 dxj{s_dx[cur ^ 1][0][t + 1], s_dx[cur ^ 1][1][t + 1], s_dx[cur ^ 1][2][t + 1]};
                 const f4 qm = hasPrev ? to4(s_q[cur][t - 1]) : f4{0, 0, 0, 1};
                 const f4 qp = (fl & FLAG_NEXT_ORI) ? to4(s_q[cur][t + 1]) : f4{0, 0, 0, 1};
-                qn = orientation_update(x, xj, dxn, dxj, lr, ks, q, qm, qp, qrp, qr, fl);
+                qn = orientation_update(x, xj, dxn, dxj, lr, ks, q, qm, qp, qrp, qr, fl, inv_norm);
             }
             s_q[cur ^ 1][t] = make_float4(qn.x, qn.y, qn.z, qn.w);
             dx = dxn;
             q = qn;
             if (RECORD && act) {
                 tape3(tp->dx, s * (a.num_itr + 1) + k + 1, tp->n, i, dx);
-                if (k + 1 < a.num_itr) tp->q[(size_t)(s * a.num_itr + k + 1) * tp->n + i] = make_float4(q.x, q.y, q.z, q.w);
+                tape4(tp->q + (size_t)(s * (a.num_itr + 1) + k + 1) * tp->n + i, q);
+                __builtin_nontemporal_store(inv_norm, tp->ilw + (size_t)(s * a.num_itr + k) * tp->n + i);
             }
             // The next iteration's first LDS write (s_f) is ordered after this
             // iteration's s_f reads by the barrier above; its s_dx/s_q writes

@@ -120,6 +120,30 @@ struct AdjointSim : Sim<T> {
         time = (T)timeF;
     }
 
+    // 1 / |w| of the forward's orientation solve of vertex i in iteration k
+    // (orientation_solve's operations on the tape's values)
+    T forward_inv_norm(const SubstepTape<T>& t, int k, int i) const {
+        const uint32_t fl = flags[i];
+        const std::vector<V3<T>>& DXn = t.dx[k + 1];
+        const std::vector<R4<T>>& Q = t.q[k];
+        V3<T> vv = ((t.x[i + 1] - t.x[i]) + (DXn[i + 1] - DXn[i])) / lRest[i];
+        vv = vv * (T(-2) * kStretch[i]);
+        R4<T> b{0, 0, 0, 0};
+        if (fl & F_PREV) {
+            const R4<T> bm = rmul(Q[i - 1], qRest[i - 1]);
+            T phi = dot4(Q[i], bm) > 0 ? T(1) : T(-1);
+            b = add4(b, scale4(phi, bm));
+        }
+        if (fl & F_NEXT_ORI) {
+            const R4<T> bp = rmul(Q[i + 1], conj(qRest[i]));
+            T phi = dot4(Q[i], bp) > 0 ? T(1) : T(-1);
+            b = add4(b, scale4(phi, bp));
+        }
+        T lambda = length(vv) + length4(b);
+        const R4<T> w = add4(rmul_e1(rmul_pure(vv, b)), scale4(lambda, b));
+        return T(1) / std::sqrt(dot4(w, w));
+    }
+
     // Reverse of one substep: adj holds the adjoints of the substep's outputs
     // (x', v', u', q') on entry and of its inputs (x, v, u, q) on exit.
     void backward_substep(const SubstepTape<T>& t, Adjoint<T>& adj) {
@@ -166,14 +190,12 @@ struct AdjointSim : Sim<T> {
                 }
                 const T lv = length(vv), lb = length4(b);
                 const T lambda = lv + lb;
-                const R4<T> m = rmul_pure(vv, b);
-                const R4<T> tq = rmul_e1(m);
-                const R4<T> w = add4(tq, scale4(lambda, b));
-                const T lw = length4(w);
-                const R4<T> qn = scale4(T(1) / lw, w);
+                // normalize: the forward's result qn = w / |w| (the tape's
+                // q^{k+1}) and its factor 1 / |w|
+                const R4<T> qn = t.q[k + 1][i];
+                const T ilw = forward_inv_norm(t, k, i);
                 const R4<T> g = adj.qb[i];
-                // normalize
-                const R4<T> wb = scale4(T(1) / lw, add4(g, scale4(-dot4(qn, g), qn)));
+                const R4<T> wb = scale4(ilw, add4(g, scale4(-dot4(qn, g), qn)));
                 // w = t + lambda b
                 R4<T> bb = scale4(lambda, wb);
                 const T lamb = dot4(wb, b);
