@@ -174,7 +174,7 @@ __device__ __forceinline__ void run_pack_colored(const StepArgs& a, const int vb
             if (E.rotates) {
                 const f4 qm = E.has_prev ? to4(L.q[l0 - 1]) : f4{0, 0, 0, 1};
                 const f4 qp = (E.fl & FLAG_NEXT_ORI) ? O.q : f4{0, 0, 0, 1};
-                qE = orientation_update(E.x, xjE, dxE, dxO, E.lr, E.ks, E.q, qm, qp, E.qrp, E.qr, E.fl);
+                qE = orientation_update(E.x, xjE, dxE, dxO, E.lr, E.invl, E.ks, E.q, qm, qp, E.qrp, E.qr, E.fl);
             }
             L.q[l0] = make_float4(qE.x, qE.y, qE.z, qE.w);
             __syncthreads();
@@ -183,7 +183,7 @@ __device__ __forceinline__ void run_pack_colored(const StepArgs& a, const int vb
             if (O.rotates) {
                 const f4 qm = O.has_prev ? qE : f4{0, 0, 0, 1};
                 const f4 qp = (O.fl & FLAG_NEXT_ORI) ? to4(L.q[l1 + 1]) : f4{0, 0, 0, 1};
-                qO = orientation_update(O.x, xjO, dxO, dxjO, O.lr, O.ks, O.q, qm, qp, O.qrp, O.qr, O.fl);
+                qO = orientation_update(O.x, xjO, dxO, dxjO, O.lr, O.invl, O.ks, O.q, qm, qp, O.qrp, O.qr, O.fl);
             }
             L.q[l1] = make_float4(qO.x, qO.y, qO.z, qO.w);
             E.dx = dxE, O.dx = dxO;

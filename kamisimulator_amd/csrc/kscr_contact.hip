@@ -303,8 +303,8 @@ __global__ void orientation_kernel(StepArgs a, int n, const float* __restrict__ 
         const f4 qm = (fl & FLAG_PREV) ? to4(q_in[i - 1]) : f4{0, 0, 0, 1};
         const f4 qp = (fl & FLAG_NEXT_ORI) ? to4(q_in[i + 1]) : f4{0, 0, 0, 1};
         const f4 qrp = (fl & FLAG_PREV) ? to4(a.q_rest[i - 1]) : f4{0, 0, 0, 1};
-        qn = orientation_update(ld3(a.x, i), ld3(a.x, i + 1), ld3(dxn, i), ld3(dxn, i + 1), mi.y, mi.z, q, qm, qp, qrp,
-                                to4(a.q_rest[i]), fl);
+        qn = orientation_update(ld3(a.x, i), ld3(a.x, i + 1), ld3(dxn, i), ld3(dxn, i + 1), mi.y, 1.0f / mi.y, mi.z, q,
+                                qm, qp, qrp, to4(a.q_rest[i]), fl);
     }
     q_out[i] = make_float4(qn.x, qn.y, qn.z, qn.w);
 }

@@ -119,10 +119,11 @@ __device__ inline void init_vertex(const StepArgs& a, bool free, const f3& x, co
 // <conj(q) qp, qr>.  With <a, b> = Re(a conj(b)) and Re(ABC) = Re(BCA) these
 // equal <q, qm qrp> and <q, qp conj(qr)>, the dot products of q with the two
 // rotors b is built from, which are formed here anyway.
-// inv_norm receives the factor 1 / |w| that normalizes w to the result.
-__device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ks, f4 q, f4 qm, f4 qp, f4 qrp,
-                                        f4 qr, uint32_t fl, float& inv_norm) {
-    f3 vv = ((xj - x) + (dxj - dxn)) / lr;
+// ilr = 1 / lr (rounded to nearest); inv_norm receives the factor 1 / |w|
+// that normalizes w to the result.
+__device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ilr, float ks, f4 q, f4 qm,
+                                        f4 qp, f4 qrp, f4 qr, uint32_t fl, float& inv_norm) {
+    f3 vv = div_rn((xj - x) + (dxj - dxn), lr, ilr);  // the three divisions by lr
     vv = vv * (-2.0f * ks);
     f4 b{0, 0, 0, 0};
     if (fl & FLAG_PREV) {
@@ -140,10 +141,10 @@ __device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, f
     inv_norm = 1.0f / sqrtf(dot(w, w));
     return f4{w.x * inv_norm, w.y * inv_norm, w.z * inv_norm, w.w * inv_norm};
 }
-__device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ks, f4 q, f4 qm, f4 qp, f4 qrp,
-                                        f4 qr, uint32_t fl) {
+__device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ilr, float ks, f4 q, f4 qm,
+                                        f4 qp, f4 qrp, f4 qr, uint32_t fl) {
     float inv_norm;
-    return orientation_update(x, xj, dxn, dxj, lr, ks, q, qm, qp, qrp, qr, fl, inv_norm);
+    return orientation_update(x, xj, dxn, dxj, lr, ilr, ks, q, qm, qp, qrp, qr, fl, inv_norm);
 }
 
 // endItr (YB/sim/iteration.cu:62-74)

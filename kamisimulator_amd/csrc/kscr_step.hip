@@ -164,7 +164,7 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
                 const f3 dxj{s_dx[cur ^ 1][0][t + 1], s_dx[cur ^ 1][1][t + 1], s_dx[cur ^ 1][2][t + 1]};
                 const f4 qm = hasPrev ? to4(s_q[cur][t - 1]) : f4{0, 0, 0, 1};
                 const f4 qp = (fl & FLAG_NEXT_ORI) ? to4(s_q[cur][t + 1]) : f4{0, 0, 0, 1};
-                qn = orientation_update(x, xj, dxn, dxj, lr, ks, q, qm, qp, qrp, qr, fl, inv_norm);
+                qn = orientation_update(x, xj, dxn, dxj, lr, invl, ks, q, qm, qp, qrp, qr, fl, inv_norm);
             }
             s_q[cur ^ 1][t] = make_float4(qn.x, qn.y, qn.z, qn.w);
             dx = dxn;
