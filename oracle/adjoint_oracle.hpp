@@ -49,6 +49,11 @@ struct Adjoint {
     }
 };
 
+template <class T>
+inline T clamp_sq(T x) {
+    return std::min(std::max(x, T(0x1p-62)), T(0x1p62));
+}
+
 // rotate(q, e1) = (w^2 + x^2 - y^2 - z^2, 2(xy + wz), 2(xz - wy)) for
 // q = (x, y, z, w); its transposed Jacobian with respect to q applied to rb.
 template <class T>
@@ -188,7 +193,9 @@ struct AdjointSim : Sim<T> {
                     phip = dot4(Q[i], bp) > 0 ? T(1) : T(-1);
                     b = add4(b, scale4(phip, bp));
                 }
-                const T lv = length(vv), lb = length4(b);
+                // lengths from the squared lengths clamped into [2^-62, 2^62]
+                // (the HIP sweep's definition; no effect on a rod's lengths)
+                const T lv = std::sqrt(clamp_sq(dot(vv, vv))), lb = std::sqrt(clamp_sq(dot4(b, b)));
                 const T lambda = lv + lb;
                 // normalize: the forward's result qn = w / |w| (the tape's
                 // q^{k+1}) and its factor 1 / |w|
@@ -205,8 +212,8 @@ struct AdjointSim : Sim<T> {
                 V3<T> vvb = rmul(mb, conj(b)).q();
                 bb = add4(bb, rmul_pure(-vv, mb));
                 // lambda = |v| + |b|
-                if (lv > 0) vvb = vvb + (lamb / lv) * vv;
-                if (lb > 0) bb = add4(bb, scale4(lamb / lb, b));
+                vvb = vvb + (lamb * (T(1) / lv)) * vv;
+                bb = add4(bb, scale4(lamb * (T(1) / lb), b));
                 // b terms
                 // bm = Q[i-1] (x) qRest[i-1], bp = Q[i+1] (x) conj(qRest[i])
                 if (fl & F_PREV) {
