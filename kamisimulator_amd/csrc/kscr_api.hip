@@ -348,7 +348,8 @@ int launch_contact_step(kscr_handle* h, float dt, bool with_sdf) {
         if ((e = contact_step_limit(a, cp, c, h->stream)) != hipSuccess) return h->hip_fail(e, "step limit");
         int cur = 0;
         for (int k = 0; k < a.num_itr; k++) {
-            if ((e = contact_iterate(a, cp, c, cur, h->d_q.p, c.q2, h->stream)) != hipSuccess)
+            if ((e = contact_iterate(a, cp, c, cur, h->d_q.p, c.q2, h->d_packs.p, h->num_packs, h->block,
+                                     h->stream)) != hipSuccess)
                 return h->hip_fail(e, "contact iteration");
             std::swap(h->d_q.p, c.q2);  // the new orientations become the current ones
             cur ^= 1;

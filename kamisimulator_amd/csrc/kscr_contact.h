@@ -48,9 +48,10 @@ hipError_t contact_init(const StepArgs& a, ContactDev& c, hipStream_t s);
 hipError_t contact_detect(const StepArgs& a, const ContactParams& cp, ContactDev& c, bool rebuild, hipStream_t s);
 // recomputeStepLimitKernel
 hipError_t contact_step_limit(const StepArgs& a, const ContactParams& cp, ContactDev& c, hipStream_t s);
-// one Jacobi iteration: dx[cur] -> dx[cur^1], q_in -> q_out
+// One Jacobi iteration, dx[cur] -> dx[cur^1] and q_in -> q_out, over the packs
+// of whole strands (at most `block` vertices each, one lane per vertex).
 hipError_t contact_iterate(const StepArgs& a, const ContactParams& cp, ContactDev& c, int cur, const float4* q_in,
-                           float4* q_out, hipStream_t s);
+                           float4* q_out, const int2* packs, int num_packs, int block, hipStream_t s);
 // endItr from dx[cur]
 hipError_t contact_end(const StepArgs& a, ContactDev& c, int cur, hipStream_t s);
 // SDF pass over all non-root vertices

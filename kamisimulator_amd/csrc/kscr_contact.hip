@@ -5,8 +5,8 @@
 //   detection (when due)       buildAABBs, LBVH, query,   (YBSimulator.cu:424-457,
 //                              buildCollisionList          collision.cu:7-92)
 //   step_limit_kernel          recomputeStepLimitKernel    (collision.cu:94-131)
-//   numItr x position_kernel   cosseratItr with IPC barrier and friction
-//            orientation_kernel quaternionLambdaItr        (cosserat.cu:25-337)
+//   numItr x iterate_kernel    cosseratItr with IPC barrier and friction,
+//                              then quaternionLambdaItr    (cosserat.cu:25-337)
 //   end_kernel                 endItr                      (iteration.cu:62-74)
 // The reference's races are replaced by ping-pong buffers (Jacobi snapshot)
 // and its atomically ordered contact lists by ascending lists, so results are
@@ -242,69 +242,101 @@ __device__ inline void segment_sums(const StepArgs& a, const ContactParams& cp, 
     for (int k2 = 1; k2 < nc; k2 += 2) contact_terms(a, cp, dx, i, L[k2], damp, s.f1, s.H1, s.f2b, s.H2b);
 }
 
-__global__ void position_kernel(StepArgs a, ContactParams cp, int n, const float* __restrict__ dx,
-                                const float4* __restrict__ q, const float* __restrict__ y,
-                                const int* __restrict__ num_cols,
-                                const int* __restrict__ cols, const float* __restrict__ max_step,
-                                float* __restrict__ dx_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 mi = a.mat[i];
-    const f3 dxi = ld3(dx, i);
-    if (mi.x == 0.f) {
-        st3(dx_out, i, dxi);
-        return;
-    }
+// One VBD iteration for a pack of whole strands, one lane per vertex:
+// cosseratItr with contacts (the position solve), then quaternionLambdaItr
+// (the orientation solve) on the new positions of the same iteration.  The
+// stretch and contact sums of segment (i, i+1) are formed once, by lane i;
+// vertex i+1's share reaches lane i+1 through LDS and is added there in the
+// order the sequential sum adds it (own segment, then the previous one).
+// The previous iterate (dx, q) is read from global memory: contact partners
+// lie in other workgroups (Jacobi snapshot).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void iterate_kernel(StepArgs a, ContactParams cp, const int2* __restrict__ packs,
+                                                        const float* __restrict__ dx, const float4* __restrict__ q_in,
+                                                        const float* __restrict__ y, const int* __restrict__ num_cols,
+                                                        const int* __restrict__ cols,
+                                                        const float* __restrict__ max_step,
+                                                        float* __restrict__ dx_out, float4* __restrict__ q_out) {
+    __shared__ float s_f2[3][BLOCK];
+    __shared__ float s_H2[6][BLOCK];
+    __shared__ float s_lim[BLOCK];
+    __shared__ float s_dxn[3][BLOCK];
+    const int2 pk = packs[blockIdx.x];
+    const int t = threadIdx.x;
+    const int i = pk.x + t;
+    const bool act = i < pk.y;
+    const float4 mi = act ? a.mat[i] : make_float4(0.f, 1.f, 0.f, 0.f);
+    const uint32_t fl = act ? __float_as_uint(mi.w) : 0u;
+    const bool free = act && mi.x != 0.f;
+    const f3 dxi = act ? ld3(dx, i) : f3{0, 0, 0};
     const float h = a.h;
     const float damp = a.damping / h;
-    const float a_in = 1.0f / (mi.x * h * h);
     SegSums s;
-    s.H0 = s3{{a_in, a_in, a_in, a_in, a_in, a_in}};
-    s.f0 = (1.0f / (h * h * mi.x)) * (ld3(y, i) - dxi);
-    segment_sums(a, cp, dx, q, num_cols, cols, max_step, i, damp, s);
-    f3 f = s.f0 + s.f1;
-    s3 H = s.H0;
-    sadd(H, s.H1);
-    float limit = s.limit;
-    if (__float_as_uint(mi.w) & FLAG_PREV) {
-        SegSums t;
-        t.f0 = f3{0, 0, 0};
-        t.H0 = s3{{0, 0, 0, 0, 0, 0}};
-        segment_sums(a, cp, dx, q, num_cols, cols, max_step, i - 1, damp, t);
-        const f3 f2 = t.f2a + t.f2b;
-        s3 H2 = t.H2a;
-        sadd(H2, t.H2b);
-        limit = gmin(limit, t.limit);
-        f = f + f2;
-        sadd(H, H2);
+    s.H0 = s3{{0, 0, 0, 0, 0, 0}};
+    s.f0 = f3{0, 0, 0};
+    if (free) {  // the vertex's inertia terms open its sum
+        const float a_in = 1.0f / (mi.x * h * h);
+        s.H0 = s3{{a_in, a_in, a_in, a_in, a_in, a_in}};
+        s.f0 = (1.0f / (h * h * mi.x)) * (ld3(y, i) - dxi);
     }
-    f3 out = dxi + a.acceleration_ratio * sym_solve(H.d[0], H.d[1], H.d[2], H.d[3], H.d[4], H.d[5], f);
-    // A contact Hessian can make H numerically singular in FP32 (the adjugate
-    // determinant cancels to 0); the reference's step is then inf/NaN, which
-    // no step limit removes.  Such a step is skipped.
-    if (!isfinite(out.x) || !isfinite(out.y) || !isfinite(out.z)) {
-        st3(dx_out, i, dxi);  // the previous iterate, as it is
-        return;
+    if (act) {
+        segment_sums(a, cp, dx, q_in, num_cols, cols, max_step, i, damp, s);
+    } else {
+        s.f1 = s.f2a = s.f2b = f3{0, 0, 0};
+        s.H1 = s.H2a = s.H2b = s3{{0, 0, 0, 0, 0, 0}};
+        s.limit = INFINITY;
     }
-    const float l = length(out);
-    if (l > limit && l > 0) out = out * (limit / l);
-    st3(dx_out, i, out);
-}
-
-__global__ void orientation_kernel(StepArgs a, int n, const float* __restrict__ dxn, const float4* __restrict__ q_in,
-                                   float4* __restrict__ q_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 mi = a.mat[i];
-    const uint32_t fl = __float_as_uint(mi.w);
+    {
+        const f3 f2 = s.f2a + s.f2b;
+        s3 H2 = s.H2a;
+        sadd(H2, s.H2b);
+        s_f2[0][t] = f2.x;
+        s_f2[1][t] = f2.y;
+        s_f2[2][t] = f2.z;
+        for (int c = 0; c < 6; c++) s_H2[c][t] = H2.d[c];
+        s_lim[t] = s.limit;
+    }
+    __syncthreads();
+    f3 out = dxi;  // a pinned vertex keeps its iterate
+    if (free) {
+        f3 f = s.f0 + s.f1;
+        s3 H = s.H0;
+        sadd(H, s.H1);
+        float limit = s.limit;
+        if (fl & FLAG_PREV) {
+            const f3 f2{s_f2[0][t - 1], s_f2[1][t - 1], s_f2[2][t - 1]};
+            s3 H2;
+            for (int c = 0; c < 6; c++) H2.d[c] = s_H2[c][t - 1];
+            limit = gmin(limit, s_lim[t - 1]);
+            f = f + f2;
+            sadd(H, H2);
+        }
+        out = dxi + a.acceleration_ratio * sym_solve(H.d[0], H.d[1], H.d[2], H.d[3], H.d[4], H.d[5], f);
+        // A contact Hessian can make H numerically singular in FP32 (the
+        // adjugate determinant cancels to 0); the reference's step is then
+        // inf/NaN, which no step limit removes.  Such a step is skipped.
+        if (!isfinite(out.x) || !isfinite(out.y) || !isfinite(out.z)) {
+            out = dxi;
+        } else {
+            const float l = length(out);
+            if (l > limit && l > 0) out = out * (limit / l);
+        }
+    }
+    if (act) st3(dx_out, i, out);
+    s_dxn[0][t] = out.x;
+    s_dxn[1][t] = out.y;
+    s_dxn[2][t] = out.z;
+    __syncthreads();
+    if (!act) return;
     const f4 q = to4(q_in[i]);
     f4 qn = q;
     if ((fl & FLAG_NEXT) && !(fl & FLAG_FIX_ORI)) {
         const f4 qm = (fl & FLAG_PREV) ? to4(q_in[i - 1]) : f4{0, 0, 0, 1};
         const f4 qp = (fl & FLAG_NEXT_ORI) ? to4(q_in[i + 1]) : f4{0, 0, 0, 1};
         const f4 qrp = (fl & FLAG_PREV) ? to4(a.q_rest[i - 1]) : f4{0, 0, 0, 1};
-        qn = orientation_update(ld3(a.x, i), ld3(a.x, i + 1), ld3(dxn, i), ld3(dxn, i + 1), mi.y, 1.0f / mi.y, mi.z, q,
-                                qm, qp, qrp, to4(a.q_rest[i]), fl);
+        const f3 dxj{s_dxn[0][t + 1], s_dxn[1][t + 1], s_dxn[2][t + 1]};
+        qn = orientation_update(ld3(a.x, i), ld3(a.x, i + 1), out, dxj, mi.y, 1.0f / mi.y, mi.z, q, qm, qp, qrp,
+                                to4(a.q_rest[i]), fl);
     }
     q_out[i] = make_float4(qn.x, qn.y, qn.z, qn.w);
 }
@@ -391,10 +423,24 @@ hipError_t contact_step_limit(const StepArgs& a, const ContactParams& cp, Contac
 }
 
 hipError_t contact_iterate(const StepArgs& a, const ContactParams& cp, ContactDev& c, int cur, const float4* q_in,
-                           float4* q_out, hipStream_t s) {
-    hipLaunchKernelGGL(position_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, cp, c.n, c.dx[cur], q_in, c.y,
-                       c.num_cols, c.cols, c.max_step, c.dx[cur ^ 1]);
-    hipLaunchKernelGGL(orientation_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, c.n, c.dx[cur ^ 1], q_in, q_out);
+                           float4* q_out, const int2* packs, int num_packs, int block, hipStream_t s) {
+    if (num_packs <= 0) return hipSuccess;
+    const dim3 grid(num_packs);
+    switch (block) {
+#define KSCR_CASE(B)                                                                                              \
+    case B:                                                                                                       \
+        hipLaunchKernelGGL(iterate_kernel<B>, grid, dim3(B), 0, s, a, cp, packs, c.dx[cur], q_in, c.y, c.num_cols, \
+                           c.cols, c.max_step, c.dx[cur ^ 1], q_out);                                           \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
