@@ -73,13 +73,48 @@ KS_FN float dot(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); 
 KS_FN f3 cross(f3 a, f3 b) {
     return f3{fmaf(a.y, b.z, -(b.y * a.z)), fmaf(a.z, b.x, -(b.z * a.x)), fmaf(a.x, b.y, -(b.x * a.y))};
 }
-KS_FN float length(f3 a) { return sqrtf(dot(a, a)); }
-KS_FN f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+// sqrtf(x), and 1.0f / sqrtf(x), with fewer operations on the device.
+// sqrt_rn is hipcc's correctly rounded sqrtf without its final special-case
+// select: inputs below 2^-64 scaled by 2^32 (exact) and the root by 2^-16,
+// the hardware root s moved to s -+ 1 ulp by the signs of the FMA residuals
+// x - s' s; zeros, infinities and NaN come out of the fix-up as they are.
+// rcp_rn is the hardware reciprocal with one FMA correction, correctly
+// rounded for every argument in [2^-126, 2^126).  Over all 2^32 inputs,
+// sqrt_rn equals sqrtf and rsqrt_rn equals 1.0f / sqrtf except at +-0 and
+// +inf, where it gives NaN for infinity or zero -- and a vector scaled by it
+// is NaN either way (0 * inf, inf * 0).
+KS_FN float sqrt_rn(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const bool small = x < 0x1p-64f;
+    const float xs = small ? x * 0x1p32f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float rm = fmaf(-sm, s, xs);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rp = fmaf(-sp, s, xs);
+    const float t0 = rm <= 0.0f ? sm : s;
+    const float t = rp > 0.0f ? sp : t0;
+    return small ? t * 0x1p-16f : t;
+#else
+    return sqrtf(x);
+#endif
+}
+KS_FN float rsqrt_rn(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = sqrt_rn(x);
+    const float y = __builtin_amdgcn_rcpf(s);
+    return fmaf(fmaf(-y, s, 1.0f), y, y);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+KS_FN float length(f3 a) { return sqrt_rn(dot(a, a)); }
+KS_FN f3 normalize(f3 a) { return a * rsqrt_rn(dot(a, a)); }
 
 KS_FN float dot(f4 a, f4 b) { return fmaf(a.w, b.w, fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x))); }
-KS_FN float length(f4 a) { return sqrtf(dot(a, a)); }
+KS_FN float length(f4 a) { return sqrt_rn(dot(a, a)); }
 KS_FN f4 normalize(f4 a) {
-    float s = 1.0f / sqrtf(dot(a, a));
+    float s = rsqrt_rn(dot(a, a));
     return f4{a.x * s, a.y * s, a.z * s, a.w * s};
 }
 KS_FN f4 operator+(f4 a, f4 b) { return f4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }

@@ -138,7 +138,7 @@ __device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, f
     }
     const float lambda = length(vv) + length(b);
     const f4 w = rmul_e1(rmul_pure(vv, b)) + lambda * b;
-    inv_norm = 1.0f / sqrtf(dot(w, w));
+    inv_norm = rsqrt_rn(dot(w, w));
     return f4{w.x * inv_norm, w.y * inv_norm, w.z * inv_norm, w.w * inv_norm};
 }
 __device__ inline f4 orientation_update(f3 x, f3 xj, f3 dxn, f3 dxj, float lr, float ilr, float ks, f4 q, f4 qm,
