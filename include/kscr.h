@@ -296,6 +296,17 @@ int kscr_lbvh_get(kscr_lbvh* b, uint32_t* codes, uint32_t* ids, uint32_t* links,
 
 /* Copies the last error message (NUL terminated) into buf. */
 int kscr_last_error(const kscr_handle* h, char* buf, size_t n);
+
+/* Self-check of the fast correctly rounded arithmetic the kernels use in
+ * place of IEEE division and square root, run on `device` (< 0: current)
+ * against the compiler's IEEE sequences; no reference counterpart.  counts
+ * receives the number of differing results: [0] sqrt over all 2^32 float
+ * bit patterns (expected 0); [1] 1/sqrt over all 2^32 (expected 3: +0, -0,
+ * +inf, where a vector scaled by either is NaN); [2] the reverse sweep's
+ * clamped square root and reciprocal over all non-negative patterns
+ * (expected 0); [3] the division by a rest length over all 2^32 numerators
+ * for each of `divisors` (expected 0).  Takes about a second per divisor. */
+int kscr_selftest_rounding(int device, const float* divisors, int num_divisors, unsigned long long counts[4]);
 /* Library-wide message of the last kscr_create failure. */
 const char* kscr_create_error(void);
 

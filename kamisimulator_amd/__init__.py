@@ -5,6 +5,6 @@ this package holds its Python binding, the solver mirror of the reference's
 Solver plugin interface, and the synthetic workload generators.
 """
 from . import scenes  # noqa: F401
-from ._lib import KscrError, KscrParams, Sim, default_params, load, mesh_sdf  # noqa: F401
+from ._lib import KscrError, KscrParams, Sim, default_params, load, mesh_sdf, selftest_rounding  # noqa: F401
 
 __all__ = ["Sim", "KscrParams", "KscrError", "default_params", "load", "scenes"]

@@ -95,6 +95,7 @@ SIGNATURES = {
     "kscr_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
     "kscr_set_sdf": (C.c_int, [_P, _IP, _FP, _FP, _FP, _FP, C.c_float, _FP, _FP]),
     "kscr_enable_sdf": (C.c_int, [_P, C.c_int]),
+    "kscr_selftest_rounding": (C.c_int, [C.c_int, _FP, C.c_int, C.POINTER(C.c_ulonglong)]),
     "kscr_mesh_sdf": (C.c_int, [C.c_int, _FP, C.c_int, _IP, C.c_int, _IP, _FP, _FP, _FP, _FP, C.c_char_p,
                                 C.c_size_t]),
     "kscr_set_sdf_from_mesh": (C.c_int, [_P, _FP, C.c_int, _IP, C.c_int, _IP, C.c_float, _FP, _FP]),
@@ -182,6 +183,20 @@ def params_to_dict(p):
         val = getattr(p, name)
         d[name] = list(val) if name == "gravity" else val
     return d
+
+
+def selftest_rounding(divisors=(1e-3,), device=-1):
+    """Differing-result counts of the kernels' fast correctly rounded
+    arithmetic against IEEE on the GPU (kscr_selftest_rounding): sqrt,
+    1/sqrt, the reverse sweep's clamped sqrt and reciprocal, division by each
+    divisor."""
+    lib = load()
+    d = np.ascontiguousarray(divisors, dtype=np.float32)
+    counts = (C.c_ulonglong * 4)()
+    rc = lib.kscr_selftest_rounding(int(device), fptr(d), len(d), counts)
+    if rc:
+        raise KscrError(rc, "kscr_selftest_rounding failed")
+    return [int(c) for c in counts]
 
 
 def mesh_sdf(vertices, triangles, dim, device=-1):

@@ -37,29 +37,6 @@ __device__ inline f4 rot_e1_vjp(f4 q, f3 rb) {
     return f4{xb, yb, zb, wb};
 }
 
-// |a| and 1 / a of the reverse sweep, for arguments clamped into the range
-// where the hardware square root and reciprocal with their FMA corrections
-// are correctly rounded (compared with sqrtf and 1.0f / x over every
-// significand at exponents -62 to 63: equal): ~4 and ~4 VALU operations
-// instead of ~16 and ~11, branch free.  The clamp, [2^-62, 2^62] for the
-// squared length, is part of the adjoint's definition (the oracle applies it
-// too); it changes nothing for a rod's lengths and only keeps a zero length
-// from dividing by zero.
-__device__ inline float clamp_sq(float x) { return __builtin_amdgcn_fmed3f(x, 0x1p-62f, 0x1p62f); }
-__device__ inline float sqrt_cr(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
-    const float rm = fmaf(-sm, s, x);
-    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
-    const float rp = fmaf(-sp, s, x);
-    const float t = rm <= 0.0f ? sm : s;
-    return rp > 0.0f ? sp : t;
-}
-__device__ inline float rcp_cr(float b) {
-    const float y = __builtin_amdgcn_rcpf(b);
-    return fmaf(fmaf(-y, b, 1.0f), y, y);
-}
-
 // a (x) conj(e1) = -(a (x) e1)
 KS_FN f4 rmul_conj_e1(f4 a) { return f4{-a.w, -a.z, a.y, a.x}; }
 
