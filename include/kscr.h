@@ -240,10 +240,13 @@ int kscr_set_material(kscr_handle* h, const float* k_stretch, const float* q_res
 int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every);
 int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar, float* k_stretch_bar,
                           float* q_rest_bar);
-/* Device time (ms) of the last kscr_adjoint_backward: re-recording the
- * segments and the reverse sweeps, measured with HIP events on the handle's
- * stream. */
-int kscr_adjoint_last_time(const kscr_handle* h, float* record_ms, float* sweep_ms);
+/* Device times (ms) of the last kscr_adjoint_backward, from HIP events:
+ * re-recording the segments (summed over the segments, on the library's
+ * recording stream), the reverse sweeps (summed, on the handle's stream) and
+ * the whole backward pass from its start to the restored end state (the
+ * re-recording of segment j - 1 overlaps the sweep of segment j, so this is
+ * less than the sum).  Any pointer may be NULL. */
+int kscr_adjoint_last_time(const kscr_handle* h, float* record_ms, float* sweep_ms, float* backward_ms);
 
 /* ---- Batched form (no counterpart in the reference: its closest analogue is
  * CantileverTestSolver, which steps independent solvers one after another,

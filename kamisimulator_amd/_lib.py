@@ -121,7 +121,7 @@ SIGNATURES = {
     "kscr_set_material": (C.c_int, [_P, _FP, _FP]),
     "kscr_adjoint_forward": (C.c_int, [_P, C.c_float, C.c_int, C.c_int]),
     "kscr_adjoint_backward": (C.c_int, [_P, _FP, _FP, _FP, _FP]),
-    "kscr_adjoint_last_time": (C.c_int, [_P, _FP, _FP]),
+    "kscr_adjoint_last_time": (C.c_int, [_P, _FP, _FP, _FP]),
     "kscr_lbvh_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "kscr_lbvh_destroy": (None, [_P]),
     "kscr_lbvh_compute": (C.c_int, [_P, _FP, C.c_int]),
@@ -409,11 +409,13 @@ class Sim:
         return ksb, qrb
 
     def adjoint_time(self):
-        """(record ms, reverse sweep ms) of the last adjoint_backward."""
+        """(re-record ms, reverse sweep ms, whole backward ms) of the last
+        adjoint_backward; re-recording overlaps the sweeps."""
         r = C.c_float(0)
         w = C.c_float(0)
-        self._check(self.lib.kscr_adjoint_last_time(self.h, C.byref(r), C.byref(w)))
-        return r.value, w.value
+        b = C.c_float(0)
+        self._check(self.lib.kscr_adjoint_last_time(self.h, C.byref(r), C.byref(w), C.byref(b)))
+        return r.value, w.value, b.value
 
     def autotune(self, dt, steps=3):
         best = C.c_int(0)

@@ -125,7 +125,16 @@ struct AdjointRun {
     // the forward pass records the last segment's tape itself
     bool last_recorded = false;
     std::vector<float> last_h_of, last_lh_of;  // h, last_h of the last segment's steps
-    float record_ms = 0, sweep_ms = 0;
+    float record_ms = 0, sweep_ms = 0, backward_ms = 0;
+    // the backward pass re-records segment j - 1 on this stream while the
+    // handle's stream sweeps segment j (tapes in two slots by segment parity)
+    hipStream_t rec_stream = nullptr;
+    AdjointRun() = default;
+    AdjointRun(const AdjointRun&) = delete;
+    AdjointRun& operator=(const AdjointRun&) = delete;
+    ~AdjointRun() {
+        if (rec_stream) (void)hipStreamDestroy(rec_stream);
+    }
 };
 }  // namespace
 
@@ -419,7 +428,7 @@ int upload_material(kscr_handle* h) {
 }
 
 // Copies the dynamic state (x, v, u, q) into checkpoint slot k (save) or back.
-hipError_t checkpoint_copy(kscr_handle* h, size_t k, bool save) {
+hipError_t checkpoint_copy(kscr_handle* h, size_t k, bool save, hipStream_t stream) {
     AdjointRun& A = h->adj;
     const size_t n = h->n, n3 = 3 * n;
     const hipMemcpyKind kind = hipMemcpyDeviceToDevice;
@@ -429,15 +438,15 @@ hipError_t checkpoint_copy(kscr_handle* h, size_t k, bool save) {
     float4* q = A.ck_q.p + k * n;
     hipError_t e;
     if (save) {
-        if ((e = hipMemcpyAsync(x, h->d_x.p, n3 * 4, kind, h->stream)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(v, h->d_v.p, n3 * 4, kind, h->stream)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(u, h->d_u.p, n3 * 4, kind, h->stream)) != hipSuccess) return e;
-        return hipMemcpyAsync(q, h->d_q.p, n * 16, kind, h->stream);
+        if ((e = hipMemcpyAsync(x, h->d_x.p, n3 * 4, kind, stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(v, h->d_v.p, n3 * 4, kind, stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(u, h->d_u.p, n3 * 4, kind, stream)) != hipSuccess) return e;
+        return hipMemcpyAsync(q, h->d_q.p, n * 16, kind, stream);
     }
-    if ((e = hipMemcpyAsync(h->d_x.p, x, n3 * 4, kind, h->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(h->d_v.p, v, n3 * 4, kind, h->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(h->d_u.p, u, n3 * 4, kind, h->stream)) != hipSuccess) return e;
-    return hipMemcpyAsync(h->d_q.p, q, n * 16, kind, h->stream);
+    if ((e = hipMemcpyAsync(h->d_x.p, x, n3 * 4, kind, stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(h->d_v.p, v, n3 * 4, kind, stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(h->d_u.p, u, n3 * 4, kind, stream)) != hipSuccess) return e;
+    return hipMemcpyAsync(h->d_q.p, q, n * 16, kind, stream);
 }
 
 AdjointRun::Clock clock_of(const kscr_handle* h) { return AdjointRun::Clock{h->h, h->last_h, h->time, h->step_counter}; }
@@ -445,10 +454,11 @@ void set_clock(kscr_handle* h, const AdjointRun::Clock& c) {
     h->h = c.h, h->last_h = c.last_h, h->time = c.time, h->step_counter = c.counter;
 }
 
-// Tape of one checkpoint segment: `every` steps of `substeps` substeps.
+// Tapes of two checkpoint segments (slots 0 and 1), each `every` steps of
+// `substeps` substeps.
 int reserve_tape(kscr_handle* h) {
     AdjointRun& A = h->adj;
-    const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = (size_t)A.every;
+    const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = 2 * (size_t)A.every;
     const size_t K = (size_t)A.num_itr;
     const size_t per_sub = S * n * every, per_dx = S * I1 * n * every, per_q = S * I1 * n * every;
     const size_t per_ilw = S * K * n * every;
@@ -462,10 +472,11 @@ int reserve_tape(kscr_handle* h) {
     return KSCR_OK;
 }
 
-// Tape slot of step j of a segment.
-TapeArgs tape_of(kscr_handle* h, size_t j) {
+// Tape of step j of the segment in tape slot `slot`.
+TapeArgs tape_of(kscr_handle* h, size_t slot, size_t j) {
     AdjointRun& A = h->adj;
     const size_t n = h->n, S = (size_t)A.substeps, K = (size_t)A.num_itr;
+    j += slot * (size_t)A.every;
     TapeArgs t{};
     t.x = A.t_x.p + j * S * n * 3;
     t.y = A.t_y.p + j * S * n * 3;
@@ -1074,7 +1085,7 @@ int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_ev
     A.clocks.clear();
     for (int s = 0; s < nsteps; s++) {
         if (s % checkpoint_every == 0) {
-            if ((e = checkpoint_copy(h, (size_t)(s / checkpoint_every), true)) != hipSuccess)
+            if ((e = checkpoint_copy(h, (size_t)(s / checkpoint_every), true, h->stream)) != hipSuccess)
                 return h->hip_fail(e, "save checkpoint");
             A.clocks.push_back(clock_of(h));
         }
@@ -1085,7 +1096,7 @@ int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_ev
         const StepArgs sa = step_args(h, dt, false);
         A.last_h_of[s - last_s0] = sa.h;
         A.last_lh_of[s - last_s0] = sa.last_h;
-        if ((e = launch_step_record(sa, tape_of(h, (size_t)(s - last_s0)), h->d_packs.p, h->num_packs, h->block,
+        if ((e = launch_step_record(sa, tape_of(h, 0, (size_t)(s - last_s0)), h->d_packs.p, h->num_packs, h->block,
                                     h->stream)) != hipSuccess)
             return h->hip_fail(e, "record kernel launch");
     }
@@ -1106,7 +1117,6 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     hipError_t e;
     if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
     const size_t n = h->n, n3 = 3 * n;
-    const size_t every = (size_t)A.every;
     if ((rc = reserve_tape(h)) != KSCR_OK) return rc;
     if (A.qb.n != n) {
         if ((e = A.xb.alloc(n3)) != hipSuccess || (e = A.vb.alloc(n3)) != hipSuccess || (e = A.ub.alloc(n3)) != hipSuccess ||
@@ -1125,8 +1135,11 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
         return h->hip_fail(e, "seed adjoint state");
     const int nseg = (int)A.clocks.size();
     const size_t end_slot = (size_t)nseg;
-    if ((e = checkpoint_copy(h, end_slot, true)) != hipSuccess) return h->hip_fail(e, "save end state");
+    if ((e = checkpoint_copy(h, end_slot, true, h->stream)) != hipSuccess) return h->hip_fail(e, "save end state");
     const AdjointRun::Clock end_clock = clock_of(h);
+    if (!A.rec_stream && (e = hipStreamCreateWithFlags(&A.rec_stream, hipStreamNonBlocking)) != hipSuccess)
+        return h->hip_fail(e, "hipStreamCreate");
+    hipStream_t ms = h->stream, rs = A.rec_stream;
 
     AdjointArgs aa{};
     aa.mat = h->d_mat.p;
@@ -1145,72 +1158,108 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     aa.qb = A.qb.p;
     aa.ksb = A.ksb.p;
     aa.qrb = A.qrb.p;
-    std::vector<hipEvent_t> ev(3 * (size_t)nseg, nullptr);
+    // events: per segment record begin/end (record stream) and sweep
+    // begin/end (handle stream); then start, record-stream done, end
+    std::vector<hipEvent_t> ev(4 * (size_t)nseg + 3, nullptr);
     for (auto& x : ev)
         if ((e = hipEventCreate(&x)) != hipSuccess) {
             rc = h->hip_fail(e, "hipEventCreate");
             break;
         }
-    std::vector<float> hs(every), lhs(every);
+    auto rec_begin = [&](int seg) { return ev[4 * seg]; };
+    auto rec_end = [&](int seg) { return ev[4 * seg + 1]; };
+    auto sweep_begin = [&](int seg) { return ev[4 * seg + 2]; };
+    auto sweep_end = [&](int seg) { return ev[4 * seg + 3]; };
+    hipEvent_t ev_start = ev[4 * nseg], ev_rec_done = ev[4 * nseg + 1], ev_end = ev[4 * nseg + 2];
+    // Segment j's tape lives in slot (nseg - 1 - j) % 2: the last segment in
+    // slot 0, where the forward pass recorded it.  Re-recording segment j - 1
+    // (record stream) overlaps the sweep of segment j (handle stream); it
+    // waits for the sweep of segment j + 1, the last user of its slot.
+    auto slot_of = [&](int seg) { return (size_t)((nseg - 1 - seg) % 2); };
+    std::vector<std::vector<float>> hs(nseg), lhs(nseg);
+    std::vector<bool> rerecorded(nseg, false);
+    if (rc == KSCR_OK) {
+        (void)hipEventRecord(ev_start, ms);
+        if ((e = hipStreamWaitEvent(rs, ev_start, 0)) != hipSuccess) rc = h->hip_fail(e, "hipStreamWaitEvent");
+    }
     for (int seg = nseg - 1; seg >= 0 && rc == KSCR_OK; seg--) {
         const int s0 = seg * A.every, s1 = std::min(A.nsteps, s0 + A.every);
-        (void)hipEventRecord(ev[3 * seg], h->stream);
         if (seg == nseg - 1 && A.last_recorded) {
-            // recorded by the forward pass
-            hs = A.last_h_of;
-            lhs = A.last_lh_of;
+            hs[seg] = A.last_h_of;  // recorded by the forward pass
+            lhs[seg] = A.last_lh_of;
         } else {
-            if ((e = checkpoint_copy(h, (size_t)seg, false)) != hipSuccess) {
+            rerecorded[seg] = true;
+            hs[seg].assign(A.every, 0.f);
+            lhs[seg].assign(A.every, 0.f);
+            if (seg + 2 <= nseg - 1 && (e = hipStreamWaitEvent(rs, sweep_end(seg + 2), 0)) != hipSuccess) {
+                rc = h->hip_fail(e, "hipStreamWaitEvent");
+                break;
+            }
+            (void)hipEventRecord(rec_begin(seg), rs);
+            if ((e = checkpoint_copy(h, (size_t)seg, false, rs)) != hipSuccess) {
                 rc = h->hip_fail(e, "restore checkpoint");
                 break;
             }
             set_clock(h, A.clocks[seg]);
             for (int s = s0; s < s1; s++) {
                 const StepArgs sa = step_args(h, A.dt, false);
-                hs[s - s0] = sa.h;
-                lhs[s - s0] = sa.last_h;
-                if ((e = launch_step_record(sa, tape_of(h, (size_t)(s - s0)), h->d_packs.p, h->num_packs, h->block,
-                                            h->stream)) != hipSuccess) {
+                hs[seg][s - s0] = sa.h;
+                lhs[seg][s - s0] = sa.last_h;
+                if ((e = launch_step_record(sa, tape_of(h, slot_of(seg), (size_t)(s - s0)), h->d_packs.p,
+                                            h->num_packs, h->block, rs)) != hipSuccess) {
                     rc = h->hip_fail(e, "record kernel launch");
                     break;
                 }
             }
+            (void)hipEventRecord(rec_end(seg), rs);
+            if (rc == KSCR_OK && (e = hipStreamWaitEvent(ms, rec_end(seg), 0)) != hipSuccess)
+                rc = h->hip_fail(e, "hipStreamWaitEvent");
         }
-        (void)hipEventRecord(ev[3 * seg + 1], h->stream);
+        (void)hipEventRecord(sweep_begin(seg), ms);
         for (int s = s1 - 1; s >= s0 && rc == KSCR_OK; s--) {
-            aa.tape = tape_of(h, (size_t)(s - s0));
-            aa.h = hs[s - s0];
-            aa.last_h = lhs[s - s0];
-            if ((e = launch_step_adjoint(aa, h->d_packs.p, h->num_packs, h->block, h->stream)) != hipSuccess)
+            aa.tape = tape_of(h, slot_of(seg), (size_t)(s - s0));
+            aa.h = hs[seg][s - s0];
+            aa.last_h = lhs[seg][s - s0];
+            if ((e = launch_step_adjoint(aa, h->d_packs.p, h->num_packs, h->block, ms)) != hipSuccess)
                 rc = h->hip_fail(e, "adjoint kernel launch");
         }
-        (void)hipEventRecord(ev[3 * seg + 2], h->stream);
+        (void)hipEventRecord(sweep_end(seg), ms);
     }
+    // the record stream's work (it writes the state arrays) ends before the
+    // end state is restored
+    (void)hipEventRecord(ev_rec_done, rs);
+    (void)hipStreamWaitEvent(ms, ev_rec_done, 0);
     if (nseg > 1) A.last_recorded = false;  // earlier segments reused the tape
-    if ((e = checkpoint_copy(h, end_slot, false)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "restore end state");
+    if ((e = checkpoint_copy(h, end_slot, false, ms)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "restore end state");
     set_clock(h, end_clock);
+    (void)hipEventRecord(ev_end, ms);
     if (rc == KSCR_OK &&
-        ((e = hipMemcpyAsync(k_stretch_bar, A.ksb.p, n * 4, hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
-         (e = hipMemcpyAsync(q_rest_bar, A.qrb.p, n * 16, hipMemcpyDeviceToHost, h->stream)) != hipSuccess))
+        ((e = hipMemcpyAsync(k_stretch_bar, A.ksb.p, n * 4, hipMemcpyDeviceToHost, ms)) != hipSuccess ||
+         (e = hipMemcpyAsync(q_rest_bar, A.qrb.p, n * 16, hipMemcpyDeviceToHost, ms)) != hipSuccess))
         rc = h->hip_fail(e, "download gradient");
-    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "hipStreamSynchronize");
-    A.record_ms = A.sweep_ms = 0;
-    for (int seg = 0; seg < nseg && rc == KSCR_OK; seg++) {
-        float r = 0, w = 0;
-        (void)hipEventElapsedTime(&r, ev[3 * seg], ev[3 * seg + 1]);
-        (void)hipEventElapsedTime(&w, ev[3 * seg + 1], ev[3 * seg + 2]);
-        A.record_ms += r;
-        A.sweep_ms += w;
+    if ((e = hipStreamSynchronize(ms)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "hipStreamSynchronize");
+    if ((e = hipStreamSynchronize(rs)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "hipStreamSynchronize");
+    A.record_ms = A.sweep_ms = A.backward_ms = 0;
+    if (rc == KSCR_OK) {
+        for (int seg = 0; seg < nseg; seg++) {
+            float r = 0, w = 0;
+            if (rerecorded[seg]) (void)hipEventElapsedTime(&r, rec_begin(seg), rec_end(seg));
+            (void)hipEventElapsedTime(&w, sweep_begin(seg), sweep_end(seg));
+            A.record_ms += r;
+            A.sweep_ms += w;
+        }
+        (void)hipEventElapsedTime(&A.backward_ms, ev_start, ev_end);
     }
     for (auto& x : ev)
         if (x) (void)hipEventDestroy(x);
     return rc;
 }
 
-int kscr_adjoint_last_time(const kscr_handle* h, float* record_ms, float* sweep_ms) {
+int kscr_adjoint_last_time(const kscr_handle* h, float* record_ms, float* sweep_ms, float* backward_ms) {
     if (!h) return KSCR_ERR_INVALID_ARGUMENT;
     if (record_ms) *record_ms = h->adj.record_ms;
     if (sweep_ms) *sweep_ms = h->adj.sweep_ms;
+    if (backward_ms) *backward_ms = h->adj.backward_ms;
     return KSCR_OK;
 }
 

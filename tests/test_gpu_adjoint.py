@@ -111,8 +111,9 @@ def test_differentiation_workload_strand_subset():
     tips = (sc.strand_offsets[1:] - 1).astype(np.int32)
     targets = sc.positions[tips].astype(np.float64) + np.array([1e-3, -3e-3, 0.0])
     _, ksb_g, qrb_g = _gpu_gradient(g, nsteps, K, tips, targets)
-    rec, sweep = g.adjoint_time()
-    print(f"[gpu-adjoint] record {rec:.2f} ms, reverse sweep {sweep:.2f} ms", flush=True)
+    rec, sweep, bwd = g.adjoint_time()
+    print(f"[gpu-adjoint] record {rec:.2f} ms, reverse sweep {sweep:.2f} ms, backward {bwd:.2f} ms", flush=True)
+    assert 0 < bwd <= rec + sweep + 5.0  # re-recording overlaps the sweeps
     nv_sub = int(sc.strand_offsets[sub])
     sc_sub = scenes.Scene("subset", sc.positions[:nv_sub], sc.strand_offsets[: sub + 1].copy(), material=sc.material)
     o = OracleSim(p, sc_sub.positions, sc_sub.strand_offsets, precision=32)
