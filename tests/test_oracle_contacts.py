@@ -224,3 +224,27 @@ def test_dense_groom_with_contacts_stays_finite():
         x, v = o.get_pos_vel()
         assert np.isfinite(x).all() and np.isfinite(v).all()
         assert o.num_pairs() < 4 * sc.num_verts and o.error() == 0
+
+
+def test_fp32_tracks_fp64_with_contacts():
+    """The FP64 error of the FP32 step with hair-hair contacts (config 2's
+    physics on dense instances; the HIP step equals the FP32 oracle bit for
+    bit): 100 steps of a 12-strand ribbon and a 6 x 6 bundle, strands inside
+    each other's contact barrier."""
+    dt = np.float32(1 / 600)
+    for sc, bound in ((scenes.ribbon(12, 64), 1e-2), (scenes.bundle(6, 6, 64), 3e-2)):
+        p = scenes.contact_params(default_params, sc.material)
+        o32 = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets, precision=32), sc)
+        o64 = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets, precision=64), sc)
+        o32.set_threads(8)
+        o64.set_threads(8)
+        seen = 0
+        for _ in range(100):
+            o32.solve_next_step(dt)
+            o64.solve_next_step(dt)
+            seen = max(seen, int(o32.contacts()[0].sum()))
+        p32, _ = o32.get_pos_vel()
+        p64, _ = o64.get_pos_vel(f64=True)
+        rel = float(np.abs(p32.astype(np.float64) - p64).max() / np.abs(p64).max())
+        print(f"{sc.name}: FP32 vs FP64 position rel {rel:.3e} after 100 steps, up to {seen} contacts")
+        assert seen > 0 and np.isfinite(rel) and rel < bound
