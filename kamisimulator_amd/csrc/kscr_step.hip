@@ -58,6 +58,8 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
     const int t = threadIdx.x;
     const int i = vbeg + t;
     const bool act = i < vend;
+    const int tm = t > 0 ? t - 1 : 0;
+    const int tp1 = t < BLOCK - 1 ? t + 1 : t;
 
     f3 x{0, 0, 0}, v{0, 0, 0}, u{0, 0, 0};
     f4 q{0, 0, 0, 1}, qr{0, 0, 0, 1}, qrp{0, 0, 0, 1};
@@ -133,7 +135,9 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
         for (int k = 0; k < a.num_itr; k++) {
             const int cur = k & 1;
             // ---- position solve, part 1: own segment (i, i+1)
-            f3 fo{0, 0, 0};
+            // fo stays -0 without a segment: x + (-0) == x for every float x
+            // (also x = -0), so it is added unconditionally below
+            f3 fo{-0.0f, -0.0f, -0.0f};
             if (hasNext) {
                 const f3 dxj{s_dx[cur][0][t + 1], s_dx[cur][1][t + 1], s_dx[cur][2][t + 1]};
                 const f3 c = ((xj - x) + (dxj - dx)) * invl - rotate_e1(q);
@@ -146,7 +150,7 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             f3 dxn = dx;
             if (free) {
                 f3 f = f_in * (y - dx);
-                if (hasNext) f = f + fo;
+                f = f + fo;
                 if (hasPrev) {
                     const float4 p = s_f[t - 1];
                     f = f + f3{p.x, p.y, p.z};
@@ -162,8 +166,10 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
             float inv_norm = 0.0f;
             if (rotates) {
                 const f3 dxj{s_dx[cur ^ 1][0][t + 1], s_dx[cur ^ 1][1][t + 1], s_dx[cur ^ 1][2][t + 1]};
-                const f4 qm = hasPrev ? to4(s_q[cur][t - 1]) : f4{0, 0, 0, 1};
-                const f4 qp = (fl & FLAG_NEXT_ORI) ? to4(s_q[cur][t + 1]) : f4{0, 0, 0, 1};
+                // read unconditionally (clamped into the pack); orientation_update
+                // uses qm only with FLAG_PREV and qp only with FLAG_NEXT_ORI
+                const f4 qm = to4(s_q[cur][tm]);
+                const f4 qp = to4(s_q[cur][tp1]);
                 qn = orientation_update(x, xj, dxn, dxj, lr, invl, ks, q, qm, qp, qrp, qr, fl, inv_norm);
             }
             s_q[cur ^ 1][t] = make_float4(qn.x, qn.y, qn.z, qn.w);
@@ -192,8 +198,14 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
     }
 }
 
+// At 512 threads the LDS of three workgroups fits a CU (6 waves per SIMD);
+// reg_waves keeps the register allocation within the 80 VGPRs that allow
+// those 6 waves (at 1024 threads the LDS admits one workgroup, 4 waves).
+constexpr int reg_waves(int block) { return block >= 1024 ? 4 : 6; }
+
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void step_fused_kernel(StepArgs a, const int2* __restrict__ packs) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves(BLOCK)))) void step_fused_kernel(
+    StepArgs a, const int2* __restrict__ packs) {
     __shared__ StepLds<BLOCK> lds;
     const int2 pk = packs[blockIdx.x];
     run_pack<BLOCK>(a, pk.x, pk.y, lds);
@@ -209,8 +221,8 @@ __global__ __launch_bounds__(BLOCK) void step_record_kernel(StepArgs a, TapeArgs
 // Batched form: every workgroup reads its groom's arguments (pointers,
 // substep lengths, gravity, SDF grid) from a device table.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void step_batch_kernel(const StepArgs* __restrict__ grooms,
-                                                           const int4* __restrict__ packs) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves(BLOCK)))) void step_batch_kernel(
+    const StepArgs* __restrict__ grooms, const int4* __restrict__ packs) {
     __shared__ StepLds<BLOCK> lds;
     const int4 pk = packs[blockIdx.x];
     const StepArgs a = grooms[pk.z];
