@@ -355,6 +355,14 @@ int launch_contact_step(kscr_handle* h, float dt, bool with_sdf) {
                 h->last_bvh_rebuild += h->h * h->p.detection_period;
         }
         if ((e = contact_step_limit(a, cp, c, h->stream)) != hipSuccess) return h->hip_fail(e, "step limit");
+        // packs without contacts run all iterations in one launch; the
+        // per-iteration launches below cover the packs with contacts
+        if ((e = contact_classify(a, c, h->d_packs.p, h->num_packs, h->block, h->stream)) != hipSuccess)
+            return h->hip_fail(e, "contact classification");
+        float4* q_fin = (a.num_itr % 2 == 0) ? h->d_q.p : c.q2;  // where the iteration ping-pong ends
+        if ((e = contact_free_iterate(a, c, h->d_q.p, q_fin, h->d_packs.p, h->num_packs, h->block, h->stream)) !=
+            hipSuccess)
+            return h->hip_fail(e, "contact-free iterations");
         int cur = 0;
         for (int k = 0; k < a.num_itr; k++) {
             if ((e = contact_iterate(a, cp, c, cur, h->d_q.p, c.q2, h->d_packs.p, h->num_packs, h->block,

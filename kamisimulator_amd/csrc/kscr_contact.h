@@ -30,6 +30,7 @@ struct ContactDev {
     int2* pairs = nullptr;
     int pair_cap = 0;
     int* error = nullptr;  // device error word (DEV_ERR_* bits)
+    uint8_t* contact_pack = nullptr;  // uint8[n] (>= packs): pack p holds a contact this substep
     LbvhDev bvh;
 };
 
@@ -53,6 +54,14 @@ hipError_t contact_step_limit(const StepArgs& a, const ContactParams& cp, Contac
 hipError_t contact_iterate(const StepArgs& a, const ContactParams& cp, ContactDev& c, int cur, const float4* q_in,
                            float4* q_out, const int2* packs, int num_packs, int block, hipStream_t s);
 // endItr from dx[cur]
+// contact_pack from the current contact lists (after detection).
+hipError_t contact_classify(const StepArgs& a, ContactDev& c, const int2* packs, int num_packs, int block,
+                            hipStream_t s);
+// All iterations of the substep for the packs without contacts: reads dx[0]
+// and q_in, writes dx[num_itr & 1] and q_fin (the buffers the last of the
+// contact_iterate calls writes).
+hipError_t contact_free_iterate(const StepArgs& a, ContactDev& c, const float4* q_in, float4* q_fin, const int2* packs,
+                                int num_packs, int block, hipStream_t s);
 hipError_t contact_end(const StepArgs& a, ContactDev& c, int cur, hipStream_t s);
 // SDF pass over all non-root vertices
 hipError_t sdf_pass(const StepArgs& a, int n, hipStream_t s);

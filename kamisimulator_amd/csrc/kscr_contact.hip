@@ -206,6 +206,24 @@ struct SegSums {
     float limit;
 };
 
+// The stretch terms of segment (i, i+1) (cosserat.cu:141-162): added to
+// vertex i's sums (f0, H0) and to vertex i+1's share (f2a, H2a).
+__device__ inline void stretch_sums(const float4 mi, f3 xi, f3 xj, f3 dxi, f3 dxj, f4 qi, float damp, SegSums& s) {
+    const float invl = 1.0f / mi.y;
+    const f3 c = ((xj - xi) + (dxj - dxi)) * invl - rotate_e1(qi);
+    const float k = mi.z * invl;
+    float d = k * invl;
+    s.f0 = s.f0 + (k * c - (damp * d) * dxi);
+    s.f2a = s.f2a + ((-k) * c - (damp * d) * dxj);
+    d = d * (1 + damp);
+    s.H0.d[0] += d;
+    s.H0.d[1] += d;
+    s.H0.d[2] += d;
+    s.H2a.d[0] += d;
+    s.H2a.d[1] += d;
+    s.H2a.d[2] += d;
+}
+
 // Everything segment i adds, by sector (sector 0: stretch + even-position
 // contacts on top of vertex i's inertia terms; sector 1: odd positions).
 __device__ inline void segment_sums(const StepArgs& a, const ContactParams& cp, const float* __restrict__ dx,
@@ -222,20 +240,7 @@ __device__ inline void segment_sums(const StepArgs& a, const ContactParams& cp, 
     const float4 mi = a.mat[i];
     if (!(__float_as_uint(mi.w) & FLAG_NEXT)) return;
     s.limit = max_step[i];
-    const float invl = 1.0f / mi.y;
-    const f3 xi = ld3(a.x, i), xj = ld3(a.x, i + 1), dxi = ld3(dx, i), dxj = ld3(dx, i + 1);
-    const f3 c = ((xj - xi) + (dxj - dxi)) * invl - rotate_e1(to4(q[i]));
-    const float k = mi.z * invl;
-    float d = k * invl;
-    s.f0 = s.f0 + (k * c - (damp * d) * dxi);
-    s.f2a = s.f2a + ((-k) * c - (damp * d) * dxj);
-    d = d * (1 + damp);
-    s.H0.d[0] += d;
-    s.H0.d[1] += d;
-    s.H0.d[2] += d;
-    s.H2a.d[0] += d;
-    s.H2a.d[1] += d;
-    s.H2a.d[2] += d;
+    stretch_sums(mi, ld3(a.x, i), ld3(a.x, i + 1), ld3(dx, i), ld3(dx, i + 1), to4(q[i]), damp, s);
     const int nc = num_cols[i];
     const int* L = cols + kMaxCols * (size_t)i;
     for (int k2 = 0; k2 < nc; k2 += 2) contact_terms(a, cp, dx, i, L[k2], damp, s.f0, s.H0, s.f2a, s.H2a);
@@ -249,18 +254,22 @@ __device__ inline void segment_sums(const StepArgs& a, const ContactParams& cp, 
 // vertex i+1's share reaches lane i+1 through LDS and is added there in the
 // order the sequential sum adds it (own segment, then the previous one).
 // The previous iterate (dx, q) is read from global memory: contact partners
-// lie in other workgroups (Jacobi snapshot).
+// lie in other workgroups (Jacobi snapshot).  Only the packs that hold a
+// contact in this substep take part (contact_pack); a partner's pack holds
+// the contact too, so no lane reads the iterate of a pack left out.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void iterate_kernel(StepArgs a, ContactParams cp, const int2* __restrict__ packs,
                                                         const float* __restrict__ dx, const float4* __restrict__ q_in,
                                                         const float* __restrict__ y, const int* __restrict__ num_cols,
                                                         const int* __restrict__ cols,
                                                         const float* __restrict__ max_step,
+                                                        const uint8_t* __restrict__ contact_pack,
                                                         float* __restrict__ dx_out, float4* __restrict__ q_out) {
     __shared__ float s_f2[3][BLOCK];
     __shared__ float s_H2[6][BLOCK];
     __shared__ float s_lim[BLOCK];
     __shared__ float s_dxn[3][BLOCK];
+    if (!contact_pack[blockIdx.x]) return;  // a pack without contacts is free_iterate_kernel's
     const int2 pk = packs[blockIdx.x];
     const int t = threadIdx.x;
     const int i = pk.x + t;
@@ -341,6 +350,135 @@ __global__ __launch_bounds__(BLOCK) void iterate_kernel(StepArgs a, ContactParam
     q_out[i] = make_float4(qn.x, qn.y, qn.z, qn.w);
 }
 
+// contact_pack[p] = 1 when any segment of pack p has a contact in the current
+// lists (the lists are symmetric: both packs of a contact are marked).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void classify_kernel(const int2* __restrict__ packs,
+                                                         const int* __restrict__ num_cols,
+                                                         uint8_t* __restrict__ contact_pack) {
+    const int2 pk = packs[blockIdx.x];
+    const int i = pk.x + (int)threadIdx.x;
+    const int any = __syncthreads_or(i < pk.y && num_cols[i] > 0);
+    if (threadIdx.x == 0) contact_pack[blockIdx.x] = any ? 1 : 0;
+}
+
+// All numItr iterations of the packs without contacts in one launch: the
+// arithmetic of iterate_kernel for segments with no contacts (stretch terms,
+// the same sums and 3x3 solve, the step limit, the orientation solve), with
+// the previous iterate of a vertex's neighbours taken from LDS instead of
+// global memory, since no other pack enters.  dx0 / q_in hold the substep's
+// initial iterate, dx_fin / q_fin receive the last one (they may be the same
+// buffers: each lane reads its own vertex before any lane writes).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void free_iterate_kernel(StepArgs a, const int2* __restrict__ packs,
+                                                             const uint8_t* __restrict__ contact_pack,
+                                                             const float* dx0, const float4* q_in,
+                                                             const float* __restrict__ y,
+                                                             const float* __restrict__ max_step, float* dx_fin,
+                                                             float4* q_fin) {
+    __shared__ float s_f2[3][BLOCK];
+    __shared__ float s_H2[6][BLOCK];
+    __shared__ float s_lim[BLOCK];
+    __shared__ float s_dx[2][3][BLOCK];
+    __shared__ float4 s_q[2][BLOCK];
+    if (contact_pack[blockIdx.x]) return;  // iterate_kernel's, one launch per iteration
+    const int2 pk = packs[blockIdx.x];
+    const int t = threadIdx.x;
+    const int i = pk.x + t;
+    const bool act = i < pk.y;
+    const int tm = t > 0 ? t - 1 : 0;
+    const int tp1 = t < BLOCK - 1 ? t + 1 : t;
+    const float4 mi = act ? a.mat[i] : make_float4(0.f, 1.f, 0.f, 0.f);
+    const uint32_t fl = act ? __float_as_uint(mi.w) : 0u;
+    const bool free = act && mi.x != 0.f;
+    const bool next = (fl & FLAG_NEXT) != 0;
+    const f3 xi = act ? ld3(a.x, i) : f3{0, 0, 0};
+    const f3 xj = next ? ld3(a.x, i + 1) : xi;
+    const f3 yi = free ? ld3(y, i) : f3{0, 0, 0};
+    const f4 qr = act ? to4(a.q_rest[i]) : f4{0, 0, 0, 1};
+    const f4 qrp = (fl & FLAG_PREV) ? to4(a.q_rest[i - 1]) : f4{0, 0, 0, 1};
+    const float lim = next ? max_step[i] : INFINITY;
+    const float h = a.h;
+    const float damp = a.damping / h;
+    const float a_in = free ? 1.0f / (mi.x * h * h) : 0.0f;
+    const float f_in = free ? 1.0f / (h * h * mi.x) : 0.0f;
+    f3 dx = act ? ld3(dx0, i) : f3{0, 0, 0};
+    f4 q = act ? to4(q_in[i]) : f4{0, 0, 0, 1};
+    s_lim[t] = lim;
+    s_dx[0][0][t] = dx.x;
+    s_dx[0][1][t] = dx.y;
+    s_dx[0][2][t] = dx.z;
+    s_q[0][t] = make_float4(q.x, q.y, q.z, q.w);
+    __syncthreads();
+    for (int k = 0; k < a.num_itr; k++) {
+        const int cur = k & 1;
+        SegSums s;
+        s.H0 = s3{{0, 0, 0, 0, 0, 0}};
+        s.f0 = f3{0, 0, 0};
+        if (free) {
+            s.H0 = s3{{a_in, a_in, a_in, a_in, a_in, a_in}};
+            s.f0 = f_in * (yi - dx);
+        }
+        s.f1 = s.f2a = s.f2b = f3{0, 0, 0};
+        s.H1 = s.H2a = s.H2b = s3{{0, 0, 0, 0, 0, 0}};
+        s.limit = lim;
+        if (next) {
+            const f3 dxj{s_dx[cur][0][t + 1], s_dx[cur][1][t + 1], s_dx[cur][2][t + 1]};
+            stretch_sums(mi, xi, xj, dx, dxj, q, damp, s);
+        }
+        {
+            const f3 f2 = s.f2a + s.f2b;
+            s3 H2 = s.H2a;
+            sadd(H2, s.H2b);
+            s_f2[0][t] = f2.x;
+            s_f2[1][t] = f2.y;
+            s_f2[2][t] = f2.z;
+            for (int c = 0; c < 6; c++) s_H2[c][t] = H2.d[c];
+        }
+        __syncthreads();
+        f3 out = dx;
+        if (free) {
+            f3 f = s.f0 + s.f1;
+            s3 H = s.H0;
+            sadd(H, s.H1);
+            float limit = s.limit;
+            if (fl & FLAG_PREV) {
+                const f3 f2{s_f2[0][t - 1], s_f2[1][t - 1], s_f2[2][t - 1]};
+                s3 H2;
+                for (int c = 0; c < 6; c++) H2.d[c] = s_H2[c][t - 1];
+                limit = gmin(limit, s_lim[t - 1]);
+                f = f + f2;
+                sadd(H, H2);
+            }
+            out = dx + a.acceleration_ratio * sym_solve(H.d[0], H.d[1], H.d[2], H.d[3], H.d[4], H.d[5], f);
+            if (!isfinite(out.x) || !isfinite(out.y) || !isfinite(out.z)) {
+                out = dx;
+            } else {
+                const float l = length(out);
+                if (l > limit && l > 0) out = out * (limit / l);
+            }
+        }
+        s_dx[cur ^ 1][0][t] = out.x;
+        s_dx[cur ^ 1][1][t] = out.y;
+        s_dx[cur ^ 1][2][t] = out.z;
+        __syncthreads();
+        f4 qn = q;
+        if (next && !(fl & FLAG_FIX_ORI)) {
+            const f4 qm = to4(s_q[cur][tm]);   // used only with FLAG_PREV
+            const f4 qp = to4(s_q[cur][tp1]);  // used only with FLAG_NEXT_ORI
+            const f3 dxj{s_dx[cur ^ 1][0][t + 1], s_dx[cur ^ 1][1][t + 1], s_dx[cur ^ 1][2][t + 1]};
+            qn = orientation_update(xi, xj, out, dxj, mi.y, 1.0f / mi.y, mi.z, q, qm, qp, qrp, qr, fl);
+        }
+        s_q[cur ^ 1][t] = make_float4(qn.x, qn.y, qn.z, qn.w);
+        dx = out;
+        q = qn;
+    }
+    if (act) {
+        st3(dx_fin, i, dx);
+        q_fin[i] = make_float4(q.x, q.y, q.z, q.w);
+    }
+}
+
 __global__ void end_kernel(StepArgs a, int n, const float* __restrict__ dx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -368,7 +506,7 @@ hipError_t dalloc(T*& p, size_t count) {
 }  // namespace
 
 void contact_release(ContactDev& c) {
-    void* ps[] = {c.dx[0], c.dx[1], c.y, c.q2, c.max_step, c.num_cols, c.cols, c.pairs, c.error};
+    void* ps[] = {c.dx[0], c.dx[1], c.y, c.q2, c.max_step, c.num_cols, c.cols, c.pairs, c.error, c.contact_pack};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     lbvh_release(c.bvh);
@@ -389,7 +527,8 @@ hipError_t contact_reserve(ContactDev& c, int n) {
         (e = dalloc(c.y, n3)) != hipSuccess || (e = dalloc(c.q2, n)) != hipSuccess ||
         (e = dalloc(c.max_step, n)) != hipSuccess || (e = dalloc(c.num_cols, n)) != hipSuccess ||
         (e = dalloc(c.cols, (size_t)kMaxCols * n)) != hipSuccess || (e = dalloc(c.pairs, (size_t)c.pair_cap)) != hipSuccess ||
-        (e = dalloc(c.error, 1)) != hipSuccess || (e = lbvh_reserve(c.bvh, n)) != hipSuccess) {
+        (e = dalloc(c.error, 1)) != hipSuccess || (e = dalloc(c.contact_pack, n)) != hipSuccess ||
+        (e = lbvh_reserve(c.bvh, n)) != hipSuccess) {
         contact_release(c);
         return e;
     }
@@ -430,7 +569,51 @@ hipError_t contact_iterate(const StepArgs& a, const ContactParams& cp, ContactDe
 #define KSCR_CASE(B)                                                                                              \
     case B:                                                                                                       \
         hipLaunchKernelGGL(iterate_kernel<B>, grid, dim3(B), 0, s, a, cp, packs, c.dx[cur], q_in, c.y, c.num_cols, \
-                           c.cols, c.max_step, c.dx[cur ^ 1], q_out);                                           \
+                           c.cols, c.max_step, c.contact_pack, c.dx[cur ^ 1], q_out);                         \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t contact_classify(const StepArgs& a, ContactDev& c, const int2* packs, int num_packs, int block,
+                            hipStream_t s) {
+    if (num_packs <= 0) return hipSuccess;
+    const dim3 grid(num_packs);
+    switch (block) {
+#define KSCR_CASE(B)                                                                                       \
+    case B:                                                                                                \
+        hipLaunchKernelGGL(classify_kernel<B>, grid, dim3(B), 0, s, packs, c.num_cols, c.contact_pack); \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t contact_free_iterate(const StepArgs& a, ContactDev& c, const float4* q_in, float4* q_fin, const int2* packs,
+                                int num_packs, int block, hipStream_t s) {
+    if (num_packs <= 0 || a.num_itr <= 0) return hipSuccess;
+    const dim3 grid(num_packs);
+    float* dx_fin = c.dx[a.num_itr & 1];
+    switch (block) {
+#define KSCR_CASE(B)                                                                                             \
+    case B:                                                                                                      \
+        hipLaunchKernelGGL(free_iterate_kernel<B>, grid, dim3(B), 0, s, a, packs, c.contact_pack, c.dx[0], q_in, \
+                           c.y, c.max_step, dx_fin, q_fin);                                                      \
         break;
         KSCR_CASE(64)
         KSCR_CASE(128)

@@ -95,3 +95,44 @@ def test_too_many_contacts_is_reported():
     o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets), sc)
     o.solve_next_step(DT)
     assert o.error() == 8
+
+
+def _packs(offsets, capacity):
+    """Greedy packs of whole strands (the library's packing)."""
+    packs, s, ns = [], 0, len(offsets) - 1
+    while s < ns:
+        beg = end = offsets[s]
+        while s < ns and offsets[s + 1] - beg <= capacity:
+            end = offsets[s + 1]
+            s += 1
+        packs.append((beg, end))
+    return packs
+
+
+@pytest.mark.parametrize("num_itr", [10, 7])
+def test_packs_without_contacts_match_oracle_bitwise(num_itr):
+    """Packs with no contact in a substep run all iterations in one launch
+    (neighbour iterates in LDS), the others one launch per iteration.  A groom
+    where both kinds occur, with an even and an odd iteration count (the two
+    ends of the dx / q ping-pong), must equal the FP32 restatement word for
+    word."""
+    sc = scenes.config_large(400, seed=5, sdf_dim=64)
+    sc.material.num_itr = num_itr
+    g, o = _pair(sc)
+    packs = _packs(sc.strand_offsets, g.block_size)
+    mixed = 0
+    for _ in range(4):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+        gn, gc = g.get_contacts()
+        hit = np.array([bool((gn[b:e] > 0).any()) for b, e in packs])
+        mixed += int(hit.any() and not hit.all())
+        print(f"[gpu-contacts] itr {num_itr}: packs with contacts {int(hit.sum())} of {len(hit)}", flush=True)
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    on, oc = o.contacts()
+    assert mixed == 4
+    assert np.array_equal(gn, on) and np.array_equal(gc, oc)
+    assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))
+    assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
+    assert np.array_equal(g.get_q().view(np.uint32), o.get_q().view(np.uint32))
