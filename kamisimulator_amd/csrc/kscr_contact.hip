@@ -257,8 +257,13 @@ __device__ inline void segment_sums(const StepArgs& a, const ContactParams& cp, 
 // lie in other workgroups (Jacobi snapshot).  Only the packs that hold a
 // contact in this substep take part (contact_pack); a partner's pack holds
 // the contact too, so no lane reads the iterate of a pack left out.
+// amdgpu_waves_per_eu(5): 96 VGPRs (3 values spilled to scratch, reloaded
+// in the loop over a segment's contacts) instead of 103, 5 waves per SIMD
+// instead of 4; with 256-thread
+// workgroups that is one more workgroup per CU (100k-strand contact groom:
+// 37.2 -> 35.9 ms/step; a cap at 6 waves spills 18 and measures 36.9).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void iterate_kernel(StepArgs a, ContactParams cp, const int2* __restrict__ packs,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void iterate_kernel(StepArgs a, ContactParams cp, const int2* __restrict__ packs,
                                                         const float* __restrict__ dx, const float4* __restrict__ q_in,
                                                         const float* __restrict__ y, const int* __restrict__ num_cols,
                                                         const int* __restrict__ cols,
