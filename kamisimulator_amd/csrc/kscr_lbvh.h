@@ -44,6 +44,8 @@ struct LbvhDev {
     int* flags = nullptr;
     int* max_depth = nullptr;
     int* count = nullptr;  // query pair counter
+    int2* redo = nullptr;     // queries deeper than the first pass's stack: (sorted index, pairs reported)
+    int* redo_count = nullptr;
     // internal nodes grouped by height (leaves 0): refit runs one level per launch
     uint32_t* node_height = nullptr;
     uint32_t* level_nodes = nullptr;

@@ -17,6 +17,8 @@ constexpr int kTile = kSortThreads * kSortItems;  // keys per radix tile
 constexpr int kScanChunk = 1024;
 constexpr int kQueryBlock = 128;
 constexpr int kQueryStack = 64;
+constexpr int kQueryStackFirst = 16;  // first pass; deeper queries are redone
+constexpr unsigned kRedoBlocks = 512;   // grid of the redo pass (grid-stride)
 
 __device__ inline uint32_t expand_bits(uint32_t v) {
     v = (v * 0x00010001u) & 0xFF0000FFu;
@@ -425,12 +427,21 @@ __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(const float* __res
 constexpr int kWaveBuf = 128;
 constexpr int kQueryWaves = kQueryBlock / 64;
 
-template <int STACK>
+// REDO = false: one query per lane over the sorted objects, with a STACK-entry
+// LDS stack.  A query that needs more stack stops and appends (its sorted
+// index, the pairs it has reported) to the redo list.
+// REDO = true: the listed queries again with a deeper stack (grid-stride over
+// the list).  The traversal order does not depend on the stack size, so a
+// redone query finds the same pairs in the same order and reports only those
+// after the ones its first pass reported.
+template <int STACK, bool REDO>
 __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restrict__ aabb,
                                                             const uint32_t* __restrict__ ids,
                                                             const LbvhNode* __restrict__ nodes, int n,
                                                             int2* __restrict__ pairs, int cap, int min_gap,
-                                                            int* __restrict__ count, int* __restrict__ error) {
+                                                            int* __restrict__ count, int* __restrict__ error,
+                                                            int2* __restrict__ redo, int* __restrict__ redo_count,
+                                                            int redo_cap) {
     __shared__ uint32_t stack[STACK][kQueryBlock];
     __shared__ int2 wbuf[kQueryWaves][kWaveBuf];
     __shared__ int wfill[kQueryWaves];
@@ -438,12 +449,9 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
     const uint64_t lane_below = (1ull << (threadIdx.x & 63)) - 1;
     if ((threadIdx.x & 63) == 0) wfill[w] = 0;
     __builtin_amdgcn_wave_barrier();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n || n < 2) return;
-    const int qid = (int)ids[t];
-    float q[6];
-    for (int k = 0; k < 6; k++) q[k] = aabb[6 * (size_t)qid + k];
-    if (!(q[0] <= q[3])) return;  // empty box (strand tip): intersects nothing
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < 2) return;
+    if (!REDO && gid >= n) return;
 
     // copies the wave's buffered pairs to the global list with the wave's
     // active lanes; every active lane sees the same fill
@@ -458,6 +466,13 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
             if (base + k < cap) pairs[base + k] = wbuf[w][k];
     };
     constexpr uint32_t kLeaf = 0x80000000u, kNone = 0xFFFFFFFFu;
+    // the query of sorted object t; the first `skip` pairs it finds are not
+    // reported (a redo's first pass reported them)
+    auto run = [&](const int t, const int skip) {
+    const int qid = (int)ids[t];
+    float q[6];
+    for (int k = 0; k < 6; k++) q[k] = aabb[6 * (size_t)qid + k];
+    if (!(q[0] <= q[3])) return;  // empty box (strand tip): intersects nothing
     // object id of a leaf paired with the query, or -1
     auto leaf_pair = [&](uint32_t leaf) -> int {
         if ((int)leaf <= t) return -1;
@@ -465,7 +480,7 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
         if (min_gap >= 0 && abs(fid - qid) <= min_gap) return -1;
         return fid;
     };
-    int sp = 0;
+    int sp = 0, seen = 0;
     uint32_t node = 0;  // the root (internal: n >= 2)
     while (node != kNone) {
         int f0 = -1, f1 = -1;
@@ -489,8 +504,8 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
                 h1 = false;
             }
             if (h0 && h1) {
-                if (sp + 1 > STACK) {  // deeper than the LDS stack: report, stop this query
-                    if (error) atomicOr(error, 4);
+                if (sp + 1 > STACK) {  // deeper than the LDS stack: stop this query
+                    if (REDO && error) atomicOr(error, 4);
                     overflow = true;
                 } else {
                     stack[sp++][threadIdx.x] = rc;
@@ -503,6 +518,12 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
             }
         }
         if (!overflow && next == kNone && sp > 0) next = stack[--sp][threadIdx.x];
+        if (REDO) {
+            if (f0 >= 0 && seen++ < skip) f0 = -1;
+            if (f1 >= 0 && seen++ < skip) f1 = -1;
+        } else {
+            seen += (f0 >= 0) + (f1 >= 0);
+        }
         // the lanes of the wave that found pairs in this pass append them to
         // the wave's buffer in lane order
         const uint64_t m0 = __ballot(f0 >= 0), m1 = __ballot(f1 >= 0);
@@ -519,9 +540,26 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
             if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)(m0 | m1)) - 1) wfill[w] = fill + found;
             __builtin_amdgcn_wave_barrier();
         }
+        if (!REDO && overflow) {  // the redo pass takes it from here
+            const int slot = atomicAdd(redo_count, 1);
+            if (slot < redo_cap)
+                redo[slot] = make_int2(t, seen);
+            else if (error)
+                atomicOr(error, 4);
+        }
         node = overflow ? kNone : next;
     }
-    // the lanes still here are the wave's lanes that ran a query
+    };
+    if (REDO) {
+        const int m = min(*redo_count, redo_cap);
+        for (int r = gid; r < m; r += gridDim.x * blockDim.x) {
+            const int2 e = redo[r];
+            run(e.x, e.y);
+        }
+    } else {
+        run(gid, 0);
+    }
+    // the lanes still here are the wave's lanes that ran queries
     __builtin_amdgcn_wave_barrier();
     const int fill = wfill[w];
     if (fill > 0) flush(fill);
@@ -548,7 +586,8 @@ hipError_t fill_i32(int* p, int value, size_t n, hipStream_t stream) {
 void lbvh_release(LbvhDev& d) {
     void* ps[] = {d.aabb,  d.codes,       d.ids,         d.codes_tmp, d.ids_tmp,   d.hist,
                   d.hist_sums, d.partial,   d.root,        d.nodes,     d.leaf_parent, d.flags,
-                  d.max_depth, d.count,     d.node_height, d.level_nodes, d.level_count, d.level_offset};
+                  d.max_depth, d.count,     d.node_height, d.level_nodes, d.level_count, d.level_offset,
+                  d.redo,      d.redo_count};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     d = LbvhDev{};
@@ -568,6 +607,7 @@ hipError_t lbvh_reserve(LbvhDev& d, int n) {
         (e = dmalloc(d.root, 6)) != hipSuccess || (e = dmalloc(d.nodes, (size_t)(n > 1 ? n - 1 : 1))) != hipSuccess ||
         (e = dmalloc(d.leaf_parent, n)) != hipSuccess || (e = dmalloc(d.flags, n)) != hipSuccess ||
         (e = dmalloc(d.max_depth, 1)) != hipSuccess || (e = dmalloc(d.count, 1)) != hipSuccess ||
+        (e = dmalloc(d.redo, (size_t)n)) != hipSuccess || (e = dmalloc(d.redo_count, 1)) != hipSuccess ||
         (e = dmalloc(d.node_height, (size_t)n)) != hipSuccess || (e = dmalloc(d.level_nodes, (size_t)n)) != hipSuccess ||
         (e = dmalloc(d.level_count, (size_t)2 * kMaxLevels)) != hipSuccess ||
         (e = dmalloc(d.level_offset, (size_t)kMaxLevels)) != hipSuccess) {
@@ -681,14 +721,21 @@ hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error,
     hipError_t e = fill_i32(d.count, 0, 1, stream);
     if (e != hipSuccess) return e;
     if (d.n < 2) return hipSuccess;
-    // the LDS stack is sized from the tree's DFS bound (known since the build)
+    // A 16-entry LDS stack (10 KB per 128-lane workgroup: 8 waves per SIMD)
+    // serves nearly every query; the few that need more are redone with the
+    // stack the tree's DFS bound calls for (known since the build).
+    if ((e = fill_i32(d.redo_count, 0, 1, stream)) != hipSuccess) return e;
     const dim3 grid((d.n + kQueryBlock - 1) / kQueryBlock);
+    hipLaunchKernelGGL((query_kernel<kQueryStackFirst, false>), grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids,
+                       d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
+    if (d.max_depth_host + 2 <= kQueryStackFirst) return hipGetLastError();  // nothing can overflow
+    const dim3 rgrid(std::min<unsigned>(grid.x, kRedoBlocks));
     if (d.max_depth_host + 2 <= 32)
-        hipLaunchKernelGGL(query_kernel<32>, grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes, d.n, pairs, cap,
-                           min_gap, d.count, error);
+        hipLaunchKernelGGL((query_kernel<32, true>), rgrid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes, d.n,
+                           pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
     else
-        hipLaunchKernelGGL(query_kernel<kQueryStack>, grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes, d.n,
-                           pairs, cap, min_gap, d.count, error);
+        hipLaunchKernelGGL((query_kernel<kQueryStack, true>), rgrid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids,
+                           d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
     return hipGetLastError();
 }
 

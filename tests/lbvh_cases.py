@@ -47,3 +47,34 @@ def brute_pairs(boxes):
         for j in np.nonzero(ov)[0]:
             out.append((i, i + 1 + j))
     return np.array(out, np.int64).reshape(-1, 2)
+
+
+def query_stack_need(tree, box, t):
+    """Deepest stack the device query of sorted object t (box = its box) uses
+    on `tree` (an OracleLbvh.get() dict): the same visit rule, pushing the
+    right child only when both child boxes overlap and neither is a leaf."""
+    leaf = 0x80000000
+    links, bounds = tree["links"], tree["bounds"]
+
+    def hit(b):  # the device test: open intervals overlap
+        return bool(np.all(b[3:] > box[:3]) and np.all(b[:3] < box[3:]))
+
+    stack, deepest, node = [], 0, 0
+    while node is not None:
+        nxt = None
+        lc, rc, fence = (int(v) for v in links[node, 1:4])
+        if max(node, fence) > t:
+            h0 = hit(bounds[node, :6]) and not (lc & leaf)
+            h1 = hit(bounds[node, 6:]) and not (rc & leaf)
+            if h0 and h1:
+                stack.append(rc)
+                deepest = max(deepest, len(stack))
+                nxt = lc
+            elif h0:
+                nxt = lc
+            elif h1:
+                nxt = rc
+        if nxt is None and stack:
+            nxt = stack.pop()
+        node = nxt
+    return deepest

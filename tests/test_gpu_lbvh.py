@@ -9,7 +9,7 @@ import pytest
 
 from kamisimulator_amd._lib import Lbvh
 from oracle.pyoracle import OracleLbvh
-from tests.lbvh_cases import segment_boxes, recipe_points
+from tests.lbvh_cases import query_stack_need, segment_boxes, recipe_points
 
 pytestmark = pytest.mark.gpu
 
@@ -72,3 +72,23 @@ def test_ties_and_one_object():
     g.compute(one)
     pairs, cnt = g.query(16)
     assert cnt == 0
+
+
+def test_queries_deeper_than_the_first_pass_stack():
+    """Boxes spanning the whole scene overlap every node on their way down,
+    so their queries need a stack as deep as the tree and are finished by the
+    redo pass (the first pass holds 16 entries); the pair set must still be
+    the restatement's, with no pair lost or repeated."""
+    rng = np.random.default_rng(12)
+    pts = rng.uniform(0, 1, size=(300_000, 3)).astype(np.float32)
+    small = np.concatenate([pts - 2e-3, pts + 2e-3], axis=1).astype(np.float32)
+    big = np.array([[-0.5, -0.5, -0.5, 1.5, 1.5, 1.5],      # centre at the middle
+                    [-1.0, -1.0, -1.0, 1.1, 1.1, 1.1],      # centre near the low corner
+                    [-0.1, -0.1, -0.1, 2.0, 2.0, 2.0]], np.float32)
+    boxes = np.concatenate([small[:100_000], big[:1], small[100_000:200_000], big[1:], small[200_000:]])
+    g, o = _compare(boxes, "whole-scene boxes")
+    tree = o.get()
+    pos = {int(obj): k for k, obj in enumerate(tree["ids"])}
+    need = [query_stack_need(tree, boxes[b], pos[b]) for b in (100_000, 200_001, 200_002)]
+    print(f"[gpu-lbvh] stack needed by the whole-scene queries: {need}", flush=True)
+    assert max(need) > 16  # so the redo pass ran
