@@ -98,7 +98,11 @@ typedef struct kscr_params {
     float wind_time_freq;
     float wind_sharpness;
     int32_t schedule;    /* KSCR_SCHEDULE_*                         */
-    int32_t block_size;  /* vertices per workgroup, 0 = autotuned   */
+    int32_t block_size;  /* vertices per workgroup; 0 = automatic:
+                          * the smallest size holding the longest
+                          * strand and at least 512 vertices (256 when
+                          * contacts or step limits are on), taken at
+                          * upload; kscr_autotune measures instead */
 } kscr_params;
 
 /* Sim::Sim defaults (YBSimulator.cu:81-112). */
