@@ -212,6 +212,21 @@ int max_block(const kscr_handle* h) { return colored(h) ? 512 : 1024; }
 
 int longest_strand(const kscr_handle* h);
 
+// Workgroups the greedy packing of whole strands gives at `capacity`
+// vertices per pack (the count build_packs produces).
+int count_packs(const kscr_handle* h, int capacity) {
+    const int ns = (int)h->strand_off.size() - 1;
+    int packs = 0, s = 0;
+    while (s < ns) {
+        const int beg = h->strand_off[s];
+        const int s0 = s;
+        while (s < ns && h->strand_off[s + 1] - beg <= capacity) s++;
+        if (s == s0) return 0;  // a strand longer than the pack
+        packs++;
+    }
+    return packs;
+}
+
 int choose_block(const kscr_handle* h) {
     const int longest = longest_strand(h);
     if (h->p.block_size > 0) return h->p.block_size <= max_block(h) ? h->p.block_size : -1;
@@ -222,10 +237,27 @@ int choose_block(const kscr_handle* h) {
     // one launch) and shorten the per-iteration launches (100k-strand
     // lognormal groom with contacts: 37.2 ms/step at 256, 41.1 at 512,
     // 42.6 at 1024).
+    // A small groom gets smaller packs, down to the smallest that holds its
+    // longest strand, until it has kMinPacks workgroups: one 512-vertex
+    // workgroup per CU leaves the step latency-bound (2000 strands x 64:
+    // forward 0.061 ms/step with 512-vertex packs, 0.054 with 64; the
+    // adjoint's backward 0.169 and 0.150 ms/step).
     const bool contact_path = h->p.enable_hair_hair_coll || h->p.use_step_size_limit;
     const int min_cap = contact_path ? 255 : 511;
+    int chosen = -1;
     for (int b : kBlockSizes)
-        if (b <= max_block(h) && pack_capacity(h, b) >= longest && pack_capacity(h, b) >= min_cap) return b;
+        if (b <= max_block(h) && pack_capacity(h, b) >= longest && pack_capacity(h, b) >= min_cap) {
+            chosen = b;
+            break;
+        }
+    if (chosen > 0 && !colored(h)) {
+        constexpr int kMinPacks = 1024;
+        while (chosen > kBlockSizes[0] && pack_capacity(h, chosen / 2) >= longest &&
+               count_packs(h, pack_capacity(h, chosen)) < kMinPacks)
+            chosen /= 2;
+        return chosen;
+    }
+    if (chosen > 0) return chosen;
     for (int b : kBlockSizes)
         if (b <= max_block(h) && pack_capacity(h, b) >= longest) return b;
     return -1;
