@@ -101,8 +101,10 @@ typedef struct kscr_params {
     int32_t block_size;  /* vertices per workgroup; 0 = automatic:
                           * the smallest size holding the longest
                           * strand and at least 512 vertices (256 when
-                          * contacts or step limits are on), taken at
-                          * upload; kscr_autotune measures instead */
+                          * contacts or step limits are on), halved
+                          * while the groom has fewer than 1024
+                          * workgroups, taken at upload;
+                          * kscr_autotune measures instead */
 } kscr_params;
 
 /* Sim::Sim defaults (YBSimulator.cu:81-112). */
