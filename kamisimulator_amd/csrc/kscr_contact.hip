@@ -381,9 +381,12 @@ __global__ __launch_bounds__(BLOCK) void free_iterate_kernel(StepArgs a, const i
                                                              const float* __restrict__ y,
                                                              const float* __restrict__ max_step, float* dx_fin,
                                                              float4* q_fin) {
+    // Without contacts the share a segment passes to its upper vertex has
+    // H2 = (h, h, h, +0, +0, +0): one float crosses LDS, and the previous
+    // vertex's step limit is loaded once (72 B of LDS per lane: 8 waves per
+    // SIMD at 256 threads).
     __shared__ float s_f2[3][BLOCK];
-    __shared__ float s_H2[6][BLOCK];
-    __shared__ float s_lim[BLOCK];
+    __shared__ float s_h2[BLOCK];
     __shared__ float s_dx[2][3][BLOCK];
     __shared__ float4 s_q[2][BLOCK];
     if (contact_pack[blockIdx.x]) return;  // iterate_kernel's, one launch per iteration
@@ -403,13 +406,13 @@ __global__ __launch_bounds__(BLOCK) void free_iterate_kernel(StepArgs a, const i
     const f4 qr = act ? to4(a.q_rest[i]) : f4{0, 0, 0, 1};
     const f4 qrp = (fl & FLAG_PREV) ? to4(a.q_rest[i - 1]) : f4{0, 0, 0, 1};
     const float lim = next ? max_step[i] : INFINITY;
+    const float lim_prev = (fl & FLAG_PREV) ? max_step[i - 1] : INFINITY;  // vertex i-1 has a segment
     const float h = a.h;
     const float damp = a.damping / h;
     const float a_in = free ? 1.0f / (mi.x * h * h) : 0.0f;
     const float f_in = free ? 1.0f / (h * h * mi.x) : 0.0f;
     f3 dx = act ? ld3(dx0, i) : f3{0, 0, 0};
     f4 q = act ? to4(q_in[i]) : f4{0, 0, 0, 1};
-    s_lim[t] = lim;
     s_dx[0][0][t] = dx.x;
     s_dx[0][1][t] = dx.y;
     s_dx[0][2][t] = dx.z;
@@ -438,7 +441,7 @@ __global__ __launch_bounds__(BLOCK) void free_iterate_kernel(StepArgs a, const i
             s_f2[0][t] = f2.x;
             s_f2[1][t] = f2.y;
             s_f2[2][t] = f2.z;
-            for (int c = 0; c < 6; c++) s_H2[c][t] = H2.d[c];
+            s_h2[t] = H2.d[0];  // == d[1] == d[2]; d[3..5] == +0
         }
         __syncthreads();
         f3 out = dx;
@@ -449,9 +452,9 @@ __global__ __launch_bounds__(BLOCK) void free_iterate_kernel(StepArgs a, const i
             float limit = s.limit;
             if (fl & FLAG_PREV) {
                 const f3 f2{s_f2[0][t - 1], s_f2[1][t - 1], s_f2[2][t - 1]};
-                s3 H2;
-                for (int c = 0; c < 6; c++) H2.d[c] = s_H2[c][t - 1];
-                limit = gmin(limit, s_lim[t - 1]);
+                const float h2 = s_h2[t - 1];
+                const s3 H2{{h2, h2, h2, 0.0f, 0.0f, 0.0f}};
+                limit = gmin(limit, lim_prev);
                 f = f + f2;
                 sadd(H, H2);
             }
