@@ -180,6 +180,19 @@ def adjoint_leg(stream, nsteps=50, every=10):
     wall_bwd = (time.perf_counter() - w0) * 1e3
     rec_ms, sweep_ms, bwd_ms = sim.adjoint_time()
     fwd_step = fwd_ms / nsteps
+    # the same gradient with the whole tape kept in HBM (one segment: the
+    # forward records every step, the backward re-records none)
+    sim.adjoint_forward(dt, nsteps, nsteps)
+    sim.adjoint_backward(seed())
+    torch.cuda.synchronize()
+    d, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    d.record(stream)
+    sim.adjoint_forward(dt, nsteps, nsteps)
+    e.record(stream)
+    torch.cuda.synchronize()
+    full_fwd_ms = d.elapsed_time(e)
+    sim.adjoint_backward(seed())
+    _, _, full_bwd_ms = sim.adjoint_time()
     # the checkpointed forward keeps the tape of the last segment; the
     # backward re-records the steps of every other segment, each segment on
     # a second stream while the previous (later) one is swept
@@ -199,11 +212,17 @@ def adjoint_leg(stream, nsteps=50, every=10):
         "backward_over_forward": round(bwd_ms / fwd_ms, 3),
         "gradient_over_forward": round((ckpt_fwd_ms + bwd_ms) / fwd_ms, 3),
         "backward_wall_ms": round(wall_bwd, 2),
+        "whole_tape_recording_forward_ms_per_step": round(full_fwd_ms / nsteps, 4),
+        "whole_tape_backward_ms_per_step": round(full_bwd_ms / nsteps, 4),
+        "whole_tape_adjoint_step_over_forward_step": round(full_bwd_ms / nsteps / fwd_step, 3),
+        "whole_tape_gradient_over_forward": round((full_fwd_ms + full_bwd_ms) / fwd_ms, 3),
         "note": "adjoint step = the whole backward pass's device time per step (each segment re-recorded from its "
                 "checkpoint on a second stream while the later segment is swept; "
                 f"{rerecorded} of {nsteps} steps re-recorded, the last segment's tape kept by the checkpointed "
                 "forward) over the plain forward step; the sequential figure adds one re-recorded step and one "
-                "reverse sweep as if they ran one after the other; gradient = checkpointed forward + backward",
+                "reverse sweep as if they ran one after the other; gradient = checkpointed forward + backward; "
+                "whole_tape_*: one checkpoint segment, the forward records the tape of every step "
+                f"({nsteps} steps held in HBM at once) and the backward only sweeps",
     }
 
 
