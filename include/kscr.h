@@ -160,6 +160,10 @@ int kscr_get_pos_vel(kscr_handle* h, float* positions, float* velocities);
  * caller that scatters the state into its own layout (the host solver's
  * DownloadHair). */
 int kscr_download_pos_vel(kscr_handle* h, const float** positions, const float** velocities);
+/* Velocities of the previous substep (float[num_verts*3]), the state the
+ * adaptive initial guess of initItr differentiates: Sim's d_lastVels
+ * (YarnBall.h:68, iteration.cu:47-50).  Synchronous. */
+int kscr_get_last_vel(kscr_handle* h, float* last_vel);
 /* Orientation state q (float[num_verts*4]) in and out. */
 int kscr_get_q(kscr_handle* h, float* q);
 int kscr_set_q(kscr_handle* h, const float* q);

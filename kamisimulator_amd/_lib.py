@@ -90,6 +90,7 @@ SIGNATURES = {
     "kscr_set_pos_vel": (C.c_int, [_P, _FP, _FP]),
     "kscr_get_pos_vel": (C.c_int, [_P, _FP, _FP]),
     "kscr_download_pos_vel": (C.c_int, [_P, C.POINTER(_FP), C.POINTER(_FP)]),
+    "kscr_get_last_vel": (C.c_int, [_P, _FP]),
     "kscr_get_q": (C.c_int, [_P, _FP]),
     "kscr_set_q": (C.c_int, [_P, _FP]),
     "kscr_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
@@ -313,6 +314,11 @@ class Sim:
         pa.flags.writeable = False
         va.flags.writeable = False
         return pa, va
+
+    def get_last_vel(self):
+        u = np.empty((self.n, 3), np.float32)
+        self._check(self.lib.kscr_get_last_vel(self.h, fptr(u)))
+        return u
 
     def get_q(self):
         q = np.empty((self.n, 4), np.float32)

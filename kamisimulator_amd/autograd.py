@@ -49,6 +49,25 @@ def strand_material(rest_dir, l_rest, counts, young_gpa, kappa, radius, deviatio
     return ks, bt[:, None] * _rotor_power(K, rest_dir)
 
 
+class _SimHolder:
+    """Owns the handle of one rollout.  Its device buffers (state, checkpoints
+    and tape, which can take GBs) are released when the autograd graph that
+    holds the rollout is freed: after the backward pass unless the graph is
+    retained, or when the outputs are dropped without one."""
+
+    def __init__(self, sim):
+        self.sim = sim
+
+    def __del__(self):
+        sim = self.sim
+        self.sim = None
+        if sim is not None:
+            try:
+                sim.close()
+            except Exception:
+                pass
+
+
 class _RolloutFn(torch.autograd.Function):
     """(kStretch[n], qRest[n, 4]) -> (x[n, 3], v[n, 3]) after the rollout."""
 
@@ -58,16 +77,18 @@ class _RolloutFn(torch.autograd.Function):
         sim.set_material(k_stretch.detach().cpu().numpy(), q_rest.detach().cpu().numpy())
         sim.adjoint_forward(rollout.dt, rollout.nsteps, rollout.checkpoint_every)
         x, v = sim.get_pos_vel()
-        ctx.sim = sim
+        ctx.holder = _SimHolder(sim)
         ctx.dev = k_stretch.device
         return torch.from_numpy(x).to(ctx.dev), torch.from_numpy(v).to(ctx.dev)
 
     @staticmethod
     def backward(ctx, gx, gv):
+        # kscr_adjoint_backward re-records every segment it needs from the
+        # checkpoints and leaves the state at the end of the rollout, so a
+        # retained graph can run it again
         xb = None if gx is None else gx.detach().float().cpu().numpy()
         vb = None if gv is None else gv.detach().float().cpu().numpy()
-        ksb, qrb = ctx.sim.adjoint_backward(xb, vb)
-        ctx.sim.close()
+        ksb, qrb = ctx.holder.sim.adjoint_backward(xb, vb)
         return torch.from_numpy(ksb).to(ctx.dev), torch.from_numpy(qrb).to(ctx.dev), None
 
 

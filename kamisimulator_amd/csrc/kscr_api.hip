@@ -185,21 +185,24 @@ struct kscr_handle {
 
 namespace {
 
-int validate_meta(kscr_handle* h) {
-    // uploadMeta (YBSimulator.cu:322-335)
-    if (h->p.detection_scaler <= 1) return h->fail(KSCR_ERR_DETECTION_SCALER, "Detection scaler must be greater than 1.");
-    if (2 * h->min_seg < 2 * h->p.radius + h->p.barrier_thickness)
+// uploadMeta (YBSimulator.cu:322-335) on a parameter set p for handle h
+// (h->min_seg is the shortest rest segment once configured).
+int validate_params(kscr_handle* h, const kscr_params& p) {
+    if (p.detection_scaler <= 1) return h->fail(KSCR_ERR_DETECTION_SCALER, "Detection scaler must be greater than 1.");
+    if (2 * h->min_seg < 2 * p.radius + p.barrier_thickness)
         return h->fail(KSCR_ERR_SEGMENT_TOO_SHORT,
                        "Use thinner yarn or use longer segments. (2 * min_seg_length must be > 2 * radius + "
                        "barrier_thickness)");
-    if (h->p.num_itr < 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr must be >= 0");
-    if (!(h->p.max_h > 0)) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "max_h must be > 0");
-    if (h->p.schedule != KSCR_SCHEDULE_JACOBI && h->p.schedule != KSCR_SCHEDULE_COLORED)
+    if (p.num_itr < 0) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr must be >= 0");
+    if (!(p.max_h > 0)) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "max_h must be > 0");
+    if (p.schedule != KSCR_SCHEDULE_JACOBI && p.schedule != KSCR_SCHEDULE_COLORED)
         return h->fail(KSCR_ERR_INVALID_ARGUMENT, "unknown schedule");
-    if (h->p.schedule == KSCR_SCHEDULE_COLORED && (h->p.enable_hair_hair_coll || h->p.use_step_size_limit))
+    if (p.schedule == KSCR_SCHEDULE_COLORED && (p.enable_hair_hair_coll || p.use_step_size_limit))
         return h->fail(KSCR_ERR_UNSUPPORTED, "the coloured schedule covers the contact-free step (no contacts, no step size limit)");
     return KSCR_OK;
 }
+
+int validate_meta(kscr_handle* h) { return validate_params(h, h->p); }
 
 bool colored(const kscr_handle* h) { return h->p.schedule == KSCR_SCHEDULE_COLORED; }
 
@@ -784,12 +787,13 @@ int kscr_fix_orientation(kscr_handle* h, const int* ids, int n) {
 
 int kscr_set_params(kscr_handle* h, const kscr_params* p) {
     if (!h || !p) return KSCR_ERR_INVALID_ARGUMENT;
-    const int old_block = h->p.block_size, old_schedule = h->p.schedule;
-    h->p = *p;
+    // a rejected parameter set leaves the handle's parameters as they were
     if (h->configured) {
-        int rc = validate_meta(h);
+        int rc = validate_params(h, *p);
         if (rc) return rc;
     }
+    const int old_block = h->p.block_size, old_schedule = h->p.schedule;
+    h->p = *p;
     // the packing depends on both: upload() rebuilds it (the automatic block
     // size's preference for the contact path applies from the next upload;
     // both paths run with any block)
@@ -865,6 +869,18 @@ int kscr_download_pos_vel(kscr_handle* h, const float** positions, const float**
     if (positions) *positions = h->pos.data();
     if (velocities) *velocities = h->vel.data();
     return check_device_error(h);
+}
+
+int kscr_get_last_vel(kscr_handle* h, float* last_vel) {
+    if (!h || !last_vel) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = hipMemcpyAsync(last_vel, h->d_u.p, (size_t)h->n * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream)) !=
+        hipSuccess)
+        return h->hip_fail(e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+    return KSCR_OK;
 }
 
 int kscr_get_q(kscr_handle* h, float* q) {
@@ -1338,7 +1354,23 @@ int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
         return hipMemcpyAsync(h->d_q.p, sq.p, (size_t)h->n * 16, k, h->stream);
     };
     if ((e = save(false)) != hipSuccess) return h->hip_fail(e, "autotune save");
-    const float h0 = h->h, lh0 = h->last_h, t0 = h->time;
+    // With contacts, the lists of the last detection are used until the next
+    // one (detection_period > 1): they are restored with the state, and so is
+    // the rebuild clock.  The tree left by the timing runs may differ from
+    // the one before them; a refit of it yields the same overlapping pairs,
+    // and the lists are sorted, so the steps that follow are unchanged.
+    DevBuf<int> s_num, s_cols;
+    const bool with_lists = h->contact.num_cols != nullptr && h->contact.n == h->n;
+    if (with_lists) {
+        if ((e = s_num.alloc(h->n)) != hipSuccess || (e = s_cols.alloc((size_t)h->n * kMaxCols)) != hipSuccess)
+            return h->hip_fail(e, "hipMalloc autotune contact lists");
+        if ((e = hipMemcpyAsync(s_num.p, h->contact.num_cols, (size_t)h->n * 4, hipMemcpyDeviceToDevice,
+                                h->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(s_cols.p, h->contact.cols, (size_t)h->n * kMaxCols * 4, hipMemcpyDeviceToDevice,
+                                h->stream)) != hipSuccess)
+            return h->hip_fail(e, "autotune save contact lists");
+    }
+    const float h0 = h->h, lh0 = h->last_h, t0 = h->time, rebuild0 = h->last_bvh_rebuild;
     const long c0 = h->step_counter;
     const int longest = longest_strand(h);
     const int saved_block = h->p.block_size;
@@ -1353,7 +1385,7 @@ int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
         h->p.block_size = b;
         if ((rc = upload_all(h)) != KSCR_OK) break;
         if ((e = save(true)) != hipSuccess) { rc = h->hip_fail(e, "autotune restore"); break; }
-        h->h = h0, h->last_h = lh0, h->time = t0;
+        h->h = h0, h->last_h = lh0, h->time = t0, h->step_counter = c0, h->last_bvh_rebuild = rebuild0;
         if ((rc = launch_step(h, dt, false)) != KSCR_OK) break;  // warm-up
         (void)hipEventRecord(ev0, h->stream);
         for (int k = 0; k < steps && rc == KSCR_OK; k++) rc = launch_step(h, dt, false);
@@ -1368,8 +1400,15 @@ int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
     h->p.block_size = (rc == KSCR_OK && best > 0) ? best : saved_block;
     int rc2 = upload_all(h);
     if ((e = save(true)) != hipSuccess) return h->hip_fail(e, "autotune restore");
+    if (with_lists && h->contact.num_cols != nullptr && h->contact.n == h->n) {
+        if ((e = hipMemcpyAsync(h->contact.num_cols, s_num.p, (size_t)h->n * 4, hipMemcpyDeviceToDevice,
+                                h->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(h->contact.cols, s_cols.p, (size_t)h->n * kMaxCols * 4, hipMemcpyDeviceToDevice,
+                                h->stream)) != hipSuccess)
+            return h->hip_fail(e, "autotune restore contact lists");
+    }
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
-    h->h = h0, h->last_h = lh0, h->time = t0, h->step_counter = c0;
+    h->h = h0, h->last_h = lh0, h->time = t0, h->step_counter = c0, h->last_bvh_rebuild = rebuild0;
     if (rc) return rc;
     if (rc2) return rc2;
     if (best_block) *best_block = h->block;
@@ -1391,6 +1430,8 @@ struct kscr_batch {
     bool own_stream = false;
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;
+    // recorded after each batch_step; every groom's own stream waits on it
+    hipEvent_t done = nullptr;
     int graph_steps = 0;
     float graph_dt = 0.f;
     std::string err;
@@ -1485,8 +1526,14 @@ int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_si
         return KSCR_ERR_HIP;
     }
     b->own_stream = true;
+    if ((e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming)) != hipSuccess) {
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return KSCR_ERR_HIP;
+    }
     const int rc = batch_set_block(b, block);
     if (rc) {
+        (void)hipEventDestroy(b->done);
         (void)hipStreamDestroy(b->stream);
         delete b;
         return rc;
@@ -1501,6 +1548,7 @@ void kscr_batch_destroy(kscr_batch* b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     b->drop_graph();
+    if (b->done) (void)hipEventDestroy(b->done);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1518,6 +1566,13 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
     if ((e = hipSetDevice(b->device)) != hipSuccess) return b->hip_fail(e, "hipSetDevice");
     for (size_t g = 0; g < b->grooms.size(); g++) {
         kscr_handle* h = b->grooms[g];
+        if (!h->uploaded) return b->fail(KSCR_ERR_NOT_CONFIGURED, "groom " + std::to_string(g) + " is not uploaded");
+        // kscr_set_params may have switched a groom to a path the batched
+        // kernel does not run (it is the contact-free Jacobi step)
+        if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit || h->p.schedule != KSCR_SCHEDULE_JACOBI)
+            return b->fail(KSCR_ERR_UNSUPPORTED, "groom " + std::to_string(g) +
+                                                     ": contacts, step size limits and the coloured schedule are "
+                                                     "not batched");
         int rc = validate_meta(h);
         if (rc) return b->fail(rc, "groom " + std::to_string(g) + ": " + h->err);
         // the handles' own streams may still hold work on their state
@@ -1535,6 +1590,16 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
                             b->stream)) != hipSuccess)
         return b->hip_fail(e, "hipMemcpyAsync batch arguments");
     const int ng = (int)b->grooms.size();
+    // later work queued on a groom's own stream (kscr_get_pos_vel,
+    // kscr_advance, ...) is ordered after this batch's kernels
+    auto order_handles = [&]() -> int {
+        hipError_t r;
+        if ((r = hipEventRecord(b->done, b->stream)) != hipSuccess) return b->hip_fail(r, "hipEventRecord");
+        for (kscr_handle* h : b->grooms)
+            if ((r = hipStreamWaitEvent(h->stream, b->done, 0)) != hipSuccess)
+                return b->hip_fail(r, "hipStreamWaitEvent");
+        return KSCR_OK;
+    };
     if (!b->use_graph) {
         for (int k = 0; k < nsteps; k++) {
             if ((e = launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream)) != hipSuccess)
@@ -1542,7 +1607,7 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
             if ((e = launch_batch_clock(b->d_args.p, ng, dt, b->stream)) != hipSuccess)
                 return b->hip_fail(e, "batch clock launch");
         }
-        return KSCR_OK;
+        return order_handles();
     }
     if (!b->graph || b->graph_steps != nsteps || b->graph_dt != dt) {
         b->drop_graph();
@@ -1569,7 +1634,7 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
         b->graph_dt = dt;
     }
     if ((e = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) return b->hip_fail(e, "hipGraphLaunch");
-    return KSCR_OK;
+    return order_handles();
 }
 
 int kscr_batch_autotune(kscr_batch* b, float dt, int steps, int* best_block) {

@@ -324,6 +324,20 @@ def config_batched(num_grooms=64, strands=(1_000, 50_000), verts=(32, 256), seed
     return out
 
 
+def config_differentiation(num_strands=2_000, verts=64, seed=4):
+    """Differentiation workload: groom of num_strands x verts (10 iterations
+    per substep) and the per-strand parameters its targets come from: Young's
+    modulus 3 GPa and rest-curvature scale 1, each perturbed by a seeded
+    uniform +-20 %.  Returns (scene, E_target[S], kappa_target[S])."""
+    sc = groom(num_strands, verts, seed=seed)
+    sc.name = f"diff{num_strands}x{verts}"
+    sc.material = Material(num_itr=10)
+    rng = np.random.default_rng(seed)
+    e_t = 3.0 * (1 + 0.2 * rng.uniform(-1, 1, num_strands))
+    k_t = 1 + 0.2 * rng.uniform(-1, 1, num_strands)
+    return sc, e_t, k_t
+
+
 def bundle(nx=6, ny=6, nv=32, spacing=6e-5, seg_len=1e-3, jitter=1e-5, seed=7):
     """A grid of nx x ny horizontal strands (along +x) with lateral spacing
     `spacing`; neighbours sit inside the contact barrier (hair-hair contacts)."""

@@ -25,6 +25,7 @@ struct Base {
     virtual int set_params(const orc::Params& p) = 0;
     virtual int upload() = 0;
     virtual void set_pos_vel(const float* p, const float* v) = 0;
+    virtual void set_last_vel(const float* u) = 0;
     virtual void get_pos_vel(float* p, float* v) const = 0;
     virtual void get_pos_vel64(double* p, double* v) const = 0;
     virtual void get_q(float* q) const = 0;
@@ -46,6 +47,7 @@ struct Base {
     virtual void set_material_t(const double* ks, const double* qr) = 0;
     virtual double gradient(float dt, int nsteps, int K, const int* tips, int ntips, const double* targets, double* ksb,
                             double* qrb) = 0;
+    virtual long clamp_log(int enable, uint8_t* out, long cap) = 0;
 };
 
 template <class T>
@@ -81,6 +83,9 @@ struct Impl : Base {
             if (p) s.x[i] = orc::V3<T>{(T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2]};
             if (v) s.v[i] = orc::V3<T>{(T)v[3 * i], (T)v[3 * i + 1], (T)v[3 * i + 2]};
         }
+    }
+    void set_last_vel(const float* u) override {
+        for (int i = 0; i < s.n; i++) s.u[i] = orc::V3<T>{(T)u[3 * i], (T)u[3 * i + 1], (T)u[3 * i + 2]};
     }
     void get_pos_vel(float* p, float* v) const override {
         for (int i = 0; i < s.n; i++) {
@@ -173,6 +178,16 @@ struct Impl : Base {
                      qrb[4 * i + 3] = (double)qb[i].w;
         }
         return (double)loss;
+    }
+    long clamp_log(int enable, uint8_t* out, long cap) override {
+        const long m = (long)s.clamp_log.size();
+        if (out)
+            for (long k = 0; k < std::min(m, cap); k++) out[k] = s.clamp_log[k];
+        if (enable >= 0) {
+            s.log_clamp = enable != 0;
+            if (enable) s.clamp_log.clear();
+        }
+        return m;
     }
     void contacts(int* num, int* cols) const override {
         for (int i = 0; i < s.n; i++) {
@@ -294,6 +309,10 @@ int oracle_set_pos_vel(oracle_handle* h, const float* p, const float* v) {
     h->b->set_pos_vel(p, v);
     return 0;
 }
+int oracle_set_last_vel(oracle_handle* h, const float* u) {
+    h->b->set_last_vel(u);
+    return 0;
+}
 int oracle_get_pos_vel(oracle_handle* h, float* p, float* v) {
     h->b->get_pos_vel(p, v);
     return 0;
@@ -378,6 +397,16 @@ int oracle_set_material64(oracle_handle* h, const double* ks, const double* qr) 
 double oracle_gradient(oracle_handle* h, float dt, int nsteps, int K, const int* tips, int ntips, const double* targets,
                        double* ksb, double* qrb) {
     return h->b->gradient(dt, nsteps, K, tips, ntips, targets, ksb, qrb);
+}
+
+// Branches of the adaptive initial guess's clamp (scr_oracle.hpp init_itr),
+// one byte per vertex and substep, logged while enabled: enable = 1 clears
+// the log and starts logging; enable = 0 stops; enable < 0 changes nothing.
+// Copies up to cap bytes of
+// the log as it was before the call into out (may be NULL) and returns its
+// length.
+long oracle_clamp_log(oracle_handle* h, int enable, uint8_t* out, long cap) {
+    return h->b->clamp_log(enable, out, cap);
 }
 
 // ---- LBVH restatement (lbvh_oracle.hpp) for the bit-exact tree tests

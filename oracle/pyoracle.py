@@ -33,6 +33,7 @@ SIGNATURES = {
     "oracle_upload": (C.c_int, [_P]),
     "oracle_set_pos_vel": (C.c_int, [_P, _FP, _FP]),
     "oracle_get_pos_vel": (C.c_int, [_P, _FP, _FP]),
+    "oracle_set_last_vel": (C.c_int, [_P, _FP]),
     "oracle_get_pos_vel64": (C.c_int, [_P, _DP, _DP]),
     "oracle_get_q": (C.c_int, [_P, _FP]),
     "oracle_get_q64": (C.c_int, [_P, _DP]),
@@ -73,6 +74,7 @@ SIGNATURES = {
     "oracle_set_material": (C.c_int, [_P, _FP, _FP]),
     "oracle_set_material64": (C.c_int, [_P, _DP, _DP]),
     "oracle_gradient": (C.c_double, [_P, C.c_float, C.c_int, C.c_int, _IP, C.c_int, _DP, _DP, _DP]),
+    "oracle_clamp_log": (C.c_long, [_P, C.c_int, C.POINTER(C.c_uint8), C.c_long]),
 }
 
 _lib = None
@@ -176,6 +178,10 @@ class OracleSim:
         v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
         self.lib.oracle_set_pos_vel(self.h, None if p is None else _f(p), None if v is None else _f(v))
 
+    def set_last_vel(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        self.lib.oracle_set_last_vel(self.h, _f(u))
+
     def get_pos_vel(self, f64=False):
         if f64:
             p = np.empty((self.n, 3), np.float64)
@@ -250,6 +256,16 @@ class OracleSim:
         loss = self.lib.oracle_gradient(self.h, dt, nsteps, ckpt_every, _i(tips), tips.size, tg.ctypes.data_as(_DP),
                                         ksb.ctypes.data_as(_DP), qrb.ctypes.data_as(_DP))
         return loss, ksb, qrb
+
+    def log_clamp(self, enable=True):
+        """Starts (enable) or stops logging the branch of the adaptive initial
+        guess's clamp per vertex and substep; returns the log so far as
+        uint8[substeps, n] (0: sigma clamped to 0, 1: interior, 2: clamped
+        to 1, 3: no guess)."""
+        m = self.lib.oracle_clamp_log(self.h, -1, None, 0)
+        out = np.empty(m, np.uint8)
+        self.lib.oracle_clamp_log(self.h, int(bool(enable)), out.ctypes.data_as(C.POINTER(C.c_uint8)), m)
+        return out.reshape(-1, self.n)
 
     def num_pairs(self):
         return self.lib.oracle_num_pairs(self.h)

@@ -862,9 +862,17 @@ struct Sim {
 
     // ------------------------------------------------------------ kernels
     // initItr (YB/sim/iteration.cu:8-59)
+    // Branch of the adaptive initial guess's clamp per vertex and substep
+    // (appended by init_itr while log_clamp is set): 0 = sigma clamped to 0,
+    // 1 = interior, 2 = clamped to 1, 3 = no guess (pinned or no acceleration).
+    bool log_clamp = false;
+    std::vector<uint8_t> clamp_log;
+
     void init_itr() {
         const T hh = h;
         const V3<T> g{(T)p.gravity[0], (T)p.gravity[1], (T)p.gravity[2]};
+        const size_t log0 = clamp_log.size();
+        if (log_clamp) clamp_log.resize(log0 + n, 3);
 #pragma omp parallel for schedule(static) num_threads(threads)
         for (int i = 0; i < n; i++) {
             const V3<T> vel = v[i];
@@ -889,7 +897,9 @@ struct Sim {
                 T ext2 = length2(ext);
                 if (ext2 > 0) {
                     V3<T> a = (vel - lastVel) / lastH;
-                    T s = std::min(std::max(dot(a, ext) / ext2, T(0)), T(1));
+                    const T raw = dot(a, ext) / ext2;
+                    if (log_clamp) clamp_log[log0 + i] = raw <= T(0) ? 0 : (raw >= T(1) ? 2 : 1);
+                    T s = std::min(std::max(raw, T(0)), T(1));
                     d = d + (hh * hh * s) * ext;
                 }
             } else {

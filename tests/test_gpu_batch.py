@@ -10,6 +10,7 @@ import pytest
 from kamisimulator_amd import scenes
 from kamisimulator_amd._lib import Batch, Sim, default_params
 from oracle.pyoracle import OracleSim
+from tests import parity
 
 pytestmark = pytest.mark.gpu
 DT = np.float32(1 / 600)
@@ -56,11 +57,11 @@ def test_batch_matches_oracle(use_graph):
     for o in orcs:
         for _ in range(8):
             o.solve_next_step(DT)
-    for g, (s, o) in enumerate(zip(sims, orcs)):
-        e = _rel(s.get_pos_vel()[0], o.get_pos_vel()[0])
-        eq = _rel(s.get_q(), o.get_q())
-        print(f"[gpu-batch] groom {g}: pos rel {e:.3e} q rel {eq:.3e}", flush=True)
-        assert e <= 1e-5 and eq <= 1e-5
+    for g, (s, o, sc) in enumerate(zip(sims, orcs, grooms)):
+        gp, gv = s.get_pos_vel()
+        op, ov = o.get_pos_vel()
+        parity.check(f"batch groom {g}", {"x": gp, "v": gv, "q": s.get_q()}, {"x": op, "v": ov, "q": o.get_q()},
+                     {"x": sc.positions})
 
 
 def test_batch_equals_per_groom_launches():
@@ -92,3 +93,41 @@ def test_autotune_keeps_state():
         for _ in range(3):
             s.solve_next_step(DT)
     assert np.array_equal(a.get_pos_vel()[0].view(np.uint32), c.get_pos_vel()[0].view(np.uint32))
+
+
+def test_64_grooms_200_steps_full_size():
+    """The batched workload at its full size: 64 grooms with log-uniform
+    1k-50k strands of 32-256 vertices (about 9e7 vertices), each with its own
+    128^3 sphere SDF and gravity direction, 200 steps in one graph-replayed
+    launch per step.  Grooms and strands are independent, so each groom's
+    first and last 3 strands, run alone in the FP32 restatement with that
+    groom's gravity and SDF, must equal the batch word for word."""
+    grooms = scenes.config_batched()
+    nv = sum(sc.num_verts for sc in grooms)
+    print(f"[gpu-batch] 64 grooms, {nv} vertices, 200 steps", flush=True)
+    sims = [_make(sc, Sim) for sc in grooms]
+    b = Batch(sims)
+    b.set_graph(True)
+    for k in range(4):
+        b.step(DT, nsteps=50, with_sdf=True)
+        b.synchronize()
+        print(f"[gpu-batch] step {50 * (k + 1)}", flush=True)
+    b.close()
+    for gi, (sc, s) in enumerate(zip(grooms, sims)):
+        x, v = s.get_pos_vel()
+        q = s.get_q()
+        assert np.isfinite(x).all() and np.isfinite(q).all()
+        ns = sc.num_strands
+        for s0, s1 in ((0, 3), (ns - 3, ns)):
+            a, e = int(sc.strand_offsets[s0]), int(sc.strand_offsets[s1])
+            part = scenes.Scene(f"batched{gi}-{s0}", sc.positions[a:e].copy(),
+                                (sc.strand_offsets[s0:s1 + 1] - a).astype(np.int32), material=sc.material)
+            part.sdf = sc.sdf
+            o = _make(part, OracleSim)
+            q0 = o.get_q()
+            for _ in range(200):
+                o.solve_next_step(DT)
+            op, ov = o.get_pos_vel()
+            parity.check(f"groom {gi} strands {s0}-{s1}", {"x": x[a:e], "v": v[a:e], "q": q[a:e]},
+                         {"x": op, "v": ov, "q": o.get_q()}, {"x": part.positions, "q": q0})
+        s.close()

@@ -9,6 +9,7 @@ import pytest
 from kamisimulator_amd import scenes
 from kamisimulator_amd._lib import SCHEDULE_COLORED, Batch, KscrError, Sim, default_params
 from oracle.pyoracle import OracleSim
+from tests import parity
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -26,16 +27,17 @@ def _run(sc, steps, block=0, label=""):
     g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
     o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets, precision=32), sc)
     o.set_threads(8)
+    q0 = g.get_q()
     for _ in range(steps):
         g.solve_next_step(DT)
         o.solve_next_step(DT)
     gp, gv = g.get_pos_vel()
     op, ov = o.get_pos_vel()
     gq, oq = g.get_q(), o.get_q()
-    ep, eq, ev = _rel(gp, op), _rel(gq, oq), _rel(gv, ov)
-    nbit = int(np.count_nonzero(gp.view(np.uint32) != op.view(np.uint32)))
-    print(f"[gpu-colored] {label}: block {g.block_size} pos rel {ep:.3e} q rel {eq:.3e} vel rel {ev:.3e} "
-          f"differing pos words {nbit}", flush=True)
+    print(f"[gpu-colored] {label}: block {g.block_size}", flush=True)
+    r = parity.check(f"colored {label}", {"x": gp, "v": gv, "q": gq}, {"x": op, "v": ov, "q": oq},
+                     {"x": sc.positions, "q": q0})
+    ep, eq, ev = r["x"]["scale_rel"], r["q"]["scale_rel"], r["v"]["scale_rel"]
     return ep, eq, ev, gp
 
 

@@ -6,6 +6,7 @@ import pytest
 from kamisimulator_amd import scenes
 from kamisimulator_amd._lib import KscrError, Sim, default_params
 from oracle.pyoracle import OracleSim
+from tests import parity
 
 pytestmark = pytest.mark.gpu
 DT = np.float32(1 / 600)
@@ -42,10 +43,8 @@ def test_ribbon_contacts_match_oracle():
     assert on.sum() > 50
     assert np.array_equal(gn, on) and np.array_equal(gc, oc)
     op, ov = o.get_pos_vel()
-    ep, eq = _rel(gp, op), _rel(g.get_q(), o.get_q())
-    print(f"[gpu-contacts] pos rel {ep:.3e} q rel {eq:.3e} differing pos words "
-          f"{int(np.count_nonzero(gp.view(np.uint32) != op.view(np.uint32)))}", flush=True)
-    assert ep <= 1e-5 and eq <= 1e-5
+    parity.check("ribbon-10-steps", {"x": gp, "v": gv, "q": g.get_q()}, {"x": op, "v": ov, "q": o.get_q()},
+                 {"x": sc.positions})
 
 
 def test_groom_with_contacts_and_sdf():
@@ -60,7 +59,10 @@ def test_groom_with_contacts_and_sdf():
     on, _ = o.contacts()
     print(f"[gpu-contacts] groom300 contacts gpu {gn.sum()} oracle {on.sum()}", flush=True)
     assert np.array_equal(gn, on)
-    assert _rel(g.get_pos_vel()[0], o.get_pos_vel()[0]) <= 1e-5
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    parity.check("groom300-contacts-sdf", {"x": gp, "v": gv, "q": g.get_q()}, {"x": op, "v": ov, "q": o.get_q()},
+                 {"x": sc.positions})
 
 
 def test_dense_lognormal_groom_with_contacts():
@@ -80,7 +82,8 @@ def test_dense_lognormal_groom_with_contacts():
     nbit = int(np.count_nonzero(gp.view(np.uint32) != op.view(np.uint32)))
     print(f"[gpu-contacts] lognormal500: contacts gpu {gn.sum()} oracle {on.sum()} differing pos words {nbit}", flush=True)
     assert np.isfinite(gp).all() and np.array_equal(gn, on)
-    assert _rel(gp, op) <= 1e-5 and _rel(gv, ov) <= 1e-5
+    parity.check("lognormal500-contacts", {"x": gp, "v": gv, "q": g.get_q()}, {"x": op, "v": ov, "q": o.get_q()},
+                 {"x": sc.positions})
 
 
 def test_too_many_contacts_is_reported():
@@ -136,3 +139,83 @@ def test_packs_without_contacts_match_oracle_bitwise(num_itr):
     assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))
     assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32))
     assert np.array_equal(g.get_q().view(np.uint32), o.get_q().view(np.uint32))
+
+
+def test_periodic_bvh_rebuild_70_steps():
+    """70 steps = 350 substeps of 1/3000 s: the tree is built at the first
+    detection and rebuilt once lastBVHRebuild reaches 0.1 s (300 substeps;
+    YBSimulator.cu:436-444), with refits in between.  Word for word against the
+    FP32 restatement."""
+    sc = scenes.ribbon(12, 32)
+    rng = np.random.default_rng(3)
+    vel = rng.uniform(-0.02, 0.02, size=sc.positions.shape).astype(np.float32)
+    g, o = _pair(sc)
+    g.set_pos_vel(sc.positions, vel)
+    o.set_pos_vel(sc.positions, vel)
+    q0 = g.get_q()
+    print(f"[gpu-contacts] ribbon verts={sc.num_verts} steps=70", flush=True)
+    for s in range(70):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+    gn, gc = g.get_contacts()
+    on, oc = o.contacts()
+    assert on.sum() > 0 and np.array_equal(gn, on) and np.array_equal(gc, oc)
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    parity.check("ribbon-70-steps", {"x": gp, "v": gv, "q": g.get_q()}, {"x": op, "v": ov, "q": o.get_q()},
+                 {"x": sc.positions, "q": q0})
+
+
+def test_large_groom_with_contacts_full_size():
+    """The large groom at its full size: 100k strands with lognormal vertex
+    counts (median 96, clipped 32-256, 10.2M vertices), sphere + capsule SDF on
+    256^3, hair-hair contacts, 100 steps of 1/600 s (500 substeps, so the
+    0.1 s tree rebuild runs too).  Roots on a spiral and 0.2-0.6 mm curls:
+    uniformly random roots put some roots microns apart, and the strands that
+    grow from them run side by side with more than the 16 contacts per segment
+    the reference treats as fatal.
+
+    The FP32 restatement is too slow at this size for the whole rollout, so the
+    checks are size-independent:
+    - finite state, unit rotors, pinned roots unmoved;
+    - no device error (16-contact lists, pair buffer, query stack);
+    - one more substep's detection from the device's own state (x, v, the
+      previous substep's velocities, q) gives contact lists equal, word for
+      word, to the restatement's LBVH broad phase plus narrow phase on that
+      same state."""
+    sc = scenes.config_large(100_000, seed=2, sdf_dim=256, curl=(2e-4, 6e-4), roots="spiral")
+    p = scenes.contact_params(default_params, sc.material)
+    print(f"[gpu-contacts] large groom verts={sc.num_verts} strands={sc.num_strands} steps=100", flush=True)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    for s in range(100):
+        g.solve_next_step(DT)
+        if (s + 1) % 20 == 0:
+            g.synchronize()
+            print(f"[gpu-contacts] large groom: step {s + 1}", flush=True)
+    x, v = g.get_pos_vel()  # raises on a device error
+    u, q = g.get_last_vel(), g.get_q()
+    n, _ = g.get_contacts()
+    print(f"[gpu-contacts] large groom: contacts after 100 steps {int(n.sum())}, max per segment {int(n.max())}",
+          flush=True)
+    assert np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(q).all()
+    assert np.array_equal(x[sc.roots].view(np.uint32), sc.positions[sc.roots].view(np.uint32))
+    np.testing.assert_allclose(np.linalg.norm(q.astype(np.float64), axis=1), 1.0, atol=1e-5)
+    assert n.sum() > 1000
+    # one substep (h = 1/3000 s, the step's own substep length, so the clock
+    # and the adaptive guess see the same h and last h): detection first
+    sub_dt = np.float32(DT / np.float32(5))
+    g.advance(sub_dt)
+    gn, gc = g.get_contacts()
+    g.get_pos_vel()
+    o = scenes.setup_sim(OracleSim(p, sc.positions, sc.strand_offsets), sc)
+    o.set_threads(16)
+    for rep in range(2):  # the first call sets the oracle's substep length
+        o.set_pos_vel(x, v)
+        o.set_last_vel(u)
+        o.set_q(q)
+        o.detect_only(sub_dt)
+    on, oc = o.contacts()
+    print(f"[gpu-contacts] large groom detection: contacts gpu {int(gn.sum())} oracle {int(on.sum())}, "
+          f"broad-phase pairs {o.num_pairs()}", flush=True)
+    assert o.error() == 0
+    assert np.array_equal(gn, on) and np.array_equal(gc, oc)

@@ -11,8 +11,10 @@ import numpy as np
 import pytest
 
 from kamisimulator_amd import scenes
+from kamisimulator_amd._lib import KscrError
 from kamisimulator_amd._lib import Sim, default_params
 from oracle.pyoracle import OracleSim
+from tests import parity
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -33,28 +35,30 @@ def _pair(sc, block=0, fp64=False):
 
 
 def _rel(a, b):
-    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max() / max(np.abs(b).max(), 1e-30))
+    return parity.scale_rel(a, b)
 
 
-def _run(sc, steps, block=0, fp64=False, label=""):
+def _run(sc, steps, block=0, fp64=False, label="", threads=8):
     print(f"[gpu-parity] {label} verts={sc.num_verts} strands={sc.num_strands} steps={steps} block={block}", flush=True)
     g, o, o64 = _pair(sc, block, fp64)
+    o.set_threads(threads)
+    start = {"x": sc.positions, "q": g.get_q()}
     for s in range(steps):
         g.solve_next_step(DT)
         o.solve_next_step(DT)
         if o64 is not None:
             o64.solve_next_step(DT)
+        if steps >= 50 and (s + 1) % 25 == 0:
+            print(f"[gpu-parity] {label}: step {s + 1}", flush=True)
     gp, gv = g.get_pos_vel()
     op, ov = o.get_pos_vel()
     gq, oq = g.get_q(), o.get_q()
-    ep, eq, ev = _rel(gp, op), _rel(gq, oq), _rel(gv, ov)
-    nbit = int(np.count_nonzero(gp.view(np.uint32) != op.view(np.uint32)))
-    msg = f"[gpu-parity] {label}: pos rel {ep:.3e} q rel {eq:.3e} vel rel {ev:.3e} differing pos words {nbit}"
+    r = parity.check(label, {"x": gp, "v": gv, "q": gq}, {"x": op, "v": ov, "q": oq}, start)
     if o64 is not None:
         p64, _ = o64.get_pos_vel(f64=True)
-        msg += f" | FP32 vs FP64 pos rel {_rel(gp, p64):.3e}"
-    print(msg, flush=True)
-    return ep, eq, ev, g, o
+        print(f"[gpu-parity] {label}: FP32 vs FP64 pos: rel {parity.scale_rel(gp, p64):.3e}, "
+              f"disp-rel {parity.disp_rel(gp, p64, sc.positions):.3e}", flush=True)
+    return r["x"]["scale_rel"], r["q"]["scale_rel"], r["v"]["scale_rel"], g, o
 
 
 def test_download_views_equal_copies():
@@ -139,7 +143,10 @@ def test_advance_without_sdf_matches():
     for s in range(10):
         g.advance(DT)
         o.advance(DT)
-    assert _rel(g.get_pos_vel()[0], o.get_pos_vel()[0]) <= TOL
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    parity.check("advance-no-sdf", {"x": gp, "v": gv, "q": g.get_q()}, {"x": op, "v": ov, "q": o.get_q()},
+                 {"x": sc.positions})
 
 
 def test_headline_size_properties():
@@ -169,4 +176,92 @@ def test_headline_size_properties():
     o = scenes.setup_sim(OracleSim(p, sub.positions, sub.strand_offsets), sub)
     for s in range(5):
         o.solve_next_step(DT)
-    assert _rel(x[s0:s1], o.get_pos_vel()[0]) <= TOL
+    op, ov = o.get_pos_vel()
+    parity.check("headline-strand-12345", {"x": x[s0:s1], "v": v[s0:s1], "q": q[s0:s1]},
+                 {"x": op, "v": ov, "q": o.get_q()}, {"x": sub.positions})
+
+
+def test_set_kstretch_and_kbend_match_oracle():
+    """Sim::setKStretch / Sim::setKBend (YBSimulator.cu:276-298), the
+    UseYoungsModule = false path of the SCR solver: rest data bit for bit,
+    then 20 steps."""
+    sc = scenes.config_groom(300, 64, with_sdf=False)
+    p = scenes.make_params(default_params, sc.material)
+    sims = []
+    for cls in (Sim, OracleSim):
+        s_ = cls(p, sc.positions, sc.strand_offsets)
+        s_.configure(sc.material.density)
+        s_.set_kstretch(2.5e-5)
+        s_.set_kbend(1.5e-9)
+        s_.pin(sc.roots)
+        s_.upload()
+        sims.append(s_)
+    g, o = sims
+    rg, ro = g.get_rest(), o.get_rest()
+    for k in rg:
+        assert np.array_equal(np.ascontiguousarray(rg[k]).view(np.uint32), np.ascontiguousarray(ro[k]).view(np.uint32)), k
+    q0 = g.get_q()
+    for _ in range(20):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    parity.check("set-kstretch-kbend", {"x": gp, "v": gv, "q": g.get_q()}, {"x": op, "v": ov, "q": o.get_q()},
+                 {"x": sc.positions, "q": q0})
+
+
+def test_upload_meta_validation_codes():
+    """uploadMeta's throws (YBSimulator.cu:326-332) as status codes; a
+    rejected parameter set leaves the handle's parameters unchanged."""
+    sc = scenes.config_groom(20, 32, with_sdf=False)
+    p = scenes.make_params(default_params, sc.material)
+    g = scenes.setup_sim(Sim(p, sc.positions, sc.strand_offsets), sc)
+    bad = scenes.make_params(default_params, sc.material)
+    bad.detection_scaler = 1.0
+    with pytest.raises(KscrError, match="DETECTION_SCALER"):
+        g.set_params(bad)
+    assert g.get_params().detection_scaler == pytest.approx(p.detection_scaler)
+    bad = scenes.make_params(default_params, sc.material)
+    bad.barrier_thickness = 5e-3  # 2 * minSegLen < 2 * radius + barrier
+    bad.schedule = 1
+    with pytest.raises(KscrError, match="SEGMENT_TOO_SHORT"):
+        g.set_params(bad)
+    kept = g.get_params()
+    assert kept.barrier_thickness == pytest.approx(p.barrier_thickness) and kept.schedule == 0
+    g.solve_next_step(DT)  # still steps with the kept parameters
+    short = scenes.cantilever(8, 4e-4)
+    h = Sim(default_params(), short.positions, short.strand_offsets)
+    h.configure(1300.0)
+    with pytest.raises(KscrError, match="SEGMENT_TOO_SHORT"):
+        h.upload()
+
+
+def test_groom_10k_x_100_sdf128_100_steps():
+    """The synthetic groom at its full size: 10k strands x 100 vertices, 128^3
+    sphere SDF, 100 steps of 1/600 s, word for word against the FP32
+    restatement over the whole groom.  The FP64 error is stated on the first
+    1000 strands (strands are independent without contacts)."""
+    sc = scenes.config_groom(10_000, 100, sdf_dim=128)
+    label = "groom10k-sdf128"
+    print(f"[gpu-parity] {label} verts={sc.num_verts} strands={sc.num_strands} steps=100", flush=True)
+    g, o, _ = _pair(sc)
+    o.set_threads(16)
+    sub = scenes.Scene("groom10k-first1000", sc.positions[:100_000].copy(), sc.strand_offsets[:1001].copy(),
+                       material=sc.material)
+    sub.sdf = sc.sdf
+    p = scenes.make_params(default_params, sc.material)
+    o64 = scenes.setup_sim(OracleSim(p, sub.positions, sub.strand_offsets, precision=64), sub)
+    o64.set_threads(16)
+    start = {"x": sc.positions, "q": g.get_q()}
+    for s in range(100):
+        g.solve_next_step(DT)
+        o.solve_next_step(DT)
+        o64.solve_next_step(DT)
+        if (s + 1) % 20 == 0:
+            print(f"[gpu-parity] {label}: step {s + 1}", flush=True)
+    gp, gv = g.get_pos_vel()
+    op, ov = o.get_pos_vel()
+    parity.check(label, {"x": gp, "v": gv, "q": g.get_q()}, {"x": op, "v": ov, "q": o.get_q()}, start)
+    p64 = o64.get_pos_vel(f64=True)[0]
+    print(f"[gpu-parity] {label} first 1000 strands, FP32 vs FP64 pos: rel {parity.scale_rel(gp[:100_000], p64):.3e}, "
+          f"disp-rel {parity.disp_rel(gp[:100_000], p64, sub.positions):.3e}", flush=True)

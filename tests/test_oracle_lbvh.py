@@ -46,7 +46,40 @@ def self_check(t, boxes):
     import sys
     sys.setrecursionlimit(10000)
     rng(0)
+    radix_prefix_check(t)
     return True
+
+
+def radix_prefix_check(t):
+    """Karras (2012) radix-tree rule, from its definition: an internal node
+    covering sorted positions [lo, hi] splits where the highest bit in which
+    the 64-bit keys (code << 32 | position) of lo and hi differ turns from 0
+    to 1.  Its left child is node `split` (a leaf if split == lo) and its right
+    child node `split + 1` (a leaf if split + 1 == hi)
+    (KE/KittenGpuLBVH/lbvh.cu:68-150).  The split is found here by scanning
+    the bit over the range, independently of the binary search the oracle
+    uses."""
+    n = len(t["codes"])
+    if n < 2:
+        return
+    keys = (t["codes"].astype(np.uint64) << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+    links = t["links"]
+    leaf = 0x80000000
+    for c in range(n - 1):
+        fence = int(links[c, 3])
+        lo, hi = min(c, fence), max(c, fence)
+        diff = int(keys[lo] ^ keys[hi])
+        assert diff != 0
+        bit = np.uint64(1) << np.uint64(diff.bit_length() - 1)
+        ones = (keys[lo:hi + 1] & bit) != 0
+        split = lo + int(np.argmax(ones)) - 1  # last position with the bit clear
+        assert not ones[:split - lo + 1].any() and ones[split - lo + 1:].all()
+        # every key in the range shares the bits above `bit`
+        assert np.all((keys[lo:hi + 1] ^ keys[lo]) < (bit << np.uint64(1)))
+        want_left = (split | leaf) if split == lo else split
+        want_right = ((split + 1) | leaf) if split + 1 == hi else split + 1
+        assert int(links[c, 1]) == want_left, (c, lo, hi, split)
+        assert int(links[c, 2]) == want_right, (c, lo, hi, split)
 
 
 def check_query(t, pairs, boxes):
