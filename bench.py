@@ -33,10 +33,17 @@ HBM_PEAK_GBS = 8000.0
 # each), q, q_rest, mat (16 B each) = 84 B; write x, v, u (12 B each), q (16 B)
 # = 52 B.
 FUSED_BYTES_PER_VERTEX = 136
-# Streaming-sweep model of the VBD iteration (bytes per update when every
-# iteration goes through HBM): read x, dx, y (12 B each), q, q_rest (16 B
-# each), invMass/lRest/kStretch (12 B), flags (4 B); write dx (12 B), q (16 B).
-STREAMING_BYTES_PER_UPDATE = 112
+# VALU issue of the fused step kernel.  rocprofv3 SQ_INSTS_VALU on the 100k x
+# 100 groom (block 512: 160,000 waves, 50 iterations per launch) counts
+# 2.16016e9 wave instructions per dispatch = 270.02 per wave and iteration
+# (the loop has no data-dependent trip counts, so the count is the same for
+# every dispatch).  gfx950 issues up to two VALU instructions per SIMD per
+# quad-cycle (SQ_ACTIVE_INST_VALU exceeds one per SIMD quad-cycle on this
+# kernel): 256 CUs x 4 SIMDs x 0.5 instructions per cycle x 2.4 GHz =
+# 1.2288e12 wave64 instructions/s, the published FP32 peak of 157.3 TFLOP/s
+# divided by the 128 FLOP of a wave64 FMA.
+VALU_INSTS_PER_WAVE_ITERATION = 270.02
+VALU_PEAK_WAVE_INSTS_PER_S = 256 * 4 * 0.5 * 2.4e9
 # HBM bytes per vertex the fused step kernel actually moves, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE on the 100k x 100 groom, each scaled by
 # what the counter reads for a plain 1 GiB device copy (FETCH x2.0, WRITE x1.0):
@@ -54,14 +61,15 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-sdf-leg", action="store_true", help="skip the with-SDF step timing")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-adjoint-leg", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-batched-leg", action="store_true", help="skip the batched-vs-per-groom timing")
     ap.add_argument("--no-contacts-leg", action="store_true", help="skip the large groom with contacts")
     ap.add_argument("--no-colored-leg", action="store_true", help="skip the even/odd coloured schedule timing")
     ap.add_argument("--no-mesh-sdf-leg", action="store_true", help="skip the mesh -> body SDF grid build timing")
+    ap.add_argument("--no-lbvh-leg", action="store_true", help="skip the LBVH build/refit/query timing")
     ap.add_argument("--mesh-sdf-only", action="store_true", help="run the mesh -> body SDF grid leg only")
-    ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--lbvh-only", action="store_true", help="run the LBVH build/refit/query leg only")
+    ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--only-legs", action="store_true", help="run the adjoint and batched legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (rocprof byte-counter calibration)")
@@ -82,53 +90,87 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_mib": mib, "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}))
 
 
+def pack_count(offsets, capacity):
+    """Workgroups of the library's greedy packing of whole strands into
+    packs of at most `capacity` vertices (one vertex per lane)."""
+    offsets = np.asarray(offsets)
+    packs, s, ns = 0, 0, len(offsets) - 1
+    while s < ns:
+        beg = offsets[s]
+        # the last strand index whose end still fits this pack
+        e = int(np.searchsorted(offsets, beg + capacity, side="right")) - 1
+        s = max(e, s + 1)
+        packs += 1
+    return packs
+
+
 def ensure_built():
     from kamisimulator_amd import build
     if not os.path.exists(build.LIB):
         build.build()
 
 
-def cpu_baseline(scene, params, seconds):
-    """FP32 restatement on the host cores: the first 10k strands of the same
-    groom, stepped until `seconds` of work or 50 steps."""
-    from kamisimulator_amd import scenes
-    from oracle import pyoracle
-
-    pyoracle.build()
-    ns = min(10_000, scene.num_strands)
-    nv = int(scene.strand_offsets[ns])
-    sub = scenes.Scene("cpu-sample", scene.positions[:nv].copy(), scene.strand_offsets[:ns + 1].copy(),
-                       material=scene.material)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    o = pyoracle.OracleSim(params, sub.positions, sub.strand_offsets, precision=32)
-    o.set_threads(threads)
-    scenes.setup_sim(o, sub)
-    dt = np.float32(1 / 600)
-    S = int(np.ceil(dt / np.float32(params.max_h)))
-    steps = 0
-    t0 = time.perf_counter()
-    while steps < 50:
-        o.advance(dt)
-        steps += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    el = time.perf_counter() - t0
-    ups = nv * S * params.num_itr * steps / el
+def cpu_model():
+    """Host CPU model name (lscpu, else /proc/cpuinfo) and logical CPU count."""
     model = ""
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for line in out.splitlines():
             if line.startswith("Model name"):
                 model = line.split(":", 1)[1].strip()
+                break
     except Exception:
         pass
+    if not model:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except Exception:
+            pass
+    return model, os.cpu_count() or 1
+
+
+def cpu_baseline(scene, params, strands=1_000, steps=100, runs=5):
+    """FP32 restatement of the reference step (oracle/) on the host cores: the
+    first `strands` strands of the same groom, `runs` runs of `steps` steps of
+    1/600 s each from the same initial state; the median run is reported.
+    Threads: OMP_NUM_THREADS (the host share this machine grants one GPU),
+    else every logical CPU."""
+    from kamisimulator_amd import scenes
+    from oracle import pyoracle
+
+    pyoracle.build()
+    ns = min(strands, scene.num_strands)
+    nv = int(scene.strand_offsets[ns])
+    sub = scenes.Scene("cpu-sample", scene.positions[:nv].copy(), scene.strand_offsets[:ns + 1].copy(),
+                       material=scene.material)
+    model, nproc = cpu_model()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+    dt = np.float32(1 / 600)
+    S = int(np.ceil(dt / np.float32(params.max_h)))
+    times = []
+    for _ in range(runs):
+        o = pyoracle.OracleSim(params, sub.positions, sub.strand_offsets, precision=32)
+        o.set_threads(threads)
+        scenes.setup_sim(o, sub)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.advance(dt)
+        times.append(time.perf_counter() - t0)
+        o.close()
+    med = float(np.median(times))
+    ups = nv * S * params.num_itr * steps / med
     return {
         "value": ups,
         "unit": "strand-vertex VBD updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP32 C++ restatement of the reference step (oracle/), {ns} strands x {scene.positions.shape[0] // scene.num_strands} "
-                  f"vertices of the same groom, {steps} steps of dt=1/600 ({el:.1f} s), OpenMP {threads} threads, {model}",
+        "sample": f"FP32 C++ restatement of the reference step (oracle/, OpenMP), the first {ns} strands x "
+                  f"{nv // ns} vertices of the same groom ({nv} vertices), median of {runs} runs of {steps} steps of "
+                  f"dt=1/600 ({S} substeps x {params.num_itr} iterations; runs {', '.join(f'{t:.2f}' for t in times)} "
+                  f"s), {threads} threads of {nproc} logical CPUs, {model}",
     }
 
 
@@ -226,6 +268,56 @@ def adjoint_leg(stream, nsteps=50, every=10):
     }
 
 
+def gradient_leg():
+    """The differentiation workload's gradient on the device: 2k strands x 64
+    vertices, 50 steps, checkpoints every 10; targets = the tips after the
+    same rollout with E and kappa perturbed by a seeded +-20 %; dL/dE and
+    dL/dkappa per strand at E = 3 GPa, kappa = 1, through the torch.autograd
+    binding.  Returns what the CPU check needs."""
+    import torch
+
+    from kamisimulator_amd import scenes
+    from kamisimulator_amd.autograd import Rollout
+
+    sc, e_t, k_t = scenes.config_differentiation()
+    ns = sc.num_strands
+    dt = np.float32(1 / 600)
+    roll = Rollout(sc, dt, 50, 10, device=0)
+    with torch.no_grad():
+        x_t, _ = roll(torch.tensor(e_t), torch.tensor(k_t))
+    tips = torch.from_numpy((sc.strand_offsets[1:] - 1).astype(np.int64))
+    targets = x_t[tips].double()
+    E = torch.full((ns,), 3.0, dtype=torch.float64, requires_grad=True)
+    kap = torch.ones(ns, dtype=torch.float64, requires_grad=True)
+    w0 = time.perf_counter()
+    x, _ = roll(E, kap)
+    loss = ((x[tips].double() - targets) ** 2).sum()
+    loss.backward()
+    wall = time.perf_counter() - w0
+    return {"scene": sc, "targets": targets.numpy(), "gE": E.grad.numpy(), "gk": kap.grad.numpy(),
+            "dt": dt, "loss": float(loss), "wall_s": wall}
+
+
+def gradient_accuracy(g, params):
+    """CPU check of the device gradient of gradient_leg against the oracle
+    (oracle/gradient_check.py): FP32 and FP64 gradients of the first 16
+    strands, FP64 central differences on the first 4."""
+    from oracle import gradient_check
+
+    ns = g["scene"].num_strands
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    r = gradient_check.check(g["scene"], params, np.full(ns, 3.0), np.ones(ns), g["dt"], 50, 10, g["targets"],
+                             g["gE"], g["gk"], strands=16, fd_strands=4, threads=threads)
+    r["check_s"] = round(time.perf_counter() - t0, 1)
+    r["note"] = ("the FP64 oracle's adjoint (the same reverse sweep in double) meets 1e-3 against FP64 central "
+                 "differences; the FP32 gradient does not: the FP32 rollout's rounding noise is larger than the "
+                 "FP64 displacement (fp32_position_noise_over_fp64_displacement), because the stretch strain "
+                 "gravity induces (~4e-9) lies below the FP32 resolution of the frame direction (~6e-8), and the "
+                 "adaptive initial guess's clamp takes another branch in FP32 at a large share of vertex-substeps")
+    return r
+
+
 def batched_leg(stream, nsteps, strands=(1_000, 50_000), num_grooms=64):
     """Batched form: num_grooms independent grooms (strand counts log-uniform
     in `strands`, 32-256 vertices per strand, own 128^3 sphere SDF and gravity
@@ -293,7 +385,7 @@ def batched_leg(stream, nsteps, strands=(1_000, 50_000), num_grooms=64):
     }
 
 
-def contacts_leg(stream, nsteps=10, strands=100_000):
+def contacts_leg(stream, nsteps=100, strands=100_000):
     """Large groom with hair-hair contacts: 100k strands with lognormal vertex
     counts (median 96, clipped 32-256) on evenly spaced (spiral) roots with
     0.2-0.6 mm curls, sphere + capsule SDF on 256^3, LBVH broad phase, contacts
@@ -335,6 +427,51 @@ def contacts_leg(stream, nsteps=10, strands=100_000):
         out["error"] = str(e)
     sim.close()
     return out
+
+
+def lbvh_leg(stream, n=100_000, reps=20):
+    """The reference's own LBVH benchmark (Kitten::testLBVH,
+    KE/KittenGpuLBVH/lbvh.cu:554-565: 100k unit-cube points padded by 0.001,
+    srand(1)), with the boxes and the pair buffer (100 n pairs) in device
+    memory as LBVH::compute / refit / query take them.  Each call is timed with
+    HIP events on the launch stream (median of `reps`); compute and query read
+    a count back to the host, as the reference's do (lbvh.cu:393, :425).  The
+    reference's README quotes 150 / 70 / 60 us on an RTX 3090
+    (KittenGpuLBVH/README.md:18)."""
+    import torch
+
+    from kamisimulator_amd import scenes
+    from kamisimulator_amd._lib import Lbvh
+
+    boxes = scenes.lbvh_recipe_boxes(n)
+    d_boxes = torch.from_numpy(boxes).to("cuda:0")
+    cap = 100 * n
+    d_pairs = torch.empty((cap, 2), dtype=torch.int32, device="cuda:0")
+    t = Lbvh(device=0)
+    sp = stream.cuda_stream
+    times = {"build": [], "refit": [], "query": []}
+    count = 0
+    for r in range(reps + 2):  # two warm-up rounds
+        for name in ("build", "refit", "query"):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            if name == "build":
+                t.compute_device(d_boxes.data_ptr(), n, sp)
+            elif name == "refit":
+                t.refit_device(d_boxes.data_ptr(), sp)
+            else:
+                count = t.query_device(d_pairs.data_ptr(), cap, sp)
+            b.record(stream)
+            b.synchronize()
+            if r >= 2:
+                times[name].append(a.elapsed_time(b) * 1e3)
+    t.close()
+    med = {k: round(float(np.median(v)), 1) for k, v in times.items()}
+    ref = {"build": 150.0, "refit": 70.0, "query": 60.0}
+    return {"boxes": n, "pairs": int(count), "us": med,
+            "reference_rtx3090_us": ref,
+            "ratio_vs_reference": {k: round(ref[k] / med[k], 2) for k in med},
+            "timing": f"HIP events on the launch stream around each call, median of {reps}"}
 
 
 def colored_leg(stream, scene, dt, updates_per_step, nsteps):
@@ -434,6 +571,9 @@ def main():
     if args.mesh_sdf_only:
         print(json.dumps({"mesh_sdf": mesh_sdf_leg(cpu=not args.no_cpu)}), flush=True)
         return
+    if args.lbvh_only:
+        print(json.dumps({"lbvh": lbvh_leg(stream)}), flush=True)
+        return
     if args.only_legs:
         legs = {}
         if not args.no_adjoint_leg:
@@ -487,6 +627,9 @@ def main():
           f"{value:.3e} updates/s", file=sys.stderr, flush=True)
 
     achieved = FUSED_BYTES_PER_VERTEX * scene.num_verts / (kern_ms / 1e3) / 1e9
+    waves = pack_count(scene.strand_offsets, block) * (block // 64)
+    valu_insts = VALU_INSTS_PER_WAVE_ITERATION * waves * S * params.num_itr
+    valu_rate = valu_insts / (kern_ms / 1e3)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -498,9 +641,23 @@ def main():
         "algorithmic_bytes_per_launch": FUSED_BYTES_PER_VERTEX * scene.num_verts,
         "avg_launch_ms": round(kern_ms, 4),
         "note": "one launch = one step (5 substeps x 10 iterations) with vertex state resident in registers/LDS; "
-                "algorithmic bytes = compulsory 136 B/vertex per launch, so the kernel is VALU-bound, not HBM-bound",
+                "algorithmic bytes = compulsory 136 B/vertex per launch: the HBM fraction is small by construction "
+                "and the bound is VALU issue (valu)",
+        "valu": {
+            "bound": "valu",
+            "achieved": valu_rate,
+            "peak": VALU_PEAK_WAVE_INSTS_PER_S,
+            "unit": "wave64 VALU instructions/s",
+            "frac": round(valu_rate / VALU_PEAK_WAVE_INSTS_PER_S, 4),
+            "insts_per_launch": valu_insts,
+            "waves_per_launch": waves,
+            "note": "achieved = SQ_INSTS_VALU per launch (270.02 per wave and iteration, rocprofv3) / the launch "
+                    "time measured here; peak at the 2.4 GHz peak engine clock, two VALU issues per SIMD per "
+                    "quad-cycle.  Under load the clock runs near 1.9 GHz (GRBM_GUI_ACTIVE over the dispatch), where "
+                    "SQ_ACTIVE_INST_VALU fills about 0.8 of the issue slots "
+                    "(profiles/step_fused_kernel_groom100kx100.md)",
+        },
     }
-    streaming_equiv = value * STREAMING_BYTES_PER_UPDATE / (HBM_PEAK_GBS * 1e9)
 
     sdf_leg = None
     if not args.no_sdf_leg:
@@ -547,8 +704,13 @@ def main():
                         "instead; not part of value"}
     sim.close()
     adjoint = None
+    grad = None
     if not args.no_adjoint_leg:
+        grad = gradient_leg()
         adjoint = adjoint_leg(stream)
+        adjoint["gradient_workload"] = ("2000 strands x 64 vertices, 50 steps, checkpoints every 10, targets from "
+                                        "E and kappa perturbed by a seeded +-20 %, gradient at E = 3 GPa, kappa = 1 "
+                                        f"(loss {grad['loss']:.6e}; rollout + backward {grad['wall_s']:.2f} s wall)")
         print(f"[bench] adjoint leg: forward {adjoint['forward_ms_per_step']} ms/step, adjoint step "
               f"{adjoint['adjoint_step_over_forward_step']}x", file=sys.stderr, flush=True)
     batched = None
@@ -556,6 +718,11 @@ def main():
         batched = batched_leg(stream, args.batched_steps)
         print(f"[bench] batched leg: per-groom {batched['per_groom_launches_ms_per_step']} ms/step, graph "
               f"{batched['batched_graph_ms_per_step']} ms/step", file=sys.stderr, flush=True)
+
+    lbvh = None
+    if not args.no_lbvh_leg:
+        lbvh = lbvh_leg(stream)
+        print(f"[bench] lbvh leg: {lbvh}", file=sys.stderr, flush=True)
 
     contacts = None
     if not args.no_contacts_leg:
@@ -569,7 +736,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, params, args.cpu_seconds)
+        if grad is not None:
+            from kamisimulator_amd import scenes as _sc
+            adjoint["gradient_accuracy"] = gradient_accuracy(grad, _sc.make_params(default_params, grad["scene"].material))
+            print(f"[bench] gradient accuracy {adjoint['gradient_accuracy']}", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(scene, params)
         print(f"[bench] cpu baseline {cpu['value']:.3e} updates/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
 
     line = {
@@ -595,13 +766,13 @@ def main():
             "block": block,
         },
         "roofline": roofline,
-        "streaming_sweep_equivalent_frac": round(streaming_equiv, 4),
         "with_sdf_step": sdf_leg,
         "colored_step": colored,
         "host_exchange": exchange,
         "adjoint": adjoint,
         "batched": batched,
         "contacts": contacts,
+        "lbvh": lbvh,
         "mesh_sdf": mesh,
         "cpu_baseline": cpu,
     }

@@ -132,6 +132,20 @@ int kscr_pin_vertices(kscr_handle* h, const int* ids, int n);
 /* Sets VertexFlags::fixOrientation on vertices (a true clamp of a root
  * segment; the flag is honoured by cosserat.cu:303). */
 int kscr_fix_orientation(kscr_handle* h, const int* ids, int n);
+/* Vertex::connectionIndex (YarnBall.h:56) of every vertex: conn[i] is the
+ * vertex that vertex i is glued to, or -1 (conn = NULL: none).  The
+ * reference sets it for the two ends of a closed curve (createFromCurves,
+ * YB/io/reader.cpp:40-44) and for curve endpoints within a search radius of
+ * another vertex (glueEndpoints, YB/io/jsonBuilder.cpp:163-188); both
+ * set both directions of a pair, and so must conn (else
+ * KSCR_ERR_INVALID_ARGUMENT).  The position solve then adds a spring of stiffness
+ * 40 towards the glued vertex (cosserat.cu:123-130), and the narrow phase
+ * skips the segment pairs a glue joins (collision.cu:44-49).  A groom with
+ * glued vertices steps through the per-iteration kernels of the contact path
+ * (the partner may lie in another workgroup); the coloured schedule, the
+ * batch and the adjoint reject it (KSCR_ERR_UNSUPPORTED).  Marks the handle
+ * not uploaded. */
+int kscr_set_connections(kscr_handle* h, const int* conn);
 /* SagFree initialisation, as Sim::SagFree(rootVertices, iteration)
  * (YBSimulator.cu:480-676), called by the SCR solver when SagFreeItr > 0
  * (StableCosseratRodsSolver.cpp:207): per strand, rest lengths shortened so
@@ -301,6 +315,18 @@ int kscr_lbvh_refit(kscr_lbvh* b, const float* aabbs);
  * (found object, query object); *count receives the number found, which may
  * exceed cap (only cap pairs are written). */
 int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count);
+/* The same three operations on boxes and pairs that live in device memory,
+ * as the reference's own interface takes them (LBVH::compute(Bound*, size),
+ * LBVH::refit(), LBVH::query(ivec2*, size): lbvh.cu:348-426): d_aabbs is
+ * float[6*n] and d_pairs int[2*cap] on the handle's device, and the work runs
+ * on `stream` (a hipStream_t; NULL = the null stream).  compute_device
+ * returns after the tree is built (it reads the level sizes back, as the
+ * reference reads the tree depth, lbvh.cu:393); query_device returns the
+ * number of pairs found (which may exceed cap), as LBVH::query does
+ * (lbvh.cu:425). */
+int kscr_lbvh_compute_device(kscr_lbvh* b, const float* d_aabbs, int n, void* stream);
+int kscr_lbvh_refit_device(kscr_lbvh* b, const float* d_aabbs, void* stream);
+int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count, void* stream);
 /* Tree contents: sorted Morton codes and object ids (n), internal nodes as
  * (parent, left, right, fence) (4 per node) and child boxes (12 per node),
  * leaf parents (n), root box (6), DFS stack bound. */

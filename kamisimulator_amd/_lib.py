@@ -91,6 +91,7 @@ SIGNATURES = {
     "kscr_get_pos_vel": (C.c_int, [_P, _FP, _FP]),
     "kscr_download_pos_vel": (C.c_int, [_P, C.POINTER(_FP), C.POINTER(_FP)]),
     "kscr_get_last_vel": (C.c_int, [_P, _FP]),
+    "kscr_set_connections": (C.c_int, [_P, _IP]),
     "kscr_get_q": (C.c_int, [_P, _FP]),
     "kscr_set_q": (C.c_int, [_P, _FP]),
     "kscr_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
@@ -128,6 +129,9 @@ SIGNATURES = {
     "kscr_lbvh_compute": (C.c_int, [_P, _FP, C.c_int]),
     "kscr_lbvh_refit": (C.c_int, [_P, _FP]),
     "kscr_lbvh_query": (C.c_int, [_P, _IP, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "kscr_lbvh_compute_device": (C.c_int, [_P, _P, C.c_int, _P]),
+    "kscr_lbvh_refit_device": (C.c_int, [_P, _P, _P]),
+    "kscr_lbvh_query_device": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t), _P]),
     "kscr_lbvh_get": (C.c_int, [_P, _UP, _UP, _UP, _FP, _UP, _FP, _IP]),
 }
 
@@ -314,6 +318,10 @@ class Sim:
         pa.flags.writeable = False
         va.flags.writeable = False
         return pa, va
+
+    def set_connections(self, conn=None):
+        c = None if conn is None else np.ascontiguousarray(conn, dtype=np.int32)
+        self._check(self.lib.kscr_set_connections(self.h, None if c is None else iptr(c)))
 
     def get_last_vel(self):
         u = np.empty((self.n, 3), np.float32)
@@ -511,6 +519,27 @@ class Lbvh:
         if rc:
             raise KscrError(rc, "kscr_lbvh_query failed")
         return out[:min(cnt.value, cap)].copy(), cnt.value
+
+    def compute_device(self, d_aabbs, n, stream=None):
+        """Build from boxes already on the device (d_aabbs: device address of
+        float[6n]); stream: hipStream_t address or None."""
+        self.n = int(n)
+        rc = self.lib.kscr_lbvh_compute_device(self.b, C.c_void_p(d_aabbs), self.n, C.c_void_p(stream))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_compute_device failed")
+
+    def refit_device(self, d_aabbs, stream=None):
+        rc = self.lib.kscr_lbvh_refit_device(self.b, C.c_void_p(d_aabbs), C.c_void_p(stream))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_refit_device failed")
+
+    def query_device(self, d_pairs, cap, stream=None):
+        """Self query into a device buffer int[2*cap]; returns the pair count."""
+        cnt = C.c_size_t(0)
+        rc = self.lib.kscr_lbvh_query_device(self.b, C.c_void_p(d_pairs), int(cap), C.byref(cnt), C.c_void_p(stream))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_query_device failed")
+        return cnt.value
 
     def get(self):
         n = self.n

@@ -148,6 +148,8 @@ struct kscr_handle {
     PinnedBuf<float> pos, vel;
     std::vector<float> inv_mass, l_rest, k_stretch;
     std::vector<uint32_t> flags;
+    std::vector<int> conn;  // Vertex::connectionIndex (YarnBall.h:56), -1 if none
+    bool has_glue = false;
     std::vector<f4> q, q_rest;
     float min_seg = FLT_MAX, max_seg = 0, density = 0;
     bool configured = false;
@@ -157,6 +159,7 @@ struct kscr_handle {
     // device
     DevBuf<float> d_x, d_v, d_u;
     DevBuf<float4> d_q, d_qr, d_mat;
+    DevBuf<int> d_conn;
     DevBuf<int2> d_packs;
     DevBuf<float4> d_grid;
     int num_packs = 0;
@@ -197,14 +200,22 @@ int validate_params(kscr_handle* h, const kscr_params& p) {
     if (!(p.max_h > 0)) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "max_h must be > 0");
     if (p.schedule != KSCR_SCHEDULE_JACOBI && p.schedule != KSCR_SCHEDULE_COLORED)
         return h->fail(KSCR_ERR_INVALID_ARGUMENT, "unknown schedule");
-    if (p.schedule == KSCR_SCHEDULE_COLORED && (p.enable_hair_hair_coll || p.use_step_size_limit))
-        return h->fail(KSCR_ERR_UNSUPPORTED, "the coloured schedule covers the contact-free step (no contacts, no step size limit)");
+    if (p.schedule == KSCR_SCHEDULE_COLORED && (p.enable_hair_hair_coll || p.use_step_size_limit || h->has_glue))
+        return h->fail(KSCR_ERR_UNSUPPORTED, "the coloured schedule covers the contact-free step (no contacts, no step "
+                                             "size limit, no glued vertices)");
     return KSCR_OK;
 }
 
 int validate_meta(kscr_handle* h) { return validate_params(h, h->p); }
 
 bool colored(const kscr_handle* h) { return h->p.schedule == KSCR_SCHEDULE_COLORED; }
+
+// The step runs as per-substep and per-iteration kernels (kscr_contact.hip)
+// with hair-hair contacts, step size limits, or glued vertices: a glued
+// vertex reads its partner's iterate, which may lie in another workgroup.
+bool global_path(const kscr_handle* h) {
+    return h->p.enable_hair_hair_coll || h->p.use_step_size_limit || h->has_glue;
+}
 
 // Vertices a workgroup of `block` threads holds: one per lane, or two for the
 // coloured step (whose pair alignment can leave one lane slot unused).
@@ -245,8 +256,7 @@ int choose_block(const kscr_handle* h) {
     // workgroup per CU leaves the step latency-bound (2000 strands x 64:
     // forward 0.061 ms/step with 512-vertex packs, 0.054 with 64; the
     // adjoint's backward 0.169 and 0.150 ms/step).
-    const bool contact_path = h->p.enable_hair_hair_coll || h->p.use_step_size_limit;
-    const int min_cap = contact_path ? 255 : 511;
+    const int min_cap = global_path(h) ? 255 : 511;
     int chosen = -1;
     for (int b : kBlockSizes)
         if (b <= max_block(h) && pack_capacity(h, b) >= longest && pack_capacity(h, b) >= min_cap) {
@@ -325,6 +335,11 @@ int upload_all(kscr_handle* h) {
         (e = hipMemcpyAsync(h->d_packs.p, packs.data(), packs.size() * sizeof(int2), hipMemcpyHostToDevice, h->stream)) !=
             hipSuccess)
         return h->hip_fail(e, "hipMemcpyAsync upload");
+    if (h->has_glue &&
+        ((e = h->d_conn.alloc(n)) != hipSuccess ||
+         (e = hipMemcpyAsync(h->d_conn.p, h->conn.data(), n * sizeof(int), hipMemcpyHostToDevice, h->stream)) !=
+             hipSuccess))
+        return h->hip_fail(e, "upload connections");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
     h->uploaded = true;
     return KSCR_OK;
@@ -360,6 +375,7 @@ StepArgs step_args(kscr_handle* h, float dt, bool with_sdf) {
     a.wind_sharpness = h->p.wind_sharpness;
     a.sdf_on = (with_sdf && h->sdf_on && h->sdf_set) ? 1 : 0;
     a.sdf = h->sdf;
+    a.conn = h->has_glue ? h->d_conn.p : nullptr;
     h->step_counter += steps;
     h->time += dt;
     return a;
@@ -443,7 +459,7 @@ int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     if (rc) return rc;
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
-    if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit) return launch_contact_step(h, dt, with_sdf);
+    if (global_path(h)) return launch_contact_step(h, dt, with_sdf);
     const StepArgs a = step_args(h, dt, with_sdf);
     e = colored(h) ? launch_step_colored(a, h->d_packs.p, h->num_packs, h->block, h->stream)
                    : launch_step_fused(a, h->d_packs.p, h->num_packs, h->block, h->stream);
@@ -540,6 +556,7 @@ TapeArgs tape_of(kscr_handle* h, size_t slot, size_t j) {
 int adjoint_supported(kscr_handle* h) {
     if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit)
         return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the contact-free step (no contacts, no step size limit)");
+    if (h->has_glue) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers strands without glued vertices");
     if (h->p.wind_peak != 0) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the step without wind");
     if (colored(h)) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the Jacobi schedule");
     return KSCR_OK;
@@ -634,6 +651,7 @@ int kscr_create(const kscr_params* p, int num_verts, const float* positions, int
     h->l_rest.assign(num_verts, 1.f);
     h->k_stretch.assign(num_verts, 100.f);
     h->flags.assign(num_verts, FLAG_NEXT);
+    h->conn.assign(num_verts, -1);
     h->q.assign(num_verts, f4{0, 0, 0, 1});
     h->q_rest.assign(num_verts, f4{0, 0, 0, 1});
     h->flags[num_verts - 1] = 0;
@@ -781,6 +799,29 @@ int kscr_fix_orientation(kscr_handle* h, const int* ids, int n) {
         if (ids[k] < 0 || ids[k] >= h->n) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "vertex id out of range");
         h->flags[ids[k]] |= FLAG_FIX_ORI;
     }
+    h->uploaded = false;
+    return KSCR_OK;
+}
+
+int kscr_set_connections(kscr_handle* h, const int* conn) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (conn) {
+        for (int i = 0; i < h->n; i++)
+            if (conn[i] < -1 || conn[i] >= h->n || conn[i] == i)
+                return h->fail(KSCR_ERR_INVALID_ARGUMENT,
+                               "connection of vertex " + std::to_string(i) + " is neither -1 nor another vertex");
+        for (int i = 0; i < h->n; i++)
+            if (conn[i] >= 0 && conn[conn[i]] != i)
+                return h->fail(KSCR_ERR_INVALID_ARGUMENT, "connections must be symmetric (vertex " +
+                                                              std::to_string(i) + " is glued to " +
+                                                              std::to_string(conn[i]) + ", which is not glued back)");
+    }
+    bool any = false;
+    for (int i = 0; i < h->n; i++) any |= conn && conn[i] >= 0;
+    if (any && h->p.schedule == KSCR_SCHEDULE_COLORED)
+        return h->fail(KSCR_ERR_UNSUPPORTED, "the coloured schedule covers strands without glued vertices");
+    for (int i = 0; i < h->n; i++) h->conn[i] = conn ? conn[i] : -1;
+    h->has_glue = any;
     h->uploaded = false;
     return KSCR_OK;
 }
@@ -1489,7 +1530,7 @@ int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_si
             delete b;
             return KSCR_ERR_NOT_CONFIGURED;
         }
-        if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit || h->p.schedule != KSCR_SCHEDULE_JACOBI) {
+        if (global_path(h) || h->p.schedule != KSCR_SCHEDULE_JACOBI) {
             delete b;
             return KSCR_ERR_UNSUPPORTED;  // the batched kernel is the contact-free Jacobi step
         }
@@ -1569,10 +1610,10 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
         if (!h->uploaded) return b->fail(KSCR_ERR_NOT_CONFIGURED, "groom " + std::to_string(g) + " is not uploaded");
         // kscr_set_params may have switched a groom to a path the batched
         // kernel does not run (it is the contact-free Jacobi step)
-        if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit || h->p.schedule != KSCR_SCHEDULE_JACOBI)
+        if (global_path(h) || h->p.schedule != KSCR_SCHEDULE_JACOBI)
             return b->fail(KSCR_ERR_UNSUPPORTED, "groom " + std::to_string(g) +
-                                                     ": contacts, step size limits and the coloured schedule are "
-                                                     "not batched");
+                                                     ": contacts, step size limits, glued vertices and the coloured "
+                                                     "schedule are not batched");
         int rc = validate_meta(h);
         if (rc) return b->fail(rc, "groom " + std::to_string(g) + ": " + h->err);
         // the handles' own streams may still hold work on their state
@@ -1808,6 +1849,45 @@ int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count) {
         return KSCR_ERR_HIP;
     const size_t m = std::min((size_t)found, cap);
     if (m && hipMemcpy(pairs, b->pairs, m * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    *count = (size_t)found;
+    return err ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
+}
+
+int kscr_lbvh_compute_device(kscr_lbvh* b, const float* d_aabbs, int n, void* stream) {
+    if (!b || !d_aabbs || n <= 0) return KSCR_ERR_INVALID_ARGUMENT;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
+    if (lbvh_reserve(b->d, n) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
+    if (hipMemcpyAsync(b->d.aabb, d_aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        lbvh_build(b->d, n, s) != hipSuccess)
+        return KSCR_ERR_HIP;
+    return KSCR_OK;
+}
+
+int kscr_lbvh_refit_device(kscr_lbvh* b, const float* d_aabbs, void* stream) {
+    if (!b || !d_aabbs || !b->d.built) return KSCR_ERR_INVALID_ARGUMENT;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
+    if (hipMemcpyAsync(b->d.aabb, d_aabbs, (size_t)b->d.n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) !=
+            hipSuccess ||
+        lbvh_refit(b->d, s) != hipSuccess)
+        return KSCR_ERR_HIP;
+    return KSCR_OK;
+}
+
+int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count, void* stream) {
+    if (!b || !count || !b->d.built || (cap > 0 && !d_pairs) || cap > (size_t)INT32_MAX)
+        return KSCR_ERR_INVALID_ARGUMENT;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
+    if (fill_i32(b->error, 0, 1, s) != hipSuccess ||
+        lbvh_query(b->d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, b->error, s) != hipSuccess)
+        return KSCR_ERR_HIP;
+    int found = 0, err = 0;
+    if (hipMemcpyAsync(&found, b->d.count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&err, b->error, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return KSCR_ERR_HIP;
     *count = (size_t)found;
     return err ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
 }

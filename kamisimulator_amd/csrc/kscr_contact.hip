@@ -72,6 +72,10 @@ __global__ void narrow_kernel(StepArgs a, ContactParams cp, const int2* __restri
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
         const int2 ids = pairs[k];
         const int ia = ids.x, ib = ids.y;
+        if (a.conn) {  // glued segments (collision.cu:44-49)
+            const int c0 = a.conn[ia], c1 = a.conn[ia + 1];
+            if (c0 == ib || c1 == ib || c0 == ib + 1 || c1 == ib + 1) continue;
+        }
         if (abs(ib - ia) <= 2) continue;  // neighbouring segments
         const f3 a0 = ld3(a.x, ia);
         const f3 a1 = ld3(a.x, ia + 1) - a0;
@@ -292,6 +296,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         const float a_in = 1.0f / (mi.x * h * h);
         s.H0 = s3{{a_in, a_in, a_in, a_in, a_in, a_in}};
         s.f0 = (1.0f / (h * h * mi.x)) * (ld3(y, i) - dxi);
+        const int g = a.conn ? a.conn[i] : -1;
+        if (g >= 0) {  // the glue to the connected vertex (cosserat.cu:123-130)
+            const float stiffness = 40.0f;
+            s.f0 = s.f0 - stiffness * (((ld3(a.x, i) - ld3(a.x, g)) + (dxi - ld3(dx, g))) + damp * dxi);
+            const float hg = stiffness * (1 + damp);
+            s.H0.d[0] += hg;
+            s.H0.d[1] += hg;
+            s.H0.d[2] += hg;
+        }
     }
     if (act) {
         segment_sums(a, cp, dx, q_in, num_cols, cols, max_step, i, damp, s);
@@ -357,13 +370,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
 
 // contact_pack[p] = 1 when any segment of pack p has a contact in the current
 // lists (the lists are symmetric: both packs of a contact are marked).
+// A pack holding a glued vertex is marked too: the glue partner's iterate is
+// read from global memory (connections are symmetric, so the partner's pack
+// is marked as well).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void classify_kernel(const int2* __restrict__ packs,
                                                          const int* __restrict__ num_cols,
+                                                         const int* __restrict__ conn,
                                                          uint8_t* __restrict__ contact_pack) {
     const int2 pk = packs[blockIdx.x];
     const int i = pk.x + (int)threadIdx.x;
-    const int any = __syncthreads_or(i < pk.y && num_cols[i] > 0);
+    const int any = __syncthreads_or(i < pk.y && (num_cols[i] > 0 || (conn && conn[i] >= 0)));
     if (threadIdx.x == 0) contact_pack[blockIdx.x] = any ? 1 : 0;
 }
 
@@ -598,7 +615,7 @@ hipError_t contact_classify(const StepArgs& a, ContactDev& c, const int2* packs,
     switch (block) {
 #define KSCR_CASE(B)                                                                                       \
     case B:                                                                                                \
-        hipLaunchKernelGGL(classify_kernel<B>, grid, dim3(B), 0, s, packs, c.num_cols, c.contact_pack); \
+        hipLaunchKernelGGL(classify_kernel<B>, grid, dim3(B), 0, s, packs, c.num_cols, a.conn, c.contact_pack); \
         break;
         KSCR_CASE(64)
         KSCR_CASE(128)

@@ -50,6 +50,9 @@ struct StepArgs {
     float wind_peak, wind_y_freq, wind_z_freq, wind_time_freq, wind_sharpness;
     int sdf_on;
     SdfArgs sdf;
+    // Vertex::connectionIndex per vertex (the glued vertex or -1), or null
+    // when no vertex is glued; read by the per-iteration kernels only
+    const int* conn;
 };
 
 // Tape of the differentiable step: for every substep of one recorded step,

@@ -26,6 +26,10 @@ struct alignas(64) LbvhNode {
 static_assert(sizeof(LbvhNode) == 64 && offsetof(LbvhNode, fence) == 12 && offsetof(LbvhNode, b) == 16,
               "LbvhNode layout");
 
+// Karras trees over 62-bit keys (30-bit Morton code, 32-bit index) are at
+// most 63 levels tall.
+constexpr int kMaxLevels = 64;
+
 struct LbvhDev {
     int n = 0;
     int cap = 0;
@@ -51,16 +55,18 @@ struct LbvhDev {
     uint32_t* level_nodes = nullptr;
     uint32_t* level_count = nullptr;   // kMaxLevels counts, then kMaxLevels scatter cursors
     uint32_t* level_offset = nullptr;  // kMaxLevels
-    std::vector<int2> levels;          // (begin, count) per height >= 1, host side
+    int2 levels[kMaxLevels];           // (begin, count) per height >= 1, host side
+    int num_levels = 0;
     int max_depth_host = 64;           // DFS stack bound of the built tree
     bool built = false;
 };
 
-// Karras trees over 62-bit keys (30-bit Morton code, 32-bit index) are at
-// most 63 levels tall.
-constexpr int kMaxLevels = 64;
-
 hipError_t lbvh_reserve(LbvhDev& d, int n);
+// Lays out the buffers of a tree over at most n objects in [base, base +
+// bytes) (256-byte aligned pieces) instead of allocating them; returns the
+// bytes the layout needs, and sets d's pointers only when they fit (base may
+// be null to ask for the size).  Such a view is never passed to lbvh_release.
+size_t lbvh_carve(LbvhDev& d, int n, char* base, size_t bytes);
 void lbvh_release(LbvhDev& d);
 // Builds the tree over d.aabb[0 .. n) (all device work on `stream`).
 hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream);

@@ -8,6 +8,8 @@
 
 #include "kscr_lbvh.h"
 
+#include <type_traits>
+
 namespace kscr {
 namespace {
 
@@ -618,6 +620,47 @@ hipError_t lbvh_reserve(LbvhDev& d, int n) {
     return hipSuccess;
 }
 
+size_t lbvh_carve(LbvhDev& d, int n, char* base, size_t bytes) {
+    const int tiles = (n + kTile - 1) / kTile;
+    const size_t hist_total = (size_t)256 * tiles;
+    const size_t chunks = (hist_total + kScanChunk - 1) / kScanChunk;
+    size_t off = 0;
+    LbvhDev v;
+    auto take = [&](auto*& ptr, size_t count) {
+        using T = std::remove_reference_t<decltype(*ptr)>;
+        off = (off + 255) & ~(size_t)255;
+        ptr = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += count * sizeof(T);
+    };
+    take(v.aabb, (size_t)6 * n);
+    take(v.codes, n);
+    take(v.ids, n);
+    take(v.codes_tmp, n);
+    take(v.ids_tmp, n);
+    take(v.hist, hist_total);
+    take(v.hist_sums, chunks);
+    take(v.partial, 6 * 1024);
+    take(v.root, 6);
+    take(v.nodes, (size_t)(n > 1 ? n - 1 : 1));
+    take(v.leaf_parent, n);
+    take(v.flags, n);
+    take(v.max_depth, 1);
+    take(v.count, 1);
+    take(v.redo, (size_t)n);
+    take(v.redo_count, 1);
+    take(v.node_height, (size_t)n);
+    take(v.level_nodes, (size_t)n);
+    take(v.level_count, (size_t)2 * kMaxLevels);
+    take(v.level_offset, (size_t)kMaxLevels);
+    if (base && off <= bytes) {
+        d = v;
+        d.cap = n;
+        d.n = 0;
+        d.built = false;
+    }
+    return off;
+}
+
 hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
     if (n > d.cap) return hipErrorInvalidValue;
     d.n = n;
@@ -653,7 +696,7 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
                            d.leaf_parent);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    d.levels.clear();
+    d.num_levels = 0;
     if (n <= 1) {
         d.built = true;
         d.max_depth_host = 1;
@@ -679,7 +722,7 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
     uint32_t acc = 0;
     for (int h = 0; h < kMaxLevels; h++) {
         offsets[h] = acc;
-        if (h > 0 && counts[h] > 0) d.levels.push_back(make_int2((int)acc, (int)counts[h]));
+        if (h > 0 && counts[h] > 0) d.levels[d.num_levels++] = make_int2((int)acc, (int)counts[h]);
         acc += counts[h];
     }
     if (acc != (uint32_t)m) return hipErrorUnknown;  // every internal node has one height
@@ -699,16 +742,16 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
 hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
     // levels from `top` on all have fewer than 2 workgroups of nodes
-    size_t top = d.levels.size();
+    int top = d.num_levels;
     while (top > 0 && d.levels[top - 1].y < 2 * kTopBlock) top--;
-    for (size_t l = 0; l < top; l++) {
+    for (int l = 0; l < top; l++) {
         const int2 lv = d.levels[l];
         hipLaunchKernelGGL(refit_level_kernel, dim3((lv.y + 255) / 256), dim3(256), 0, stream, d.aabb, d.ids, d.nodes,
                            d.level_nodes, lv.x, lv.y);
     }
-    if (top < d.levels.size()) {
+    if (top < d.num_levels) {
         LevelList list{};
-        list.num = (int)(d.levels.size() - top);
+        list.num = d.num_levels - top;
         for (int l = 0; l < list.num; l++) list.lv[l] = d.levels[top + l];
         hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, d.aabb, d.ids, d.nodes,
                            d.level_nodes, list);

@@ -338,6 +338,24 @@ def config_differentiation(num_strands=2_000, verts=64, seed=4):
     return sc, e_t, k_t
 
 
+def lbvh_recipe_boxes(n=100_000, pad=0.001):
+    """The reference's LBVH test set (Kitten::testLBVH,
+    KE/KittenGpuLBVH/lbvh.cu:554-565): srand(1), point i = (rand(), rand(),
+    rand()) / (float)RAND_MAX, each box the point padded by R = 0.001.  The C
+    library's rand() is called through ctypes, so on glibc these are the
+    reference's own boxes.  float32[n, 6] (min xyz, max xyz)."""
+    import ctypes
+    import ctypes.util
+
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.rand.restype = ctypes.c_int
+    libc.srand(1)
+    rand_max = 2147483647
+    raw = np.array([libc.rand() for _ in range(3 * n)], np.float64).reshape(n, 3)
+    p = (raw.astype(np.float32) / np.float32(rand_max)).astype(np.float32)
+    return np.concatenate([p - np.float32(pad), p + np.float32(pad)], axis=1).astype(np.float32)
+
+
 def bundle(nx=6, ny=6, nv=32, spacing=6e-5, seg_len=1e-3, jitter=1e-5, seed=7):
     """A grid of nx x ny horizontal strands (along +x) with lateral spacing
     `spacing`; neighbours sit inside the contact barrier (hair-hair contacts)."""

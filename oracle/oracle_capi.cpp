@@ -26,6 +26,7 @@ struct Base {
     virtual int upload() = 0;
     virtual void set_pos_vel(const float* p, const float* v) = 0;
     virtual void set_last_vel(const float* u) = 0;
+    virtual void set_connections(const int* conn) = 0;
     virtual void get_pos_vel(float* p, float* v) const = 0;
     virtual void get_pos_vel64(double* p, double* v) const = 0;
     virtual void get_q(float* q) const = 0;
@@ -83,6 +84,9 @@ struct Impl : Base {
             if (p) s.x[i] = orc::V3<T>{(T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2]};
             if (v) s.v[i] = orc::V3<T>{(T)v[3 * i], (T)v[3 * i + 1], (T)v[3 * i + 2]};
         }
+    }
+    void set_connections(const int* conn) override {
+        for (int i = 0; i < s.n; i++) s.conn[i] = conn ? conn[i] : -1;
     }
     void set_last_vel(const float* u) override {
         for (int i = 0; i < s.n; i++) s.u[i] = orc::V3<T>{(T)u[3 * i], (T)u[3 * i + 1], (T)u[3 * i + 2]};
@@ -307,6 +311,11 @@ int oracle_set_params(oracle_handle* h, const void* params) {
 int oracle_upload(oracle_handle* h) { return h->b->upload(); }
 int oracle_set_pos_vel(oracle_handle* h, const float* p, const float* v) {
     h->b->set_pos_vel(p, v);
+    return 0;
+}
+// Vertex::connectionIndex of every vertex (NULL: none), as kscr_set_connections.
+int oracle_set_connections(oracle_handle* h, const int* conn) {
+    h->b->set_connections(conn);
     return 0;
 }
 int oracle_set_last_vel(oracle_handle* h, const float* u) {

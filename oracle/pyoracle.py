@@ -34,6 +34,7 @@ SIGNATURES = {
     "oracle_set_pos_vel": (C.c_int, [_P, _FP, _FP]),
     "oracle_get_pos_vel": (C.c_int, [_P, _FP, _FP]),
     "oracle_set_last_vel": (C.c_int, [_P, _FP]),
+    "oracle_set_connections": (C.c_int, [_P, _IP]),
     "oracle_get_pos_vel64": (C.c_int, [_P, _DP, _DP]),
     "oracle_get_q": (C.c_int, [_P, _FP]),
     "oracle_get_q64": (C.c_int, [_P, _DP]),
@@ -177,6 +178,10 @@ class OracleSim:
         p = None if pos is None else np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
         v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
         self.lib.oracle_set_pos_vel(self.h, None if p is None else _f(p), None if v is None else _f(v))
+
+    def set_connections(self, conn=None):
+        c = None if conn is None else np.ascontiguousarray(conn, dtype=np.int32)
+        self.lib.oracle_set_connections(self.h, None if c is None else _i(c))
 
     def set_last_vel(self, u):
         u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)

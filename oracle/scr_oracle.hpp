@@ -519,6 +519,10 @@ struct Sim {
     std::vector<float> pos0;  // positions used by configure (float)
     std::vector<float> invMassF, lRestF, kStretchF;
     std::vector<uint32_t> flags;
+    // Vertex::connectionIndex (YarnBall.h:56): the vertex this one is glued to,
+    // -1 if none (closed curves, reader.cpp:40-44; glueEndpoints,
+    // jsonBuilder.cpp:163-188)
+    std::vector<int> conn;
     std::vector<R4<float>> qF, qRestF;
     bool configured = false;
     float density = 0, minSegLen = 0, maxSegLen = 0;
@@ -557,6 +561,7 @@ struct Sim {
         lRestF.assign(nv, 1.f);
         kStretchF.assign(nv, 100.f);
         flags.assign(nv, F_NEXT);
+        conn.assign(nv, -1);
         qF.assign(nv, R4<float>{0, 0, 0, 1});
         qRestF.assign(nv, R4<float>{0, 0, 0, 1});
         flags[nv - 1] = 0;
@@ -979,6 +984,16 @@ struct Sim {
         SegSums s;
         s.H0 = Sym3<T>{{a_in, a_in, a_in, a_in, a_in, a_in}};
         s.f0 = (T(1) / (hh * hh * invMass[i])) * (y[i] - dxi);
+        if (conn[i] >= 0) {
+            // the glue to the connected vertex (cosserat.cu:123-130)
+            const int j = conn[i];
+            const T stiffness = T(40.0f);
+            s.f0 = s.f0 - stiffness * (((x[i] - x[j]) + (dxi - dx[j])) + damp * dxi);
+            const T g = stiffness * (1 + damp);
+            s.H0.d[0] += g;
+            s.H0.d[1] += g;
+            s.H0.d[2] += g;
+        }
         segment_sums(i, damp, s);
         V3<T> f = s.f0 + s.f1;
         Sym3<T> H = s.H0;
@@ -1053,6 +1068,9 @@ struct Sim {
         for (auto& L : cols) L.clear();
         for (const auto& pr : pairs) {
             const int a = pr.first, b = pr.second;
+            // glued segments (collision.cu:44-49)
+            const int c0 = conn[a], c1 = conn[a + 1];
+            if (c0 == b || c1 == b || c0 == b + 1 || c1 == b + 1) continue;
             if (std::abs(b - a) <= 2) continue;  // neighbouring segments
             const V3<T> a0 = x[a];
             const V3<T> a1 = x[a + 1] - a0;

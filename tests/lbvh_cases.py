@@ -4,22 +4,13 @@ recipe_points follows the reference's own recipe (Kitten::testLBVH,
 KE/KittenGpuLBVH/lbvh.cu:554-565): srand(1), points (rand()/RAND_MAX)^3 in the
 unit cube, each box padded by R = 0.001.  glibc's rand() is called through
 ctypes, so the points are the reference's points on this platform."""
-import ctypes
-import ctypes.util
-
 import numpy as np
+
+from kamisimulator_amd import scenes
 
 
 def recipe_points(n=100_000, pad=0.001):
-    libc = ctypes.CDLL(ctypes.util.find_library("c"))
-    libc.rand.restype = ctypes.c_int
-    libc.srand(1)
-    RAND_MAX = 2147483647
-    raw = np.array([libc.rand() for _ in range(3 * n)], np.float64).reshape(n, 3)
-    # rand() / (float)RAND_MAX in float
-    p = (raw.astype(np.float32) / np.float32(RAND_MAX)).astype(np.float32)
-    boxes = np.concatenate([p - np.float32(pad), p + np.float32(pad)], axis=1).astype(np.float32)
-    return boxes
+    return scenes.lbvh_recipe_boxes(n, pad)
 
 
 def segment_boxes(positions, offsets, pad):
