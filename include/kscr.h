@@ -327,6 +327,18 @@ int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count);
 int kscr_lbvh_compute_device(kscr_lbvh* b, const float* d_aabbs, int n, void* stream);
 int kscr_lbvh_refit_device(kscr_lbvh* b, const float* d_aabbs, void* stream);
 int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count, void* stream);
+/* No counterpart in the reference: the LBVH self query without device
+ * allocation, for callers that cannot allocate (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).  kscr_lbvh_compute_device
+ * then kscr_lbvh_query_device over d_aabbs (float[6*n]) into d_pairs
+ * (int[2*cap]), on the current device and `stream`, with the tree in a
+ * fixed device array of the library (one per device, 24 MiB): at most
+ * 131072 boxes, more is KSCR_ERR_INVALID_ARGUMENT.  *count receives the
+ * number of pairs found (which may exceed cap).  Calls that overlap on one
+ * device share that array and race.  err (may be NULL) receives a message on
+ * failure. */
+int kscr_lbvh_self_query_static(const float* d_aabbs, int n, int* d_pairs, size_t cap, size_t* count, void* stream,
+                                char* err, size_t err_len);
 /* Tree contents: sorted Morton codes and object ids (n), internal nodes as
  * (parent, left, right, fence) (4 per node) and child boxes (12 per node),
  * leaf parents (n), root box (6), DFS stack bound. */

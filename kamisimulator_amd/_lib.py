@@ -132,6 +132,8 @@ SIGNATURES = {
     "kscr_lbvh_compute_device": (C.c_int, [_P, _P, C.c_int, _P]),
     "kscr_lbvh_refit_device": (C.c_int, [_P, _P, _P]),
     "kscr_lbvh_query_device": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t), _P]),
+    "kscr_lbvh_self_query_static": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.POINTER(C.c_size_t), _P, C.c_char_p,
+                                              C.c_size_t]),
     "kscr_lbvh_get": (C.c_int, [_P, _UP, _UP, _UP, _FP, _UP, _FP, _IP]),
 }
 

@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libkscr.so")
 
 SOURCES = ["kscr_api.hip", "kscr_step.hip", "kscr_colored.hip", "kscr_adjoint.hip", "kscr_contact.hip", "kscr_lbvh.hip",
-           "kscr_sagfree.hip", "kscr_meshsdf.hip", "kscr_selftest.hip"]
+           "kscr_sagfree.hip", "kscr_meshsdf.hip", "kscr_selftest.hip", "kscr_lbvh_static.hip"]
 HEADERS = ["kscr_math.h", "kscr_device.h", "kscr_solve.h", "kscr_contact.h", "kscr_lbvh.h"]
 
 HIPCC_FLAGS = [
