@@ -1771,14 +1771,12 @@ int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n) {
 // ------------------------------------------------------------------ LBVH
 struct kscr_lbvh {
     int device = 0;
-    LbvhDev d;
+    LbvhDev d;  // its counter block holds the query's error word at d.count[2]
     int2* pairs = nullptr;
-    int* error = nullptr;
     size_t pair_cap = 0;
     ~kscr_lbvh() {
         lbvh_release(d);
         if (pairs) (void)hipFree(pairs);
-        if (error) (void)hipFree(error);
     }
 };
 
@@ -1792,7 +1790,7 @@ int kscr_lbvh_create(int device, kscr_lbvh** out) {
     if (device >= count) return KSCR_ERR_INVALID_ARGUMENT;
     auto* b = new kscr_lbvh;
     b->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipMalloc((void**)&b->error, sizeof(int)) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess) {
         delete b;
         return KSCR_ERR_HIP;
     }
@@ -1840,13 +1838,12 @@ int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count) {
         if (hipMalloc((void**)&b->pairs, cap * sizeof(int2)) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
         b->pair_cap = cap;
     }
-    if (fill_i32(b->error, 0, 1, nullptr) != hipSuccess) return KSCR_ERR_HIP;
-    if (lbvh_query(b->d, b->pairs, (int)cap, -1, b->error, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    if (lbvh_query(b->d, b->pairs, (int)cap, -1, b->d.count + 2, nullptr) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
         return KSCR_ERR_HIP;
-    int found = 0, err = 0;
-    if (hipMemcpy(&found, b->d.count, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&err, b->error, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-        return KSCR_ERR_HIP;
+    int cnt[3] = {0, 0, 0};  // pairs found, redo count, error word
+    if (hipMemcpy(cnt, b->d.count, sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    const int found = cnt[0], err = cnt[2];
     const size_t m = std::min((size_t)found, cap);
     if (m && hipMemcpy(pairs, b->pairs, m * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
     *count = (size_t)found;
@@ -1880,16 +1877,14 @@ int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count
         return KSCR_ERR_INVALID_ARGUMENT;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (fill_i32(b->error, 0, 1, s) != hipSuccess ||
-        lbvh_query(b->d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, b->error, s) != hipSuccess)
+    if (lbvh_query(b->d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, b->d.count + 2, s) != hipSuccess)
         return KSCR_ERR_HIP;
-    int found = 0, err = 0;
-    if (hipMemcpyAsync(&found, b->d.count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(&err, b->error, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    int cnt[3] = {0, 0, 0};  // pairs found, redo count, error word: one copy back
+    if (hipMemcpyAsync(cnt, b->d.count, sizeof cnt, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return KSCR_ERR_HIP;
-    *count = (size_t)found;
-    return err ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
+    *count = (size_t)cnt[0];
+    return cnt[2] ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
 }
 
 int kscr_lbvh_get(kscr_lbvh* b, uint32_t* codes, uint32_t* ids, uint32_t* links, float* bounds, uint32_t* leaf_parent,

@@ -47,17 +47,18 @@ struct LbvhDev {
     uint32_t* leaf_parent = nullptr;
     int* flags = nullptr;
     int* max_depth = nullptr;
-    int* count = nullptr;  // query pair counter
-    int2* redo = nullptr;     // queries deeper than the first pass's stack: (sorted index, pairs reported)
-    int* redo_count = nullptr;
-    // internal nodes grouped by height (leaves 0): refit runs one level per launch
+    // query counters, one block: [0] pairs found, [1] queries to redo,
+    // [2] the error word of the library's own LBVH handle
+    int* count = nullptr;
+    int* redo_count = nullptr;  // count + 1
+    int2* redo = nullptr;       // queries deeper than the first pass's stack: (sorted index, pairs reported)
+    // internal nodes grouped by height (leaves 0): refit runs one level per
+    // launch, from (offset, count) per height kept on the device
     uint32_t* node_height = nullptr;
     uint32_t* level_nodes = nullptr;
-    uint32_t* level_count = nullptr;   // kMaxLevels counts, then kMaxLevels scatter cursors
-    uint32_t* level_offset = nullptr;  // kMaxLevels
-    int2 levels[kMaxLevels];           // (begin, count) per height >= 1, host side
-    int num_levels = 0;
-    int max_depth_host = 64;           // DFS stack bound of the built tree
+    uint32_t* level_count = nullptr;  // kMaxLevels counts, then kMaxLevels scatter cursors
+    int2* level_range = nullptr;      // kMaxLevels (offset, count)
+    int max_depth_host = -1;          // DFS stack bound of the built tree once read back (lbvh_read_depth)
     bool built = false;
 };
 
@@ -72,6 +73,9 @@ void lbvh_release(LbvhDev& d);
 hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream);
 // Recomputes the node boxes from d.aabb for the existing structure.
 hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream);
+// Copies the built tree's DFS stack bound to d.max_depth_host (synchronises
+// the stream).
+hipError_t lbvh_read_depth(LbvhDev& d, hipStream_t stream);
 // Self query: writes pairs (found object, query object), the found leaf lying
 // at a larger sorted position than the query, into pairs[0 .. cap) and the
 // number found (which may exceed cap) into *d.count.  Pairs whose object ids

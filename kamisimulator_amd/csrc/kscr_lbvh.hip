@@ -4,6 +4,8 @@
 // and the tree are bit-exact with the reference's (the sort is stable, the
 // Karras build is deterministic given the sorted keys); refit results do not
 // depend on thread order (min/max and the stack-depth recursion commute).
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 
 #include "kscr_lbvh.h"
@@ -70,20 +72,7 @@ __global__ void root_partial_kernel(const float* __restrict__ aabb, int n, float
     if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void root_final_kernel(const float* __restrict__ partial, int m, float* __restrict__ root) {
-    if (threadIdx.x >= 6) return;
-    const int k = threadIdx.x;
-    float v = k < 3 ? INFINITY : -INFINITY;
-    for (int b = 0; b < m; b++) v = k < 3 ? fminf(v, partial[6 * b + k]) : fmaxf(v, partial[6 * b + k]);
-    root[k] = v;
-}
-
-// mortonKernel (lbvh.cu:53-60)
-__global__ void morton_kernel(const float* __restrict__ aabb, const float* __restrict__ root, int n,
-                              uint32_t* __restrict__ codes, uint32_t* __restrict__ ids) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* a = aabb + 6 * (size_t)i;
+__device__ inline uint32_t morton_code(const float* __restrict__ a, const float* root) {
     int c[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -92,8 +81,7 @@ __global__ void morton_kernel(const float* __restrict__ aabb, const float* __res
         const int v = cvt_rzi(coord);
         c[k] = v < 0 ? 0 : (v > 1023 ? 1023 : v);
     }
-    codes[i] = expand_bits((uint32_t)c[0]) * 4 + expand_bits((uint32_t)c[1]) * 2 + expand_bits((uint32_t)c[2]);
-    ids[i] = (uint32_t)i;
+    return expand_bits((uint32_t)c[0]) * 4 + expand_bits((uint32_t)c[1]) * 2 + expand_bits((uint32_t)c[2]);
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -109,6 +97,84 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// mortonKernel (lbvh.cu:53-60) over one radix tile (kTile objects), fused
+// with the final reduction of the root box (every workgroup reduces the
+// partials of root_partial_kernel; min and max give the same box in any
+// order) and with the first sort pass's digit histogram of the tile.
+__global__ __launch_bounds__(kSortThreads) void morton_hist_kernel(const float* __restrict__ aabb,
+                                                                   const float* __restrict__ partial, int pblocks,
+                                                                   int n, int tiles, float* __restrict__ root_out,
+                                                                   uint32_t* __restrict__ codes,
+                                                                   uint32_t* __restrict__ ids,
+                                                                   uint32_t* __restrict__ hist) {
+    __shared__ float s[6][kSortThreads];
+    __shared__ uint32_t cnt[256];
+    const int t = threadIdx.x;
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = t; b < pblocks; b += kSortThreads)
+        for (int k = 0; k < 6; k++) v[k] = k < 3 ? fminf(v[k], partial[6 * b + k]) : fmaxf(v[k], partial[6 * b + k]);
+    for (int k = 0; k < 6; k++) s[k][t] = v[k];
+    cnt[t] = 0;
+    __syncthreads();
+    for (int w = kSortThreads / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 6; k++)
+                s[k][t] = k < 3 ? fminf(s[k][t], s[k][t + w]) : fmaxf(s[k][t], s[k][t + w]);
+        __syncthreads();
+    }
+    float root[6];
+    for (int k = 0; k < 6; k++) root[k] = s[k][0];
+    if (blockIdx.x == 0 && t < 6) root_out[t] = root[t];
+    const int base = blockIdx.x * kTile;
+    for (int r = 0; r < kSortItems; r++) {
+        const int i = base + r * kSortThreads + t;
+        if (i < n) {
+            const uint32_t code = morton_code(aabb + 6 * (size_t)i, root);
+            codes[i] = code;
+            ids[i] = (uint32_t)i;
+            atomicAdd(&cnt[code & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(size_t)t * tiles + blockIdx.x] = cnt[t];
+}
+
+// Exclusive scan of at most kSmallScanMax values in one workgroup: the values
+// are staged in LDS with coalesced loads, each lane sums a run of consecutive
+// values there, the lanes' sums are scanned, and each lane writes its run's
+// prefixes back to LDS, which is stored coalesced.  (Integer sums: the same
+// values as the three-kernel scan below.)
+constexpr int kSmallScanThreads = 1024;
+constexpr int kSmallScanMax = 1 << 15;
+__global__ __launch_bounds__(kSmallScanThreads) void scan_small_kernel(uint32_t* __restrict__ v, int total) {
+    __shared__ uint32_t vals[kSmallScanMax];
+    __shared__ uint32_t s[kSmallScanThreads];
+    const int t = threadIdx.x;
+    for (int i = t; i < total; i += kSmallScanThreads) vals[i] = v[i];
+    __syncthreads();
+    const int per = (total + kSmallScanThreads - 1) / kSmallScanThreads;
+    const int b = t * per;
+    const int e = min(b + per, total);
+    uint32_t sum = 0;
+    for (int i = b; i < e; i++) sum += vals[i];
+    s[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < kSmallScanThreads; off <<= 1) {
+        const uint32_t x = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = t > 0 ? s[t - 1] : 0u;
+    for (int i = b; i < e; i++) {
+        const uint32_t x = vals[i];
+        vals[i] = run;
+        run += x;
+    }
+    __syncthreads();
+    for (int i = t; i < total; i += kSmallScanThreads) v[i] = vals[i];
 }
 
 // exclusive scan of chunks of kScanChunk values; chunk totals to sums
@@ -224,10 +290,15 @@ __device__ inline int clz64(uint64_t x) { return x ? __clzll(x) : 64; }
 __device__ inline uint64_t sort_key(const uint32_t* codes, int i) { return ((uint64_t)codes[i] << 32) | (uint32_t)i; }
 
 // Karras's internal node construction (lbvh.cu:68-150)
+// Also clears what the height pass accumulates into: each internal node's
+// arrival word and the per-height counters (2 kMaxLevels words).
 __global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n, LbvhNode* __restrict__ nodes,
-                                      uint32_t* __restrict__ leaf_parent) {
+                                      uint32_t* __restrict__ leaf_parent, int* __restrict__ word,
+                                      uint32_t* __restrict__ level_count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * kMaxLevels) level_count[i] = 0;
     if (i >= n - 1) return;
+    word[i] = 0;
     int lo, hi;
     if (i == 0) {
         lo = 0;
@@ -340,13 +411,27 @@ __global__ __launch_bounds__(kLevelBlock) void level_count_kernel(const uint32_t
         if (bins[i]) atomicAdd(level_count + i, bins[i]);
 }
 
+// The offset of each height's nodes is the exclusive sum of the counts below
+// it; every workgroup forms it in LDS, and workgroup 0 publishes (offset,
+// count) per height for the refit kernels.
 __global__ __launch_bounds__(kLevelBlock) void level_scatter_kernel(const uint32_t* __restrict__ node_height, int m,
-                                                                    const uint32_t* __restrict__ level_offset,
-                                                                    uint32_t* __restrict__ cursor,
+                                                                    uint32_t* __restrict__ level_count,
+                                                                    int2* __restrict__ level_range,
                                                                     uint32_t* __restrict__ level_nodes) {
-    __shared__ uint32_t bins[kMaxLevels], start[kMaxLevels];
+    __shared__ uint32_t bins[kMaxLevels], start[kMaxLevels], offset[kMaxLevels];
+    uint32_t* cursor = level_count + kMaxLevels;
     for (int i = threadIdx.x; i < kMaxLevels; i += kLevelBlock) bins[i] = 0;
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int h = 0; h < kMaxLevels; h++) {
+            offset[h] = acc;
+            acc += level_count[h];
+        }
+    }
     __syncthreads();
+    if (blockIdx.x == 0)
+        for (int h = threadIdx.x; h < kMaxLevels; h += kLevelBlock)
+            level_range[h] = make_int2((int)offset[h], (int)level_count[h]);
     const int base = blockIdx.x * kLevelChunk;
     uint32_t h[kLevelPerThread], rank[kLevelPerThread];
     for (int j = 0; j < kLevelPerThread; j++) {
@@ -360,7 +445,7 @@ __global__ __launch_bounds__(kLevelBlock) void level_scatter_kernel(const uint32
     __syncthreads();
     for (int j = 0; j < kLevelPerThread; j++) {
         const int p = base + j * kLevelBlock + threadIdx.x;
-        if (h[j] < (uint32_t)kMaxLevels) level_nodes[level_offset[h[j]] + start[h[j]] + rank[j]] = (uint32_t)p;
+        if (h[j] < (uint32_t)kMaxLevels) level_nodes[offset[h[j]] + start[h[j]] + rank[j]] = (uint32_t)p;
     }
 }
 
@@ -387,27 +472,29 @@ __device__ inline void refit_node(const float* __restrict__ aabb, const uint32_t
     }
 }
 
+// All nodes of height h (grid-stride; the level's range from the device).
 __global__ void refit_level_kernel(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
-                                   LbvhNode* __restrict__ nodes, const uint32_t* __restrict__ level_nodes, int begin,
-                                   int count) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) refit_node(aabb, ids, nodes, level_nodes[begin + k]);
+                                   LbvhNode* __restrict__ nodes, const uint32_t* __restrict__ level_nodes,
+                                   const int2* __restrict__ level_range, int h) {
+    const int2 lv = level_range[h];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < lv.y; k += gridDim.x * blockDim.x)
+        refit_node(aabb, ids, nodes, level_nodes[lv.x + k]);
 }
 
-// The upper levels, each too small to fill the device, in one workgroup: a
-// barrier between levels orders them (the workgroup's writes are visible to
-// it after the barrier), which saves one launch per level.
-struct LevelList {
-    int num;
-    int2 lv[kMaxLevels];  // (offset, count) per level, lowest first
-};
+// The levels from height h0 up to the root, in one workgroup: a barrier
+// between levels orders them (the workgroup's writes are visible to it after
+// the barrier), which saves one launch per level.  Every height from 1 to the
+// root's has nodes (a node's taller child is one lower), so the first empty
+// height ends the tree.
 constexpr int kTopBlock = 1024;
 __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(const float* __restrict__ aabb,
                                                               const uint32_t* __restrict__ ids,
                                                               LbvhNode* __restrict__ nodes,
-                                                              const uint32_t* __restrict__ level_nodes, LevelList levels) {
-    for (int l = 0; l < levels.num; l++) {
-        const int2 lv = levels.lv[l];
+                                                              const uint32_t* __restrict__ level_nodes,
+                                                              const int2* __restrict__ level_range, int h0) {
+    for (int h = h0; h < kMaxLevels; h++) {
+        const int2 lv = level_range[h];
+        if (lv.y == 0) break;
         for (int k = threadIdx.x; k < lv.y; k += kTopBlock) refit_node(aabb, ids, nodes, level_nodes[lv.x + k]);
         __syncthreads();
     }
@@ -588,8 +675,8 @@ hipError_t fill_i32(int* p, int value, size_t n, hipStream_t stream) {
 void lbvh_release(LbvhDev& d) {
     void* ps[] = {d.aabb,  d.codes,       d.ids,         d.codes_tmp, d.ids_tmp,   d.hist,
                   d.hist_sums, d.partial,   d.root,        d.nodes,     d.leaf_parent, d.flags,
-                  d.max_depth, d.count,     d.node_height, d.level_nodes, d.level_count, d.level_offset,
-                  d.redo,      d.redo_count};
+                  d.max_depth, d.count,     d.node_height, d.level_nodes, d.level_count, d.level_range,
+                  d.redo};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     d = LbvhDev{};
@@ -608,14 +695,15 @@ hipError_t lbvh_reserve(LbvhDev& d, int n) {
         (e = dmalloc(d.hist_sums, chunks)) != hipSuccess || (e = dmalloc(d.partial, 6 * 1024)) != hipSuccess ||
         (e = dmalloc(d.root, 6)) != hipSuccess || (e = dmalloc(d.nodes, (size_t)(n > 1 ? n - 1 : 1))) != hipSuccess ||
         (e = dmalloc(d.leaf_parent, n)) != hipSuccess || (e = dmalloc(d.flags, n)) != hipSuccess ||
-        (e = dmalloc(d.max_depth, 1)) != hipSuccess || (e = dmalloc(d.count, 1)) != hipSuccess ||
-        (e = dmalloc(d.redo, (size_t)n)) != hipSuccess || (e = dmalloc(d.redo_count, 1)) != hipSuccess ||
+        (e = dmalloc(d.max_depth, 1)) != hipSuccess || (e = dmalloc(d.count, 4)) != hipSuccess ||
+        (e = dmalloc(d.redo, (size_t)n)) != hipSuccess ||
         (e = dmalloc(d.node_height, (size_t)n)) != hipSuccess || (e = dmalloc(d.level_nodes, (size_t)n)) != hipSuccess ||
         (e = dmalloc(d.level_count, (size_t)2 * kMaxLevels)) != hipSuccess ||
-        (e = dmalloc(d.level_offset, (size_t)kMaxLevels)) != hipSuccess) {
+        (e = dmalloc(d.level_range, (size_t)kMaxLevels)) != hipSuccess) {
         lbvh_release(d);
         return e;
     }
+    d.redo_count = d.count + 1;
     d.cap = n;
     return hipSuccess;
 }
@@ -645,13 +733,13 @@ size_t lbvh_carve(LbvhDev& d, int n, char* base, size_t bytes) {
     take(v.leaf_parent, n);
     take(v.flags, n);
     take(v.max_depth, 1);
-    take(v.count, 1);
+    take(v.count, 4);
     take(v.redo, (size_t)n);
-    take(v.redo_count, 1);
     take(v.node_height, (size_t)n);
     take(v.level_nodes, (size_t)n);
     take(v.level_count, (size_t)2 * kMaxLevels);
-    take(v.level_offset, (size_t)kMaxLevels);
+    take(v.level_range, (size_t)kMaxLevels);
+    v.redo_count = v.count ? v.count + 1 : nullptr;
     if (base && off <= bytes) {
         d = v;
         d.cap = n;
@@ -661,26 +749,56 @@ size_t lbvh_carve(LbvhDev& d, int n, char* base, size_t bytes) {
     return off;
 }
 
+// Heights at or below this are refit one launch per height; the rest by
+// refit_top_kernel.  In Karras trees over spread-out boxes the node count of
+// height h falls by about 0.66 per height from ~0.4 n (100k boxes: 36k, 23k,
+// 15k, 10k, 6.4k, 4.4k, 2.6k, ...); the launches stop near 2 kTopBlock nodes.
+// Any value gives the same boxes.
+static int bottom_levels(int n) {
+    int h = 0;
+    double c = 0.4 * n;
+    while (c >= 2.0 * kTopBlock && h < kMaxLevels - 2) {
+        h++;
+        c *= 0.66;
+    }
+    return h;
+}
+
+// Grid of the refit launch for height h (grid-stride over the actual count).
+static unsigned level_blocks(int n, int h) {
+    double c = 0.4 * n;
+    for (int k = 1; k < h; k++) c *= 0.66;
+    const double b = std::ceil(c / 256.0);
+    return (unsigned)std::max(8.0, std::min(b, 65536.0));
+}
+
 hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
     if (n > d.cap) return hipErrorInvalidValue;
     d.n = n;
     d.built = false;
+    d.max_depth_host = -1;
     if (n <= 0) return hipSuccess;
     const int nb = (n + 255) / 256;
     const int pblocks = nb < 1024 ? nb : 1024;
     hipLaunchKernelGGL(root_partial_kernel, dim3(pblocks), dim3(256), 0, stream, d.aabb, n, d.partial);
-    hipLaunchKernelGGL(root_final_kernel, dim3(1), dim3(64), 0, stream, d.partial, pblocks, d.root);
-    hipLaunchKernelGGL(morton_kernel, dim3(nb), dim3(256), 0, stream, d.aabb, d.root, n, d.codes, d.ids);
     const int tiles = (n + kTile - 1) / kTile;
     d.tiles = tiles;
+    hipLaunchKernelGGL(morton_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, d.aabb, d.partial, pblocks, n,
+                       tiles, d.root, d.codes, d.ids, d.hist);
     const size_t total = (size_t)256 * tiles;
     const int chunks = (int)((total + kScanChunk - 1) / kScanChunk);
     uint32_t *kin = d.codes, *vin = d.ids, *kout = d.codes_tmp, *vout = d.ids_tmp;
     for (int shift = 0; shift < 32; shift += 8) {
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, n, shift, tiles, d.hist);
-        hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
-        hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, stream, d.hist_sums, chunks);
-        hipLaunchKernelGGL(scan_add_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+        if (shift > 0)  // the first pass's histogram comes from morton_hist_kernel
+            hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, n, shift, tiles,
+                               d.hist);
+        if (total <= (size_t)kSmallScanMax) {
+            hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kSmallScanThreads), 0, stream, d.hist, (int)total);
+        } else {
+            hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+            hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, stream, d.hist_sums, chunks);
+            hipLaunchKernelGGL(scan_add_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+        }
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, n, shift, tiles,
                            d.hist, kout, vout);
         uint32_t* t1 = kin;
@@ -691,94 +809,82 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
         vout = t1;
     }
     // four passes: the sorted data is back in d.codes / d.ids
-    if (n > 1)
-        hipLaunchKernelGGL(build_internal_kernel, dim3((n - 1 + 255) / 256), dim3(256), 0, stream, d.codes, n, d.nodes,
-                           d.leaf_parent);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    d.num_levels = 0;
+    hipError_t e;
     if (n <= 1) {
+        if ((e = hipGetLastError()) != hipSuccess) return e;
         d.built = true;
-        d.max_depth_host = 1;
         return fill_i32(d.max_depth, 1, 1, stream);
     }
-    // heights and stack depths (topology only), then the nodes grouped by height
     const int m = n - 1;
-    if ((e = fill_i32(d.flags, 0, (size_t)m, stream)) != hipSuccess ||
-        (e = fill_i32((int*)d.level_count, 0, (size_t)2 * kMaxLevels, stream)) != hipSuccess)
-        return e;
+    const int bthreads = std::max(m, 2 * kMaxLevels);
+    hipLaunchKernelGGL(build_internal_kernel, dim3((bthreads + 255) / 256), dim3(256), 0, stream, d.codes, n, d.nodes,
+                       d.leaf_parent, d.flags, d.level_count);
+    // heights and stack depths (topology only), then the nodes grouped by height
     hipLaunchKernelGGL(height_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d.leaf_parent, d.nodes,
                        (uint32_t*)d.flags, n, d.node_height, d.max_depth);
     const dim3 level_grid((m + kLevelChunk - 1) / kLevelChunk);
     hipLaunchKernelGGL(level_count_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, m, d.level_count);
-    uint32_t counts[kMaxLevels];
-    int depth = 0;
-    if ((e = hipMemcpyAsync(counts, d.level_count, sizeof counts, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(&depth, d.max_depth, sizeof depth, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(stream)) != hipSuccess)
-        return e;
-    d.max_depth_host = depth;
-    uint32_t offsets[kMaxLevels];
-    uint32_t acc = 0;
-    for (int h = 0; h < kMaxLevels; h++) {
-        offsets[h] = acc;
-        if (h > 0 && counts[h] > 0) d.levels[d.num_levels++] = make_int2((int)acc, (int)counts[h]);
-        acc += counts[h];
-    }
-    if (acc != (uint32_t)m) return hipErrorUnknown;  // every internal node has one height
-    if ((e = hipMemcpyAsync(d.level_offset, offsets, sizeof offsets, hipMemcpyHostToDevice, stream)) != hipSuccess)
-        return e;
     hipLaunchKernelGGL(level_scatter_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, m,
-                       d.level_offset, d.level_count + kMaxLevels, d.level_nodes);
+                       d.level_count, d.level_range, d.level_nodes);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    // the host arrays above must outlive the copy
-    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
     d.built = true;
     return lbvh_refit(d, stream);
 }
 
-// One launch per tree level, lowest first; the launch boundaries order the
-// levels (no atomics, no fences).
+// One launch per low tree level, then one workgroup for the rest; the launch
+// boundaries and the workgroup barriers order the levels (no atomics, no
+// fences).  The level ranges live on the device, so nothing is read back.
 hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
-    // levels from `top` on all have fewer than 2 workgroups of nodes
-    int top = d.num_levels;
-    while (top > 0 && d.levels[top - 1].y < 2 * kTopBlock) top--;
-    for (int l = 0; l < top; l++) {
-        const int2 lv = d.levels[l];
-        hipLaunchKernelGGL(refit_level_kernel, dim3((lv.y + 255) / 256), dim3(256), 0, stream, d.aabb, d.ids, d.nodes,
-                           d.level_nodes, lv.x, lv.y);
-    }
-    if (top < d.num_levels) {
-        LevelList list{};
-        list.num = d.num_levels - top;
-        for (int l = 0; l < list.num; l++) list.lv[l] = d.levels[top + l];
-        hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, d.aabb, d.ids, d.nodes,
-                           d.level_nodes, list);
-    }
+    if (d.n < 2) return hipSuccess;
+    const int K = bottom_levels(d.n);
+    for (int h = 1; h <= K; h++)
+        hipLaunchKernelGGL(refit_level_kernel, dim3(level_blocks(d.n, h)), dim3(256), 0, stream, d.aabb, d.ids,
+                           d.nodes, d.level_nodes, d.level_range, h);
+    hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, d.aabb, d.ids, d.nodes, d.level_nodes,
+                       d.level_range, K + 1);
     return hipGetLastError();
 }
 
+hipError_t lbvh_read_depth(LbvhDev& d, hipStream_t stream) {
+    if (!d.built) return hipErrorInvalidValue;
+    int depth = 0;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(&depth, d.max_depth, sizeof depth, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(stream)) != hipSuccess)
+        return e;
+    d.max_depth_host = depth;
+    return hipSuccess;
+}
+
+// Up to this many objects the query runs in one pass with the full stack:
+// its LDS (32 KB per 128-lane workgroup) then still admits enough lanes per
+// CU for all queries.
+constexpr int kOnePassMax = 1 << 20;
+
 hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
-    hipError_t e = fill_i32(d.count, 0, 1, stream);
+    // the pair and redo counters (and the library handle's error word, when
+    // that is the one passed) in one fill
+    hipError_t e = fill_i32(d.count, 0, error == d.count + 2 ? 3 : 2, stream);
     if (e != hipSuccess) return e;
     if (d.n < 2) return hipSuccess;
+    const dim3 grid((d.n + kQueryBlock - 1) / kQueryBlock);
+    if (d.n <= kOnePassMax) {
+        // a Karras tree over 62-bit keys is at most 63 levels tall: a
+        // kQueryStack-entry stack never overflows
+        hipLaunchKernelGGL((query_kernel<kQueryStack, false>), grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids,
+                           d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
+        return hipGetLastError();
+    }
     // A 16-entry LDS stack (10 KB per 128-lane workgroup: 8 waves per SIMD)
     // serves nearly every query; the few that need more are redone with the
-    // stack the tree's DFS bound calls for (known since the build).
-    if ((e = fill_i32(d.redo_count, 0, 1, stream)) != hipSuccess) return e;
-    const dim3 grid((d.n + kQueryBlock - 1) / kQueryBlock);
+    // full stack (the redo pass returns at once when none was listed).
     hipLaunchKernelGGL((query_kernel<kQueryStackFirst, false>), grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids,
                        d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
-    if (d.max_depth_host + 2 <= kQueryStackFirst) return hipGetLastError();  // nothing can overflow
     const dim3 rgrid(std::min<unsigned>(grid.x, kRedoBlocks));
-    if (d.max_depth_host + 2 <= 32)
-        hipLaunchKernelGGL((query_kernel<32, true>), rgrid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes, d.n,
-                           pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
-    else
-        hipLaunchKernelGGL((query_kernel<kQueryStack, true>), rgrid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids,
-                           d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
+    hipLaunchKernelGGL((query_kernel<kQueryStack, true>), rgrid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes,
+                       d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
     return hipGetLastError();
 }
 

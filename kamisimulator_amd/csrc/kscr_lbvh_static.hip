@@ -81,28 +81,26 @@ extern "C" int kscr_lbvh_self_query_static(const float* d_aabbs, int n, int* d_p
         return KSCR_ERR_HIP;
     }
     LbvhDev d;
-    if (lbvh_carve(d, n, base, kScratchBytes - 256) > kScratchBytes - 256) {
+    if (lbvh_carve(d, n, base, kScratchBytes) > kScratchBytes) {
         set_err(err, err_len, "kscr_lbvh_self_query_static: tree does not fit the static buffer");
         return KSCR_ERR_INVALID_ARGUMENT;
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    // the error word sits after the tree: its last 256 bytes are never carved
-    int* error = reinterpret_cast<int*>(base + kScratchBytes - 256);
+    // the query's error word is the third word of the tree's counter block
     if (hipMemcpyAsync(d.aabb, d_aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        lbvh_build(d, n, s) != hipSuccess || fill_i32(error, 0, 1, s) != hipSuccess ||
-        lbvh_query(d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, error, s) != hipSuccess) {
+        lbvh_build(d, n, s) != hipSuccess ||
+        lbvh_query(d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, d.count + 2, s) != hipSuccess) {
         set_err(err, err_len, "kscr_lbvh_self_query_static: kernel launch failed");
         return KSCR_ERR_HIP;
     }
-    int found = 0, e = 0;
-    if (hipMemcpyAsync(&found, d.count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(&e, error, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    int cnt[3] = {0, 0, 0};  // pairs found, redo count, error word
+    if (hipMemcpyAsync(cnt, d.count, sizeof cnt, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         set_err(err, err_len, "kscr_lbvh_self_query_static: copy of the pair count failed");
         return KSCR_ERR_HIP;
     }
-    *count = (size_t)found;
-    if (e) {
+    *count = (size_t)cnt[0];
+    if (cnt[2]) {
         set_err(err, err_len, "kscr_lbvh_self_query_static: a query exceeded the traversal stack");
         return KSCR_ERR_MAX_COLLISIONS;
     }

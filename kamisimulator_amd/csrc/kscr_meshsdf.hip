@@ -734,7 +734,8 @@ int mesh_sdf_build(const float* verts, int nv, const int* tris, int nt, const in
     // nodes outside the sampled cells stay NaN: a read of one would show in the grid
     hipLaunchKernelGGL(fill_nan_kernel, dim3((unsigned)((n_nodes + 255) / 256)), dim3(256), 0, stream, d_coef.p,
                        n_nodes);
-    if ((e = lbvh_build(bvh.d, nt, stream)) != hipSuccess) return fail(e, "triangle BVH build");
+    if ((e = lbvh_build(bvh.d, nt, stream)) != hipSuccess || (e = lbvh_read_depth(bvh.d, stream)) != hipSuccess)
+        return fail(e, "triangle BVH build");
     if (bvh.d.max_depth_host > 64) {
         err = "mesh SDF: triangle BVH deeper than 64 levels";
         return KSCR_ERR_UNSUPPORTED;
