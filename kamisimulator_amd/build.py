@@ -46,8 +46,8 @@ def hipcc():
 
 
 HOST = os.path.join(PKG, "host")
-HOST_SOURCES = ["json_lite.cpp", "hair.cpp", "curve_io.cpp", "scr_solver.cpp", "solver_cli.cpp"]
-HOST_HEADERS = ["json_lite.h", "hair.h", "curve_io.h", "scr_solver.h"]
+HOST_SOURCES = ["json_lite.cpp", "hair.cpp", "curve_io.cpp", "yarn_scene.cpp", "scr_solver.cpp", "solver_cli.cpp"]
+HOST_HEADERS = ["json_lite.h", "hair.h", "curve_io.h", "yarn_scene.h", "scr_solver.h"]
 CLI = os.path.join(PKG, "kscr_solver_cli")
 ROCM_LIB = "/opt/rocm/lib"
 

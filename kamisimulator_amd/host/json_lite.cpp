@@ -90,29 +90,29 @@ struct Reader {
             v.str = string_lit();
         } else if (c == '{') {
             p++;
+            v.kind = JsonValue::OBJECT;
             if (!eat('}')) {
                 do {
-                    (void)string_lit();
+                    std::string k = string_lit();
                     expect(':');
-                    (void)value();
+                    v.members[k] = value();  // a repeated key keeps the last value
                 } while (eat(','));
                 expect('}');
             }
-            v.kind = JsonValue::OTHER;
         } else if (c == '[') {
             p++;
             bool all_numbers = true;
             if (!eat(']')) {
                 do {
-                    JsonValue e = value();
-                    if (e.kind == JsonValue::NUMBER)
-                        v.nums.push_back(e.num);
+                    v.items.push_back(value());
+                    if (v.items.back().kind == JsonValue::NUMBER)
+                        v.nums.push_back(v.items.back().num);
                     else
                         all_numbers = false;
                 } while (eat(','));
                 expect(']');
             }
-            v.kind = all_numbers ? JsonValue::NUMBERS : JsonValue::OTHER;
+            v.kind = all_numbers ? JsonValue::NUMBERS : JsonValue::ARRAY;
         } else if (word("true")) {
             v.kind = JsonValue::BOOL;
             v.b = true;
@@ -135,15 +135,9 @@ JsonFile JsonFile::parse(const std::string& text) {
     JsonFile f;
     f.text_ = text;
     Reader r{f.text_};
-    r.expect('{');
-    if (!r.eat('}')) {
-        do {
-            std::string k = r.string_lit();
-            r.expect(':');
-            f.values_[k] = r.value();
-        } while (r.eat(','));
-        r.expect('}');
-    }
+    r.ws();
+    if (r.p >= f.text_.size() || f.text_[r.p] != '{') r.fail("expected '{'");
+    f.root_ = r.value();
     r.ws();
     if (r.p != f.text_.size()) r.fail("trailing characters");
     return f;
@@ -158,8 +152,8 @@ JsonFile JsonFile::load(const std::string& path) {
 }
 
 const JsonValue& JsonFile::at(const std::string& k, JsonValue::Kind want) const {
-    auto it = values_.find(k);
-    if (it == values_.end()) throw std::runtime_error("JSON key " + k + " is missing");
+    auto it = root_.members.find(k);
+    if (it == root_.members.end()) throw std::runtime_error("JSON key " + k + " is missing");
     const JsonValue& v = it->second;
     const bool ok = v.kind == want || (want == JsonValue::NUMBER && v.kind == JsonValue::BOOL) ||
                     (want == JsonValue::BOOL && v.kind == JsonValue::NUMBER);
@@ -181,5 +175,69 @@ float JsonFile::GetFloat(const std::string& k) const {
 }
 std::string JsonFile::GetString(const std::string& k) const { return at(k, JsonValue::STRING).str; }
 std::vector<double> JsonFile::GetNumbers(const std::string& k) const { return at(k, JsonValue::NUMBERS).nums; }
+
+namespace {
+const JsonValue& null_value() {
+    static const JsonValue v;
+    return v;
+}
+[[noreturn]] void not_convertible(const char* what, const char* to) {
+    throw std::runtime_error(std::string("JSON: ") + what + " is not convertible to " + to);
+}
+}  // namespace
+
+size_t JsonValue::size() const {
+    if (is_array()) return items.size();
+    if (is_object()) return members.size();
+    return 0;
+}
+
+const JsonValue& JsonValue::operator[](size_t i) const {
+    if (kind == NUL) return null_value();
+    if (!is_array()) throw std::runtime_error("JSON: element access on a value that is not an array");
+    return i < items.size() ? items[i] : null_value();
+}
+
+const JsonValue& JsonValue::operator[](const std::string& key) const {
+    if (kind == NUL) return null_value();
+    if (!is_object()) throw std::runtime_error("JSON: member " + key + " read from a value that is not an object");
+    auto it = members.find(key);
+    return it == members.end() ? null_value() : it->second;
+}
+
+float JsonValue::as_float(const char* what) const {
+    switch (kind) {
+        case NUL: return 0.f;
+        case BOOL: return b ? 1.f : 0.f;
+        case NUMBER: return (float)num;
+        default: not_convertible(what, "a number");
+    }
+}
+
+int JsonValue::as_int(const char* what) const {
+    switch (kind) {
+        case NUL: return 0;
+        case BOOL: return b ? 1 : 0;
+        case NUMBER:
+            if (!(num >= -2147483648.0 && num <= 2147483647.0)) not_convertible(what, "an int (out of range)");
+            return (int)num;
+        default: not_convertible(what, "an int");
+    }
+}
+
+bool JsonValue::as_bool(const char* what) const {
+    switch (kind) {
+        case NUL: return false;
+        case BOOL: return b;
+        case NUMBER: return num != 0;
+        default: not_convertible(what, "a bool");
+    }
+}
+
+std::string JsonValue::as_string(const char* what) const {
+    if (kind == NUL) return std::string();
+    if (kind != STRING) not_convertible(what, "a string");
+    return str;
+}
 
 }  // namespace kami_amd

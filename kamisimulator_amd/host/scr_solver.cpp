@@ -211,8 +211,7 @@ void StableCosseratRodsSolver::build_sim() {
 void StableCosseratRodsSolver::LoadSettings(const std::string& settingJsonPath) {
     if (!settingJsonPath.empty() && std::ifstream(settingJsonPath).good())
         jp_ = LoadSettingJsonContent(settingJsonPath, &json_dump_);
-    if (!jp_.YBJsonPath.empty() && std::ifstream(jp_.YBJsonPath).good())
-        throw std::runtime_error("YBJsonPath scenes are not supported by this solver build");
+    use_yarn_scene_ = !jp_.YBJsonPath.empty() && std::ifstream(jp_.YBJsonPath).good();
     if (jp_.SagFreeItr < 0) throw std::runtime_error("SagFreeItr must be >= 0");
 }
 
@@ -222,14 +221,32 @@ void StableCosseratRodsSolver::Construct(const std::string& settingJsonPath) {
         kscr_destroy(sim_);
         sim_ = nullptr;
     }
-    PrepareCurves();
-    build_sim();
+    yarn_.reset();
+    if (!use_yarn_scene_) {
+        PrepareCurves();
+        build_sim();
+        return;
+    }
+    // DivideHair runs before the scene choice (StableCosseratRodsSolver.cpp:40-46)
+    if (hair_ && jp_.SubdivCoeff > 1) divide_hair(*hair_, (uint32_t)jp_.SubdivCoeff);
+    yarn_.reset(new YarnScene(load_yarn_scene(jp_.YBJsonPath)));
+    root_vertices_.clear();
+    strand_refs_.clear();
+    strand_offsets_ = yarn_->strand_offsets;
+    flat_pos_ = yarn_->glued_positions;
+    flat_vel_.assign(flat_pos_.size(), 0.f);
+    sim_ = build_yarn_sim(*yarn_, device_, jp_.Schedule, jp_.BlockSize);
 }
 
 // StableCosseratRodsSolver::SolveNextStep (StableCosseratRodsSolver.cpp:178-198):
 // advance, then the SDF pass (here on the device), then the Hair is updated.
 void StableCosseratRodsSolver::SolveNextStep(float dt) {
     if (!sim_) throw std::runtime_error("Construct() has not been called");
+    if (yarn_) {
+        // no strand is exchanged with the Hair, so the body pass cannot reach the yarn
+        check(kscr_advance(sim_, dt), "advance");
+        return;
+    }
     if (jp_.EnableSDF) {
         if (!sdf_ && !mesh_)
             throw std::runtime_error(
@@ -247,6 +264,9 @@ void StableCosseratRodsSolver::SolveNextStep(float dt) {
 // StableCosseratRodsSolver::UploadHair (StableCosseratRodsSolver.cpp:202-221)
 void StableCosseratRodsSolver::UploadHair() {
     if (!sim_) throw std::runtime_error("Construct() has not been called");
+    // a yarn scene exchanges no strand: the reference re-uploads the state it
+    // downloaded last, which leaves the simulation as it is
+    if (yarn_) return;
     const Hair& h = *hair_;
     const int nv = NumSimVertices();
     ParallelStrands(strand_refs_.size(), [&](size_t i) {
@@ -271,6 +291,7 @@ void StableCosseratRodsSolver::DownloadHair() {
     // layout, strands split over host threads (the reference's OpenMP loop)
     const float *P = nullptr, *V = nullptr;
     check(kscr_download_pos_vel(sim_, &P, &V), "download positions and velocities");
+    if (yarn_) return;  // no strand to exchange
     Hair& h = *hair_;
     const int nv = NumSimVertices();
     ParallelStrands(strand_refs_.size(), [&](size_t i) {
@@ -301,6 +322,7 @@ std::string StableCosseratRodsSolver::GetInfoString() const {
         s << "BendK: " << jp_.BendK << std::endl;
     }
     s << "Sag-free itr: " << jp_.SagFreeItr << std::endl;
+    if (yarn_) s << "YarnBall scene: " << jp_.YBJsonPath << " (" << yarn_->curve_file << ")" << std::endl;
     if (sim_) s << "Simulated vertices: " << NumSimVertices() << ", workgroup " << kscr_block_size(sim_) << std::endl;
     s << "Json:" << std::endl << json_dump_;
     return s.str();

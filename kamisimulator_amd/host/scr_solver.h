@@ -14,8 +14,17 @@
 // runs the SDF pass on the device instead of the host round trip; the
 // settings accept Gravity, SDFThreshold, MaxSubstep, Schedule, BlockSize and
 // DeviceResidentHair (SolveNextStep then leaves the Hair to DownloadHair,
-// which the driver calls once per draw frame).  Not available (Construct
-// throws): YBJsonPath scenes.
+// which the driver calls once per draw frame).
+//
+// YBJsonPath: when the setting names an existing YarnBall scene file, the
+// simulation is built from that scene (yarn_scene.h) instead of the Hair, as
+// the reference does (StableCosseratRodsSolver.cpp:44-45): no strand is
+// exchanged with the Hair (rootVertices and strandRefs stay empty), the
+// scene's own parameters apply, and the settings keys that configure the
+// Hair-built simulation are not used (of the additive keys, Schedule and
+// BlockSize still apply).  The reference's SolveNextStep still runs its host
+// SDF pass over the (unsimulated) Hair, which leaves the yarn untouched; here
+// SolveNextStep advances the yarn only.
 #pragma once
 
 #include <memory>
@@ -24,6 +33,7 @@
 
 #include "hair.h"
 #include "kscr.h"
+#include "yarn_scene.h"
 
 namespace kami_amd {
 
@@ -107,8 +117,8 @@ class StableCosseratRodsSolver {
     // Reads the JSON keys (StableCosseratRodsSolver.cpp:76-110 plus the
     // additive ones); *dump receives the file text.
     static JsonParams LoadSettingJsonContent(const std::string& jsonPath, std::string* dump = nullptr);
-    // The settings part of Construct: reads the JSON file when it exists and
-    // rejects the settings this build does not provide.
+    // The settings part of Construct: reads the JSON file when it exists,
+    // checks SagFreeItr and notes whether YBJsonPath names an existing scene.
     void LoadSettings(const std::string& settingJsonPath);
     // Curve preparation of Construct without the simulation: DivideHair when
     // SubdivCoeff > 1, then per strand (>= 4 vertices) the Catmull-Rom
@@ -120,6 +130,9 @@ class StableCosseratRodsSolver {
     const std::vector<int>& StrandRefs() const { return strand_refs_; }
     int NumSimVertices() const { return (int)flat_pos_.size() / 3; }
     kscr_handle* handle() const { return sim_; }
+    // true when the simulation was built from the YBJsonPath scene
+    bool FromYarnScene() const { return yarn_ != nullptr; }
+    const YarnScene* Scene() const { return yarn_.get(); }
 
    private:
     void check(int rc, const char* what) const;
@@ -132,6 +145,8 @@ class StableCosseratRodsSolver {
     kscr_handle* sim_ = nullptr;
     std::unique_ptr<BodySdf> sdf_;
     std::unique_ptr<BodyMesh> mesh_;
+    std::unique_ptr<YarnScene> yarn_;
+    bool use_yarn_scene_ = false;
     void apply_body();
     std::vector<int> root_vertices_, strand_refs_;
     std::vector<float> flat_pos_, flat_vel_;

@@ -6,10 +6,17 @@
 //   kscr_solver_cli prep <settings.json> <hair.in> <hair.out>
 //       curve preparation of Construct only (no GPU); prints rootVertices and
 //       strandRefs as JSON
-//   kscr_solver_cli run <settings.json> <hair.in> <sdf.bin|mesh.obj|-> <steps> <dt> <hair.out>
+//   kscr_solver_cli run <settings.json> <hair.in|-> <sdf.bin|mesh.obj|-> <steps> <dt> <hair.out>
 //       Construct, then `steps` SolveNextStep(dt); writes the Hair.  A .obj
 //       body is a triangle mesh whose SDF grid (64 x 128 x 64) is built on
-//       the device
+//       the device.  When the settings name a YarnBall scene (YBJsonPath),
+//       the Hair may be "-" and the output is the simulation state instead:
+//       i32 n, f32 positions[3n], f32 velocities[3n]
+//   kscr_solver_cli ybscene <scene.json> <state.out>
+//       the host part of a YarnBall scene (no GPU): prints the parameters,
+//       curve offsets, closed flags, connections and pinned vertices as JSON;
+//       writes i32 n, f32 positions[3n] (as read), f32 positions[3n] (after
+//       glueEndpoints)
 //   kscr_solver_cli mesh2sdf <mesh.obj> <dim_x> <dim_y> <dim_z> <sdf.out>
 //       the body SDF grid of a triangle mesh, built on the device
 //   kscr_solver_cli usc2hair <in.data> <hair.out> [max_strands max_length]
@@ -122,10 +129,57 @@ Transform parse_transform(const char* text) {
     return t;
 }
 
+void write_state(const std::string& path, int n, const float* a, const float* b) {
+    std::ofstream out(path, std::ios::binary);
+    if (!out) throw std::runtime_error("cannot write " + path);
+    out.write((const char*)&n, 4);
+    out.write((const char*)a, 12 * (std::streamsize)n);
+    out.write((const char*)b, 12 * (std::streamsize)n);
+    if (!out) throw std::runtime_error("cannot write " + path);
+}
+
+template <class T>
+void print_list(const char* key, const std::vector<T>& v, const char* fmt, bool last = false) {
+    std::printf("\"%s\": [", key);
+    for (size_t i = 0; i < v.size(); i++) {
+        if (i) std::printf(", ");
+        std::printf(fmt, v[i]);
+    }
+    std::printf("]%s", last ? "" : ", ");
+}
+
+void print_yarn_scene(const YarnScene& sc) {
+    const kscr_params& p = sc.params;
+    std::printf("{\"curveFile\": \"%s\", \"resample\": %d, \"resampleLength\": %.9g, \"breakUpClosedCurves\": %d, ",
+                sc.curve_file.c_str(), (int)sc.resample, sc.resample_length, (int)sc.break_up_closed_curves);
+    std::printf("\"transform\": [");
+    for (int k = 0; k < 16; k++) std::printf("%s%.9g", k ? ", " : "", sc.transform[k]);
+    std::printf("], \"density\": %.9g, \"kStretch\": %.9g, \"kBend\": %.9g, \"glueRadius\": %.9g, ", sc.density,
+                sc.k_stretch, sc.k_bend, sc.glue_radius);
+    std::printf(
+        "\"params\": {\"gravity\": [%.9g, %.9g, %.9g], \"num_itr\": %d, \"max_h\": %.9g, \"damping\": %.9g, "
+        "\"velocity_decay\": %.9g, \"radius\": %.9g, \"barrier_thickness\": %.9g, \"acceleration_ratio\": %.9g, "
+        "\"k_collision\": %.9g, \"k_friction\": %.9g, \"friction_coeff\": %.9g, \"detection_scaler\": %.9g, "
+        "\"bvh_rebuild_period\": %.9g, \"detection_period\": %d, \"use_step_size_limit\": %d, "
+        "\"use_velocity_radius\": %d, \"enable_hair_hair_coll\": %d, \"external_force_multiplier\": %.9g, "
+        "\"wind_peak\": %.9g, \"wind_y_freq\": %.9g, \"wind_z_freq\": %.9g, \"wind_time_freq\": %.9g, "
+        "\"wind_sharpness\": %.9g}, ",
+        p.gravity[0], p.gravity[1], p.gravity[2], p.num_itr, p.max_h, p.damping, p.velocity_decay, p.radius,
+        p.barrier_thickness, p.acceleration_ratio, p.k_collision, p.k_friction, p.friction_coeff, p.detection_scaler,
+        p.bvh_rebuild_period, p.detection_period, p.use_step_size_limit, p.use_velocity_radius,
+        p.enable_hair_hair_coll, p.external_force_multiplier, p.wind_peak, p.wind_y_freq, p.wind_z_freq,
+        p.wind_time_freq, p.wind_sharpness);
+    print_list("strandOffsets", sc.strand_offsets, "%d");
+    print_list("closed", sc.closed, "%d");
+    print_list("connections", sc.connections, "%d");
+    print_list("pinned", sc.pinned, "%d", true);
+    std::printf("}\n");
+}
+
 int usage() {
     std::fprintf(stderr,
                  "usage: kscr_solver_cli parse <json> | prep <json> <hair.in> <hair.out> | "
-                 "run <json> <hair.in> <sdf.bin|mesh.obj|-> <steps> <dt> <hair.out> | "
+                 "run <json> <hair.in|-> <sdf.bin|mesh.obj|-> <steps> <dt> <out> | ybscene <scene.json> <state.out> | "
                  "mesh2sdf <mesh.obj> <dx> <dy> <dz> <sdf.out> | (file conversions: see the source)\n");
     return 2;
 }
@@ -194,8 +248,14 @@ int main(int argc, char** argv) {
             write_sdf(argv[6], g);
             return 0;
         }
+        if (cmd == "ybscene" && argc == 4) {
+            const YarnScene sc = load_yarn_scene(argv[2]);
+            write_state(argv[3], (int)sc.positions.size() / 3, sc.positions.data(), sc.glued_positions.data());
+            print_yarn_scene(sc);
+            return 0;
+        }
         if (cmd == "run" && argc == 8) {
-            auto hair = std::make_shared<Hair>(Hair::read(argv[3]));
+            auto hair = std::string(argv[3]) == "-" ? nullptr : std::make_shared<Hair>(Hair::read(argv[3]));
             StableCosseratRodsSolver s(hair);
             const std::string body = argv[4];
             if (ends_with(body, ".obj") || ends_with(body, ".OBJ")) {
@@ -209,6 +269,17 @@ int main(int argc, char** argv) {
             const int steps = std::atoi(argv[5]);
             const float dt = (float)std::atof(argv[6]);
             for (int k = 0; k < steps; k++) s.SolveNextStep(dt);
+            if (s.FromYarnScene()) {
+                const int n = s.NumSimVertices();
+                std::vector<float> pos(3 * (size_t)n), vel(3 * (size_t)n);
+                if (kscr_get_pos_vel(s.handle(), pos.data(), vel.data()) != KSCR_OK)
+                    throw std::runtime_error("kscr_get_pos_vel failed");
+                write_state(argv[7], n, pos.data(), vel.data());
+                print_refs(s);
+                std::fprintf(stderr, "%s", s.GetInfoString().c_str());
+                return 0;
+            }
+            if (!hair) throw std::runtime_error("a Hair file is needed unless the settings name a YarnBall scene");
             s.DownloadHair();  // the driver's draw-frame download (KamiSimulator.cpp:389)
             hair->write(argv[7]);
             print_refs(s);
