@@ -41,7 +41,10 @@ typedef enum kscr_status {
     KSCR_ERR_SEGMENT_TOO_SHORT = 6,    /* uploadMeta, 2 minSegLen < 2r + b  */
     KSCR_ERR_NOT_CONFIGURED = 7,       /* setYoung/setK* before configure   */
     KSCR_ERR_MAX_COLLISIONS = 8,       /* > 16 contacts on one segment      */
-    KSCR_ERR_STRAND_TOO_LONG = 9,      /* strand longer than one workgroup  */
+    KSCR_ERR_STRAND_TOO_LONG = 9,      /* strand longer than the coloured
+                                        * schedule's workgroup (1023) or
+                                        * than an explicit block_size on a
+                                        * path that cannot cut strands     */
     KSCR_ERR_HIP = 10,                 /* HIP runtime failure               */
     KSCR_ERR_OUT_OF_MEMORY = 11,
     KSCR_ERR_UNSUPPORTED = 12

@@ -143,6 +143,7 @@ struct kscr_handle {
     int n = 0;
     int device = 0;
     std::vector<int> strand_off;
+    int longest = 0;  // vertices of the longest strand
     // host mirror (YarnBall::Sim::verts/vels/qs/qRests); positions and
     // velocities page-locked for the per-step exchange
     PinnedBuf<float> pos, vel;
@@ -210,11 +211,22 @@ int validate_meta(kscr_handle* h) { return validate_params(h, h->p); }
 
 bool colored(const kscr_handle* h) { return h->p.schedule == KSCR_SCHEDULE_COLORED; }
 
+// Largest workgroup of the step kernels: a strand longer than this is split
+// over several packs.
+constexpr int kMaxBlock = 1024;
+
+// Jacobi strands longer than the largest workgroup are split over packs that
+// overlap by a one-vertex halo at each cut (iterate_kernel), as the
+// reference runs chains of any length in overlapping blocks
+// (cosserat.cu:30-37).
+bool split_strands(const kscr_handle* h) { return !colored(h) && h->longest > kMaxBlock; }
+
 // The step runs as per-substep and per-iteration kernels (kscr_contact.hip)
-// with hair-hair contacts, step size limits, or glued vertices: a glued
-// vertex reads its partner's iterate, which may lie in another workgroup.
+// with hair-hair contacts, step size limits, glued vertices (a glued vertex
+// reads its partner's iterate, which may lie in another workgroup) or split
+// strands (a cut's neighbours lie in another workgroup).
 bool global_path(const kscr_handle* h) {
-    return h->p.enable_hair_hair_coll || h->p.use_step_size_limit || h->has_glue;
+    return h->p.enable_hair_hair_coll || h->p.use_step_size_limit || h->has_glue || split_strands(h);
 }
 
 // Vertices a workgroup of `block` threads holds: one per lane, or two for the
@@ -244,6 +256,8 @@ int count_packs(const kscr_handle* h, int capacity) {
 int choose_block(const kscr_handle* h) {
     const int longest = longest_strand(h);
     if (h->p.block_size > 0) return h->p.block_size <= max_block(h) ? h->p.block_size : -1;
+    // split strands: the contact path's pack size; longer strands are cut
+    if (split_strands(h)) return 256;
     // Smallest instantiated block that holds the longest strand, but holding
     // at least 512 vertices so that several ~100-vertex strands share a
     // workgroup (fewer idle lanes).  The contact path prefers 256: smaller
@@ -276,10 +290,16 @@ int choose_block(const kscr_handle* h) {
     return -1;
 }
 
-// Greedy packing of whole strands into workgroups of at most `block` vertices.
+// Greedy packing of whole strands into workgroups of at most `block`
+// vertices.  With split strands, a strand longer than `block` gets packs of
+// its own: each owns a run of vertices and adds the vertex before it (when
+// the strand continues before) and the vertex after it (when it continues
+// after) as halos, so the pack [x, y) starts with hasPrev or ends with hasNext
+// exactly where it is cut.
 int build_packs(kscr_handle* h, int block, std::vector<int2>& packs) {
     packs.clear();
     const int ns = (int)h->strand_off.size() - 1;
+    const bool split = split_strands(h);
     int s = 0;
     while (s < ns) {
         const int beg = h->strand_off[s];
@@ -288,14 +308,27 @@ int build_packs(kscr_handle* h, int block, std::vector<int2>& packs) {
             end = h->strand_off[s + 1];
             s++;
         }
+        if (end == beg && split && block >= 3) {
+            const int e = h->strand_off[s + 1];
+            for (int own = beg; own < e;) {
+                const int before = own > beg ? 1 : 0;
+                const int own_end = (e - own) + before <= block ? e : own + (block - before - 1);
+                packs.push_back(make_int2(own - before, own_end + (own_end < e ? 1 : 0)));
+                own = own_end;
+            }
+            s++;
+            continue;
+        }
         if (end == beg)
             return h->fail(KSCR_ERR_STRAND_TOO_LONG, "strand " + std::to_string(s) + " has " +
                                                          std::to_string(h->strand_off[s + 1] - h->strand_off[s]) +
                                                          " vertices, more than the workgroup size " + std::to_string(block));
         packs.push_back(make_int2(beg, end));
     }
-    // The kernel reads neighbours at t-1 / t+1 only through hasPrev / hasNext;
-    // a pack must therefore start without hasPrev and end without hasNext.
+    if (split) return KSCR_OK;
+    // The fused kernels read neighbours at t-1 / t+1 only through hasPrev /
+    // hasNext; a pack must therefore start without hasPrev and end without
+    // hasNext.
     for (const int2& pk : packs) {
         if (h->flags[pk.x] & FLAG_PREV) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "pack starts inside a strand");
         if (h->flags[pk.y - 1] & FLAG_NEXT) return h->fail(KSCR_ERR_INVALID_ARGUMENT, "pack ends inside a strand");
@@ -467,11 +500,7 @@ int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     return KSCR_OK;
 }
 
-int longest_strand(const kscr_handle* h) {
-    int longest = 0;
-    for (size_t s = 0; s + 1 < h->strand_off.size(); s++) longest = std::max(longest, h->strand_off[s + 1] - h->strand_off[s]);
-    return longest;
-}
+int longest_strand(const kscr_handle* h) { return h->longest; }
 
 // Re-uploads (invMass, lRest, kStretch, flags) and qRest from the host mirror,
 // leaving the dynamic state on the device untouched.
@@ -557,6 +586,9 @@ int adjoint_supported(kscr_handle* h) {
     if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit)
         return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the contact-free step (no contacts, no step size limit)");
     if (h->has_glue) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers strands without glued vertices");
+    if (split_strands(h))
+        return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers strands of at most " + std::to_string(kMaxBlock) +
+                                                 " vertices");
     if (h->p.wind_peak != 0) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the step without wind");
     if (colored(h)) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the Jacobi schedule");
     return KSCR_OK;
@@ -638,6 +670,7 @@ int kscr_create(const kscr_params* p, int num_verts, const float* positions, int
     h->n = num_verts;
     h->device = device;
     h->strand_off.assign(strand_offsets, strand_offsets + num_strands + 1);
+    for (int s = 0; s < num_strands; s++) h->longest = std::max(h->longest, strand_offsets[s + 1] - strand_offsets[s]);
     const size_t n3 = (size_t)num_verts * 3;
     if (h->pos.resize(n3) != hipSuccess || h->vel.resize(n3) != hipSuccess) {
         g_create_error = "hipHostMalloc of the host mirror failed";
@@ -1422,7 +1455,7 @@ int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
     float best_ms = 0;
     int rc = KSCR_OK;
     for (int b : kBlockSizes) {
-        if (b > max_block(h) || pack_capacity(h, b) < longest) continue;
+        if (b > max_block(h) || (pack_capacity(h, b) < longest && !split_strands(h))) continue;
         h->p.block_size = b;
         if ((rc = upload_all(h)) != KSCR_OK) break;
         if ((e = save(true)) != hipSuccess) { rc = h->hip_fail(e, "autotune restore"); break; }

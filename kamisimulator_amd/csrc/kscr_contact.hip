@@ -261,6 +261,13 @@ __device__ inline void segment_sums(const StepArgs& a, const ContactParams& cp, 
 // lie in other workgroups (Jacobi snapshot).  Only the packs that hold a
 // contact in this substep take part (contact_pack); a partner's pack holds
 // the contact too, so no lane reads the iterate of a pack left out.
+// A pack cut out of a long strand starts with a halo vertex (hasPrev at lane
+// 0) and/or ends with one (hasNext at the last vertex).  The first halo's
+// lane forms its segment's sums, whose share its right neighbour needs; the
+// last halo's lane runs its position solve, whose new dx the last owned
+// vertex's orientation solve needs.  Both repeat, operation for operation,
+// what the owning pack computes for those vertices; only owned vertices are
+// written.
 // amdgpu_waves_per_eu(5): 96 VGPRs (3 values spilled to scratch, reloaded
 // in the loop over a segment's contacts) instead of 103, 5 waves per SIMD
 // instead of 4; with 256-thread
@@ -285,6 +292,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
     const bool act = i < pk.y;
     const float4 mi = act ? a.mat[i] : make_float4(0.f, 1.f, 0.f, 0.f);
     const uint32_t fl = act ? __float_as_uint(mi.w) : 0u;
+    const bool halo_before = t == 0 && (fl & FLAG_PREV);
+    const bool own = act && !halo_before && !(i == pk.y - 1 && (fl & FLAG_NEXT));
     const bool free = act && mi.x != 0.f;
     const f3 dxi = act ? ld3(dx, i) : f3{0, 0, 0};
     const float h = a.h;
@@ -330,7 +339,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         s3 H = s.H0;
         sadd(H, s.H1);
         float limit = s.limit;
-        if (fl & FLAG_PREV) {
+        if ((fl & FLAG_PREV) && !halo_before) {
             const f3 f2{s_f2[0][t - 1], s_f2[1][t - 1], s_f2[2][t - 1]};
             s3 H2;
             for (int c = 0; c < 6; c++) H2.d[c] = s_H2[c][t - 1];
@@ -349,12 +358,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
             if (l > limit && l > 0) out = out * (limit / l);
         }
     }
-    if (act) st3(dx_out, i, out);
+    if (own) st3(dx_out, i, out);
     s_dxn[0][t] = out.x;
     s_dxn[1][t] = out.y;
     s_dxn[2][t] = out.z;
     __syncthreads();
-    if (!act) return;
+    if (!own) return;
     const f4 q = to4(q_in[i]);
     f4 qn = q;
     if ((fl & FLAG_NEXT) && !(fl & FLAG_FIX_ORI)) {
@@ -372,15 +381,24 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
 // lists (the lists are symmetric: both packs of a contact are marked).
 // A pack holding a glued vertex is marked too: the glue partner's iterate is
 // read from global memory (connections are symmetric, so the partner's pack
-// is marked as well).
+// is marked as well).  So is every pack cut out of a long strand (it starts
+// with hasPrev or ends with hasNext): its neighbours across a cut lie in
+// another pack, which is cut too.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void classify_kernel(const int2* __restrict__ packs,
+                                                         const float4* __restrict__ mat,
                                                          const int* __restrict__ num_cols,
                                                          const int* __restrict__ conn,
                                                          uint8_t* __restrict__ contact_pack) {
     const int2 pk = packs[blockIdx.x];
     const int i = pk.x + (int)threadIdx.x;
-    const int any = __syncthreads_or(i < pk.y && (num_cols[i] > 0 || (conn && conn[i] >= 0)));
+    bool mark = false;
+    if (i < pk.y) {
+        const uint32_t fl = __float_as_uint(mat[i].w);
+        const bool cut = (i == pk.x && (fl & FLAG_PREV)) || (i == pk.y - 1 && (fl & FLAG_NEXT));
+        mark = cut || num_cols[i] > 0 || (conn && conn[i] >= 0);
+    }
+    const int any = __syncthreads_or(mark);
     if (threadIdx.x == 0) contact_pack[blockIdx.x] = any ? 1 : 0;
 }
 
@@ -615,7 +633,7 @@ hipError_t contact_classify(const StepArgs& a, ContactDev& c, const int2* packs,
     switch (block) {
 #define KSCR_CASE(B)                                                                                       \
     case B:                                                                                                \
-        hipLaunchKernelGGL(classify_kernel<B>, grid, dim3(B), 0, s, packs, c.num_cols, a.conn, c.contact_pack); \
+        hipLaunchKernelGGL(classify_kernel<B>, grid, dim3(B), 0, s, packs, a.mat, c.num_cols, a.conn, c.contact_pack); \
         break;
         KSCR_CASE(64)
         KSCR_CASE(128)
