@@ -200,6 +200,11 @@ std::vector<double> resample_two_way(const std::vector<Vec3>& cmr, double start,
 }
 }  // namespace
 
+std::vector<double> resample_cmr_coords(const std::vector<Vec3>& cmr, double start, double end, double tar) {
+    if (cmr.size() < 4) throw std::runtime_error("resample_cmr needs at least 4 control points");
+    return resample_two_way(cmr, start, end, tar);
+}
+
 void sample_cmr(const std::vector<Vec3>& cmr, double t, double out[3]) {
     const D3 p = sample(cmr, t);
     out[0] = p.x, out[1] = p.y, out[2] = p.z;

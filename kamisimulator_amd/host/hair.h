@@ -54,5 +54,9 @@ std::vector<Vec3> resample_cmr(const std::vector<Vec3>& cmr, double start, doubl
 
 // The spline point at parameter t (Resample::sampleCurve, resample.h:10-18).
 void sample_cmr(const std::vector<Vec3>& cmr, double t, double out[3]);
+// One pass of resample_cmr: the spline parameters of the points, marched
+// from both ends at chord length tar_seg_len and spliced where the two
+// lists come closest (twoDirectionResampleCMRCoords, resample.h:82-134).
+std::vector<double> resample_cmr_coords(const std::vector<Vec3>& cmr, double start, double end, double tar_seg_len);
 
 }  // namespace kami_amd

@@ -26,6 +26,9 @@
 //       prints the closed flags as JSON; m: column-major transform
 //   kscr_solver_cli hair2bcc <hair.in> <out.bcc> <polyline 0|1>
 //   kscr_solver_cli hair2obj <hair.in> <out.obj>
+//   kscr_solver_cli cmrcoords <points.f32> <start> <end> <tar>
+//       one two-direction resampling pass over the control points (raw
+//       float32 xyz): prints the spline parameters as JSON
 //   (no GPU for the file conversions)
 //
 // Hair files: see Hair::read.  SDF files: i32 dim[3], f32 min[3], f32 max[3],
@@ -246,6 +249,19 @@ int main(int argc, char** argv) {
                                          g.phi.data(), g.grad.data(), g.grid_min, g.grid_max, err, sizeof err);
             if (rc != KSCR_OK) throw std::runtime_error(std::string("kscr_mesh_sdf: ") + err);
             write_sdf(argv[6], g);
+            return 0;
+        }
+        if (cmd == "cmrcoords" && argc == 6) {
+            std::ifstream in(argv[2], std::ios::binary);
+            if (!in) throw std::runtime_error(std::string("cannot open ") + argv[2]);
+            std::vector<Vec3> pts;
+            Vec3 p;
+            while (in.read((char*)&p, sizeof p)) pts.push_back(p);
+            const std::vector<double> ts =
+                resample_cmr_coords(pts, std::atof(argv[3]), std::atof(argv[4]), std::atof(argv[5]));
+            std::printf("[");
+            for (size_t i = 0; i < ts.size(); i++) std::printf("%s%.17g", i ? ", " : "", ts[i]);
+            std::printf("]\n");
             return 0;
         }
         if (cmd == "ybscene" && argc == 4) {

@@ -169,3 +169,98 @@ def test_resample_is_capped_at_hair_size(tmp_path):
     stride = np.float32(L - 2) / np.float32(30)
     idx = [int(np.float32(stride) * np.float32(v)) for v in range(1, 31)]
     np.testing.assert_allclose(x[1:31, 1], 1e-3 + 2.5e-4 * np.array(idx), atol=2e-9)
+
+
+
+def _splice_curve(n=160):
+    """Control points along x, spaced widest in the middle, on a gentle wave."""
+    k = np.arange(n - 1)
+    spacing = 1e-3 * (1 + 0.3 * np.exp(-((k - n / 2) / (n / 8)) ** 2))
+    xs = np.concatenate([[0.0], np.cumsum(spacing)])
+    return np.stack([xs, 2e-3 * np.sin(xs / 0.013), np.zeros(n)], 1).astype(np.float32)
+
+
+def _t_of_x(P, x, lo, hi):
+    """The parameter where the spline through P (x increasing) reaches abscissa x."""
+    a, b = lo, hi
+    for _ in range(80):
+        m = 0.5 * (a + b)
+        if _cmr(P, m)[0] < x:
+            a = m
+        else:
+            b = m
+    return 0.5 * (a + b)
+
+
+def _march(P, t0, t_stop, L, direction):
+    """Parameters at successive chord length L from t0 towards t_stop (the
+    first crossing, by bisection on the chord)."""
+    ts = [t0]
+    while True:
+        c0 = _cmr(P, ts[-1])
+        a = ts[-1]
+        b = a + direction * 0.25
+        if (b - t_stop) * direction > 0:
+            break
+        while np.linalg.norm(_cmr(P, b) - c0) < L:
+            a, b = b, b + direction * 0.25
+            if (b - t_stop) * direction > 0:
+                return np.array(ts)
+        for _ in range(80):
+            m = 0.5 * (a + b)
+            if np.linalg.norm(_cmr(P, m) - c0) < L:
+                a = m
+            else:
+                b = m
+        ts.append(0.5 * (a + b))
+    return np.array(ts)
+
+
+def test_resample_splice_of_the_two_marching_directions(tmp_path):
+    """One resampling pass marches from both ends and splices the two lists
+    (resample.h:82-134).  On a long curve whose control points are spaced
+    widest in the middle, the spline is fastest there, so the two lists'
+    parameters come closest in the middle, away from both ends.  Checked on
+    the pass's parameter list (`cmrcoords`), from the definition of the
+    method rather than its formulas:
+    - away from the splice, consecutive points are exactly one step length
+      apart (chord), marching from each end;
+    - the splice sits at the pair of forward and backward parameters that
+      come closest (every pair tried; parameters found by bisection);
+    - around it, each point's parameter is corrected by a linear ramp over
+      the 9 segments on each side, from 0 to the midpoint of the two
+      parameters at the splice: the gap is spread evenly.
+    (The kept pass of the full resampler marches at the first pass's average
+    chord, which leaves almost no gap to splice; the pass itself is where a
+    splice mistake shows.)"""
+    P = _splice_curve()
+    n = len(P)
+    P.tofile(tmp_path / "pts.f32")
+    P64 = P.astype(np.float64)
+    for tar in (7e-4, 6.1e-4, 9.3e-4):
+        ts = np.array(json.loads(hairio.cli("cmrcoords", tmp_path / "pts.f32", 1.0, n - 2.0, repr(tar)).stdout))
+        m = len(ts)
+        pos = np.array([_cmr(P64, t) for t in ts])
+        chord = np.linalg.norm(np.diff(pos, axis=0), axis=1)
+        off = np.flatnonzero(np.abs(chord - tar) > 1e-12)
+        # exactly the 18 segments around the splice leave the lattice
+        assert len(off) == 18 and off[-1] - off[0] == 17, (tar, off)
+        sx = int(off[0]) + 9  # the splice point
+        sy = m - 1 - sx       # its index in the backward list
+        assert 20 < sx < m - 20
+        tf = _march(P64, 1.0, n - 2.0, tar, +1)
+        tb = _march(P64, n - 2.0, 1.0, tar, -1)
+        # the splice is the closest pair of forward and backward parameters
+        gaps = np.abs(tf[8:len(tf) - 8, None] - tb[None, :])
+        i0, j0 = np.unravel_index(np.argmin(gaps), gaps.shape)
+        assert (int(i0) + 8, int(j0)) == (sx, sy), tar
+        # away from the splice the points are the marched ones
+        np.testing.assert_allclose(ts[:sx - 8], tf[:sx - 8], rtol=0, atol=1e-9)
+        np.testing.assert_allclose(ts[sx + 9:], tb[sy - 9::-1], rtol=0, atol=1e-9)
+        # parameter corrections around the splice: a linear ramp on each side
+        mid = 0.5 * (tf[sx] + tb[sy])
+        np.testing.assert_allclose(ts[sx], mid, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(ts[sx - 9:sx] - tf[sx - 9:sx], np.arange(9) / 9.0 * (mid - tf[sx]), atol=1e-9)
+        np.testing.assert_allclose(ts[sx + 1:sx + 10] - tb[sy - 1:sy - 10:-1],
+                                   (8 - np.arange(9)) / 9.0 * (mid - tb[sy]), atol=1e-9)
+        assert abs(mid - tf[sx]) > 1e-3  # the splice had a gap to spread
