@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--no-lbvh-leg", action="store_true", help="skip the LBVH build/refit/query timing")
     ap.add_argument("--mesh-sdf-only", action="store_true", help="run the mesh -> body SDF grid leg only")
     ap.add_argument("--lbvh-only", action="store_true", help="run the LBVH build/refit/query leg only")
+    ap.add_argument("--contacts-only", action="store_true", help="run the contacts leg only")
+    ap.add_argument("--contacts-steps", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--only-legs", action="store_true", help="run the adjoint and batched legs only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -574,6 +576,9 @@ def main():
     if args.lbvh_only:
         print(json.dumps({"lbvh": lbvh_leg(stream)}), flush=True)
         return
+    if args.contacts_only:
+        print(json.dumps({"contacts": contacts_leg(stream, args.contacts_steps)}), flush=True)
+        return
     if args.only_legs:
         legs = {}
         if not args.no_adjoint_leg:
@@ -582,7 +587,7 @@ def main():
         if not args.no_batched_leg:
             legs["batched"] = batched_leg(stream, args.batched_steps)
         if not args.no_contacts_leg:
-            legs["contacts"] = contacts_leg(stream)
+            legs["contacts"] = contacts_leg(stream, args.contacts_steps)
         print(json.dumps(legs), flush=True)
         return
 

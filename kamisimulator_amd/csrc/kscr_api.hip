@@ -433,9 +433,13 @@ int launch_contact_step(kscr_handle* h, float dt, bool with_sdf) {
     cp.use_step_size_limit = h->p.use_step_size_limit;
     cp.use_velocity_radius = h->p.use_velocity_radius;
     ContactDev& c = h->contact;
+    int cur = -1;  // where the previous substep's last iterate is (none yet)
     for (int s = 0; s < a.num_substeps; s++) {
-        if ((e = contact_init(a, c, h->stream)) != hipSuccess) return h->hip_fail(e, "contact init");
-        if (h->p.enable_hair_hair_coll && h->p.detection_period > 0 && (counter0 + s) % h->p.detection_period == 0) {
+        const bool detect =
+            h->p.enable_hair_hair_coll && h->p.detection_period > 0 && (counter0 + s) % h->p.detection_period == 0;
+        // the previous substep's endItr, this one's initItr and, when due, the segment boxes
+        if ((e = contact_begin(a, cp, c, cur, detect, h->stream)) != hipSuccess) return h->hip_fail(e, "substep start");
+        if (detect) {
             const bool rebuild = h->last_bvh_rebuild >= h->p.bvh_rebuild_period;
             if ((e = contact_detect(a, cp, c, rebuild, h->stream)) != hipSuccess)
                 return h->hip_fail(e, "collision detection");
@@ -453,7 +457,7 @@ int launch_contact_step(kscr_handle* h, float dt, bool with_sdf) {
         if ((e = contact_free_iterate(a, c, h->d_q.p, q_fin, h->d_packs.p, h->num_packs, h->block, h->stream)) !=
             hipSuccess)
             return h->hip_fail(e, "contact-free iterations");
-        int cur = 0;
+        cur = 0;
         for (int k = 0; k < a.num_itr; k++) {
             if ((e = contact_iterate(a, cp, c, cur, h->d_q.p, c.q2, h->d_packs.p, h->num_packs, h->block,
                                      h->stream)) != hipSuccess)
@@ -461,8 +465,9 @@ int launch_contact_step(kscr_handle* h, float dt, bool with_sdf) {
             std::swap(h->d_q.p, c.q2);  // the new orientations become the current ones
             cur ^= 1;
         }
-        if ((e = contact_end(a, c, cur, h->stream)) != hipSuccess) return h->hip_fail(e, "contact end");
     }
+    // the last substep's endItr (the others' ran at the start of the next substep)
+    if (cur >= 0 && (e = contact_end(a, c, cur, h->stream)) != hipSuccess) return h->hip_fail(e, "contact end");
     StepArgs a2 = a;
     a2.q = h->d_q.p;
     if (a.sdf_on && (e = sdf_pass(a2, h->n, h->stream)) != hipSuccess) return h->hip_fail(e, "SDF pass");

@@ -43,9 +43,13 @@ struct ContactParams {
     int use_step_size_limit, use_velocity_radius;
 };
 
-// initItr into c.dx[0] and c.y (and lastVel update).
-hipError_t contact_init(const StepArgs& a, ContactDev& c, hipStream_t s);
-// detectCollisions: AABBs, LBVH build or refit, query, narrow phase, sorted lists.
+// The start of a substep: endItr of the previous one from c.dx[prev] (prev
+// < 0: none), initItr into c.dx[0] and c.y (and the lastVel update), and with
+// `boxes` the padded segment boxes into c.bvh.aabb (buildAABBs).
+hipError_t contact_begin(const StepArgs& a, const ContactParams& cp, ContactDev& c, int prev, bool boxes,
+                         hipStream_t s);
+// detectCollisions after contact_begin wrote the boxes: LBVH build or refit,
+// query, narrow phase, sorted lists.
 hipError_t contact_detect(const StepArgs& a, const ContactParams& cp, ContactDev& c, bool rebuild, hipStream_t s);
 // recomputeStepLimitKernel
 hipError_t contact_step_limit(const StepArgs& a, const ContactParams& cp, ContactDev& c, hipStream_t s);

@@ -1,13 +1,15 @@
 // kscr_contact.hip -- Stable Cosserat Rods step with hair-hair contacts.
 //
 // Per substep (YB/sim/step.cpp:49-57 and the captured graph of :17-30):
-//   contact_init_kernel        initItr                     (iteration.cu:8-59)
-//   detection (when due)       buildAABBs, LBVH, query,   (YBSimulator.cu:424-457,
-//                              buildCollisionList          collision.cu:7-92)
+//   substep_begin_kernel       endItr of the previous substep (iteration.cu:62-74),
+//                              initItr (iteration.cu:8-59) and, when
+//                              detection is due, buildAABBs (collision.cu:7-34)
+//   detection (when due)       LBVH, query,                (YBSimulator.cu:424-457,
+//                              buildCollisionList          collision.cu:36-92)
 //   step_limit_kernel          recomputeStepLimitKernel    (collision.cu:94-131)
 //   numItr x iterate_kernel    cosseratItr with IPC barrier and friction,
 //                              then quaternionLambdaItr    (cosserat.cu:25-337)
-//   end_kernel                 endItr                      (iteration.cu:62-74)
+// and end_kernel (endItr) after the last substep.
 // The reference's races are replaced by ping-pong buffers (Jacobi snapshot)
 // and its atomically ordered contact lists by ascending lists, so results are
 // deterministic and equal to the CPU restatement's.
@@ -22,43 +24,89 @@ constexpr int kThreads = 256;
 
 inline dim3 grid_for(int n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
-__global__ void contact_init_kernel(StepArgs a, int n, float* __restrict__ dx0, float* __restrict__ yout) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const f3 x = ld3(a.x, i), v = ld3(a.v, i);
-    f3 u = ld3(a.u, i);
-    const float im = a.mat[i].x;
-    f3 y, dx;
-    init_vertex(a, im != 0.f, x, v, u, y, dx);
-    st3(a.u, i, u);
-    st3(dx0, i, dx);
-    st3(yout, i, y);
-}
-
-// buildAABBs<USE_VEL_RADIUS> (collision.cu:7-34): padded box of segment i
-__global__ void aabb_kernel(StepArgs a, ContactParams cp, int n, const float* __restrict__ dx, float* __restrict__ box) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t fl = __float_as_uint(a.mat[i].w);
+// buildAABBs<USE_VEL_RADIUS> (collision.cu:7-34): padded box of the segment
+// from p0 to p1 with initial guesses d0, d1 (no segment: the empty box).
+__device__ inline void store_segment_box(const ContactParams& cp, bool has_segment, f3 p0, f3 p1, f3 d0, f3 d1,
+                                         float* __restrict__ out) {
     float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    if (fl & FLAG_NEXT) {
-        const f3 p0 = ld3(a.x, i), p1 = ld3(a.x, i + 1);
+    if (has_segment) {
         const float P0[3] = {p0.x, p0.y, p0.z}, P1[3] = {p1.x, p1.y, p1.z};
         for (int k = 0; k < 3; k++) {
             b[k] = gmin(gmin(b[k], P0[k]), P1[k]);
             b[3 + k] = gmax(gmax(b[3 + k], P0[k]), P1[k]);
         }
         float pad = cp.scaled_detection_radius;
-        if (cp.use_velocity_radius) {
-            const f3 d0 = ld3(dx, i), d1 = ld3(dx, i + 1);
-            pad += sqrtf(gmax(dot(d0, d0), dot(d1, d1)));
-        }
+        if (cp.use_velocity_radius) pad += sqrtf(gmax(dot(d0, d0), dot(d1, d1)));
         for (int k = 0; k < 3; k++) {
             b[k] -= pad;
             b[3 + k] += pad;
         }
     }
-    for (int k = 0; k < 6; k++) box[6 * (size_t)i + k] = b[k];
+    for (int k = 0; k < 6; k++) out[k] = b[k];
+}
+
+// The start of a substep, one lane per vertex: endItr of the previous
+// substep (END; iteration.cu:62-74, from its last iterate dx_prev), initItr
+// (iteration.cu:8-59) into dx0 and y, and with BOX the padded segment boxes
+// of buildAABBs.  A segment box needs the next vertex's new position and
+// initial guess: they come through LDS from the next lane; the last lane of
+// a workgroup leaves a segment that continues into the next workgroup to
+// box_fixup_kernel, which runs after this kernel (reading another
+// workgroup's vertex here could see it before or after its update).
+// dx_prev may alias dx0: each lane reads its own vertex before writing it.
+template <bool END, bool BOX>
+__global__ __launch_bounds__(kThreads) void substep_begin_kernel(StepArgs a, ContactParams cp, int n,
+                                                                 const float* dx_prev, float* dx0,
+                                                                 float* __restrict__ yout, float* __restrict__ box) {
+    __shared__ float s_x[3][BOX ? kThreads : 1];
+    __shared__ float s_d[3][BOX ? kThreads : 1];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * kThreads + t;
+    const bool act = i < n;
+    f3 x{0, 0, 0}, dx{0, 0, 0};
+    uint32_t fl = 0;
+    if (act) {
+        x = ld3(a.x, i);
+        f3 v = ld3(a.v, i);
+        const float4 m = a.mat[i];
+        fl = __float_as_uint(m.w);
+        const bool free = m.x != 0.f;
+        if (END) {
+            end_vertex(a, free, ld3(dx_prev, i), x, v);
+            st3(a.x, i, x);
+            st3(a.v, i, v);
+        }
+        f3 u = ld3(a.u, i), y;
+        init_vertex(a, free, x, v, u, y, dx);
+        st3(a.u, i, u);
+        st3(dx0, i, dx);
+        st3(yout, i, y);
+    }
+    if (BOX) {
+        s_x[0][t] = x.x;
+        s_x[1][t] = x.y;
+        s_x[2][t] = x.z;
+        s_d[0][t] = dx.x;
+        s_d[1][t] = dx.y;
+        s_d[2][t] = dx.z;
+        __syncthreads();
+        const bool seg = (fl & FLAG_NEXT) != 0;
+        if (act && (!seg || t + 1 < kThreads)) {
+            const int tn = seg ? t + 1 : t;
+            const f3 xn{s_x[0][tn], s_x[1][tn], s_x[2][tn]}, dn{s_d[0][tn], s_d[1][tn], s_d[2][tn]};
+            store_segment_box(cp, seg, x, xn, dx, dn, box + 6 * (size_t)i);
+        }
+    }
+}
+
+// The segment boxes substep_begin_kernel leaves: segments from the last
+// vertex of a workgroup to the first of the next.
+__global__ void box_fixup_kernel(StepArgs a, ContactParams cp, int n, const float* __restrict__ dx0,
+                                 float* __restrict__ box) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) * kThreads + (kThreads - 1);
+    if (i >= n) return;
+    if (!(__float_as_uint(a.mat[i].w) & FLAG_NEXT)) return;
+    store_segment_box(cp, true, ld3(a.x, i), ld3(a.x, i + 1), ld3(dx0, i), ld3(dx0, i + 1), box + 6 * (size_t)i);
 }
 
 // buildCollisionList<USE_VEL_RADIUS> (collision.cu:36-92) over a device-sized
@@ -581,13 +629,30 @@ hipError_t contact_reserve(ContactDev& c, int n) {
     return hipDeviceSynchronize();
 }
 
-hipError_t contact_init(const StepArgs& a, ContactDev& c, hipStream_t s) {
-    hipLaunchKernelGGL(contact_init_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, c.n, c.dx[0], c.y);
+hipError_t contact_begin(const StepArgs& a, const ContactParams& cp, ContactDev& c, int prev, bool boxes, hipStream_t s) {
+    const dim3 grid = grid_for(c.n), block(kThreads);
+    float* const dx_prev = prev >= 0 ? c.dx[prev] : nullptr;
+    if (prev < 0 && !boxes)
+        hipLaunchKernelGGL((substep_begin_kernel<false, false>), grid, block, 0, s, a, cp, c.n, dx_prev, c.dx[0], c.y,
+                           c.bvh.aabb);
+    else if (prev < 0)
+        hipLaunchKernelGGL((substep_begin_kernel<false, true>), grid, block, 0, s, a, cp, c.n, dx_prev, c.dx[0], c.y,
+                           c.bvh.aabb);
+    else if (!boxes)
+        hipLaunchKernelGGL((substep_begin_kernel<true, false>), grid, block, 0, s, a, cp, c.n, dx_prev, c.dx[0], c.y,
+                           c.bvh.aabb);
+    else
+        hipLaunchKernelGGL((substep_begin_kernel<true, true>), grid, block, 0, s, a, cp, c.n, dx_prev, c.dx[0], c.y,
+                           c.bvh.aabb);
+    if (boxes) {
+        const int seams = (c.n + kThreads - 1) / kThreads;
+        hipLaunchKernelGGL(box_fixup_kernel, dim3((unsigned)((seams + 255) / 256)), dim3(256), 0, s, a, cp, c.n,
+                           c.dx[0], c.bvh.aabb);
+    }
     return hipGetLastError();
 }
 
 hipError_t contact_detect(const StepArgs& a, const ContactParams& cp, ContactDev& c, bool rebuild, hipStream_t s) {
-    hipLaunchKernelGGL(aabb_kernel, grid_for(c.n), dim3(kThreads), 0, s, a, cp, c.n, c.dx[0], c.bvh.aabb);
     hipError_t e = rebuild ? lbvh_build(c.bvh, c.n, s) : lbvh_refit(c.bvh, s);
     if (e != hipSuccess) return e;
     if ((e = lbvh_query(c.bvh, c.pairs, c.pair_cap, 2, c.error, s)) != hipSuccess) return e;
