@@ -388,15 +388,28 @@ __global__ void height_kernel(const uint32_t* __restrict__ leaf_parent, const Lb
     }
 }
 
-// Internal nodes per height, and the nodes grouped by height (offsets from
-// the host).  A workgroup counts its kLevelChunk nodes in LDS and touches each
+// Internal nodes per height that cross a refit tile boundary, and those nodes
+// grouped by height.  A workgroup counts its kLevelChunk nodes in LDS and touches each
 // global counter once: per-node global atomics on a few dozen counters
 // serialise (most nodes sit at the lowest heights).  A height past the bound
 // can only come from a corrupt tree; the host then finds the counts short and
 // reports an error.
 constexpr int kLevelBlock = 256, kLevelPerThread = 16, kLevelChunk = kLevelBlock * kLevelPerThread;
 
-__global__ __launch_bounds__(kLevelBlock) void level_count_kernel(const uint32_t* __restrict__ node_height, int m,
+// Refit tiles: kRefitTile consecutive leaves (sorted positions).  An internal
+// node whose leaf range lies inside one tile has its whole subtree there;
+// refit_tile_kernel refits those in one workgroup per tile, and only the
+// nodes whose ranges cross a tile boundary go into the per-height lists.
+constexpr int kRefitTile = 1024;
+
+__device__ inline bool crosses_tiles(const LbvhNode* __restrict__ nodes, int p) {
+    const uint32_t f = nodes[p].fence;
+    const uint32_t lo = min((uint32_t)p, f), hi = max((uint32_t)p, f);
+    return lo / kRefitTile != hi / kRefitTile;
+}
+
+__global__ __launch_bounds__(kLevelBlock) void level_count_kernel(const uint32_t* __restrict__ node_height,
+                                                                  const LbvhNode* __restrict__ nodes, int m,
                                                                   uint32_t* __restrict__ level_count) {
     __shared__ uint32_t bins[kMaxLevels];
     for (int i = threadIdx.x; i < kMaxLevels; i += kLevelBlock) bins[i] = 0;
@@ -404,7 +417,8 @@ __global__ __launch_bounds__(kLevelBlock) void level_count_kernel(const uint32_t
     const int base = blockIdx.x * kLevelChunk;
     for (int j = 0; j < kLevelPerThread; j++) {
         const int p = base + j * kLevelBlock + threadIdx.x;
-        if (p < m && node_height[p] < (uint32_t)kMaxLevels) atomicAdd(&bins[node_height[p]], 1u);
+        if (p < m && node_height[p] < (uint32_t)kMaxLevels && crosses_tiles(nodes, p))
+            atomicAdd(&bins[node_height[p]], 1u);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kMaxLevels; i += kLevelBlock)
@@ -414,7 +428,8 @@ __global__ __launch_bounds__(kLevelBlock) void level_count_kernel(const uint32_t
 // The offset of each height's nodes is the exclusive sum of the counts below
 // it; every workgroup forms it in LDS, and workgroup 0 publishes (offset,
 // count) per height for the refit kernels.
-__global__ __launch_bounds__(kLevelBlock) void level_scatter_kernel(const uint32_t* __restrict__ node_height, int m,
+__global__ __launch_bounds__(kLevelBlock) void level_scatter_kernel(const uint32_t* __restrict__ node_height,
+                                                                    const LbvhNode* __restrict__ nodes, int m,
                                                                     uint32_t* __restrict__ level_count,
                                                                     int2* __restrict__ level_range,
                                                                     uint32_t* __restrict__ level_nodes) {
@@ -436,7 +451,7 @@ __global__ __launch_bounds__(kLevelBlock) void level_scatter_kernel(const uint32
     uint32_t h[kLevelPerThread], rank[kLevelPerThread];
     for (int j = 0; j < kLevelPerThread; j++) {
         const int p = base + j * kLevelBlock + threadIdx.x;
-        h[j] = p < m ? node_height[p] : (uint32_t)kMaxLevels;
+        h[j] = (p < m && crosses_tiles(nodes, p)) ? node_height[p] : (uint32_t)kMaxLevels;
         if (h[j] < (uint32_t)kMaxLevels) rank[j] = atomicAdd(&bins[h[j]], 1u);
     }
     __syncthreads();
@@ -472,7 +487,63 @@ __device__ inline void refit_node(const float* __restrict__ aabb, const uint32_t
     }
 }
 
-// All nodes of height h (grid-stride; the level's range from the device).
+// The internal nodes whose leaf range lies inside one tile, one workgroup per
+// tile: the tile's leaf boxes and each node's union go to LDS, and the nodes
+// are refit height by height with a barrier between heights (a node's
+// children are lower).  The unions are formed as refit_node forms a child's
+// union from the child's two boxes, so the boxes equal the per-height
+// kernels' bit for bit.
+__global__ __launch_bounds__(kRefitTile) void refit_tile_kernel(const float* __restrict__ aabb,
+                                                                const uint32_t* __restrict__ ids,
+                                                                LbvhNode* __restrict__ nodes,
+                                                                const uint32_t* __restrict__ node_height, int n) {
+    __shared__ float s_leaf[6][kRefitTile];
+    __shared__ float s_node[6][kRefitTile];
+    __shared__ uint32_t s_top;
+    const int t = threadIdx.x;
+    const int L = blockIdx.x * kRefitTile;
+    const int leaf = L + t, p = L + t;
+    if (t == 0) s_top = 0;
+    if (leaf < n) {
+        const float* a = aabb + 6 * (size_t)ids[leaf];
+        for (int j = 0; j < 6; j++) s_leaf[j][t] = a[j];
+    }
+    bool in = false;
+    uint32_t height = 0, child[2] = {0, 0};
+    if (p < n - 1) {
+        const uint4 w = *reinterpret_cast<const uint4*>(nodes + p);  // parent, left, right, fence
+        const uint32_t lo = min((uint32_t)p, w.w), hi = max((uint32_t)p, w.w);
+        in = lo / kRefitTile == hi / kRefitTile;
+        if (in) {
+            height = node_height[p];
+            child[0] = w.y;
+            child[1] = w.z;
+        }
+    }
+    __syncthreads();
+    if (in) atomicMax(&s_top, height);
+    __syncthreads();
+    const uint32_t top = s_top;
+    for (uint32_t h = 1; h <= top; h++) {
+        if (in && height == h) {
+            float b[2][6];
+            for (int side = 0; side < 2; side++) {
+                const uint32_t c = child[side];
+                const int k = (int)(c & 0x7FFFFFFFu) - L;  // in this tile, as the node is
+                for (int j = 0; j < 6; j++) b[side][j] = (c & 0x80000000u) ? s_leaf[j][k] : s_node[j][k];
+            }
+            for (int side = 0; side < 2; side++)
+                for (int j = 0; j < 6; j++) nodes[p].b[side][j] = b[side][j];
+            for (int j = 0; j < 3; j++) {
+                s_node[j][t] = b[1][j] < b[0][j] ? b[1][j] : b[0][j];
+                s_node[3 + j][t] = b[0][3 + j] < b[1][3 + j] ? b[1][3 + j] : b[0][3 + j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// All crossing nodes of height h (grid-stride; the level's range from the device).
 __global__ void refit_level_kernel(const float* __restrict__ aabb, const uint32_t* __restrict__ ids,
                                    LbvhNode* __restrict__ nodes, const uint32_t* __restrict__ level_nodes,
                                    const int2* __restrict__ level_range, int h) {
@@ -481,20 +552,22 @@ __global__ void refit_level_kernel(const float* __restrict__ aabb, const uint32_
         refit_node(aabb, ids, nodes, level_nodes[lv.x + k]);
 }
 
-// The levels from height h0 up to the root, in one workgroup: a barrier
-// between levels orders them (the workgroup's writes are visible to it after
-// the barrier), which saves one launch per level.  Every height from 1 to the
-// root's has nodes (a node's taller child is one lower), so the first empty
-// height ends the tree.
+// The crossing nodes of heights h0 and up, in one workgroup: a barrier
+// between heights orders them (the workgroup's writes are visible to it after
+// the barrier), which saves one launch per height.  Heights without crossing
+// nodes are skipped (the test is uniform, so no barrier is needed there).
 constexpr int kTopBlock = 1024;
 __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(const float* __restrict__ aabb,
                                                               const uint32_t* __restrict__ ids,
                                                               LbvhNode* __restrict__ nodes,
                                                               const uint32_t* __restrict__ level_nodes,
                                                               const int2* __restrict__ level_range, int h0) {
+    __shared__ int2 s_range[kMaxLevels];
+    for (int h = threadIdx.x; h < kMaxLevels; h += kTopBlock) s_range[h] = level_range[h];
+    __syncthreads();
     for (int h = h0; h < kMaxLevels; h++) {
-        const int2 lv = level_range[h];
-        if (lv.y == 0) break;
+        const int2 lv = s_range[h];
+        if (lv.y == 0) continue;
         for (int k = threadIdx.x; k < lv.y; k += kTopBlock) refit_node(aabb, ids, nodes, level_nodes[lv.x + k]);
         __syncthreads();
     }
@@ -754,21 +827,27 @@ size_t lbvh_carve(LbvhDev& d, int n, char* base, size_t bytes) {
 // height h falls by about 0.66 per height from ~0.4 n (100k boxes: 36k, 23k,
 // 15k, 10k, 6.4k, 4.4k, 2.6k, ...); the launches stop near 2 kTopBlock nodes.
 // Any value gives the same boxes.
-static int bottom_levels(int n) {
-    int h = 0;
+// Estimated crossing nodes of height h: all nodes of that height (about
+// 0.4 n 0.66^(h-1) in Karras trees over spread-out boxes), but at most a
+// couple per tile boundary.
+static double crossing_estimate(int n, int h) {
     double c = 0.4 * n;
-    while (c >= 2.0 * kTopBlock && h < kMaxLevels - 2) {
-        h++;
-        c *= 0.66;
-    }
-    return h;
+    for (int k = 1; k < h; k++) c *= 0.66;
+    return std::min(c, 2.0 * n / kRefitTile + 64.0);
+}
+
+// Heights that get launches of their own: up to the last one estimated to
+// hold two workgroups' worth of crossing nodes.
+static int bottom_levels(int n) {
+    int last = 0;
+    for (int h = 1; h < kMaxLevels - 1; h++)
+        if (crossing_estimate(n, h) >= 2.0 * kTopBlock) last = h;
+    return last;
 }
 
 // Grid of the refit launch for height h (grid-stride over the actual count).
 static unsigned level_blocks(int n, int h) {
-    double c = 0.4 * n;
-    for (int k = 1; k < h; k++) c *= 0.66;
-    const double b = std::ceil(c / 256.0);
+    const double b = std::ceil(crossing_estimate(n, h) / 256.0);
     return (unsigned)std::max(8.0, std::min(b, 65536.0));
 }
 
@@ -823,20 +902,24 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
     hipLaunchKernelGGL(height_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d.leaf_parent, d.nodes,
                        (uint32_t*)d.flags, n, d.node_height, d.max_depth);
     const dim3 level_grid((m + kLevelChunk - 1) / kLevelChunk);
-    hipLaunchKernelGGL(level_count_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, m, d.level_count);
-    hipLaunchKernelGGL(level_scatter_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, m,
+    hipLaunchKernelGGL(level_count_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, d.nodes, m,
+                       d.level_count);
+    hipLaunchKernelGGL(level_scatter_kernel, level_grid, dim3(kLevelBlock), 0, stream, d.node_height, d.nodes, m,
                        d.level_count, d.level_range, d.level_nodes);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     d.built = true;
     return lbvh_refit(d, stream);
 }
 
-// One launch per low tree level, then one workgroup for the rest; the launch
-// boundaries and the workgroup barriers order the levels (no atomics, no
-// fences).  The level ranges live on the device, so nothing is read back.
+// The nodes inside tiles in one launch, then the crossing nodes: one launch
+// per low height, then one workgroup for the rest.  Launch boundaries and
+// workgroup barriers order the levels (no atomics, no fences); the level
+// ranges live on the device, so nothing is read back.
 hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
     if (d.n < 2) return hipSuccess;
+    hipLaunchKernelGGL(refit_tile_kernel, dim3((unsigned)((d.n + kRefitTile - 1) / kRefitTile)), dim3(kRefitTile), 0,
+                       stream, d.aabb, d.ids, d.nodes, d.node_height, d.n);
     const int K = bottom_levels(d.n);
     for (int h = 1; h <= K; h++)
         hipLaunchKernelGGL(refit_level_kernel, dim3(level_blocks(d.n, h)), dim3(256), 0, stream, d.aabb, d.ids,
