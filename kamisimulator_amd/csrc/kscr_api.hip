@@ -11,24 +11,17 @@
 //   setKStretch/KBend/Young  YB/YBSimulator.cu:276-320
 //   uploadMeta validation    YB/YBSimulator.cu:322-335
 //   Sim::advance             YB/sim/step.cpp:38-62
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cfloat>
-#include <cmath>
+//
+// The differentiation entry points are in kscr_api_adjoint.hip, the batch in
+// kscr_api_batch.hip and the LBVH in kscr_api_lbvh.hip; the handle and the
+// helpers they share in kscr_api_internal.h.
 #include <cstdio>
 #include <cstring>
-#include <string>
-#include <thread>
-#include <vector>
 
-#include "../../include/kscr.h"
-#include "kscr_contact.h"
-#include "kscr_device.h"
-#include "kscr_lbvh.h"
-#include "kscr_math.h"
+#include "kscr_api_internal.h"
 
 using namespace kscr;
+using namespace kscr::api;
 
 static_assert(sizeof(kscr_params) == 27 * 4, "kscr_params layout is part of the ABI");
 
@@ -36,51 +29,10 @@ namespace {
 const double kPi = 3.141592653589793238462643;
 
 thread_local std::string g_create_error;
+}  // namespace
 
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t alloc(size_t count) {
-        release();
-        if (count == 0) return hipSuccess;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e == hipSuccess) n = count;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-};
-// Page-locked host array (hipHostMalloc): the host mirror of the exchanged
-// state, so that device-host copies run at full link speed.
-template <class T>
-struct PinnedBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t resize(size_t count) {
-        if (count == n) return hipSuccess;
-        release();
-        if (count == 0) return hipSuccess;
-        hipError_t e = hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault);
-        if (e == hipSuccess) n = count;
-        else p = nullptr;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    T* data() { return p; }
-    const T* data() const { return p; }
-    T& operator[](size_t i) { return p[i]; }
-    const T& operator[](size_t i) const { return p[i]; }
-    ~PinnedBuf() { release(); }
-};
+namespace kscr {
+namespace api {
 
 // memcpy of large host arrays split over host threads
 void host_copy(void* dst, const void* src, size_t bytes) {
@@ -99,95 +51,6 @@ void host_copy(void* dst, const void* src, size_t bytes) {
     }
     for (auto& t : pool) t.join();
 }
-
-// Device state of a differentiable rollout (kscr_adjoint_forward/backward).
-struct AdjointRun {
-    float dt = 0;
-    int nsteps = 0, every = 0, substeps = 0, num_itr = 0;
-    bool ready = false;
-    // checkpoints: state at steps 0, every, 2 every, ... and at the end
-    DevBuf<float> ck_x, ck_v, ck_u;
-    DevBuf<float4> ck_q;
-    struct Clock {
-        float h, last_h, time;
-        long counter;
-    };
-    std::vector<Clock> clocks;
-    Clock end_clock{};
-    // tape of one segment of `every` steps
-    DevBuf<float> t_x, t_y, t_dx;
-    DevBuf<unsigned char> t_sigma;
-    DevBuf<float4> t_q;
-    DevBuf<float> t_ilw;
-    // adjoint state
-    DevBuf<float> xb, vb, ub, ksb;
-    DevBuf<float4> qb, qrb;
-    // the forward pass records the last segment's tape itself
-    bool last_recorded = false;
-    std::vector<float> last_h_of, last_lh_of;  // h, last_h of the last segment's steps
-    float record_ms = 0, sweep_ms = 0, backward_ms = 0;
-    // the backward pass re-records segment j - 1 on this stream while the
-    // handle's stream sweeps segment j (tapes in two slots by segment parity)
-    hipStream_t rec_stream = nullptr;
-    AdjointRun() = default;
-    AdjointRun(const AdjointRun&) = delete;
-    AdjointRun& operator=(const AdjointRun&) = delete;
-    ~AdjointRun() {
-        if (rec_stream) (void)hipStreamDestroy(rec_stream);
-    }
-};
-}  // namespace
-
-struct kscr_handle {
-    kscr_params p{};
-    int n = 0;
-    int device = 0;
-    std::vector<int> strand_off;
-    int longest = 0;  // vertices of the longest strand
-    // host mirror (YarnBall::Sim::verts/vels/qs/qRests); positions and
-    // velocities page-locked for the per-step exchange
-    PinnedBuf<float> pos, vel;
-    std::vector<float> inv_mass, l_rest, k_stretch;
-    std::vector<uint32_t> flags;
-    std::vector<int> conn;  // Vertex::connectionIndex (YarnBall.h:56), -1 if none
-    bool has_glue = false;
-    std::vector<f4> q, q_rest;
-    float min_seg = FLT_MAX, max_seg = 0, density = 0;
-    bool configured = false;
-    // step state
-    float h = 4e-4f, last_h = 4e-4f, time = 0.f;
-    long step_counter = 0;
-    // device
-    DevBuf<float> d_x, d_v, d_u;
-    DevBuf<float4> d_q, d_qr, d_mat;
-    DevBuf<int> d_conn;
-    DevBuf<int2> d_packs;
-    DevBuf<float4> d_grid;
-    int num_packs = 0;
-    int block = 0;
-    bool uploaded = false;
-    hipStream_t stream = nullptr;
-    // SDF
-    bool sdf_set = false, sdf_on = false;
-    SdfArgs sdf{};
-    // hair-hair contacts
-    ContactDev contact;
-    float last_bvh_rebuild = INFINITY;
-    AdjointRun adj;
-    std::string err;
-    ~kscr_handle() { contact_release(contact); }
-
-    int fail(int code, const std::string& msg) {
-        err = msg;
-        return code;
-    }
-    int hip_fail(hipError_t e, const char* what) {
-        err = std::string(what) + ": " + hipGetErrorString(e);
-        return KSCR_ERR_HIP;
-    }
-};
-
-namespace {
 
 // uploadMeta (YBSimulator.cu:322-335) on a parameter set p for handle h
 // (h->min_seg is the shortest rest segment once configured).
@@ -211,10 +74,6 @@ int validate_meta(kscr_handle* h) { return validate_params(h, h->p); }
 
 bool colored(const kscr_handle* h) { return h->p.schedule == KSCR_SCHEDULE_COLORED; }
 
-// Largest workgroup of the step kernels: a strand longer than this is split
-// over several packs.
-constexpr int kMaxBlock = 1024;
-
 // Jacobi strands longer than the largest workgroup are split over packs that
 // overlap by a one-vertex halo at each cut (iterate_kernel), as the
 // reference runs chains of any length in overlapping blocks
@@ -235,8 +94,6 @@ int pack_capacity(const kscr_handle* h, int block) { return colored(h) ? 2 * blo
 
 // Largest instantiated block of the step kernel in use.
 int max_block(const kscr_handle* h) { return colored(h) ? 512 : 1024; }
-
-int longest_strand(const kscr_handle* h);
 
 // Workgroups the greedy packing of whole strands gives at `capacity`
 // vertices per pack (the count build_packs produces).
@@ -527,79 +384,9 @@ int upload_material(kscr_handle* h) {
 }
 
 // Copies the dynamic state (x, v, u, q) into checkpoint slot k (save) or back.
-hipError_t checkpoint_copy(kscr_handle* h, size_t k, bool save, hipStream_t stream) {
-    AdjointRun& A = h->adj;
-    const size_t n = h->n, n3 = 3 * n;
-    const hipMemcpyKind kind = hipMemcpyDeviceToDevice;
-    float* x = A.ck_x.p + k * n3;
-    float* v = A.ck_v.p + k * n3;
-    float* u = A.ck_u.p + k * n3;
-    float4* q = A.ck_q.p + k * n;
-    hipError_t e;
-    if (save) {
-        if ((e = hipMemcpyAsync(x, h->d_x.p, n3 * 4, kind, stream)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(v, h->d_v.p, n3 * 4, kind, stream)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(u, h->d_u.p, n3 * 4, kind, stream)) != hipSuccess) return e;
-        return hipMemcpyAsync(q, h->d_q.p, n * 16, kind, stream);
-    }
-    if ((e = hipMemcpyAsync(h->d_x.p, x, n3 * 4, kind, stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(h->d_v.p, v, n3 * 4, kind, stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(h->d_u.p, u, n3 * 4, kind, stream)) != hipSuccess) return e;
-    return hipMemcpyAsync(h->d_q.p, q, n * 16, kind, stream);
-}
 
-AdjointRun::Clock clock_of(const kscr_handle* h) { return AdjointRun::Clock{h->h, h->last_h, h->time, h->step_counter}; }
-void set_clock(kscr_handle* h, const AdjointRun::Clock& c) {
-    h->h = c.h, h->last_h = c.last_h, h->time = c.time, h->step_counter = c.counter;
-}
-
-// Tapes of two checkpoint segments (slots 0 and 1), each `every` steps of
-// `substeps` substeps.
-int reserve_tape(kscr_handle* h) {
-    AdjointRun& A = h->adj;
-    const size_t n = h->n, S = (size_t)A.substeps, I1 = (size_t)A.num_itr + 1, every = 2 * (size_t)A.every;
-    const size_t K = (size_t)A.num_itr;
-    const size_t per_sub = S * n * every, per_dx = S * I1 * n * every, per_q = S * I1 * n * every;
-    const size_t per_ilw = S * K * n * every;
-    if (A.t_dx.n >= per_dx * 3 && A.t_q.n >= per_q && A.t_x.n >= per_sub * 3 && A.t_ilw.n >= per_ilw)
-        return KSCR_OK;
-    hipError_t e;
-    if ((e = A.t_x.alloc(per_sub * 3)) != hipSuccess || (e = A.t_y.alloc(per_sub * 3)) != hipSuccess ||
-        (e = A.t_sigma.alloc(per_sub)) != hipSuccess || (e = A.t_dx.alloc(per_dx * 3)) != hipSuccess ||
-        (e = A.t_q.alloc(per_q)) != hipSuccess || (e = A.t_ilw.alloc(std::max<size_t>(per_ilw, 1))) != hipSuccess)
-        return h->hip_fail(e, "hipMalloc tape");
-    return KSCR_OK;
-}
-
-// Tape of step j of the segment in tape slot `slot`.
-TapeArgs tape_of(kscr_handle* h, size_t slot, size_t j) {
-    AdjointRun& A = h->adj;
-    const size_t n = h->n, S = (size_t)A.substeps, K = (size_t)A.num_itr;
-    j += slot * (size_t)A.every;
-    TapeArgs t{};
-    t.x = A.t_x.p + j * S * n * 3;
-    t.y = A.t_y.p + j * S * n * 3;
-    t.sigma_free = A.t_sigma.p + j * S * n;
-    t.dx = A.t_dx.p + j * S * (K + 1) * n * 3;
-    t.q = A.t_q.p + j * S * (K + 1) * n;
-    t.ilw = A.t_ilw.p + j * S * K * n;
-    t.n = (int)n;
-    return t;
-}
-
-int adjoint_supported(kscr_handle* h) {
-    if (h->p.enable_hair_hair_coll || h->p.use_step_size_limit)
-        return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the contact-free step (no contacts, no step size limit)");
-    if (h->has_glue) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers strands without glued vertices");
-    if (split_strands(h))
-        return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers strands of at most " + std::to_string(kMaxBlock) +
-                                                 " vertices");
-    if (h->p.wind_peak != 0) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the step without wind");
-    if (colored(h)) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the Jacobi schedule");
-    return KSCR_OK;
-}
-
-}  // namespace
+}  // namespace api
+}  // namespace kscr
 
 extern "C" {
 
@@ -1183,229 +970,6 @@ int kscr_last_error(const kscr_handle* h, char* buf, size_t n) {
     return KSCR_OK;
 }
 
-int kscr_set_material(kscr_handle* h, const float* k_stretch, const float* q_rest) {
-    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
-    if (!h->configured) return h->fail(KSCR_ERR_NOT_CONFIGURED, "No rest length. Must call configure()");
-    for (int i = 0; i < h->n; i++) {
-        if (k_stretch) h->k_stretch[i] = k_stretch[i];
-        if (q_rest) h->q_rest[i] = f4{q_rest[4 * i], q_rest[4 * i + 1], q_rest[4 * i + 2], q_rest[4 * i + 3]};
-    }
-    if (!h->uploaded) return KSCR_OK;
-    hipError_t e = hipSetDevice(h->device);
-    if (e != hipSuccess) return h->hip_fail(e, "hipSetDevice");
-    return upload_material(h);
-}
-
-// ------------------------------------------------------------------ adjoint
-int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every) {
-    if (!h || !(dt > 0) || nsteps <= 0 || checkpoint_every <= 0) return KSCR_ERR_INVALID_ARGUMENT;
-    if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
-    int rc = validate_meta(h);
-    if (rc) return rc;
-    if ((rc = adjoint_supported(h))) return rc;
-    hipError_t e;
-    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
-    AdjointRun& A = h->adj;
-    A.ready = false;
-    const size_t n = h->n;
-    const size_t slots = (size_t)(nsteps + checkpoint_every - 1) / checkpoint_every + 1;  // + end state
-    if (A.ck_q.n < slots * n) {
-        if ((e = A.ck_x.alloc(slots * n * 3)) != hipSuccess || (e = A.ck_v.alloc(slots * n * 3)) != hipSuccess ||
-            (e = A.ck_u.alloc(slots * n * 3)) != hipSuccess || (e = A.ck_q.alloc(slots * n)) != hipSuccess)
-            return h->hip_fail(e, "hipMalloc checkpoints");
-    }
-    A.dt = dt;
-    A.nsteps = nsteps;
-    A.every = checkpoint_every;
-    A.substeps = std::max(1, (int)std::ceil(dt / h->p.max_h));
-    A.num_itr = h->p.num_itr;
-    A.last_recorded = false;
-    if ((rc = reserve_tape(h)) != KSCR_OK) return rc;
-    // steps of the last segment write their tape as they run
-    const int last_s0 = ((nsteps - 1) / checkpoint_every) * checkpoint_every;
-    A.last_h_of.assign(checkpoint_every, 0.f);
-    A.last_lh_of.assign(checkpoint_every, 0.f);
-    A.clocks.clear();
-    for (int s = 0; s < nsteps; s++) {
-        if (s % checkpoint_every == 0) {
-            if ((e = checkpoint_copy(h, (size_t)(s / checkpoint_every), true, h->stream)) != hipSuccess)
-                return h->hip_fail(e, "save checkpoint");
-            A.clocks.push_back(clock_of(h));
-        }
-        if (s < last_s0) {
-            if ((rc = launch_step(h, dt, false)) != KSCR_OK) return rc;
-            continue;
-        }
-        const StepArgs sa = step_args(h, dt, false);
-        A.last_h_of[s - last_s0] = sa.h;
-        A.last_lh_of[s - last_s0] = sa.last_h;
-        if ((e = launch_step_record(sa, tape_of(h, 0, (size_t)(s - last_s0)), h->d_packs.p, h->num_packs, h->block,
-                                    h->stream)) != hipSuccess)
-            return h->hip_fail(e, "record kernel launch");
-    }
-    A.last_recorded = true;
-    A.ready = true;
-    return KSCR_OK;
-}
-
-int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar, float* k_stretch_bar,
-                          float* q_rest_bar) {
-    if (!h || !k_stretch_bar || !q_rest_bar) return KSCR_ERR_INVALID_ARGUMENT;
-    AdjointRun& A = h->adj;
-    if (!A.ready) return h->fail(KSCR_ERR_NOT_CONFIGURED, "kscr_adjoint_forward has not been run");
-    if (h->p.num_itr != A.num_itr || std::max(1, (int)std::ceil(A.dt / h->p.max_h)) != A.substeps)
-        return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr or max_h changed since the forward pass");
-    int rc = adjoint_supported(h);
-    if (rc) return rc;
-    hipError_t e;
-    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
-    const size_t n = h->n, n3 = 3 * n;
-    if ((rc = reserve_tape(h)) != KSCR_OK) return rc;
-    if (A.qb.n != n) {
-        if ((e = A.xb.alloc(n3)) != hipSuccess || (e = A.vb.alloc(n3)) != hipSuccess || (e = A.ub.alloc(n3)) != hipSuccess ||
-            (e = A.ksb.alloc(n)) != hipSuccess || (e = A.qb.alloc(n)) != hipSuccess || (e = A.qrb.alloc(n)) != hipSuccess)
-            return h->hip_fail(e, "hipMalloc adjoint state");
-    }
-    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
-    if ((e = x_bar ? hipMemcpyAsync(A.xb.p, x_bar, n3 * 4, h2d, h->stream) : hipMemsetAsync(A.xb.p, 0, n3 * 4, h->stream)) !=
-            hipSuccess ||
-        (e = v_bar ? hipMemcpyAsync(A.vb.p, v_bar, n3 * 4, h2d, h->stream) : hipMemsetAsync(A.vb.p, 0, n3 * 4, h->stream)) !=
-            hipSuccess ||
-        (e = hipMemsetAsync(A.ub.p, 0, n3 * 4, h->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(A.ksb.p, 0, n * 4, h->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(A.qb.p, 0, n * 16, h->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(A.qrb.p, 0, n * 16, h->stream)) != hipSuccess)
-        return h->hip_fail(e, "seed adjoint state");
-    const int nseg = (int)A.clocks.size();
-    const size_t end_slot = (size_t)nseg;
-    if ((e = checkpoint_copy(h, end_slot, true, h->stream)) != hipSuccess) return h->hip_fail(e, "save end state");
-    const AdjointRun::Clock end_clock = clock_of(h);
-    if (!A.rec_stream && (e = hipStreamCreateWithFlags(&A.rec_stream, hipStreamNonBlocking)) != hipSuccess)
-        return h->hip_fail(e, "hipStreamCreate");
-    hipStream_t ms = h->stream, rs = A.rec_stream;
-
-    AdjointArgs aa{};
-    aa.mat = h->d_mat.p;
-    aa.q_rest = h->d_qr.p;
-    aa.num_substeps = A.substeps;
-    aa.num_itr = A.num_itr;
-    aa.damping = h->p.damping;
-    aa.velocity_decay = h->p.velocity_decay;
-    aa.acceleration_ratio = h->p.acceleration_ratio;
-    for (int k = 0; k < 3; k++) aa.ext[k] = h->p.gravity[k] * h->p.external_force_multiplier;
-    // |ext|^2 as the device dot product forms it
-    aa.ext2 = std::fma(aa.ext[2], aa.ext[2], std::fma(aa.ext[1], aa.ext[1], aa.ext[0] * aa.ext[0]));
-    aa.xb = A.xb.p;
-    aa.vb = A.vb.p;
-    aa.ub = A.ub.p;
-    aa.qb = A.qb.p;
-    aa.ksb = A.ksb.p;
-    aa.qrb = A.qrb.p;
-    // events: per segment record begin/end (record stream) and sweep
-    // begin/end (handle stream); then start, record-stream done, end
-    std::vector<hipEvent_t> ev(4 * (size_t)nseg + 3, nullptr);
-    for (auto& x : ev)
-        if ((e = hipEventCreate(&x)) != hipSuccess) {
-            rc = h->hip_fail(e, "hipEventCreate");
-            break;
-        }
-    auto rec_begin = [&](int seg) { return ev[4 * seg]; };
-    auto rec_end = [&](int seg) { return ev[4 * seg + 1]; };
-    auto sweep_begin = [&](int seg) { return ev[4 * seg + 2]; };
-    auto sweep_end = [&](int seg) { return ev[4 * seg + 3]; };
-    hipEvent_t ev_start = ev[4 * nseg], ev_rec_done = ev[4 * nseg + 1], ev_end = ev[4 * nseg + 2];
-    // Segment j's tape lives in slot (nseg - 1 - j) % 2: the last segment in
-    // slot 0, where the forward pass recorded it.  Re-recording segment j - 1
-    // (record stream) overlaps the sweep of segment j (handle stream); it
-    // waits for the sweep of segment j + 1, the last user of its slot.
-    auto slot_of = [&](int seg) { return (size_t)((nseg - 1 - seg) % 2); };
-    std::vector<std::vector<float>> hs(nseg), lhs(nseg);
-    std::vector<bool> rerecorded(nseg, false);
-    if (rc == KSCR_OK) {
-        (void)hipEventRecord(ev_start, ms);
-        if ((e = hipStreamWaitEvent(rs, ev_start, 0)) != hipSuccess) rc = h->hip_fail(e, "hipStreamWaitEvent");
-    }
-    for (int seg = nseg - 1; seg >= 0 && rc == KSCR_OK; seg--) {
-        const int s0 = seg * A.every, s1 = std::min(A.nsteps, s0 + A.every);
-        if (seg == nseg - 1 && A.last_recorded) {
-            hs[seg] = A.last_h_of;  // recorded by the forward pass
-            lhs[seg] = A.last_lh_of;
-        } else {
-            rerecorded[seg] = true;
-            hs[seg].assign(A.every, 0.f);
-            lhs[seg].assign(A.every, 0.f);
-            if (seg + 2 <= nseg - 1 && (e = hipStreamWaitEvent(rs, sweep_end(seg + 2), 0)) != hipSuccess) {
-                rc = h->hip_fail(e, "hipStreamWaitEvent");
-                break;
-            }
-            (void)hipEventRecord(rec_begin(seg), rs);
-            if ((e = checkpoint_copy(h, (size_t)seg, false, rs)) != hipSuccess) {
-                rc = h->hip_fail(e, "restore checkpoint");
-                break;
-            }
-            set_clock(h, A.clocks[seg]);
-            for (int s = s0; s < s1; s++) {
-                const StepArgs sa = step_args(h, A.dt, false);
-                hs[seg][s - s0] = sa.h;
-                lhs[seg][s - s0] = sa.last_h;
-                if ((e = launch_step_record(sa, tape_of(h, slot_of(seg), (size_t)(s - s0)), h->d_packs.p,
-                                            h->num_packs, h->block, rs)) != hipSuccess) {
-                    rc = h->hip_fail(e, "record kernel launch");
-                    break;
-                }
-            }
-            (void)hipEventRecord(rec_end(seg), rs);
-            if (rc == KSCR_OK && (e = hipStreamWaitEvent(ms, rec_end(seg), 0)) != hipSuccess)
-                rc = h->hip_fail(e, "hipStreamWaitEvent");
-        }
-        (void)hipEventRecord(sweep_begin(seg), ms);
-        for (int s = s1 - 1; s >= s0 && rc == KSCR_OK; s--) {
-            aa.tape = tape_of(h, slot_of(seg), (size_t)(s - s0));
-            aa.h = hs[seg][s - s0];
-            aa.last_h = lhs[seg][s - s0];
-            if ((e = launch_step_adjoint(aa, h->d_packs.p, h->num_packs, h->block, ms)) != hipSuccess)
-                rc = h->hip_fail(e, "adjoint kernel launch");
-        }
-        (void)hipEventRecord(sweep_end(seg), ms);
-    }
-    // the record stream's work (it writes the state arrays) ends before the
-    // end state is restored
-    (void)hipEventRecord(ev_rec_done, rs);
-    (void)hipStreamWaitEvent(ms, ev_rec_done, 0);
-    if (nseg > 1) A.last_recorded = false;  // earlier segments reused the tape
-    if ((e = checkpoint_copy(h, end_slot, false, ms)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "restore end state");
-    set_clock(h, end_clock);
-    (void)hipEventRecord(ev_end, ms);
-    if (rc == KSCR_OK &&
-        ((e = hipMemcpyAsync(k_stretch_bar, A.ksb.p, n * 4, hipMemcpyDeviceToHost, ms)) != hipSuccess ||
-         (e = hipMemcpyAsync(q_rest_bar, A.qrb.p, n * 16, hipMemcpyDeviceToHost, ms)) != hipSuccess))
-        rc = h->hip_fail(e, "download gradient");
-    if ((e = hipStreamSynchronize(ms)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "hipStreamSynchronize");
-    if ((e = hipStreamSynchronize(rs)) != hipSuccess && rc == KSCR_OK) rc = h->hip_fail(e, "hipStreamSynchronize");
-    A.record_ms = A.sweep_ms = A.backward_ms = 0;
-    if (rc == KSCR_OK) {
-        for (int seg = 0; seg < nseg; seg++) {
-            float r = 0, w = 0;
-            if (rerecorded[seg]) (void)hipEventElapsedTime(&r, rec_begin(seg), rec_end(seg));
-            (void)hipEventElapsedTime(&w, sweep_begin(seg), sweep_end(seg));
-            A.record_ms += r;
-            A.sweep_ms += w;
-        }
-        (void)hipEventElapsedTime(&A.backward_ms, ev_start, ev_end);
-    }
-    for (auto& x : ev)
-        if (x) (void)hipEventDestroy(x);
-    return rc;
-}
-
-int kscr_adjoint_last_time(const kscr_handle* h, float* record_ms, float* sweep_ms, float* backward_ms) {
-    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
-    if (record_ms) *record_ms = h->adj.record_ms;
-    if (sweep_ms) *sweep_ms = h->adj.sweep_ms;
-    if (backward_ms) *backward_ms = h->adj.backward_ms;
-    return KSCR_OK;
-}
-
 // ------------------------------------------------------------------ autotuner
 int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
     if (!h || steps <= 0 || !(dt > 0)) return KSCR_ERR_INVALID_ARGUMENT;
@@ -1496,462 +1060,3 @@ int kscr_autotune(kscr_handle* h, float dt, int steps, int* best_block) {
 
 }  // extern "C"
 
-// ------------------------------------------------------------------ batch
-struct kscr_batch {
-    std::vector<kscr_handle*> grooms;
-    int device = 0;
-    int block = 0;
-    int num_packs = 0;
-    DevBuf<StepArgs> d_args;
-    DevBuf<int4> d_packs;
-    std::vector<StepArgs> args;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    bool use_graph = true;
-    hipGraphExec_t graph = nullptr;
-    // recorded after each batch_step; every groom's own stream waits on it
-    hipEvent_t done = nullptr;
-    int graph_steps = 0;
-    float graph_dt = 0.f;
-    std::string err;
-    int longest = 0;
-    int fail(int code, const std::string& m) {
-        err = m;
-        return code;
-    }
-    int hip_fail(hipError_t e, const char* what) {
-        err = std::string(what) + ": " + hipGetErrorString(e);
-        return KSCR_ERR_HIP;
-    }
-    void drop_graph() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
-        graph_steps = 0;
-    }
-};
-
-namespace {
-// Packs every groom's strands into workgroups of `block` vertices; a pack is
-// (begin, end, groom, 0).
-int batch_set_block(kscr_batch* b, int block) {
-    std::vector<int4> packs;
-    for (size_t g = 0; g < b->grooms.size(); g++) {
-        std::vector<int2> p2;
-        const int rc = build_packs(b->grooms[g], block, p2);
-        if (rc) return rc;
-        for (const int2& pk : p2) packs.push_back(make_int4(pk.x, pk.y, (int)g, 0));
-    }
-    hipError_t e;
-    if (b->d_packs.n < packs.size() && (e = b->d_packs.alloc(packs.size())) != hipSuccess)
-        return b->hip_fail(e, "hipMalloc batch packs");
-    if ((e = hipMemcpyAsync(b->d_packs.p, packs.data(), packs.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream)) !=
-            hipSuccess ||
-        (e = hipStreamSynchronize(b->stream)) != hipSuccess)
-        return b->hip_fail(e, "upload batch packs");
-    b->block = block;
-    b->num_packs = (int)packs.size();
-    b->drop_graph();
-    return KSCR_OK;
-}
-}  // namespace
-
-extern "C" {
-
-int kscr_batch_create(kscr_handle* const* handles, int num_handles, int block_size, kscr_batch** out) {
-    if (!handles || num_handles <= 0 || !out) return KSCR_ERR_INVALID_ARGUMENT;
-    *out = nullptr;
-    auto* b = new kscr_batch;
-    int longest = 0;
-    for (int g = 0; g < num_handles; g++) {
-        kscr_handle* h = handles[g];
-        if (!h || !h->uploaded) {
-            delete b;
-            return KSCR_ERR_NOT_CONFIGURED;
-        }
-        if (global_path(h) || h->p.schedule != KSCR_SCHEDULE_JACOBI) {
-            delete b;
-            return KSCR_ERR_UNSUPPORTED;  // the batched kernel is the contact-free Jacobi step
-        }
-        if (g > 0 && h->device != handles[0]->device) {
-            delete b;
-            return KSCR_ERR_INVALID_ARGUMENT;
-        }
-        longest = std::max(longest, longest_strand(h));
-        b->grooms.push_back(h);
-    }
-    b->device = handles[0]->device;
-    int block = block_size;
-    if (block <= 0) {
-        for (int c : kBlockSizes)
-            if (c >= longest && c >= 512) { block = c; break; }
-        if (block <= 0)
-            for (int c : kBlockSizes)
-                if (c >= longest) { block = c; break; }
-    }
-    bool ok = false;
-    for (int c : kBlockSizes) ok |= (c == block);
-    if (!ok || block < longest) {
-        delete b;
-        return KSCR_ERR_STRAND_TOO_LONG;
-    }
-    b->longest = longest;
-    hipError_t e;
-    if ((e = hipSetDevice(b->device)) != hipSuccess || (e = b->d_args.alloc(num_handles)) != hipSuccess) {
-        delete b;
-        return KSCR_ERR_HIP;
-    }
-    if ((e = hipStreamCreate(&b->stream)) != hipSuccess) {
-        delete b;
-        return KSCR_ERR_HIP;
-    }
-    b->own_stream = true;
-    if ((e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming)) != hipSuccess) {
-        (void)hipStreamDestroy(b->stream);
-        delete b;
-        return KSCR_ERR_HIP;
-    }
-    const int rc = batch_set_block(b, block);
-    if (rc) {
-        (void)hipEventDestroy(b->done);
-        (void)hipStreamDestroy(b->stream);
-        delete b;
-        return rc;
-    }
-    b->args.resize(num_handles);
-    *out = b;
-    return KSCR_OK;
-}
-
-void kscr_batch_destroy(kscr_batch* b) {
-    if (!b) return;
-    (void)hipSetDevice(b->device);
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    b->drop_graph();
-    if (b->done) (void)hipEventDestroy(b->done);
-    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
-    delete b;
-}
-
-int kscr_batch_set_graph(kscr_batch* b, int use_graph) {
-    if (!b) return KSCR_ERR_INVALID_ARGUMENT;
-    b->use_graph = use_graph != 0;
-    return KSCR_OK;
-}
-
-int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
-    if (!b || nsteps < 0) return KSCR_ERR_INVALID_ARGUMENT;
-    if (nsteps == 0 || !(dt > 0)) return KSCR_OK;
-    hipError_t e;
-    if ((e = hipSetDevice(b->device)) != hipSuccess) return b->hip_fail(e, "hipSetDevice");
-    for (size_t g = 0; g < b->grooms.size(); g++) {
-        kscr_handle* h = b->grooms[g];
-        if (!h->uploaded) return b->fail(KSCR_ERR_NOT_CONFIGURED, "groom " + std::to_string(g) + " is not uploaded");
-        // kscr_set_params may have switched a groom to a path the batched
-        // kernel does not run (it is the contact-free Jacobi step)
-        if (global_path(h) || h->p.schedule != KSCR_SCHEDULE_JACOBI)
-            return b->fail(KSCR_ERR_UNSUPPORTED, "groom " + std::to_string(g) +
-                                                     ": contacts, step size limits, glued vertices and the coloured "
-                                                     "schedule are not batched");
-        int rc = validate_meta(h);
-        if (rc) return b->fail(rc, "groom " + std::to_string(g) + ": " + h->err);
-        // the handles' own streams may still hold work on their state
-        if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return b->hip_fail(e, "hipStreamSynchronize");
-        b->args[g] = step_args(h, dt, with_sdf != 0);
-        // later steps of this call: the clock kernel advances the device copy;
-        // the host mirror follows the same float arithmetic
-        for (int k = 1; k < nsteps; k++) {
-            h->last_h = h->h;
-            h->time += dt;
-            h->step_counter += b->args[g].num_substeps;
-        }
-    }
-    if ((e = hipMemcpyAsync(b->d_args.p, b->args.data(), b->args.size() * sizeof(StepArgs), hipMemcpyHostToDevice,
-                            b->stream)) != hipSuccess)
-        return b->hip_fail(e, "hipMemcpyAsync batch arguments");
-    const int ng = (int)b->grooms.size();
-    // later work queued on a groom's own stream (kscr_get_pos_vel,
-    // kscr_advance, ...) is ordered after this batch's kernels
-    auto order_handles = [&]() -> int {
-        hipError_t r;
-        if ((r = hipEventRecord(b->done, b->stream)) != hipSuccess) return b->hip_fail(r, "hipEventRecord");
-        for (kscr_handle* h : b->grooms)
-            if ((r = hipStreamWaitEvent(h->stream, b->done, 0)) != hipSuccess)
-                return b->hip_fail(r, "hipStreamWaitEvent");
-        return KSCR_OK;
-    };
-    if (!b->use_graph) {
-        for (int k = 0; k < nsteps; k++) {
-            if ((e = launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream)) != hipSuccess)
-                return b->hip_fail(e, "batch step launch");
-            if ((e = launch_batch_clock(b->d_args.p, ng, dt, b->stream)) != hipSuccess)
-                return b->hip_fail(e, "batch clock launch");
-        }
-        return order_handles();
-    }
-    if (!b->graph || b->graph_steps != nsteps || b->graph_dt != dt) {
-        b->drop_graph();
-        hipGraph_t graph;
-        if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-            return b->hip_fail(e, "hipStreamBeginCapture");
-        hipError_t le = hipSuccess;
-        for (int k = 0; k < nsteps && le == hipSuccess; k++) {
-            le = launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream);
-            if (le == hipSuccess) le = launch_batch_clock(b->d_args.p, ng, dt, b->stream);
-        }
-        if ((e = hipStreamEndCapture(b->stream, &graph)) != hipSuccess) return b->hip_fail(e, "hipStreamEndCapture");
-        if (le != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            return b->hip_fail(le, "batch launch under capture");
-        }
-        e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            b->graph = nullptr;
-            return b->hip_fail(e, "hipGraphInstantiate");
-        }
-        b->graph_steps = nsteps;
-        b->graph_dt = dt;
-    }
-    if ((e = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) return b->hip_fail(e, "hipGraphLaunch");
-    return order_handles();
-}
-
-int kscr_batch_autotune(kscr_batch* b, float dt, int steps, int* best_block) {
-    if (!b || steps <= 0 || !(dt > 0)) return KSCR_ERR_INVALID_ARGUMENT;
-    hipError_t e;
-    if ((e = hipSetDevice(b->device)) != hipSuccess) return b->hip_fail(e, "hipSetDevice");
-    // save every groom's state and clock
-    struct Saved {
-        DevBuf<float> x, v, u;
-        DevBuf<float4> q;
-        float h, last_h, time;
-        long counter;
-    };
-    std::vector<Saved> saved(b->grooms.size());
-    const hipMemcpyKind d2d = hipMemcpyDeviceToDevice;
-    auto copy_state = [&](kscr_handle* h, Saved& sv, bool save) -> hipError_t {
-        const size_t n3 = (size_t)h->n * 3 * 4, nq = (size_t)h->n * 16;
-        hipError_t r;
-        if (save) {
-            if ((r = sv.x.alloc((size_t)h->n * 3)) != hipSuccess || (r = sv.v.alloc((size_t)h->n * 3)) != hipSuccess ||
-                (r = sv.u.alloc((size_t)h->n * 3)) != hipSuccess || (r = sv.q.alloc(h->n)) != hipSuccess)
-                return r;
-            sv.h = h->h, sv.last_h = h->last_h, sv.time = h->time, sv.counter = h->step_counter;
-            if ((r = hipMemcpyAsync(sv.x.p, h->d_x.p, n3, d2d, b->stream)) != hipSuccess) return r;
-            if ((r = hipMemcpyAsync(sv.v.p, h->d_v.p, n3, d2d, b->stream)) != hipSuccess) return r;
-            if ((r = hipMemcpyAsync(sv.u.p, h->d_u.p, n3, d2d, b->stream)) != hipSuccess) return r;
-            return hipMemcpyAsync(sv.q.p, h->d_q.p, nq, d2d, b->stream);
-        }
-        h->h = sv.h, h->last_h = sv.last_h, h->time = sv.time, h->step_counter = sv.counter;
-        if ((r = hipMemcpyAsync(h->d_x.p, sv.x.p, n3, d2d, b->stream)) != hipSuccess) return r;
-        if ((r = hipMemcpyAsync(h->d_v.p, sv.v.p, n3, d2d, b->stream)) != hipSuccess) return r;
-        if ((r = hipMemcpyAsync(h->d_u.p, sv.u.p, n3, d2d, b->stream)) != hipSuccess) return r;
-        return hipMemcpyAsync(h->d_q.p, sv.q.p, nq, d2d, b->stream);
-    };
-    for (size_t g = 0; g < b->grooms.size(); g++) {
-        if ((e = hipStreamSynchronize(b->grooms[g]->stream)) != hipSuccess) return b->hip_fail(e, "hipStreamSynchronize");
-        if ((e = copy_state(b->grooms[g], saved[g], true)) != hipSuccess) return b->hip_fail(e, "autotune save");
-    }
-    const bool graph_was = b->use_graph;
-    b->use_graph = false;
-    hipEvent_t ev0, ev1;
-    if ((e = hipEventCreate(&ev0)) != hipSuccess || (e = hipEventCreate(&ev1)) != hipSuccess)
-        return b->hip_fail(e, "hipEventCreate");
-    const int start_block = b->block;
-    int best = -1, rc = KSCR_OK;
-    float best_ms = 0;
-    for (int c : kBlockSizes) {
-        if (c < b->longest) continue;
-        if ((rc = batch_set_block(b, c)) != KSCR_OK) break;
-        if ((rc = kscr_batch_step(b, dt, 1, 0)) != KSCR_OK) break;  // warm-up
-        (void)hipEventRecord(ev0, b->stream);
-        if ((rc = kscr_batch_step(b, dt, steps, 0)) != KSCR_OK) break;
-        (void)hipEventRecord(ev1, b->stream);
-        if ((e = hipEventSynchronize(ev1)) != hipSuccess) {
-            rc = b->hip_fail(e, "hipEventSynchronize");
-            break;
-        }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev0, ev1);
-        if (best < 0 || ms < best_ms) best = c, best_ms = ms;
-    }
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    b->use_graph = graph_was;
-    for (size_t g = 0; g < b->grooms.size(); g++)
-        if ((e = copy_state(b->grooms[g], saved[g], false)) != hipSuccess && rc == KSCR_OK)
-            rc = b->hip_fail(e, "autotune restore");
-    const int rc2 = batch_set_block(b, (rc == KSCR_OK && best > 0) ? best : start_block);
-    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess && rc == KSCR_OK) rc = b->hip_fail(e, "hipStreamSynchronize");
-    if (rc) return rc;
-    if (rc2) return rc2;
-    if (best_block) *best_block = b->block;
-    return KSCR_OK;
-}
-
-int kscr_batch_synchronize(kscr_batch* b) {
-    if (!b) return KSCR_ERR_INVALID_ARGUMENT;
-    hipError_t e = hipStreamSynchronize(b->stream);
-    if (e != hipSuccess) return b->hip_fail(e, "hipStreamSynchronize");
-    return KSCR_OK;
-}
-
-int kscr_batch_block_size(const kscr_batch* b) { return b ? b->block : -1; }
-
-int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n) {
-    if (!b || !buf || n == 0) return KSCR_ERR_INVALID_ARGUMENT;
-    std::snprintf(buf, n, "%s", b->err.c_str());
-    return KSCR_OK;
-}
-
-}  // extern "C"
-
-// ------------------------------------------------------------------ LBVH
-struct kscr_lbvh {
-    int device = 0;
-    LbvhDev d;  // its counter block holds the query's error word at d.count[2]
-    int2* pairs = nullptr;
-    size_t pair_cap = 0;
-    ~kscr_lbvh() {
-        lbvh_release(d);
-        if (pairs) (void)hipFree(pairs);
-    }
-};
-
-extern "C" {
-
-int kscr_lbvh_create(int device, kscr_lbvh** out) {
-    if (!out) return KSCR_ERR_INVALID_ARGUMENT;
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return KSCR_ERR_HIP;
-    if (device < 0) device = count - 1;
-    if (device >= count) return KSCR_ERR_INVALID_ARGUMENT;
-    auto* b = new kscr_lbvh;
-    b->device = device;
-    if (hipSetDevice(device) != hipSuccess) {
-        delete b;
-        return KSCR_ERR_HIP;
-    }
-    *out = b;
-    return KSCR_OK;
-}
-
-void kscr_lbvh_destroy(kscr_lbvh* b) {
-    if (!b) return;
-    (void)hipSetDevice(b->device);
-    (void)hipDeviceSynchronize();
-    delete b;
-}
-
-static int lbvh_upload(kscr_lbvh* b, const float* aabbs, int n) {
-    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (lbvh_reserve(b->d, n) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
-    if (hipMemcpy(b->d.aabb, aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-        return KSCR_ERR_HIP;
-    return KSCR_OK;
-}
-
-int kscr_lbvh_compute(kscr_lbvh* b, const float* aabbs, int n) {
-    if (!b || !aabbs || n <= 0) return KSCR_ERR_INVALID_ARGUMENT;
-    int rc = lbvh_upload(b, aabbs, n);
-    if (rc) return rc;
-    if (lbvh_build(b->d, n, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KSCR_ERR_HIP;
-    return KSCR_OK;
-}
-
-int kscr_lbvh_refit(kscr_lbvh* b, const float* aabbs) {
-    if (!b || !aabbs || !b->d.built) return KSCR_ERR_INVALID_ARGUMENT;
-    int rc = lbvh_upload(b, aabbs, b->d.n);
-    if (rc) return rc;
-    if (lbvh_refit(b->d, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KSCR_ERR_HIP;
-    return KSCR_OK;
-}
-
-int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count) {
-    if (!b || !count || !b->d.built || (cap > 0 && !pairs)) return KSCR_ERR_INVALID_ARGUMENT;
-    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (cap > b->pair_cap) {
-        if (b->pairs) (void)hipFree(b->pairs);
-        b->pairs = nullptr;
-        if (hipMalloc((void**)&b->pairs, cap * sizeof(int2)) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
-        b->pair_cap = cap;
-    }
-    if (lbvh_query(b->d, b->pairs, (int)cap, -1, b->d.count + 2, nullptr) != hipSuccess ||
-        hipDeviceSynchronize() != hipSuccess)
-        return KSCR_ERR_HIP;
-    int cnt[3] = {0, 0, 0};  // pairs found, redo count, error word
-    if (hipMemcpy(cnt, b->d.count, sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
-    const int found = cnt[0], err = cnt[2];
-    const size_t m = std::min((size_t)found, cap);
-    if (m && hipMemcpy(pairs, b->pairs, m * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
-    *count = (size_t)found;
-    return err ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
-}
-
-int kscr_lbvh_compute_device(kscr_lbvh* b, const float* d_aabbs, int n, void* stream) {
-    if (!b || !d_aabbs || n <= 0) return KSCR_ERR_INVALID_ARGUMENT;
-    const hipStream_t s = static_cast<hipStream_t>(stream);
-    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (lbvh_reserve(b->d, n) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
-    if (hipMemcpyAsync(b->d.aabb, d_aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        lbvh_build(b->d, n, s) != hipSuccess)
-        return KSCR_ERR_HIP;
-    return KSCR_OK;
-}
-
-int kscr_lbvh_refit_device(kscr_lbvh* b, const float* d_aabbs, void* stream) {
-    if (!b || !d_aabbs || !b->d.built) return KSCR_ERR_INVALID_ARGUMENT;
-    const hipStream_t s = static_cast<hipStream_t>(stream);
-    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (hipMemcpyAsync(b->d.aabb, d_aabbs, (size_t)b->d.n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) !=
-            hipSuccess ||
-        lbvh_refit(b->d, s) != hipSuccess)
-        return KSCR_ERR_HIP;
-    return KSCR_OK;
-}
-
-int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count, void* stream) {
-    if (!b || !count || !b->d.built || (cap > 0 && !d_pairs) || cap > (size_t)INT32_MAX)
-        return KSCR_ERR_INVALID_ARGUMENT;
-    const hipStream_t s = static_cast<hipStream_t>(stream);
-    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (lbvh_query(b->d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, b->d.count + 2, s) != hipSuccess)
-        return KSCR_ERR_HIP;
-    int cnt[3] = {0, 0, 0};  // pairs found, redo count, error word: one copy back
-    if (hipMemcpyAsync(cnt, b->d.count, sizeof cnt, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return KSCR_ERR_HIP;
-    *count = (size_t)cnt[0];
-    return cnt[2] ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
-}
-
-int kscr_lbvh_get(kscr_lbvh* b, uint32_t* codes, uint32_t* ids, uint32_t* links, float* bounds, uint32_t* leaf_parent,
-                  float* root, int* max_stack) {
-    if (!b || !b->d.built) return KSCR_ERR_INVALID_ARGUMENT;
-    const int n = b->d.n;
-    if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (codes && hipMemcpy(codes, b->d.codes, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
-    if (ids && hipMemcpy(ids, b->d.ids, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
-    if (leaf_parent && hipMemcpy(leaf_parent, b->d.leaf_parent, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
-        return KSCR_ERR_HIP;
-    if (root && hipMemcpy(root, b->d.root, 6 * 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
-    if (max_stack && hipMemcpy(max_stack, b->d.max_depth, 4, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
-    if ((links || bounds) && n > 1) {
-        std::vector<LbvhNode> nodes(n - 1);
-        if (hipMemcpy(nodes.data(), b->d.nodes, (size_t)(n - 1) * sizeof(LbvhNode), hipMemcpyDeviceToHost) != hipSuccess)
-            return KSCR_ERR_HIP;
-        for (int i = 0; i < n - 1; i++) {
-            if (links) {
-                links[4 * i] = nodes[i].parent;
-                links[4 * i + 1] = nodes[i].left;
-                links[4 * i + 2] = nodes[i].right;
-                links[4 * i + 3] = nodes[i].fence;
-            }
-            if (bounds)
-                for (int k = 0; k < 12; k++) bounds[12 * i + k] = nodes[i].b[k / 6][k % 6];
-        }
-    }
-    return KSCR_OK;
-}
-
-}  // extern "C"

@@ -1,4 +1,4 @@
-// kscr_sagfree.hip -- SagFree initialisation of a groom (host code).
+// kscr_sagfree.cpp -- SagFree initialisation of a groom (host code).
 //
 // Sim::SagFree / SingleSagFree (YB/YBSimulator.cu:480-676) runs once, on the
 // host, between setYoung / root pinning and upload
