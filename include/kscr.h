@@ -261,10 +261,16 @@ int kscr_set_material(kscr_handle* h, const float* k_stretch, const float* q_res
  * segment of checkpoint_every steps, last first, it restores the segment's
  * checkpoint, re-runs the segment recording its tape (the last segment's tape
  * is kept from kscr_adjoint_forward), and sweeps its steps backwards.  It returns d(loss)/d(kStretch) (float[n]) and d(loss)/d(qRest)
- * (float[4n]) and leaves the state at the end of the rollout.  Returns
- * KSCR_ERR_UNSUPPORTED for contacts, step size limits or wind.  Steps follow
- * kscr_advance (no SDF pass). */
+ * (float[4n]) and leaves the state at the end of the rollout.  Wind is
+ * differentiated through its dependence on the positions; returns
+ * KSCR_ERR_UNSUPPORTED for contacts, step size limits, glued vertices or
+ * strands longer than 1024 vertices.  Steps follow kscr_advance (no SDF
+ * pass). */
 int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every);
+/* The same with steps that follow kscr_solve_next_step: the SDF pass after
+ * each step, differentiated as the projection it is within a grid cell (the
+ * cell choice and the inside/outside test count as constant). */
+int kscr_adjoint_forward_sdf(kscr_handle* h, float dt, int nsteps, int checkpoint_every);
 int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar, float* k_stretch_bar,
                           float* q_rest_bar);
 /* Device times (ms) of the last kscr_adjoint_backward, from HIP events:

@@ -122,6 +122,7 @@ SIGNATURES = {
     "kscr_get_contacts": (C.c_int, [_P, _IP, _IP]),
     "kscr_set_material": (C.c_int, [_P, _FP, _FP]),
     "kscr_adjoint_forward": (C.c_int, [_P, C.c_float, C.c_int, C.c_int]),
+    "kscr_adjoint_forward_sdf": (C.c_int, [_P, C.c_float, C.c_int, C.c_int]),
     "kscr_adjoint_backward": (C.c_int, [_P, _FP, _FP, _FP, _FP]),
     "kscr_adjoint_last_time": (C.c_int, [_P, _FP, _FP, _FP]),
     "kscr_lbvh_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
@@ -407,9 +408,11 @@ class Sim:
         self._check(self.lib.kscr_set_material(self.h, None if ks is None else fptr(ks),
                                                None if qr is None else fptr(qr)))
 
-    def adjoint_forward(self, dt, nsteps, checkpoint_every):
-        """nsteps steps of dt keeping a checkpoint every checkpoint_every steps."""
-        self._check(self.lib.kscr_adjoint_forward(self.h, dt, nsteps, checkpoint_every))
+    def adjoint_forward(self, dt, nsteps, checkpoint_every, with_sdf=False):
+        """nsteps steps of dt keeping a checkpoint every checkpoint_every steps
+        (with_sdf: each followed by the SDF pass, as solve_next_step)."""
+        fn = self.lib.kscr_adjoint_forward_sdf if with_sdf else self.lib.kscr_adjoint_forward
+        self._check(fn(self.h, dt, nsteps, checkpoint_every))
 
     def adjoint_backward(self, x_bar=None, v_bar=None):
         """Adjoints of the final (x, v) -> (d/dkStretch (n,), d/dqRest (n, 4))."""

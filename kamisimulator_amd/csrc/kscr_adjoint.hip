@@ -112,6 +112,18 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
     const int ic = min(i, n - 1);
     const int inx = min(ic + 1, n - 1), ipv = max(ic - 1, 0);
 
+    // ---- the SDF pass after the last substep (non-root vertices), at the
+    // state endItr left: x + dx^K and dx^K / h * decay (free) or v (pinned,
+    // which initItr's y holds)
+    if (a.sdf_on && act && has_prev) {
+        const size_t s1 = (size_t)(a.num_substeps - 1);
+        const f3 dxK = ld3(a.tape.dx + (s1 * (K + 1) * n + (size_t)K * n) * 3, ic);
+        f3 xp = ld3(a.tape.x + s1 * n * 3, ic);
+        const f3 vp = free ? dxK * (1.0f / hh) * (1 - a.velocity_decay * hh) : ld3(a.tape.y + s1 * n * 3, ic);
+        xp = xp + dxK;
+        sdf_project_vjp(a.sdf, xp, vp, xb, vb);
+    }
+
     for (int sub = a.num_substeps - 1; sub >= 0; sub--) {
         const size_t so = (size_t)sub * n;
         const f3 x = ld3(a.tape.x + so * 3, ic);
@@ -313,7 +325,32 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
         f3 vbn = vb + ub;
         vbn = vbn + hh * dxb;
         f3 ubn = zero3();
-        if (act && free) {
+        if (act && free && a.wind_peak != 0) {
+            // ext depends on x through the wind: ext = (g + (wind(x), 0, 0)) efm
+            vbn = vbn + hh * yb;
+            const WindForce w = wind_force<true>(a.wind_peak, a.wind_y_freq, a.wind_z_freq, a.wind_time_freq,
+                                                 a.wind_sharpness, a.time, x);
+            const f3 g{a.gravity[0], a.gravity[1], a.gravity[2]};
+            const f3 ext = (g + f3{w.fx, 0.f, 0.f}) * a.external_force_multiplier;
+            const float ext2 = dot(ext, ext);
+            f3 extb = (hh * hh) * yb;
+            if (ext2 > 0) {
+                // s = clamp(raw, 0, 1), raw = acc . ext / |ext|^2 (acc from the tape)
+                const f3 acc = ld3(a.tape.acc + so * 3, ic);
+                const float raw = dot(acc, ext) / ext2;
+                extb = extb + (hh * hh * clamp01(raw)) * dxb;
+                if (a.tape.sigma_free[so + i]) {
+                    const float sb = hh * hh * dot(dxb, ext);
+                    const f3 ab = (sb / ext2) * ext;
+                    vbn = vbn + ab / a.last_h;
+                    ubn = ubn - ab / a.last_h;
+                    extb = extb + sb * (acc / ext2 - ((2.0f * raw) / ext2) * ext);
+                }
+            }
+            const float wb = extb.x * a.external_force_multiplier;
+            xb.y = xb.y + wb * w.d_dy;
+            xb.z = xb.z + wb * w.d_dz;
+        } else if (act && free) {
             vbn = vbn + hh * yb;
             if (a.ext2 > 0) {
                 const f3 ext{a.ext[0], a.ext[1], a.ext[2]};

@@ -46,9 +46,11 @@ int reserve_tape(kscr_handle* h) {
     const size_t K = (size_t)A.num_itr;
     const size_t per_sub = S * n * every, per_dx = S * I1 * n * every, per_q = S * I1 * n * every;
     const size_t per_ilw = S * K * n * every;
+    hipError_t e;
+    if (A.wind && A.t_acc.n < per_sub * 3 && (e = A.t_acc.alloc(per_sub * 3)) != hipSuccess)
+        return h->hip_fail(e, "hipMalloc tape");
     if (A.t_dx.n >= per_dx * 3 && A.t_q.n >= per_q && A.t_x.n >= per_sub * 3 && A.t_ilw.n >= per_ilw)
         return KSCR_OK;
-    hipError_t e;
     if ((e = A.t_x.alloc(per_sub * 3)) != hipSuccess || (e = A.t_y.alloc(per_sub * 3)) != hipSuccess ||
         (e = A.t_sigma.alloc(per_sub)) != hipSuccess || (e = A.t_dx.alloc(per_dx * 3)) != hipSuccess ||
         (e = A.t_q.alloc(per_q)) != hipSuccess || (e = A.t_ilw.alloc(std::max<size_t>(per_ilw, 1))) != hipSuccess)
@@ -68,6 +70,7 @@ TapeArgs tape_of(kscr_handle* h, size_t slot, size_t j) {
     t.dx = A.t_dx.p + j * S * (K + 1) * n * 3;
     t.q = A.t_q.p + j * S * (K + 1) * n;
     t.ilw = A.t_ilw.p + j * S * K * n;
+    t.acc = A.wind ? A.t_acc.p + j * S * n * 3 : nullptr;
     t.n = (int)n;
     return t;
 }
@@ -79,7 +82,6 @@ int adjoint_supported(kscr_handle* h) {
     if (split_strands(h))
         return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers strands of at most " + std::to_string(kMaxBlock) +
                                                  " vertices");
-    if (h->p.wind_peak != 0) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the step without wind");
     if (colored(h)) return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint covers the Jacobi schedule");
     return KSCR_OK;
 }
@@ -102,7 +104,8 @@ int kscr_set_material(kscr_handle* h, const float* k_stretch, const float* q_res
 }
 
 // ------------------------------------------------------------------ adjoint
-int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every) {
+namespace {
+int adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every, bool with_sdf) {
     if (!h || !(dt > 0) || nsteps <= 0 || checkpoint_every <= 0) return KSCR_ERR_INVALID_ARGUMENT;
     if (!h->uploaded) return h->fail(KSCR_ERR_NOT_CONFIGURED, "upload() has not been called");
     int rc = validate_meta(h);
@@ -125,11 +128,14 @@ int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_ev
     A.substeps = std::max(1, (int)std::ceil(dt / h->p.max_h));
     A.num_itr = h->p.num_itr;
     A.last_recorded = false;
+    A.with_sdf = with_sdf;
+    A.wind = h->p.wind_peak != 0;
     if ((rc = reserve_tape(h)) != KSCR_OK) return rc;
     // steps of the last segment write their tape as they run
     const int last_s0 = ((nsteps - 1) / checkpoint_every) * checkpoint_every;
     A.last_h_of.assign(checkpoint_every, 0.f);
     A.last_lh_of.assign(checkpoint_every, 0.f);
+    A.last_t_of.assign(checkpoint_every, 0.f);
     A.clocks.clear();
     for (int s = 0; s < nsteps; s++) {
         if (s % checkpoint_every == 0) {
@@ -138,12 +144,13 @@ int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_ev
             A.clocks.push_back(clock_of(h));
         }
         if (s < last_s0) {
-            if ((rc = launch_step(h, dt, false)) != KSCR_OK) return rc;
+            if ((rc = launch_step(h, dt, with_sdf)) != KSCR_OK) return rc;
             continue;
         }
-        const StepArgs sa = step_args(h, dt, false);
+        const StepArgs sa = step_args(h, dt, with_sdf);
         A.last_h_of[s - last_s0] = sa.h;
         A.last_lh_of[s - last_s0] = sa.last_h;
+        A.last_t_of[s - last_s0] = sa.time;
         if ((e = launch_step_record(sa, tape_of(h, 0, (size_t)(s - last_s0)), h->d_packs.p, h->num_packs, h->block,
                                     h->stream)) != hipSuccess)
             return h->hip_fail(e, "record kernel launch");
@@ -151,6 +158,15 @@ int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_ev
     A.last_recorded = true;
     A.ready = true;
     return KSCR_OK;
+}
+}  // namespace
+
+int kscr_adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every) {
+    return adjoint_forward(h, dt, nsteps, checkpoint_every, false);
+}
+
+int kscr_adjoint_forward_sdf(kscr_handle* h, float dt, int nsteps, int checkpoint_every) {
+    return adjoint_forward(h, dt, nsteps, checkpoint_every, true);
 }
 
 int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar, float* k_stretch_bar,
@@ -160,6 +176,8 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     if (!A.ready) return h->fail(KSCR_ERR_NOT_CONFIGURED, "kscr_adjoint_forward has not been run");
     if (h->p.num_itr != A.num_itr || std::max(1, (int)std::ceil(A.dt / h->p.max_h)) != A.substeps)
         return h->fail(KSCR_ERR_INVALID_ARGUMENT, "num_itr or max_h changed since the forward pass");
+    if ((h->p.wind_peak != 0) != A.wind)
+        return h->fail(KSCR_ERR_INVALID_ARGUMENT, "the wind was switched on or off since the forward pass");
     int rc = adjoint_supported(h);
     if (rc) return rc;
     hipError_t e;
@@ -200,6 +218,16 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     for (int k = 0; k < 3; k++) aa.ext[k] = h->p.gravity[k] * h->p.external_force_multiplier;
     // |ext|^2 as the device dot product forms it
     aa.ext2 = std::fma(aa.ext[2], aa.ext[2], std::fma(aa.ext[1], aa.ext[1], aa.ext[0] * aa.ext[0]));
+    for (int k = 0; k < 3; k++) aa.gravity[k] = h->p.gravity[k];
+    aa.external_force_multiplier = h->p.external_force_multiplier;
+    aa.wind_peak = h->p.wind_peak;
+    aa.wind_y_freq = h->p.wind_y_freq;
+    aa.wind_z_freq = h->p.wind_z_freq;
+    aa.wind_time_freq = h->p.wind_time_freq;
+    aa.wind_sharpness = h->p.wind_sharpness;
+    // the SDF pass as step_args gives it to the forward's steps
+    aa.sdf_on = (A.with_sdf && h->sdf_on && h->sdf_set) ? 1 : 0;
+    aa.sdf = h->sdf;
     aa.xb = A.xb.p;
     aa.vb = A.vb.p;
     aa.ub = A.ub.p;
@@ -224,7 +252,7 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
     // (record stream) overlaps the sweep of segment j (handle stream); it
     // waits for the sweep of segment j + 1, the last user of its slot.
     auto slot_of = [&](int seg) { return (size_t)((nseg - 1 - seg) % 2); };
-    std::vector<std::vector<float>> hs(nseg), lhs(nseg);
+    std::vector<std::vector<float>> hs(nseg), lhs(nseg), ts(nseg);
     std::vector<bool> rerecorded(nseg, false);
     if (rc == KSCR_OK) {
         (void)hipEventRecord(ev_start, ms);
@@ -235,10 +263,12 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
         if (seg == nseg - 1 && A.last_recorded) {
             hs[seg] = A.last_h_of;  // recorded by the forward pass
             lhs[seg] = A.last_lh_of;
+            ts[seg] = A.last_t_of;
         } else {
             rerecorded[seg] = true;
             hs[seg].assign(A.every, 0.f);
             lhs[seg].assign(A.every, 0.f);
+            ts[seg].assign(A.every, 0.f);
             if (seg + 2 <= nseg - 1 && (e = hipStreamWaitEvent(rs, sweep_end(seg + 2), 0)) != hipSuccess) {
                 rc = h->hip_fail(e, "hipStreamWaitEvent");
                 break;
@@ -250,9 +280,10 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
             }
             set_clock(h, A.clocks[seg]);
             for (int s = s0; s < s1; s++) {
-                const StepArgs sa = step_args(h, A.dt, false);
+                const StepArgs sa = step_args(h, A.dt, A.with_sdf);
                 hs[seg][s - s0] = sa.h;
                 lhs[seg][s - s0] = sa.last_h;
+                ts[seg][s - s0] = sa.time;
                 if ((e = launch_step_record(sa, tape_of(h, slot_of(seg), (size_t)(s - s0)), h->d_packs.p,
                                             h->num_packs, h->block, rs)) != hipSuccess) {
                     rc = h->hip_fail(e, "record kernel launch");
@@ -268,6 +299,7 @@ int kscr_adjoint_backward(kscr_handle* h, const float* x_bar, const float* v_bar
             aa.tape = tape_of(h, slot_of(seg), (size_t)(s - s0));
             aa.h = hs[seg][s - s0];
             aa.last_h = lhs[seg][s - s0];
+            aa.time = ts[seg][s - s0];
             if ((e = launch_step_adjoint(aa, h->d_packs.p, h->num_packs, h->block, ms)) != hipSuccess)
                 rc = h->hip_fail(e, "adjoint kernel launch");
         }

@@ -94,7 +94,10 @@ struct AdjointRun {
     DevBuf<float4> qb, qrb;
     // the forward pass records the last segment's tape itself
     bool last_recorded = false;
-    std::vector<float> last_h_of, last_lh_of;  // h, last_h of the last segment's steps
+    std::vector<float> last_h_of, last_lh_of, last_t_of;  // h, last_h, time of the last segment's steps
+    bool with_sdf = false;  // steps are kscr_solve_next_step's (the SDF pass after each)
+    bool wind = false;      // the tape holds the adaptive guess's acceleration
+    DevBuf<float> t_acc;
     float record_ms = 0, sweep_ms = 0, backward_ms = 0;
     // the backward pass re-records segment j - 1 on this stream while the
     // handle's stream sweeps segment j (tapes in two slots by segment parity)

@@ -69,6 +69,7 @@ struct TapeArgs {
     float* dx;
     float4* q;   // q^0 .. q^K per substep
     float* ilw;  // per iteration: the orientation solve's 1 / |w| (w normalized to q^{k+1})
+    float* acc;  // per substep, with wind (else null): the adaptive guess's acceleration (v - u) / lastH
     int n;
 };
 
@@ -88,6 +89,13 @@ struct AdjointArgs {
     float acceleration_ratio;
     float ext[3];  // gravity * external_force_multiplier
     float ext2;    // |ext|^2
+    // wind (wind_peak != 0: the tape holds acc) at the step's time
+    float gravity[3];
+    float external_force_multiplier;
+    float wind_peak, wind_y_freq, wind_z_freq, wind_time_freq, wind_sharpness, time;
+    // the SDF pass at the end of the step (sdf_on)
+    int sdf_on;
+    SdfArgs sdf;
     float* xb;
     float* vb;
     float* ub;

@@ -114,9 +114,11 @@ __device__ __forceinline__ void run_pack(const StepArgs& a, const int vbeg, cons
         f3 y, dx;
         if (RECORD && act) tape3(tp->x, s, tp->n, i, x);
         bool sigma_free;
-        init_vertex(a, free, x, v, u, y, dx, sigma_free);
+        f3 acc;
+        init_vertex(a, free, x, v, u, y, dx, sigma_free, acc);
         if (RECORD && act) {
             tape3(tp->y, s, tp->n, i, y);
+            if (tp->acc) tape3(tp->acc, s, tp->n, i, acc);
             tp->sigma_free[(size_t)s * tp->n + i] = sigma_free ? 1 : 0;
             tape3(tp->dx, s * (a.num_itr + 1), tp->n, i, dx);
             tape4(tp->q + (size_t)s * (a.num_itr + 1) * tp->n + i, q);

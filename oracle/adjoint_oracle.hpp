@@ -28,7 +28,7 @@ struct SubstepTape {
 template <class T>
 struct StepTape {
     std::vector<SubstepTape<T>> sub;
-    T h, lastH;
+    T h, lastH, time;
 };
 
 template <class T>
@@ -71,6 +71,37 @@ inline R4<T> rmul_conj_e1(R4<T> a) {
     return R4<T>{-a.w, -a.z, a.y, a.x};
 }
 
+// The wind's force along x at position x (init_itr's expressions) and its
+// derivatives with respect to x.y and x.z on the branch x lies on; sin(t) is
+// cos(t - pi / 2) by the same cos as the forward's.
+template <class T>
+struct WindForce {
+    T fx, d_dy, d_dz;
+};
+template <class T>
+inline WindForce<T> wind_force(const Params& p, T time, V3<T> x) {
+    WindForce<T> w{0, 0, 0};
+    const double thy = 2 * M_PI * (double)((T)p.wind_y_freq * x.y + (T)p.wind_time_freq * time);
+    const double thz = 2 * M_PI * (double)((T)p.wind_z_freq * std::fabs(x.z) + (T)p.wind_time_freq * time);
+    const T wy = x.y > 0 ? wind_cos_t<T>(thy) : T(0);
+    const T wz = wind_cos_t<T>(thz);
+    const T yz = wy * wz;
+    const T sg = (T)((T(0) < yz) - (yz < T(0)));
+    const T pw = wind_pow_t(yz, (T)p.wind_sharpness) * sg;
+    const T mx = std::isnan(pw) ? T(0) : std::max(T(0), pw);
+    w.fx = (T)p.wind_peak * mx;
+    if (pw > T(0)) {
+        const T s = (T)p.wind_sharpness;
+        const T dmx = sg * (s * wind_pow_t(yz, s - T(1)));
+        const T dwy = x.y > 0 ? -wind_cos_t<T>(thy - 0.5 * M_PI) * ((T)(2 * M_PI) * (T)p.wind_y_freq) : T(0);
+        const T sz = (T)((T(0) < x.z) - (x.z < T(0)));
+        const T dwz = -wind_cos_t<T>(thz - 0.5 * M_PI) * (((T)(2 * M_PI) * (T)p.wind_z_freq) * sz);
+        w.d_dy = (((T)p.wind_peak * dmx) * wz) * dwy;
+        w.d_dz = (((T)p.wind_peak * dmx) * wy) * dwz;
+    }
+    return w;
+}
+
 template <class T>
 struct AdjointSim : Sim<T> {
     using Sim<T>::n;
@@ -93,6 +124,7 @@ struct AdjointSim : Sim<T> {
     using Sim<T>::timeF;
     using Sim<T>::time;
     using Sim<T>::step_counter;
+    using Sim<T>::sdf;
 
     // Forward step (advance) recording the tape.
     void record_step(float dtF, StepTape<T>& tape) {
@@ -103,6 +135,7 @@ struct AdjointSim : Sim<T> {
         h = (T)hF;
         tape.h = h;
         tape.lastH = lastH;
+        tape.time = time;
         tape.sub.assign(steps, SubstepTape<T>{});
         for (int s = 0; s < steps; s++, step_counter++) {
             SubstepTape<T>& t = tape.sub[s];
@@ -147,6 +180,73 @@ struct AdjointSim : Sim<T> {
         T lambda = length(vv) + length4(b);
         const R4<T> w = add4(rmul_e1(rmul_pure(vv, b)), scale4(lambda, b));
         return T(1) / std::sqrt(dot4(w, w));
+    }
+
+    // Reverse of apply_sdf at the particle (x, v) it was applied to: xb, vb
+    // hold the adjoints of the projected position and velocity on entry and
+    // of x, v on exit (the cell choice and the inside test count as
+    // constant; within a cell the weights are affine in x).
+    void sdf_vjp(V3<T> x, V3<T> v, V3<T>& xb, V3<T>& vb) const {
+        const R4<T> invq{-(T)sdf.rot[0], -(T)sdf.rot[1], -(T)sdf.rot[2], (T)sdf.rot[3]};
+        const R4<T> rq{(T)sdf.rot[0], (T)sdf.rot[1], (T)sdf.rot[2], (T)sdf.rot[3]};
+        const V3<T> tr{(T)sdf.trans[0], (T)sdf.trans[1], (T)sdf.trans[2]};
+        const V3<T> mn{(T)sdf.gmin[0], (T)sdf.gmin[1], (T)sdf.gmin[2]};
+        const V3<T> lp = rotate(invq, x - tr) - mn;
+        const T l[3] = {lp.x, lp.y, lp.z};
+        int idx[3];
+        T ud[3];
+        for (int a = 0; a < 3; a++) idx[a] = (int)((double)(l[a] * (T)sdf.inv_ds[a]) + 0.5);
+        if (!this->sdf_inside(idx[0], idx[1], idx[2])) return;
+        for (int a = 0; a < 3; a++)
+            ud[a] = (T)((double)sdf.inv_ds[a] * ((double)l[a] - ((double)idx[a] - 0.5) * (double)sdf.ds[a]));
+        T c[8][4];
+        for (int o = 0; o < 8; o++)
+            this->sdf_corner(idx[0] + ((o >> 2) & 1), idx[1] + ((o >> 1) & 1), idx[2] + (o & 1), c[o]);
+        T res[4];
+        for (int k = 0; k < 4; k++) {
+            T c00 = c[0][k] * (1.f - ud[0]) + c[4][k] * ud[0];
+            T c01 = c[1][k] * (1.f - ud[0]) + c[5][k] * ud[0];
+            T c10 = c[2][k] * (1.f - ud[0]) + c[6][k] * ud[0];
+            T c11 = c[3][k] * (1.f - ud[0]) + c[7][k] * ud[0];
+            T c0 = c00 * (1.f - ud[1]) + c10 * ud[1];
+            T c1 = c01 * (1.f - ud[1]) + c11 * ud[1];
+            res[k] = c0 * (1.f - ud[2]) + c1 * ud[2];
+        }
+        const V3<T> m{res[0], res[1], res[2]};
+        const T inv_m = T(1) / std::sqrt(dot(m, m));
+        const V3<T> nrm = m * inv_m;
+        const T phi = res[3] - (T)sdf.thresh;
+        if (!(phi < 0)) return;
+        V3<T> nb = (-phi) * xb;
+        const T phib = -dot(nrm, xb);
+        const T vl = length(v);
+        if ((double)vl >= 1e-6) {
+            const V3<T> vn = v / vl;
+            const T beta = dot(vn, nrm);
+            const T betab = -dot(nrm, vb);
+            nb = nb - beta * vb;
+            nb = nb + betab * vn;
+            vb = vb + (betab / vl) * (nrm - beta * vn);
+        }
+        const V3<T> mb = (nb - dot(nrm, nb) * nrm) * inv_m;
+        const T rb[4] = {mb.x, mb.y, mb.z, phib};
+        T udb[3] = {0, 0, 0};
+        for (int k = 0; k < 4; k++) {
+            T c00 = c[0][k] * (1.f - ud[0]) + c[4][k] * ud[0];
+            T c01 = c[1][k] * (1.f - ud[0]) + c[5][k] * ud[0];
+            T c10 = c[2][k] * (1.f - ud[0]) + c[6][k] * ud[0];
+            T c11 = c[3][k] * (1.f - ud[0]) + c[7][k] * ud[0];
+            T c0 = c00 * (1.f - ud[1]) + c10 * ud[1];
+            T c1 = c01 * (1.f - ud[1]) + c11 * ud[1];
+            const T c0b = rb[k] * (1.f - ud[2]), c1b = rb[k] * ud[2];
+            udb[2] += rb[k] * (c1 - c0);
+            const T c00b = c0b * (1.f - ud[1]), c10b = c0b * ud[1], c01b = c1b * (1.f - ud[1]), c11b = c1b * ud[1];
+            udb[1] += c0b * (c10 - c00) + c1b * (c11 - c01);
+            udb[0] += ((c00b * (c[4][k] - c[0][k]) + c01b * (c[5][k] - c[1][k])) + c10b * (c[6][k] - c[2][k])) +
+                      c11b * (c[7][k] - c[3][k]);
+        }
+        const V3<T> lb{udb[0] * (T)sdf.inv_ds[0], udb[1] * (T)sdf.inv_ds[1], udb[2] * (T)sdf.inv_ds[2]};
+        xb = xb + rotate(rq, lb);
     }
 
     // Reverse of one substep: adj holds the adjoints of the substep's outputs
@@ -291,13 +391,35 @@ struct AdjointSim : Sim<T> {
         }
         // ---- initItr: dx^0 = h v (+ h^2 sigma ext), y = h v + h^2 ext, u' = v
         const V3<T> g{(T)p.gravity[0], (T)p.gravity[1], (T)p.gravity[2]};
-        const V3<T> ext = g * (T)p.external_force_multiplier;  // no wind on the adjoint path
+        const V3<T> ext = g * (T)p.external_force_multiplier;  // without wind
         const T ext2 = length2(ext);
         std::vector<V3<T>> ubn(n, V3<T>{0, 0, 0});
         for (int i = 0; i < n; i++) {
             V3<T> vb = adj.vb[i] + adj.ub[i];  // u' = v
             vb = vb + hh * dxb[i];
-            if (invMass[i] != 0) {
+            if (invMass[i] != 0 && p.wind_peak != 0) {
+                // ext depends on x through the wind: ext = (g + (wind(x), 0, 0)) efm
+                vb = vb + hh * yb[i];
+                const WindForce<T> w = wind_force<T>(p, time, t.x[i]);
+                const V3<T> ew = (g + V3<T>{w.fx, 0, 0}) * (T)p.external_force_multiplier;
+                const T ew2 = length2(ew);
+                V3<T> extb = (hh * hh) * yb[i];
+                if (ew2 > 0) {
+                    const V3<T> acc = (t.v[i] - t.u[i]) / lastH;
+                    const T raw = dot(acc, ew) / ew2;
+                    extb = extb + (hh * hh * std::min(std::max(raw, T(0)), T(1))) * dxb[i];
+                    if (raw > 0 && raw < 1) {
+                        const T sb = hh * hh * dot(dxb[i], ew);
+                        const V3<T> ab = (sb / ew2) * ew;
+                        vb = vb + ab / lastH;
+                        ubn[i] = ubn[i] - ab / lastH;
+                        extb = extb + sb * (acc / ew2 - ((T(2) * raw) / ew2) * ew);
+                    }
+                }
+                const T wb = extb.x * (T)p.external_force_multiplier;
+                adj.xb[i].y = adj.xb[i].y + wb * w.d_dy;
+                adj.xb[i].z = adj.xb[i].z + wb * w.d_dz;
+            } else if (invMass[i] != 0) {
                 vb = vb + hh * yb[i];
                 if (ext2 > 0) {
                     const V3<T> a = (t.v[i] - t.u[i]) / lastH;
@@ -337,13 +459,17 @@ struct AdjointSim : Sim<T> {
     // Loss L = sum_s |x_tip(s) - target_s|^2 after nsteps steps of dtF, and its
     // gradient with respect to every vertex's kStretch and qRest.  Forward with
     // checkpoints every K steps; backward segment by segment, recomputing the
-    // segment's steps from its checkpoint to record their tapes.
+    // segment's steps from its checkpoint to record their tapes.  With
+    // with_sdf every step is followed by the SDF pass (solve_next_step).  x0b,
+    // v0b (optional) receive dL/dx and dL/dv of the initial state.
     T gradient(float dtF, int nsteps, int K, const std::vector<int>& tips, const std::vector<T>& targets,
-               std::vector<T>& ksb, std::vector<R4<T>>& qrb) {
+               std::vector<T>& ksb, std::vector<R4<T>>& qrb, bool with_sdf = false,
+               std::vector<V3<T>>* x0b = nullptr, std::vector<V3<T>>* v0b = nullptr) {
         std::vector<Checkpoint> ck;
         for (int s = 0; s < nsteps; s++) {
             if (s % K == 0) ck.push_back(save());
             this->advance(dtF);
+            if (with_sdf) this->apply_sdf();
         }
         T loss = 0;
         Adjoint<T> adj;
@@ -358,22 +484,41 @@ struct AdjointSim : Sim<T> {
             restore(ck[seg]);
             const int s0 = seg * K, s1 = std::min(nsteps, s0 + K);
             std::vector<StepTape<T>> tapes(s1 - s0);
-            for (int s = s0; s < s1; s++) record_step(dtF, tapes[s - s0]);
-            for (int s = s1 - 1; s >= s0; s--) backward_step(tapes[s - s0], adj);
+            for (int s = s0; s < s1; s++) {
+                record_step(dtF, tapes[s - s0]);
+                if (with_sdf) this->apply_sdf();
+            }
+            for (int s = s1 - 1; s >= s0; s--) backward_step(tapes[s - s0], adj, with_sdf);
         }
         restore(end);
         ksb = adj.ksb;
         qrb = adj.qrb;
+        if (x0b) *x0b = adj.xb;
+        if (v0b) *v0b = adj.vb;
         return loss;
     }
 
-    void backward_step(const StepTape<T>& tape, Adjoint<T>& adj) {
-        const T h_save = h, lh_save = lastH;
+    void backward_step(const StepTape<T>& tape, Adjoint<T>& adj, bool with_sdf = false) {
+        const T h_save = h, lh_save = lastH, t_save = time;
         h = tape.h;
         lastH = tape.lastH;
+        time = tape.time;
+        if (with_sdf && sdf.enabled) {
+            // the SDF pass after the last substep, at the state end_itr left
+            const SubstepTape<T>& t = tape.sub.back();
+            const int K = p.num_itr;
+            const T invH = T(1) / h;
+            for (int i = 0; i < n; i++) {
+                if (!(flags[i] & F_PREV)) continue;  // strand roots are skipped
+                const V3<T> vp = invMass[i] != 0 ? t.dx[K][i] * invH * (T(1) - T(p.velocity_decay) * h) : t.v[i];
+                const V3<T> xp = t.x[i] + t.dx[K][i];
+                sdf_vjp(xp, vp, adj.xb[i], adj.vb[i]);
+            }
+        }
         for (int s = (int)tape.sub.size() - 1; s >= 0; s--) backward_substep(tape.sub[s], adj);
         h = h_save;
         lastH = lh_save;
+        time = t_save;
     }
 };
 

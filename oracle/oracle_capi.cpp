@@ -25,6 +25,7 @@ struct Base {
     virtual int set_params(const orc::Params& p) = 0;
     virtual int upload() = 0;
     virtual void set_pos_vel(const float* p, const float* v) = 0;
+    virtual void set_pos_vel_t(const double* p, const double* v) = 0;
     virtual void set_last_vel(const float* u) = 0;
     virtual void set_connections(const int* conn) = 0;
     virtual void get_pos_vel(float* p, float* v) const = 0;
@@ -47,7 +48,7 @@ struct Base {
     virtual void get_dx(float* out) const = 0;
     virtual void set_material_t(const double* ks, const double* qr) = 0;
     virtual double gradient(float dt, int nsteps, int K, const int* tips, int ntips, const double* targets, double* ksb,
-                            double* qrb) = 0;
+                            double* qrb, int with_sdf, double* x0b, double* v0b) = 0;
     virtual long clamp_log(int enable, uint8_t* out, long cap) = 0;
 };
 
@@ -80,6 +81,12 @@ struct Impl : Base {
         return 0;
     }
     void set_pos_vel(const float* p, const float* v) override {
+        for (int i = 0; i < s.n; i++) {
+            if (p) s.x[i] = orc::V3<T>{(T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2]};
+            if (v) s.v[i] = orc::V3<T>{(T)v[3 * i], (T)v[3 * i + 1], (T)v[3 * i + 2]};
+        }
+    }
+    void set_pos_vel_t(const double* p, const double* v) override {
         for (int i = 0; i < s.n; i++) {
             if (p) s.x[i] = orc::V3<T>{(T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2]};
             if (v) s.v[i] = orc::V3<T>{(T)v[3 * i], (T)v[3 * i + 1], (T)v[3 * i + 2]};
@@ -169,13 +176,18 @@ struct Impl : Base {
     }
     void set_material_t(const double* ks, const double* qr) override { s.set_material_t(ks, qr); }
     double gradient(float dt, int nsteps, int K, const int* tips, int ntips, const double* targets, double* ksb,
-                    double* qrb) override {
+                    double* qrb, int with_sdf, double* x0b, double* v0b) override {
         std::vector<int> tp(tips, tips + ntips);
         std::vector<T> tg(3 * ntips);
         for (int k = 0; k < 3 * ntips; k++) tg[k] = (T)targets[k];
         std::vector<T> kb;
         std::vector<orc::R4<T>> qb;
-        const T loss = s.gradient(dt, nsteps, K, tp, tg, kb, qb);
+        std::vector<orc::V3<T>> xb, vb;
+        const T loss = s.gradient(dt, nsteps, K, tp, tg, kb, qb, with_sdf != 0, &xb, &vb);
+        for (int i = 0; i < s.n; i++) {
+            if (x0b) x0b[3 * i] = (double)xb[i].x, x0b[3 * i + 1] = (double)xb[i].y, x0b[3 * i + 2] = (double)xb[i].z;
+            if (v0b) v0b[3 * i] = (double)vb[i].x, v0b[3 * i + 1] = (double)vb[i].y, v0b[3 * i + 2] = (double)vb[i].z;
+        }
         for (int i = 0; i < s.n; i++) {
             ksb[i] = (double)kb[i];
             qrb[4 * i] = (double)qb[i].x, qrb[4 * i + 1] = (double)qb[i].y, qrb[4 * i + 2] = (double)qb[i].z,
@@ -398,14 +410,22 @@ int oracle_set_material(oracle_handle* h, const float* ks, const float* qr) {
     h->b->set_material(ks, qr);
     return 0;
 }
+// Positions and velocities in double (the FP64 oracle's state without a
+// float rounding; either pointer may be null).
+int oracle_set_pos_vel64(oracle_handle* h, const double* p, const double* v) {
+    h->b->set_pos_vel_t(p, v);
+    return 0;
+}
 int oracle_set_material64(oracle_handle* h, const double* ks, const double* qr) {
     h->b->set_material_t(ks, qr);
     return 0;
 }
-// Loss and dLoss/d(kStretch, qRest) of nsteps steps (checkpoints every K).
+// Loss and dLoss/d(kStretch, qRest) of nsteps steps (checkpoints every K);
+// with_sdf: each step followed by the SDF pass; x0b, v0b (may be null):
+// dLoss/d(initial x, v).
 double oracle_gradient(oracle_handle* h, float dt, int nsteps, int K, const int* tips, int ntips, const double* targets,
-                       double* ksb, double* qrb) {
-    return h->b->gradient(dt, nsteps, K, tips, ntips, targets, ksb, qrb);
+                       double* ksb, double* qrb, int with_sdf, double* x0b, double* v0b) {
+    return h->b->gradient(dt, nsteps, K, tips, ntips, targets, ksb, qrb, with_sdf, x0b, v0b);
 }
 
 // Branches of the adaptive initial guess's clamp (scr_oracle.hpp init_itr),

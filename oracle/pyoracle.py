@@ -74,7 +74,8 @@ SIGNATURES = {
     "oracle_contacts": (None, [_P, _IP, _IP]),
     "oracle_set_material": (C.c_int, [_P, _FP, _FP]),
     "oracle_set_material64": (C.c_int, [_P, _DP, _DP]),
-    "oracle_gradient": (C.c_double, [_P, C.c_float, C.c_int, C.c_int, _IP, C.c_int, _DP, _DP, _DP]),
+    "oracle_set_pos_vel64": (C.c_int, [_P, _DP, _DP]),
+    "oracle_gradient": (C.c_double, [_P, C.c_float, C.c_int, C.c_int, _IP, C.c_int, _DP, _DP, _DP, C.c_int, _DP, _DP]),
     "oracle_clamp_log": (C.c_long, [_P, C.c_int, C.POINTER(C.c_uint8), C.c_long]),
 }
 
@@ -179,6 +180,13 @@ class OracleSim:
         v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
         self.lib.oracle_set_pos_vel(self.h, None if p is None else _f(p), None if v is None else _f(v))
 
+    def set_pos_vel64(self, pos=None, vel=None):
+        """Positions and velocities without rounding to float (for the FP64 state)."""
+        p = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        self.lib.oracle_set_pos_vel64(self.h, None if p is None else p.ctypes.data_as(_DP),
+                                      None if v is None else v.ctypes.data_as(_DP))
+
     def set_connections(self, conn=None):
         c = None if conn is None else np.ascontiguousarray(conn, dtype=np.int32)
         self.lib.oracle_set_connections(self.h, None if c is None else _i(c))
@@ -252,14 +260,21 @@ class OracleSim:
         qr = np.ascontiguousarray(q_rest, np.float64).reshape(-1)
         self.lib.oracle_set_material64(self.h, ks.ctypes.data_as(_DP), qr.ctypes.data_as(_DP))
 
-    def gradient(self, dt, nsteps, ckpt_every, tips, targets):
-        """Loss and (dL/dkStretch[n], dL/dqRest[n,4]) in float64."""
+    def gradient(self, dt, nsteps, ckpt_every, tips, targets, with_sdf=False, initial=False):
+        """Loss and (dL/dkStretch[n], dL/dqRest[n,4]) in float64; with_sdf:
+        each step followed by the SDF pass; initial: also dL/dx and dL/dv of
+        the initial state ([n,3] each)."""
         tips = np.ascontiguousarray(tips, np.int32)
         tg = np.ascontiguousarray(targets, np.float64).reshape(-1)
         ksb = np.empty(self.n, np.float64)
         qrb = np.empty((self.n, 4), np.float64)
+        x0b = np.empty((self.n, 3), np.float64)
+        v0b = np.empty((self.n, 3), np.float64)
         loss = self.lib.oracle_gradient(self.h, dt, nsteps, ckpt_every, _i(tips), tips.size, tg.ctypes.data_as(_DP),
-                                        ksb.ctypes.data_as(_DP), qrb.ctypes.data_as(_DP))
+                                        ksb.ctypes.data_as(_DP), qrb.ctypes.data_as(_DP), 1 if with_sdf else 0,
+                                        x0b.ctypes.data_as(_DP), v0b.ctypes.data_as(_DP))
+        if initial:
+            return loss, ksb, qrb, x0b, v0b
         return loss, ksb, qrb
 
     def log_clamp(self, enable=True):

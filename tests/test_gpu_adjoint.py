@@ -221,3 +221,49 @@ def test_differentiation_workload_gradient_accuracy():
     assert r["fp64_adjoint_vs_fp64_fd_best_step_max_rel"]["dkappa"] < 1e-3
     assert r["device_vs_fp64_gradient_rel"]["dkappa_median"] < 0.1
     assert r["fp32_position_noise_over_fp64_displacement"] > 1  # the regime the docstring states
+
+
+@pytest.mark.parametrize("case", ["wind", "sdf", "wind+sdf"])
+def test_adjoint_with_wind_and_sdf_matches_fp32_oracle(case):
+    """Wind (a force that depends on the positions and the time) and the
+    body SDF pass after every step (kscr_adjoint_forward_sdf, the
+    solve_next_step rollout), from random initial velocities, against the
+    FP32 restatement of the reverse sweep, bit for bit."""
+    from tests.test_oracle_adjoint import SDF_POSE
+
+    ns, nv, nsteps, K = 24, 64, 10, 3
+    sc = scenes.groom(ns, nv, seed=31)
+    sc.material = scenes.Material(num_itr=10, wind_peak=40.0 if "wind" in case else 0.0)
+    p = scenes.make_params(default_params, sc.material)
+    with_sdf = "sdf" in case
+    if with_sdf:
+        # reaches the outer vertices of the 63 mm strands on a 0.1 m head
+        sc.sdf = scenes.sdf_grid(scenes.sphere_sdf((0, 0, 0), 0.158), (-0.2,) * 3, (0.2,) * 3, (64,) * 3)
+    print(f"[gpu-adjoint] {case}: {ns}x{nv} steps={nsteps} K={K}", flush=True)
+    rng = np.random.default_rng(9)
+    v0 = rng.normal(0.0, 0.05, sc.positions.shape).astype(np.float32)
+    v0[sc.strand_offsets[:-1]] = 0.0
+    g = Sim(p, sc.positions, sc.strand_offsets, device=0)
+    o = OracleSim(p, sc.positions, sc.strand_offsets, precision=32)
+    for s in (g, o):
+        scenes.setup_sim(s, sc)
+        if with_sdf:
+            s.set_sdf(sc.sdf, 0.0, **SDF_POSE)
+        s.set_pos_vel(None, v0)
+    ks, qr = _perturbed_material(g, sc, p, np.random.default_rng(5))
+    g.set_material(ks, qr)
+    o.set_material(ks, qr)
+    tips = (sc.strand_offsets[1:] - 1).astype(np.int32)
+    targets = sc.positions[tips].astype(np.float64) + np.array([0.0, -2e-3, 1e-3])
+    g.adjoint_forward(DT, nsteps, K, with_sdf=with_sdf)
+    x, _ = g.get_pos_vel()
+    xb = np.zeros_like(x)
+    xb[tips] = 2 * (x[tips] - targets.astype(np.float32))
+    ksb_g, qrb_g = g.adjoint_backward(xb)
+    _, ksb_o, qrb_o = o.gradient(DT, nsteps, K, tips, targets, with_sdf=with_sdf)
+    xo, _ = o.get_pos_vel()
+    assert np.array_equal(x, xo)
+    if with_sdf:  # the pass moved vertices: the scene exercises it
+        r = np.linalg.norm(x - np.asarray(SDF_POSE["translation"], np.float32), axis=1)
+        assert np.sum(np.abs(r - 0.158) < 1e-4) > 10
+    _compare(case, ksb_g, qrb_g, ksb_o, qrb_o)

@@ -12,9 +12,17 @@ from oracle.pyoracle import OracleSim
 DT = np.float32(1 / 600)
 
 
+# the body SDF's pose in the scenes that set one: a turn of 0.5 rad about
+# (1, 2, 3) and a shift of a few mm (the grid's frame is not the world's)
+_SDF_AXIS = np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0)
+SDF_POSE = dict(rotation=tuple(np.append(np.sin(0.25) * _SDF_AXIS, np.cos(0.25))), translation=(3e-3, -2e-3, 1e-3))
+
+
 def _setup(sc, p, E, kap, prec=64):
     o = OracleSim(p, sc.positions, sc.strand_offsets, precision=prec)
     scenes.setup_sim(o, sc)
+    if sc.sdf is not None:
+        o.set_sdf(sc.sdf, sc.sdf_threshold, **SDF_POSE)
     rest = o.get_rest()
     mat = materials.strand_materials(materials.rest_directions(rest["q_rest"]), rest["l_rest"], sc.strand_offsets, E, kap,
                                      p.radius)
@@ -65,3 +73,106 @@ def test_adjoint_matches_finite_differences(oracle_lib, K):
             rel = abs(g[s] - fd) / max(abs(fd), 1e-300)
             print(f"strand {s} d/d{name}: adjoint {g[s]:.6e} fd {fd:.6e} rel {rel:.2e}")
             assert rel < 1e-3, (s, name, g[s], fd)
+
+
+def _roll(sc, p, E, kap, nsteps, with_sdf, x0=None, v0=None):
+    """FP64 rollout from the set-up state (positions and velocities replaced
+    by x0, v0 when given, without a float rounding); solve_next_step when
+    with_sdf, else advance."""
+    o, _ = _setup(sc, p, E, kap)
+    if x0 is not None or v0 is not None:
+        o.set_pos_vel64(x0, v0)
+    for _ in range(nsteps):
+        (o.solve_next_step if with_sdf else o.advance)(DT)
+    return o.get_pos_vel(f64=True)[0]
+
+
+@pytest.mark.parametrize("case", ["wind", "sdf"])
+def test_adjoint_with_wind_and_sdf_matches_finite_differences(oracle_lib, case):
+    """The wind (its force depends on the positions) and the SDF pass after
+    each step (a projection onto the body surface within a grid cell):
+    dL/d(E, kappa) per strand and dL/d(initial position, velocity) of single
+    vertices against FP64 central differences.
+
+    The strands start with random velocities.  From rest, every vertex out
+    of reach of the pinned root falls freely in the first substep, so its
+    adaptive guess sits exactly on the clamp of the guess's weight at 1
+    (a kink: the loss has no derivative there in the initial positions)."""
+    ns, nv, nsteps, K = 3, 48, 8, 4
+    sc = scenes.groom(ns, nv, seed=23)
+    wind = 40.0 if case == "wind" else 0.0
+    m = scenes.Material(num_itr=10, wind_peak=wind)
+    sc.material = m
+    with_sdf = case == "sdf"
+    if with_sdf:
+        # a sphere about the head reaching the tips (the strands are 47 mm
+        # long and grow out of a 0.1 m sphere): the outer vertices of each
+        # curl are pushed onto its surface every step
+        R = 0.145
+        sc.sdf = scenes.sdf_grid(scenes.sphere_sdf((0, 0, 0), R), (-0.2,) * 3, (0.2,) * 3, (64,) * 3)
+        sc.sdf_threshold = 0.0
+    p = scenes.make_params(default_params, m)
+    tips = (sc.strand_offsets[1:] - 1).astype(np.int32)
+    E0, k0 = np.full(ns, 3.0), np.ones(ns)
+    rng = np.random.default_rng(6)
+    v_start = rng.normal(0.0, 0.05, sc.positions.shape)
+    v_start[sc.strand_offsets[:-1]] = 0.0
+    targets = _roll(sc, p, E0 * (1 + 0.2 * rng.uniform(-1, 1, ns)), k0 * (1 + 0.2 * rng.uniform(-1, 1, ns)), nsteps,
+                    with_sdf, v0=v_start)[tips]
+    o, mat = _setup(sc, p, E0, k0)
+    o.set_pos_vel64(None, v_start)
+    x_start = o.get_pos_vel(f64=True)[0]
+    loss, ksb, qrb, x0b, v0b = o.gradient(DT, nsteps, K, tips, targets, with_sdf=with_sdf, initial=True)
+    gE, gk = materials.strand_gradient(mat, sc.strand_offsets, ksb, qrb)
+
+    def L(E=E0, kap=k0, x0=None, v0=v_start):
+        return float(np.sum((_roll(sc, p, E, kap, nsteps, with_sdf, x0, v0)[tips] - targets) ** 2))
+
+    assert loss == pytest.approx(L(), rel=1e-12)
+    if with_sdf:  # the pass moved vertices: the scene exercises it
+        plain = _roll(sc, p, E0, k0, nsteps, False, v0=v_start)
+        assert np.abs(_roll(sc, p, E0, k0, nsteps, True, v0=v_start) - plain).max() > 1e-5
+    # The loss has kinks where the adaptive guess's clamp or the wind's
+    # max(0, .) switches branch; a central difference whose interval straddles
+    # one is off, so each derivative takes the best of three steps.  Errors
+    # are relative to the derivative, with a floor of 1e-2 of the largest
+    # derivative of its kind (a strand far from the body has a tiny dL/dE).
+    def best_fd(f, g, steps, floor):
+        rels, fds = [], []
+        for hstep in steps:
+            fd = (f(+hstep) - f(-hstep)) / (2 * hstep)
+            fds.append(fd)
+            rels.append(abs(g - fd) / max(abs(fd), floor, 1e-300))
+        k = int(np.argmin(rels))
+        return rels[k], fds[k]
+
+    for s in range(ns):
+        for name, base, g in (("E", E0, gE), ("kappa", k0, gk)):
+            def f(d, name=name, base=base, s=s):
+                b = base.copy()
+                b[s] += d
+                return L(E=b) if name == "E" else L(kap=b)
+            rel, fd = best_fd(f, g[s], [base[s] * r for r in (1e-5, 1e-6, 1e-7)], 1e-2 * np.abs(g).max())
+            print(f"{case} strand {s} d/d{name}: adjoint {g[s]:.6e} fd {fd:.6e} rel {rel:.2e}")
+            assert rel < 1e-3, (case, s, name, g[s], fd)
+    # initial state: the wind acts through y and z; the SDF pass through every
+    # coordinate of the vertices it projects
+    checked = 0
+    for what, grad, base, steps in (("x0", x0b, x_start, (1e-7, 1e-8, 1e-9)), ("v0", v0b, v_start, (1e-5, 1e-6, 1e-7))):
+        for s in range(ns):
+            for v in (nv // 3, nv // 2, nv - 4):
+                i = int(sc.strand_offsets[s]) + v
+                for c in range(3):
+                    g = grad[i, c]
+                    if abs(g) < 1e-3 * np.abs(grad).max():
+                        continue
+
+                    def f(d, i=i, c=c, what=what, base=base):
+                        b = base.copy()
+                        b[i, c] += d
+                        return L(x0=b) if what == "x0" else L(v0=b)
+                    rel, fd = best_fd(f, g, steps, 1e-2 * np.abs(grad).max())
+                    print(f"{case} dL/d{what}[{i},{c}]: adjoint {g:.6e} fd {fd:.6e} rel {rel:.2e}")
+                    assert rel < 1e-3, (case, what, i, c, g, fd)
+                    checked += 1
+    assert checked >= 12
