@@ -197,6 +197,21 @@ int kscr_set_sdf(kscr_handle* h, const int dim[3], const float grid_min[3], cons
                  const float* phi, const float* grad, float threshold, const float rotation[4],
                  const float translation[3]);
 int kscr_enable_sdf(kscr_handle* h, int enable);
+/* Strain limiting after the body SDF pass: SDFBodyCollisionSolver's
+ * EnableStrainLimiting (SDFBodyCollisionSolver.cpp:189), StrainError
+ * (SDFBodyCollisionSolver.h:26, default 0.1) and the StrainLimiting kernel
+ * (SDF/cudaSDFCollision.cu:176-199) it launches after SDFHeadCollision
+ * (SDFBodyCollisionSolver.cpp:182-187).  After every step that applies the
+ * SDF pass (kscr_solve_next_step), each strand is walked from its root and a
+ * segment whose length l differs from its rest length l0 by more than
+ * strain_error * l0 is given length l0 by moving its upper vertex along the
+ * segment; velocities are unchanged.  rest_lengths float[num_verts] holds the
+ * rest length of segment (i, i + 1) at i (the last vertex of a strand has
+ * none); NULL takes the segment lengths of the handle's host positions now,
+ * as InitAdditionalHairData does from the Hair (SDFBodyCollisionSolver.cpp:
+ * 93-157).  enable = 0 switches it off.  The batched step and the adjoint
+ * return KSCR_ERR_UNSUPPORTED with it on. */
+int kscr_set_strain_limit(kscr_handle* h, int enable, float strain_error, const float* rest_lengths);
 /* Body SDF grid from a closed triangle mesh, built on the device:
  * PNMesh::GenerateSDF + PNMesh::StoreSDFQueries (src/PNMesh.cpp:100-171)
  * through Kami::SDF::Init / CollisionTest / Distance

@@ -97,6 +97,7 @@ SIGNATURES = {
     "kscr_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
     "kscr_set_sdf": (C.c_int, [_P, _IP, _FP, _FP, _FP, _FP, C.c_float, _FP, _FP]),
     "kscr_enable_sdf": (C.c_int, [_P, C.c_int]),
+    "kscr_set_strain_limit": (C.c_int, [_P, C.c_int, C.c_float, _FP]),
     "kscr_selftest_rounding": (C.c_int, [C.c_int, _FP, C.c_int, C.POINTER(C.c_ulonglong)]),
     "kscr_mesh_sdf": (C.c_int, [C.c_int, _FP, C.c_int, _IP, C.c_int, _IP, _FP, _FP, _FP, _FP, C.c_char_p,
                                 C.c_size_t]),
@@ -374,6 +375,17 @@ class Sim:
 
     def enable_sdf(self, on=True):
         self._check(self.lib.kscr_enable_sdf(self.h, int(bool(on))))
+
+    def set_strain_limit(self, enable=True, strain_error=0.1, rest_lengths=None):
+        """Strain limiting after the SDF pass (kscr_set_strain_limit);
+        rest_lengths (n,) or None for the current segment lengths."""
+        rl = None
+        if rest_lengths is not None:
+            rl = np.ascontiguousarray(rest_lengths, dtype=np.float32).reshape(-1)
+            if rl.size != self.n:
+                raise ValueError("rest_lengths must hold one value per vertex")
+        self._check(self.lib.kscr_set_strain_limit(self.h, int(bool(enable)), float(strain_error),
+                                                   None if rl is None else fptr(rl)))
 
     def advance(self, dt):
         self._check(self.lib.kscr_advance(self.h, dt))

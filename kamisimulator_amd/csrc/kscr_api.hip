@@ -328,6 +328,16 @@ int launch_contact_step(kscr_handle* h, float dt, bool with_sdf) {
     StepArgs a2 = a;
     a2.q = h->d_q.p;
     if (a.sdf_on && (e = sdf_pass(a2, h->n, h->stream)) != hipSuccess) return h->hip_fail(e, "SDF pass");
+    return strain_pass(h, a);
+}
+
+// Strain limiting after the step's SDF pass (SDFBodyCollisionSolver.cpp:
+// 182-187: StrainLimiting runs when UseStrainLimiting is set).
+int strain_pass(kscr_handle* h, const StepArgs& a) {
+    if (!a.sdf_on || !h->strain_on) return KSCR_OK;
+    const hipError_t e = launch_strain_limit(h->d_x.p, h->d_strand_off.p, h->d_strain_rest.p,
+                                             (int)h->strand_off.size() - 1, h->strain_error, h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "strain limiting");
     return KSCR_OK;
 }
 
@@ -359,7 +369,7 @@ int launch_step(kscr_handle* h, float dt, bool with_sdf) {
     e = colored(h) ? launch_step_colored(a, h->d_packs.p, h->num_packs, h->block, h->stream)
                    : launch_step_fused(a, h->d_packs.p, h->num_packs, h->block, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "step kernel launch");
-    return KSCR_OK;
+    return strain_pass(h, a);
 }
 
 int longest_strand(const kscr_handle* h) { return h->longest; }
@@ -934,6 +944,48 @@ int kscr_set_sdf_from_mesh(kscr_handle* h, const float* vertices, int num_vertic
 int kscr_enable_sdf(kscr_handle* h, int enable) {
     if (!h) return KSCR_ERR_INVALID_ARGUMENT;
     h->sdf_on = enable != 0;
+    return KSCR_OK;
+}
+
+int kscr_set_strain_limit(kscr_handle* h, int enable, float strain_error, const float* rest_lengths) {
+    if (!h) return KSCR_ERR_INVALID_ARGUMENT;
+    if (!enable) {
+        h->strain_on = false;
+        return KSCR_OK;
+    }
+    if (!(strain_error >= 0) || !std::isfinite(strain_error))
+        return h->fail(KSCR_ERR_INVALID_ARGUMENT, "strain_error must be finite and >= 0");
+    const int n = h->n;
+    const int ns = (int)h->strand_off.size() - 1;
+    // rest length of segment (i, i + 1); the last vertex of a strand has none
+    std::vector<float> rest(n, 0.0f);
+    for (int s = 0; s < ns; s++)
+        for (int i = h->strand_off[s]; i + 1 < h->strand_off[s + 1]; i++) {
+            if (rest_lengths) {
+                rest[i] = rest_lengths[i];
+            } else {
+                // InitAdditionalHairData (SDFBodyCollisionSolver.cpp:108-120):
+                // glm length of the segment on the host, no FMA
+                const float dx = h->pos[3 * (i + 1)] - h->pos[3 * i];
+                const float dy = h->pos[3 * (i + 1) + 1] - h->pos[3 * i + 1];
+                const float dz = h->pos[3 * (i + 1) + 2] - h->pos[3 * i + 2];
+                rest[i] = std::sqrt((dx * dx + dy * dy) + dz * dz);
+            }
+            if (!(rest[i] >= 0) || !std::isfinite(rest[i]))
+                return h->fail(KSCR_ERR_INVALID_ARGUMENT, "rest lengths must be finite and >= 0");
+        }
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
+    if ((e = h->d_strand_off.alloc(ns + 1)) != hipSuccess || (e = h->d_strain_rest.alloc(n)) != hipSuccess)
+        return h->hip_fail(e, "hipMalloc strain limiting");
+    if ((e = hipMemcpyAsync(h->d_strand_off.p, h->strand_off.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice,
+                            h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->d_strain_rest.p, rest.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return h->hip_fail(e, "upload strain limiting");
+    h->strain_error = strain_error;
+    h->strain_on = true;
     return KSCR_OK;
 }
 

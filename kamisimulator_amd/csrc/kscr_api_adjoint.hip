@@ -111,6 +111,8 @@ int adjoint_forward(kscr_handle* h, float dt, int nsteps, int checkpoint_every, 
     int rc = validate_meta(h);
     if (rc) return rc;
     if ((rc = adjoint_supported(h))) return rc;
+    if (with_sdf && h->strain_on && h->sdf_on && h->sdf_set)
+        return h->fail(KSCR_ERR_UNSUPPORTED, "the adjoint does not cover strain limiting after the SDF pass");
     hipError_t e;
     if ((e = hipSetDevice(h->device)) != hipSuccess) return h->hip_fail(e, "hipSetDevice");
     AdjointRun& A = h->adj;

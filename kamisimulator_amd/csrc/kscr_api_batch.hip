@@ -163,6 +163,9 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
             return b->fail(KSCR_ERR_UNSUPPORTED, "groom " + std::to_string(g) +
                                                      ": contacts, step size limits, glued vertices and the coloured "
                                                      "schedule are not batched");
+        if (with_sdf && h->strain_on && h->sdf_on && h->sdf_set)
+            return b->fail(KSCR_ERR_UNSUPPORTED, "groom " + std::to_string(g) +
+                                                     ": strain limiting after the SDF pass is not batched");
         int rc = validate_meta(h);
         if (rc) return b->fail(rc, "groom " + std::to_string(g) + ": " + h->err);
         // the handles' own streams may still hold work on their state

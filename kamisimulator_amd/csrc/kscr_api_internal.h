@@ -145,6 +145,12 @@ struct kscr_handle {
     // SDF
     bool sdf_set = false, sdf_on = false;
     kscr::SdfArgs sdf{};
+    // strain limiting after the SDF pass (kscr_set_strain_limit): strand
+    // offsets and per-segment rest lengths on the device
+    bool strain_on = false;
+    float strain_error = 0.1f;
+    kscr::api::DevBuf<int> d_strand_off;
+    kscr::api::DevBuf<float> d_strain_rest;
     // hair-hair contacts
     kscr::ContactDev contact;
     float last_bvh_rebuild = INFINITY;
@@ -182,6 +188,7 @@ StepArgs step_args(kscr_handle* h, float dt, bool with_sdf);
 int launch_contact_step(kscr_handle* h, float dt, bool with_sdf);
 int check_device_error(kscr_handle* h);
 int launch_step(kscr_handle* h, float dt, bool with_sdf);
+int strain_pass(kscr_handle* h, const StepArgs& a);
 int upload_material(kscr_handle* h);
 
 }  // namespace api

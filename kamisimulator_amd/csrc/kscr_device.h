@@ -111,7 +111,7 @@ hipError_t launch_step_fused(const StepArgs& a, const int2* packs, int num_packs
 // Even/odd coloured step (kscr_colored.hip): block threads, two vertices per
 // lane, packs of at most 2 block - 1 vertices; block <= 512.
 hipError_t launch_step_colored(const StepArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream);
-// SagFree initialisation on the host mirror of a groom (kscr_sagfree.hip);
+// SagFree initialisation on the host mirror of a groom (kscr_sagfree.cpp);
 // q and q_rest are (x, y, z, w) rotors, 4 floats each.
 void sag_free_groom(const float* pos, const float* inv_mass, float* l_rest, const float* k_stretch, float* q,
                     float* q_rest, const float gravity[3], const int* roots, int num_roots, int num_verts, int iterations);
@@ -129,6 +129,11 @@ hipError_t launch_step_adjoint(const AdjointArgs& a, const int2* packs, int num_
 // Batched form: packs[p] = (begin, end, groom, 0) index the device table of
 // per-groom arguments.
 hipError_t launch_step_batch(const StepArgs* grooms, const int4* packs, int num_packs, int block, hipStream_t stream);
+// Strain limiting after the SDF pass (kscr_strain.hip): strands
+// [strand_off[s], strand_off[s + 1]), rest[i] the rest length of segment
+// (i, i + 1).
+hipError_t launch_strain_limit(float* x, const int* strand_off, const float* rest, int num_strands,
+                               float strain_error, hipStream_t stream);
 // last_h <- h, time <- time + dt for n grooms of a device table.
 hipError_t launch_batch_clock(StepArgs* grooms, int n, float dt, hipStream_t stream);
 

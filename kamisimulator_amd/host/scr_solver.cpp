@@ -76,6 +76,8 @@ StableCosseratRodsSolver::JsonParams StableCosseratRodsSolver::LoadSettingJsonCo
         for (int k = 0; k < 3; k++) p.Gravity[k] = (float)g[k];
     }
     if (j.CheckKey("SDFThreshold")) p.SDFThreshold = j.GetFloat("SDFThreshold");
+    if (j.CheckKey("StrainError")) p.StrainError = j.GetFloat("StrainError");
+    if (j.CheckKey("UseStrainLimiting")) p.UseStrainLimiting = j.GetBool("UseStrainLimiting");
     if (j.CheckKey("MaxSubstep")) p.MaxSubstep = j.GetFloat("MaxSubstep");
     if (j.CheckKey("Schedule")) {
         const std::string s = j.GetString("Schedule");
@@ -205,6 +207,11 @@ void StableCosseratRodsSolver::build_sim() {
         check(kscr_sag_free(sim_, root_vertices_.data(), (int)root_vertices_.size(), jp_.SagFreeItr), "SagFree");
     check(kscr_upload(sim_), "upload");
     apply_body();
+    // the SDF solver's strain limiting (SDFBodyCollisionSolver.cpp:182-187),
+    // with rest lengths from the positions at construction, as its
+    // InitAdditionalHairData takes them from the Hair (:93-157)
+    if (jp_.EnableSDF && jp_.UseStrainLimiting)
+        check(kscr_set_strain_limit(sim_, 1, jp_.StrainError, nullptr), "set_strain_limit");
 }
 
 // StableCosseratRodsSolver::Construct (StableCosseratRodsSolver.cpp:33-61)

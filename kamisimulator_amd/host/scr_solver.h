@@ -12,8 +12,8 @@
 // (SubstituteBodySdf) or as a triangle mesh whose grid the library builds on
 // the device (SubstituteBodyMesh) instead of a Body object; SolveNextStep
 // runs the SDF pass on the device instead of the host round trip; the
-// settings accept Gravity, SDFThreshold, MaxSubstep, Schedule, BlockSize and
-// DeviceResidentHair (SolveNextStep then leaves the Hair to DownloadHair,
+// settings accept Gravity, SDFThreshold, StrainError, UseStrainLimiting (the
+// SDF solver's keys), MaxSubstep, Schedule, BlockSize and DeviceResidentHair (SolveNextStep then leaves the Hair to DownloadHair,
 // which the driver calls once per draw frame).
 //
 // YBJsonPath: when the setting names an existing YarnBall scene file, the
@@ -89,6 +89,9 @@ class StableCosseratRodsSolver {
         bool HasGravity = false;
         float Gravity[3] = {0.f, -9.8f, 0.f};
         float SDFThreshold = 0.0f;
+        // the SDF solver's strain limiting (SDFBodyCollisionSolver.h:26-27)
+        float StrainError = 0.1f;
+        bool UseStrainLimiting = false;
         float MaxSubstep = 4e-4f;  // Sim::maxH
         int Schedule = KSCR_SCHEDULE_JACOBI;
         int BlockSize = 0;

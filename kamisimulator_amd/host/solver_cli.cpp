@@ -88,13 +88,14 @@ void print_params(const StableCosseratRodsSolver::JsonParams& p) {
         "\"BendK\": %.9g, \"YoungsModule\": %.9g, \"StiffnessDeviationCoeff\": %.9g, \"CurveRadius\": %.9g, "
         "\"Density\": %.9g, \"ResampleLen\": %.9g, \"VelocityDecay\": %.9g, \"ExternalForceMultiplier\": %.9g, "
         "\"WindPeak\": %.9g, \"WindYFreq\": %.9g, \"WindZFreq\": %.9g, \"WindTimeFreq\": %.9g, \"WindSharpness\": %.9g, "
-        "\"HasGravity\": %d, \"Gravity\": [%.9g, %.9g, %.9g], \"SDFThreshold\": %.9g, \"MaxSubstep\": %.9g, "
-        "\"Schedule\": %d, \"BlockSize\": %d, \"DeviceResidentHair\": %d}\n",
+        "\"HasGravity\": %d, \"Gravity\": [%.9g, %.9g, %.9g], \"SDFThreshold\": %.9g, \"StrainError\": %.9g, "
+        "\"UseStrainLimiting\": %d, \"MaxSubstep\": %.9g, \"Schedule\": %d, \"BlockSize\": %d, "
+        "\"DeviceResidentHair\": %d}\n",
         p.YBJsonPath.c_str(), p.EnableResample, p.EnableSDF, p.EnableHairHairColl, p.UseYoungsModule, p.NumItr,
         p.SagFreeItr, p.SubdivCoeff, p.StretchK, p.BendK, p.YoungsModule, p.StiffnessDeviationCoeff, p.CurveRadius,
         p.Density, p.ResampleLen, p.VelocityDecay, p.ExternalForceMultiplier, p.WindPeak, p.WindYFreq, p.WindZFreq,
         p.WindTimeFreq, p.WindSharpness, p.HasGravity, p.Gravity[0], p.Gravity[1], p.Gravity[2], p.SDFThreshold,
-        p.MaxSubstep, p.Schedule, p.BlockSize, p.DeviceResidentHair);
+        p.StrainError, p.UseStrainLimiting, p.MaxSubstep, p.Schedule, p.BlockSize, p.DeviceResidentHair);
 }
 
 void print_refs(const StableCosseratRodsSolver& s) {

@@ -35,6 +35,7 @@ struct Base {
     virtual void set_q(const float* q) = 0;
     virtual void get_rest(float* im, float* lr, float* ks, uint32_t* fl, float* qr) const = 0;
     virtual orc::Sdf& sdf() = 0;
+    virtual void set_strain_limit(int on, float err, const float* rest) = 0;
     virtual void advance(float dt) = 0;
     virtual void solve_next_step(float dt) = 0;
     virtual void set_threads(int t) = 0;
@@ -160,6 +161,15 @@ struct Impl : Base {
         }
     }
     orc::Sdf& sdf() override { return s.sdf; }
+    void set_strain_limit(int on, float err, const float* rest) override {
+        s.strainOn = on != 0;
+        s.strainError = err;
+        if (!on) return;
+        if (rest)
+            s.strainRest.assign(rest, rest + s.n);
+        else
+            s.strain_rest_from_positions();
+    }
     void advance(float dt) override { s.advance(dt); }
     void solve_next_step(float dt) override { s.solve_next_step(dt); }
     void set_threads(int t) override { s.threads = t < 1 ? 1 : t; }
@@ -384,6 +394,12 @@ int oracle_set_sdf(oracle_handle* h, const int* dim, const float* gmin, const fl
 }
 int oracle_enable_sdf(oracle_handle* h, int on) {
     h->b->sdf().enabled = on != 0;
+    return 0;
+}
+// strain limiting after the SDF pass; rest float[n] or null for the
+// current segment lengths
+int oracle_set_strain_limit(oracle_handle* h, int on, float err, const float* rest) {
+    h->b->set_strain_limit(on, err, rest);
     return 0;
 }
 int oracle_advance(oracle_handle* h, float dt) {

@@ -42,6 +42,7 @@ SIGNATURES = {
     "oracle_get_rest": (C.c_int, [_P, _FP, _FP, _FP, _UP, _FP]),
     "oracle_set_sdf": (C.c_int, [_P, _IP, _FP, _FP, _FP, _FP, C.c_float, _FP, _FP]),
     "oracle_enable_sdf": (C.c_int, [_P, C.c_int]),
+    "oracle_set_strain_limit": (C.c_int, [_P, C.c_int, C.c_float, C.POINTER(C.c_float)]),
     "oracle_advance": (C.c_int, [_P, C.c_float]),
     "oracle_solve_next_step": (C.c_int, [_P, C.c_float]),
     "oracle_set_threads": (C.c_int, [_P, C.c_int]),
@@ -241,6 +242,13 @@ class OracleSim:
 
     def enable_sdf(self, on=True):
         self.lib.oracle_enable_sdf(self.h, int(bool(on)))
+
+    def set_strain_limit(self, enable=True, strain_error=0.1, rest_lengths=None):
+        rl = None if rest_lengths is None else np.ascontiguousarray(rest_lengths, np.float32).reshape(-1)
+        if rl is not None and rl.size != self.n:
+            raise ValueError("rest_lengths must hold one value per vertex")
+        self.lib.oracle_set_strain_limit(self.h, int(bool(enable)), float(strain_error),
+                                         None if rl is None else _f(rl))
 
     def advance(self, dt):
         self.lib.oracle_advance(self.h, dt)

@@ -517,6 +517,7 @@ struct Sim {
     // are stored in float exactly as the reference computes them; the dynamic
     // state is in T.
     std::vector<float> pos0;  // positions used by configure (float)
+    std::vector<int> strandOff;  // first vertex of each strand, and n
     std::vector<float> invMassF, lRestF, kStretchF;
     std::vector<uint32_t> flags;
     // Vertex::connectionIndex (YarnBall.h:56): the vertex this one is glued to,
@@ -557,6 +558,7 @@ struct Sim {
         n = nv;
         p = prm;
         pos0.assign(P, P + 3 * (size_t)nv);
+        strandOff.assign(off, off + ns + 1);
         invMassF.assign(nv, 1.f);
         lRestF.assign(nv, 1.f);
         kStretchF.assign(nv, 100.f);
@@ -1282,10 +1284,53 @@ struct Sim {
         }
     }
 
+    // ----------------------------------------------------- strain limiting
+    // StrainLimiting (SDF/cudaSDFCollision.cu:176-199), run by
+    // SDFBodyCollisionSolver::SolveNextStep right after the SDF pass when
+    // UseStrainLimiting is set (SDFBodyCollisionSolver.cpp:182-187): each
+    // strand walked from its root; a segment whose length l departs from its
+    // rest length l0 by more than StrainError * l0 is given length l0 by moving
+    // its upper vertex along the segment (helper_math length and normalize);
+    // velocities are untouched.
+    bool strainOn = false;
+    float strainError = 0.1f;
+    std::vector<float> strainRest;  // rest length of segment (i, i + 1)
+    // Rest lengths of InitAdditionalHairData (SDFBodyCollisionSolver.cpp:
+    // 108-120): glm length of the current segments, float, no FMA.
+    void strain_rest_from_positions() {
+        strainRest.assign(n, 0.f);
+        for (size_t s = 0; s + 1 < strandOff.size(); s++)
+            for (int i = strandOff[s]; i + 1 < strandOff[s + 1]; i++) {
+                const float dx = (float)x[i + 1].x - (float)x[i].x;
+                const float dy = (float)x[i + 1].y - (float)x[i].y;
+                const float dz = (float)x[i + 1].z - (float)x[i].z;
+                strainRest[i] = std::sqrt((dx * dx + dy * dy) + dz * dz);
+            }
+    }
+    void apply_strain_limit() {
+        if (!strainOn || !sdf.enabled) return;
+        const T err = (T)strainError;
+        for (size_t s = 0; s + 1 < strandOff.size(); s++) {
+            V3<T> cur = x[strandOff[s]];
+            for (int i = strandOff[s]; i + 1 < strandOff[s + 1]; i++) {
+                V3<T> nxt = x[i + 1];
+                const T l0 = (T)strainRest[i];
+                const T l = length(nxt - cur);
+                if (std::fabs(l - l0) > err * l0) {
+                    const V3<T> dir = normalize(cur - nxt);
+                    nxt = nxt + (l - l0) * dir;
+                    x[i + 1] = nxt;
+                }
+                cur = nxt;
+            }
+        }
+    }
+
     // StableCosseratRodsSolver::SolveNextStep (StableCosseratRodsSolver.cpp:216-234)
     void solve_next_step(float dt) {
         advance(dt);
         apply_sdf();
+        apply_strain_limit();
     }
 };
 

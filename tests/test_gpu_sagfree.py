@@ -1,4 +1,4 @@
-"""SagFree of the library (kscr_sag_free, host code in kscr_sagfree.hip)
+"""SagFree of the library (kscr_sag_free, host code in kscr_sagfree.cpp)
 against the oracle's restatement (scr_oracle.hpp: sag_free) on identical
 grooms: rest lengths, rest rotors and frames bit for bit, then the steps that
 follow.  Runs with the GPU tests because a library handle owns device memory."""

@@ -28,6 +28,8 @@ def test_defaults_are_the_reference_solver_defaults(tmp_path):
                      WindYFreq=1, WindZFreq=0.7, WindTimeFreq=0.4, WindSharpness=3).items():
         assert d[k] == pytest.approx(v, rel=1e-7), k
     assert d["HasGravity"] == 0 and d["MaxSubstep"] == pytest.approx(4e-4)
+    # the SDF solver's strain limiting (SDFBodyCollisionSolver.h:26-27)
+    assert d["UseStrainLimiting"] == 0 and d["StrainError"] == pytest.approx(0.1, rel=1e-7)
 
 
 def test_every_reference_key_is_read(tmp_path):
@@ -37,7 +39,8 @@ def test_every_reference_key_is_read(tmp_path):
          "StiffnessDeviationCoeff": 0.5, "CurveRadius": 5e-5, "Density": 1.3e6, "ResampleLen": 2e-3,
          "VelocityDecay": 0.1, "ExternalForceMultiplier": 2, "WindPeak": 10, "WindYFreq": 2, "WindZFreq": 0.5,
          "WindTimeFreq": 0.3, "WindSharpness": 4.0, "YBJsonPath": "none.json",
-         "Gravity": [0.0, -9.81, 1.0], "SDFThreshold": 0.002, "MaxSubstep": 2e-4, "Schedule": "colored",
+         "Gravity": [0.0, -9.81, 1.0], "SDFThreshold": 0.002, "StrainError": 0.05, "UseStrainLimiting": True,
+         "MaxSubstep": 2e-4, "Schedule": "colored",
          "BlockSize": 256, "DeviceResidentHair": True, "Unrelated": {"x": [1, "a"]}}
     d = _parse(tmp_path, s)
     for k, v in s.items():
