@@ -123,14 +123,22 @@ def cpu_model():
                 break
     except Exception:
         pass
+    ids = {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping") and k not in ids:
+                ids[k] = v.strip()
+            if line.strip() == "":
+                break
+    except Exception:
+        pass
     if not model:
-        try:
-            for line in open("/proc/cpuinfo"):
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-        except Exception:
-            pass
+        model = ids.get("model name", "")
+    # a virtualised host may report only the product line: add the CPUID numbers
+    if all(k in ids for k in ("cpu family", "model", "stepping")):
+        model += f" (family {ids['cpu family']}, model {ids['model']}, stepping {ids['stepping']})"
     return model, os.cpu_count() or 1
 
 

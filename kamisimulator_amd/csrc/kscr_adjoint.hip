@@ -40,6 +40,23 @@ __device__ inline f4 rot_e1_vjp(f4 q, f3 rb) {
 // a (x) conj(e1) = -(a (x) e1)
 KS_FN f4 rmul_conj_e1(f4 a) { return f4{-a.w, -a.z, a.y, a.x}; }
 
+// Accumulations of the sweep's adjoint-only terms, each component one FMA
+// chain (the oracle's adjoint forms them the same way):
+// acc + s a
+KS_FN f3 fma3(float s, f3 a, f3 acc) { return f3{fmaf(s, a.x, acc.x), fmaf(s, a.y, acc.y), fmaf(s, a.z, acc.z)}; }
+KS_FN f4 fma4(float s, f4 a, f4 acc) {
+    return f4{fmaf(s, a.x, acc.x), fmaf(s, a.y, acc.y), fmaf(s, a.z, acc.z), fmaf(s, a.w, acc.w)};
+}
+// acc + a . b
+KS_FN float dot_acc(f3 a, f3 b, float acc) { return fmaf(a.z, b.z, fmaf(a.y, b.y, fmaf(a.x, b.x, acc))); }
+// acc + (v, 0) (x) b = acc + (b.w v + v x vec(b), -v . vec(b))
+KS_FN f4 rmul_pure_acc(f3 v, f4 b, f4 acc) {
+    return f4{fmaf(b.w, v.x, fmaf(v.y, b.z, fmaf(-b.y, v.z, acc.x))),
+              fmaf(b.w, v.y, fmaf(v.z, b.x, fmaf(-b.z, v.x, acc.y))),
+              fmaf(b.w, v.z, fmaf(v.x, b.y, fmaf(-b.x, v.y, acc.z))),
+              fmaf(-v.z, b.z, fmaf(-v.y, b.y, fmaf(-v.x, b.x, acc.w)))};
+}
+
 __device__ inline f4 ld4(const float4* p, size_t i) { return to4(p[i]); }
 __device__ inline void st4(float4* p, size_t i, f4 a) { p[i] = make_float4(a.x, a.y, a.z, a.w); }
 
@@ -55,15 +72,23 @@ struct Iterate {
 template <int BLOCK>
 struct AdjLds {
     float eb[3][BLOCK];   // d(segment vector e) of the orientation solve
-    float4 qpb[BLOCK];    // contribution to the next vertex's rotor adjoint
-    float4 qmb[BLOCK];    // contribution to the previous vertex's rotor adjoint
-    float4 qrp[BLOCK];    // contribution to the previous vertex's qRest adjoint
-    int mask[BLOCK];      // 1: orientation solved, 2: previous term, 4: next term
+    // contributions to the next vertex's rotor adjoint and to the previous
+    // vertex's rotor and qRest adjoints, before their sign switch phi: the
+    // reader adds phi times them (phi = +-1, so phi times a product is the
+    // product of phi times a factor, exactly)
+    float4 qpb[BLOCK];
+    float4 qmb[BLOCK];
+    float4 qrp[BLOCK];
+    // 1: orientation solved, 2: previous term, 4: next term; 8 / 16: the
+    // previous / next term's phi is -1
+    int mask[BLOCK];
     float4 fbD[BLOCK];    // adjoint of the position solve's force, and of D
     float c[3][BLOCK];    // stretch constraint c of the segment at iterate k
 };
 
-template <int BLOCK>
+// EXT: the step has wind or the SDF pass (their adjoints are compiled in);
+// without them the kernel holds fewer registers.
+template <int BLOCK, bool EXT>
 __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbeg, const int vend, AdjLds<BLOCK>& L) {
     const int t = threadIdx.x;
     const int i = vbeg + t;
@@ -99,6 +124,12 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
     const float ilp = 1.0f / lrp, kkp = ksp * ilp, ddp = kkp * ilp;  // previous segment
     const float sc = (-2.0f * ks) / lr;                              // d(v)/d(e) of the orientation solve
     const float m2lr = -2.0f / lr;
+    // coefficients of the position solve's adjoint terms in kStretch and of
+    // the damping terms
+    const float il2 = il * il;
+    const float k_damp = -(damp * il2), k_diag = (1.0f + damp) * il2;
+    const float nd = -(damp * dd), ndp = -(damp * ddp);
+    const bool unit_ratio = a.acceleration_ratio == 1.0f;  // ratio * dxb == dxb exactly
     const float a_in = 1.0f / (im * hh * hh);
     const float c1 = 1.0f / (hh * hh * im);
     float hd = a_in;
@@ -115,7 +146,7 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
     // ---- the SDF pass after the last substep (non-root vertices), at the
     // state endItr left: x + dx^K and dx^K / h * decay (free) or v (pinned,
     // which initItr's y holds)
-    if (a.sdf_on && act && has_prev) {
+    if (EXT && a.sdf_on && act && has_prev) {
         const size_t s1 = (size_t)(a.num_substeps - 1);
         const f3 dxK = ld3(a.tape.dx + (s1 * (K + 1) * n + (size_t)K * n) * 3, ic);
         f3 xp = ld3(a.tape.x + s1 * n * 3, ic);
@@ -135,16 +166,14 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
         if (free) vb = zero3();
         f3 yb = zero3();
 
-        // Iterates of the tape: k + 1 (dxn, dxn_n) is carried over from the
-        // previous pass of the loop, k is loaded one pass ahead (the loads stay
-        // in flight across the barriers, which wait on LDS traffic only).
+        // Iterates of the tape: k + 1 is carried over from the previous pass
+        // of the loop, k - 1 is loaded one pass ahead (the loads stay in
+        // flight across the barriers, which wait on LDS traffic only).
         const size_t o_sub = (size_t)sub * (K + 1) * n;  // dx^k and q^k: K + 1 per substep
         const size_t w_sub = (size_t)sub * K * n;        // 1 / |w| of iteration k: K per substep
         // Reverse iteration k reads iterate k + 1 (dx, dx_n) and iterate k
         // (dx, dx_n and the rotors) and prefetches iterate k - 1 (iterate 0
-        // again on the last pass).  Three register sets take the three roles
-        // in turn (loop unrolled by three), so nothing is copied between
-        // passes.
+        // again on the last pass).
         const auto load_iterate = [&](int k, Iterate& it) __attribute__((always_inline)) {
             const float* DXk = a.tape.dx + (o_sub + (size_t)k * n) * 3;
             const float4* Qk = a.tape.q + o_sub + (size_t)k * n;
@@ -160,33 +189,31 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             const f4 Q = cu.q, Qm = cu.qm, Qp = cu.qp;
 
             // stretch constraint of segment i at iterate k (the forward's c);
-            // the next vertex reads it for its previous segment
-            f3 c_next = zero3();
-            if (has_next) {
-                c_next = ((xn - x) + (dx_n - dx)) * il - rotate_e1(Q);
-                L.c[0][t] = c_next.x;
-                L.c[1][t] = c_next.y;
-                L.c[2][t] = c_next.z;
-            }
+            // the next vertex reads it for its previous segment (a vertex
+            // without a segment forms a value that nobody reads)
+            const f3 c_next = ((xn - x) + (dx_n - dx)) * il - rotate_e1(Q);
+            L.c[0][t] = c_next.x;
+            L.c[1][t] = c_next.y;
+            L.c[2][t] = c_next.z;
 
             // ---- phase A: orientation solve of segment i (quaternionLambdaItr)
-            f3 eb = zero3();
             int mask = 0;
             if (act && !pass) {
                 mask = 1;
                 const f3 e = (xn - x) + (dxn_n - dxn);
                 const f3 vv = e * sc;
-                f4 b = zero4(), bm = zero4(), bp = zero4();
-                float phim = 0.0f, phip = 0.0f;
+                // b = phi_m Q_{i-1} qRest_{i-1} + phi_p Q_{i+1} conj(qRest_i)
+                f4 b = zero4();
+                float phim = 1.0f, phip = 1.0f;
                 if (has_prev) {
-                    bm = rmul(Qm, qrm);
+                    const f4 bm = rmul(Qm, qrm);
                     phim = dot(Q, bm) > 0 ? 1.0f : -1.0f;
-                    b = b + phim * bm;
+                    b = phim * bm;
                 }
                 if (has_next_ori) {
-                    bp = rmul(Qp, conj(qr));
+                    const f4 bp = rmul(Qp, conj(qr));
                     phip = dot(Q, bp) > 0 ? 1.0f : -1.0f;
-                    b = b + phip * bp;
+                    b = has_prev ? b + phip * bp : phip * bp;
                 }
                 const float lv = sqrt_cr(clamp_sq(dot(vv, vv))), lb = sqrt_cr(clamp_sq(dot(b, b)));
                 const float lambda = lv + lb;
@@ -194,33 +221,35 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
                 // tape (the forward's own values), not from a recomputation
                 const f4 qn = nx.q;
                 const f4 g = qb;
-                const f4 wb = cu.ilw * (g + (-dot(qn, g)) * qn);
-                f4 bb = lambda * wb;
+                const f4 wb = cu.ilw * fma4(-dot(qn, g), qn, g);
                 const float lamb = dot(wb, b);
+                // w = (vv, 0) (x) b (x) e1 + lambda b: mb = wb (x) conj(e1)
                 const f4 mb = rmul_conj_e1(wb);
                 f3 vvb = vec(rmul(mb, conj(b)));
-                bb = bb + rmul_pure(neg(vv), mb);
-                vvb = vvb + (lamb * rcp_cr(lv)) * vv;
-                bb = bb + (lamb * rcp_cr(lb)) * b;
-                // neighbours read these entries only under the mask bits
+                f4 bb = rmul_pure_acc(neg(vv), mb, lambda * wb);
+                // lambda = |v| + |b|
+                vvb = fma3(lamb * rcp_cr(lv), vv, vvb);
+                bb = fma4(lamb * rcp_cr(lb), b, bb);
+                // neighbours read these entries only under the mask bits and
+                // add phi times them
                 if (has_prev) {
-                    const f4 c = phim * bb;
-                    const f4 qmb = rmul(c, conj(qrm));
-                    const f4 qrp = rmul(conj(Qm), c);
+                    const f4 qmb = rmul(bb, conj(qrm));
+                    const f4 qrp = rmul(conj(Qm), bb);
                     L.qmb[t] = make_float4(qmb.x, qmb.y, qmb.z, qmb.w);
                     L.qrp[t] = make_float4(qrp.x, qrp.y, qrp.z, qrp.w);
-                    mask |= 2;
+                    mask |= phim < 0 ? (2 | 8) : 2;
                 }
                 if (has_next_ori) {
-                    const f4 c = phip * bb;
-                    const f4 qpb = rmul(c, qr);
-                    const f4 cqb = rmul(conj(Qp), c);
-                    qrb = qrb + conj(cqb);
+                    const f4 qpb = rmul(bb, qr);
+                    // qrb + conj(phi_p conj(Q_{i+1}) (x) bb)
+                    const f4 cqb = rmul(conj(Qp), bb);
+                    qrb = f4{fmaf(-phip, cqb.x, qrb.x), fmaf(-phip, cqb.y, qrb.y), fmaf(-phip, cqb.z, qrb.z),
+                             fmaf(phip, cqb.w, qrb.w)};
                     L.qpb[t] = make_float4(qpb.x, qpb.y, qpb.z, qpb.w);
-                    mask |= 4;
+                    mask |= phip < 0 ? (4 | 16) : 4;
                 }
-                eb = vvb * sc;
-                ksb += dot(vvb, e) * m2lr;
+                const f3 eb = vvb * sc;
+                ksb = fmaf(dot(vvb, e), m2lr, ksb);
                 L.eb[0][t] = eb.x;
                 L.eb[1][t] = eb.y;
                 L.eb[2][t] = eb.z;
@@ -231,7 +260,7 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             f4 qb2 = zero4();
             if (has_prev) {
                 const int mm = L.mask[t - 1];
-                if ((mm & 4)) qb2 = qb2 + to4(L.qpb[t - 1]);
+                if (mm & 4) qb2 = fma4((mm & 16) ? -1.0f : 1.0f, to4(L.qpb[t - 1]), qb2);
                 if (mm & 1) {
                     const f3 ebm{L.eb[0][t - 1], L.eb[1][t - 1], L.eb[2][t - 1]};
                     dxb = dxb + ebm;
@@ -240,14 +269,16 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             }
             if (act && pass) qb2 = qb2 + qb;
             if (mask & 1) {
+                const f3 eb{L.eb[0][t], L.eb[1][t], L.eb[2][t]};
                 dxb = dxb - eb;
                 xb = xb - eb;
             }
             if (has_next) {
                 const int mn = L.mask[t + 1];
                 if (mn & 2) {
-                    qb2 = qb2 + to4(L.qmb[t + 1]);
-                    qrb = qrb + to4(L.qrp[t + 1]);
+                    const float ph = (mn & 8) ? -1.0f : 1.0f;
+                    qb2 = fma4(ph, to4(L.qmb[t + 1]), qb2);
+                    qrb = fma4(ph, to4(L.qrp[t + 1]), qrb);
                 }
             }
 
@@ -263,44 +294,40 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
                     f = f + ((-kkp) * c - (damp * ddp) * dx);
                 }
                 const f3 delta = inertia_solve(hinv, f);
-                fb = inertia_solve(hinv, a.acceleration_ratio * dxb);
+                fb = inertia_solve(hinv, unit_ratio ? dxb : a.acceleration_ratio * dxb);
                 Db = -dot(fb, delta);
-                yb = yb + c1 * fb;
-                dxb2 = dxb2 - c1 * fb;
+                yb = fma3(c1, fb, yb);
+                dxb2 = fma3(-c1, fb, dxb2);
             }
             L.fbD[t] = make_float4(fb.x, fb.y, fb.z, Db);
             __syncthreads();
 
-            // ---- phase B2: stretch constraint of segment i, and the same
-            // value for segment i - 1 recomputed from the published adjoints
-            // (identical operations, so no third exchange is needed)
-            f3 s_own = zero3(), s_prev = zero3();
-            f4 qrot = zero4();
+            // ---- phase B2: stretch constraint of segment i - 1, recomputed
+            // from the published adjoints with the operations vertex i - 1
+            // uses (no third exchange), and of segment i
+            if (has_prev) {
+                const float4 pb = L.fbD[t - 1];
+                const f3 fbp{pb.x, pb.y, pb.z};
+                const f3 s_prev = (kkp * (fbp - fb)) * ilp;
+                xb = xb + s_prev;
+                dxb2 = dxb2 + s_prev;
+                dxb2 = fma3(ndp, fb, dxb2);
+            }
             if (has_next) {
                 const float4 nb = L.fbD[t + 1];
                 const f3 fbn{nb.x, nb.y, nb.z};
                 const float Dbn = nb.w;
-                const f3 cb = kk * (fb - fbn);
-                s_own = cb * il;
-                qrot = rot_e1_vjp(Q, neg(cb));
-                ksb += dot(c_next, fb - fbn) * il - damp * (dot(dx, fb) + dot(dx_n, fbn)) * il * il +
-                       (Db + Dbn) * (1.0f + damp) * il * il;
-            }
-            if (has_prev) {
-                const float4 pb = L.fbD[t - 1];
-                const f3 fbp{pb.x, pb.y, pb.z};
-                s_prev = (kkp * (fbp - fb)) * ilp;
-            }
-            if (has_prev) {
-                xb = xb + s_prev;
-                dxb2 = dxb2 + s_prev;
-                dxb2 = dxb2 - (damp * ddp) * fb;
-            }
-            if (has_next) {
+                const f3 fd = fb - fbn;
+                const f3 cb = kk * fd;
+                const f3 s_own = cb * il;
+                // kStretch through c's 1 / l, the damping term and the diagonal
+                ksb = fmaf(dot(c_next, fd), il, ksb);
+                ksb = fmaf(k_damp, dot_acc(dx_n, fbn, dot(dx, fb)), ksb);
+                ksb = fmaf(k_diag, Db + Dbn, ksb);
                 xb = xb - s_own;
                 dxb2 = dxb2 - s_own;
-                qb2 = qb2 + qrot;
-                dxb2 = dxb2 - (damp * dd) * fb;
+                qb2 = qb2 + rot_e1_vjp(Q, neg(cb));
+                dxb2 = fma3(nd, fb, dxb2);
             }
             dxb = dxb2;
             qb = qb2;
@@ -312,20 +339,22 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             A.q = ld4(a.tape.q + o_sub + (size_t)K * n, ic);
         }
         if (K > 0) load_iterate(K - 1, B);
-        int k = K - 1;
-        for (; k >= 2; k -= 3) {
+        // The loop is not unrolled: the roles rotate by register copies.
+        // Unrolled by three (no copies) the kernel needs ~210 VGPRs instead
+        // of ~160, and two of its waves then leave no room on a SIMD for the
+        // re-record kernel that runs beside it on the other stream.
+#pragma nounroll
+        for (int k = K - 1; k >= 0; k--) {
             iteration(k, A, B, C);
-            iteration(k - 1, B, C, A);
-            iteration(k - 2, C, A, B);
+            A = B;
+            B = C;
         }
-        if (k >= 0) iteration(k, A, B, C);
-        if (k >= 1) iteration(k - 1, B, C, A);
 
         // ---- initItr: y = x + h v + h^2 ext (free), dx^0 = h v (+ h^2 s ext), u' = v
         f3 vbn = vb + ub;
         vbn = vbn + hh * dxb;
         f3 ubn = zero3();
-        if (act && free && a.wind_peak != 0) {
+        if (EXT && act && free && a.wind_peak != 0) {
             // ext depends on x through the wind: ext = (g + (wind(x), 0, 0)) efm
             vbn = vbn + hh * yb;
             const WindForce w = wind_force<true>(a.wind_peak, a.wind_y_freq, a.wind_z_freq, a.wind_time_freq,
@@ -377,11 +406,11 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, bool EXT>
 __global__ __launch_bounds__(BLOCK) void step_adjoint_kernel(AdjointArgs a, const int2* __restrict__ packs) {
     __shared__ AdjLds<BLOCK> lds;
     const int2 pk = packs[blockIdx.x];
-    adjoint_pack<BLOCK>(a, pk.x, pk.y, lds);
+    adjoint_pack<BLOCK, EXT>(a, pk.x, pk.y, lds);
 }
 
 }  // namespace
@@ -389,10 +418,14 @@ __global__ __launch_bounds__(BLOCK) void step_adjoint_kernel(AdjointArgs a, cons
 hipError_t launch_step_adjoint(const AdjointArgs& a, const int2* packs, int num_packs, int block, hipStream_t stream) {
     if (num_packs <= 0) return hipSuccess;
     const dim3 grid(num_packs);
+    const bool ext = a.wind_peak != 0 || a.sdf_on;
     switch (block) {
-#define KSCR_CASE(B)                                                                          \
-    case B:                                                                                   \
-        hipLaunchKernelGGL(step_adjoint_kernel<B>, grid, dim3(B), 0, stream, a, packs);       \
+#define KSCR_CASE(B)                                                                                  \
+    case B:                                                                                           \
+        if (ext)                                                                                      \
+            hipLaunchKernelGGL((step_adjoint_kernel<B, true>), grid, dim3(B), 0, stream, a, packs);   \
+        else                                                                                          \
+            hipLaunchKernelGGL((step_adjoint_kernel<B, false>), grid, dim3(B), 0, stream, a, packs);  \
         break;
         KSCR_CASE(64)
         KSCR_CASE(128)

@@ -213,8 +213,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves
     run_pack<BLOCK>(a, pk.x, pk.y, lds);
 }
 
+// The record kernel runs beside the reverse sweep (two streams) and is held
+// at the plain step's register budget so that two waves of each fit a SIMD
+// (2 x 80 + 2 x 168 VGPRs); it then spills a few values to scratch.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void step_record_kernel(StepArgs a, TapeArgs tape, const int2* __restrict__ packs) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves(BLOCK)))) void step_record_kernel(
+    StepArgs a, TapeArgs tape, const int2* __restrict__ packs) {
     __shared__ StepLds<BLOCK> lds;
     const int2 pk = packs[blockIdx.x];
     run_pack<BLOCK, true>(a, pk.x, pk.y, lds, &tape);

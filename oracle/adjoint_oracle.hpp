@@ -71,6 +71,29 @@ inline R4<T> rmul_conj_e1(R4<T> a) {
     return R4<T>{-a.w, -a.z, a.y, a.x};
 }
 
+// Accumulations of the sweep's adjoint-only terms, one FMA chain per
+// component (the HIP sweep forms them the same way): acc + s a, acc + a . b,
+// acc + (v, 0) (x) b.
+template <class T>
+inline V3<T> fma3(T s, V3<T> a, V3<T> acc) {
+    return V3<T>{std::fma(s, a.x, acc.x), std::fma(s, a.y, acc.y), std::fma(s, a.z, acc.z)};
+}
+template <class T>
+inline R4<T> fma4(T s, R4<T> a, R4<T> acc) {
+    return R4<T>{std::fma(s, a.x, acc.x), std::fma(s, a.y, acc.y), std::fma(s, a.z, acc.z), std::fma(s, a.w, acc.w)};
+}
+template <class T>
+inline T dot_acc(V3<T> a, V3<T> b, T acc) {
+    return std::fma(a.z, b.z, std::fma(a.y, b.y, std::fma(a.x, b.x, acc)));
+}
+template <class T>
+inline R4<T> rmul_pure_acc(V3<T> v, R4<T> b, R4<T> acc) {
+    return R4<T>{std::fma(b.w, v.x, std::fma(v.y, b.z, std::fma(-b.y, v.z, acc.x))),
+                 std::fma(b.w, v.y, std::fma(v.z, b.x, std::fma(-b.z, v.x, acc.y))),
+                 std::fma(b.w, v.z, std::fma(v.x, b.y, std::fma(-b.x, v.y, acc.z))),
+                 std::fma(-v.z, b.z, std::fma(-v.y, b.y, std::fma(-v.x, b.x, acc.w)))};
+}
+
 // The wind's force along x at position x (init_itr's expressions) and its
 // derivatives with respect to x.y and x.z on the branch x lies on; sin(t) is
 // cos(t - pi / 2) by the same cos as the forward's.
@@ -281,17 +304,18 @@ struct AdjointSim : Sim<T> {
                 const V3<T> e = (t.x[i + 1] - t.x[i]) + (DXn[i + 1] - DXn[i]);
                 const T sc = T(-2) * kStretch[i] / lRest[i];
                 const V3<T> vv = e * sc;
-                R4<T> b{0, 0, 0, 0}, bm{0, 0, 0, 0}, bp{0, 0, 0, 0};
-                T phim = 0, phip = 0;
+                // b = phi_m Q[i-1] qRest[i-1] + phi_p Q[i+1] conj(qRest[i])
+                R4<T> b{0, 0, 0, 0};
+                T phim = 1, phip = 1;
                 if (fl & F_PREV) {
-                    bm = rmul(Q[i - 1], qRest[i - 1]);
+                    const R4<T> bm = rmul(Q[i - 1], qRest[i - 1]);
                     phim = dot4(Q[i], bm) > 0 ? T(1) : T(-1);
-                    b = add4(b, scale4(phim, bm));
+                    b = scale4(phim, bm);
                 }
                 if (fl & F_NEXT_ORI) {
-                    bp = rmul(Q[i + 1], conj(qRest[i]));
+                    const R4<T> bp = rmul(Q[i + 1], conj(qRest[i]));
                     phip = dot4(Q[i], bp) > 0 ? T(1) : T(-1);
-                    b = add4(b, scale4(phip, bp));
+                    b = (fl & F_PREV) ? add4(b, scale4(phip, bp)) : scale4(phip, bp);
                 }
                 // lengths from the squared lengths clamped into [2^-62, 2^62]
                 // (the HIP sweep's definition; no effect on a rod's lengths)
@@ -302,33 +326,33 @@ struct AdjointSim : Sim<T> {
                 const R4<T> qn = t.q[k + 1][i];
                 const T ilw = forward_inv_norm(t, k, i);
                 const R4<T> g = adj.qb[i];
-                const R4<T> wb = scale4(ilw, add4(g, scale4(-dot4(qn, g), qn)));
-                // w = t + lambda b
-                R4<T> bb = scale4(lambda, wb);
+                const R4<T> wb = scale4(ilw, fma4(-dot4(qn, g), qn, g));
                 const T lamb = dot4(wb, b);
-                // t = m (x) e1: mbar = wbar (x) conj(e1)
+                // w = t + lambda b, t = m (x) e1: mbar = wbar (x) conj(e1)
                 const R4<T> mb = rmul_conj_e1(wb);
-                // m = (vv, 0) (x) b: vvbar = vec(mbar (x) conj(b)), bbar += (-vv, 0) (x) mbar
+                // m = (vv, 0) (x) b: vvbar = vec(mbar (x) conj(b)), bbar = lambda wbar + (-vv, 0) (x) mbar
                 V3<T> vvb = rmul(mb, conj(b)).q();
-                bb = add4(bb, rmul_pure(-vv, mb));
+                R4<T> bb = rmul_pure_acc(-vv, mb, scale4(lambda, wb));
                 // lambda = |v| + |b|
-                vvb = vvb + (lamb * (T(1) / lv)) * vv;
-                bb = add4(bb, scale4(lamb * (T(1) / lb), b));
-                // b terms
-                // bm = Q[i-1] (x) qRest[i-1], bp = Q[i+1] (x) conj(qRest[i])
+                vvb = fma3(lamb * (T(1) / lv), vv, vvb);
+                bb = fma4(lamb * (T(1) / lb), b, bb);
+                // b terms, bm = Q[i-1] (x) qRest[i-1], bp = Q[i+1] (x) conj(qRest[i]):
+                // phi times the product with bb (phi = +-1 commutes with the
+                // rounding, so this is the product with phi bb)
                 if (fl & F_PREV) {
-                    const R4<T> c = scale4(phim, bb);
-                    qb2[i - 1] = add4(qb2[i - 1], rmul(c, conj(qRest[i - 1])));
-                    adj.qrb[i - 1] = add4(adj.qrb[i - 1], rmul(conj(Q[i - 1]), c));
+                    qb2[i - 1] = fma4(phim, rmul(bb, conj(qRest[i - 1])), qb2[i - 1]);
+                    adj.qrb[i - 1] = fma4(phim, rmul(conj(Q[i - 1]), bb), adj.qrb[i - 1]);
                 }
                 if (fl & F_NEXT_ORI) {
-                    const R4<T> c = scale4(phip, bb);
-                    qb2[i + 1] = add4(qb2[i + 1], rmul(c, qRest[i]));
-                    adj.qrb[i] = add4(adj.qrb[i], conj(rmul(conj(Q[i + 1]), c)));
+                    qb2[i + 1] = fma4(phip, rmul(bb, qRest[i]), qb2[i + 1]);
+                    const R4<T> cqb = rmul(conj(Q[i + 1]), bb);
+                    const R4<T> r = adj.qrb[i];
+                    adj.qrb[i] = R4<T>{std::fma(-phip, cqb.x, r.x), std::fma(-phip, cqb.y, r.y),
+                                       std::fma(-phip, cqb.z, r.z), std::fma(phip, cqb.w, r.w)};
                 }
                 // v = e * sc
                 const V3<T> eb = vvb * sc;
-                adj.ksb[i] += dot(vvb, e) * (T(-2) / lRest[i]);
+                adj.ksb[i] = std::fma(dot(vvb, e), T(-2) / lRest[i], adj.ksb[i]);
                 dxb[i + 1] = dxb[i + 1] + eb;
                 dxb[i] = dxb[i] - eb;
                 adj.xb[i + 1] = adj.xb[i + 1] + eb;
@@ -365,8 +389,9 @@ struct AdjointSim : Sim<T> {
                 const V3<T> delta = sym_solve(H, f);
                 fb[i] = sym_solve(H, T(p.acceleration_ratio) * dxb[i]);
                 Db[i] = -dot(fb[i], delta);  // D enters the diagonal of H: Hbar = -fbar delta^T
-                yb[i] = yb[i] + (T(1) / (hh * hh * invMass[i])) * fb[i];
-                dxb2[i] = dxb2[i] - (T(1) / (hh * hh * invMass[i])) * fb[i];
+                const T c1 = T(1) / (hh * hh * invMass[i]);
+                yb[i] = fma3(c1, fb[i], yb[i]);
+                dxb2[i] = fma3(-c1, fb[i], dxb2[i]);
             }
             for (int j = 0; j < n; j++) {
                 if (!(flags[j] & F_NEXT)) continue;
@@ -381,10 +406,15 @@ struct AdjointSim : Sim<T> {
                 dxb2[j] = dxb2[j] - s;
                 const R4<T> qrot = rot_e1_vjp(Q[j], -cb);
                 qb2[j] = add4(qb2[j], qrot);
-                dxb2[j] = dxb2[j] - (damp * dd) * fb[j];
-                dxb2[j + 1] = dxb2[j + 1] - (damp * dd) * fb[j + 1];
-                adj.ksb[j] += dot(c, fb[j] - fb[j + 1]) * il - damp * (dot(DX[j], fb[j]) + dot(DX[j + 1], fb[j + 1])) * il * il +
-                              (Db[j] + Db[j + 1]) * (1 + damp) * il * il;
+                const T nd = -(damp * dd);
+                dxb2[j] = fma3(nd, fb[j], dxb2[j]);
+                dxb2[j + 1] = fma3(nd, fb[j + 1], dxb2[j + 1]);
+                // kStretch through c's 1 / l, the damping term and the diagonal:
+                // dot(c, fb_j - fb_j+1) il - damp il^2 (DX_j fb_j + DX_j+1 fb_j+1) + (1 + damp) il^2 (D_j + D_j+1)
+                const T il2 = il * il;
+                adj.ksb[j] = std::fma(dot(c, fb[j] - fb[j + 1]), il, adj.ksb[j]);
+                adj.ksb[j] = std::fma(-(damp * il2), dot_acc(DX[j + 1], fb[j + 1], dot(DX[j], fb[j])), adj.ksb[j]);
+                adj.ksb[j] = std::fma((T(1) + damp) * il2, Db[j] + Db[j + 1], adj.ksb[j]);
             }
             dxb.swap(dxb2);
             adj.qb.swap(qb2);
