@@ -22,6 +22,10 @@ constexpr int kScanChunk = 1024;
 constexpr int kQueryBlock = 128;
 constexpr int kQueryStack = 64;
 constexpr int kQueryStackFirst = 12;  // first pass; deeper queries are redone
+#ifndef KSCR_QUERY2_STACK
+#define KSCR_QUERY2_STACK 8
+#endif
+constexpr int kQueryStackFirst2 = KSCR_QUERY2_STACK;  // entries per query of the two-query first pass
 constexpr unsigned kRedoBlocks = 512;   // grid of the redo pass (grid-stride)
 
 __device__ inline uint32_t expand_bits(uint32_t v) {
@@ -728,6 +732,147 @@ __global__ __launch_bounds__(kQueryBlock) void query_kernel(const float* __restr
     if (fill > 0) flush(fill);
 }
 
+// The first pass over large trees: two queries per lane, the sorted objects
+// base + lane and base + 64 + lane of the wave's 128.  A query's walk is a
+// chain of dependent node loads; the two walks advance in lockstep with both
+// node loads issued before either is used, so a lane keeps two loads in
+// flight.  Each query walks, reports and stops exactly as query_kernel's
+// first pass does (same order, same pairs, same redo entry), so the redo
+// pass is unchanged.
+template <int STACK>
+__global__ __launch_bounds__(kQueryBlock) void query2_kernel(const float* __restrict__ aabb,
+                                                             const uint32_t* __restrict__ ids,
+                                                             const LbvhNode* __restrict__ nodes, int n,
+                                                             int2* __restrict__ pairs, int cap, int min_gap,
+                                                             int* __restrict__ count, int* __restrict__ error,
+                                                             int2* __restrict__ redo, int* __restrict__ redo_count,
+                                                             int redo_cap) {
+    __shared__ uint32_t stack[STACK][2 * kQueryBlock];  // a column per query
+    __shared__ int2 wbuf[kQueryWaves][kWaveBuf];
+    __shared__ int wfill[kQueryWaves];
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lane_below = (1ull << lane) - 1;
+    if (lane == 0) wfill[w] = 0;
+    __builtin_amdgcn_wave_barrier();
+    const int base = blockIdx.x * (2 * kQueryBlock) + w * 128;
+    if (n < 2 || base + lane >= n) return;  // neither of the lane's queries exists
+
+    auto flush = [&](int fill) {
+        const uint64_t act = __ballot(1);
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        int b = 0;
+        if (lane == leader) b = atomicAdd(count, fill);
+        b = __shfl(b, leader);
+        const int nact = __popcll(act);
+        for (int k = __popcll(act & lane_below); k < fill; k += nact)
+            if (b + k < cap) pairs[b + k] = wbuf[w][k];
+    };
+    // appends the pass's pairs (f0, qid), (f1, qid) of the active lanes to the
+    // wave's buffer in lane order
+    auto append = [&](int f0, int f1, int qid) {
+        const uint64_t m0 = __ballot(f0 >= 0), m1 = __ballot(f1 >= 0);
+        const int found = __popcll(m0) + __popcll(m1);
+        if (!found) return;
+        int fill = wfill[w];
+        if (fill + found > kWaveBuf) {
+            flush(fill);
+            fill = 0;
+        }
+        if (f0 >= 0) wbuf[w][fill + __popcll(m0 & lane_below)] = make_int2(f0, qid);
+        if (f1 >= 0) wbuf[w][fill + __popcll(m0) + __popcll(m1 & lane_below)] = make_int2(f1, qid);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == __ffsll((unsigned long long)(m0 | m1)) - 1) wfill[w] = fill + found;
+        __builtin_amdgcn_wave_barrier();
+    };
+    constexpr uint32_t kLeaf = 0x80000000u, kNone = 0xFFFFFFFFu;
+    struct Walk {
+        int t, qid, sp, seen;
+        uint32_t node;  // kNone: finished (or no query)
+        float q[6];
+    };
+    auto start = [&](Walk& s, int t) {
+        s.t = t;
+        s.sp = 0;
+        s.seen = 0;
+        s.qid = 0;
+        s.node = kNone;
+        if (t >= n) return;
+        s.qid = (int)ids[t];
+        for (int k = 0; k < 6; k++) s.q[k] = aabb[6 * (size_t)s.qid + k];
+        if (s.q[0] <= s.q[3]) s.node = 0;  // an empty box (strand tip) intersects nothing
+    };
+    // one pass of a walk over the node (w0..w3): the pairs it reports and its
+    // next node
+    auto pass = [&](Walk& s, int col, float4 w0, float4 w1, float4 w2, float4 w3, int& f0, int& f1) {
+        f0 = f1 = -1;
+        if (s.node == kNone) return;
+        auto leaf_pair = [&](uint32_t leaf) -> int {
+            if ((int)leaf <= s.t) return -1;
+            const int fid = (int)ids[leaf];
+            if (min_gap >= 0 && abs(fid - s.qid) <= min_gap) return -1;
+            return fid;
+        };
+        uint32_t next = kNone;
+        const uint32_t fence = __float_as_uint(w0.w);
+        bool overflow = false;
+        if (max(s.node, fence) > (uint32_t)s.t) {
+            const uint32_t lc = __float_as_uint(w0.y), rc = __float_as_uint(w0.z);
+            const float b0[6] = {w1.x, w1.y, w1.z, w1.w, w2.x, w2.y};
+            const float b1[6] = {w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+            bool h0 = intersects(b0, s.q), h1 = intersects(b1, s.q);
+            if (h0 && (lc & kLeaf)) {
+                f0 = leaf_pair(lc & ~kLeaf);
+                h0 = false;
+            }
+            if (h1 && (rc & kLeaf)) {
+                f1 = leaf_pair(rc & ~kLeaf);
+                h1 = false;
+            }
+            if (h0 && h1) {
+                if (s.sp + 1 > STACK) {  // deeper than the LDS stack: the redo pass takes it
+                    overflow = true;
+                } else {
+                    stack[s.sp++][col] = rc;
+                    next = lc;
+                }
+            } else if (h0) {
+                next = lc;
+            } else if (h1) {
+                next = rc;
+            }
+        }
+        if (!overflow && next == kNone && s.sp > 0) next = stack[--s.sp][col];
+        s.seen += (f0 >= 0) + (f1 >= 0);
+        if (overflow) {
+            const int slot = atomicAdd(redo_count, 1);
+            if (slot < redo_cap)
+                redo[slot] = make_int2(s.t, s.seen);
+            else if (error)
+                atomicOr(error, 4);
+        }
+        s.node = overflow ? kNone : next;
+    };
+    Walk A, B;
+    start(A, base + lane);
+    start(B, base + 64 + lane);
+    while (A.node != kNone || B.node != kNone) {
+        // both node loads first (a finished walk loads the root, which stays cached)
+        const float4* pa = reinterpret_cast<const float4*>(nodes + (A.node != kNone ? A.node : 0u));
+        const float4* pb = reinterpret_cast<const float4*>(nodes + (B.node != kNone ? B.node : 0u));
+        const float4 a0 = pa[0], a1 = pa[1], a2 = pa[2], a3 = pa[3];
+        const float4 c0 = pb[0], c1 = pb[1], c2 = pb[2], c3 = pb[3];
+        int fa0, fa1, fb0, fb1;
+        pass(A, threadIdx.x, a0, a1, a2, a3, fa0, fa1);
+        pass(B, threadIdx.x + kQueryBlock, c0, c1, c2, c3, fb0, fb1);
+        append(fa0, fa1, A.qid);
+        append(fb0, fb1, B.qid);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int fill = wfill[w];
+    if (fill > 0) flush(fill);
+}
+
 __global__ void fill_i32_kernel(int* p, int value, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
@@ -961,12 +1106,12 @@ hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error,
                            d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
         return hipGetLastError();
     }
-    // A 12-entry LDS stack (8 KB per 128-lane workgroup: 8 waves per SIMD;
-    // 16 entries left room for 7) serves nearly every query; the few that
-    // need more are redone with the full stack (the redo pass returns at
-    // once when none was listed).
-    hipLaunchKernelGGL((query_kernel<kQueryStackFirst, false>), grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids,
-                       d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
+    // The first pass walks two queries per lane with a kQueryStackFirst2-entry
+    // LDS stack each; the few queries that need more are redone with the full
+    // stack (the redo pass returns at once when none was listed).
+    const dim3 grid2((d.n + 2 * kQueryBlock - 1) / (2 * kQueryBlock));
+    hipLaunchKernelGGL((query2_kernel<kQueryStackFirst2>), grid2, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes,
+                       d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
     const dim3 rgrid(std::min<unsigned>(grid.x, kRedoBlocks));
     hipLaunchKernelGGL((query_kernel<kQueryStack, true>), rgrid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes,
                        d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);

@@ -14,7 +14,7 @@ from tests.lbvh_cases import query_stack_need, segment_boxes, recipe_points
 pytestmark = pytest.mark.gpu
 
 
-def _compare(boxes, label):
+def _compare(boxes, label, cap=None):
     print(f"[gpu-lbvh] {label}: n={len(boxes)}", flush=True)
     g = Lbvh(device=0)
     t0 = time.perf_counter()
@@ -27,7 +27,7 @@ def _compare(boxes, label):
     for k in ("bounds", "root"):
         assert np.array_equal(tg[k].view(np.uint32), to[k].view(np.uint32)), k
     assert tg["max_stack"] == to["max_stack"]
-    cap = 200 * len(boxes)
+    cap = 200 * len(boxes) if cap is None else cap
     gp, cnt = g.query(cap)
     op = o.query(cap)
     assert cnt == len(op) and cnt <= cap
@@ -74,11 +74,34 @@ def test_ties_and_one_object():
     assert cnt == 0
 
 
+def test_two_pass_query_above_2_20_objects():
+    """Above 2^20 objects the query is two passes: a first pass walking two
+    queries per lane with a short LDS stack, and a redo pass that finishes
+    the queries needing more.  Boxes spanning the whole scene overlap every
+    node on their way down, so their queries need a stack as deep as the tree
+    and go to the redo pass; the pair set must still be the restatement's,
+    with no pair lost or repeated."""
+    rng = np.random.default_rng(13)
+    n = (1 << 20) + 150_000
+    pts = rng.uniform(0, 1, size=(n, 3)).astype(np.float32)
+    small = np.concatenate([pts - 2e-3, pts + 2e-3], axis=1).astype(np.float32)
+    big = np.array([[-0.5, -0.5, -0.5, 1.5, 1.5, 1.5],
+                    [-1.0, -1.0, -1.0, 1.1, 1.1, 1.1],
+                    [-0.1, -0.1, -0.1, 2.0, 2.0, 2.0]], np.float32)
+    boxes = np.concatenate([small[:300_000], big[:1], small[300_000:700_000], big[1:], small[700_000:]])
+    g, o = _compare(boxes, "two-pass query", cap=8_000_000)
+    tree = o.get()
+    pos = {int(obj): k for k, obj in enumerate(tree["ids"])}
+    need = query_stack_need(tree, boxes[300_000], pos[300_000])
+    print(f"[gpu-lbvh] stack needed by a whole-scene query: {need}", flush=True)
+    assert need > 12  # deeper than the first pass's stack: the redo pass ran
+
+
 def test_queries_deeper_than_the_first_pass_stack():
     """Boxes spanning the whole scene overlap every node on their way down,
-    so their queries need a stack as deep as the tree and are finished by the
-    redo pass (the first pass holds 16 entries); the pair set must still be
-    the restatement's, with no pair lost or repeated."""
+    so their queries need a stack as deep as the tree; at 300k objects the
+    query is one pass with the full stack.  The pair set must still be the
+    restatement's, with no pair lost or repeated."""
     rng = np.random.default_rng(12)
     pts = rng.uniform(0, 1, size=(300_000, 3)).astype(np.float32)
     small = np.concatenate([pts - 2e-3, pts + 2e-3], axis=1).astype(np.float32)
