@@ -49,6 +49,9 @@ KS_FN f4 fma4(float s, f4 a, f4 acc) {
 }
 // acc + a . b
 KS_FN float dot_acc(f3 a, f3 b, float acc) { return fmaf(a.z, b.z, fmaf(a.y, b.y, fmaf(a.x, b.x, acc))); }
+// c ? a : b per component
+KS_FN f3 sel3(bool c, f3 a, f3 b) { return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+KS_FN f4 sel4(bool c, f4 a, f4 b) { return f4{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w}; }
 // acc + (v, 0) (x) b = acc + (b.w v + v x vec(b), -v . vec(b))
 KS_FN f4 rmul_pure_acc(f3 v, f4 b, f4 acc) {
     return f4{fmaf(b.w, v.x, fmaf(v.y, b.z, fmaf(-b.y, v.z, acc.x))),
@@ -196,139 +199,117 @@ __device__ __forceinline__ void adjoint_pack(const AdjointArgs& a, const int vbe
             L.c[1][t] = c_next.y;
             L.c[2][t] = c_next.z;
 
+            // The iteration has no branches: every lane forms every term from
+            // valid inputs (tape indices and neighbour lanes clamped into
+            // range) and keeps or drops it with a select, which leaves the
+            // scheduler one block to interleave; the kept values are those of
+            // the term-by-term conditions below.
+            const int tm = t > 0 ? t - 1 : 0, tp = t < BLOCK - 1 ? t + 1 : t;
+
             // ---- phase A: orientation solve of segment i (quaternionLambdaItr)
-            int mask = 0;
-            if (act && !pass) {
-                mask = 1;
-                const f3 e = (xn - x) + (dxn_n - dxn);
-                const f3 vv = e * sc;
-                // b = phi_m Q_{i-1} qRest_{i-1} + phi_p Q_{i+1} conj(qRest_i)
-                f4 b = zero4();
-                float phim = 1.0f, phip = 1.0f;
-                if (has_prev) {
-                    const f4 bm = rmul(Qm, qrm);
-                    phim = dot(Q, bm) > 0 ? 1.0f : -1.0f;
-                    b = phim * bm;
-                }
-                if (has_next_ori) {
-                    const f4 bp = rmul(Qp, conj(qr));
-                    phip = dot(Q, bp) > 0 ? 1.0f : -1.0f;
-                    b = has_prev ? b + phip * bp : phip * bp;
-                }
-                const float lv = sqrt_cr(clamp_sq(dot(vv, vv))), lb = sqrt_cr(clamp_sq(dot(b, b)));
-                const float lambda = lv + lb;
-                // the solve's result q^{k+1} = w / |w| and 1 / |w| come from the
-                // tape (the forward's own values), not from a recomputation
-                const f4 qn = nx.q;
-                const f4 g = qb;
-                const f4 wb = cu.ilw * fma4(-dot(qn, g), qn, g);
-                const float lamb = dot(wb, b);
-                // w = (vv, 0) (x) b (x) e1 + lambda b: mb = wb (x) conj(e1)
-                const f4 mb = rmul_conj_e1(wb);
-                f3 vvb = vec(rmul(mb, conj(b)));
-                f4 bb = rmul_pure_acc(neg(vv), mb, lambda * wb);
-                // lambda = |v| + |b|
-                vvb = fma3(lamb * rcp_cr(lv), vv, vvb);
-                bb = fma4(lamb * rcp_cr(lb), b, bb);
-                // neighbours read these entries only under the mask bits and
-                // add phi times them
-                if (has_prev) {
-                    const f4 qmb = rmul(bb, conj(qrm));
-                    const f4 qrp = rmul(conj(Qm), bb);
-                    L.qmb[t] = make_float4(qmb.x, qmb.y, qmb.z, qmb.w);
-                    L.qrp[t] = make_float4(qrp.x, qrp.y, qrp.z, qrp.w);
-                    mask |= phim < 0 ? (2 | 8) : 2;
-                }
-                if (has_next_ori) {
-                    const f4 qpb = rmul(bb, qr);
-                    // qrb + conj(phi_p conj(Q_{i+1}) (x) bb)
-                    const f4 cqb = rmul(conj(Qp), bb);
-                    qrb = f4{fmaf(-phip, cqb.x, qrb.x), fmaf(-phip, cqb.y, qrb.y), fmaf(-phip, cqb.z, qrb.z),
-                             fmaf(phip, cqb.w, qrb.w)};
-                    L.qpb[t] = make_float4(qpb.x, qpb.y, qpb.z, qpb.w);
-                    mask |= phip < 0 ? (4 | 16) : 4;
-                }
-                const f3 eb = vvb * sc;
-                ksb = fmaf(dot(vvb, e), m2lr, ksb);
-                L.eb[0][t] = eb.x;
-                L.eb[1][t] = eb.y;
-                L.eb[2][t] = eb.z;
-            }
+            const bool solve = act && !pass;
+            const f3 e = (xn - x) + (dxn_n - dxn);
+            const f3 vv = e * sc;
+            // b = phi_m Q_{i-1} qRest_{i-1} + phi_p Q_{i+1} conj(qRest_i)
+            const f4 bm = rmul(Qm, qrm);
+            const f4 bp = rmul(Qp, conj(qr));
+            const float phim = dot(Q, bm) > 0 ? 1.0f : -1.0f;
+            const float phip = dot(Q, bp) > 0 ? 1.0f : -1.0f;
+            const f4 bms = phim * bm, bps = phip * bp;
+            const f4 b = has_prev ? sel4(has_next_ori, bms + bps, bms) : sel4(has_next_ori, bps, zero4());
+            const float lv = sqrt_cr(clamp_sq(dot(vv, vv))), lb = sqrt_cr(clamp_sq(dot(b, b)));
+            const float lambda = lv + lb;
+            // the solve's result q^{k+1} = w / |w| and 1 / |w| come from the
+            // tape (the forward's own values), not from a recomputation
+            const f4 qn = nx.q;
+            const f4 g = qb;
+            const f4 wb = cu.ilw * fma4(-dot(qn, g), qn, g);
+            const float lamb = dot(wb, b);
+            // w = (vv, 0) (x) b (x) e1 + lambda b: mb = wb (x) conj(e1)
+            const f4 mb = rmul_conj_e1(wb);
+            f3 vvb = vec(rmul(mb, conj(b)));
+            f4 bb = rmul_pure_acc(neg(vv), mb, lambda * wb);
+            // lambda = |v| + |b|
+            vvb = fma3(lamb * rcp_cr(lv), vv, vvb);
+            bb = fma4(lamb * rcp_cr(lb), b, bb);
+            // the neighbours' terms, before their sign switch: they read them
+            // only under the mask bits and add phi times them
+            const f4 qmb = rmul(bb, conj(qrm));
+            const f4 qrp = rmul(conj(Qm), bb);
+            const f4 qpb = rmul(bb, qr);
+            // qrb + conj(phi_p conj(Q_{i+1}) (x) bb)
+            const f4 cqb = rmul(conj(Qp), bb);
+            qrb = sel4(solve && has_next_ori,
+                       f4{fmaf(-phip, cqb.x, qrb.x), fmaf(-phip, cqb.y, qrb.y), fmaf(-phip, cqb.z, qrb.z),
+                          fmaf(phip, cqb.w, qrb.w)},
+                       qrb);
+            const f3 eb = vvb * sc;
+            ksb = solve ? fmaf(dot(vvb, e), m2lr, ksb) : ksb;
+            const int mask = solve ? (1 | (has_prev ? (phim < 0 ? (2 | 8) : 2) : 0) |
+                                      (has_next_ori ? (phip < 0 ? (4 | 16) : 4) : 0))
+                                   : 0;
+            L.qmb[t] = make_float4(qmb.x, qmb.y, qmb.z, qmb.w);
+            L.qrp[t] = make_float4(qrp.x, qrp.y, qrp.z, qrp.w);
+            L.qpb[t] = make_float4(qpb.x, qpb.y, qpb.z, qpb.w);
+            L.eb[0][t] = eb.x;
+            L.eb[1][t] = eb.y;
+            L.eb[2][t] = eb.z;
             L.mask[t] = mask;
             __syncthreads();
             // gather, in the order the sequential sweep adds the terms
+            const int mm = L.mask[tm], mn = L.mask[tp];
+            const f4 qpbm = to4(L.qpb[tm]);
+            const f3 ebm{L.eb[0][tm], L.eb[1][tm], L.eb[2][tm]};
+            const f4 qmbn = to4(L.qmb[tp]), qrpn = to4(L.qrp[tp]);
             f4 qb2 = zero4();
-            if (has_prev) {
-                const int mm = L.mask[t - 1];
-                if (mm & 4) qb2 = fma4((mm & 16) ? -1.0f : 1.0f, to4(L.qpb[t - 1]), qb2);
-                if (mm & 1) {
-                    const f3 ebm{L.eb[0][t - 1], L.eb[1][t - 1], L.eb[2][t - 1]};
-                    dxb = dxb + ebm;
-                    xb = xb + ebm;
-                }
-            }
-            if (act && pass) qb2 = qb2 + qb;
-            if (mask & 1) {
-                const f3 eb{L.eb[0][t], L.eb[1][t], L.eb[2][t]};
-                dxb = dxb - eb;
-                xb = xb - eb;
-            }
-            if (has_next) {
-                const int mn = L.mask[t + 1];
-                if (mn & 2) {
-                    const float ph = (mn & 8) ? -1.0f : 1.0f;
-                    qb2 = fma4(ph, to4(L.qmb[t + 1]), qb2);
-                    qrb = fma4(ph, to4(L.qrp[t + 1]), qrb);
-                }
-            }
+            qb2 = sel4(has_prev && (mm & 4), fma4((mm & 16) ? -1.0f : 1.0f, qpbm, qb2), qb2);
+            const bool from_prev = has_prev && (mm & 1);
+            dxb = sel3(from_prev, dxb + ebm, dxb);
+            xb = sel3(from_prev, xb + ebm, xb);
+            qb2 = sel4(act && pass, qb2 + qb, qb2);
+            dxb = sel3(solve, dxb - eb, dxb);
+            xb = sel3(solve, xb - eb, xb);
+            const bool from_next = has_next && (mn & 2);
+            const float ph = (mn & 8) ? -1.0f : 1.0f;
+            qb2 = sel4(from_next, fma4(ph, qmbn, qb2), qb2);
+            qrb = sel4(from_next, fma4(ph, qrpn, qrb), qrb);
 
             // ---- phase B1: position solve of vertex i (cosseratItr)
-            f3 dxb2 = dxb;
-            f3 fb = zero3();
-            float Db = 0.0f;
-            if (act && free) {
-                f3 f = c1 * (y - dx);
-                if (has_next) f = f + (kk * c_next - (damp * dd) * dx);
-                if (has_prev) {
-                    const f3 c{L.c[0][t - 1], L.c[1][t - 1], L.c[2][t - 1]};
-                    f = f + ((-kkp) * c - (damp * ddp) * dx);
-                }
-                const f3 delta = inertia_solve(hinv, f);
-                fb = inertia_solve(hinv, unit_ratio ? dxb : a.acceleration_ratio * dxb);
-                Db = -dot(fb, delta);
-                yb = fma3(c1, fb, yb);
-                dxb2 = fma3(-c1, fb, dxb2);
-            }
+            const bool moves = act && free;
+            const f3 cprev{L.c[0][tm], L.c[1][tm], L.c[2][tm]};
+            f3 f = c1 * (y - dx);
+            f = sel3(has_next, f + (kk * c_next - (damp * dd) * dx), f);
+            f = sel3(has_prev, f + ((-kkp) * cprev - (damp * ddp) * dx), f);
+            const f3 delta = inertia_solve(hinv, f);
+            const f3 fbm = inertia_solve(hinv, unit_ratio ? dxb : a.acceleration_ratio * dxb);
+            const f3 fb = sel3(moves, fbm, zero3());
+            const float Db = moves ? -dot(fbm, delta) : 0.0f;
+            yb = sel3(moves, fma3(c1, fbm, yb), yb);
+            f3 dxb2 = sel3(moves, fma3(-c1, fbm, dxb), dxb);
             L.fbD[t] = make_float4(fb.x, fb.y, fb.z, Db);
             __syncthreads();
 
             // ---- phase B2: stretch constraint of segment i - 1, recomputed
             // from the published adjoints with the operations vertex i - 1
             // uses (no third exchange), and of segment i
-            if (has_prev) {
-                const float4 pb = L.fbD[t - 1];
-                const f3 fbp{pb.x, pb.y, pb.z};
-                const f3 s_prev = (kkp * (fbp - fb)) * ilp;
-                xb = xb + s_prev;
-                dxb2 = dxb2 + s_prev;
-                dxb2 = fma3(ndp, fb, dxb2);
-            }
-            if (has_next) {
-                const float4 nb = L.fbD[t + 1];
-                const f3 fbn{nb.x, nb.y, nb.z};
-                const float Dbn = nb.w;
-                const f3 fd = fb - fbn;
-                const f3 cb = kk * fd;
-                const f3 s_own = cb * il;
-                // kStretch through c's 1 / l, the damping term and the diagonal
-                ksb = fmaf(dot(c_next, fd), il, ksb);
-                ksb = fmaf(k_damp, dot_acc(dx_n, fbn, dot(dx, fb)), ksb);
-                ksb = fmaf(k_diag, Db + Dbn, ksb);
-                xb = xb - s_own;
-                dxb2 = dxb2 - s_own;
-                qb2 = qb2 + rot_e1_vjp(Q, neg(cb));
-                dxb2 = fma3(nd, fb, dxb2);
-            }
+            const float4 pb = L.fbD[tm], nb = L.fbD[tp];
+            const f3 fbp{pb.x, pb.y, pb.z};
+            const f3 fbn{nb.x, nb.y, nb.z};
+            const float Dbn = nb.w;
+            const f3 s_prev = (kkp * (fbp - fb)) * ilp;
+            xb = sel3(has_prev, xb + s_prev, xb);
+            dxb2 = sel3(has_prev, fma3(ndp, fb, dxb2 + s_prev), dxb2);
+            const f3 fd = fb - fbn;
+            const f3 cb = kk * fd;
+            const f3 s_own = cb * il;
+            // kStretch through c's 1 / l, the damping term and the diagonal
+            float ksn = fmaf(dot(c_next, fd), il, ksb);
+            ksn = fmaf(k_damp, dot_acc(dx_n, fbn, dot(dx, fb)), ksn);
+            ksn = fmaf(k_diag, Db + Dbn, ksn);
+            ksb = has_next ? ksn : ksb;
+            xb = sel3(has_next, xb - s_own, xb);
+            qb2 = sel4(has_next, qb2 + rot_e1_vjp(Q, neg(cb)), qb2);
+            dxb2 = sel3(has_next, fma3(nd, fb, dxb2 - s_own), dxb2);
             dxb = dxb2;
             qb = qb2;
         };

@@ -15,9 +15,14 @@ struct kscr_lbvh {
     LbvhDev d;  // its counter block holds the query's error word at d.count[2]
     int2* pairs = nullptr;
     size_t pair_cap = 0;
+    // page-locked copy of the query's counter block (pairs found, redo count,
+    // error word): a device-to-host copy into pageable memory goes through a
+    // staging buffer and costs tens of microseconds more than the query
+    int* h_cnt = nullptr;
     ~kscr_lbvh() {
         lbvh_release(d);
         if (pairs) (void)hipFree(pairs);
+        if (h_cnt) (void)hipHostFree(h_cnt);
     }
 };
 
@@ -34,6 +39,11 @@ int kscr_lbvh_create(int device, kscr_lbvh** out) {
     if (hipSetDevice(device) != hipSuccess) {
         delete b;
         return KSCR_ERR_HIP;
+    }
+    if (hipHostMalloc((void**)&b->h_cnt, 3 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        b->h_cnt = nullptr;
+        delete b;
+        return KSCR_ERR_OUT_OF_MEMORY;
     }
     *out = b;
     return KSCR_OK;
@@ -82,8 +92,8 @@ int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count) {
     if (lbvh_query(b->d, b->pairs, (int)cap, -1, b->d.count + 2, nullptr) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess)
         return KSCR_ERR_HIP;
-    int cnt[3] = {0, 0, 0};  // pairs found, redo count, error word
-    if (hipMemcpy(cnt, b->d.count, sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
+    int* cnt = b->h_cnt;  // pairs found, redo count, error word
+    if (hipMemcpy(cnt, b->d.count, 3 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
     const int found = cnt[0], err = cnt[2];
     const size_t m = std::min((size_t)found, cap);
     if (m && hipMemcpy(pairs, b->pairs, m * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess) return KSCR_ERR_HIP;
@@ -120,8 +130,8 @@ int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count
     if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
     if (lbvh_query(b->d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, b->d.count + 2, s) != hipSuccess)
         return KSCR_ERR_HIP;
-    int cnt[3] = {0, 0, 0};  // pairs found, redo count, error word: one copy back
-    if (hipMemcpyAsync(cnt, b->d.count, sizeof cnt, hipMemcpyDeviceToHost, s) != hipSuccess ||
+    int* cnt = b->h_cnt;  // pairs found, redo count, error word: one copy back
+    if (hipMemcpyAsync(cnt, b->d.count, 3 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return KSCR_ERR_HIP;
     *count = (size_t)cnt[0];

@@ -18,6 +18,8 @@
 //   endItr                 YB/sim/iteration.cu:62-74
 //   substep split          YB/sim/step.cpp:38-62 (computed on the host)
 //   SDF pass               SDF/cudaSDFCollision.cu:88-155, :205-268
+#include <algorithm>
+
 #include "kscr_device.h"
 #include "kscr_math.h"
 #include "kscr_solve.h"
@@ -235,6 +237,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves
     run_pack<BLOCK>(a, pk.x, pk.y, lds);
 }
 
+// Persistent form of the batched step: a grid of as many workgroups as the
+// GPU holds at once, each taking the packs blockIdx.x, blockIdx.x + gridDim.x,
+// ... in turn (the packs are ordered largest first, so the round robin hands
+// out the longest work first).  Packs are independent within a step, so the
+// order changes nothing in the results.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves(BLOCK)))) void step_batch_persistent_kernel(
+    const StepArgs* __restrict__ grooms, const int4* __restrict__ packs, int num_packs) {
+    __shared__ StepLds<BLOCK> lds;
+    for (int p = blockIdx.x; p < num_packs; p += gridDim.x) {
+        const int4 pk = packs[p];
+        const StepArgs a = grooms[pk.z];
+        run_pack<BLOCK>(a, pk.x, pk.y, lds);
+        __syncthreads();  // the pack's LDS reads end before the next pack's writes
+    }
+}
+
 // After a batched step: last_h <- h and time <- time + dt for every groom
 // (Sim::advance, YB/sim/step.cpp:41-43 and :59).
 __global__ void batch_clock_kernel(StepArgs* grooms, int n, float dt) {
@@ -305,6 +324,51 @@ hipError_t launch_step_batch(const StepArgs* grooms, const int4* packs, int num_
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_step_batch_persistent(const StepArgs* grooms, const int4* packs, int num_packs, int block, int grid,
+                                        hipStream_t stream) {
+    if (num_packs <= 0) return hipSuccess;
+    const dim3 g(std::max(1, std::min(grid, num_packs)));
+    switch (block) {
+#define KSCR_CASE(B)                                                                                           \
+    case B:                                                                                                    \
+        hipLaunchKernelGGL(step_batch_persistent_kernel<B>, g, dim3(B), 0, stream, grooms, packs, num_packs);  \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int batch_persistent_grid(int block) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (block) {
+#define KSCR_CASE(B)                                                                                        \
+    case B:                                                                                                 \
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_batch_persistent_kernel<B>, B, 0);   \
+        break;
+        KSCR_CASE(64)
+        KSCR_CASE(128)
+        KSCR_CASE(256)
+        KSCR_CASE(512)
+        KSCR_CASE(1024)
+#undef KSCR_CASE
+        default:
+            break;
+    }
+    if (e != hipSuccess || per_cu <= 0) return 0;
+    return cus * per_cu;
 }
 
 hipError_t launch_batch_clock(StepArgs* grooms, int n, float dt, hipStream_t stream) {
