@@ -336,7 +336,7 @@ def batched_leg(stream, nsteps, strands=(1_000, 50_000), num_grooms=64):
     import torch
 
     from kamisimulator_amd import scenes
-    from kamisimulator_amd._lib import Batch, Sim, default_params
+    from kamisimulator_amd._lib import BATCH_GRID, BATCH_PERSISTENT, Batch, Sim, default_params
 
     t0 = time.perf_counter()
     grooms = scenes.config_batched(num_grooms=num_grooms, strands=strands)
@@ -362,14 +362,23 @@ def batched_leg(stream, nsteps, strands=(1_000, 50_000), num_grooms=64):
     batch = Batch(sims)
     default_block = batch.block_size
     best = batch.autotune(dt, 3)
-    batch.set_graph(True)
-    batch.step(dt, nsteps, with_sdf=True)  # capture + first replay
-    batch.synchronize()
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    batch.step(dt, nsteps, with_sdf=True)
-    batch.synchronize()
-    graph_ms = (time.perf_counter() - w0) * 1e3
+    best_mapping = batch.mapping
+
+    def graph_run(mapping):
+        batch.set_mapping(mapping)
+        batch.set_graph(True)
+        batch.step(dt, nsteps, with_sdf=True)  # capture + first replay
+        batch.synchronize()
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        batch.step(dt, nsteps, with_sdf=True)
+        batch.synchronize()
+        return (time.perf_counter() - w0) * 1e3
+
+    other = BATCH_GRID if best_mapping == BATCH_PERSISTENT else BATCH_PERSISTENT
+    other_ms = graph_run(other)
+    graph_ms = graph_run(best_mapping)
+    names = {BATCH_GRID: "grid", BATCH_PERSISTENT: "persistent"}
     batch.set_graph(False)
     w0 = time.perf_counter()
     batch.step(dt, nsteps, with_sdf=True)
@@ -390,8 +399,11 @@ def batched_leg(stream, nsteps, strands=(1_000, 50_000), num_grooms=64):
         "speedup_graph_vs_per_groom": round(per_groom_ms / graph_ms, 3),
         "batched_graph_updates_per_s": ups * nsteps / (graph_ms / 1e3),
         "autotuned_block": best,
+        "autotuned_mapping": names[best_mapping],
+        f"batched_graph_{names[other]}_ms_per_step": round(other_ms / nsteps, 4),
         "default_block": default_block,
-        "timing": "host wall clock between device synchronisations",
+        "timing": "host wall clock between device synchronisations; mappings: grid = one workgroup per pack, "
+                  "persistent = as many workgroups as the GPU holds taking packs from a device work counter",
     }
 
 

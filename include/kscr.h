@@ -312,10 +312,19 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf);
 /* use_graph = 0 launches the kernels directly instead of replaying a graph. */
 int kscr_batch_set_graph(kscr_batch* b, int use_graph);
 int kscr_batch_synchronize(kscr_batch* b);
+/* Mapping of the packs (whole strands of one groom, at most the workgroup
+ * size in vertices) onto workgroups: KSCR_BATCH_GRID launches one workgroup
+ * per pack; KSCR_BATCH_PERSISTENT launches as many workgroups as the device
+ * holds at once, each taking the next pack from a device work counter until
+ * none is left.  The results are the same. */
+enum { KSCR_BATCH_GRID = 0, KSCR_BATCH_PERSISTENT = 1 };
+int kscr_batch_set_mapping(kscr_batch* b, int mapping);
+int kscr_batch_mapping(const kscr_batch* b);
 /* Autotuner of the batched launch: times `steps` steps of dt for every
  * instantiated workgroup size that holds the longest strand of any groom,
- * keeps the fastest (every groom's state and clock are saved and restored)
- * and returns it in best_block (may be NULL). */
+ * under both mappings, keeps the fastest pair (every groom's state and clock
+ * are saved and restored) and returns the workgroup size in best_block (may
+ * be NULL; kscr_batch_mapping gives the mapping). */
 int kscr_batch_autotune(kscr_batch* b, float dt, int steps, int* best_block);
 int kscr_batch_block_size(const kscr_batch* b);
 int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n);

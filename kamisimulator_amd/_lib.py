@@ -14,6 +14,9 @@ LIB_PATH = os.path.join(PKG, "libkscr.so")
 SCHEDULE_JACOBI = 0
 SCHEDULE_COLORED = 1
 
+BATCH_GRID = 0
+BATCH_PERSISTENT = 1
+
 FLAG_HAS_PREV = 1
 FLAG_HAS_NEXT = 2
 FLAG_HAS_NEXT_ORIENTATION = 4
@@ -119,6 +122,8 @@ SIGNATURES = {
     "kscr_batch_synchronize": (C.c_int, [_P]),
     "kscr_batch_autotune": (C.c_int, [_P, C.c_float, C.c_int, _IP]),
     "kscr_batch_block_size": (C.c_int, [_P]),
+    "kscr_batch_set_mapping": (C.c_int, [_P, C.c_int]),
+    "kscr_batch_mapping": (C.c_int, [_P]),
     "kscr_batch_last_error": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "kscr_get_contacts": (C.c_int, [_P, _IP, _IP]),
     "kscr_set_material": (C.c_int, [_P, _FP, _FP]),
@@ -480,6 +485,14 @@ class Batch:
         best = C.c_int(0)
         self._check(self.lib.kscr_batch_autotune(self.b, dt, steps, C.byref(best)))
         return best.value
+
+    @property
+    def mapping(self):
+        """BATCH_GRID or BATCH_PERSISTENT (kscr_batch_set_mapping)."""
+        return self.lib.kscr_batch_mapping(self.b)
+
+    def set_mapping(self, mapping):
+        self._check(self.lib.kscr_batch_set_mapping(self.b, int(mapping)))
 
     def set_graph(self, on):
         self._check(self.lib.kscr_batch_set_graph(self.b, int(bool(on))))

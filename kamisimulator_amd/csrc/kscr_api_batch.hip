@@ -1,5 +1,6 @@
 // kscr_api_batch.hip -- batched grooms of the C ABI (kscr_batch_*): one
 // launch per step over the packs of many handles, replayed as a hipGraph.
+#include <algorithm>
 #include <cstring>
 
 #include "kscr_api_internal.h"
@@ -14,7 +15,8 @@ struct kscr_batch {
     int block = 0;
     int num_packs = 0;
     DevBuf<StepArgs> d_args;
-    DevBuf<int4> d_packs;
+    DevBuf<int4> d_packs;  // groom by groom, strands in memory order
+    DevBuf<int> d_next;    // work counter of the persistent mapping
     std::vector<StepArgs> args;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -26,6 +28,10 @@ struct kscr_batch {
     float graph_dt = 0.f;
     std::string err;
     int longest = 0;
+    // KSCR_BATCH_GRID: one workgroup per pack; KSCR_BATCH_PERSISTENT:
+    // persistent_grid workgroups walk the packs
+    int mapping = KSCR_BATCH_GRID;
+    int persistent_grid = 0;
     int fail(int code, const std::string& m) {
         err = m;
         return code;
@@ -43,7 +49,8 @@ struct kscr_batch {
 
 namespace {
 // Packs every groom's strands into workgroups of `block` vertices; a pack is
-// (begin, end, groom, 0).
+// (begin, end, groom, 0), in memory order (consecutive workgroups touch
+// consecutive memory).
 int batch_set_block(kscr_batch* b, int block) {
     std::vector<int4> packs;
     for (size_t g = 0; g < b->grooms.size(); g++) {
@@ -52,7 +59,10 @@ int batch_set_block(kscr_batch* b, int block) {
         if (rc) return rc;
         for (const int2& pk : p2) packs.push_back(make_int4(pk.x, pk.y, (int)g, 0));
     }
+    if ((b->persistent_grid = batch_persistent_grid(block)) <= 0)
+        return b->fail(KSCR_ERR_HIP, "occupancy of the persistent batched kernel");
     hipError_t e;
+    if (!b->d_next.p && (e = b->d_next.alloc(1)) != hipSuccess) return b->hip_fail(e, "hipMalloc batch work counter");
     if (b->d_packs.n < packs.size() && (e = b->d_packs.alloc(packs.size())) != hipSuccess)
         return b->hip_fail(e, "hipMalloc batch packs");
     if ((e = hipMemcpyAsync(b->d_packs.p, packs.data(), packs.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream)) !=
@@ -183,6 +193,16 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
                             b->stream)) != hipSuccess)
         return b->hip_fail(e, "hipMemcpyAsync batch arguments");
     const int ng = (int)b->grooms.size();
+    auto launch = [&]() -> hipError_t {
+        if (b->mapping != KSCR_BATCH_PERSISTENT)
+            return launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream);
+        // the work counter starts at 0 (a kernel, not a memset: the launches
+        // may be captured into a graph)
+        hipError_t r = fill_i32(b->d_next.p, 0, 1, b->stream);
+        if (r != hipSuccess) return r;
+        return launch_step_batch_persistent(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->persistent_grid,
+                                            b->d_next.p, b->stream);
+    };
     // later work queued on a groom's own stream (kscr_get_pos_vel,
     // kscr_advance, ...) is ordered after this batch's kernels
     auto order_handles = [&]() -> int {
@@ -195,8 +215,7 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
     };
     if (!b->use_graph) {
         for (int k = 0; k < nsteps; k++) {
-            if ((e = launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream)) != hipSuccess)
-                return b->hip_fail(e, "batch step launch");
+            if ((e = launch()) != hipSuccess) return b->hip_fail(e, "batch step launch");
             if ((e = launch_batch_clock(b->d_args.p, ng, dt, b->stream)) != hipSuccess)
                 return b->hip_fail(e, "batch clock launch");
         }
@@ -209,7 +228,7 @@ int kscr_batch_step(kscr_batch* b, float dt, int nsteps, int with_sdf) {
             return b->hip_fail(e, "hipStreamBeginCapture");
         hipError_t le = hipSuccess;
         for (int k = 0; k < nsteps && le == hipSuccess; k++) {
-            le = launch_step_batch(b->d_args.p, b->d_packs.p, b->num_packs, b->block, b->stream);
+            le = launch();
             if (le == hipSuccess) le = launch_batch_clock(b->d_args.p, ng, dt, b->stream);
         }
         if ((e = hipStreamEndCapture(b->stream, &graph)) != hipSuccess) return b->hip_fail(e, "hipStreamEndCapture");
@@ -271,24 +290,29 @@ int kscr_batch_autotune(kscr_batch* b, float dt, int steps, int* best_block) {
     hipEvent_t ev0, ev1;
     if ((e = hipEventCreate(&ev0)) != hipSuccess || (e = hipEventCreate(&ev1)) != hipSuccess)
         return b->hip_fail(e, "hipEventCreate");
-    const int start_block = b->block;
-    int best = -1, rc = KSCR_OK;
+    const int start_block = b->block, start_mapping = b->mapping;
+    int best = -1, best_mapping = start_mapping, rc = KSCR_OK;
     float best_ms = 0;
     for (int c : kBlockSizes) {
         if (c < b->longest) continue;
         if ((rc = batch_set_block(b, c)) != KSCR_OK) break;
-        if ((rc = kscr_batch_step(b, dt, 1, 0)) != KSCR_OK) break;  // warm-up
-        (void)hipEventRecord(ev0, b->stream);
-        if ((rc = kscr_batch_step(b, dt, steps, 0)) != KSCR_OK) break;
-        (void)hipEventRecord(ev1, b->stream);
-        if ((e = hipEventSynchronize(ev1)) != hipSuccess) {
-            rc = b->hip_fail(e, "hipEventSynchronize");
-            break;
+        for (int m : {KSCR_BATCH_GRID, KSCR_BATCH_PERSISTENT}) {
+            b->mapping = m;
+            if ((rc = kscr_batch_step(b, dt, 1, 0)) != KSCR_OK) break;  // warm-up
+            (void)hipEventRecord(ev0, b->stream);
+            if ((rc = kscr_batch_step(b, dt, steps, 0)) != KSCR_OK) break;
+            (void)hipEventRecord(ev1, b->stream);
+            if ((e = hipEventSynchronize(ev1)) != hipSuccess) {
+                rc = b->hip_fail(e, "hipEventSynchronize");
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev0, ev1);
+            if (best < 0 || ms < best_ms) best = c, best_mapping = m, best_ms = ms;
         }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev0, ev1);
-        if (best < 0 || ms < best_ms) best = c, best_ms = ms;
+        if (rc != KSCR_OK) break;
     }
+    b->mapping = (rc == KSCR_OK && best > 0) ? best_mapping : start_mapping;
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     b->use_graph = graph_was;
@@ -311,6 +335,15 @@ int kscr_batch_synchronize(kscr_batch* b) {
 }
 
 int kscr_batch_block_size(const kscr_batch* b) { return b ? b->block : -1; }
+
+int kscr_batch_set_mapping(kscr_batch* b, int mapping) {
+    if (!b || (mapping != KSCR_BATCH_GRID && mapping != KSCR_BATCH_PERSISTENT)) return KSCR_ERR_INVALID_ARGUMENT;
+    if (mapping != b->mapping) b->drop_graph();
+    b->mapping = mapping;
+    return KSCR_OK;
+}
+
+int kscr_batch_mapping(const kscr_batch* b) { return b ? b->mapping : -1; }
 
 int kscr_batch_last_error(const kscr_batch* b, char* buf, size_t n) {
     if (!b || !buf || n == 0) return KSCR_ERR_INVALID_ARGUMENT;

@@ -134,11 +134,11 @@ hipError_t launch_step_batch(const StepArgs* grooms, const int4* packs, int num_
 // (i, i + 1).
 hipError_t launch_strain_limit(float* x, const int* strand_off, const float* rest, int num_strands,
                                float strain_error, hipStream_t stream);
-// Persistent form: `grid` workgroups walk the packs with a stride of grid
-// (packs largest first); batch_persistent_grid(block) is the number of
-// workgroups the current device holds at once (0 if unknown).
+// Persistent form: `grid` workgroups take the packs from the device counter
+// *next, which must be 0 at the launch; batch_persistent_grid(block) is the
+// number of workgroups the current device holds at once (0 if unknown).
 hipError_t launch_step_batch_persistent(const StepArgs* grooms, const int4* packs, int num_packs, int block, int grid,
-                                        hipStream_t stream);
+                                        int* next, hipStream_t stream);
 int batch_persistent_grid(int block);
 // last_h <- h, time <- time + dt for n grooms of a device table.
 hipError_t launch_batch_clock(StepArgs* grooms, int n, float dt, hipStream_t stream);

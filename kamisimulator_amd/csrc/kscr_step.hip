@@ -238,19 +238,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves
 }
 
 // Persistent form of the batched step: a grid of as many workgroups as the
-// GPU holds at once, each taking the packs blockIdx.x, blockIdx.x + gridDim.x,
-// ... in turn (the packs are ordered largest first, so the round robin hands
-// out the longest work first).  Packs are independent within a step, so the
-// order changes nothing in the results.
+// GPU holds at once, each taking the next pack from a device counter (reset
+// to 0 before the launch) until the counter passes the last pack.  Packs are
+// taken in memory order, and a workgroup that starts late only finds the
+// queue shorter.  Packs are independent within a step, so the order in which
+// they are taken changes nothing in the results.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(reg_waves(BLOCK)))) void step_batch_persistent_kernel(
-    const StepArgs* __restrict__ grooms, const int4* __restrict__ packs, int num_packs) {
+    const StepArgs* __restrict__ grooms, const int4* __restrict__ packs, int num_packs, int* __restrict__ next) {
     __shared__ StepLds<BLOCK> lds;
-    for (int p = blockIdx.x; p < num_packs; p += gridDim.x) {
+    __shared__ int s_pack;
+    for (;;) {
+        if (threadIdx.x == 0) s_pack = atomicAdd(next, 1);
+        __syncthreads();
+        const int p = s_pack;
+        __syncthreads();  // every lane has read s_pack before lane 0 takes the next one
+        if (p >= num_packs) break;
         const int4 pk = packs[p];
         const StepArgs a = grooms[pk.z];
         run_pack<BLOCK>(a, pk.x, pk.y, lds);
-        __syncthreads();  // the pack's LDS reads end before the next pack's writes
     }
 }
 
@@ -327,13 +333,13 @@ hipError_t launch_step_batch(const StepArgs* grooms, const int4* packs, int num_
 }
 
 hipError_t launch_step_batch_persistent(const StepArgs* grooms, const int4* packs, int num_packs, int block, int grid,
-                                        hipStream_t stream) {
+                                        int* next, hipStream_t stream) {
     if (num_packs <= 0) return hipSuccess;
     const dim3 g(std::max(1, std::min(grid, num_packs)));
     switch (block) {
-#define KSCR_CASE(B)                                                                                           \
-    case B:                                                                                                    \
-        hipLaunchKernelGGL(step_batch_persistent_kernel<B>, g, dim3(B), 0, stream, grooms, packs, num_packs);  \
+#define KSCR_CASE(B)                                                                                                 \
+    case B:                                                                                                          \
+        hipLaunchKernelGGL(step_batch_persistent_kernel<B>, g, dim3(B), 0, stream, grooms, packs, num_packs, next);  \
         break;
         KSCR_CASE(64)
         KSCR_CASE(128)

@@ -1,14 +1,14 @@
 """Batched form (kscr_batch_*) and the autotuner on the GPU.
 
 A batch advances several independent grooms (own sizes, gravity directions
-and SDF grids) with one launch per step replayed from a hipGraph; each groom
-must match the FP32 restatement run on its own (1e-5 relative; the FP32
-arithmetic is the same, so bit-exact agreement is expected)."""
+and SDF grids) with one launch per step replayed from a hipGraph, under
+either mapping of packs onto workgroups (one per pack, or a persistent grid);
+each groom must match the FP32 restatement run on its own word for word."""
 import numpy as np
 import pytest
 
 from kamisimulator_amd import scenes
-from kamisimulator_amd._lib import Batch, Sim, default_params
+from kamisimulator_amd._lib import BATCH_GRID, BATCH_PERSISTENT, Batch, Sim, default_params
 from oracle.pyoracle import OracleSim
 from tests import parity
 
@@ -42,15 +42,19 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_matches_oracle(use_graph):
+@pytest.mark.parametrize("mapping", [BATCH_GRID, BATCH_PERSISTENT])
+def test_batch_matches_oracle(use_graph, mapping):
     grooms = _grooms()
-    print(f"[gpu-batch] grooms={len(grooms)} verts={[g.num_verts for g in grooms]} graph={use_graph}", flush=True)
+    print(f"[gpu-batch] grooms={len(grooms)} verts={[g.num_verts for g in grooms]} graph={use_graph} "
+          f"mapping={mapping}", flush=True)
     sims = [_make(sc, Sim) for sc in grooms]
     orcs = [_make(sc, OracleSim) for sc in grooms]
     for o in orcs:
         o.set_threads(8)
     b = Batch(sims)
     b.set_graph(use_graph)
+    b.set_mapping(mapping)
+    assert b.mapping == mapping
     b.step(DT, nsteps=4, with_sdf=True)
     b.step(DT, nsteps=4, with_sdf=True)
     b.synchronize()
@@ -64,11 +68,13 @@ def test_batch_matches_oracle(use_graph):
                      {"x": sc.positions})
 
 
-def test_batch_equals_per_groom_launches():
+@pytest.mark.parametrize("mapping", [BATCH_GRID, BATCH_PERSISTENT])
+def test_batch_equals_per_groom_launches(mapping):
     grooms = _grooms(3, seed=5)
     a = [_make(sc, Sim) for sc in grooms]
     c = [_make(sc, Sim) for sc in grooms]
     b = Batch(a)
+    b.set_mapping(mapping)
     b.step(DT, nsteps=6, with_sdf=True)
     b.synchronize()
     for s in c:
@@ -95,6 +101,28 @@ def test_autotune_keeps_state():
     assert np.array_equal(a.get_pos_vel()[0].view(np.uint32), c.get_pos_vel()[0].view(np.uint32))
 
 
+def test_batch_autotune_picks_size_and_mapping():
+    """The batch autotuner times every workgroup size under both mappings and
+    keeps the fastest pair; the grooms' states and clocks come back as they
+    were, so the batch then steps exactly as per-groom launches do."""
+    grooms = _grooms(3, seed=8)
+    a = [_make(sc, Sim) for sc in grooms]
+    c = [_make(sc, Sim) for sc in grooms]
+    b = Batch(a)
+    best = b.autotune(DT, steps=2)
+    print(f"[gpu-batch] batch autotune: block {best}, mapping {b.mapping}", flush=True)
+    assert best in (256, 512, 1024) and b.mapping in (BATCH_GRID, BATCH_PERSISTENT)
+    with pytest.raises(Exception):
+        b.set_mapping(7)
+    b.step(DT, nsteps=3, with_sdf=True)
+    b.synchronize()
+    for s in c:
+        for _ in range(3):
+            s.solve_next_step(DT)
+    for s, t in zip(a, c):
+        assert np.array_equal(s.get_pos_vel()[0].view(np.uint32), t.get_pos_vel()[0].view(np.uint32))
+
+
 def test_64_grooms_200_steps_full_size():
     """The batched workload at its full size: 64 grooms with log-uniform
     1k-50k strands of 32-256 vertices (about 9e7 vertices), each with its own
@@ -109,6 +137,8 @@ def test_64_grooms_200_steps_full_size():
     b = Batch(sims)
     b.set_graph(True)
     for k in range(4):
+        # the first 100 steps one workgroup per pack, the last 100 persistent
+        b.set_mapping(BATCH_GRID if k < 2 else BATCH_PERSISTENT)
         b.step(DT, nsteps=50, with_sdf=True)
         b.synchronize()
         print(f"[gpu-batch] step {50 * (k + 1)}", flush=True)
