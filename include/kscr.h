@@ -352,11 +352,12 @@ int kscr_lbvh_query(kscr_lbvh* b, int* pairs, size_t cap, size_t* count);
  * as the reference's own interface takes them (LBVH::compute(Bound*, size),
  * LBVH::refit(), LBVH::query(ivec2*, size): lbvh.cu:348-426): d_aabbs is
  * float[6*n] and d_pairs int[2*cap] on the handle's device, and the work runs
- * on `stream` (a hipStream_t; NULL = the null stream).  compute_device
- * returns after the tree is built (it reads the level sizes back, as the
- * reference reads the tree depth, lbvh.cu:393); query_device returns the
- * number of pairs found (which may exceed cap), as LBVH::query does
- * (lbvh.cu:425). */
+ * on `stream` (a hipStream_t; NULL = the null stream).  d_aabbs is read in
+ * place, as the reference keeps the pointer compute is given: it must hold
+ * the boxes until the work queued by the next compute_device, refit_device or
+ * query_device has run.  compute_device and refit_device queue their work and
+ * return; query_device waits for it and returns the number of pairs found
+ * (which may exceed cap), as LBVH::query does (lbvh.cu:425). */
 int kscr_lbvh_compute_device(kscr_lbvh* b, const float* d_aabbs, int n, void* stream);
 int kscr_lbvh_refit_device(kscr_lbvh* b, const float* d_aabbs, void* stream);
 int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count, void* stream);
@@ -372,6 +373,12 @@ int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count
  * failure. */
 int kscr_lbvh_self_query_static(const float* d_aabbs, int n, int* d_pairs, size_t cap, size_t* count, void* stream,
                                 char* err, size_t err_len);
+/* No counterpart in the reference (a tuning and test hook): how many of the
+ * nodes whose leaf ranges cross a 512-leaf tile boundary the fit of a tree
+ * over at most 131072 boxes keeps in LDS, 0 .. 4096 (the default).  Trees
+ * with more such nodes (about 2k at 100k boxes) fit them from global memory.
+ * Applies from the next compute or refit; the tree is the same either way. */
+int kscr_lbvh_set_fit_slots(kscr_lbvh* b, int slots);
 /* Tree contents: sorted Morton codes and object ids (n), internal nodes as
  * (parent, left, right, fence) (4 per node) and child boxes (12 per node),
  * leaf parents (n), root box (6), DFS stack bound. */

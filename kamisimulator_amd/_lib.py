@@ -142,6 +142,7 @@ SIGNATURES = {
     "kscr_lbvh_self_query_static": (C.c_int, [_P, C.c_int, _P, C.c_size_t, C.POINTER(C.c_size_t), _P, C.c_char_p,
                                               C.c_size_t]),
     "kscr_lbvh_get": (C.c_int, [_P, _UP, _UP, _UP, _FP, _UP, _FP, _IP]),
+    "kscr_lbvh_set_fit_slots": (C.c_int, [_P, C.c_int]),
 }
 
 _lib = None
@@ -570,6 +571,12 @@ class Lbvh:
         if rc:
             raise KscrError(rc, "kscr_lbvh_query_device failed")
         return cnt.value
+
+    def set_fit_slots(self, slots):
+        """Crossing nodes the fit of a small tree keeps in LDS (0 .. 4096)."""
+        rc = self.lib.kscr_lbvh_set_fit_slots(self.b, int(slots))
+        if rc:
+            raise KscrError(rc, "kscr_lbvh_set_fit_slots failed")
 
     def get(self):
         n = self.n

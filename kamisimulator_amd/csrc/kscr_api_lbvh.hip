@@ -61,6 +61,7 @@ static int lbvh_upload(kscr_lbvh* b, const float* aabbs, int n) {
     if (lbvh_reserve(b->d, n) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
     if (hipMemcpy(b->d.aabb, aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
         return KSCR_ERR_HIP;
+    b->d.src = nullptr;  // the uploaded copy
     return KSCR_OK;
 }
 
@@ -106,9 +107,8 @@ int kscr_lbvh_compute_device(kscr_lbvh* b, const float* d_aabbs, int n, void* st
     const hipStream_t s = static_cast<hipStream_t>(stream);
     if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
     if (lbvh_reserve(b->d, n) != hipSuccess) return KSCR_ERR_OUT_OF_MEMORY;
-    if (hipMemcpyAsync(b->d.aabb, d_aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        lbvh_build(b->d, n, s) != hipSuccess)
-        return KSCR_ERR_HIP;
+    b->d.src = d_aabbs;  // read in place until the next compute or refit
+    if (lbvh_build(b->d, n, s) != hipSuccess) return KSCR_ERR_HIP;
     return KSCR_OK;
 }
 
@@ -116,10 +116,8 @@ int kscr_lbvh_refit_device(kscr_lbvh* b, const float* d_aabbs, void* stream) {
     if (!b || !d_aabbs || !b->d.built) return KSCR_ERR_INVALID_ARGUMENT;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     if (hipSetDevice(b->device) != hipSuccess) return KSCR_ERR_HIP;
-    if (hipMemcpyAsync(b->d.aabb, d_aabbs, (size_t)b->d.n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) !=
-            hipSuccess ||
-        lbvh_refit(b->d, s) != hipSuccess)
-        return KSCR_ERR_HIP;
+    b->d.src = d_aabbs;
+    if (lbvh_refit(b->d, s) != hipSuccess) return KSCR_ERR_HIP;
     return KSCR_OK;
 }
 
@@ -136,6 +134,12 @@ int kscr_lbvh_query_device(kscr_lbvh* b, int* d_pairs, size_t cap, size_t* count
         return KSCR_ERR_HIP;
     *count = (size_t)cnt[0];
     return cnt[2] ? KSCR_ERR_MAX_COLLISIONS : KSCR_OK;
+}
+
+int kscr_lbvh_set_fit_slots(kscr_lbvh* b, int slots) {
+    if (!b || slots < 0 || slots > kLbvhFitSlots) return KSCR_ERR_INVALID_ARGUMENT;
+    b->d.fit_slots = slots;
+    return KSCR_OK;
 }
 
 int kscr_lbvh_get(kscr_lbvh* b, uint32_t* codes, uint32_t* ids, uint32_t* links, float* bounds, uint32_t* leaf_parent,

@@ -30,11 +30,21 @@ static_assert(sizeof(LbvhNode) == 64 && offsetof(LbvhNode, fence) == 12 && offse
 // most 63 levels tall.
 constexpr int kMaxLevels = 64;
 
+// Trees over at most 131072 objects are fit by rounds (no height pass); the
+// nodes that cross a 512-leaf tile boundary, about 2k of them at 100k objects,
+// are fit from LDS up to this many and from global memory beyond.
+constexpr int kLbvhFitSlots = 4096;
+
 struct LbvhDev {
     int n = 0;
     int cap = 0;
     int tiles = 0;
     float* aabb = nullptr;  // 6 floats per object
+    // boxes the build, refit and query read in place of aabb when set (device
+    // memory of the caller's, kept as the reference's LBVH keeps the pointer
+    // compute was given)
+    const float* src = nullptr;
+    const float* boxes() const { return src ? src : aabb; }
     uint32_t* codes = nullptr;
     uint32_t* ids = nullptr;
     uint32_t* codes_tmp = nullptr;
@@ -59,6 +69,14 @@ struct LbvhDev {
     uint32_t* level_count = nullptr;  // kMaxLevels counts, then kMaxLevels scatter cursors
     int2* level_range = nullptr;      // kMaxLevels (offset, count)
     int max_depth_host = -1;          // DFS stack bound of the built tree once read back (lbvh_read_depth)
+    // Trees over at most 131072 objects ("small"): the sort runs one launch per
+    // digit pass and the fit runs by rounds; then node_height holds each
+    // node's DFS stack depth, level_nodes the nodes that cross a tile
+    // boundary, redo their child words, flags each such node's index in that
+    // list, level_count[0] their number, and codes_tmp the done words of the
+    // global-memory fit.
+    bool small = false;
+    int fit_slots = kLbvhFitSlots;  // crossing nodes fit from LDS (0 .. kLbvhFitSlots)
     bool built = false;
 };
 

@@ -19,6 +19,11 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kTile = kSortThreads * kSortItems;  // keys per radix tile
 constexpr int kScanChunk = 1024;
+// Sorts of at most kFusedTiles tiles run one launch per digit pass
+// (radix_scatter_kernel<true>); their per-tile digit counts are kept
+// digit-major with kFusedTiles entries per digit, one such block per pass.
+constexpr int kFusedTiles = 64;
+constexpr int kFusedHist = 256 * kFusedTiles;
 constexpr int kQueryBlock = 128;
 constexpr int kQueryStack = 64;
 constexpr int kQueryStackFirst = 12;  // first pass; deeper queries are redone
@@ -47,6 +52,31 @@ __device__ inline int cvt_rzi(float f) {
 
 __device__ inline bool intersects(const float* a, const float* b) {
     return !(a[3] <= b[0] || a[4] <= b[1] || a[5] <= b[2] || a[0] >= b[3] || a[1] >= b[4] || a[2] >= b[5]);
+}
+
+// Workgroup barrier that orders LDS only: it waits for the wave's LDS
+// accesses, not for its global stores and atomics (which __syncthreads waits
+// for).  For kernels that never read in the kernel what they write to global
+// memory.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Whether any wave of the workgroup passed a nonzero word to any_wave_flag's
+// LDS row this round: the row is read with 16-byte loads and OR'd without
+// short-circuits, so the reads are in flight together.
+template <int WAVES>
+__device__ inline bool any_wave_flag(const uint32_t* row) {
+    static_assert(WAVES % 4 == 0, "flag rows are read four words at a time");
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < WAVES / 4; k++) {
+        const uint4 v = reinterpret_cast<const uint4*>(row)[k];
+        acc |= v.x | v.y | v.z | v.w;
+    }
+    return acc != 0;
 }
 
 // ---------------------------------------------------------------- root box
@@ -106,13 +136,18 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
 // mortonKernel (lbvh.cu:53-60) over one radix tile (kTile objects), fused
 // with the final reduction of the root box (every workgroup reduces the
 // partials of root_partial_kernel; min and max give the same box in any
-// order) and with the first sort pass's digit histogram of the tile.
+// order) and with the first sort pass's digit histogram of the tile
+// (hist[digit * stride + tile]).  For a one-launch-per-pass sort, each
+// workgroup also clears its tile's counts of the three later passes
+// (hist_next, kFusedHist words per pass) and workgroup 0 the word at zero_word.
 __global__ __launch_bounds__(kSortThreads) void morton_hist_kernel(const float* __restrict__ aabb,
                                                                    const float* __restrict__ partial, int pblocks,
-                                                                   int n, int tiles, float* __restrict__ root_out,
+                                                                   int n, int stride, float* __restrict__ root_out,
                                                                    uint32_t* __restrict__ codes,
                                                                    uint32_t* __restrict__ ids,
-                                                                   uint32_t* __restrict__ hist) {
+                                                                   uint32_t* __restrict__ hist,
+                                                                   uint32_t* __restrict__ hist_next,
+                                                                   uint32_t* __restrict__ zero_word) {
     __shared__ float s[6][kSortThreads];
     __shared__ uint32_t cnt[256];
     const int t = threadIdx.x;
@@ -142,7 +177,10 @@ __global__ __launch_bounds__(kSortThreads) void morton_hist_kernel(const float* 
         }
     }
     __syncthreads();
-    hist[(size_t)t * tiles + blockIdx.x] = cnt[t];
+    hist[(size_t)t * stride + blockIdx.x] = cnt[t];
+    if (hist_next)
+        for (int k = 0; k < 3; k++) hist_next[(size_t)k * kFusedHist + t * kFusedTiles + blockIdx.x] = 0u;
+    if (zero_word && blockIdx.x == 0 && t == 0) *zero_word = 0u;
 }
 
 // Exclusive scan of at most kSmallScanMax values in one workgroup: the values
@@ -239,32 +277,80 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t* __restrict__ v,
 }
 
 // Stable scatter: within a tile, items keep their order (round, wave, lane);
-// tiles keep their order through the digit-major, tile-minor offsets.
+// tiles keep their order through the digit-major, tile-minor offsets.  The
+// kernel reads none of its global writes, so its barriers order LDS only.
+// FUSED = false: offsets is the exclusive scan of the per-tile digit counts
+// (digit-major, `tiles` per digit).
+// FUSED = true (at most kFusedTiles tiles): offsets holds the per-tile digit
+// counts themselves (kFusedTiles per digit) and every workgroup forms its own
+// offsets from them: the digit totals below its digit plus its digit's counts
+// in the earlier tiles, the same values as the scan.  With hist_next (cleared
+// beforehand), each key also counts its next digit in the tile of its
+// destination, one global atomic per key: the next pass's per-tile counts,
+// so the pass needs no launch of its own for them.
+template <bool FUSED>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t* __restrict__ keys,
                                                                      const uint32_t* __restrict__ vals, int n, int shift,
                                                                      int tiles, const uint32_t* __restrict__ offsets,
                                                                      uint32_t* __restrict__ keys_out,
-                                                                     uint32_t* __restrict__ vals_out) {
+                                                                     uint32_t* __restrict__ vals_out,
+                                                                     uint32_t* __restrict__ hist_next) {
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t running[256];
     __shared__ uint32_t wave_cnt[kWaves][256];
+    __shared__ uint32_t wave_sum[kWaves];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    running[t] = offsets[(size_t)t * tiles + blockIdx.x];
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // every round's keys and values are loaded before the first is used, so
+    // their latency is paid once, not once per round
     const int base = blockIdx.x * kTile;
+    uint32_t kreg[kSortItems], vreg[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int i = base + r * kSortThreads + t;
+        kreg[r] = i < n ? keys[i] : 0u;
+        vreg[r] = i < n ? vals[i] : 0u;
+    }
+    if (FUSED) {
+        // entries past `tiles` are never written: masked, not summed
+        const uint32_t* row = offsets + (size_t)t * kFusedTiles;
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int b = 0; b < kFusedTiles; b += 4) {
+            const uint4 v = *reinterpret_cast<const uint4*>(row + b);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t x = b + k < tiles ? w[k] : 0u;
+                total += x;
+                before += b + k < (int)blockIdx.x ? x : 0u;
+            }
+        }
+        // exclusive sum of the digit totals over the digits below t
+        uint32_t incl = total;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wave_sum[wave] = incl;
+        lds_barrier();
+        uint32_t below = incl - total;
+        for (int w = 0; w < wave; w++) below += wave_sum[w];
+        running[t] = below + before;
+    } else {
+        running[t] = offsets[(size_t)t * tiles + blockIdx.x];
+    }
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         for (int w = 0; w < kWaves; w++) wave_cnt[w][t] = 0;
-        __syncthreads();
+        lds_barrier();
         const int i = base + r * kSortThreads + t;
         const bool valid = i < n;
-        uint32_t key = 0, val = 0, digit = 0;
-        if (valid) {
-            key = keys[i];
-            val = vals[i];
-            digit = (key >> shift) & 255u;
-        }
+        const uint32_t key = kreg[r], val = vreg[r];
+        const uint32_t digit = valid ? (key >> shift) & 255u : 0u;
         uint64_t match = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -274,18 +360,20 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
         }
         const uint32_t rank = (uint32_t)__popcll(match & lt_mask);
         if (valid && (match & lt_mask) == 0) wave_cnt[wave][digit] = (uint32_t)__popcll(match);
-        __syncthreads();
+        lds_barrier();
         if (valid) {
             uint32_t off = running[digit] + rank;
             for (int w = 0; w < wave; w++) off += wave_cnt[w][digit];
             keys_out[off] = key;
             vals_out[off] = val;
+            if (FUSED && hist_next)
+                atomicAdd(hist_next + ((key >> (shift + 8)) & 255u) * kFusedTiles + off / kTile, 1u);
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t add = 0;
         for (int w = 0; w < kWaves; w++) add += wave_cnt[w][t];
         running[t] += add;
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -293,21 +381,12 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
 __device__ inline int clz64(uint64_t x) { return x ? __clzll(x) : 64; }
 __device__ inline uint64_t sort_key(const uint32_t* codes, int i) { return ((uint64_t)codes[i] << 32) | (uint32_t)i; }
 
-// Karras's internal node construction (lbvh.cu:68-150)
-// Also clears what the height pass accumulates into: each internal node's
-// arrival word and the per-height counters (2 kMaxLevels words).
-__global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n, LbvhNode* __restrict__ nodes,
-                                      uint32_t* __restrict__ leaf_parent, int* __restrict__ word,
-                                      uint32_t* __restrict__ level_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 2 * kMaxLevels) level_count[i] = 0;
-    if (i >= n - 1) return;
-    word[i] = 0;
-    int lo, hi;
+// Karras's internal node construction (lbvh.cu:68-150) for node i: its leaf
+// range [lo, hi] and the split (left child ends at split).
+__device__ inline void karras_node(const uint32_t* __restrict__ codes, int n, int i, int& lo, int& hi, int& split) {
     if (i == 0) {
         lo = 0;
         hi = n - 1;
-        nodes[0].parent = 0;  // the root has no parent (value-initialised in the reference)
     } else {
         const uint64_t self = sort_key(codes, i);
         const int dl = clz64(self ^ sort_key(codes, i - 1));
@@ -330,15 +409,23 @@ __global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n,
         lo = min(i, j);
         hi = max(i, j);
     }
-    nodes[i].fence = (i == lo) ? (uint32_t)hi : (uint32_t)lo;
     const uint64_t fk = sort_key(codes, lo);
     const int dnode = clz64(fk ^ sort_key(codes, hi));
-    int split = lo, stride = hi - lo;
+    int s = lo, stride = hi - lo;
     do {
         stride = (stride + 1) >> 1;
-        const int mid = split + stride;
-        if (mid < hi && clz64(fk ^ sort_key(codes, mid)) > dnode) split = mid;
+        const int mid = s + stride;
+        if (mid < hi && clz64(fk ^ sort_key(codes, mid)) > dnode) s = mid;
     } while (stride > 1);
+    split = s;
+}
+
+// Writes node i's links and its children's parent links; returns the child
+// words (leaf bit in the MSB).
+__device__ inline uint2 write_links(LbvhNode* __restrict__ nodes, uint32_t* __restrict__ leaf_parent, int i, int lo,
+                                    int hi, int split) {
+    if (i == 0) nodes[0].parent = 0;  // the root has no parent (value-initialised in the reference)
+    nodes[i].fence = (i == lo) ? (uint32_t)hi : (uint32_t)lo;
     uint32_t left, right;
     if (lo == split) {
         leaf_parent[split] = (uint32_t)i;
@@ -356,6 +443,21 @@ __global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n,
     }
     nodes[i].left = left;
     nodes[i].right = right;
+    return make_uint2(left, right);
+}
+
+// All internal nodes; also clears what the height pass accumulates into: each
+// internal node's arrival word and the per-height counters (2 kMaxLevels words).
+__global__ void build_internal_kernel(const uint32_t* __restrict__ codes, int n, LbvhNode* __restrict__ nodes,
+                                      uint32_t* __restrict__ leaf_parent, int* __restrict__ word,
+                                      uint32_t* __restrict__ level_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * kMaxLevels) level_count[i] = 0;
+    if (i >= n - 1) return;
+    word[i] = 0;
+    int lo, hi, split;
+    karras_node(codes, n, i, lo, hi, split);
+    write_links(nodes, leaf_parent, i, lo, hi, split);
 }
 
 // Bottom-up refit (lbvh.cu:153-188): the second thread to reach a node
@@ -575,6 +677,448 @@ __global__ __launch_bounds__(kTopBlock) void refit_top_kernel(const float* __res
         if (lv.y == 0) continue;
         for (int k = threadIdx.x; k < lv.y; k += kTopBlock) refit_node(aabb, ids, nodes, level_nodes[lv.x + k]);
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- small trees
+// Trees over at most kSmallBuild objects are fit without heights or level
+// lists, bottom-up in LDS as mergeUpKernel (lbvh.cu:153-188) fits in global
+// memory: tile_fit_kernel fits the nodes inside each tile of kRefitTile
+// leaves, and cross_fit_kernel the nodes that cross a tile boundary, in one
+// workgroup.  Every finished node's lane takes an LDS atomic arrival at its
+// parent; the second child to arrive forms the parent's union and goes on up.
+// Each union is written before the arrival that publishes it (release) and
+// read after the arrival that finds it (acquire), LDS-scoped fences.  The
+// child boxes go to the nodes after one barrier.  The unions are formed as
+// refit_node forms a child's union from the child's two boxes, so every node
+// box equals the height-ordered refit's bit for bit.
+constexpr int kSmallBuild = kFusedTiles * kTile;
+constexpr uint16_t kNoParent = 0xFFFFu;
+
+__device__ inline void child_union(const float* b0, const float* b1, float* u) {
+    for (int j = 0; j < 3; j++) {
+        u[j] = b1[j] < b0[j] ? b1[j] : b0[j];
+        u[3 + j] = b0[3 + j] < b1[3 + j] ? b1[3 + j] : b0[3 + j];
+    }
+}
+
+__device__ inline void store_child_boxes(LbvhNode* __restrict__ node, const float (&b)[2][6]) {
+    float4* q = reinterpret_cast<float4*>(node->b[0]);
+    q[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
+    q[1] = make_float4(b[0][4], b[0][5], b[1][0], b[1][1]);
+    q[2] = make_float4(b[1][2], b[1][3], b[1][4], b[1][5]);
+}
+
+__device__ inline bool range_crosses(uint32_t p, uint32_t fence) {
+    return min(p, fence) / kRefitTile != max(p, fence) / kRefitTile;
+}
+
+// The arrival of a child at its parent's LDS counter: everything the lane
+// wrote to LDS before is visible to the lane that finds the count it left,
+// and that lane sees what the first arrival wrote.  Returns the count before.
+__device__ inline uint32_t lds_arrive(uint32_t* counter) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    const uint32_t old = atomicAdd(counter, 1u);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    return old;
+}
+
+// In the crossing list's child words: the child is an internal node whose
+// range crosses a tile boundary too (indices of small trees stay below 2^30).
+constexpr uint32_t kCrossBit = 0x40000000u;
+
+// One workgroup per tile of kRefitTile sorted leaves and the internal nodes
+// with the same indices.  BUILD: the nodes are built here (karras_node,
+// write_links, as build_internal_kernel), each in-tile node's DFS stack depth
+// (leaf 1, node max(dL, dR + 1): the bound mergeUpKernel computes,
+// lbvh.cu:153-188) goes to node_dep and the root's to *max_depth, and the
+// nodes whose ranges cross a tile boundary are listed for cross_fit_kernel
+// (cross_list[slot] = node, cross_child[slot] = its child words with
+// kCrossBit on crossing children, slot_of[node] = slot; the slot order varies
+// from run to run, the boxes do not).  Otherwise the links are read.  A node
+// whose range lies in the tile has its whole subtree there: its children
+// register with it in LDS, and each leaf's lane walks up until it arrives
+// first or reaches a node whose parent crosses.
+template <bool BUILD>
+__global__ __launch_bounds__(kRefitTile) void tile_fit_kernel(const uint32_t* __restrict__ codes,
+                                                              const float* __restrict__ aabb,
+                                                              const uint32_t* __restrict__ ids,
+                                                              LbvhNode* __restrict__ nodes,
+                                                              uint32_t* __restrict__ leaf_parent, int n,
+                                                              uint32_t* __restrict__ node_dep,
+                                                              uint32_t* __restrict__ cross_list,
+                                                              uint2* __restrict__ cross_child,
+                                                              uint32_t* __restrict__ slot_of,
+                                                              uint32_t* __restrict__ cross_count,
+                                                              int* __restrict__ max_depth) {
+    __shared__ float s_leaf[6][kRefitTile];
+    __shared__ float s_node[6][kRefitTile];
+    __shared__ uint32_t s_child[2][kRefitTile];
+    __shared__ uint32_t s_arrive[kRefitTile];
+    __shared__ uint16_t s_par_leaf[kRefitTile], s_par_node[kRefitTile];  // local parent | side << 15
+    __shared__ uint8_t s_dep[kRefitTile];
+    __shared__ uint32_t s_ncross, s_base;
+    const int t = threadIdx.x;
+    const int L = blockIdx.x * kRefitTile;
+    const int p = L + t, m = n - 1;
+    if (t == 0) s_ncross = 0;
+    s_arrive[t] = 0;
+    s_par_leaf[t] = s_par_node[t] = kNoParent;
+    if (p < n) {
+        const float* a = aabb + 6 * (size_t)ids[p];
+        for (int j = 0; j < 6; j++) s_leaf[j][t] = a[j];
+    }
+    bool in = false, cross = false;
+    uint32_t child[2] = {0, 0}, cross_mark = 0;
+    if (p < m) {
+        int lo, hi;
+        if (BUILD) {
+            int split;
+            karras_node(codes, n, p, lo, hi, split);
+            const uint2 c = write_links(nodes, leaf_parent, p, lo, hi, split);
+            child[0] = c.x;
+            child[1] = c.y;
+            // internal children whose own ranges cross a tile boundary, for
+            // the crossing list's child words
+            if (!(c.x & 0x80000000u) && lo / kRefitTile != split / kRefitTile) cross_mark |= 1u;
+            if (!(c.y & 0x80000000u) && (split + 1) / kRefitTile != hi / kRefitTile) cross_mark |= 2u;
+        } else {
+            const uint4 w = *reinterpret_cast<const uint4*>(nodes + p);  // parent, left, right, fence
+            lo = min(p, (int)w.w);
+            hi = max(p, (int)w.w);
+            child[0] = w.y;
+            child[1] = w.z;
+        }
+        in = lo / kRefitTile == hi / kRefitTile;
+        cross = !in;
+    }
+    s_child[0][t] = child[0];
+    s_child[1][t] = child[1];
+    // the kernel reads none of its global writes: its barriers order LDS only
+    lds_barrier();
+    if (in)
+        for (int side = 0; side < 2; side++) {
+            const uint32_t c = child[side];
+            const int k = (int)(c & 0x7FFFFFFFu) - L;  // in this tile, as the node is
+            const uint16_t up = (uint16_t)(t | (side << 15));
+            if (c & 0x80000000u)
+                s_par_leaf[k] = up;
+            else
+                s_par_node[k] = up;
+        }
+    uint32_t rank = 0;
+    if (BUILD && cross) rank = atomicAdd(&s_ncross, 1u);
+    lds_barrier();
+    if (BUILD && t == 0) s_base = s_ncross ? atomicAdd(cross_count, s_ncross) : 0u;
+    // Walk from leaf t.  The walker holds the union it brings; at the parent
+    // it reads the sibling's (a node's union and depth are in LDS before its
+    // arrival).  The parent's links are read before the arrival: they do not
+    // change during the walks.
+    if (p < n) {
+        uint32_t up = s_par_leaf[t];
+        uint32_t dep = 1;
+        float own[6];
+        for (int j = 0; j < 6; j++) own[j] = s_leaf[j][t];
+        // a subtree is at most kMaxLevels tall: the bound only ends a walk on a corrupt tree
+        for (int step = 0; step <= kMaxLevels && up != kNoParent; step++) {
+            const int q = (int)(up & 0x7FFFu);
+            const int side = (int)(up >> 15);
+            const uint32_t sib = s_child[side ^ 1][q];
+            const uint32_t next = s_par_node[q];
+            if (lds_arrive(&s_arrive[q]) == 0) break;  // the sibling goes on
+            const int k = (int)(sib & 0x7FFFFFFFu) - L;
+            const bool leaf = sib & 0x80000000u;
+            // left and right by select (an index by `side` would select
+            // over every element)
+            float bl[6], br[6];
+            for (int j = 0; j < 6; j++) {
+                const float o = leaf ? s_leaf[j][k] : s_node[j][k];
+                bl[j] = side ? o : own[j];
+                br[j] = side ? own[j] : o;
+            }
+            const uint32_t ds = leaf ? 1u : s_dep[k];
+            const uint32_t d0 = side ? ds : dep, d1 = side ? dep : ds;
+            child_union(bl, br, own);
+            for (int j = 0; j < 6; j++) s_node[j][q] = own[j];
+            dep = d0 > d1 + 1 ? d0 : d1 + 1;
+            s_dep[q] = (uint8_t)dep;
+            up = next;
+        }
+    }
+    lds_barrier();
+    // each in-tile node's child boxes, from the finished unions
+    if (in) {
+        float b[2][6];
+        for (int side = 0; side < 2; side++) {
+            const uint32_t c = child[side];
+            const int k = (int)(c & 0x7FFFFFFFu) - L;
+            const bool leaf = c & 0x80000000u;
+            for (int j = 0; j < 6; j++) b[side][j] = leaf ? s_leaf[j][k] : s_node[j][k];
+        }
+        store_child_boxes(nodes + p, b);
+    }
+    if (BUILD) {
+        if (in) {
+            const uint32_t dep = s_dep[t];
+            node_dep[p] = dep;
+            if (p == 0) *max_depth = (int)dep;
+        }
+        if (cross) {
+            const uint32_t slot = s_base + rank;
+            cross_list[slot] = (uint32_t)p;
+            cross_child[slot] = make_uint2(child[0] | (cross_mark & 1u ? kCrossBit : 0u),
+                                           child[1] | (cross_mark & 2u ? kCrossBit : 0u));
+            slot_of[p] = slot;
+        }
+    }
+}
+
+// The nodes that cross a tile boundary (*cross_count of them, listed by
+// tile_fit_kernel<true>), in one workgroup.  Up to `slots` of them (at most
+// kCrossCap) are fit in LDS: each lane owns kCrossPer slots and gathers once
+// the boxes of their children that are leaves or in-tile nodes (fit by
+// tile_fit_kernel), keeping them in registers for the final writes.  A slot
+// with one such child starts with that child's box and depth in its LDS entry
+// and one arrival counted; a slot with two is fit at once and its lane walks
+// up the crossing nodes.  Beyond `slots`, rounds over the slots with the
+// boxes in global memory (refit_node) and a done word per slot (`done`, the
+// round in which the slot was fit).  BUILD: DFS stack depths as in
+// tile_fit_kernel, the root's to *max_depth.
+constexpr int kCrossBlock = 1024;
+constexpr int kCrossPer = 4;
+constexpr int kCrossCap = kCrossBlock * kCrossPer;
+static_assert(kCrossCap == kLbvhFitSlots, "LDS slots of the crossing-node fit");
+static_assert(kSmallBuild == 131072, "small-tree bound stated in kscr_lbvh.h");
+static_assert(kCrossCap < (int)kNoParent, "slot indices fit 15 bits with the side bit");
+
+__device__ inline bool internal_crosses(const LbvhNode* __restrict__ nodes, uint32_t c) {
+    return !(c & 0x80000000u) && range_crosses(c, nodes[c].fence);
+}
+
+template <bool BUILD>
+__global__ __launch_bounds__(kCrossBlock) void cross_fit_kernel(const float* __restrict__ aabb,
+                                                                const uint32_t* __restrict__ ids,
+                                                                LbvhNode* __restrict__ nodes,
+                                                                uint32_t* __restrict__ node_dep,
+                                                                const uint32_t* __restrict__ cross_list,
+                                                                const uint2* __restrict__ cross_child,
+                                                                const uint32_t* __restrict__ slot_of,
+                                                                const uint32_t* __restrict__ cross_count, int slots,
+                                                                uint32_t* __restrict__ done,
+                                                                int* __restrict__ max_depth) {
+    __shared__ float s_box[6][kCrossCap];         // a slot's union (first: its external child's box)
+    __shared__ uint32_t s_arrive[kCrossCap];
+    __shared__ uint32_t s_ref[kCrossCap];         // crossing children's slots, kNoParent for external (16 bits each)
+    __shared__ uint16_t s_par[kCrossCap];         // parent slot | side << 15
+    __shared__ uint8_t s_dep[kCrossCap];
+    const int t = threadIdx.x;
+    const int count = (int)*cross_count;
+    if (count > slots) {
+        for (int s = t; s < count; s += kCrossBlock) done[s] = 0;
+        __syncthreads();
+        for (uint32_t r = 1; r <= (uint32_t)kMaxLevels + 1; r++) {
+            int pending = 0;
+            for (int s = t; s < count; s += kCrossBlock) {
+                if (done[s]) continue;
+                const uint32_t p = cross_list[s];
+                const uint32_t child[2] = {nodes[p].left, nodes[p].right};
+                bool ready = true;
+                for (int side = 0; side < 2; side++)
+                    if (internal_crosses(nodes, child[side])) {
+                        const uint32_t dc = done[slot_of[child[side]]];
+                        ready = ready && dc != 0 && dc < r;  // fit in an earlier round
+                    }
+                if (!ready) {
+                    pending = 1;
+                    continue;
+                }
+                refit_node(aabb, ids, nodes, p);
+                if (BUILD) {
+                    uint32_t d[2];
+                    for (int side = 0; side < 2; side++)
+                        d[side] = (child[side] & 0x80000000u) ? 1u : node_dep[child[side]];
+                    const uint32_t dep = d[0] > d[1] + 1 ? d[0] : d[1] + 1;
+                    node_dep[p] = dep;
+                    if (p == 0) *max_depth = (int)dep;
+                }
+                done[s] = r;
+            }
+            if (!__syncthreads_or(pending)) break;
+        }
+        return;
+    }
+    uint32_t node[kCrossPer], dep_ext[kCrossPer];
+    int ref[kCrossPer][2];
+    float ext[kCrossPer][6];
+    bool start[kCrossPer];
+#pragma unroll
+    for (int j = 0; j < kCrossPer; j++) {
+        const int s = t + j * kCrossBlock;
+        start[j] = false;
+        node[j] = 0;
+        dep_ext[j] = 0;
+        ref[j][0] = ref[j][1] = -1;
+        for (int k = 0; k < 6; k++) ext[j][k] = 0.0f;
+        if (s < count) s_par[s] = kNoParent;
+    }
+    lds_barrier();
+    // Two slots per lane at a time, in three steps whose loads are all in
+    // flight together: the slot's node and child words; per child its slot
+    // (crossing), object id (leaf) or boxes and depth (in-tile node); the
+    // leaves' boxes.
+#pragma unroll
+    for (int jb = 0; jb < kCrossPer; jb += 2) {
+        if (jb * kCrossBlock >= count) break;
+        bool valid[2];
+        uint32_t cw[2][2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int s = t + (jb + q) * kCrossBlock;
+            valid[q] = s < count;
+            const uint2 c = valid[q] ? cross_child[s] : make_uint2(0u, 0u);
+            node[jb + q] = valid[q] ? cross_list[s] : 0u;
+            cw[q][0] = c.x;
+            cw[q][1] = c.y;
+        }
+        float b[2][2][12];
+        uint32_t aux[2][2];
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                const uint32_t c = cw[q][side];
+                aux[q][side] = 0;
+                for (int k = 0; k < 12; k++) b[q][side][k] = 0.0f;
+                if (!valid[q]) continue;
+                if (c & 0x80000000u) {
+                    aux[q][side] = ids[c & 0x7FFFFFFFu];
+                } else if (c & kCrossBit) {
+                    aux[q][side] = slot_of[c & ~kCrossBit];
+                } else {
+                    const float4* cb = reinterpret_cast<const float4*>(nodes[c].b[0]);
+                    const float4 x0 = cb[0], x1 = cb[1], x2 = cb[2];
+                    const float v[12] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w, x2.x, x2.y, x2.z, x2.w};
+                    for (int k = 0; k < 12; k++) b[q][side][k] = v[k];
+                    aux[q][side] = node_dep[c];
+                }
+            }
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int side = 0; side < 2; side++)
+                if (valid[q] && (cw[q][side] & 0x80000000u)) {
+                    const float* a = aabb + 6 * (size_t)aux[q][side];
+                    for (int k = 0; k < 6; k++) b[q][side][k] = a[k];
+                }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int j = jb + q;
+            if (!valid[q]) continue;
+            const int s = t + j * kCrossBlock;
+            float box[2][6];
+            uint32_t d[2];
+            for (int side = 0; side < 2; side++) {
+                const uint32_t c = cw[q][side];
+                d[side] = 1;
+                if (c & 0x80000000u) {
+                    for (int k = 0; k < 6; k++) box[side][k] = b[q][side][k];
+                } else if (c & kCrossBit) {
+                    ref[j][side] = (int)aux[q][side];
+                    s_par[aux[q][side]] = (uint16_t)(s | (side << 15));
+                    for (int k = 0; k < 6; k++) box[side][k] = 0.0f;
+                } else {
+                    child_union(&b[q][side][0], &b[q][side][6], box[side]);
+                    d[side] = aux[q][side];
+                }
+            }
+            s_ref[s] = (uint32_t)(ref[j][0] < 0 ? kNoParent : ref[j][0]) |
+                       ((uint32_t)(ref[j][1] < 0 ? kNoParent : ref[j][1]) << 16);
+            if (ref[j][0] < 0 && ref[j][1] < 0) {
+                store_child_boxes(nodes + node[j], box);
+                float u[6];
+                child_union(box[0], box[1], u);
+                for (int k = 0; k < 6; k++) s_box[k][s] = u[k];
+                const uint32_t dep = d[0] > d[1] + 1 ? d[0] : d[1] + 1;
+                s_dep[s] = (uint8_t)dep;
+                s_arrive[s] = 0;
+                start[j] = true;
+                if (BUILD && node[j] == 0) *max_depth = (int)dep;
+            } else {
+                const int e = ref[j][0] < 0 ? 0 : 1;  // the external side, if any
+                for (int k = 0; k < 6; k++) ext[j][k] = box[e][k];
+                dep_ext[j] = d[e];
+                const bool one = ref[j][0] < 0 || ref[j][1] < 0;
+                if (one) {
+                    for (int k = 0; k < 6; k++) s_box[k][s] = box[e][k];
+                    s_dep[s] = (uint8_t)d[e];
+                }
+                s_arrive[s] = one ? 1u : 0u;  // the external child counts as arrived
+            }
+        }
+    }
+    lds_barrier();
+    // Walks from the slots fit above, one after another on each lane (the
+    // next starts as soon as one ends).  The walker holds the union it
+    // brings; at the parent it reads the other side's box (a crossing child's
+    // union, or the external child's box staged in the parent's own entry).
+    // The parent's links are read before the arrival: they do not change
+    // during the walks.
+    {
+        uint32_t todo = 0;
+#pragma unroll
+        for (int j = 0; j < kCrossPer; j++) todo |= start[j] ? 1u << j : 0u;
+        uint32_t up = kNoParent, dep = 0;
+        float own[6] = {};
+        // each walk is at most kMaxLevels steps: the bound only ends the loop on a corrupt tree
+        for (int step = 0; step < kCrossPer * (kMaxLevels + 2); step++) {
+            if (up == kNoParent) {
+                if (!todo) break;
+                const int s = t + __builtin_ctz(todo) * kCrossBlock;
+                todo &= todo - 1;
+                for (int i = 0; i < 6; i++) own[i] = s_box[i][s];
+                dep = s_dep[s];
+                up = s_par[s];
+                continue;
+            }
+            const int q = (int)(up & 0x7FFFu);
+            const int side = (int)(up >> 15);
+            const uint32_t rw = s_ref[q];
+            const uint32_t next = s_par[q];
+            if (lds_arrive(&s_arrive[q]) == 0) {  // the other child goes on
+                up = kNoParent;
+                continue;
+            }
+            const uint32_t other = side ? (rw & 0xFFFFu) : (rw >> 16);
+            const int k = other == kNoParent ? q : (int)other;
+            // left and right by select (an index by `side` would select
+            // over every element)
+            float bl[6], br[6];
+            for (int i = 0; i < 6; i++) {
+                const float o = s_box[i][k];
+                bl[i] = side ? o : own[i];
+                br[i] = side ? own[i] : o;
+            }
+            const uint32_t ds = s_dep[k];
+            const uint32_t d0 = side ? ds : dep, d1 = side ? dep : ds;
+            child_union(bl, br, own);
+            for (int i = 0; i < 6; i++) s_box[i][q] = own[i];
+            dep = d0 > d1 + 1 ? d0 : d1 + 1;
+            s_dep[q] = (uint8_t)dep;
+            up = next;
+            if (BUILD && up == kNoParent) *max_depth = (int)dep;  // the root is the only slot without one
+        }
+    }
+    lds_barrier();
+    // the child boxes of the slots with a crossing child, from the finished unions
+#pragma unroll
+    for (int j = 0; j < kCrossPer; j++) {
+        if (ref[j][0] < 0 && ref[j][1] < 0) continue;  // fit above, or no slot
+        float b[2][6];
+        for (int side = 0; side < 2; side++) {
+            const int r = ref[j][side];
+            for (int k = 0; k < 6; k++) b[side][k] = r >= 0 ? s_box[k][r] : ext[j][k];
+        }
+        store_child_boxes(nodes + node[j], b);
     }
 }
 
@@ -898,14 +1442,21 @@ void lbvh_release(LbvhDev& d) {
                   d.redo};
     for (void* p : ps)
         if (p) (void)hipFree(p);
+    const int slots = d.fit_slots;
     d = LbvhDev{};
+    d.fit_slots = slots;
 }
+
+// Digit-count words of the sort: the one-launch-per-pass layout of small
+// sorts (four passes of kFusedHist) or 256 per tile, whichever is larger, so
+// a reservation for n also fits every smaller n.
+static size_t hist_words(int tiles) { return std::max((size_t)4 * kFusedHist, (size_t)256 * tiles); }
 
 hipError_t lbvh_reserve(LbvhDev& d, int n) {
     if (n <= d.cap) return hipSuccess;
     lbvh_release(d);
     const int tiles = (n + kTile - 1) / kTile;
-    const size_t hist_total = (size_t)256 * tiles;
+    const size_t hist_total = hist_words(tiles);
     const size_t chunks = (hist_total + kScanChunk - 1) / kScanChunk;
     hipError_t e;
     if ((e = dmalloc(d.aabb, (size_t)6 * n)) != hipSuccess || (e = dmalloc(d.codes, n)) != hipSuccess ||
@@ -929,7 +1480,7 @@ hipError_t lbvh_reserve(LbvhDev& d, int n) {
 
 size_t lbvh_carve(LbvhDev& d, int n, char* base, size_t bytes) {
     const int tiles = (n + kTile - 1) / kTile;
-    const size_t hist_total = (size_t)256 * tiles;
+    const size_t hist_total = hist_words(tiles);
     const size_t chunks = (hist_total + kScanChunk - 1) / kScanChunk;
     size_t off = 0;
     LbvhDev v;
@@ -960,6 +1511,7 @@ size_t lbvh_carve(LbvhDev& d, int n, char* base, size_t bytes) {
     take(v.level_range, (size_t)kMaxLevels);
     v.redo_count = v.count ? v.count + 1 : nullptr;
     if (base && off <= bytes) {
+        v.fit_slots = d.fit_slots;
         d = v;
         d.cap = n;
         d.n = 0;
@@ -997,23 +1549,41 @@ static unsigned level_blocks(int n, int h) {
     return (unsigned)std::max(8.0, std::min(b, 65536.0));
 }
 
+// The crossing list's child words of a small tree live in the query's redo
+// buffer (n int2, unused outside a query), and the global-memory fit's done
+// words in the sort's second key buffer (free after the sort).
+static uint2* cross_child(const LbvhDev& d) { return reinterpret_cast<uint2*>(d.redo); }
+
 hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
     if (n > d.cap) return hipErrorInvalidValue;
     d.n = n;
     d.built = false;
     d.max_depth_host = -1;
     if (n <= 0) return hipSuccess;
+    const float* src = d.boxes();
     const int nb = (n + 255) / 256;
     const int pblocks = nb < 1024 ? nb : 1024;
-    hipLaunchKernelGGL(root_partial_kernel, dim3(pblocks), dim3(256), 0, stream, d.aabb, n, d.partial);
+    hipLaunchKernelGGL(root_partial_kernel, dim3(pblocks), dim3(256), 0, stream, src, n, d.partial);
     const int tiles = (n + kTile - 1) / kTile;
     d.tiles = tiles;
-    hipLaunchKernelGGL(morton_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, d.aabb, d.partial, pblocks, n,
-                       tiles, d.root, d.codes, d.ids, d.hist);
+    const bool small = tiles <= kFusedTiles;  // n <= kSmallBuild
+    d.small = small;
+    hipLaunchKernelGGL(morton_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, src, d.partial, pblocks, n,
+                       small ? kFusedTiles : tiles, d.root, d.codes, d.ids, d.hist,
+                       small ? d.hist + kFusedHist : nullptr, small ? d.level_count : nullptr);
     const size_t total = (size_t)256 * tiles;
     const int chunks = (int)((total + kScanChunk - 1) / kScanChunk);
     uint32_t *kin = d.codes, *vin = d.ids, *kout = d.codes_tmp, *vout = d.ids_tmp;
     for (int shift = 0; shift < 32; shift += 8) {
+        if (small) {
+            // one launch per pass; each pass counts the next pass's digits
+            uint32_t* hist = d.hist + (size_t)(shift / 8) * kFusedHist;
+            hipLaunchKernelGGL(radix_scatter_kernel<true>, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, n,
+                               shift, tiles, hist, kout, vout, shift < 24 ? hist + kFusedHist : nullptr);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+            continue;
+        }
         if (shift > 0)  // the first pass's histogram comes from morton_hist_kernel
             hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, n, shift, tiles,
                                d.hist);
@@ -1024,14 +1594,10 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
             hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, stream, d.hist_sums, chunks);
             hipLaunchKernelGGL(scan_add_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
         }
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, n, shift, tiles,
-                           d.hist, kout, vout);
-        uint32_t* t1 = kin;
-        kin = kout;
-        kout = t1;
-        t1 = vin;
-        vin = vout;
-        vout = t1;
+        hipLaunchKernelGGL(radix_scatter_kernel<false>, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, n, shift,
+                           tiles, d.hist, kout, vout, nullptr);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
     }
     // four passes: the sorted data is back in d.codes / d.ids
     hipError_t e;
@@ -1039,6 +1605,19 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
         d.built = true;
         return fill_i32(d.max_depth, 1, 1, stream);
+    }
+    if (small) {
+        hipLaunchKernelGGL(tile_fit_kernel<true>, dim3((n + kRefitTile - 1) / kRefitTile), dim3(kRefitTile), 0, stream,
+                           d.codes, src, d.ids, d.nodes, d.leaf_parent, n, d.node_height, d.level_nodes,
+                           cross_child(d), (uint32_t*)d.flags, d.level_count, d.max_depth);
+        if (n > kRefitTile)
+            hipLaunchKernelGGL(cross_fit_kernel<true>, dim3(1), dim3(kCrossBlock), 0, stream, src, d.ids, d.nodes,
+                               d.node_height, d.level_nodes, cross_child(d), (const uint32_t*)d.flags, d.level_count,
+                               std::clamp(d.fit_slots, 0, kCrossCap), d.codes_tmp,
+                               d.max_depth);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        d.built = true;
+        return hipSuccess;
     }
     const int m = n - 1;
     const int bthreads = std::max(m, 2 * kMaxLevels);
@@ -1064,13 +1643,24 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
 hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
     if (!d.built) return hipErrorInvalidValue;
     if (d.n < 2) return hipSuccess;
+    const float* src = d.boxes();
+    if (d.small) {
+        hipLaunchKernelGGL(tile_fit_kernel<false>, dim3((d.n + kRefitTile - 1) / kRefitTile), dim3(kRefitTile), 0,
+                           stream, nullptr, src, d.ids, d.nodes, nullptr, d.n, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, nullptr);
+        if (d.n > kRefitTile)
+            hipLaunchKernelGGL(cross_fit_kernel<false>, dim3(1), dim3(kCrossBlock), 0, stream, src, d.ids, d.nodes,
+                               d.node_height, d.level_nodes, cross_child(d), (const uint32_t*)d.flags, d.level_count,
+                               std::clamp(d.fit_slots, 0, kCrossCap), d.codes_tmp, nullptr);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(refit_tile_kernel, dim3((unsigned)((d.n + kRefitTile - 1) / kRefitTile)), dim3(kRefitTile), 0,
-                       stream, d.aabb, d.ids, d.nodes, d.node_height, d.n);
+                       stream, src, d.ids, d.nodes, d.node_height, d.n);
     const int K = bottom_levels(d.n);
     for (int h = 1; h <= K; h++)
-        hipLaunchKernelGGL(refit_level_kernel, dim3(level_blocks(d.n, h)), dim3(256), 0, stream, d.aabb, d.ids,
+        hipLaunchKernelGGL(refit_level_kernel, dim3(level_blocks(d.n, h)), dim3(256), 0, stream, src, d.ids,
                            d.nodes, d.level_nodes, d.level_range, h);
-    hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, d.aabb, d.ids, d.nodes, d.level_nodes,
+    hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(kTopBlock), 0, stream, src, d.ids, d.nodes, d.level_nodes,
                        d.level_range, K + 1);
     return hipGetLastError();
 }
@@ -1098,11 +1688,12 @@ hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error,
     hipError_t e = fill_i32(d.count, 0, error == d.count + 2 ? 3 : 2, stream);
     if (e != hipSuccess) return e;
     if (d.n < 2) return hipSuccess;
+    const float* src = d.boxes();
     const dim3 grid((d.n + kQueryBlock - 1) / kQueryBlock);
     if (d.n <= kOnePassMax) {
         // a Karras tree over 62-bit keys is at most 63 levels tall: a
         // kQueryStack-entry stack never overflows
-        hipLaunchKernelGGL((query_kernel<kQueryStack, false>), grid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids,
+        hipLaunchKernelGGL((query_kernel<kQueryStack, false>), grid, dim3(kQueryBlock), 0, stream, src, d.ids,
                            d.nodes, d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
         return hipGetLastError();
     }
@@ -1110,10 +1701,10 @@ hipError_t lbvh_query(LbvhDev& d, int2* pairs, int cap, int min_gap, int* error,
     // LDS stack each; the few queries that need more are redone with the full
     // stack (the redo pass returns at once when none was listed).
     const dim3 grid2((d.n + 2 * kQueryBlock - 1) / (2 * kQueryBlock));
-    hipLaunchKernelGGL((query2_kernel<kQueryStackFirst2>), grid2, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes,
+    hipLaunchKernelGGL((query2_kernel<kQueryStackFirst2>), grid2, dim3(kQueryBlock), 0, stream, src, d.ids, d.nodes,
                        d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
     const dim3 rgrid(std::min<unsigned>(grid.x, kRedoBlocks));
-    hipLaunchKernelGGL((query_kernel<kQueryStack, true>), rgrid, dim3(kQueryBlock), 0, stream, d.aabb, d.ids, d.nodes,
+    hipLaunchKernelGGL((query_kernel<kQueryStack, true>), rgrid, dim3(kQueryBlock), 0, stream, src, d.ids, d.nodes,
                        d.n, pairs, cap, min_gap, d.count, error, d.redo, d.redo_count, d.n);
     return hipGetLastError();
 }

@@ -87,8 +87,8 @@ extern "C" int kscr_lbvh_self_query_static(const float* d_aabbs, int n, int* d_p
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
     // the query's error word is the third word of the tree's counter block
-    if (hipMemcpyAsync(d.aabb, d_aabbs, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        lbvh_build(d, n, s) != hipSuccess ||
+    d.src = d_aabbs;  // the boxes are read in place
+    if (lbvh_build(d, n, s) != hipSuccess ||
         lbvh_query(d, reinterpret_cast<int2*>(d_pairs), (int)cap, -1, d.count + 2, s) != hipSuccess) {
         set_err(err, err_len, "kscr_lbvh_self_query_static: kernel launch failed");
         return KSCR_ERR_HIP;

@@ -7,16 +7,18 @@ import time
 import numpy as np
 import pytest
 
-from kamisimulator_amd._lib import Lbvh
+from kamisimulator_amd._lib import KscrError, Lbvh
 from oracle.pyoracle import OracleLbvh
 from tests.lbvh_cases import query_stack_need, segment_boxes, recipe_points
 
 pytestmark = pytest.mark.gpu
 
 
-def _compare(boxes, label, cap=None):
+def _compare(boxes, label, cap=None, slots=None):
     print(f"[gpu-lbvh] {label}: n={len(boxes)}", flush=True)
     g = Lbvh(device=0)
+    if slots is not None:
+        g.set_fit_slots(slots)
     t0 = time.perf_counter()
     g.compute(boxes)
     t1 = time.perf_counter()
@@ -57,6 +59,50 @@ def test_segments_and_refit():
     tg, to = g.get(), o.get()
     assert np.array_equal(tg["bounds"].view(np.uint32), to["bounds"].view(np.uint32))
     assert tg["max_stack"] == to["max_stack"]
+
+
+def _refit_compare(g, o, boxes):
+    g.refit(boxes)
+    o.refit(boxes)
+    tg, to = g.get(), o.get()
+    assert np.array_equal(tg["bounds"].view(np.uint32), to["bounds"].view(np.uint32))
+    assert np.array_equal(tg["root"].view(np.uint32), to["root"].view(np.uint32))
+    assert tg["max_stack"] == to["max_stack"]
+
+
+@pytest.mark.parametrize("n", [2, 3, 511, 512, 513, 2049, 131072, 131073])
+def test_sizes_around_the_small_tree_bound(n):
+    """Trees over at most 131072 boxes sort with one launch per digit pass and
+    fit by rounds (the nodes inside each 512-leaf tile, then the nodes that
+    cross a tile boundary, from LDS); larger ones take the multi-launch sort
+    and the height-ordered refit.  Both against the restatement at the tile
+    (512), radix-tile (2048) and small-tree bounds, with empty boxes (+inf,
+    -inf) among the objects, then a refit with moved boxes."""
+    rng = np.random.default_rng(n)
+    P = rng.random((n, 3)).astype(np.float32)
+    boxes = np.concatenate([P - 1e-3, P + 1e-3], 1).astype(np.float32)
+    if n > 64:
+        boxes[5::61] = [np.inf, np.inf, np.inf, -np.inf, -np.inf, -np.inf]
+    g, o = _compare(boxes, f"uniform n={n}")
+    moved = boxes + np.repeat(rng.normal(scale=2e-3, size=(n, 3)), 2, axis=0).reshape(n, 6).astype(np.float32)
+    _refit_compare(g, o, moved)
+
+
+def test_crossing_nodes_fit_from_global_memory():
+    """With fewer LDS slots than tile-crossing nodes (about 1.8k on the
+    recipe; 0 slots here) the small-tree fit runs its rounds from global
+    memory: the same tree, boxes and stack bound, also after a refit."""
+    boxes = recipe_points(100_000)
+    g, o = _compare(boxes, "testLBVH recipe, crossing nodes from global memory", slots=0)
+    rng = np.random.default_rng(2)
+    moved = boxes + np.repeat(rng.normal(scale=1e-3, size=(len(boxes), 3)), 2, axis=0).reshape(-1, 6).astype(
+        np.float32)
+    _refit_compare(g, o, moved)
+    g.set_fit_slots(4096)  # back to LDS for the same tree
+    _refit_compare(g, o, boxes)
+    for bad in (-1, 4097):
+        with pytest.raises(KscrError, match="INVALID_ARGUMENT"):
+            g.set_fit_slots(bad)
 
 
 def test_ties_and_one_object():
