@@ -19,7 +19,7 @@ LIB = os.path.join(PKG, "libkscr.so")
 SOURCES = ["kscr_api.hip", "kscr_api_adjoint.hip", "kscr_api_batch.hip", "kscr_api_lbvh.hip", "kscr_step.hip",
            "kscr_colored.hip", "kscr_adjoint.hip", "kscr_contact.hip", "kscr_lbvh.hip", "kscr_sagfree.cpp",
            "kscr_meshsdf.hip", "kscr_selftest.hip", "kscr_lbvh_static.hip", "kscr_strain.hip"]
-HEADERS = ["kscr_math.h", "kscr_device.h", "kscr_solve.h", "kscr_contact.h", "kscr_lbvh.h", "kscr_api_internal.h"]
+HEADERS = ["kscr_math.h", "kscr_device.h", "kscr_solve.h", "kscr_contact.h", "kscr_lbvh.h", "kscr_api_internal.h", "kscr_sync.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -16,6 +16,7 @@
 #include "kscr_contact.h"
 #include "kscr_math.h"
 #include "kscr_solve.h"
+#include "kscr_sync.h"
 
 namespace kscr {
 namespace {
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kThreads) void substep_begin_kernel(StepArgs a, Con
         s_d[0][t] = dx.x;
         s_d[1][t] = dx.y;
         s_d[2][t] = dx.z;
-        __syncthreads();
+        lds_barrier();  // the state stores above are not read here
         const bool seg = (fl & FLAG_NEXT) != 0;
         if (act && (!seg || t + 1 < kThreads)) {
             const int tn = seg ? t + 1 : t;
@@ -380,7 +381,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         for (int c = 0; c < 6; c++) s_H2[c][t] = H2.d[c];
         s_lim[t] = s.limit;
     }
-    __syncthreads();
+    // the kernel reads none of its global writes: its barriers order LDS only
+    lds_barrier();
     f3 out = dxi;  // a pinned vertex keeps its iterate
     if (free) {
         f3 f = s.f0 + s.f1;
@@ -410,7 +412,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
     s_dxn[0][t] = out.x;
     s_dxn[1][t] = out.y;
     s_dxn[2][t] = out.z;
-    __syncthreads();
+    lds_barrier();
     if (!own) return;
     const f4 q = to4(q_in[i]);
     f4 qn = q;

@@ -9,6 +9,7 @@
 #include <cstdio>
 
 #include "kscr_lbvh.h"
+#include "kscr_sync.h"
 
 #include <type_traits>
 
@@ -52,16 +53,6 @@ __device__ inline int cvt_rzi(float f) {
 
 __device__ inline bool intersects(const float* a, const float* b) {
     return !(a[3] <= b[0] || a[4] <= b[1] || a[5] <= b[2] || a[0] >= b[3] || a[1] >= b[4] || a[2] >= b[5]);
-}
-
-// Workgroup barrier that orders LDS only: it waits for the wave's LDS
-// accesses, not for its global stores and atomics (which __syncthreads waits
-// for).  For kernels that never read in the kernel what they write to global
-// memory.
-__device__ inline void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Whether any wave of the workgroup passed a nonzero word to any_wave_flag's
