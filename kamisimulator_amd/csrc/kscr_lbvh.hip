@@ -267,9 +267,9 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t* __restrict__ v,
         if (base + k < total) v[base + k] += add;
 }
 
-// Stable scatter: within a tile, items keep their order (round, wave, lane);
-// tiles keep their order through the digit-major, tile-minor offsets.  The
-// kernel reads none of its global writes, so its barriers order LDS only.
+// Stable scatter: within a tile, items keep their index order; tiles keep
+// their order through the digit-major, tile-minor offsets.  The kernel reads
+// none of its global writes, so its barriers order LDS only.
 // FUSED = false: offsets is the exclusive scan of the per-tile digit counts
 // (digit-major, `tiles` per digit).
 // FUSED = true (at most kFusedTiles tiles): offsets holds the per-tile digit
@@ -293,16 +293,23 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    // every round's keys and values are loaded before the first is used, so
-    // their latency is paid once, not once per round
-    const int base = blockIdx.x * kTile;
+    // Wave w owns the tile's keys w * kWaveKeys .. (w + 1) * kWaveKeys - 1,
+    // loaded before any is used (one latency, not one per round).  It ranks
+    // them round by round (64 keys a round) against its own digit counts in
+    // LDS, wave-synchronously; the waves' counts then give each wave's offset
+    // per digit.  A key's place is (digits below, earlier tiles, earlier
+    // waves, earlier rounds, earlier lanes) with the same digit: its index
+    // order, so the pass is stable.
+    constexpr int kWaveKeys = kSortItems * 64;
+    const int base = blockIdx.x * kTile + wave * kWaveKeys;
     uint32_t kreg[kSortItems], vreg[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        const int i = base + r * kSortThreads + t;
+        const int i = base + r * 64 + lane;
         kreg[r] = i < n ? keys[i] : 0u;
         vreg[r] = i < n ? vals[i] : 0u;
     }
+    for (int w = 0; w < kWaves; w++) wave_cnt[w][t] = 0;
     if (FUSED) {
         // entries past `tiles` are never written: masked, not summed
         const uint32_t* row = offsets + (size_t)t * kFusedTiles;
@@ -333,15 +340,13 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
     } else {
         running[t] = offsets[(size_t)t * tiles + blockIdx.x];
     }
+    lds_barrier();  // the wave counts are zero
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t rank[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        for (int w = 0; w < kWaves; w++) wave_cnt[w][t] = 0;
-        lds_barrier();
-        const int i = base + r * kSortThreads + t;
-        const bool valid = i < n;
-        const uint32_t key = kreg[r], val = vreg[r];
-        const uint32_t digit = valid ? (key >> shift) & 255u : 0u;
+        const bool valid = base + r * 64 + lane < n;
+        const uint32_t digit = valid ? (kreg[r] >> shift) & 255u : 0u;
         uint64_t match = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -349,22 +354,32 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
             const uint64_t ones = __ballot(valid && bit);
             match &= bit ? ones : ~ones;
         }
-        const uint32_t rank = (uint32_t)__popcll(match & lt_mask);
-        if (valid && (match & lt_mask) == 0) wave_cnt[wave][digit] = (uint32_t)__popcll(match);
-        lds_barrier();
-        if (valid) {
-            uint32_t off = running[digit] + rank;
-            for (int w = 0; w < wave; w++) off += wave_cnt[w][digit];
-            keys_out[off] = key;
-            vals_out[off] = val;
-            if (FUSED && hist_next)
-                atomicAdd(hist_next + ((key >> (shift + 8)) & 255u) * kFusedTiles + off / kTile, 1u);
+        const uint32_t earlier = wave_cnt[wave][digit];  // this digit in the wave's earlier rounds
+        rank[r] = earlier + (uint32_t)__popcll(match & lt_mask);
+        __builtin_amdgcn_wave_barrier();  // every lane has read before the count moves on
+        if (valid && (match & lt_mask) == 0) wave_cnt[wave][digit] = earlier + (uint32_t)__popcll(match);
+        __builtin_amdgcn_wave_barrier();
+    }
+    lds_barrier();
+    {  // each wave's offset per digit: the counts of the waves before it
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            const uint32_t c = wave_cnt[w][t];
+            wave_cnt[w][t] = acc;
+            acc += c;
         }
-        lds_barrier();
-        uint32_t add = 0;
-        for (int w = 0; w < kWaves; w++) add += wave_cnt[w][t];
-        running[t] += add;
-        lds_barrier();
+    }
+    lds_barrier();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        if (base + r * 64 + lane >= n) continue;
+        const uint32_t key = kreg[r];
+        const uint32_t digit = (key >> shift) & 255u;
+        const uint32_t off = running[digit] + wave_cnt[wave][digit] + rank[r];
+        keys_out[off] = key;
+        vals_out[off] = vreg[r];
+        if (FUSED && hist_next) atomicAdd(hist_next + ((key >> (shift + 8)) & 255u) * kFusedTiles + off / kTile, 1u);
     }
 }
 
