@@ -21,10 +21,16 @@ constexpr int kSortItems = 8;
 constexpr int kTile = kSortThreads * kSortItems;  // keys per radix tile
 constexpr int kScanChunk = 1024;
 // Sorts of at most kFusedTiles tiles run one launch per digit pass
-// (radix_scatter_kernel<true>); their per-tile digit counts are kept
-// digit-major with kFusedTiles entries per digit, one such block per pass.
+// (radix_scatter_kernel<true, kFusedBits>): four passes of 8-bit digits
+// cover the 30-bit Morton codes (three 10-bit passes measured 21 us each
+// against 14: every workgroup reads four times the digit counts).  Their
+// per-tile digit counts are kept digit-major with kFusedTiles entries per
+// digit, one such block per pass.  Larger sorts also take 8-bit passes.
 constexpr int kFusedTiles = 64;
-constexpr int kFusedHist = 256 * kFusedTiles;
+constexpr int kFusedBits = 8;
+constexpr int kFusedPasses = 4;
+constexpr int kFusedHist = (1 << kFusedBits) * kFusedTiles;
+static_assert(kFusedBits * kFusedPasses >= 30, "the passes cover the Morton code");
 constexpr int kQueryBlock = 128;
 constexpr int kQueryStack = 64;
 constexpr int kQueryStackFirst = 12;  // first pass; deeper queries are redone
@@ -128,25 +134,36 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
 // with the final reduction of the root box (every workgroup reduces the
 // partials of root_partial_kernel; min and max give the same box in any
 // order) and with the first sort pass's digit histogram of the tile
-// (hist[digit * stride + tile]).  For a one-launch-per-pass sort, each
-// workgroup also clears its tile's counts of the three later passes
+// (hist[digit * stride + tile], BITS-bit digits).  For a one-launch-per-pass
+// sort, each workgroup also clears its tile's counts of the later passes
 // (hist_next, kFusedHist words per pass) and workgroup 0 the word at zero_word.
+template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void morton_hist_kernel(const float* __restrict__ aabb,
                                                                    const float* __restrict__ partial, int pblocks,
                                                                    int n, int stride, float* __restrict__ root_out,
                                                                    uint32_t* __restrict__ codes,
                                                                    uint32_t* __restrict__ ids,
                                                                    uint32_t* __restrict__ hist,
-                                                                   uint32_t* __restrict__ hist_next,
+                                                                   uint32_t* __restrict__ hist_next, int next_passes,
                                                                    uint32_t* __restrict__ zero_word) {
+    constexpr int D = 1 << BITS;
     __shared__ float s[6][kSortThreads];
-    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t cnt[D];
     const int t = threadIdx.x;
+    const int base = blockIdx.x * kTile;
+    // the tile's boxes are loaded before the root box is reduced, so the two
+    // loads' latencies overlap
+    float box[kSortItems][6];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int i = base + r * kSortThreads + t;
+        for (int k = 0; k < 6; k++) box[r][k] = i < n ? aabb[6 * (size_t)i + k] : 0.0f;
+    }
     float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (int b = t; b < pblocks; b += kSortThreads)
         for (int k = 0; k < 6; k++) v[k] = k < 3 ? fminf(v[k], partial[6 * b + k]) : fmaxf(v[k], partial[6 * b + k]);
     for (int k = 0; k < 6; k++) s[k][t] = v[k];
-    cnt[t] = 0;
+    for (int d = t; d < D; d += kSortThreads) cnt[d] = 0;
     __syncthreads();
     for (int w = kSortThreads / 2; w > 0; w >>= 1) {
         if (t < w)
@@ -157,20 +174,22 @@ __global__ __launch_bounds__(kSortThreads) void morton_hist_kernel(const float* 
     float root[6];
     for (int k = 0; k < 6; k++) root[k] = s[k][0];
     if (blockIdx.x == 0 && t < 6) root_out[t] = root[t];
-    const int base = blockIdx.x * kTile;
+#pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         const int i = base + r * kSortThreads + t;
         if (i < n) {
-            const uint32_t code = morton_code(aabb + 6 * (size_t)i, root);
+            const uint32_t code = morton_code(box[r], root);
             codes[i] = code;
             ids[i] = (uint32_t)i;
-            atomicAdd(&cnt[code & 255u], 1u);
+            atomicAdd(&cnt[code & (uint32_t)(D - 1)], 1u);
         }
     }
     __syncthreads();
-    hist[(size_t)t * stride + blockIdx.x] = cnt[t];
+    for (int d = t; d < D; d += kSortThreads) hist[(size_t)d * stride + blockIdx.x] = cnt[d];
     if (hist_next)
-        for (int k = 0; k < 3; k++) hist_next[(size_t)k * kFusedHist + t * kFusedTiles + blockIdx.x] = 0u;
+        for (int k = 0; k < next_passes; k++)
+            for (int d = t; d < D; d += kSortThreads)
+                hist_next[(size_t)k * kFusedHist + d * kFusedTiles + blockIdx.x] = 0u;
     if (zero_word && blockIdx.x == 0 && t == 0) *zero_word = 0u;
 }
 
@@ -269,9 +288,10 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t* __restrict__ v,
 
 // Stable scatter: within a tile, items keep their index order; tiles keep
 // their order through the digit-major, tile-minor offsets.  The kernel reads
-// none of its global writes, so its barriers order LDS only.
-// FUSED = false: offsets is the exclusive scan of the per-tile digit counts
-// (digit-major, `tiles` per digit).
+// none of its global writes, so its barriers order LDS only.  BITS-bit
+// digits at `shift`.
+// FUSED = false (8-bit digits): offsets is the exclusive scan of the per-tile
+// digit counts (digit-major, `tiles` per digit).
 // FUSED = true (at most kFusedTiles tiles): offsets holds the per-tile digit
 // counts themselves (kFusedTiles per digit) and every workgroup forms its own
 // offsets from them: the digit totals below its digit plus its digit's counts
@@ -279,16 +299,21 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t* __restrict__ v,
 // beforehand), each key also counts its next digit in the tile of its
 // destination, one global atomic per key: the next pass's per-tile counts,
 // so the pass needs no launch of its own for them.
-template <bool FUSED>
+template <bool FUSED, int BITS>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint32_t* __restrict__ keys,
                                                                      const uint32_t* __restrict__ vals, int n, int shift,
                                                                      int tiles, const uint32_t* __restrict__ offsets,
                                                                      uint32_t* __restrict__ keys_out,
                                                                      uint32_t* __restrict__ vals_out,
                                                                      uint32_t* __restrict__ hist_next) {
+    constexpr int D = 1 << BITS;
+    constexpr int DPT = D / kSortThreads;  // digits per thread, consecutive
+    static_assert(D % kSortThreads == 0, "whole digits per thread");
+    static_assert(FUSED || DPT == 1, "the scanned offsets are one digit per thread");
+    constexpr uint32_t kMask = D - 1;
     constexpr int kWaves = kSortThreads / 64;
-    __shared__ uint32_t running[256];
-    __shared__ uint32_t wave_cnt[kWaves][256];
+    __shared__ uint32_t running[D];
+    __shared__ uint32_t wave_cnt[kWaves][D];
     __shared__ uint32_t wave_sum[kWaves];
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -309,24 +334,33 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
         kreg[r] = i < n ? keys[i] : 0u;
         vreg[r] = i < n ? vals[i] : 0u;
     }
-    for (int w = 0; w < kWaves; w++) wave_cnt[w][t] = 0;
+    for (int w = 0; w < kWaves; w++)
+        for (int k = 0; k < DPT; k++) wave_cnt[w][t * DPT + k] = 0;
     if (FUSED) {
-        // entries past `tiles` are never written: masked, not summed
-        const uint32_t* row = offsets + (size_t)t * kFusedTiles;
-        uint32_t before = 0, total = 0;
+        // this thread's digits: their totals and their counts in the earlier
+        // tiles (entries past `tiles` are never written: masked, not summed)
+        uint32_t before[DPT], total[DPT];
 #pragma unroll
-        for (int b = 0; b < kFusedTiles; b += 4) {
-            const uint4 v = *reinterpret_cast<const uint4*>(row + b);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        for (int k = 0; k < DPT; k++) {
+            const uint32_t* row = offsets + (size_t)(t * DPT + k) * kFusedTiles;
+            before[k] = total[k] = 0;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t x = b + k < tiles ? w[k] : 0u;
-                total += x;
-                before += b + k < (int)blockIdx.x ? x : 0u;
+            for (int b = 0; b < kFusedTiles; b += 4) {
+                const uint4 v = *reinterpret_cast<const uint4*>(row + b);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t x = b + j < tiles ? w[j] : 0u;
+                    total[k] += x;
+                    before[k] += b + j < (int)blockIdx.x ? x : 0u;
+                }
             }
         }
-        // exclusive sum of the digit totals over the digits below t
-        uint32_t incl = total;
+        // exclusive sum of the digit totals over the digits below each
+        uint32_t tsum = 0;
+#pragma unroll
+        for (int k = 0; k < DPT; k++) tsum += total[k];
+        uint32_t incl = tsum;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(incl, o);
@@ -334,9 +368,13 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
         }
         if (lane == 63) wave_sum[wave] = incl;
         lds_barrier();
-        uint32_t below = incl - total;
+        uint32_t below = incl - tsum;
         for (int w = 0; w < wave; w++) below += wave_sum[w];
-        running[t] = below + before;
+#pragma unroll
+        for (int k = 0; k < DPT; k++) {
+            running[t * DPT + k] = below + before[k];
+            below += total[k];
+        }
     } else {
         running[t] = offsets[(size_t)t * tiles + blockIdx.x];
     }
@@ -346,10 +384,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         const bool valid = base + r * 64 + lane < n;
-        const uint32_t digit = valid ? (kreg[r] >> shift) & 255u : 0u;
+        const uint32_t digit = valid ? (kreg[r] >> shift) & kMask : 0u;
         uint64_t match = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < BITS; b++) {
             const bool bit = (digit >> b) & 1u;
             const uint64_t ones = __ballot(valid && bit);
             match &= bit ? ones : ~ones;
@@ -361,12 +399,14 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
         __builtin_amdgcn_wave_barrier();
     }
     lds_barrier();
-    {  // each wave's offset per digit: the counts of the waves before it
+    // each wave's offset per digit: the counts of the waves before it
+#pragma unroll
+    for (int k = 0; k < DPT; k++) {
         uint32_t acc = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; w++) {
-            const uint32_t c = wave_cnt[w][t];
-            wave_cnt[w][t] = acc;
+            const uint32_t c = wave_cnt[w][t * DPT + k];
+            wave_cnt[w][t * DPT + k] = acc;
             acc += c;
         }
     }
@@ -375,11 +415,12 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(const uint3
     for (int r = 0; r < kSortItems; r++) {
         if (base + r * 64 + lane >= n) continue;
         const uint32_t key = kreg[r];
-        const uint32_t digit = (key >> shift) & 255u;
+        const uint32_t digit = (key >> shift) & kMask;
         const uint32_t off = running[digit] + wave_cnt[wave][digit] + rank[r];
         keys_out[off] = key;
         vals_out[off] = vreg[r];
-        if (FUSED && hist_next) atomicAdd(hist_next + ((key >> (shift + 8)) & 255u) * kFusedTiles + off / kTile, 1u);
+        if (FUSED && hist_next)
+            atomicAdd(hist_next + ((key >> (shift + BITS)) & kMask) * kFusedTiles + off / kTile, 1u);
     }
 }
 
@@ -1454,9 +1495,9 @@ void lbvh_release(LbvhDev& d) {
 }
 
 // Digit-count words of the sort: the one-launch-per-pass layout of small
-// sorts (four passes of kFusedHist) or 256 per tile, whichever is larger, so
+// sorts (kFusedPasses blocks of kFusedHist) or 256 per tile, whichever is larger, so
 // a reservation for n also fits every smaller n.
-static size_t hist_words(int tiles) { return std::max((size_t)4 * kFusedHist, (size_t)256 * tiles); }
+static size_t hist_words(int tiles) { return std::max((size_t)kFusedPasses * kFusedHist, (size_t)256 * tiles); }
 
 hipError_t lbvh_reserve(LbvhDev& d, int n) {
     if (n <= d.cap) return hipSuccess;
@@ -1574,38 +1615,48 @@ hipError_t lbvh_build(LbvhDev& d, int n, hipStream_t stream) {
     d.tiles = tiles;
     const bool small = tiles <= kFusedTiles;  // n <= kSmallBuild
     d.small = small;
-    hipLaunchKernelGGL(morton_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, src, d.partial, pblocks, n,
-                       small ? kFusedTiles : tiles, d.root, d.codes, d.ids, d.hist,
-                       small ? d.hist + kFusedHist : nullptr, small ? d.level_count : nullptr);
-    const size_t total = (size_t)256 * tiles;
-    const int chunks = (int)((total + kScanChunk - 1) / kScanChunk);
     uint32_t *kin = d.codes, *vin = d.ids, *kout = d.codes_tmp, *vout = d.ids_tmp;
-    for (int shift = 0; shift < 32; shift += 8) {
-        if (small) {
-            // one launch per pass; each pass counts the next pass's digits
-            uint32_t* hist = d.hist + (size_t)(shift / 8) * kFusedHist;
-            hipLaunchKernelGGL(radix_scatter_kernel<true>, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, n,
-                               shift, tiles, hist, kout, vout, shift < 24 ? hist + kFusedHist : nullptr);
+    if (small) {
+        // one launch per pass; each pass counts the next pass's digits
+        hipLaunchKernelGGL(morton_hist_kernel<kFusedBits>, dim3(tiles), dim3(kSortThreads), 0, stream, src, d.partial,
+                           pblocks, n, kFusedTiles, d.root, d.codes, d.ids, d.hist, d.hist + kFusedHist,
+                           kFusedPasses - 1, d.level_count);
+        for (int pass = 0; pass < kFusedPasses; pass++) {
+            uint32_t* hist = d.hist + (size_t)pass * kFusedHist;
+            hipLaunchKernelGGL((radix_scatter_kernel<true, kFusedBits>), dim3(tiles), dim3(kSortThreads), 0, stream,
+                               kin, vin, n, pass * kFusedBits, tiles, hist, kout, vout,
+                               pass + 1 < kFusedPasses ? hist + kFusedHist : nullptr);
             std::swap(kin, kout);
             std::swap(vin, vout);
-            continue;
         }
-        if (shift > 0)  // the first pass's histogram comes from morton_hist_kernel
-            hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, n, shift, tiles,
-                               d.hist);
-        if (total <= (size_t)kSmallScanMax) {
-            hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kSmallScanThreads), 0, stream, d.hist, (int)total);
-        } else {
-            hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
-            hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, stream, d.hist_sums, chunks);
-            hipLaunchKernelGGL(scan_add_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+        // an odd number of passes leaves the sorted data in the second buffers
+        if (kin != d.codes) {
+            std::swap(d.codes, d.codes_tmp);
+            std::swap(d.ids, d.ids_tmp);
         }
-        hipLaunchKernelGGL(radix_scatter_kernel<false>, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, n, shift,
-                           tiles, d.hist, kout, vout, nullptr);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
+    } else {
+        hipLaunchKernelGGL(morton_hist_kernel<8>, dim3(tiles), dim3(kSortThreads), 0, stream, src, d.partial, pblocks,
+                           n, tiles, d.root, d.codes, d.ids, d.hist, nullptr, 0, nullptr);
+        const size_t total = (size_t)256 * tiles;
+        const int chunks = (int)((total + kScanChunk - 1) / kScanChunk);
+        for (int shift = 0; shift < 32; shift += 8) {
+            if (shift > 0)  // the first pass's histogram comes from morton_hist_kernel
+                hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(kSortThreads), 0, stream, kin, n, shift, tiles,
+                                   d.hist);
+            if (total <= (size_t)kSmallScanMax) {
+                hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kSmallScanThreads), 0, stream, d.hist, (int)total);
+            } else {
+                hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+                hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, stream, d.hist_sums, chunks);
+                hipLaunchKernelGGL(scan_add_kernel, dim3(chunks), dim3(256), 0, stream, d.hist, total, d.hist_sums);
+            }
+            hipLaunchKernelGGL((radix_scatter_kernel<false, 8>), dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin,
+                               n, shift, tiles, d.hist, kout, vout, nullptr);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
     }
-    // four passes: the sorted data is back in d.codes / d.ids
+    // the sorted data is in d.codes / d.ids
     hipError_t e;
     if (n <= 1) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
