@@ -552,8 +552,9 @@ constexpr int kLevelBlock = 256, kLevelPerThread = 16, kLevelChunk = kLevelBlock
 // Refit tiles: kRefitTile consecutive leaves (sorted positions; 256, 512 and
 // 1024 measure within 2 % of each other on 10M segments).  An internal
 // node whose leaf range lies inside one tile has its whole subtree there;
-// refit_tile_kernel refits those in one workgroup per tile, and only the
-// nodes whose ranges cross a tile boundary go into the per-height lists.
+// tile_fit_kernel fits those in one workgroup per tile (bottom-up in LDS, for
+// trees of every size), and for larger trees only the nodes whose ranges
+// cross a tile boundary go into the per-height lists.
 constexpr int kRefitTile = 512;
 
 __device__ inline bool crosses_tiles(const LbvhNode* __restrict__ nodes, int p) {
@@ -638,62 +639,6 @@ __device__ inline void refit_node(const float* __restrict__ aabb, const uint32_t
             }
         }
         for (int j = 0; j < 6; j++) nodes[p].b[side][j] = b[j];
-    }
-}
-
-// The internal nodes whose leaf range lies inside one tile, one workgroup per
-// tile: the tile's leaf boxes and each node's union go to LDS, and the nodes
-// are refit height by height with a barrier between heights (a node's
-// children are lower).  The unions are formed as refit_node forms a child's
-// union from the child's two boxes, so the boxes equal the per-height
-// kernels' bit for bit.
-__global__ __launch_bounds__(kRefitTile) void refit_tile_kernel(const float* __restrict__ aabb,
-                                                                const uint32_t* __restrict__ ids,
-                                                                LbvhNode* __restrict__ nodes,
-                                                                const uint32_t* __restrict__ node_height, int n) {
-    __shared__ float s_leaf[6][kRefitTile];
-    __shared__ float s_node[6][kRefitTile];
-    __shared__ uint32_t s_top;
-    const int t = threadIdx.x;
-    const int L = blockIdx.x * kRefitTile;
-    const int leaf = L + t, p = L + t;
-    if (t == 0) s_top = 0;
-    if (leaf < n) {
-        const float* a = aabb + 6 * (size_t)ids[leaf];
-        for (int j = 0; j < 6; j++) s_leaf[j][t] = a[j];
-    }
-    bool in = false;
-    uint32_t height = 0, child[2] = {0, 0};
-    if (p < n - 1) {
-        const uint4 w = *reinterpret_cast<const uint4*>(nodes + p);  // parent, left, right, fence
-        const uint32_t lo = min((uint32_t)p, w.w), hi = max((uint32_t)p, w.w);
-        in = lo / kRefitTile == hi / kRefitTile;
-        if (in) {
-            height = node_height[p];
-            child[0] = w.y;
-            child[1] = w.z;
-        }
-    }
-    __syncthreads();
-    if (in) atomicMax(&s_top, height);
-    __syncthreads();
-    const uint32_t top = s_top;
-    for (uint32_t h = 1; h <= top; h++) {
-        if (in && height == h) {
-            float b[2][6];
-            for (int side = 0; side < 2; side++) {
-                const uint32_t c = child[side];
-                const int k = (int)(c & 0x7FFFFFFFu) - L;  // in this tile, as the node is
-                for (int j = 0; j < 6; j++) b[side][j] = (c & 0x80000000u) ? s_leaf[j][k] : s_node[j][k];
-            }
-            for (int side = 0; side < 2; side++)
-                for (int j = 0; j < 6; j++) nodes[p].b[side][j] = b[side][j];
-            for (int j = 0; j < 3; j++) {
-                s_node[j][t] = b[1][j] < b[0][j] ? b[1][j] : b[0][j];
-                s_node[3 + j][t] = b[0][3 + j] < b[1][3 + j] ? b[1][3 + j] : b[0][3 + j];
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -1711,8 +1656,9 @@ hipError_t lbvh_refit(LbvhDev& d, hipStream_t stream) {
                                std::clamp(d.fit_slots, 0, kCrossCap), d.codes_tmp, nullptr);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(refit_tile_kernel, dim3((unsigned)((d.n + kRefitTile - 1) / kRefitTile)), dim3(kRefitTile), 0,
-                       stream, src, d.ids, d.nodes, d.node_height, d.n);
+    hipLaunchKernelGGL(tile_fit_kernel<false>, dim3((unsigned)((d.n + kRefitTile - 1) / kRefitTile)), dim3(kRefitTile),
+                       0, stream, nullptr, src, d.ids, d.nodes, nullptr, d.n, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr);
     const int K = bottom_levels(d.n);
     for (int h = 1; h <= K; h++)
         hipLaunchKernelGGL(refit_level_kernel, dim3(level_blocks(d.n, h)), dim3(256), 0, stream, src, d.ids,
