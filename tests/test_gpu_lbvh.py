@@ -6,6 +6,7 @@ import time
 
 import numpy as np
 import pytest
+import torch  # before the library loads the HIP runtime (device buffers of the in-place test)
 
 from kamisimulator_amd._lib import KscrError, Lbvh
 from oracle.pyoracle import OracleLbvh
@@ -103,6 +104,44 @@ def test_crossing_nodes_fit_from_global_memory():
     for bad in (-1, 4097):
         with pytest.raises(KscrError, match="INVALID_ARGUMENT"):
             g.set_fit_slots(bad)
+
+
+def test_device_entry_points_read_boxes_in_place():
+    """kscr_lbvh_compute_device and kscr_lbvh_refit_device read the caller's
+    boxes where they lie, as the reference's LBVH keeps the pointer compute
+    was given: the tree built from one device buffer, refit from a second
+    (moved boxes) and queried, equals the restatement at each step; the
+    refit and the query follow the second buffer, and a host-array refit
+    afterwards takes its own upload."""
+    torch.cuda.set_device(0)
+    n = 30_000
+    boxes = recipe_points(n)
+    rng = np.random.default_rng(12)
+    moved = (boxes + np.repeat(rng.normal(scale=2e-3, size=(n, 3)), 2, axis=0).reshape(n, 6)).astype(np.float32)
+    d_a = torch.from_numpy(boxes).to("cuda:0")
+    d_b = torch.from_numpy(moved).to("cuda:0")
+    cap = 64 * n
+    d_pairs = torch.empty((cap, 2), dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    g = Lbvh(device=0)
+    o = OracleLbvh(boxes)
+    g.compute_device(d_a.data_ptr(), n)
+    torch.cuda.synchronize()
+    tg, to = g.get(), o.get()
+    for k in ("codes", "ids", "links", "leaf_parent"):
+        assert np.array_equal(tg[k], to[k]), k
+    assert np.array_equal(tg["bounds"].view(np.uint32), to["bounds"].view(np.uint32))
+    g.refit_device(d_b.data_ptr())
+    cnt = g.query_device(d_pairs.data_ptr(), cap)
+    o.refit(moved)
+    tg, to = g.get(), o.get()
+    assert np.array_equal(tg["bounds"].view(np.uint32), to["bounds"].view(np.uint32))
+    op = o.query(cap)
+    gp = d_pairs[:cnt].cpu().numpy()
+    key = lambda p: np.sort(p[:, 0].astype(np.int64) * n + p[:, 1])
+    assert cnt == len(op) and np.array_equal(key(gp), key(op))
+    print(f"[gpu-lbvh] in-place device boxes: {cnt} pairs after the refit from the second buffer", flush=True)
+    _refit_compare(g, o, boxes)  # host arrays: uploaded into the handle's own buffer
 
 
 def test_ties_and_one_object():
