@@ -89,6 +89,33 @@ def test_sizes_around_the_small_tree_bound(n):
     _refit_compare(g, o, moved)
 
 
+@pytest.mark.parametrize("n", [5000, 140_000])
+def test_nan_coordinates(n):
+    """Boxes with a NaN coordinate (a NaN centre gives Morton cell 0; unions
+    pass NaN on through the same selects as glm's min/max) on a small tree
+    and a large one: the tree, node boxes, root box and stack bound equal the
+    restatement after the build and after a refit.  (Such a box overlaps
+    everything in the reference's intersects, so the query is not part of
+    this check.)"""
+    rng = np.random.default_rng(n + 1)
+    P = rng.random((n, 3)).astype(np.float32)
+    boxes = np.concatenate([P - 1e-3, P + 1e-3], 1).astype(np.float32)
+    boxes[11::89, 1] = np.nan
+    boxes[13::97, 5] = np.nan
+    g = Lbvh(device=0)
+    g.compute(boxes)
+    o = OracleLbvh(boxes)
+    tg, to = g.get(), o.get()
+    for k in ("codes", "ids", "links", "leaf_parent"):
+        assert np.array_equal(tg[k], to[k]), k
+    for k in ("bounds", "root"):
+        assert np.array_equal(tg[k].view(np.uint32), to[k].view(np.uint32)), k
+    assert tg["max_stack"] == to["max_stack"]
+    assert np.isnan(tg["bounds"]).any()  # NaN reached the node boxes
+    moved = boxes + np.float32(1e-3)
+    _refit_compare(g, o, moved)
+
+
 def test_crossing_nodes_fit_from_global_memory():
     """With fewer LDS slots than tile-crossing nodes (about 1.8k on the
     recipe; 0 slots here) the small-tree fit runs its rounds from global
