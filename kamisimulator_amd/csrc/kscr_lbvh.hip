@@ -61,21 +61,6 @@ __device__ inline bool intersects(const float* a, const float* b) {
     return !(a[3] <= b[0] || a[4] <= b[1] || a[5] <= b[2] || a[0] >= b[3] || a[1] >= b[4] || a[2] >= b[5]);
 }
 
-// Whether any wave of the workgroup passed a nonzero word to any_wave_flag's
-// LDS row this round: the row is read with 16-byte loads and OR'd without
-// short-circuits, so the reads are in flight together.
-template <int WAVES>
-__device__ inline bool any_wave_flag(const uint32_t* row) {
-    static_assert(WAVES % 4 == 0, "flag rows are read four words at a time");
-    uint32_t acc = 0;
-#pragma unroll
-    for (int k = 0; k < WAVES / 4; k++) {
-        const uint4 v = reinterpret_cast<const uint4*>(row)[k];
-        acc |= v.x | v.y | v.z | v.w;
-    }
-    return acc != 0;
-}
-
 // ---------------------------------------------------------------- root box
 __global__ void root_partial_kernel(const float* __restrict__ aabb, int n, float* __restrict__ partial) {
     __shared__ float s[6][256];
