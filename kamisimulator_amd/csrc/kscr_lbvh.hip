@@ -33,11 +33,9 @@ constexpr int kFusedHist = (1 << kFusedBits) * kFusedTiles;
 static_assert(kFusedBits * kFusedPasses >= 30, "the passes cover the Morton code");
 constexpr int kQueryBlock = 128;
 constexpr int kQueryStack = 64;
-constexpr int kQueryStackFirst = 12;  // first pass; deeper queries are redone
-#ifndef KSCR_QUERY2_STACK
-#define KSCR_QUERY2_STACK 8
-#endif
-constexpr int kQueryStackFirst2 = KSCR_QUERY2_STACK;  // entries per query of the two-query first pass
+// entries per query of the two-query first pass above 2^20 objects (deeper
+// queries are redone with kQueryStack; 12 measured slower: fewer waves per SIMD)
+constexpr int kQueryStackFirst2 = 8;
 constexpr unsigned kRedoBlocks = 512;   // grid of the redo pass (grid-stride)
 
 __device__ inline uint32_t expand_bits(uint32_t v) {
